@@ -1,0 +1,209 @@
+/*
+ * omx_fdtd.h -- C-ABI of libomx_fdtd_hip.so, the MI355X (gfx950) engine for the
+ * 3D field-propagation hot path of One-Mxll-to-Rule-Them-All (OMxRTA).
+ *
+ * Every entry point takes plain pointers and sizes, returns an int status
+ * (OMX_OK == 0) and never throws.  The text of the last failure on the calling
+ * thread is returned by omx_last_error().  Host arrays that mirror reference
+ * arrays use the reference's Fortran layout (first index fastest) and the
+ * reference's index bounds; the device layout is private (z fastest, padded).
+ *
+ * Reference interfaces replaced (file:line relative to the reference tree):
+ *   TMxll%init           src/FDTD/mxll_base_mod.f90:27,36-51  -> omx3d_create (+ set_media)
+ *   TMxll%kill           src/FDTD/mxll_base_mod.f90:28,53-56  -> omx3d_destroy
+ *   TMxll%td_propagate_H src/FDTD/mxll_base_mod.f90:29,58-61  -> omx3d_step_h
+ *   TMxll%td_propagate_E src/FDTD/mxll_base_mod.f90:30,63-67  -> omx3d_step_e
+ * Callers of the hot path that touch the solver's arrays directly:
+ *   source_interactions (3D)  src/FDTD/interactions_mod.f90:102-156      -> omx3d_add_source / omx3d_inject_sources
+ *   send_E_3D_to_J_ranks      src/FDTD/interactions_mod.f90:534,654-691  -> omx3d_gather_e
+ *   send_J_to_E_3D_ranks      src/FDTD/interactions_mod.f90:1046,1177-1218 -> omx3d_scatter_j
+ *   write_3D_field (stencils) src/FDTD/write_fields_subs_mod.f90:751-1509 -> omx3d_add_detector / omx3d_record
+ */
+#ifndef OMX_FDTD_H
+#define OMX_FDTD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define OMX_OK          0
+#define OMX_ERR_ARG     1   /* invalid argument or shape                    */
+#define OMX_ERR_HIP     2   /* a HIP runtime call failed                     */
+#define OMX_ERR_NOMEM   3   /* device allocation failed                      */
+#define OMX_ERR_STATE   4   /* call out of order (e.g. media set twice)      */
+#define OMX_ERR_RANGE   5   /* index or size beyond what the engine supports */
+
+/* ---- codes shared with the reference (src/FDTD/constants_mod.f90:58-102) - */
+#define OMX_BC_CLOSE     1
+#define OMX_BC_PERIODIC  2
+#define OMX_BC_CPML      3
+
+#define OMX_MEDIUM_NONE        0
+#define OMX_MEDIUM_DIELECTRIC  1
+#define OMX_MEDIUM_DRUDE       2
+#define OMX_MEDIUM_DL          3
+
+#define OMX_DET_POINT     1
+#define OMX_DET_LINE_X    2
+#define OMX_DET_LINE_Y    3
+#define OMX_DET_LINE_Z    4
+#define OMX_DET_PLANE_XY  5
+#define OMX_DET_PLANE_YZ  6
+#define OMX_DET_PLANE_ZX  7
+#define OMX_DET_VOLUME    8
+
+#define OMX_EX 1
+#define OMX_EY 2
+#define OMX_EZ 3
+#define OMX_HX 4
+#define OMX_HY 5
+#define OMX_HZ 6
+
+/* ---- engine limits ------------------------------------------------------ */
+#define OMX_MAX_POLES   8    /* Lorentz poles per Drude-Lorentz medium      */
+#define OMX_MAX_MEDIA   15   /* media per grid (cell tags are one byte)     */
+
+#define OMX_FP32 4
+#define OMX_FP64 8
+
+typedef struct omx3d_ctx omx3d_ctx;
+
+/* Grid description: the arguments of TMxll%init (mxll_base_mod.f90:36-51)
+ * for a single, non-MPI domain. */
+typedef struct omx3d_desc {
+    int32_t nx, ny, nz;        /* grid_Ndims                                  */
+    int32_t npml;              /* CPML thickness in cells (>= 2 if any CPML)  */
+    int32_t boundaries[3];     /* OMX_BC_* per axis                           */
+    int32_t precision;         /* OMX_FP32 or OMX_FP64 field arithmetic       */
+    double  dt, dr;            /* atomic units                                */
+} omx3d_desc;
+
+/* One classical medium (TClassicalMedium, classical_medium_mod.f90:6-37).
+ * Frequencies and rates in atomic units.  a_k = f_k * omega_p^2. */
+typedef struct omx_medium {
+    int32_t kind;              /* OMX_MEDIUM_*                                */
+    int32_t n_poles;           /* Lorentz poles (DL only)                     */
+    double  eps_r;             /* dielectric eps_r, or eps_inf of Drude / DL  */
+    double  omega_d, gamma_d;  /* Drude pole                                  */
+    double  omega_k[OMX_MAX_POLES];
+    double  gamma_k[OMX_MAX_POLES];
+    double  a_k[OMX_MAX_POLES];
+} omx_medium;
+
+/* Update coefficients of one medium, computed on the device in FP64 with the
+ * formulas of init_classical_medium (classical_medium_mod.f90:60-101). */
+typedef struct omx_medium_coeffs {
+    double A1, A2, C1, C2, C3, C4;
+    double inv_eps;            /* 1/eps_r (dielectric)                        */
+    double alpha_k[OMX_MAX_POLES], beta_k[OMX_MAX_POLES], gamma_k[OMX_MAX_POLES];
+    double B1_k[OMX_MAX_POLES], B2_k[OMX_MAX_POLES];
+} omx_medium_coeffs;
+
+/* Geometric primitive rasterised on the device into the cell medium map.
+ * Coordinates are in cell-index units: cell (i,j,k) (1-based) sits at (i,j,k).
+ * medium == 0 carves (sets cells back to no medium); otherwise a cell takes the
+ * first medium that covers it, as the reference's point-list reader does
+ * (classical_medium_mod.f90:667-686). */
+#define OMX_SHAPE_SPHERE      1   /* p: cx cy cz r                              */
+#define OMX_SHAPE_ELLIPSOID   2   /* p: cx cy cz a b c R00 R01 .. R22 (row-major body<-grid rotation) */
+#define OMX_SHAPE_BOX         3   /* p: xlo ylo zlo xhi yhi zhi                 */
+#define OMX_SHAPE_CYLINDER_Z  4   /* p: cx cy r zlo zhi                         */
+typedef struct omx_shape {
+    int32_t type;
+    int32_t medium;
+    double  p[15];
+} omx_shape;
+
+const char* omx_last_error(void);
+int omx_version(void);
+/* Library pole data of the reference (get_DL_library_data,
+ * classical_medium_mod.f90:125-199), including its single-precision literals.
+ * name: "Ag", "Al" or "Au". */
+int omx_medium_library(const char* name, double eps_r, omx_medium* out);
+
+/* ---- lifecycle (TMxll%init / TMxll%kill) --------------------------------- */
+int omx3d_create(const omx3d_desc* desc, omx3d_ctx** out);
+int omx3d_destroy(omx3d_ctx* ctx);
+/* Tile shape of the H/E kernels: y rows per workgroup (1,2,4,8) and x planes
+ * marched per workgroup (>= 1).  0 keeps the current value. */
+int omx3d_set_tiling(omx3d_ctx* ctx, int32_t rows_y, int32_t xchunk);
+/* CPML profiles and den arrays (mxll_3D_mod.f90:282-420), device FP64.
+ * be,ce: npml; bh,ch: npml-1; den: den_ex(nx) den_ey(ny) den_ez(nz) den_hx(nx) den_hy(ny) den_hz(nz). */
+int omx3d_get_cpml(omx3d_ctx* ctx, double* be, double* ce, double* bh, double* ch, double* den);
+
+/* ---- media (read_init_media_3D, classical_medium_mod.f90:546-769) -------- */
+/* aux: the cell medium map (nx*ny*nz, Fortran order, values 0..n_media) the
+ * reference builds from the medium point lists; the per-component staggered
+ * map is derived from it on the device. */
+int omx3d_set_media(omx3d_ctx* ctx, int32_t n_media, const omx_medium* table, const int8_t* aux);
+int omx3d_set_media_shapes(omx3d_ctx* ctx, int32_t n_media, const omx_medium* table,
+                           int32_t n_shapes, const omx_shape* shapes);
+/* media_map(nx,ny,nz,3) of the reference, Fortran order. */
+int omx3d_get_media_map(omx3d_ctx* ctx, int32_t* media_map);
+int omx3d_get_medium_coeffs(omx3d_ctx* ctx, int32_t medium, omx_medium_coeffs* out);
+
+/* ---- time stepping (td_propagate_H_3Dfield / td_propagate_E_3Dfield) ----- */
+int omx3d_step_h(omx3d_ctx* ctx);
+/* time and t_skip are TMxll%time and TMxll%t_skip (mxim_mxll.f90:93-101). */
+int omx3d_step_e(omx3d_ctx* ctx, double time, double t_skip);
+
+/* ---- sources (TSource + source_interactions) ----------------------------- */
+/* ind: (2n+1)^3 x 3 Fortran array (ind_i, ind_j, ind_k blocks, i fastest);
+ * ker: ker_mat(-n:n,-n:n,-n:n).  dir: 1,2,3 for x,y,z.  Times in au. */
+int omx3d_add_source(omx3d_ctx* ctx, int32_t dir, double j_amp, double w0, double t0, double tau,
+                     double t_init, double t_final, double phase, int32_t n_ker,
+                     const int32_t* ind, const double* ker);
+int omx3d_inject_sources(omx3d_ctx* ctx, double time);
+
+/* ---- detectors (TDetector + write_3D_field stencils) -------------------- */
+/* indx_list(nd,3), Fortran order, 1-based local indices. */
+int omx3d_add_detector(omx3d_ctx* ctx, int32_t det_type, int32_t field, int32_t nd,
+                       const int32_t* indx_list);
+/* Device ring of samples: capacity rows of (sum of nd) doubles. */
+int omx3d_set_record_capacity(omx3d_ctx* ctx, int32_t rows);
+int omx3d_record(omx3d_ctx* ctx);
+int omx3d_read_records(omx3d_ctx* ctx, double* out, int32_t* rows);
+int omx3d_sample(omx3d_ctx* ctx, double* out);
+
+/* ---- embedded quantum systems, "material" groups ------------------------ */
+/* Sites: map(n,3:5) of q_group_mod.f90:505-509 as an (n,3) Fortran array. */
+int omx3d_set_q_sites(omx3d_ctx* ctx, int32_t n, const int32_t* ijk, double density, double dt_q);
+int omx3d_gather_e(omx3d_ctx* ctx, double* e_sites);              /* (n,3) */
+int omx3d_scatter_j(omx3d_ctx* ctx, const double* dpdt_sites);    /* (n,3) */
+/* Two-level emitters propagated on the device in FP64 (RK4 over dt_q). */
+int omx3d_set_two_level(omx3d_ctx* ctx, const double* omega0, const double* mu,
+                        const double* pol, const double* gamma2);
+int omx3d_two_level_step(omx3d_ctx* ctx);                          /* gather, RK4, scatter */
+int omx3d_get_two_level(omx3d_ctx* ctx, double* dipole, double* rho); /* (n,3), (n,4) */
+
+/* ---- field access and reductions ---------------------------------------- */
+/* host: the reference array of that field with its bounds (mxll_3D_mod.f90:123-131),
+ * Fortran order.  to_device = 0 copies device -> host. */
+int omx3d_copy_field(omx3d_ctx* ctx, int32_t field, double* host, int32_t to_device);
+/* W = 1/2 dr^3 sum(eps0 eps_r E^2 + mu0 H^2) over the interior cells; w64 is
+ * accumulated in FP64, w32c with compensated FP32 sums. */
+int omx3d_energy(omx3d_ctx* ctx, double* w64, double* w32c);
+
+/* ---- total-field / scattered-field plane wave (x-polarised, along +z) ---- */
+int omx3d_set_tfsf(omx3d_ctx* ctx, const int32_t lo[3], const int32_t hi[3],
+                   double e0, double w0, double t0, double tau, int32_t max_steps);
+
+/* ---- whole time loop on the device (mxim_mxll.f90:92-132 step order) ----- */
+/* Steps tt = tt_first .. tt_first+n_steps-1; records detectors when
+ * mod(tt, record_every) == 0 (record_every <= 0: never). */
+int omx3d_run(omx3d_ctx* ctx, int32_t tt_first, int32_t n_steps, int32_t record_every);
+/* Per-kernel timing (HIP events around each H and E launch of omx3d_run). */
+int omx3d_set_profiling(omx3d_ctx* ctx, int32_t enable);
+int omx3d_get_profile(omx3d_ctx* ctx, double* ms_h, double* ms_e, int32_t* n_launches);
+int omx3d_synchronize(omx3d_ctx* ctx);
+/* Bytes of device memory the context holds. */
+int omx3d_device_bytes(omx3d_ctx* ctx, int64_t* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OMX_FDTD_H */

@@ -1,0 +1,1157 @@
+// omx_fdtd.hip -- C-ABI of libomx_fdtd_hip (declared in include/omx_fdtd.h).
+//
+// Host side of the MI355X FDTD engine: context lifecycle, coefficient and
+// media set-up, the per-step launch sequence of the reference driver loop
+// (src/FDTD/mxim_mxll.f90:92-132) and the device-resident couplings
+// (sources, detectors, embedded quantum systems, TF/SF, energy).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "omx_kernels.hpp"
+
+using namespace omx;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                      \
+    do {                                                                               \
+        hipError_t e_ = (x);                                                           \
+        if (e_ != hipSuccess)                                                          \
+            return fail(OMX_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define CHECK_CTX(c) \
+    do {             \
+        if (!(c)) return fail(OMX_ERR_ARG, "null context"); \
+    } while (0)
+
+int dalloc(omx3d_ctx* c, void** p, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(OMX_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+    }
+    c->device_bytes += (long)bytes;
+    e = hipMemsetAsync(*p, 0, bytes, c->s);
+    if (e != hipSuccess) return fail(OMX_ERR_HIP, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
+    return OMX_OK;
+}
+
+void dfree(void*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+template <typename P>
+void dfreet(P*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+inline unsigned blocks_for(long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// Kernel argument block for precision T.
+template <typename T>
+YeeArgs<T> make_args(omx3d_ctx* c) {
+    YeeArgs<T> a;
+    std::memset(&a, 0, sizeof(a));
+    a.ex = (T*)c->f[0]; a.ey = (T*)c->f[1]; a.ez = (T*)c->f[2];
+    a.hx = (T*)c->f[3]; a.hy = (T*)c->f[4]; a.hz = (T*)c->f[5];
+    const T* tc = (const T*)c->tcoef;
+    a.be = tc + c->co_be; a.ced = tc + c->co_ce; a.bh = tc + c->co_bh; a.chd = tc + c->co_ch;
+    a.den_ex = tc + c->co_den[0]; a.den_ey = tc + c->co_den[1]; a.den_ez = tc + c->co_den[2];
+    a.den_hx = tc + c->co_den[3]; a.den_hy = tc + c->co_den[4]; a.den_hz = tc + c->co_den[5];
+    for (int q = 0; q < 6; ++q) { a.ph[q] = (T*)c->psi_h[q]; a.pe[q] = (T*)c->psi_e[q]; }
+    a.tag = c->d_tag; a.cls = c->d_cls; a.med = (const MedDev<T>*)c->d_med;
+    for (int q = 0; q < 3; ++q) {
+        a.eold[q] = (T*)c->disp_eold[q]; a.pd[q] = (T*)c->disp_pd[q];
+        a.pl[q] = (T*)c->disp_pl[q]; a.plold[q] = (T*)c->disp_plold[q];
+    }
+    a.pstride = c->L.nalloc;
+    a.dt_mu0 = (T)(c->d.dt / kMu0);
+    a.dt_eps0 = (T)(c->d.dt / kEps0);
+    a.nx = c->L.nx; a.ny = c->L.ny; a.nz = c->L.nz; a.sy = c->L.sy; a.sz = c->L.sz;
+    a.sx = c->L.sx; a.off = c->L.off;
+    a.npml = c->d.npml;
+    a.cpml = c->cpml_mask;
+    a.xchunk = c->xchunk;
+    a.ntz = (c->L.nz + kLanes - 1) / kLanes;
+    a.nty = (c->L.ny + c->rows_y - 1) / c->rows_y;
+    return a;
+}
+
+template <typename T>
+unsigned yee_blocks(omx3d_ctx* c, const YeeArgs<T>& a) {
+    const long ntx = (c->L.nx + c->xchunk - 1) / c->xchunk;
+    return (unsigned)((long)a.ntz * a.nty * ntx);
+}
+
+template <typename T>
+int wrap_e(omx3d_ctx* c) {
+    const Layout& L = c->L;
+    T** f = (T**)c->f;
+    if (c->d.boundaries[0] == OMX_BC_PERIODIC)
+        hipLaunchKernelGGL(k_wrap<T>, dim3(blocks_for(L.nz + 2, 256), L.ny + 2), dim3(256), 0, c->s, f[2], f[1], L, 0, L.nx + 1, 1);
+    if (c->d.boundaries[1] == OMX_BC_PERIODIC)
+        hipLaunchKernelGGL(k_wrap<T>, dim3(blocks_for(L.nz + 2, 256), L.nx + 2), dim3(256), 0, c->s, f[0], f[2], L, 1, L.ny + 1, 1);
+    if (c->d.boundaries[2] == OMX_BC_PERIODIC)
+        hipLaunchKernelGGL(k_wrap<T>, dim3(blocks_for(L.ny + 2, 256), L.nx + 2), dim3(256), 0, c->s, f[0], f[1], L, 2, L.nz + 1, 1);
+    HIPCHK(hipGetLastError());
+    return OMX_OK;
+}
+
+template <typename T>
+int wrap_h(omx3d_ctx* c) {
+    const Layout& L = c->L;
+    T** f = (T**)c->f;
+    if (c->d.boundaries[0] == OMX_BC_PERIODIC)
+        hipLaunchKernelGGL(k_wrap<T>, dim3(blocks_for(L.nz + 2, 256), L.ny + 2), dim3(256), 0, c->s, f[4], f[5], L, 0, 0, L.nx);
+    if (c->d.boundaries[1] == OMX_BC_PERIODIC)
+        hipLaunchKernelGGL(k_wrap<T>, dim3(blocks_for(L.nz + 2, 256), L.nx + 2), dim3(256), 0, c->s, f[3], f[5], L, 1, 0, L.ny);
+    if (c->d.boundaries[2] == OMX_BC_PERIODIC)
+        hipLaunchKernelGGL(k_wrap<T>, dim3(blocks_for(L.ny + 2, 256), L.nx + 2), dim3(256), 0, c->s, f[3], f[4], L, 2, 0, L.nz);
+    HIPCHK(hipGetLastError());
+    return OMX_OK;
+}
+
+template <typename T>
+int launch_h(omx3d_ctx* c) {
+    int rc = wrap_e<T>(c);
+    if (rc) return rc;
+    YeeArgs<T> a = make_args<T>(c);
+    const dim3 blk(kLanes, c->rows_y);
+    const unsigned nb = yee_blocks(c, a);
+    if (c->cpml_mask) hipLaunchKernelGGL((k_h_update<T, true>), dim3(nb), blk, 0, c->s, a);
+    else hipLaunchKernelGGL((k_h_update<T, false>), dim3(nb), blk, 0, c->s, a);
+    HIPCHK(hipGetLastError());
+    return OMX_OK;
+}
+
+template <typename T>
+int launch_j_correct(omx3d_ctx* c, double tau) {
+    if (c->nq == 0) return OMX_OK;
+    const double dt_eps0 = c->d.dt / kEps0;
+    for (int comp = 0; comp < 3; ++comp) {
+        if (c->n_aff[comp] == 0) continue;
+        hipLaunchKernelGGL(k_j_correct<T>, dim3(blocks_for(c->n_aff[comp], 256)), dim3(256), 0, c->s,
+                           (T*)c->f[comp], c->d_aff_off[comp], c->d_aff_a[comp], c->d_aff_b[comp],
+                           c->n_aff[comp], comp, c->d_J, tau, dt_eps0,
+                           c->media_set ? c->d_tag : (const uint8_t*)nullptr, c->d_cls, c->d_kinds);
+    }
+    HIPCHK(hipGetLastError());
+    return OMX_OK;
+}
+
+template <typename T>
+int launch_e(omx3d_ctx* c, double time, double t_skip) {
+    int rc = wrap_h<T>(c);
+    if (rc) return rc;
+    YeeArgs<T> a = make_args<T>(c);
+    const dim3 blk(kLanes, c->rows_y);
+    const unsigned nb = yee_blocks(c, a);
+    const bool cp = c->cpml_mask != 0, md = c->media_set && c->n_classes > 1;
+    if (cp && md) hipLaunchKernelGGL((k_e_update<T, true, true>), dim3(nb), blk, 0, c->s, a);
+    else if (cp) hipLaunchKernelGGL((k_e_update<T, true, false>), dim3(nb), blk, 0, c->s, a);
+    else if (md) hipLaunchKernelGGL((k_e_update<T, false, true>), dim3(nb), blk, 0, c->s, a);
+    else hipLaunchKernelGGL((k_e_update<T, false, false>), dim3(nb), blk, 0, c->s, a);
+    HIPCHK(hipGetLastError());
+    return launch_j_correct<T>(c, time - t_skip);
+}
+
+template <typename T>
+int launch_sources(omx3d_ctx* c, double time) {
+    const double dt_eps0 = c->d.dt / kEps0;
+    const double c_src = ((dt_eps0 / c->d.dr) / kC0) / 2.0;  // interactions_mod.f90:32
+    for (auto& s : c->sources) {
+        if (!(time >= s.t_init && time <= s.t_final)) continue;  // J_mat = 0 outside the window
+        const double x = (time - s.t0) / s.tau;
+        const double env = std::exp(-(x * x));
+        const double cs = std::cos(s.w0 * (time - s.t0) + s.phase);
+        const double amp = (s.j_amp * env) * cs;
+        hipLaunchKernelGGL(k_source<T>, dim3(blocks_for(s.npts, 256)), dim3(256), 0, c->s,
+                           (T*)c->f[s.dir - 1], s.d_off, s.d_ka, s.d_kb, s.npts, amp, c_src);
+    }
+    HIPCHK(hipGetLastError());
+    return OMX_OK;
+}
+
+TfsfArgs tfsf_args(omx3d_ctx* c) {
+    TfsfArgs t;
+    t.i0 = c->tf_lo[0]; t.j0 = c->tf_lo[1]; t.k0 = c->tf_lo[2];
+    t.i1 = c->tf_hi[0]; t.j1 = c->tf_hi[1]; t.k1 = c->tf_hi[2];
+    t.kb = c->tf_kb; t.n1 = c->tf_n; t.ks = c->tf_ks;
+    t.dt_mu0 = c->d.dt / kMu0; t.dt_eps0 = c->d.dt / kEps0; t.inv_dr = 1.0 / c->d.dr;
+    return t;
+}
+
+template <typename T>
+int launch_tfsf_h(omx3d_ctx* c) {
+    if (!c->tfsf) return OMX_OK;
+    TfsfArgs t = tfsf_args(c);
+    const long nxi = t.i1 - t.i0, nyj = t.j1 - t.j0 + 1, nzk = t.k1 - t.k0 + 1;
+    const long n = 2 * nxi * nyj + 2 * nxi * nzk + t.n1;
+    hipLaunchKernelGGL(k_tfsf_h<T>, dim3(blocks_for(n, 256)), dim3(256), 0, c->s, make_args<T>(c), t,
+                       c->d_tf, c->d_tf + c->tf_n);
+    HIPCHK(hipGetLastError());
+    return OMX_OK;
+}
+
+template <typename T>
+int launch_tfsf_e(omx3d_ctx* c, double t_e) {
+    if (!c->tfsf) return OMX_OK;
+    TfsfArgs t = tfsf_args(c);
+    const long nxi = t.i1 - t.i0, nyj = t.j1 - t.j0 + 1, nzk = t.k1 - t.k0;
+    const long n = 2 * nxi * nyj + 2 * nyj * nzk + t.n1;
+    const double x = (t_e - c->tf_t0) / c->tf_tau;
+    const double src = c->tf_e0 * std::exp(-(x * x)) * std::cos(c->tf_w0 * (t_e - c->tf_t0));
+    hipLaunchKernelGGL(k_tfsf_e<T>, dim3(blocks_for(n, 256)), dim3(256), 0, c->s, make_args<T>(c), t,
+                       c->d_tf, c->d_tf + c->tf_n, src);
+    HIPCHK(hipGetLastError());
+    return OMX_OK;
+}
+
+template <typename T>
+int launch_record(omx3d_ctx* c, double* out) {
+    if (c->n_samples == 0) return OMX_OK;
+    hipLaunchKernelGGL(k_sample<T>, dim3(blocks_for(c->n_samples, 256)), dim3(256), 0, c->s, make_args<T>(c),
+                       c->d_det_field, c->d_det_off, c->n_samples, out);
+    HIPCHK(hipGetLastError());
+    return OMX_OK;
+}
+
+int upload_detectors(omx3d_ctx* c) {
+    if (!c->det_dirty) return OMX_OK;
+    dfreet(c->d_det_field);
+    dfreet(c->d_det_off);
+    if (c->n_samples > 0) {
+        int rc = dalloc(c, (void**)&c->d_det_field, sizeof(int) * c->n_samples);
+        if (rc) return rc;
+        rc = dalloc(c, (void**)&c->d_det_off, sizeof(long) * c->n_samples);
+        if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(c->d_det_field, c->h_det_field.data(), sizeof(int) * c->n_samples, hipMemcpyHostToDevice, c->s));
+        HIPCHK(hipMemcpyAsync(c->d_det_off, c->h_det_off.data(), sizeof(long) * c->n_samples, hipMemcpyHostToDevice, c->s));
+    }
+    c->det_dirty = false;
+    return OMX_OK;
+}
+
+template <typename T>
+int launch_two_level(omx3d_ctx* c) {
+    if (c->nq == 0 || !c->tls) return OMX_OK;
+    hipLaunchKernelGGL(k_gather_e<T>, dim3(blocks_for(c->nq, 256)), dim3(256), 0, c->s, make_args<T>(c),
+                       c->d_q_off, c->nq, c->d_q_e);
+    hipLaunchKernelGGL(k_tls_step, dim3(blocks_for(c->nq, 256)), dim3(256), 0, c->s, c->d_tls, c->d_q_e, c->nq,
+                       c->q_dt, c->d_q_dpdt);
+    hipLaunchKernelGGL(k_scatter_j, dim3(blocks_for(c->nslot, 256)), dim3(256), 0, c->s, c->d_q_winner, c->nslot,
+                       c->d_q_dpdt, c->q_density, c->q_dt, c->d_J);
+    HIPCHK(hipGetLastError());
+    return OMX_OK;
+}
+
+int ensure_events(omx3d_ctx* c, size_t n) {
+    while (c->ev.size() < n) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        c->ev.push_back(e);
+    }
+    return OMX_OK;
+}
+
+template <typename T>
+int run_impl(omx3d_ctx* c, int tt_first, int n_steps, int record_every) {
+    int rc = upload_detectors(c);
+    if (rc) return rc;
+    const double dt = c->d.dt;
+    size_t evi = 0;
+    if (c->prof) {
+        rc = ensure_events(c, (size_t)4 * n_steps);
+        if (rc) return rc;
+    }
+    for (int tt = tt_first; tt < tt_first + n_steps; ++tt) {
+        const double time = dt * ((double)tt - 0.5);
+        // n_skip_steps = 1: a new quantum step every step, so t_skip = time
+        // (mxim_mxll.f90:73-77,97-101).
+        const double t_skip = time;
+        if (c->prof) HIPCHK(hipEventRecord(c->ev[evi++], c->s));
+        rc = launch_h<T>(c);
+        if (rc) return rc;
+        if (c->prof) HIPCHK(hipEventRecord(c->ev[evi++], c->s));
+        rc = launch_tfsf_h<T>(c);
+        if (rc) return rc;
+        rc = launch_sources<T>(c, time);
+        if (rc) return rc;
+        if (c->prof) HIPCHK(hipEventRecord(c->ev[evi++], c->s));
+        rc = launch_e<T>(c, time, t_skip);
+        if (rc) return rc;
+        if (c->prof) HIPCHK(hipEventRecord(c->ev[evi++], c->s));
+        rc = launch_tfsf_e<T>(c, dt * (double)tt);
+        if (rc) return rc;
+        rc = launch_two_level<T>(c);
+        if (rc) return rc;
+        if (record_every > 0 && tt % record_every == 0 && c->n_samples > 0) {
+            if (c->rec_rows >= c->rec_cap) return fail(OMX_ERR_RANGE, "detector record buffer full");
+            rc = launch_record<T>(c, c->d_rec + (long)c->rec_rows * c->n_samples);
+            if (rc) return rc;
+            c->rec_rows++;
+        }
+    }
+    if (c->prof) {
+        HIPCHK(hipStreamSynchronize(c->s));
+        for (size_t q = 0; q + 3 < evi; q += 4) {
+            float mh = 0.f, me = 0.f;
+            HIPCHK(hipEventElapsedTime(&mh, c->ev[q], c->ev[q + 1]));
+            HIPCHK(hipEventElapsedTime(&me, c->ev[q + 2], c->ev[q + 3]));
+            c->prof_ms_h += mh;
+            c->prof_ms_e += me;
+            c->prof_n += 1;
+        }
+    }
+    return OMX_OK;
+}
+
+template <typename T>
+int energy_impl(omx3d_ctx* c, double* w64, double* w32c) {
+    const int nb = 1024;
+    if (!c->d_red) {
+        int rc = dalloc(c, (void**)&c->d_red, sizeof(double) * (3 * nb + 2));
+        if (rc) return rc;
+    }
+    YeeArgs<T> a = make_args<T>(c);
+    const bool md = c->media_set && c->n_classes > 1;
+    if (md) hipLaunchKernelGGL((k_energy<T, true>), dim3(nb), dim3(256), 0, c->s, a, c->d_eps_class, c->d_red);
+    else hipLaunchKernelGGL((k_energy<T, false>), dim3(nb), dim3(256), 0, c->s, a, c->d_eps_class, c->d_red);
+    const double dr = c->d.dr;
+    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, c->s, c->d_red, nb, dr * dr * dr, c->d_red + 3 * nb);
+    HIPCHK(hipGetLastError());
+    double out[2];
+    HIPCHK(hipMemcpyAsync(out, c->d_red + 3 * nb, sizeof(out), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    if (w64) *w64 = out[0];
+    if (w32c) *w32c = out[1];
+    return OMX_OK;
+}
+
+// Field bounds of the reference arrays (mxll_3D_mod.f90:123-131).
+void field_bounds(const Layout& L, int field, int lo[3], int hi[3]) {
+    const int n[3] = {L.nx, L.ny, L.nz};
+    for (int a = 0; a < 3; ++a) { lo[a] = 1; hi[a] = n[a] + 1; }
+    if (field <= 3) {
+        const int ax = field - 1;       // E component: 0:n along its own axis
+        lo[ax] = 0; hi[ax] = n[ax];
+    } else {
+        const int ax = field - 4;       // H component: 1:n along its axis, 0:n across
+        for (int a = 0; a < 3; ++a) { lo[a] = 0; hi[a] = n[a]; }
+        lo[ax] = 1; hi[ax] = n[ax];
+    }
+}
+
+template <typename T>
+int copy_field_impl(omx3d_ctx* c, int field, double* host, int to_device) {
+    const Layout& L = c->L;
+    std::vector<T> stage(L.nalloc);
+    T* dptr = (T*)c->f[field - 1];
+    HIPCHK(hipMemcpyAsync(stage.data(), dptr, sizeof(T) * L.nalloc, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    int lo[3], hi[3];
+    field_bounds(L, field, lo, hi);
+    const long e0 = hi[0] - lo[0] + 1, e1 = hi[1] - lo[1] + 1;
+    for (int k = lo[2]; k <= hi[2]; ++k)
+        for (int j = lo[1]; j <= hi[1]; ++j)
+            for (int i = lo[0]; i <= hi[0]; ++i) {
+                const long h = (i - lo[0]) + e0 * ((j - lo[1]) + e1 * (k - lo[2]));
+                if (to_device) stage[L.at(i, j, k)] = (T)host[h];
+                else host[h] = (double)stage[L.at(i, j, k)];
+            }
+    if (to_device) {
+        HIPCHK(hipMemcpyAsync(dptr, stage.data(), sizeof(T) * L.nalloc, hipMemcpyHostToDevice, c->s));
+        HIPCHK(hipStreamSynchronize(c->s));
+    }
+    return OMX_OK;
+}
+
+// Classes of per-component medium triples present in the grid; tag = class.
+int build_tags(omx3d_ctx* c) {
+    const Layout& L = c->L;
+    const int nm = c->n_media;
+    const int ncode = (nm + 1) * (nm + 1) * (nm + 1);
+    const long ncell = (long)L.nx * L.ny * L.nz;
+    unsigned* d_present = nullptr;
+    uint8_t* d_lut = nullptr;
+    int rc = dalloc(c, (void**)&d_present, sizeof(unsigned) * ((ncode + 31) / 32));
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_media_codes, dim3(blocks_for(ncell, 256)), dim3(256), 0, c->s, c->d_aux, L.nx, L.ny, L.nz, nm,
+                       d_present, (const uint8_t*)nullptr, (uint8_t*)nullptr, L, (int32_t*)nullptr);
+    HIPCHK(hipGetLastError());
+    std::vector<unsigned> present((ncode + 31) / 32);
+    HIPCHK(hipMemcpyAsync(present.data(), d_present, sizeof(unsigned) * present.size(), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    std::vector<uint8_t> lut(ncode, 0);
+    std::vector<int8_t> cls = {0, 0, 0};
+    std::vector<double> epsc = {1.0, 1.0, 1.0};
+    int ncls = 1;
+    for (int code = 1; code < ncode; ++code) {
+        if (!((present[code >> 5] >> (code & 31)) & 1u)) continue;
+        if (ncls > 255) return fail(OMX_ERR_RANGE, "more than 255 distinct per-cell medium combinations");
+        lut[code] = (uint8_t)ncls++;
+        const int m0 = code % (nm + 1), m1 = (code / (nm + 1)) % (nm + 1), m2 = code / ((nm + 1) * (nm + 1));
+        const int mm[3] = {m0, m1, m2};
+        for (int q = 0; q < 3; ++q) {
+            cls.push_back((int8_t)mm[q]);
+            epsc.push_back(mm[q] ? c->media[mm[q]].eps_r : 1.0);
+        }
+    }
+    c->n_classes = ncls;
+    rc = dalloc(c, (void**)&d_lut, ncode);
+    if (rc) return rc;
+    rc = dalloc(c, (void**)&c->d_cls, cls.size());
+    if (rc) return rc;
+    rc = dalloc(c, (void**)&c->d_eps_class, sizeof(double) * epsc.size());
+    if (rc) return rc;
+    rc = dalloc(c, (void**)&c->d_tag, L.nalloc);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(d_lut, lut.data(), ncode, hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(c->d_cls, cls.data(), cls.size(), hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(c->d_eps_class, epsc.data(), sizeof(double) * epsc.size(), hipMemcpyHostToDevice, c->s));
+    hipLaunchKernelGGL(k_media_codes, dim3(blocks_for(ncell, 256)), dim3(256), 0, c->s, c->d_aux, L.nx, L.ny, L.nz, nm,
+                       (unsigned*)nullptr, d_lut, c->d_tag, L, (int32_t*)nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->s));
+    c->device_bytes -= (long)(sizeof(unsigned) * ((ncode + 31) / 32) + ncode);
+    dfreet(d_present);
+    dfreet(d_lut);
+    return OMX_OK;
+}
+
+int media_common(omx3d_ctx* c, int n_media, const omx_medium* table) {
+    if (c->media_set) return fail(OMX_ERR_STATE, "media already set for this context");
+    if (n_media < 0 || n_media > OMX_MAX_MEDIA) return fail(OMX_ERR_RANGE, "n_media must be in 0..15");
+    if (n_media > 0 && !table) return fail(OMX_ERR_ARG, "null media table");
+    c->n_media = n_media;
+    std::memset(c->media, 0, sizeof(c->media));
+    c->maxp = 0;
+    c->has_drude = c->has_dl = false;
+    for (int m = 1; m <= n_media; ++m) {
+        c->media[m] = table[m - 1];
+        const omx_medium& M = c->media[m];
+        if (M.kind < OMX_MEDIUM_DIELECTRIC || M.kind > OMX_MEDIUM_DL)
+            return fail(OMX_ERR_ARG, "medium " + std::to_string(m) + ": unknown kind");
+        if (M.kind == OMX_MEDIUM_DL && (M.n_poles < 1 || M.n_poles > OMX_MAX_POLES))
+            return fail(OMX_ERR_RANGE, "medium " + std::to_string(m) + ": n_poles must be in 1..8");
+        if (!(M.eps_r > 0.0)) return fail(OMX_ERR_ARG, "medium " + std::to_string(m) + ": eps_r must be > 0");
+        if (M.kind == OMX_MEDIUM_DRUDE) c->has_drude = true;
+        if (M.kind == OMX_MEDIUM_DL) { c->has_dl = true; c->maxp = std::max(c->maxp, (int)M.n_poles); }
+    }
+    int rc = dalloc(c, (void**)&c->d_med_in, sizeof(omx_medium) * (n_media + 1));
+    if (rc) return rc;
+    rc = dalloc(c, (void**)&c->d_med_co, sizeof(omx_medium_coeffs) * (n_media + 1));
+    if (rc) return rc;
+    const size_t mds = c->prec == 4 ? sizeof(MedDev<float>) : sizeof(MedDev<double>);
+    rc = dalloc(c, (void**)&c->d_med, mds * (n_media + 1));
+    if (rc) return rc;
+    rc = dalloc(c, (void**)&c->d_aux, (size_t)c->L.nx * c->L.ny * c->L.nz);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(c->d_med_in, c->media, sizeof(omx_medium) * (n_media + 1), hipMemcpyHostToDevice, c->s));
+    const double dt = c->d.dt, dt_eps0 = dt / kEps0;
+    if (c->prec == 4)
+        hipLaunchKernelGGL(k_medium_coeffs<float>, dim3(1), dim3(64), 0, c->s, c->d_med_in, n_media, dt, dt_eps0,
+                           c->d_med_co, (MedDev<float>*)c->d_med);
+    else
+        hipLaunchKernelGGL(k_medium_coeffs<double>, dim3(1), dim3(64), 0, c->s, c->d_med_in, n_media, dt, dt_eps0,
+                           c->d_med_co, (MedDev<double>*)c->d_med);
+    HIPCHK(hipGetLastError());
+    return OMX_OK;
+}
+
+int media_finish(omx3d_ctx* c) {
+    int rc = build_tags(c);
+    if (rc) return rc;
+    const size_t bytes = c->esz * (size_t)c->L.nalloc;
+    if (c->has_drude || c->has_dl) {
+        for (int q = 0; q < 3; ++q) {
+            rc = dalloc(c, &c->disp_pd[q], bytes);
+            if (rc) return rc;
+        }
+    }
+    if (c->has_dl) {
+        for (int q = 0; q < 3; ++q) {
+            rc = dalloc(c, &c->disp_eold[q], bytes);
+            if (rc) return rc;
+            rc = dalloc(c, &c->disp_pl[q], bytes * c->maxp);
+            if (rc) return rc;
+            rc = dalloc(c, &c->disp_plold[q], bytes * c->maxp);
+            if (rc) return rc;
+        }
+    }
+    // medium kinds for the J gate of k_j_correct
+    std::vector<int> kinds(OMX_MAX_MEDIA + 1, 0);
+    for (int m = 1; m <= c->n_media; ++m) kinds[m] = c->media[m].kind;
+    rc = dalloc(c, (void**)&c->d_kinds, sizeof(int) * kinds.size());
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(c->d_kinds, kinds.data(), sizeof(int) * kinds.size(), hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    c->media_set = true;
+    return OMX_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+const char* omx_last_error(void) { return g_err.c_str(); }
+
+int omx_version(void) { return 1; }
+
+// get_DL_library_data (classical_medium_mod.f90:125-199).  The reference's
+// literals are default REAL: products of two literals are single precision,
+// a literal times a double is the single value promoted.
+int omx_medium_library(const char* name, double eps_r, omx_medium* out) {
+    if (!name || !out) return fail(OMX_ERR_ARG, "null argument");
+    std::memset(out, 0, sizeof(*out));
+    out->kind = OMX_MEDIUM_DL;
+    out->eps_r = eps_r;
+    const double ev = kEvToAu;
+    auto F = [](float v) { return (double)v; };
+    double wp;
+    std::string n(name);
+    if (n == "Ag") {
+        out->n_poles = 5;
+        out->gamma_d = F(0.048f) * ev; wp = F(9.01f) * ev;
+        out->omega_d = F(9.01f * 0.919238815542512f) * ev;
+        const float w[5] = {0.816f, 4.481f, 8.185f, 9.083f, 20.29f};
+        const float g[5] = {3.886f, 0.452f, 0.065f, 0.916f, 2.419f};
+        const float f[5] = {0.065f, 0.124f, 0.011f, 0.840f, 5.646f};
+        for (int q = 0; q < 5; ++q) { out->omega_k[q] = F(w[q]) * ev; out->gamma_k[q] = F(g[q]) * ev; out->a_k[q] = F(f[q]) * (wp * wp); }
+    } else if (n == "Al") {
+        out->n_poles = 4;
+        out->gamma_d = F(0.047f) * ev; wp = F(14.98f) * ev;
+        out->omega_d = F(14.98f * 0.723187389270582f) * ev;
+        const float w[4] = {0.162f, 1.544f, 1.808f, 3.473f};
+        const float g[4] = {0.333f, 0.312f, 1.351f, 3.382f};
+        const float f[4] = {0.227f, 0.050f, 0.166f, 0.030f};
+        for (int q = 0; q < 4; ++q) { out->omega_k[q] = F(w[q]) * ev; out->gamma_k[q] = F(g[q]) * ev; out->a_k[q] = F(f[q]) * (wp * wp); }
+    } else if (n == "Au") {
+        out->n_poles = 5;
+        out->gamma_d = F(0.053f) * ev; wp = F(9.03f) * ev;
+        out->omega_d = F(9.03f * 0.871779788708135f) * ev;
+        const float w[5] = {0.415f, 0.830f, 2.969f, 4.304f, 13.32f};
+        const float g[5] = {0.241f, 0.345f, 0.870f, 2.494f, 2.214f};
+        const float f[5] = {0.024f, 0.010f, 0.071f, 0.601f, 4.384f};
+        for (int q = 0; q < 5; ++q) { out->omega_k[q] = F(w[q]) * ev; out->gamma_k[q] = F(g[q]) * ev; out->a_k[q] = F(f[q]) * (wp * wp); }
+    } else {
+        return fail(OMX_ERR_ARG, "unknown Drude-Lorentz library material '" + n + "' (Ag, Al, Au)");
+    }
+    return OMX_OK;
+}
+
+int omx3d_create(const omx3d_desc* desc, omx3d_ctx** out) {
+    if (!desc || !out) return fail(OMX_ERR_ARG, "null argument");
+    *out = nullptr;
+    const omx3d_desc& d = *desc;
+    if (d.nx < 2 || d.ny < 2 || d.nz < 2) return fail(OMX_ERR_ARG, "grid must be at least 2 cells per axis");
+    if (d.precision != OMX_FP32 && d.precision != OMX_FP64) return fail(OMX_ERR_ARG, "precision must be 4 or 8");
+    if (!(d.dt > 0.0) || !(d.dr > 0.0)) return fail(OMX_ERR_ARG, "dt and dr must be positive");
+    unsigned mask = 0;
+    const int n[3] = {d.nx, d.ny, d.nz};
+    for (int a = 0; a < 3; ++a) {
+        const int b = d.boundaries[a];
+        if (b != OMX_BC_CLOSE && b != OMX_BC_PERIODIC && b != OMX_BC_CPML)
+            return fail(OMX_ERR_ARG, "boundary code of axis " + std::to_string(a + 1) + " must be 1, 2 or 3");
+        if (b == OMX_BC_CPML) {
+            if (d.npml < 2) return fail(OMX_ERR_ARG, "CPML needs npml >= 2");
+            if (2 * d.npml > n[a]) return fail(OMX_ERR_ARG, "CPML thicker than half the grid on axis " + std::to_string(a + 1));
+            mask |= 3u << (2 * a);
+        }
+    }
+    omx3d_ctx* c = new omx3d_ctx();
+    c->d = d;
+    c->prec = d.precision;
+    c->esz = (size_t)d.precision;
+    c->cpml_mask = mask;
+    Layout& L = c->L;
+    const int A = 128 / (int)c->esz;
+    L.nx = d.nx; L.ny = d.ny; L.nz = d.nz;
+    L.sy = d.ny + 2;
+    L.sz = ((d.nz + 2 + A - 1) / A) * A;
+    L.sx = (long)L.sy * L.sz;
+    L.off = A - 1;
+    L.nalloc = (long)(d.nx + 2) * L.sx + A;
+    int rc = OMX_OK;
+    auto bail = [&](int r) { omx3d_destroy(c); return r; };
+    if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(OMX_ERR_HIP, "hipStreamCreate failed"));
+    for (int q = 0; q < 6; ++q)
+        if ((rc = dalloc(c, &c->f[q], c->esz * L.nalloc))) return bail(rc);
+
+    // coefficient tables
+    const int np = d.npml > 0 ? d.npml : 1;
+    c->co_be = 0; c->co_ce = np + 1; c->co_bh = 2 * (np + 1); c->co_ch = c->co_bh + np;
+    long p = c->co_ch + np;
+    for (int q = 0; q < 6; ++q) { c->co_den[q] = p; p += n[q % 3] + 2; }
+    c->ncoef = p;
+    if (np > 64 && mask) return bail(fail(OMX_ERR_RANGE, "npml above 64 is not supported"));
+    if ((rc = dalloc(c, (void**)&c->dcoef, sizeof(double) * c->ncoef))) return bail(rc);
+    if ((rc = dalloc(c, &c->tcoef, c->esz * c->ncoef))) return bail(rc);
+    CpmlProfile pr;
+    pr.npml = d.npml; pr.dt = d.dt; pr.dr = d.dr;
+    const double sig_to_au = ((kCToAu * kCToAu) * kSecToAu) / (kKgToAu * ((kMToAu * kMToAu) * kMToAu));
+    pr.sig_max = (double)(0.8f * (float)(kGradeM + 1)) / (d.dr * std::sqrt(kMu0 / kEps0));
+    pr.alpha_scale = kAlphaCPML * sig_to_au;
+    const int kmax = std::max(std::max(d.nx, d.ny), std::max(d.nz, np));
+    double* dc = c->dcoef;
+    hipLaunchKernelGGL(k_cpml_coeffs, dim3(blocks_for(kmax, 128)), dim3(128), 0, c->s, pr, d.nx, d.ny, d.nz, mask,
+                       dc + c->co_be, dc + c->co_ce, dc + c->co_bh, dc + c->co_ch, dc + c->co_den[0], dc + c->co_den[1],
+                       dc + c->co_den[2], dc + c->co_den[3], dc + c->co_den[4], dc + c->co_den[5]);
+    if (hipGetLastError() != hipSuccess) return bail(fail(OMX_ERR_HIP, "coefficient kernel launch failed"));
+    std::vector<double> hc(c->ncoef);
+    if (hipMemcpyAsync(hc.data(), dc, sizeof(double) * c->ncoef, hipMemcpyDeviceToHost, c->s) != hipSuccess ||
+        hipStreamSynchronize(c->s) != hipSuccess)
+        return bail(fail(OMX_ERR_HIP, "coefficient readback failed"));
+    for (long q = c->co_ce; q < c->co_ce + np + 1; ++q) hc[q] /= d.dr;  // ce/dr
+    for (long q = c->co_ch; q < c->co_ch + np; ++q) hc[q] /= d.dr;      // ch/dr
+    if (c->prec == 4) {
+        std::vector<float> hf(c->ncoef);
+        for (long q = 0; q < c->ncoef; ++q) hf[q] = (float)hc[q];
+        if (hipMemcpyAsync(c->tcoef, hf.data(), sizeof(float) * c->ncoef, hipMemcpyHostToDevice, c->s) != hipSuccess ||
+            hipStreamSynchronize(c->s) != hipSuccess)
+            return bail(fail(OMX_ERR_HIP, "coefficient upload failed"));
+        float* tf = (float*)c->tcoef;
+        hipLaunchKernelGGL(k_cpml_bm1, dim3(1), dim3(128), 0, c->s, pr, mask, tf + c->co_be, tf + c->co_bh);
+    } else {
+        if (hipMemcpyAsync(c->tcoef, hc.data(), sizeof(double) * c->ncoef, hipMemcpyHostToDevice, c->s) != hipSuccess)
+            return bail(fail(OMX_ERR_HIP, "coefficient upload failed"));
+    }
+
+    // CPML psi arrays (mxll_3D_mod.f90:236-276), non-MPI: two sections per axis.
+    if (mask) {
+        c->nsh = 2 * (d.npml - 1);
+        c->nse = 2 * d.npml;
+        const size_t A2 = (size_t)A;
+        const size_t xs[2] = {c->nsh * L.sx + A2, c->nse * L.sx + A2};
+        const size_t ys[2] = {(size_t)(d.nx + 2) * c->nsh * L.sz + A2, (size_t)(d.nx + 2) * c->nse * L.sz + A2};
+        const size_t zs[2] = {(size_t)(d.nx + 2) * L.sy * c->nsh, (size_t)(d.nx + 2) * L.sy * c->nse};
+        for (int a = 0; a < 3; ++a) {
+            if (!((mask >> (2 * a)) & 3u)) continue;
+            const size_t* sz = a == 0 ? xs : (a == 1 ? ys : zs);
+            for (int q = 0; q < 2; ++q) {
+                if ((rc = dalloc(c, &c->psi_h[2 * a + q], c->esz * sz[0]))) return bail(rc);
+                if ((rc = dalloc(c, &c->psi_e[2 * a + q], c->esz * sz[1]))) return bail(rc);
+            }
+        }
+    }
+    if (hipStreamSynchronize(c->s) != hipSuccess) return bail(fail(OMX_ERR_HIP, "create: stream sync failed"));
+    *out = c;
+    return OMX_OK;
+}
+
+int omx3d_destroy(omx3d_ctx* c) {
+    if (!c) return OMX_OK;
+    if (c->s) (void)hipStreamSynchronize(c->s);
+    for (auto& p : c->f) dfree(p);
+    dfreet(c->dcoef); dfree(c->tcoef);
+    for (auto& p : c->psi_h) dfree(p);
+    for (auto& p : c->psi_e) dfree(p);
+    dfreet(c->d_aux); dfreet(c->d_tag); dfreet(c->d_cls); dfree(c->d_med); dfreet(c->d_med_in); dfreet(c->d_med_co);
+    dfreet(c->d_eps_class); dfreet(c->d_kinds);
+    for (int q = 0; q < 3; ++q) {
+        dfree(c->disp_eold[q]); dfree(c->disp_pd[q]); dfree(c->disp_pl[q]); dfree(c->disp_plold[q]);
+        dfreet(c->d_aff_off[q]); dfreet(c->d_aff_a[q]); dfreet(c->d_aff_b[q]);
+    }
+    for (auto& s : c->sources) { dfreet(s.d_off); dfreet(s.d_ka); dfreet(s.d_kb); }
+    dfreet(c->d_det_field); dfreet(c->d_det_off); dfreet(c->d_rec);
+    dfreet(c->d_q_off); dfreet(c->d_q_winner); dfreet(c->d_J); dfreet(c->d_q_e); dfreet(c->d_q_dpdt); dfreet(c->d_tls);
+    dfreet(c->d_tf); dfreet(c->d_red);
+    for (auto e : c->ev) (void)hipEventDestroy(e);
+    if (c->s) (void)hipStreamDestroy(c->s);
+    delete c;
+    return OMX_OK;
+}
+
+int omx3d_set_tiling(omx3d_ctx* c, int32_t rows_y, int32_t xchunk) {
+    CHECK_CTX(c);
+    if (rows_y != 0) {
+        if (rows_y != 1 && rows_y != 2 && rows_y != 4) return fail(OMX_ERR_ARG, "rows_y must be 1, 2 or 4");
+        c->rows_y = rows_y;
+    }
+    if (xchunk != 0) {
+        if (xchunk < 1) return fail(OMX_ERR_ARG, "xchunk must be >= 1");
+        c->xchunk = xchunk;
+    }
+    return OMX_OK;
+}
+
+int omx3d_get_cpml(omx3d_ctx* c, double* be, double* ce, double* bh, double* ch, double* den) {
+    CHECK_CTX(c);
+    std::vector<double> h(c->ncoef);
+    HIPCHK(hipMemcpyAsync(h.data(), c->dcoef, sizeof(double) * c->ncoef, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    const int np = c->d.npml;
+    if (c->cpml_mask) {
+        for (int k = 1; k <= np; ++k) {
+            if (be) be[k - 1] = h[c->co_be + k];
+            if (ce) ce[k - 1] = h[c->co_ce + k];
+        }
+        for (int k = 1; k <= np - 1; ++k) {
+            if (bh) bh[k - 1] = h[c->co_bh + k];
+            if (ch) ch[k - 1] = h[c->co_ch + k];
+        }
+    }
+    if (den) {
+        const int n[3] = {c->d.nx, c->d.ny, c->d.nz};
+        long w = 0;
+        for (int q = 0; q < 6; ++q)
+            for (int k = 1; k <= n[q % 3]; ++k) den[w++] = h[c->co_den[q] + k];
+    }
+    return OMX_OK;
+}
+
+int omx3d_set_media(omx3d_ctx* c, int32_t n_media, const omx_medium* table, const int8_t* aux) {
+    CHECK_CTX(c);
+    if (!aux) return fail(OMX_ERR_ARG, "null aux map");
+    const long ncell = (long)c->L.nx * c->L.ny * c->L.nz;
+    for (long q = 0; q < ncell; ++q)
+        if (aux[q] < 0 || aux[q] > n_media) return fail(OMX_ERR_ARG, "aux map value out of 0..n_media");
+    int rc = media_common(c, n_media, table);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(c->d_aux, aux, ncell, hipMemcpyHostToDevice, c->s));
+    return media_finish(c);
+}
+
+int omx3d_set_media_shapes(omx3d_ctx* c, int32_t n_media, const omx_medium* table, int32_t n_shapes,
+                           const omx_shape* shapes) {
+    CHECK_CTX(c);
+    if (n_shapes < 0 || (n_shapes > 0 && !shapes)) return fail(OMX_ERR_ARG, "bad shape list");
+    for (int s = 0; s < n_shapes; ++s)
+        if (shapes[s].medium < 0 || shapes[s].medium > n_media || shapes[s].type < 1 || shapes[s].type > 4)
+            return fail(OMX_ERR_ARG, "shape " + std::to_string(s) + ": bad type or medium");
+    int rc = media_common(c, n_media, table);
+    if (rc) return rc;
+    omx_shape* d_sh = nullptr;
+    rc = dalloc(c, (void**)&d_sh, sizeof(omx_shape) * std::max(1, (int)n_shapes));
+    if (rc) return rc;
+    if (n_shapes) HIPCHK(hipMemcpyAsync(d_sh, shapes, sizeof(omx_shape) * n_shapes, hipMemcpyHostToDevice, c->s));
+    const long ncell = (long)c->L.nx * c->L.ny * c->L.nz;
+    hipLaunchKernelGGL(k_rasterize, dim3(blocks_for(ncell, 256)), dim3(256), 0, c->s, d_sh, n_shapes, c->L.nx, c->L.ny,
+                       c->L.nz, c->d_aux);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->s));
+    c->device_bytes -= (long)(sizeof(omx_shape) * std::max(1, (int)n_shapes));
+    dfreet(d_sh);
+    return media_finish(c);
+}
+
+int omx3d_get_media_map(omx3d_ctx* c, int32_t* map) {
+    CHECK_CTX(c);
+    if (!map) return fail(OMX_ERR_ARG, "null output");
+    const long ncell = (long)c->L.nx * c->L.ny * c->L.nz;
+    if (!c->media_set) {
+        std::memset(map, 0, sizeof(int32_t) * 3 * ncell);
+        return OMX_OK;
+    }
+    int32_t* d_map = nullptr;
+    int rc = dalloc(c, (void**)&d_map, sizeof(int32_t) * 3 * ncell);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_media_codes, dim3(blocks_for(ncell, 256)), dim3(256), 0, c->s, c->d_aux, c->L.nx, c->L.ny,
+                       c->L.nz, c->n_media, (unsigned*)nullptr, (const uint8_t*)nullptr, (uint8_t*)nullptr, c->L, d_map);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(map, d_map, sizeof(int32_t) * 3 * ncell, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    c->device_bytes -= (long)(sizeof(int32_t) * 3 * ncell);
+    dfreet(d_map);
+    return OMX_OK;
+}
+
+int omx3d_get_medium_coeffs(omx3d_ctx* c, int32_t m, omx_medium_coeffs* out) {
+    CHECK_CTX(c);
+    if (!c->media_set) return fail(OMX_ERR_STATE, "no media set");
+    if (m < 1 || m > c->n_media || !out) return fail(OMX_ERR_ARG, "medium index out of range");
+    HIPCHK(hipMemcpyAsync(out, c->d_med_co + m, sizeof(omx_medium_coeffs), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return OMX_OK;
+}
+
+int omx3d_step_h(omx3d_ctx* c) {
+    CHECK_CTX(c);
+    return c->prec == 4 ? launch_h<float>(c) : launch_h<double>(c);
+}
+
+int omx3d_step_e(omx3d_ctx* c, double time, double t_skip) {
+    CHECK_CTX(c);
+    return c->prec == 4 ? launch_e<float>(c, time, t_skip) : launch_e<double>(c, time, t_skip);
+}
+
+int omx3d_add_source(omx3d_ctx* c, int32_t dir, double j_amp, double w0, double t0, double tau, double t_init,
+                     double t_final, double phase, int32_t n_ker, const int32_t* ind, const double* ker) {
+    CHECK_CTX(c);
+    if (dir < 1 || dir > 3) return fail(OMX_ERR_ARG, "source direction must be 1, 2 or 3");
+    if (n_ker < 0 || !ind || !ker) return fail(OMX_ERR_ARG, "bad source kernel");
+    const int w = 2 * n_ker + 1;
+    const long np = (long)w * w * w;
+    std::vector<long> off(np);
+    std::vector<double> ka(np), kb(np);
+    const int n[3] = {c->d.nx, c->d.ny, c->d.nz};
+    for (int k = 0; k < w; ++k)
+        for (int j = 0; j < w; ++j)
+            for (int i = 0; i < w; ++i) {
+                const long p = i + (long)w * (j + (long)w * k);
+                const int ii = ind[p], jj = ind[p + np], kk = ind[p + 2 * np];
+                const int idx[3] = {ii, jj, kk};
+                for (int a = 0; a < 3; ++a)
+                    if (idx[a] < 1 || idx[a] > n[a]) return fail(OMX_ERR_RANGE, "source point outside the grid");
+                off[p] = c->L.at(ii, jj, kk);
+                ka[p] = ker[p];
+                int ni = i, nj = j, nk = k;
+                if (dir == 1) ++ni; else if (dir == 2) ++nj; else ++nk;
+                kb[p] = (ni < w && nj < w && nk < w) ? ker[ni + (long)w * (nj + (long)w * nk)] : 0.0;
+            }
+    SourceDev s;
+    s.dir = dir; s.j_amp = j_amp; s.w0 = w0; s.t0 = t0; s.tau = tau; s.t_init = t_init; s.t_final = t_final;
+    s.phase = phase; s.npts = (int)np;
+    s.d_off = nullptr; s.d_ka = nullptr; s.d_kb = nullptr;
+    int rc = dalloc(c, (void**)&s.d_off, sizeof(long) * np);
+    if (!rc) rc = dalloc(c, (void**)&s.d_ka, sizeof(double) * np);
+    if (!rc) rc = dalloc(c, (void**)&s.d_kb, sizeof(double) * np);
+    if (rc) { dfreet(s.d_off); dfreet(s.d_ka); dfreet(s.d_kb); return rc; }
+    HIPCHK(hipMemcpyAsync(s.d_off, off.data(), sizeof(long) * np, hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(s.d_ka, ka.data(), sizeof(double) * np, hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(s.d_kb, kb.data(), sizeof(double) * np, hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    c->sources.push_back(s);
+    return OMX_OK;
+}
+
+int omx3d_inject_sources(omx3d_ctx* c, double time) {
+    CHECK_CTX(c);
+    return c->prec == 4 ? launch_sources<float>(c, time) : launch_sources<double>(c, time);
+}
+
+int omx3d_add_detector(omx3d_ctx* c, int32_t det_type, int32_t field, int32_t nd, const int32_t* idx) {
+    CHECK_CTX(c);
+    if (det_type < OMX_DET_POINT || det_type > OMX_DET_VOLUME) return fail(OMX_ERR_ARG, "bad detector type");
+    if (field < OMX_EX || field > OMX_HZ) return fail(OMX_ERR_ARG, "bad detector field");
+    if (nd < 0 || (nd > 0 && !idx)) return fail(OMX_ERR_ARG, "bad detector index list");
+    if (c->rec_cap > 0) {  // the row width changes: the record buffer is sized again by the caller
+        HIPCHK(hipStreamSynchronize(c->s));
+        c->device_bytes -= (long)(sizeof(double) * (size_t)c->rec_cap * c->n_samples);
+        dfreet(c->d_rec);
+        c->rec_cap = c->rec_rows = 0;
+    }
+    const int n[3] = {c->d.nx, c->d.ny, c->d.nz};
+    for (int q = 0; q < nd; ++q) {
+        const int ii = idx[q], jj = idx[q + nd], kk = idx[q + 2 * nd];
+        const int v[3] = {ii, jj, kk};
+        for (int a = 0; a < 3; ++a)
+            if (v[a] < 1 || v[a] > n[a]) return fail(OMX_ERR_RANGE, "detector index outside the grid");
+        c->h_det_field.push_back(2 * field + (det_type == OMX_DET_POINT ? 1 : 0));
+        c->h_det_off.push_back(c->L.at(ii, jj, kk));
+    }
+    DetectorHost dh;
+    dh.type = det_type; dh.field = field; dh.nd = nd; dh.first = c->n_samples;
+    c->dets.push_back(dh);
+    c->n_samples += nd;
+    c->det_dirty = true;
+    return OMX_OK;
+}
+
+int omx3d_set_record_capacity(omx3d_ctx* c, int32_t rows) {
+    CHECK_CTX(c);
+    if (rows < 0) return fail(OMX_ERR_ARG, "rows must be >= 0");
+    dfreet(c->d_rec);
+    c->rec_cap = 0;
+    c->rec_rows = 0;
+    if (rows > 0 && c->n_samples > 0) {
+        int rc = dalloc(c, (void**)&c->d_rec, sizeof(double) * (size_t)rows * c->n_samples);
+        if (rc) return rc;
+        c->rec_cap = rows;
+    }
+    return upload_detectors(c);
+}
+
+int omx3d_record(omx3d_ctx* c) {
+    CHECK_CTX(c);
+    int rc = upload_detectors(c);
+    if (rc) return rc;
+    if (c->n_samples == 0) return OMX_OK;
+    if (c->rec_rows >= c->rec_cap) return fail(OMX_ERR_RANGE, "detector record buffer full");
+    double* dst = c->d_rec + (long)c->rec_rows * c->n_samples;
+    rc = c->prec == 4 ? launch_record<float>(c, dst) : launch_record<double>(c, dst);
+    if (rc) return rc;
+    c->rec_rows++;
+    return OMX_OK;
+}
+
+int omx3d_read_records(omx3d_ctx* c, double* out, int32_t* rows) {
+    CHECK_CTX(c);
+    if (rows) *rows = c->rec_rows;
+    if (out && c->rec_rows > 0)
+        HIPCHK(hipMemcpyAsync(out, c->d_rec, sizeof(double) * (size_t)c->rec_rows * c->n_samples, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return OMX_OK;
+}
+
+int omx3d_sample(omx3d_ctx* c, double* out) {
+    CHECK_CTX(c);
+    if (!out) return fail(OMX_ERR_ARG, "null output");
+    int rc = upload_detectors(c);
+    if (rc) return rc;
+    if (c->n_samples == 0) return OMX_OK;
+    double* tmp = nullptr;
+    rc = dalloc(c, (void**)&tmp, sizeof(double) * c->n_samples);
+    if (rc) return rc;
+    rc = c->prec == 4 ? launch_record<float>(c, tmp) : launch_record<double>(c, tmp);
+    if (!rc) {
+        if (hipMemcpyAsync(out, tmp, sizeof(double) * c->n_samples, hipMemcpyDeviceToHost, c->s) != hipSuccess ||
+            hipStreamSynchronize(c->s) != hipSuccess)
+            rc = fail(OMX_ERR_HIP, "sample readback failed");
+    }
+    c->device_bytes -= (long)(sizeof(double) * c->n_samples);
+    dfreet(tmp);
+    return rc;
+}
+
+int omx3d_set_q_sites(omx3d_ctx* c, int32_t n, const int32_t* ijk, double density, double dt_q) {
+    CHECK_CTX(c);
+    if (c->nq) return fail(OMX_ERR_STATE, "quantum sites already set");
+    if (n <= 0 || !ijk) return fail(OMX_ERR_ARG, "bad site list");
+    if (!(dt_q > 0.0)) return fail(OMX_ERR_ARG, "dt_q must be positive");
+    const int nn[3] = {c->d.nx, c->d.ny, c->d.nz};
+    std::map<std::tuple<int, int, int>, int> slot_of;
+    std::vector<std::tuple<int, int, int>> slot_cell;
+    std::vector<int> winner;
+    std::vector<long> q_off(n);
+    for (int q = 0; q < n; ++q) {
+        const int v[3] = {ijk[q], ijk[q + n], ijk[q + 2 * n]};
+        for (int a = 0; a < 3; ++a)
+            if (v[a] < 1 || v[a] > nn[a]) return fail(OMX_ERR_RANGE, "site index outside the grid");
+        auto key = std::make_tuple(v[0], v[1], v[2]);
+        auto it = slot_of.find(key);
+        if (it == slot_of.end()) {
+            slot_of[key] = (int)slot_cell.size();
+            slot_cell.push_back(key);
+            winner.push_back(q);
+        } else {
+            winner[it->second] = q;
+        }
+        q_off[q] = c->L.at(v[0], v[1], v[2]);
+    }
+    const int ns = (int)slot_cell.size();
+    int rc = dalloc(c, (void**)&c->d_q_off, sizeof(long) * n);
+    if (!rc) rc = dalloc(c, (void**)&c->d_q_winner, sizeof(int) * ns);
+    if (!rc) rc = dalloc(c, (void**)&c->d_J, sizeof(double) * 9 * ns);
+    if (!rc) rc = dalloc(c, (void**)&c->d_q_e, sizeof(double) * 3 * n);
+    if (!rc) rc = dalloc(c, (void**)&c->d_q_dpdt, sizeof(double) * 3 * n);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(c->d_q_off, q_off.data(), sizeof(long) * n, hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(c->d_q_winner, winner.data(), sizeof(int) * ns, hipMemcpyHostToDevice, c->s));
+    // cells whose E component sees J: the site cell (term i) and the cell one
+    // below along the component axis (term i+1)
+    for (int comp = 0; comp < 3; ++comp) {
+        std::map<std::tuple<int, int, int>, std::pair<int, int>> aff;
+        for (int s = 0; s < ns; ++s) {
+            int v[3] = {std::get<0>(slot_cell[s]), std::get<1>(slot_cell[s]), std::get<2>(slot_cell[s])};
+            aff[std::make_tuple(v[0], v[1], v[2])];
+            v[comp] -= 1;
+            if (v[comp] >= 1) aff[std::make_tuple(v[0], v[1], v[2])];
+        }
+        std::vector<long> o;
+        std::vector<int> A, B;
+        for (auto& kv : aff) {
+            int v[3] = {std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first)};
+            auto ia = slot_of.find(kv.first);
+            int w[3] = {v[0], v[1], v[2]};
+            w[comp] += 1;
+            auto ib = slot_of.find(std::make_tuple(w[0], w[1], w[2]));
+            o.push_back(c->L.at(v[0], v[1], v[2]));
+            A.push_back(ia == slot_of.end() ? -1 : ia->second);
+            B.push_back(ib == slot_of.end() ? -1 : ib->second);
+        }
+        c->n_aff[comp] = (int)o.size();
+        rc = dalloc(c, (void**)&c->d_aff_off[comp], sizeof(long) * o.size());
+        if (!rc) rc = dalloc(c, (void**)&c->d_aff_a[comp], sizeof(int) * A.size());
+        if (!rc) rc = dalloc(c, (void**)&c->d_aff_b[comp], sizeof(int) * B.size());
+        if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(c->d_aff_off[comp], o.data(), sizeof(long) * o.size(), hipMemcpyHostToDevice, c->s));
+        HIPCHK(hipMemcpyAsync(c->d_aff_a[comp], A.data(), sizeof(int) * A.size(), hipMemcpyHostToDevice, c->s));
+        HIPCHK(hipMemcpyAsync(c->d_aff_b[comp], B.data(), sizeof(int) * B.size(), hipMemcpyHostToDevice, c->s));
+    }
+    HIPCHK(hipStreamSynchronize(c->s));
+    c->nq = n;
+    c->nslot = ns;
+    c->q_density = density;
+    c->q_dt = dt_q;
+    return OMX_OK;
+}
+
+int omx3d_gather_e(omx3d_ctx* c, double* e) {
+    CHECK_CTX(c);
+    if (!c->nq) return fail(OMX_ERR_STATE, "no quantum sites");
+    if (c->prec == 4)
+        hipLaunchKernelGGL(k_gather_e<float>, dim3(blocks_for(c->nq, 256)), dim3(256), 0, c->s, make_args<float>(c),
+                           c->d_q_off, c->nq, c->d_q_e);
+    else
+        hipLaunchKernelGGL(k_gather_e<double>, dim3(blocks_for(c->nq, 256)), dim3(256), 0, c->s, make_args<double>(c),
+                           c->d_q_off, c->nq, c->d_q_e);
+    HIPCHK(hipGetLastError());
+    std::vector<double> h(3 * (size_t)c->nq);
+    HIPCHK(hipMemcpyAsync(h.data(), c->d_q_e, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    if (e)
+        for (int q = 0; q < c->nq; ++q)
+            for (int a = 0; a < 3; ++a) e[q + (long)c->nq * a] = h[3 * q + a];
+    return OMX_OK;
+}
+
+int omx3d_scatter_j(omx3d_ctx* c, const double* dpdt) {
+    CHECK_CTX(c);
+    if (!c->nq) return fail(OMX_ERR_STATE, "no quantum sites");
+    if (!dpdt) return fail(OMX_ERR_ARG, "null dPt/dt");
+    std::vector<double> h(3 * (size_t)c->nq);
+    for (int q = 0; q < c->nq; ++q)
+        for (int a = 0; a < 3; ++a) h[3 * q + a] = dpdt[q + (long)c->nq * a];
+    HIPCHK(hipMemcpyAsync(c->d_q_dpdt, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->s));
+    hipLaunchKernelGGL(k_scatter_j, dim3(blocks_for(c->nslot, 256)), dim3(256), 0, c->s, c->d_q_winner, c->nslot,
+                       c->d_q_dpdt, c->q_density, c->q_dt, c->d_J);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->s));
+    return OMX_OK;
+}
+
+int omx3d_set_two_level(omx3d_ctx* c, const double* omega0, const double* mu, const double* pol,
+                        const double* gamma2) {
+    CHECK_CTX(c);
+    if (!c->nq) return fail(OMX_ERR_STATE, "set quantum sites first");
+    if (!omega0 || !mu || !pol || !gamma2) return fail(OMX_ERR_ARG, "null emitter parameter");
+    const int n = c->nq;
+    std::vector<double> st(16 * (size_t)n, 0.0);
+    for (int q = 0; q < n; ++q) {
+        double* s = &st[16 * (size_t)q];
+        s[0] = omega0[q]; s[1] = mu[q];
+        const double px = pol[q], py = pol[q + n], pz = pol[q + 2 * n];
+        const double nr = std::sqrt(px * px + py * py + pz * pz);
+        if (!(nr > 0.0)) return fail(OMX_ERR_ARG, "zero emitter polarisation");
+        s[2] = px / nr; s[3] = py / nr; s[4] = pz / nr;
+        s[5] = gamma2[q];
+        s[6] = 1.0;  // ground state
+    }
+    if (!c->d_tls) {
+        int rc = dalloc(c, (void**)&c->d_tls, sizeof(double) * st.size());
+        if (rc) return rc;
+    }
+    HIPCHK(hipMemcpyAsync(c->d_tls, st.data(), sizeof(double) * st.size(), hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    c->tls = true;
+    return OMX_OK;
+}
+
+int omx3d_two_level_step(omx3d_ctx* c) {
+    CHECK_CTX(c);
+    if (!c->tls) return fail(OMX_ERR_STATE, "no two-level emitters");
+    return c->prec == 4 ? launch_two_level<float>(c) : launch_two_level<double>(c);
+}
+
+int omx3d_get_two_level(omx3d_ctx* c, double* dipole, double* rho) {
+    CHECK_CTX(c);
+    if (!c->tls) return fail(OMX_ERR_STATE, "no two-level emitters");
+    const int n = c->nq;
+    std::vector<double> st(16 * (size_t)n);
+    HIPCHK(hipMemcpyAsync(st.data(), c->d_tls, sizeof(double) * st.size(), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    for (int q = 0; q < n; ++q) {
+        const double* s = &st[16 * (size_t)q];
+        for (int a = 0; a < 3; ++a) if (dipole) dipole[q + (long)n * a] = s[10 + a];
+        for (int a = 0; a < 4; ++a) if (rho) rho[q + (long)n * a] = s[6 + a];
+    }
+    return OMX_OK;
+}
+
+int omx3d_copy_field(omx3d_ctx* c, int32_t field, double* host, int32_t to_device) {
+    CHECK_CTX(c);
+    if (field < 1 || field > 6 || !host) return fail(OMX_ERR_ARG, "bad field or null buffer");
+    return c->prec == 4 ? copy_field_impl<float>(c, field, host, to_device)
+                        : copy_field_impl<double>(c, field, host, to_device);
+}
+
+int omx3d_energy(omx3d_ctx* c, double* w64, double* w32c) {
+    CHECK_CTX(c);
+    return c->prec == 4 ? energy_impl<float>(c, w64, w32c) : energy_impl<double>(c, w64, w32c);
+}
+
+int omx3d_set_tfsf(omx3d_ctx* c, const int32_t lo[3], const int32_t hi[3], double e0, double w0, double t0,
+                   double tau, int32_t max_steps) {
+    CHECK_CTX(c);
+    if (!lo || !hi) return fail(OMX_ERR_ARG, "null box");
+    const int n[3] = {c->d.nx, c->d.ny, c->d.nz};
+    for (int a = 0; a < 3; ++a) {
+        const int guard = c->d.boundaries[a] == OMX_BC_CPML ? c->d.npml + 1 : 2;
+        if (lo[a] < guard || hi[a] > n[a] - guard || hi[a] - lo[a] < 2)
+            return fail(OMX_ERR_RANGE, "TF/SF box must lie inside the non-CPML interior with at least 2 cells per axis");
+    }
+    if (max_steps < 1) return fail(OMX_ERR_ARG, "max_steps must be >= 1");
+    for (int a = 0; a < 3; ++a) { c->tf_lo[a] = lo[a]; c->tf_hi[a] = hi[a]; }
+    // 1D line long enough that nothing reflected at its ends reaches the box
+    // within max_steps (the pulse moves c*dt/dr = 1/2 cell per step).
+    const int pad = max_steps / 2 + 16;
+    c->tf_ks = lo[2] - 3;
+    c->tf_kb = c->tf_ks - pad;
+    const int ke = hi[2] + pad;
+    c->tf_n = ke - c->tf_kb + 1;
+    c->tf_e0 = e0; c->tf_w0 = w0; c->tf_t0 = t0; c->tf_tau = tau;
+    dfreet(c->d_tf);
+    int rc = dalloc(c, (void**)&c->d_tf, sizeof(double) * 2 * c->tf_n);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->s));
+    c->tfsf = true;
+    return OMX_OK;
+}
+
+int omx3d_run(omx3d_ctx* c, int32_t tt_first, int32_t n_steps, int32_t record_every) {
+    CHECK_CTX(c);
+    if (n_steps < 0) return fail(OMX_ERR_ARG, "n_steps must be >= 0");
+    return c->prec == 4 ? run_impl<float>(c, tt_first, n_steps, record_every)
+                        : run_impl<double>(c, tt_first, n_steps, record_every);
+}
+
+int omx3d_set_profiling(omx3d_ctx* c, int32_t enable) {
+    CHECK_CTX(c);
+    c->prof = enable != 0;
+    c->prof_ms_h = c->prof_ms_e = 0.0;
+    c->prof_n = 0;
+    return OMX_OK;
+}
+
+int omx3d_get_profile(omx3d_ctx* c, double* ms_h, double* ms_e, int32_t* n) {
+    CHECK_CTX(c);
+    if (ms_h) *ms_h = c->prof_ms_h;
+    if (ms_e) *ms_e = c->prof_ms_e;
+    if (n) *n = c->prof_n;
+    return OMX_OK;
+}
+
+int omx3d_synchronize(omx3d_ctx* c) {
+    CHECK_CTX(c);
+    HIPCHK(hipStreamSynchronize(c->s));
+    return OMX_OK;
+}
+
+int omx3d_device_bytes(omx3d_ctx* c, int64_t* bytes) {
+    CHECK_CTX(c);
+    if (bytes) *bytes = c->device_bytes;
+    return OMX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,893 @@
+// omx_kernels.hpp -- gfx950 kernels of libomx_fdtd_hip.
+//
+// Every stencil kernel works on the private layout of omx::Layout (z fastest).
+// A workgroup is 64 lanes along z times `rows` rows along y and marches a
+// chunk of x planes, carrying the x-neighbour values in registers; the y and
+// z neighbours are re-read from L1/L2.  Workgroup ids are remapped so that
+// consecutive tiles (which share halo rows) land on the same XCD.
+#pragma once
+
+#include "omx_internal.hpp"
+
+namespace omx {
+
+constexpr int kLanes = 64;
+
+// Bijective map from the hardware workgroup id (dispatched round-robin over
+// the 8 XCDs of an MI355X) to a tile id such that each XCD receives one
+// contiguous run of tiles.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+    const int per = nb >> 3, rem = nb & 7;
+    const int x = b & 7, q = b >> 3;
+    return x * per + (x < rem ? x : rem) + q;
+}
+
+// ---------------------------------------------------------------------------
+// K8: FP64 coefficient precompute on the device.
+// CPML grading follows init_3Dgrid (src/FDTD/mxll_3D_mod.f90:294-318): the
+// grading ratios and their cubes are evaluated in single precision, exactly
+// as the reference's default-REAL expressions are, then promoted.
+// ---------------------------------------------------------------------------
+__device__ inline double cube_f(float r) { return (double)((r * r) * r); }
+
+struct CpmlProfile {
+    double sig_max, alpha_scale, dt, dr;
+    int npml;
+};
+
+__device__ inline void cpml_e(const CpmlProfile& p, int kk, double& sig, double& alpha, double& kappa) {
+#pragma clang fp contract(off)
+    const float den = (float)p.npml - 1.0f;
+    const float r = (float)(p.npml - kk) / den;
+    const double r3 = cube_f(r);
+    sig = p.sig_max * r3;
+    alpha = p.alpha_scale * (double)((float)(kk - 1) / den);
+    kappa = 1.0 + (kKappaCPML - 1.0) * r3;
+}
+
+__device__ inline void cpml_h(const CpmlProfile& p, int kk, double& sig, double& alpha, double& kappa) {
+#pragma clang fp contract(off)
+    const float den = (float)p.npml - 1.0f;
+    const float r = ((float)(p.npml - kk) - 0.5f) / den;
+    const double r3 = cube_f(r);
+    sig = p.sig_max * r3;
+    alpha = p.alpha_scale * (double)(((float)kk - 0.5f) / den);
+    kappa = 1.0 + (kKappaCPML - 1.0) * r3;
+}
+
+// One thread per 1-based index k in 1..max(nx,ny,nz,npml).
+// out layout: be[0..npml] ce[..] bh[0..npml-1] ch[..] den_ex[0..nx+1] den_ey den_ez den_hx den_hy den_hz
+__global__ void k_cpml_coeffs(CpmlProfile p, int nx, int ny, int nz, unsigned mask,
+                              double* be, double* ce, double* bh, double* ch,
+                              double* dex, double* dey, double* dez,
+                              double* dhx, double* dhy, double* dhz) {
+#pragma clang fp contract(off)
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int npml = p.npml;
+    const bool any = mask != 0u;
+    if (any && k <= npml) {
+        double s, a, kap;
+        cpml_e(p, k, s, a, kap);
+        const double b = exp(-((s / kap + a) * p.dt) / kEps0);
+        be[k] = b;
+        if (s == 0.0 && a == 0.0 && k == npml) ce[k] = 0.0;
+        else ce[k] = ((s * (b - 1.0)) / (s + kap * a)) / kap;
+    }
+    if (any && k <= npml - 1) {
+        double s, a, kap;
+        cpml_h(p, k, s, a, kap);
+        const double b = exp(-((s / kap + a) * p.dt) / kEps0);
+        bh[k] = b;
+        ch[k] = ((s * (b - 1.0)) / (s + kap * a)) / kap;
+    }
+    const double inv = 1.0 / p.dr;
+    const int nn[3] = {nx, ny, nz};
+    double* de[3] = {dex, dey, dez};
+    double* dh[3] = {dhx, dhy, dhz};
+    for (int ax = 0; ax < 3; ++ax) {
+        const int n = nn[ax];
+        if (k > n) continue;
+        double ve = inv, vh = inv;
+        const bool lo = (mask >> (2 * ax)) & 1u, hi = (mask >> (2 * ax + 1)) & 1u;
+        double s, a, kap;
+        if (lo && k <= npml) { cpml_e(p, k, s, a, kap); ve = 1.0 / (kap * p.dr); }
+        if (lo && k <= npml - 1) { cpml_h(p, k, s, a, kap); vh = 1.0 / (kap * p.dr); }
+        if (hi && k >= n + 1 - npml && k <= n - 1) {
+            cpml_e(p, n + 1 - k, s, a, kap); ve = 1.0 / (kap * p.dr);
+            cpml_h(p, n - k, s, a, kap);     vh = 1.0 / (kap * p.dr);
+        }
+        de[ax][k] = ve;
+        dh[ax][k] = vh;
+    }
+}
+
+// FP32 b-1 with full relative precision: recompute from the exponent.
+__global__ void k_cpml_bm1(CpmlProfile p, unsigned mask, float* be_t, float* bh_t) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    if (mask == 0u) return;
+    double s, a, kap;
+    if (k <= p.npml) { cpml_e(p, k, s, a, kap); be_t[k] = (float)expm1(-((s / kap + a) * p.dt) / kEps0); }
+    if (k <= p.npml - 1) { cpml_h(p, k, s, a, kap); bh_t[k] = (float)expm1(-((s / kap + a) * p.dt) / kEps0); }
+}
+
+// ---------------------------------------------------------------------------
+// K8 (media): update coefficients of the classical media, device FP64,
+// following init_classical_medium (classical_medium_mod.f90:60-101).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_medium_coeffs(const omx_medium* __restrict__ in, int n, double dt, double dt_eps0,
+                                omx_medium_coeffs* __restrict__ co, MedDev<T>* __restrict__ md) {
+#pragma clang fp contract(off)
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m > n) return;
+    omx_medium_coeffs c;
+    MedDev<T> d;
+    for (int q = 0; q < OMX_MAX_POLES; ++q) {
+        c.alpha_k[q] = c.beta_k[q] = c.gamma_k[q] = c.B1_k[q] = c.B2_k[q] = 0.0;
+        d.al[q] = d.bt[q] = d.ga[q] = d.B1[q] = d.B2[q] = (T)0;
+    }
+    c.A1 = c.A2 = c.C1 = c.C2 = c.C3 = c.C4 = 0.0;
+    c.inv_eps = 1.0;
+    d.kind = OMX_MEDIUM_NONE; d.np = 0;
+    d.cdt = (T)dt_eps0;
+    d.C1 = d.C2 = d.C3 = d.C4 = d.A1 = d.A2 = (T)0;
+    if (m > 0) {
+        const omx_medium& M = in[m];
+        d.kind = M.kind;
+        if (M.kind == OMX_MEDIUM_DIELECTRIC) {
+            c.inv_eps = 1.0 / M.eps_r;
+            d.cdt = (T)(dt_eps0 * c.inv_eps);
+        } else if (M.kind == OMX_MEDIUM_DRUDE) {
+            const double gd = M.gamma_d * dt;
+            c.A1 = (2.0 - gd) / (2.0 + gd);
+            c.A2 = ((kEps0 * (M.omega_d * M.omega_d)) * dt) / (2.0 + gd);
+            const double X = (M.eps_r * kEps0) / dt;
+            c.C1 = (X - 0.5 * c.A2) / (X + 0.5 * c.A2);
+            c.C3 = 1.0 / (X + 0.5 * c.A2);
+            c.C4 = (0.5 * (c.A1 + 1.0)) / (X + 0.5 * c.A2);
+            const double c1m1 = -c.A2 / (X + 0.5 * c.A2);
+            const double a1m1 = -2.0 * gd / (2.0 + gd);
+            d.C1 = (T)(sizeof(T) == 4 ? c1m1 : c.C1);
+            d.A1 = (T)(sizeof(T) == 4 ? a1m1 : c.A1);
+            d.A2 = (T)c.A2; d.C3 = (T)c.C3; d.C4 = (T)c.C4;
+        } else if (M.kind == OMX_MEDIUM_DL) {
+            const int np = M.n_poles;
+            d.np = np;
+            double S = 0.0;
+            for (int q = 0; q < np; ++q) {
+                const double w = M.omega_k[q], g = M.gamma_k[q];
+                c.alpha_k[q] = (2.0 - (w * w) * (dt * dt)) / (1.0 + (0.5 * g) * dt);
+                c.beta_k[q] = (g * dt - 2.0) / (g * dt + 2.0);
+                c.gamma_k[q] = ((kEps0 * M.a_k[q]) * dt) / (g * dt + 2.0);
+                S = S + c.gamma_k[q];
+            }
+            const double gd = M.gamma_d * dt;
+            c.A1 = (2.0 - gd) / (2.0 + gd);
+            c.A2 = ((kEps0 * (M.omega_d * M.omega_d)) * dt) / (2.0 + gd);
+            const double X = (M.eps_r * kEps0) / dt;
+            const double D = (X + 0.5 * c.A2) + 0.5 * S;
+            c.C1 = (X - 0.5 * c.A2) / D;
+            c.C2 = (0.5 * S) / D;
+            c.C3 = 1.0 / D;
+            c.C4 = (0.5 * (c.A1 + 1.0)) / D;
+            for (int q = 0; q < np; ++q) {
+                c.B1_k[q] = (0.5 * (1.0 + c.alpha_k[q])) / D;
+                c.B2_k[q] = (0.5 * c.beta_k[q]) / D;
+            }
+            const bool f32 = sizeof(T) == 4;
+            d.C1 = (T)(f32 ? -(c.A2 + 0.5 * S) / D : c.C1);
+            d.A1 = (T)(f32 ? -2.0 * gd / (2.0 + gd) : c.A1);
+            d.C2 = (T)c.C2; d.C3 = (T)c.C3; d.C4 = (T)c.C4; d.A2 = (T)c.A2;
+            for (int q = 0; q < np; ++q) {
+                const double w = M.omega_k[q], g = M.gamma_k[q];
+                const double am2 = -((w * w) * (dt * dt) + g * dt) / (1.0 + (0.5 * g) * dt);
+                const double bp1 = (2.0 * g * dt) / (g * dt + 2.0);
+                d.al[q] = (T)(f32 ? am2 : c.alpha_k[q]);
+                d.bt[q] = (T)(f32 ? bp1 : c.beta_k[q]);
+                d.ga[q] = (T)c.gamma_k[q];
+                d.B1[q] = (T)c.B1_k[q];
+                d.B2[q] = (T)c.B2_k[q];
+            }
+        }
+    }
+    co[m] = c;
+    md[m] = d;
+}
+
+// ---------------------------------------------------------------------------
+// Media map: cell map (aux, Fortran order) -> staggered per-component map
+// (read_init_media_3D, classical_medium_mod.f90:694-745), then one-byte tags.
+// ---------------------------------------------------------------------------
+__global__ void k_rasterize(const omx_shape* __restrict__ sh, int ns, int nx, int ny, int nz,
+                            int8_t* __restrict__ aux) {
+#pragma clang fp contract(off)
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = (long)nx * ny * nz;
+    if (c >= n) return;
+    const int i = (int)(c % nx) + 1;
+    const int j = (int)((c / nx) % ny) + 1;
+    const int k = (int)(c / ((long)nx * ny)) + 1;
+    const double x = i, y = j, z = k;
+    int v = 0;
+    for (int s = 0; s < ns; ++s) {
+        const double* p = sh[s].p;
+        bool in = false;
+        switch (sh[s].type) {
+            case OMX_SHAPE_SPHERE: {
+                const double dx = x - p[0], dy = y - p[1], dz = z - p[2];
+                in = dx * dx + dy * dy + dz * dz <= p[3] * p[3];
+            } break;
+            case OMX_SHAPE_ELLIPSOID: {
+                const double dx = x - p[0], dy = y - p[1], dz = z - p[2];
+                const double u = p[6] * dx + p[7] * dy + p[8] * dz;
+                const double w = p[9] * dx + p[10] * dy + p[11] * dz;
+                const double t = p[12] * dx + p[13] * dy + p[14] * dz;
+                in = (u / p[3]) * (u / p[3]) + (w / p[4]) * (w / p[4]) + (t / p[5]) * (t / p[5]) <= 1.0;
+            } break;
+            case OMX_SHAPE_BOX:
+                in = x >= p[0] && y >= p[1] && z >= p[2] && x <= p[3] && y <= p[4] && z <= p[5];
+                break;
+            case OMX_SHAPE_CYLINDER_Z: {
+                const double dx = x - p[0], dy = y - p[1];
+                in = dx * dx + dy * dy <= p[2] * p[2] && z >= p[3] && z <= p[4];
+            } break;
+            default: break;
+        }
+        if (!in) continue;
+        if (sh[s].medium == 0) v = 0;
+        else if (v == 0) v = sh[s].medium;
+    }
+    aux[c] = (int8_t)v;
+}
+
+// Per-component medium ids of cell (i,j,k), a literal restatement of the
+// reference's per-medium if/else chain.  Neighbours past the grid read 0.
+__device__ inline void stagger(const int8_t* __restrict__ aux, int nx, int ny, int nz, int nm,
+                               int i, int j, int k, int m[3]) {
+    auto A = [&](int a, int b, int c) -> int {
+        if (a > nx || b > ny || c > nz) return 0;
+        return aux[(long)(a - 1) + (long)nx * ((b - 1) + (long)ny * (c - 1))];
+    };
+    const int a0 = A(i, j, k);
+    const int ax = A(i + 1, j, k), ay = A(i, j + 1, k), az = A(i, j, k + 1);
+    m[0] = m[1] = m[2] = 0;
+    for (int n = 1; n <= nm; ++n) {
+        if (i == nx && a0 == n) m[0] = n;
+        else if (j == ny && a0 == n) m[0] = n;
+        else if (k == nz && a0 == n) m[0] = n;
+        else {
+            if ((a0 == n && ax == n) || (a0 != 0 && ax == n)) m[0] = n;
+            if ((a0 == n && ay == n) || (a0 != 0 && ay == n)) m[1] = n;
+            if ((a0 == n && az == n) || (a0 != 0 && az == n)) m[2] = n;
+        }
+    }
+}
+
+// Pass 1: presence of each (mx,my,mz) code.  Pass 2: tag = lut[code].
+__global__ void k_media_codes(const int8_t* __restrict__ aux, int nx, int ny, int nz, int nm,
+                              unsigned* __restrict__ present, const uint8_t* __restrict__ lut,
+                              uint8_t* __restrict__ tag, Layout L, int32_t* __restrict__ map_out) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = (long)nx * ny * nz;
+    if (c >= n) return;
+    const int i = (int)(c % nx) + 1;
+    const int j = (int)((c / nx) % ny) + 1;
+    const int k = (int)(c / ((long)nx * ny)) + 1;
+    int m[3];
+    stagger(aux, nx, ny, nz, nm, i, j, k, m);
+    const int code = m[0] + (nm + 1) * (m[1] + (nm + 1) * m[2]);
+    if (present) atomicOr(&present[code >> 5], 1u << (code & 31));
+    if (lut) tag[L.at(i, j, k)] = lut[code];
+    if (map_out) { map_out[c] = m[0]; map_out[c + n] = m[1]; map_out[c + 2 * n] = m[2]; }
+}
+
+// ---------------------------------------------------------------------------
+// H update: td_propagate_H_3Dfield (mxll_3D_mod.f90:546-694): bulk curl of E
+// for i,j,k = 1..n, then the CPML psi recursions of the six faces, applied to
+// each component in the reference's face order.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T psi_step(T psi, T b, T c, T d) {
+    if constexpr (sizeof(T) == 4) return psi + (b * psi + c * d);  // b holds b-1
+    else return b * psi + c * d;
+}
+
+template <typename T, bool CPML>
+__global__ __launch_bounds__(256) void k_h_update(const YeeArgs<T> a) {
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int tz = t % a.ntz;
+    const int r = t / a.ntz;
+    const int ty = r % a.nty;
+    const int tx = r / a.nty;
+    const int k = tz * kLanes + threadIdx.x + 1;
+    const int j = ty * blockDim.y + threadIdx.y + 1;
+    if (k > a.nz || j > a.ny) return;
+    const int i0 = tx * a.xchunk + 1;
+    const int i1 = min(i0 + a.xchunk - 1, a.nx);
+    const long sx = a.sx;
+    const int sz = a.sz;
+    const long jk = (long)j * sz + k + a.off;
+    long o = (long)i0 * sx + jk;
+    const T dmu = a.dt_mu0;
+    const T dhy = a.den_hy[j], dhz = a.den_hz[k];
+    T ezc = a.ez[o], eyc = a.ey[o];
+    const int npml = a.npml;
+    for (int i = i0; i <= i1; ++i, o += sx) {
+        const T ezn = a.ez[o + sx], eyn = a.ey[o + sx];
+        const T ezj = a.ez[o + sz], eyk = a.ey[o + 1];
+        const T exc = a.ex[o], exk = a.ex[o + 1], exj = a.ex[o + sz];
+        const T dhx = a.den_hx[i];
+        T hx = a.hx[o] + dmu * ((ezc - ezj) * dhy + (eyk - eyc) * dhz);
+        T hy = a.hy[o] + dmu * ((ezn - ezc) * dhx + (exc - exk) * dhz);
+        T hz = a.hz[o] + dmu * ((eyc - eyn) * dhx + (exj - exc) * dhy);
+        if constexpr (CPML) {
+            const unsigned cp = a.cpml;
+            const int nsh = 2 * (npml - 1);
+            // x faces: psix_Hyx, psix_Hzx
+            if ((cp & 1u) && i <= npml - 1) {
+                const int c = i;
+                const long ps = (long)(i - 1) * sx + jk;
+                T p0 = psi_step(a.ph[0][ps], a.bh[c], a.chd[c], ezn - ezc);
+                T p1 = psi_step(a.ph[1][ps], a.bh[c], a.chd[c], eyc - eyn);
+                a.ph[0][ps] = p0; a.ph[1][ps] = p1;
+                hy = hy + dmu * p0; hz = hz + dmu * p1;
+            }
+            if ((cp & 2u) && i >= a.nx + 1 - npml && i <= a.nx - 1) {
+                const int c = a.nx - i;
+                const long ps = (long)(npml - 2 + c) * sx + jk;
+                T p0 = psi_step(a.ph[0][ps], a.bh[c], a.chd[c], ezn - ezc);
+                T p1 = psi_step(a.ph[1][ps], a.bh[c], a.chd[c], eyc - eyn);
+                a.ph[0][ps] = p0; a.ph[1][ps] = p1;
+                hy = hy + dmu * p0; hz = hz + dmu * p1;
+            }
+            // y faces: psiy_Hxy, psiy_Hzy
+            if ((cp & 4u) && j <= npml - 1) {
+                const int c = j;
+                const long ps = ((long)i * nsh + (j - 1)) * sz + k + a.off;
+                T p0 = psi_step(a.ph[2][ps], a.bh[c], a.chd[c], ezc - ezj);
+                T p1 = psi_step(a.ph[3][ps], a.bh[c], a.chd[c], exj - exc);
+                a.ph[2][ps] = p0; a.ph[3][ps] = p1;
+                hx = hx + dmu * p0; hz = hz + dmu * p1;
+            }
+            if ((cp & 8u) && j >= a.ny + 1 - npml && j <= a.ny - 1) {
+                const int c = a.ny - j;
+                const long ps = ((long)i * nsh + (npml - 2 + c)) * sz + k + a.off;
+                T p0 = psi_step(a.ph[2][ps], a.bh[c], a.chd[c], ezc - ezj);
+                T p1 = psi_step(a.ph[3][ps], a.bh[c], a.chd[c], exj - exc);
+                a.ph[2][ps] = p0; a.ph[3][ps] = p1;
+                hx = hx + dmu * p0; hz = hz + dmu * p1;
+            }
+            // z faces: psiz_Hxz, psiz_Hyz
+            if ((cp & 16u) && k <= npml - 1) {
+                const int c = k;
+                const long ps = ((long)i * a.sy + j) * nsh + (k - 1);
+                T p0 = psi_step(a.ph[4][ps], a.bh[c], a.chd[c], eyk - eyc);
+                T p1 = psi_step(a.ph[5][ps], a.bh[c], a.chd[c], exc - exk);
+                a.ph[4][ps] = p0; a.ph[5][ps] = p1;
+                hx = hx + dmu * p0; hy = hy + dmu * p1;
+            }
+            if ((cp & 32u) && k >= a.nz + 1 - npml && k <= a.nz - 1) {
+                const int c = a.nz - k;
+                const long ps = ((long)i * a.sy + j) * nsh + (npml - 2 + c);
+                T p0 = psi_step(a.ph[4][ps], a.bh[c], a.chd[c], eyk - eyc);
+                T p1 = psi_step(a.ph[5][ps], a.bh[c], a.chd[c], exc - exk);
+                a.ph[4][ps] = p0; a.ph[5][ps] = p1;
+                hx = hx + dmu * p0; hy = hy + dmu * p1;
+            }
+        }
+        a.hx[o] = hx; a.hy[o] = hy; a.hz[o] = hz;
+        ezc = ezn; eyc = eyn;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// E update: td_propagate_E_3Dfield (mxll_3D_mod.f90:742-921) with the per-cell
+// medium update of get_medium_polarization (classical_medium_mod.f90:773-833).
+// Currents of embedded quantum systems are applied by k_j_correct afterwards.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T medium_update(T e, T rot, int m, const MedDev<T>* __restrict__ med, T dt_eps0,
+                                           long o, T* __restrict__ eold, T* __restrict__ pd,
+                                           T* __restrict__ pl, T* __restrict__ plold, long ps) {
+    if (m == 0) return e + dt_eps0 * rot;
+    const MedDev<T>& M = med[m];
+    if (M.kind == OMX_MEDIUM_DRUDE) {
+        const T p = pd[o];
+        if constexpr (sizeof(T) == 8) {
+            const T t = M.C1 * e + M.C3 * rot - M.C4 * p;
+            pd[o] = M.A1 * p + M.A2 * (t + e);
+            return t;
+        } else {
+            const T t = e + (M.C1 * e + M.C3 * rot - M.C4 * p);
+            pd[o] = p + (M.A1 * p + M.A2 * (t + e));
+            return t;
+        }
+    }
+    if (M.kind == OMX_MEDIUM_DL) {
+        const T eo = eold[o];
+        const T p = pd[o];
+        const int np = M.np;
+        T t;
+        if constexpr (sizeof(T) == 8) {
+            t = M.C1 * e + M.C2 * eo + M.C3 * rot - M.C4 * p;
+            for (int q = 0; q < np; ++q) t = t - M.B1[q] * pl[o + q * ps] - M.B2[q] * plold[o + q * ps];
+            pd[o] = M.A1 * p + M.A2 * (t + e);
+        } else {
+            T dlt = M.C1 * e + M.C2 * eo + M.C3 * rot - M.C4 * p;
+            for (int q = 0; q < np; ++q) dlt = dlt - M.B1[q] * pl[o + q * ps] - M.B2[q] * plold[o + q * ps];
+            t = e + dlt;
+            pd[o] = p + (M.A1 * p + M.A2 * (t + e));
+        }
+        const T de = t - eo;
+        for (int q = 0; q < np; ++q) {
+            const T L = pl[o + q * ps], Lo = plold[o + q * ps];
+            T nl;
+            if constexpr (sizeof(T) == 8) nl = M.al[q] * L + M.bt[q] * Lo + M.ga[q] * de;
+            else nl = L + (((L - Lo) + M.al[q] * L) + M.bt[q] * Lo + M.ga[q] * de);
+            plold[o + q * ps] = L;
+            pl[o + q * ps] = nl;
+        }
+        eold[o] = e;
+        return t;
+    }
+    return e + M.cdt * rot;  // dielectric: dt_eps0 * (1/eps_r) * rotH
+}
+
+template <typename T, bool CPML, bool MEDIA>
+__global__ __launch_bounds__(256) void k_e_update(const YeeArgs<T> a) {
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int tz = t % a.ntz;
+    const int r = t / a.ntz;
+    const int ty = r % a.nty;
+    const int tx = r / a.nty;
+    const int k = tz * kLanes + threadIdx.x + 1;
+    const int j = ty * blockDim.y + threadIdx.y + 1;
+    if (k > a.nz || j > a.ny) return;
+    const int i0 = tx * a.xchunk + 1;
+    const int i1 = min(i0 + a.xchunk - 1, a.nx);
+    const long sx = a.sx;
+    const int sz = a.sz;
+    const long jk = (long)j * sz + k + a.off;
+    long o = (long)i0 * sx + jk;
+    const T dep = a.dt_eps0;
+    const T dey = a.den_ey[j], dez = a.den_ez[k];
+    T hzi = a.hz[o - sx], hyi = a.hy[o - sx];
+    const int npml = a.npml;
+    for (int i = i0; i <= i1; ++i, o += sx) {
+        const T hzc = a.hz[o], hyc = a.hy[o], hxc = a.hx[o];
+        const T hzj = a.hz[o - sz], hxj = a.hx[o - sz];
+        const T hyk = a.hy[o - 1], hxk = a.hx[o - 1];
+        const T dex = a.den_ex[i];
+        const T rx = (hzc - hzj) * dey + (hyk - hyc) * dez;
+        const T ry = (hzi - hzc) * dex + (hxc - hxk) * dez;
+        const T rz = (hyc - hyi) * dex + (hxj - hxc) * dey;
+        T ex = a.ex[o], ey = a.ey[o], ez = a.ez[o];
+        if constexpr (MEDIA) {
+            const int tg = a.tag[o];
+            if (tg == 0) {
+                ex = ex + dep * rx; ey = ey + dep * ry; ez = ez + dep * rz;
+            } else {
+                const int8_t* c3 = a.cls + 3 * tg;
+                ex = medium_update(ex, rx, (int)c3[0], a.med, dep, o, a.eold[0], a.pd[0], a.pl[0], a.plold[0], a.pstride);
+                ey = medium_update(ey, ry, (int)c3[1], a.med, dep, o, a.eold[1], a.pd[1], a.pl[1], a.plold[1], a.pstride);
+                ez = medium_update(ez, rz, (int)c3[2], a.med, dep, o, a.eold[2], a.pd[2], a.pl[2], a.plold[2], a.pstride);
+            }
+        } else {
+            ex = ex + dep * rx; ey = ey + dep * ry; ez = ez + dep * rz;
+        }
+        if constexpr (CPML) {
+            const unsigned cp = a.cpml;
+            const int nse = 2 * npml;
+            // x faces: psix_Eyx, psix_Ezx (low side i = 2..npml)
+            if ((cp & 1u) && i >= 2 && i <= npml) {
+                const int c = i;
+                const long ps = (long)(i - 1) * sx + jk;
+                T p0 = psi_step(a.pe[0][ps], a.be[c], a.ced[c], hzi - hzc);
+                T p1 = psi_step(a.pe[1][ps], a.be[c], a.ced[c], hyc - hyi);
+                a.pe[0][ps] = p0; a.pe[1][ps] = p1;
+                ey = ey + dep * p0; ez = ez + dep * p1;
+            }
+            if ((cp & 2u) && i >= a.nx + 1 - npml && i <= a.nx - 1) {
+                const int c = a.nx + 1 - i;
+                const long ps = (long)(npml - 1 + c) * sx + jk;
+                T p0 = psi_step(a.pe[0][ps], a.be[c], a.ced[c], hzi - hzc);
+                T p1 = psi_step(a.pe[1][ps], a.be[c], a.ced[c], hyc - hyi);
+                a.pe[0][ps] = p0; a.pe[1][ps] = p1;
+                ey = ey + dep * p0; ez = ez + dep * p1;
+            }
+            // y faces: psiy_Exy, psiy_Ezy
+            if ((cp & 4u) && j >= 2 && j <= npml) {
+                const int c = j;
+                const long ps = ((long)i * nse + (j - 1)) * sz + k + a.off;
+                T p0 = psi_step(a.pe[2][ps], a.be[c], a.ced[c], hzc - hzj);
+                T p1 = psi_step(a.pe[3][ps], a.be[c], a.ced[c], hxj - hxc);
+                a.pe[2][ps] = p0; a.pe[3][ps] = p1;
+                ex = ex + dep * p0; ez = ez + dep * p1;
+            }
+            if ((cp & 8u) && j >= a.ny + 1 - npml && j <= a.ny - 1) {
+                const int c = a.ny + 1 - j;
+                const long ps = ((long)i * nse + (npml - 1 + c)) * sz + k + a.off;
+                T p0 = psi_step(a.pe[2][ps], a.be[c], a.ced[c], hzc - hzj);
+                T p1 = psi_step(a.pe[3][ps], a.be[c], a.ced[c], hxj - hxc);
+                a.pe[2][ps] = p0; a.pe[3][ps] = p1;
+                ex = ex + dep * p0; ez = ez + dep * p1;
+            }
+            // z faces: psiz_Exz, psiz_Eyz
+            if ((cp & 16u) && k >= 2 && k <= npml) {
+                const int c = k;
+                const long ps = ((long)i * a.sy + j) * nse + (k - 1);
+                T p0 = psi_step(a.pe[4][ps], a.be[c], a.ced[c], hyk - hyc);
+                T p1 = psi_step(a.pe[5][ps], a.be[c], a.ced[c], hxc - hxk);
+                a.pe[4][ps] = p0; a.pe[5][ps] = p1;
+                ex = ex + dep * p0; ey = ey + dep * p1;
+            }
+            if ((cp & 32u) && k >= a.nz + 1 - npml && k <= a.nz - 1) {
+                const int c = a.nz + 1 - k;
+                const long ps = ((long)i * a.sy + j) * nse + (npml - 1 + c);
+                T p0 = psi_step(a.pe[4][ps], a.be[c], a.ced[c], hyk - hyc);
+                T p1 = psi_step(a.pe[5][ps], a.be[c], a.ced[c], hxc - hxk);
+                a.pe[4][ps] = p0; a.pe[5][ps] = p1;
+                ex = ex + dep * p0; ey = ey + dep * p1;
+            }
+        }
+        a.ex[o] = ex; a.ey[o] = ey; a.ez[o] = ez;
+        hzi = hzc; hyi = hyc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Periodic wrap: whole-plane copies before each half step, non-MPI branch of
+// mxll_3D_mod.f90:528-541 (E) and :725-738 (H).  One launch per axis, in the
+// reference's x, y, z order.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_wrap(T* __restrict__ f0, T* __restrict__ f1, Layout L, int axis, int dst, int src) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    const int v = blockIdx.y;
+    int na, nb;
+    if (axis == 0) { na = L.nz + 2; nb = L.ny + 2; }
+    else if (axis == 1) { na = L.nz + 2; nb = L.nx + 2; }
+    else { na = L.ny + 2; nb = L.nx + 2; }
+    if (u >= na || v >= nb) return;
+    long od, os;
+    if (axis == 0) { od = L.at(dst, v, u); os = L.at(src, v, u); }
+    else if (axis == 1) { od = L.at(v, dst, u); os = L.at(v, src, u); }
+    else { od = L.at(v, u, dst); os = L.at(v, u, src); }
+    f0[od] = f0[os];
+    f1[od] = f1[os];
+}
+
+// ---------------------------------------------------------------------------
+// Soft current source: update_source (source_mod.f90:106-132) and the 3D
+// branch of source_interactions (interactions_mod.f90:102-156).  The J_mat
+// entry one step past the kernel edge reads as zero.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_source(T* __restrict__ f, const long* __restrict__ off, const double* __restrict__ ka,
+                         const double* __restrict__ kb, int n, double s, double c_src) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const double jav = 0.5 * (s * ka[p] + s * kb[p]);
+    f[off[p]] = (T)((double)f[off[p]] + c_src * jav);
+}
+
+// ---------------------------------------------------------------------------
+// Detector sampling with the stencils of write_3D_field
+// (write_fields_subs_mod.f90:813-1509); point detectors read Ez unaveraged.
+// code = 2*field + (point ? 1 : 0).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_sample(const YeeArgs<T> a, const int* __restrict__ code, const long* __restrict__ off,
+                         long n, double* __restrict__ out) {
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int cd = code[s];
+    const int field = cd >> 1;
+    const bool point = cd & 1;
+    const long o = off[s];
+    const long sx = a.sx, sz = a.sz;
+    double v = 0.0;
+    switch (field) {
+        case OMX_EX: v = 0.5 * ((double)a.ex[o - sx] + (double)a.ex[o]); break;
+        case OMX_EY: v = 0.5 * ((double)a.ey[o - sz] + (double)a.ey[o]); break;
+        case OMX_EZ: v = point ? (double)a.ez[o] : 0.5 * ((double)a.ez[o - 1] + (double)a.ez[o]); break;
+        case OMX_HX: v = 0.25 * ((((double)a.hx[o - sz] + (double)a.hx[o]) + (double)a.hx[o - 1]) + (double)a.hx[o - sz - 1]); break;
+        case OMX_HY: v = 0.25 * ((((double)a.hy[o - sx] + (double)a.hy[o]) + (double)a.hy[o - 1]) + (double)a.hy[o - sx - 1]); break;
+        case OMX_HZ: v = 0.25 * ((((double)a.hz[o] + (double)a.hz[o - sx]) + (double)a.hz[o - sz]) + (double)a.hz[o - sx - sz]); break;
+        default: break;
+    }
+    out[s] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Embedded quantum systems ("material" groups).
+// ---------------------------------------------------------------------------
+// E at each site: send_E_3D_to_J_ranks, non-MPI (interactions_mod.f90:656-691).
+template <typename T>
+__global__ void k_gather_e(const YeeArgs<T> a, const long* __restrict__ off, int n, double* __restrict__ e) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const long o = off[q];
+    e[3 * q + 0] = 0.5 * ((double)a.ex[o - a.sx] + (double)a.ex[o]);
+    e[3 * q + 1] = 0.5 * ((double)a.ey[o - a.sz] + (double)a.ey[o]);
+    e[3 * q + 2] = 0.5 * ((double)a.ez[o - 1] + (double)a.ez[o]);
+}
+
+// J_old <- J; J(site) = density * dPt/dt; dJ = (J - J_old)/dt_q, restricted
+// to the cells that hold a site (send_J_to_E_3D_ranks,
+// interactions_mod.f90:1067-1069,1184-1190,1216-1218).  The last site mapped
+// to a cell wins, as in the reference's sequential loop.
+__global__ void k_scatter_j(const int* __restrict__ winner, int nslot, const double* __restrict__ dpdt,
+                            double density, double dt_q, double* __restrict__ J) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslot) return;
+    const int w = winner[s];
+    for (int c = 0; c < 3; ++c) {
+        const double old = J[9 * s + c];
+        const double nw = density * dpdt[3 * w + c];
+        J[9 * s + 3 + c] = old;
+        J[9 * s + c] = nw;
+        J[9 * s + 6 + c] = (nw - old) / dt_q;
+    }
+}
+
+// E -= dt_eps0 * Jav at no-medium cells next to a site (the J term of
+// mxll_3D_mod.f90:757-760,780-783,805-808).  a: slot of cell i, b: slot of
+// cell i+1 along the component axis (-1: none, J = 0 there).
+template <typename T>
+__global__ void k_j_correct(T* __restrict__ f, const long* __restrict__ off, const int* __restrict__ sa,
+                            const int* __restrict__ sb, int n, int comp, const double* __restrict__ J,
+                            double tau, double dt_eps0, const uint8_t* __restrict__ tag,
+                            const int8_t* __restrict__ cls, const int* __restrict__ kind) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const long o = off[q];
+    if (tag) {
+        const int m = cls[3 * (int)tag[o] + comp];
+        if (m != 0 && (kind[m] == OMX_MEDIUM_DRUDE || kind[m] == OMX_MEDIUM_DL)) return;
+    }
+    const int A = sa[q], B = sb[q];
+    const double ja = A >= 0 ? J[9 * A + 3 + comp] : 0.0;
+    const double da = A >= 0 ? J[9 * A + 6 + comp] : 0.0;
+    const double jb = B >= 0 ? J[9 * B + 3 + comp] : 0.0;
+    const double db = B >= 0 ? J[9 * B + 6 + comp] : 0.0;
+    const double jav = 0.5 * (((ja + tau * da) + jb) + tau * db);
+    f[o] = (T)((double)f[o] - dt_eps0 * jav);
+}
+
+// Two-level emitter, FP64 RK4 over one quantum step with the gathered field
+// held constant.  State per site (stride 16):
+//   0 omega0  1 mu  2..4 unit polarisation  5 gamma2 (dephasing rate)
+//   6 rho_gg  7 rho_ee  8 Re rho_eg  9 Im rho_eg  10..12 dipole  13..15 dip_old
+// H = diag(-w0/2, w0/2) - mu E_p sigma_x (hbar = 1); coherences decay at gamma2.
+// dipole = 2 mu Re(rho_eg) * pol; dPt/dt = (dipole - dip_old)/dt_q.
+__device__ inline void tls_rhs(double w0, double V, double g2, const double y[4], double d[4]) {
+    const double pop = y[0] - y[1];
+    d[0] = 2.0 * V * y[3];
+    d[1] = -2.0 * V * y[3];
+    d[2] = w0 * y[3] - g2 * y[2];
+    d[3] = -w0 * y[2] - V * pop - g2 * y[3];
+}
+
+__global__ void k_tls_step(double* __restrict__ st, const double* __restrict__ e, int n, double dt_q,
+                           double* __restrict__ dpdt) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    double* s = st + 16 * q;
+    const double w0 = s[0], mu = s[1], g2 = s[5];
+    const double ep = e[3 * q] * s[2] + e[3 * q + 1] * s[3] + e[3 * q + 2] * s[4];
+    const double V = -mu * ep;
+    double y[4] = {s[6], s[7], s[8], s[9]};
+    double k1[4], k2[4], k3[4], k4[4], yt[4];
+    const double h = dt_q;
+    tls_rhs(w0, V, g2, y, k1);
+    for (int c = 0; c < 4; ++c) yt[c] = y[c] + 0.5 * h * k1[c];
+    tls_rhs(w0, V, g2, yt, k2);
+    for (int c = 0; c < 4; ++c) yt[c] = y[c] + 0.5 * h * k2[c];
+    tls_rhs(w0, V, g2, yt, k3);
+    for (int c = 0; c < 4; ++c) yt[c] = y[c] + h * k3[c];
+    tls_rhs(w0, V, g2, yt, k4);
+    for (int c = 0; c < 4; ++c) y[c] = y[c] + (h / 6.0) * (k1[c] + 2.0 * k2[c] + 2.0 * k3[c] + k4[c]);
+    s[6] = y[0]; s[7] = y[1]; s[8] = y[2]; s[9] = y[3];
+    const double amp = 2.0 * mu * y[2];
+    for (int c = 0; c < 3; ++c) {
+        const double dn = amp * s[2 + c];
+        const double dold = s[10 + c];
+        s[13 + c] = dold;
+        s[10 + c] = dn;
+        dpdt[3 * q + c] = (dn - dold) / dt_q;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Energy: W = 1/2 dr^3 sum (eps0 eps_r E^2 + mu0 H^2), FP64 and compensated
+// FP32 (Neumaier) partial sums per workgroup, then one final workgroup.
+// ---------------------------------------------------------------------------
+__device__ inline void neumaier(float& s, float& c, float x) {
+    const float t = s + x;
+    if (fabsf(s) >= fabsf(x)) c += (s - t) + x;
+    else c += (x - t) + s;
+    s = t;
+}
+
+template <typename T, bool MEDIA>
+__global__ __launch_bounds__(256) void k_energy(const YeeArgs<T> a, const double* __restrict__ eps_cls,
+                                                double* __restrict__ part) {
+    __shared__ double sd[256];
+    __shared__ float ss[256], sc[256];
+    const long ncell = (long)a.nx * a.ny * a.nz;
+    double acc = 0.0;
+    float fs = 0.0f, fc = 0.0f;
+    for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += (long)gridDim.x * blockDim.x) {
+        const int k = (int)(c % a.nz) + 1;
+        const int j = (int)((c / a.nz) % a.ny) + 1;
+        const int i = (int)(c / ((long)a.nz * a.ny)) + 1;
+        const long o = (long)i * a.sx + (long)j * a.sz + k + a.off;
+        double ex = a.ex[o], ey = a.ey[o], ez = a.ez[o];
+        double hx = a.hx[o], hy = a.hy[o], hz = a.hz[o];
+        double er[3] = {1.0, 1.0, 1.0};
+        if constexpr (MEDIA) {
+            const int tg = a.tag[o];
+            er[0] = eps_cls[3 * tg]; er[1] = eps_cls[3 * tg + 1]; er[2] = eps_cls[3 * tg + 2];
+        }
+        const double we = kEps0 * ((er[0] * ex * ex + er[1] * ey * ey) + er[2] * ez * ez);
+        const double wh = kMu0 * ((hx * hx + hy * hy) + hz * hz);
+        acc += we + wh;
+        const float fex = (float)a.ex[o], fey = (float)a.ey[o], fez = (float)a.ez[o];
+        const float fhx = (float)a.hx[o], fhy = (float)a.hy[o], fhz = (float)a.hz[o];
+        neumaier(fs, fc, (float)kEps0 * ((float)er[0] * fex * fex));
+        neumaier(fs, fc, (float)kEps0 * ((float)er[1] * fey * fey));
+        neumaier(fs, fc, (float)kEps0 * ((float)er[2] * fez * fez));
+        neumaier(fs, fc, (float)kMu0 * (fhx * fhx));
+        neumaier(fs, fc, (float)kMu0 * (fhy * fhy));
+        neumaier(fs, fc, (float)kMu0 * (fhz * fhz));
+    }
+    const int t = threadIdx.x;
+    sd[t] = acc; ss[t] = fs; sc[t] = fc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            sd[t] += sd[t + w];
+            float s = ss[t], cc = sc[t] + sc[t + w];
+            neumaier(s, cc, ss[t + w]);
+            ss[t] = s; sc[t] = cc;
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        part[3 * blockIdx.x] = sd[0];
+        part[3 * blockIdx.x + 1] = (double)ss[0];
+        part[3 * blockIdx.x + 2] = (double)sc[0];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_energy_final(const double* __restrict__ part, int nb, double scale,
+                                                      double* __restrict__ out) {
+    __shared__ double sd[256];
+    __shared__ float ss[256], sc[256];
+    const int t = threadIdx.x;
+    double acc = 0.0;
+    float fs = 0.0f, fc = 0.0f;
+    for (int b = t; b < nb; b += 256) {
+        acc += part[3 * b];
+        neumaier(fs, fc, (float)part[3 * b + 1]);
+        fc += (float)part[3 * b + 2];
+    }
+    sd[t] = acc; ss[t] = fs; sc[t] = fc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            sd[t] += sd[t + w];
+            float s = ss[t], cc = sc[t] + sc[t + w];
+            neumaier(s, cc, ss[t + w]);
+            ss[t] = s; sc[t] = cc;
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        out[0] = 0.5 * scale * sd[0];
+        out[1] = 0.5 * scale * (double)(ss[0] + sc[0]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// TF/SF plane wave, x-polarised, travelling along +z.  The incident Ex (at
+// z = k) and Hy (at z = k + 1/2) come from a 1D Yee line with the 3D grid's
+// dt and dr, so the incident wave has the 3D grid's numerical dispersion
+// along z.  Total-field region: Ex i in [i0,i1-1], j in [j0,j1], k in [k0,k1];
+// Hy i in [i0,i1-1], j in [j0,j1], k in [k0,k1-1]; Hz i in [i0,i1-1],
+// j in [j0,j1-1], k in [k0,k1]; Ez i in [i0,i1], j in [j0,j1], k in [k0,k1-1].
+// ---------------------------------------------------------------------------
+struct TfsfArgs {
+    int i0, i1, j0, j1, k0, k1;
+    int kb, n1, ks;     // 1D line covers k = kb .. kb+n1-1; source node ks
+    double dt_mu0, dt_eps0, inv_dr;
+};
+
+template <typename T>
+__global__ void k_tfsf_h(const YeeArgs<T> a, TfsfArgs t, double* __restrict__ ex1, double* __restrict__ hy1) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nxi = t.i1 - t.i0;            // Hy/Hz x range size
+    const int nyj = t.j1 - t.j0 + 1;        // Hy y range
+    const int nzk = t.k1 - t.k0 + 1;        // Hz z range
+    const int sa = nxi * nyj, sb = nxi * nzk;
+    if (q < 2 * sa) {                        // Hy on the z faces
+        const int side = q / sa, r = q % sa;
+        const int i = t.i0 + r % nxi, j = t.j0 + r / nxi;
+        if (side == 0) {
+            const int k = t.k0 - 1;
+            const long o = a.sx * i + (long)a.sz * j + k + a.off;
+            a.hy[o] = (T)((double)a.hy[o] + t.dt_mu0 * (double)a.den_hz[k] * ex1[t.k0 - t.kb]);
+        } else {
+            const int k = t.k1;
+            const long o = a.sx * i + (long)a.sz * j + k + a.off;
+            a.hy[o] = (T)((double)a.hy[o] - t.dt_mu0 * (double)a.den_hz[k] * ex1[t.k1 - t.kb]);
+        }
+        return;
+    }
+    const int q2 = q - 2 * sa;
+    if (q2 < 2 * sb) {                       // Hz on the y faces
+        const int side = q2 / sb, r = q2 % sb;
+        const int i = t.i0 + r % nxi, k = t.k0 + r / nxi;
+        const double inc = ex1[k - t.kb];
+        if (side == 0) {
+            const int j = t.j0 - 1;
+            const long o = a.sx * i + (long)a.sz * j + k + a.off;
+            a.hz[o] = (T)((double)a.hz[o] - t.dt_mu0 * (double)a.den_hy[j] * inc);
+        } else {
+            const int j = t.j1;
+            const long o = a.sx * i + (long)a.sz * j + k + a.off;
+            a.hz[o] = (T)((double)a.hz[o] + t.dt_mu0 * (double)a.den_hy[j] * inc);
+        }
+        return;
+    }
+    const int m = q2 - 2 * sb;               // 1D H update
+    if (m < t.n1 - 1) hy1[m] = hy1[m] + t.dt_mu0 * t.inv_dr * (ex1[m] - ex1[m + 1]);
+}
+
+template <typename T>
+__global__ void k_tfsf_e(const YeeArgs<T> a, TfsfArgs t, double* __restrict__ ex1, const double* __restrict__ hy1,
+                         double src) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nxi = t.i1 - t.i0, nyj = t.j1 - t.j0 + 1;
+    const int nyz = t.j1 - t.j0 + 1, nzk = t.k1 - t.k0;
+    const int sa = nxi * nyj, sb = nyz * nzk;
+    if (q < 2 * sa) {                        // Ex on the z faces
+        const int side = q / sa, r = q % sa;
+        const int i = t.i0 + r % nxi, j = t.j0 + r / nxi;
+        if (side == 0) {
+            const int k = t.k0;
+            const long o = a.sx * i + (long)a.sz * j + k + a.off;
+            a.ex[o] = (T)((double)a.ex[o] + t.dt_eps0 * (double)a.den_ez[k] * hy1[t.k0 - 1 - t.kb]);
+        } else {
+            const int k = t.k1;
+            const long o = a.sx * i + (long)a.sz * j + k + a.off;
+            a.ex[o] = (T)((double)a.ex[o] - t.dt_eps0 * (double)a.den_ez[k] * hy1[t.k1 - t.kb]);
+        }
+        return;
+    }
+    const int q2 = q - 2 * sa;
+    if (q2 < 2 * sb) {                       // Ez on the x faces
+        const int side = q2 / sb, r = q2 % sb;
+        const int j = t.j0 + r % nyz, k = t.k0 + r / nyz;
+        const double inc = hy1[k - t.kb];
+        if (side == 0) {
+            const int i = t.i0;
+            const long o = a.sx * i + (long)a.sz * j + k + a.off;
+            a.ez[o] = (T)((double)a.ez[o] - t.dt_eps0 * (double)a.den_ex[i] * inc);
+        } else {
+            const int i = t.i1;
+            const long o = a.sx * i + (long)a.sz * j + k + a.off;
+            a.ez[o] = (T)((double)a.ez[o] + t.dt_eps0 * (double)a.den_ex[i] * inc);
+        }
+        return;
+    }
+    const int m = q2 - 2 * sb;               // 1D E update (ends held at zero)
+    if (m >= 1 && m < t.n1 - 1) {
+        double v = ex1[m] + t.dt_eps0 * t.inv_dr * (hy1[m - 1] - hy1[m]);
+        if (m == t.ks - t.kb) v += src;
+        ex1[m] = v;
+    }
+}
+
+}  // namespace omx

@@ -1,0 +1,271 @@
+"""GPU parity tests: the HIP engine (through its C-ABI) against the oracle.
+
+Bar: FP64 detector time series within relL2 <= 1e-12 of the oracle (the
+engine contracts multiply-adds, the oracle does not, so bitwise equality is
+not expected); integer/index work (media maps, classes) bit-exact; FP32 runs
+within relL2 <= 2e-5 of the FP64 oracle on these short runs.
+"""
+import numpy as np
+import pytest
+
+import omx_fdtd
+from omx_fdtd import cases, grid as G
+from omx_fdtd.units import (BC_CPML, BC_PERIODIC, BC_CLOSE, DET_POINT, DET_LINE_X, DET_PLANE_XY,
+                            DET_PLANE_ZX, DET_VOLUME, EX, EY, EZ, HX, HY, HZ, EV_TO_AU, FS_TO_AU)
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+TOL32 = 2e-5
+
+
+def rel_l2(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def both(case, precision=8):
+    dr, dt = case.dr_dt
+    print(f"[case] {case.name} grid={case.grid} steps={case.steps} precision={precision}", flush=True)
+    g = omx_fdtd.Grid3D(case.grid, case.npml, case.boundaries, dt, dr, precision=precision)
+    cases.apply(case, g, library=G.library_medium, make_medium=G.medium)
+    o = oracle.OracleGrid(case.grid, case.npml, case.boundaries, dt, dr)
+    cases.apply(case, o, library=oracle.library_medium, make_medium=oracle.make_medium,
+                rasterize=oracle.rasterize)
+    return g, o
+
+
+def run_both(case, precision=8):
+    g, o = both(case, precision)
+    g.run(1, case.steps, case.record_every)
+    got = g.read_records()
+    want = o.run(1, case.steps, case.record_every)
+    return g, o, got, want
+
+
+def small_case(name, n=(22, 20, 18), npml=4, bc=(BC_CPML,) * 3, steps=60, media=(), shapes=(),
+               sources=None, dets=None, dx=5.0):
+    nx, ny, nz = n
+    if sources is None:
+        sources = [("z", 1.0, 1.549802, 0.6, 0.3, 0.0, 0.0, 0.0, 0.0, 0.0, 1.0e6, 0.0)]
+    if dets is None:
+        pts = [(nx // 2 + 2, ny // 2 - 1, nz // 2 + 1), (nx // 2 - 3, ny // 2 + 2, nz // 2 - 2),
+               (npml + 1, ny // 2, nz // 2)]
+        dets = cases.point_detectors(pts, ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"))
+        dets.append((DET_LINE_X, EZ, np.array([(i, ny // 2, nz // 2 + 1) for i in range(1, nx + 1)])))
+        dets.append((DET_PLANE_XY, HX, cases.plane_xy(n, nz // 2 - 1)))
+    return cases.Case(name=name, grid=n, dx_nm=dx, npml=npml, boundaries=bc, steps=steps, record_every=1,
+                      media=list(media), shapes=list(shapes), sources=sources, detectors=dets)
+
+
+# ---------------------------------------------------------------------------
+def test_cpml_tables_match_oracle():
+    for npml, n in ((4, (20, 18, 16)), (10, (64, 64, 64)), (12, (40, 30, 26))):
+        case = small_case("cpml", n=n, npml=npml, bc=(BC_CPML, BC_CPML, BC_CPML))
+        g, o = both(case)
+        a, b = g.cpml_tables(), o.cpml_tables()
+        for k in b:
+            np.testing.assert_allclose(a[k], b[k], rtol=2e-15, atol=0, err_msg=k)
+
+
+def test_media_map_and_coeffs_match_oracle():
+    n = (26, 24, 22)
+    shapes = [(1, 1, [9.0, 10.0, 11.0, 5.2]), (3, 2, [15, 3, 3, 24, 20, 8]),
+              (2, 3, [14.0, 14.0, 14.0, 6.0, 3.0, 4.0, 0.8, 0.6, 0.0, -0.6, 0.8, 0.0, 0.0, 0.0, 1.0]),
+              (4, 0, [20.0, 12.0, 2.5, 1, 22])]
+    media = [("drude-lorentz", "Ag", 1.0), ("dielectric", 2.25), ("drude", 9.0, 0.07, 1.0)]
+    case = small_case("media", n=n, media=media, shapes=shapes)
+    g, o = both(case)
+    np.testing.assert_array_equal(g.media_map(), o.media_map())
+    for m in (1, 2, 3):
+        c, w = g.medium_coeffs(m), o.medium_coeffs(m)
+        for k in ("A1", "A2", "C1", "C2", "C3", "C4", "inv_eps"):
+            np.testing.assert_allclose(getattr(c, k), w[k], rtol=1e-15, atol=0, err_msg=f"{m}:{k}")
+        for k in ("alpha_k", "beta_k", "gamma_k", "B1_k", "B2_k"):
+            np.testing.assert_allclose(np.array(getattr(c, k)), w[k], rtol=1e-15, atol=0, err_msg=f"{m}:{k}")
+
+
+def test_library_media_match_oracle():
+    for name in ("Ag", "Al", "Au"):
+        a = G.library_medium(name, 1.3)
+        b = oracle.library_medium(name, 1.3)
+        assert a.n_poles == b.n_poles
+        for k in ("eps_r", "omega_d", "gamma_d"):
+            assert getattr(a, k) == getattr(b, k), (name, k)
+        for k in ("omega_k", "gamma_k", "a_k"):
+            assert list(getattr(a, k)) == list(getattr(b, k)), (name, k)
+
+
+@pytest.mark.parametrize("variant", ["cpml", "close", "periodic", "mixed"])
+def test_vacuum_fp64(variant):
+    bc = {"cpml": (BC_CPML,) * 3, "close": (BC_CLOSE,) * 3, "periodic": (BC_PERIODIC,) * 3,
+          "mixed": (BC_PERIODIC, BC_CPML, BC_CLOSE)}[variant]
+    case = small_case(f"vacuum-{variant}", bc=bc, steps=80)
+    g, o, got, want = run_both(case)
+    assert np.abs(want).max() > 0
+    r = rel_l2(got, want)
+    print(f"relL2 = {r:.3e}")
+    assert r <= TOL64
+
+
+@pytest.mark.parametrize("direction", ["x", "y", "z"])
+def test_source_kernel_fp64(direction):
+    src = [(direction, 2.0, 1.2, 0.5, 0.25, 1.0, -2.0, 0.5, 10.0, 0.05, 1.0e6, 0.3)]
+    case = small_case(f"kernel-{direction}", n=(24, 22, 20), sources=src, steps=50)
+    g, o, got, want = run_both(case)
+    r = rel_l2(got, want)
+    print(f"relL2 = {r:.3e}")
+    assert r <= TOL64
+
+
+@pytest.mark.parametrize("medium", ["Ag", "Au", "Al", "drude", "dielectric", "custom"])
+def test_media_fp64(medium):
+    spec = {"Ag": ("drude-lorentz", "Ag", 1.0), "Au": ("drude-lorentz", "Au", 2.0),
+            "Al": ("drude-lorentz", "Al", 1.0), "drude": ("drude", 9.0, 0.07, 1.5),
+            "dielectric": ("dielectric", 2.25),
+            "custom": ("dl-custom", 5.0, 9.0, 0.07, [(1.2, 3.1, 0.3), (0.7, 2.7, 0.15)])}[medium]
+    shapes = [(1, 1, [11.0, 10.0, 9.0, 4.5]), (3, 1, [3, 3, 2, 20, 18, 4])]
+    case = small_case(f"medium-{medium}", media=[spec], shapes=shapes, steps=100)
+    g, o, got, want = run_both(case)
+    r = rel_l2(got, want)
+    print(f"relL2 = {r:.3e}")
+    assert r <= TOL64
+
+
+def test_dl_sphere_fp32_against_fp64_oracle():
+    spec = ("dl-custom", 5.0, 9.0, 0.07, [(1.2, 3.1, 0.3), (0.7, 2.7, 0.15)])
+    case = small_case("fp32-dl", n=(28, 26, 24), media=[spec], shapes=[(1, 1, [14.0, 13.0, 12.0, 5.0])],
+                      steps=150)
+    g, o, got, want = run_both(case, precision=4)
+    r = rel_l2(got, want)
+    print(f"FP32 vs FP64 oracle relL2 = {r:.3e}")
+    assert r <= TOL32
+
+
+def test_tiling_variants_identical():
+    spec = ("drude-lorentz", "Ag", 1.0)
+    case = small_case("tiling", n=(30, 27, 70), media=[spec], shapes=[(1, 1, [15.0, 13.0, 30.0, 6.0])],
+                      steps=40)
+    ref = None
+    for rows, xc in ((4, 16), (1, 1), (2, 7), (4, 64)):
+        dr, dt = case.dr_dt
+        g = omx_fdtd.Grid3D(case.grid, case.npml, case.boundaries, dt, dr, precision=8)
+        g.set_tiling(rows, xc)
+        cases.apply(case, g, library=G.library_medium, make_medium=G.medium)
+        g.run(1, case.steps, 1)
+        rec = g.read_records()
+        if ref is None:
+            ref = rec
+        else:
+            np.testing.assert_array_equal(rec, ref)
+        g.kill()
+
+
+def test_stepwise_api_with_quantum_sites():
+    """Drive the reference step order call by call (mxim_mxll.f90:92-132) with
+    a prescribed dPt/dt per site, checking E at the sites and the detectors."""
+    n = (20, 18, 16)
+    sites = np.array([(8, 9, 8), (9, 9, 8), (12, 6, 10), (8, 9, 8)], dtype=np.int32)
+    case = small_case("qsites", n=n, media=[("dielectric", 2.0)], shapes=[(3, 1, [1, 1, 1, 20, 18, 4])],
+                      steps=0)
+    dr, dt = case.dr_dt
+    g, o = both(case)
+    dens = 1e-3 / omx_fdtd.NM_TO_AU ** 3
+    g.set_q_sites(sites, dens, dt)
+    o.set_q_sites(sites, dens, dt)
+    rng = np.random.default_rng(7)
+    amp = rng.uniform(-1, 1, size=(len(sites), 3)) * 1e-2
+    tt_0 = 1
+    for tt in range(1, 41):
+        time = dt * (tt - 0.5)
+        for x in (g, o):
+            x.time = time
+            x.t_skip = time
+            x.td_propagate_H_field()
+            x.inject_sources(time)
+            x.td_propagate_E_field(tt)
+        eg, eo = g.gather_E(), o.gather_E()
+        np.testing.assert_allclose(eg, eo, rtol=1e-11, atol=1e-300)
+        dpdt = amp * np.sin(0.3 * tt + np.arange(3))[None, :]
+        g.scatter_J(dpdt)
+        o.scatter_J(dpdt)
+    r = rel_l2(g.sample(), o.sample())
+    print(f"relL2 = {r:.3e}")
+    assert r <= TOL64
+
+
+def test_two_level_emitters_fp64():
+    n = (20, 18, 16)
+    sites = np.array([(8, 9, 8), (11, 8, 9), (6, 12, 7)], dtype=np.int32)
+    case = small_case("tls", n=n, steps=0)
+    dr, dt = case.dr_dt
+    g, o = both(case)
+    for x in (g, o):
+        x.set_q_sites(sites, 1e-3 / omx_fdtd.NM_TO_AU ** 3, dt)
+        x.set_two_level(np.full(3, 2.1 * EV_TO_AU), np.full(3, 10.0 * omx_fdtd.DEBYE_TO_AU),
+                        np.array([[0, 0, 1.0], [1, 1, 0], [0.2, 0.3, 0.9]]), np.full(3, 0.02 * EV_TO_AU))
+    dets = cases.point_detectors([(9, 9, 9)])
+    for d in dets:
+        g.add_detector(*d)
+        o.add_detector(*d)
+    g.set_record_capacity(200)
+    g.run(1, 150, 1)
+    want = o.run(1, 150, 1)
+    got = g.read_records()
+    r = rel_l2(got, want)
+    print(f"relL2 = {r:.3e}")
+    assert r <= TOL64
+    (dg, rg), (do, ro) = g.two_level_state(), o.two_level_state()
+    np.testing.assert_allclose(rg, ro, rtol=1e-11, atol=1e-15)
+    assert np.abs(do).max() > 0
+    np.testing.assert_allclose(dg, do, rtol=1e-9, atol=1e-12 * np.abs(do).max())
+
+
+def test_field_copy_roundtrip_and_energy():
+    case = small_case("fields", n=(14, 12, 10), steps=0)
+    dr, dt = case.dr_dt
+    g = omx_fdtd.Grid3D(case.grid, case.npml, case.boundaries, dt, dr, precision=8)
+    rng = np.random.default_rng(3)
+    tot = []
+    for name in G.FIELD_NAMES:
+        f = g.get_field(name)
+        v = rng.uniform(-1, 1, size=f.shape)
+        g.set_field(name, v)
+        np.testing.assert_array_equal(g.get_field(name), v)
+        tot.append(v)
+    w64, w32 = g.energy()
+    s = 0.0
+    import math
+    terms = []
+    for q, v in enumerate(tot):
+        # interior cells 1..n of each reference array
+        lo = [0 if (q < 3 and a == q) else 1 for a in range(3)] if q < 3 else \
+             [1 if a == q - 3 else 0 for a in range(3)]
+        core = v[tuple(slice(1 - lo[a], 1 - lo[a] + (g.nx, g.ny, g.nz)[a]) for a in range(3))]
+        c = omx_fdtd.EPS0 if q < 3 else omx_fdtd.MU0
+        terms.extend((c * core.ravel() ** 2).tolist())
+    want = 0.5 * dr ** 3 * math.fsum(terms)
+    assert abs(w64 - want) <= 1e-12 * want
+    assert abs(w32 - want) <= 1e-6 * want
+
+
+def test_tfsf_vacuum_leakage():
+    n = (40, 40, 60)
+    lo, hi = (10, 10, 12), (30, 30, 48)
+    case = cases.Case(name="tfsf", grid=n, dx_nm=2.0, npml=6, boundaries=(BC_CPML,) * 3, steps=500,
+                      record_every=1,
+                      tfsf=dict(lo=lo, hi=hi, e0=1.0, w0=2.066 * EV_TO_AU, t0=0.6 * FS_TO_AU, tau=0.2 * FS_TO_AU))
+    inside = (20, 20, 40)
+    outside = [(20, 20, 52), (20, 4 + 2, 30), (34, 20, 30), (20, 20, 9)]
+    case.detectors = cases.point_detectors([inside] + outside, ("Ex",))
+    g, o, got, want = run_both(case)
+    r = rel_l2(got, want)
+    print(f"relL2 vs oracle = {r:.3e}")
+    assert r <= TOL64
+    peak = np.abs(got[:, 0]).max()
+    leak = np.abs(got[:, 1:]).max()
+    print(f"peak inside {peak:.3e}, max outside {leak:.3e}")
+    assert peak > 1e-3
+    assert leak <= 1e-3 * peak
