@@ -38,6 +38,26 @@ def build_engine():
     return out
 
 
+FC = os.environ.get("OMX_FC", "/opt/rocm/bin/amdflang")
+F_SRC = ["omx_fdtd_cabi.f90", "omx_mxll_base.f90", "omx_mxll_3d_hip.f90", "omx_fdtd_main.f90"]
+
+
+def build_fortran():
+    """The Fortran host layer (TMxll_3D_hip) and the omx_fdtd_3d driver,
+    linked against the engine with an rpath to this directory."""
+    bdir = os.path.join(PKG, "fortran", "_build")
+    os.makedirs(bdir, exist_ok=True)
+    objs = []
+    for f in F_SRC:
+        run([FC, "-O2", "-c", os.path.join(PKG, "fortran", f)], cwd=bdir)
+        objs.append(f.replace(".f90", ".o"))
+    exe = os.path.join(PKG, "omx_fdtd_3d")
+    run([FC, "-O2", "-o", exe] + objs + ["-L" + PKG, "-lomx_fdtd_hip", "-Wl,-rpath," + PKG,
+                                          "-Wl,-rpath,$ORIGIN"], cwd=bdir)
+    return exe
+
+
 if __name__ == "__main__":
     build_engine()
+    build_fortran()
     sys.exit(0)
