@@ -127,7 +127,7 @@ int omx_medium_library(const char* name, double eps_r, omx_medium* out);
 /* ---- lifecycle (TMxll%init / TMxll%kill) --------------------------------- */
 int omx3d_create(const omx3d_desc* desc, omx3d_ctx** out);
 int omx3d_destroy(omx3d_ctx* ctx);
-/* Tile shape of the H/E kernels: y rows per workgroup (1,2,4,8) and x planes
+/* Tile shape of the H/E kernels: y rows per workgroup (1, 2 or 4) and x planes
  * marched per workgroup (>= 1).  0 keeps the current value. */
 int omx3d_set_tiling(omx3d_ctx* ctx, int32_t rows_y, int32_t xchunk);
 /* CPML profiles and den arrays (mxll_3D_mod.f90:282-420), device FP64.
@@ -201,6 +201,20 @@ int omx3d_get_profile(omx3d_ctx* ctx, double* ms_h, double* ms_e, int32_t* n_lau
 int omx3d_synchronize(omx3d_ctx* ctx);
 /* Bytes of device memory the context holds. */
 int omx3d_device_bytes(omx3d_ctx* ctx, int64_t* bytes);
+/* Tile-shape autotuner: times rows_y x xchunk candidates with `steps` H+E
+ * steps each, keeps the fastest, then clears fields, CPML and dispersive
+ * state (call before a run).  No reference counterpart (engine tuning). */
+int omx3d_autotune(omx3d_ctx* ctx, int32_t steps, int32_t* rows_y, int32_t* xchunk, double* ms_per_step);
+
+/* ---- batches of independent grids (no reference counterpart) ------------ */
+/* One launch per phase (H, sources, E, detectors) covers every grid of the
+ * batch; each grid computes exactly what its own omx3d_run computes.  The
+ * grids are set up with the single-grid calls first (same precision, no
+ * TF/SF or quantum sites); use_graph replays a captured step graph. */
+typedef struct omxb_ctx omxb_ctx;
+int omxb_create(int32_t n, omx3d_ctx* const* grids, omxb_ctx** out);
+int omxb_run(omxb_ctx* batch, int32_t tt_first, int32_t n_steps, int32_t record_every, int32_t use_graph);
+int omxb_destroy(omxb_ctx* batch);
 
 #ifdef __cplusplus
 }

@@ -90,6 +90,7 @@ YeeArgs<T> make_args(omx3d_ctx* c) {
     a.xchunk = c->xchunk;
     a.ntz = (c->L.nz + kLanes - 1) / kLanes;
     a.nty = (c->L.ny + c->rows_y - 1) / c->rows_y;
+    a.rows = c->rows_y;
     return a;
 }
 
@@ -100,36 +101,36 @@ unsigned yee_blocks(omx3d_ctx* c, const YeeArgs<T>& a) {
 }
 
 template <typename T>
-int wrap_e(omx3d_ctx* c) {
+int wrap_e(omx3d_ctx* c, hipStream_t st) {
     const Layout& L = c->L;
     T** f = (T**)c->f;
     if (c->d.boundaries[0] == OMX_BC_PERIODIC)
-        hipLaunchKernelGGL(k_wrap<T>, dim3(blocks_for(L.nz + 2, 256), L.ny + 2), dim3(256), 0, c->s, f[2], f[1], L, 0, L.nx + 1, 1);
+        hipLaunchKernelGGL(k_wrap<T>, dim3(blocks_for(L.nz + 2, 256), L.ny + 2), dim3(256), 0, st, f[2], f[1], L, 0, L.nx + 1, 1);
     if (c->d.boundaries[1] == OMX_BC_PERIODIC)
-        hipLaunchKernelGGL(k_wrap<T>, dim3(blocks_for(L.nz + 2, 256), L.nx + 2), dim3(256), 0, c->s, f[0], f[2], L, 1, L.ny + 1, 1);
+        hipLaunchKernelGGL(k_wrap<T>, dim3(blocks_for(L.nz + 2, 256), L.nx + 2), dim3(256), 0, st, f[0], f[2], L, 1, L.ny + 1, 1);
     if (c->d.boundaries[2] == OMX_BC_PERIODIC)
-        hipLaunchKernelGGL(k_wrap<T>, dim3(blocks_for(L.ny + 2, 256), L.nx + 2), dim3(256), 0, c->s, f[0], f[1], L, 2, L.nz + 1, 1);
+        hipLaunchKernelGGL(k_wrap<T>, dim3(blocks_for(L.ny + 2, 256), L.nx + 2), dim3(256), 0, st, f[0], f[1], L, 2, L.nz + 1, 1);
     HIPCHK(hipGetLastError());
     return OMX_OK;
 }
 
 template <typename T>
-int wrap_h(omx3d_ctx* c) {
+int wrap_h(omx3d_ctx* c, hipStream_t st) {
     const Layout& L = c->L;
     T** f = (T**)c->f;
     if (c->d.boundaries[0] == OMX_BC_PERIODIC)
-        hipLaunchKernelGGL(k_wrap<T>, dim3(blocks_for(L.nz + 2, 256), L.ny + 2), dim3(256), 0, c->s, f[4], f[5], L, 0, 0, L.nx);
+        hipLaunchKernelGGL(k_wrap<T>, dim3(blocks_for(L.nz + 2, 256), L.ny + 2), dim3(256), 0, st, f[4], f[5], L, 0, 0, L.nx);
     if (c->d.boundaries[1] == OMX_BC_PERIODIC)
-        hipLaunchKernelGGL(k_wrap<T>, dim3(blocks_for(L.nz + 2, 256), L.nx + 2), dim3(256), 0, c->s, f[3], f[5], L, 1, 0, L.ny);
+        hipLaunchKernelGGL(k_wrap<T>, dim3(blocks_for(L.nz + 2, 256), L.nx + 2), dim3(256), 0, st, f[3], f[5], L, 1, 0, L.ny);
     if (c->d.boundaries[2] == OMX_BC_PERIODIC)
-        hipLaunchKernelGGL(k_wrap<T>, dim3(blocks_for(L.ny + 2, 256), L.nx + 2), dim3(256), 0, c->s, f[3], f[4], L, 2, 0, L.nz);
+        hipLaunchKernelGGL(k_wrap<T>, dim3(blocks_for(L.ny + 2, 256), L.nx + 2), dim3(256), 0, st, f[3], f[4], L, 2, 0, L.nz);
     HIPCHK(hipGetLastError());
     return OMX_OK;
 }
 
 template <typename T>
 int launch_h(omx3d_ctx* c) {
-    int rc = wrap_e<T>(c);
+    int rc = wrap_e<T>(c, c->s);
     if (rc) return rc;
     YeeArgs<T> a = make_args<T>(c);
     const dim3 blk(kLanes, c->rows_y);
@@ -157,7 +158,7 @@ int launch_j_correct(omx3d_ctx* c, double tau) {
 
 template <typename T>
 int launch_e(omx3d_ctx* c, double time, double t_skip) {
-    int rc = wrap_h<T>(c);
+    int rc = wrap_h<T>(c, c->s);
     if (rc) return rc;
     YeeArgs<T> a = make_args<T>(c);
     const dim3 blk(kLanes, c->rows_y);
@@ -171,18 +172,21 @@ int launch_e(omx3d_ctx* c, double time, double t_skip) {
     return launch_j_correct<T>(c, time - t_skip);
 }
 
+SourceParams source_params(const omx3d_ctx* c, const SourceDev& s) {
+    SourceParams p;
+    p.j_amp = s.j_amp; p.w0 = s.w0; p.t0 = s.t0; p.tau = s.tau; p.t_init = s.t_init; p.t_final = s.t_final;
+    p.phase = s.phase;
+    p.c_src = (((c->d.dt / kEps0) / c->d.dr) / kC0) / 2.0;  // interactions_mod.f90:32
+    p.dt = c->d.dt;
+    return p;
+}
+
 template <typename T>
 int launch_sources(omx3d_ctx* c, double time) {
-    const double dt_eps0 = c->d.dt / kEps0;
-    const double c_src = ((dt_eps0 / c->d.dr) / kC0) / 2.0;  // interactions_mod.f90:32
     for (auto& s : c->sources) {
         if (!(time >= s.t_init && time <= s.t_final)) continue;  // J_mat = 0 outside the window
-        const double x = (time - s.t0) / s.tau;
-        const double env = std::exp(-(x * x));
-        const double cs = std::cos(s.w0 * (time - s.t0) + s.phase);
-        const double amp = (s.j_amp * env) * cs;
         hipLaunchKernelGGL(k_source<T>, dim3(blocks_for(s.npts, 256)), dim3(256), 0, c->s,
-                           (T*)c->f[s.dir - 1], s.d_off, s.d_ka, s.d_kb, s.npts, amp, c_src);
+                           (T*)c->f[s.dir - 1], s.d_off, s.d_ka, s.d_kb, s.npts, source_params(c, s), time);
     }
     HIPCHK(hipGetLastError());
     return OMX_OK;
@@ -508,6 +512,210 @@ int media_finish(omx3d_ctx* c) {
 
 }  // namespace
 
+
+// ---------------------------------------------------------------------------
+// Batch of independent grids: one launch per phase covers all of them.
+// ---------------------------------------------------------------------------
+struct omxb_ctx {
+    int n = 0, prec = 8;
+    std::vector<omx3d_ctx*> g;
+    hipStream_t s = nullptr;
+    void* d_args = nullptr;
+    int* d_tile_start = nullptr;
+    int total_tiles = 0;
+    std::vector<int> pass_first, pass_count;
+    void** d_src_field = nullptr;
+    long* d_src_off = nullptr;
+    double* d_src_ka = nullptr;
+    double* d_src_kb = nullptr;
+    int* d_src_of = nullptr;
+    SourceParams* d_sp = nullptr;
+    int n_src_pts = 0;
+    BatchRecord* d_br = nullptr;
+    int* d_grid_of = nullptr;
+    int* d_local = nullptr;
+    int* d_code = nullptr;
+    long* d_off = nullptr;
+    long n_samp = 0;
+    int* d_overflow = nullptr;
+    long* d_step = nullptr;
+    bool periodic = false;
+};
+
+namespace {
+
+template <typename P>
+int bupload(hipStream_t s, P** d, const std::vector<P>& h) {
+    dfreet(*d);
+    if (hipMalloc((void**)d, sizeof(P) * std::max<size_t>(1, h.size())) != hipSuccess)
+        return fail(OMX_ERR_NOMEM, "batch allocation failed");
+    if (!h.empty()) HIPCHK(hipMemcpyAsync(*d, h.data(), sizeof(P) * h.size(), hipMemcpyHostToDevice, s));
+    return OMX_OK;
+}
+
+template <typename T>
+int batch_setup(omxb_ctx* b) {
+    std::vector<YeeArgs<T>> args;
+    std::vector<int> ts;
+    int tot = 0;
+    for (auto* c : b->g) {
+        YeeArgs<T> a = make_args<T>(c);
+        args.push_back(a);
+        ts.push_back(tot);
+        tot += (int)yee_blocks(c, a);
+    }
+    ts.push_back(tot);
+    b->total_tiles = tot;
+    int rc = bupload(b->s, (YeeArgs<T>**)&b->d_args, args);
+    if (!rc) rc = bupload(b->s, &b->d_tile_start, ts);
+    if (rc) return rc;
+    // sources: one pass per repeated (field, cell), in source order
+    std::map<std::pair<void*, long>, int> seen;
+    std::vector<std::vector<void*>> pf;
+    std::vector<std::vector<long>> po;
+    std::vector<std::vector<double>> pa, pb;
+    std::vector<std::vector<int>> ps;
+    std::vector<SourceParams> sp;
+    for (auto* c : b->g) {
+        for (auto& src : c->sources) {
+            const int sid = (int)sp.size();
+            sp.push_back(source_params(c, src));
+            std::vector<long> off(src.npts);
+            std::vector<double> ka(src.npts), kb(src.npts);
+            HIPCHK(hipMemcpy(off.data(), src.d_off, sizeof(long) * src.npts, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(ka.data(), src.d_ka, sizeof(double) * src.npts, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(kb.data(), src.d_kb, sizeof(double) * src.npts, hipMemcpyDeviceToHost));
+            void* fp = c->f[src.dir - 1];
+            for (int q = 0; q < src.npts; ++q) {
+                const int pass = seen[std::make_pair(fp, off[q])]++;
+                if ((int)pf.size() <= pass) { pf.resize(pass + 1); po.resize(pass + 1); pa.resize(pass + 1); pb.resize(pass + 1); ps.resize(pass + 1); }
+                pf[pass].push_back(fp); po[pass].push_back(off[q]); pa[pass].push_back(ka[q]);
+                pb[pass].push_back(kb[q]); ps[pass].push_back(sid);
+            }
+        }
+    }
+    std::vector<void*> F; std::vector<long> O; std::vector<double> A, B; std::vector<int> S;
+    b->pass_first.clear(); b->pass_count.clear();
+    for (size_t q = 0; q < pf.size(); ++q) {
+        b->pass_first.push_back((int)F.size());
+        b->pass_count.push_back((int)pf[q].size());
+        F.insert(F.end(), pf[q].begin(), pf[q].end()); O.insert(O.end(), po[q].begin(), po[q].end());
+        A.insert(A.end(), pa[q].begin(), pa[q].end()); B.insert(B.end(), pb[q].begin(), pb[q].end());
+        S.insert(S.end(), ps[q].begin(), ps[q].end());
+    }
+    b->n_src_pts = (int)F.size();
+    rc = bupload(b->s, &b->d_src_field, F);
+    if (!rc) rc = bupload(b->s, &b->d_src_off, O);
+    if (!rc) rc = bupload(b->s, &b->d_src_ka, A);
+    if (!rc) rc = bupload(b->s, &b->d_src_kb, B);
+    if (!rc) rc = bupload(b->s, &b->d_src_of, S);
+    if (!rc) rc = bupload(b->s, &b->d_sp, sp);
+    if (rc) return rc;
+    // detectors
+    std::vector<int> go, lo, cd;
+    std::vector<long> of;
+    for (int q = 0; q < b->n; ++q) {
+        omx3d_ctx* c = b->g[q];
+        for (long k = 0; k < c->n_samples; ++k) {
+            go.push_back(q); lo.push_back((int)k); cd.push_back(c->h_det_field[k]); of.push_back(c->h_det_off[k]);
+        }
+    }
+    b->n_samp = (long)go.size();
+    rc = bupload(b->s, &b->d_grid_of, go);
+    if (!rc) rc = bupload(b->s, &b->d_local, lo);
+    if (!rc) rc = bupload(b->s, &b->d_code, cd);
+    if (!rc) rc = bupload(b->s, &b->d_off, of);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(b->s));
+    return OMX_OK;
+}
+
+template <typename T>
+int batch_step_launches(omxb_ctx* b) {
+    const dim3 blk(kLanes, 4);
+    if (b->periodic)
+        for (auto* c : b->g) { int rc = wrap_e<T>(c, b->s); if (rc) return rc; }
+    hipLaunchKernelGGL(k_h_update_batched<T>, dim3(b->total_tiles), blk, 0, b->s, (const YeeArgs<T>*)b->d_args,
+                       b->d_tile_start, b->n);
+    for (size_t q = 0; q < b->pass_first.size(); ++q) {
+        const int f0 = b->pass_first[q], nq = b->pass_count[q];
+        hipLaunchKernelGGL(k_source_batched<T>, dim3(blocks_for(nq, 256)), dim3(256), 0, b->s,
+                           (T* const*)(b->d_src_field + f0), b->d_src_off + f0, b->d_src_ka + f0, b->d_src_kb + f0,
+                           b->d_src_of + f0, b->d_sp, nq, (const long*)b->d_step);
+    }
+    if (b->periodic)
+        for (auto* c : b->g) { int rc = wrap_h<T>(c, b->s); if (rc) return rc; }
+    hipLaunchKernelGGL(k_e_update_batched<T>, dim3(b->total_tiles), blk, 0, b->s, (const YeeArgs<T>*)b->d_args,
+                       b->d_tile_start, b->n);
+    if (b->n_samp > 0)
+        hipLaunchKernelGGL(k_sample_batched<T>, dim3(blocks_for(b->n_samp, 256)), dim3(256), 0, b->s,
+                           (const YeeArgs<T>*)b->d_args, b->d_br, b->d_grid_of, b->d_local, b->d_code, b->d_off,
+                           b->n_samp, (const long*)b->d_step, b->d_overflow);
+    hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(1), 0, b->s, b->d_step);
+    HIPCHK(hipGetLastError());
+    return OMX_OK;
+}
+
+template <typename T>
+int batch_run(omxb_ctx* b, int tt_first, int n_steps, int every, int use_graph) {
+    std::vector<BatchRecord> br(b->n);
+    int first_tt = -1;
+    if (every > 0)
+        for (int tt = tt_first; tt < tt_first + n_steps; ++tt)
+            if (tt % every == 0) { first_tt = tt; break; }
+    int nrec = 0;
+    if (every > 0)
+        for (int tt = tt_first; tt < tt_first + n_steps; ++tt) nrec += (tt % every == 0);
+    for (int q = 0; q < b->n; ++q) {
+        omx3d_ctx* c = b->g[q];
+        int rc = upload_detectors(c);
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(c->s));
+        br[q].rec = c->d_rec;
+        br[q].n_samples = c->n_samples;
+        br[q].every = (c->n_samples > 0 && c->rec_cap > 0) ? every : 0;
+        br[q].row0 = first_tt > 0 ? first_tt / every - c->rec_rows : 0;
+        br[q].cap = c->rec_cap;
+        if (br[q].every > 0 && c->rec_rows + nrec > c->rec_cap)
+            return fail(OMX_ERR_RANGE, "batch grid " + std::to_string(q) + ": detector record buffer too small");
+    }
+    int rc = bupload(b->s, &b->d_br, br);
+    if (rc) return rc;
+    const long st = tt_first;
+    HIPCHK(hipMemcpyAsync(b->d_step, &st, sizeof(long), hipMemcpyHostToDevice, b->s));
+    const int zero = 0;
+    HIPCHK(hipMemcpyAsync(b->d_overflow, &zero, sizeof(int), hipMemcpyHostToDevice, b->s));
+    if (use_graph && n_steps > 0) {
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+        HIPCHK(hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
+        rc = batch_step_launches<T>(b);
+        hipGraph_t gtmp;
+        const hipError_t ec = hipStreamEndCapture(b->s, &gtmp);
+        if (rc) return rc;
+        if (ec != hipSuccess) return fail(OMX_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ec));
+        graph = gtmp;
+        HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        for (int t = 0; t < n_steps; ++t) HIPCHK(hipGraphLaunch(exec, b->s));
+        HIPCHK(hipStreamSynchronize(b->s));
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+    } else {
+        for (int t = 0; t < n_steps; ++t) {
+            rc = batch_step_launches<T>(b);
+            if (rc) return rc;
+        }
+        HIPCHK(hipStreamSynchronize(b->s));
+    }
+    int ov = 0;
+    HIPCHK(hipMemcpy(&ov, b->d_overflow, sizeof(int), hipMemcpyDeviceToHost));
+    if (ov) return fail(OMX_ERR_RANGE, "batch detector rows out of range");
+    for (auto* c : b->g)
+        if (c->n_samples > 0 && c->rec_cap > 0) c->rec_rows += nrec;
+    return OMX_OK;
+}
+
+}  // namespace
 // ===========================================================================
 // C-ABI
 // ===========================================================================
@@ -1151,6 +1359,116 @@ int omx3d_synchronize(omx3d_ctx* c) {
 int omx3d_device_bytes(omx3d_ctx* c, int64_t* bytes) {
     CHECK_CTX(c);
     if (bytes) *bytes = c->device_bytes;
+    return OMX_OK;
+}
+
+int omxb_create(int32_t n, omx3d_ctx* const* grids, omxb_ctx** out) {
+    if (!out || !grids || n < 1) return fail(OMX_ERR_ARG, "bad batch arguments");
+    *out = nullptr;
+    omxb_ctx* b = new omxb_ctx();
+    b->n = n;
+    b->prec = grids[0] ? grids[0]->prec : 8;
+    for (int q = 0; q < n; ++q) {
+        omx3d_ctx* c = grids[q];
+        if (!c) { delete b; return fail(OMX_ERR_ARG, "null grid in batch"); }
+        if (c->prec != b->prec) { delete b; return fail(OMX_ERR_ARG, "batch grids must share one precision"); }
+        if (c->rows_y > 4) { delete b; return fail(OMX_ERR_ARG, "batch grids need rows_y <= 4"); }
+        if (c->tfsf || c->nq) { delete b; return fail(OMX_ERR_ARG, "batched grids take no TF/SF or quantum sites"); }
+        for (int a = 0; a < 3; ++a) if (c->d.boundaries[a] == OMX_BC_PERIODIC) b->periodic = true;
+        b->g.push_back(c);
+    }
+    if (hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking) != hipSuccess) { delete b; return fail(OMX_ERR_HIP, "stream"); }
+    if (hipMalloc((void**)&b->d_step, sizeof(long)) != hipSuccess ||
+        hipMalloc((void**)&b->d_overflow, sizeof(int)) != hipSuccess) {
+        omxb_destroy(b);
+        return fail(OMX_ERR_NOMEM, "batch allocation failed");
+    }
+    for (auto* c : b->g) (void)hipStreamSynchronize(c->s);
+    int rc = b->prec == 4 ? batch_setup<float>(b) : batch_setup<double>(b);
+    if (rc) { omxb_destroy(b); return rc; }
+    *out = b;
+    return OMX_OK;
+}
+
+int omxb_run(omxb_ctx* b, int32_t tt_first, int32_t n_steps, int32_t record_every, int32_t use_graph) {
+    if (!b) return fail(OMX_ERR_ARG, "null batch");
+    if (n_steps < 0) return fail(OMX_ERR_ARG, "n_steps must be >= 0");
+    return b->prec == 4 ? batch_run<float>(b, tt_first, n_steps, record_every, use_graph)
+                        : batch_run<double>(b, tt_first, n_steps, record_every, use_graph);
+}
+
+int omxb_destroy(omxb_ctx* b) {
+    if (!b) return OMX_OK;
+    if (b->s) (void)hipStreamSynchronize(b->s);
+    dfree(b->d_args); dfreet(b->d_tile_start);
+    dfreet(b->d_src_field); dfreet(b->d_src_off); dfreet(b->d_src_ka); dfreet(b->d_src_kb); dfreet(b->d_src_of);
+    dfreet(b->d_sp); dfreet(b->d_br); dfreet(b->d_grid_of); dfreet(b->d_local); dfreet(b->d_code); dfreet(b->d_off);
+    dfreet(b->d_overflow); dfreet(b->d_step);
+    if (b->s) (void)hipStreamDestroy(b->s);
+    delete b;
+    return OMX_OK;
+}
+
+// Times the tile shapes (rows_y in {1,2,4} x xchunk in {4,8,16,32,64}) with
+// `steps` H+E steps each and keeps the fastest.  The fields, CPML and
+// dispersive state are cleared afterwards, so call it before a run.
+int omx3d_autotune(omx3d_ctx* c, int32_t steps, int32_t* rows_y, int32_t* xchunk, double* ms_per_step) {
+    CHECK_CTX(c);
+    if (steps < 1) return fail(OMX_ERR_ARG, "steps must be >= 1");
+    const int rows_c[3] = {1, 2, 4};
+    const int xc_c[5] = {4, 8, 16, 32, 64};
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    double best = 1e30;
+    int br = c->rows_y, bx = c->xchunk;
+    for (int r : rows_c)
+        for (int x : xc_c) {
+            c->rows_y = r; c->xchunk = x;
+            for (int w = 0; w < 2; ++w) {   // warm-up launch pair
+                int rc = c->prec == 4 ? launch_h<float>(c) : launch_h<double>(c);
+                if (!rc) rc = c->prec == 4 ? launch_e<float>(c, 0.0, 0.0) : launch_e<double>(c, 0.0, 0.0);
+                if (rc) return rc;
+            }
+            HIPCHK(hipEventRecord(e0, c->s));
+            for (int t = 0; t < steps; ++t) {
+                int rc = c->prec == 4 ? launch_h<float>(c) : launch_h<double>(c);
+                if (!rc) rc = c->prec == 4 ? launch_e<float>(c, 0.0, 0.0) : launch_e<double>(c, 0.0, 0.0);
+                if (rc) return rc;
+            }
+            HIPCHK(hipEventRecord(e1, c->s));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            if (ms / steps < best) { best = ms / steps; br = r; bx = x; }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->rows_y = br; c->xchunk = bx;
+    // back to the initial state: zero fields, psi and dispersive state
+    const size_t fb = c->esz * c->L.nalloc;
+    for (auto p : c->f) HIPCHK(hipMemsetAsync(p, 0, fb, c->s));
+    const size_t A = 128 / c->esz;
+    const size_t xs[2] = {c->nsh * c->L.sx + A, c->nse * c->L.sx + A};
+    const size_t ys[2] = {(size_t)(c->d.nx + 2) * c->nsh * c->L.sz + A, (size_t)(c->d.nx + 2) * c->nse * c->L.sz + A};
+    const size_t zs[2] = {(size_t)(c->d.nx + 2) * c->L.sy * c->nsh, (size_t)(c->d.nx + 2) * c->L.sy * c->nse};
+    for (int a = 0; a < 3; ++a) {
+        const size_t* sz = a == 0 ? xs : (a == 1 ? ys : zs);
+        for (int q = 0; q < 2; ++q) {
+            if (c->psi_h[2 * a + q]) HIPCHK(hipMemsetAsync(c->psi_h[2 * a + q], 0, c->esz * sz[0], c->s));
+            if (c->psi_e[2 * a + q]) HIPCHK(hipMemsetAsync(c->psi_e[2 * a + q], 0, c->esz * sz[1], c->s));
+        }
+    }
+    for (int q = 0; q < 3; ++q) {
+        if (c->disp_pd[q]) HIPCHK(hipMemsetAsync(c->disp_pd[q], 0, fb, c->s));
+        if (c->disp_eold[q]) HIPCHK(hipMemsetAsync(c->disp_eold[q], 0, fb, c->s));
+        if (c->disp_pl[q]) HIPCHK(hipMemsetAsync(c->disp_pl[q], 0, fb * c->maxp, c->s));
+        if (c->disp_plold[q]) HIPCHK(hipMemsetAsync(c->disp_plold[q], 0, fb * c->maxp, c->s));
+    }
+    HIPCHK(hipStreamSynchronize(c->s));
+    if (rows_y) *rows_y = br;
+    if (xchunk) *xchunk = bx;
+    if (ms_per_step) *ms_per_step = best;
     return OMX_OK;
 }
 

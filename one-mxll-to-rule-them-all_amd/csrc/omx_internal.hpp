@@ -72,7 +72,7 @@ struct YeeArgs {
     int off;
     int npml;
     unsigned cpml;             // bit f set: CPML on face f (x-lo, x-hi, y-lo, y-hi, z-lo, z-hi)
-    int ntz, nty, xchunk;
+    int ntz, nty, xchunk, rows;
 };
 
 struct SourceDev {

@@ -293,14 +293,13 @@ __device__ __forceinline__ T psi_step(T psi, T b, T c, T d) {
 }
 
 template <typename T, bool CPML>
-__global__ __launch_bounds__(256) void k_h_update(const YeeArgs<T> a) {
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
+__device__ __forceinline__ void h_tile(const YeeArgs<T>& a, const int t) {
     const int tz = t % a.ntz;
     const int r = t / a.ntz;
     const int ty = r % a.nty;
     const int tx = r / a.nty;
     const int k = tz * kLanes + threadIdx.x + 1;
-    const int j = ty * blockDim.y + threadIdx.y + 1;
+    const int j = ty * a.rows + threadIdx.y + 1;
     if (k > a.nz || j > a.ny) return;
     const int i0 = tx * a.xchunk + 1;
     const int i1 = min(i0 + a.xchunk - 1, a.nx);
@@ -434,14 +433,13 @@ __device__ __forceinline__ T medium_update(T e, T rot, int m, const MedDev<T>* _
 }
 
 template <typename T, bool CPML, bool MEDIA>
-__global__ __launch_bounds__(256) void k_e_update(const YeeArgs<T> a) {
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
+__device__ __forceinline__ void e_tile(const YeeArgs<T>& a, const int t) {
     const int tz = t % a.ntz;
     const int r = t / a.ntz;
     const int ty = r % a.nty;
     const int tx = r / a.nty;
     const int k = tz * kLanes + threadIdx.x + 1;
-    const int j = ty * blockDim.y + threadIdx.y + 1;
+    const int j = ty * a.rows + threadIdx.y + 1;
     if (k > a.nz || j > a.ny) return;
     const int i0 = tx * a.xchunk + 1;
     const int i1 = min(i0 + a.xchunk - 1, a.nx);
@@ -462,7 +460,7 @@ __global__ __launch_bounds__(256) void k_e_update(const YeeArgs<T> a) {
         const T ry = (hzi - hzc) * dex + (hxc - hxk) * dez;
         const T rz = (hyc - hyi) * dex + (hxj - hxc) * dey;
         T ex = a.ex[o], ey = a.ey[o], ez = a.ez[o];
-        if constexpr (MEDIA) {
+        if (MEDIA && a.tag != nullptr) {
             const int tg = a.tag[o];
             if (tg == 0) {
                 ex = ex + dep * rx; ey = ey + dep * ry; ez = ez + dep * rz;
@@ -535,6 +533,54 @@ __global__ __launch_bounds__(256) void k_e_update(const YeeArgs<T> a) {
     }
 }
 
+template <typename T, bool CPML>
+__global__ __launch_bounds__(256) void k_h_update(const YeeArgs<T> a) {
+    h_tile<T, CPML>(a, xcd_tile(blockIdx.x, gridDim.x));
+}
+
+template <typename T, bool CPML, bool MEDIA>
+__global__ __launch_bounds__(256) void k_e_update(const YeeArgs<T> a) {
+    e_tile<T, CPML, MEDIA>(a, xcd_tile(blockIdx.x, gridDim.x));
+}
+
+// ---------------------------------------------------------------------------
+// Batched launches: one launch covers the tiles of many independent grids.
+// tile_start[g] is the first tile of grid g (tile_start[n] = total); each
+// workgroup finds its grid by binary search and runs the single-grid tile
+// code on that grid's argument block, so every grid's arithmetic is
+// identical to its own single-grid launch.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int find_grid(const int* __restrict__ tile_start, int n, int t) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tile_start[mid] <= t) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_h_update_batched(const YeeArgs<T>* __restrict__ args,
+                                                          const int* __restrict__ tile_start, int n) {
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int g = find_grid(tile_start, n, t);
+    const YeeArgs<T>& a = args[g];
+    if ((int)threadIdx.y >= a.rows) return;
+    if (a.cpml) h_tile<T, true>(a, t - tile_start[g]);
+    else h_tile<T, false>(a, t - tile_start[g]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_e_update_batched(const YeeArgs<T>* __restrict__ args,
+                                                          const int* __restrict__ tile_start, int n) {
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int g = find_grid(tile_start, n, t);
+    const YeeArgs<T>& a = args[g];
+    if ((int)threadIdx.y >= a.rows) return;
+    if (a.cpml) e_tile<T, true, true>(a, t - tile_start[g]);
+    else e_tile<T, false, true>(a, t - tile_start[g]);
+}
+
 // ---------------------------------------------------------------------------
 // Periodic wrap: whole-plane copies before each half step, non-MPI branch of
 // mxll_3D_mod.f90:528-541 (E) and :725-738 (H).  One launch per axis, in the
@@ -562,14 +608,48 @@ __global__ void k_wrap(T* __restrict__ f0, T* __restrict__ f1, Layout L, int axi
 // branch of source_interactions (interactions_mod.f90:102-156).  The J_mat
 // entry one step past the kernel edge reads as zero.
 // ---------------------------------------------------------------------------
+struct SourceParams {
+    double j_amp, w0, t0, tau, t_init, t_final, phase, c_src, dt;
+};
+
+// J_mat = j_amp * exp(-((t-t0)/tau)^2) * cos(w0 (t-t0) + phase) * ker, zero
+// outside [t_init, t_final]; E += c_src * 1/2 (J_mat(p) + J_mat(p+1)).
+__device__ inline double source_amp(const SourceParams& s, double time) {
+    if (!(time >= s.t_init && time <= s.t_final)) return 0.0;
+    const double x = (time - s.t0) / s.tau;
+    return (s.j_amp * exp(-(x * x))) * cos(s.w0 * (time - s.t0) + s.phase);
+}
+
 template <typename T>
 __global__ void k_source(T* __restrict__ f, const long* __restrict__ off, const double* __restrict__ ka,
-                         const double* __restrict__ kb, int n, double s, double c_src) {
+                         const double* __restrict__ kb, int n, SourceParams sp, double time) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
+    const double s = source_amp(sp, time);
     const double jav = 0.5 * (s * ka[p] + s * kb[p]);
-    f[off[p]] = (T)((double)f[off[p]] + c_src * jav);
+    f[off[p]] = (T)((double)f[off[p]] + sp.c_src * jav);
 }
+
+// All points of all sources of a batch; the step index comes from device
+// memory so that the launch can be replayed from a captured graph.  Points
+// of one source never share a cell; sources that share a cell are kept in
+// separate launches (pass), in their order.
+template <typename T>
+__global__ void k_source_batched(T* const* __restrict__ field, const long* __restrict__ off,
+                                 const double* __restrict__ ka, const double* __restrict__ kb,
+                                 const int* __restrict__ src_of, const SourceParams* __restrict__ sp, int n,
+                                 const long* __restrict__ step) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const SourceParams& q = sp[src_of[p]];
+    const double time = q.dt * ((double)step[0] - 0.5);
+    const double s = source_amp(q, time);
+    const double jav = 0.5 * (s * ka[p] + s * kb[p]);
+    T* f = field[p];
+    f[off[p]] = (T)((double)f[off[p]] + q.c_src * jav);
+}
+
+__global__ void k_step_advance(long* step) { step[0] += 1; }
 
 // ---------------------------------------------------------------------------
 // Detector sampling with the stencils of write_3D_field
@@ -597,6 +677,46 @@ __global__ void k_sample(const YeeArgs<T> a, const int* __restrict__ code, const
         default: break;
     }
     out[s] = v;
+}
+
+struct BatchRecord {
+    double* rec;        // record buffer of the grid
+    long n_samples;     // samples per row of the grid
+    int every, row0, cap;
+};
+
+// Detector samples of all grids of a batch at step `step`, when
+// mod(step, every) == 0; row = step/every - row0 of that grid.
+template <typename T>
+__global__ void k_sample_batched(const YeeArgs<T>* __restrict__ args, const BatchRecord* __restrict__ br,
+                                 const int* __restrict__ grid_of, const int* __restrict__ local,
+                                 const int* __restrict__ code, const long* __restrict__ off, long n,
+                                 const long* __restrict__ step, int* __restrict__ overflow) {
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int g = grid_of[s];
+    const BatchRecord& R = br[g];
+    const long st = step[0];
+    if (R.every <= 0 || st % R.every != 0) return;
+    const long row = st / R.every - R.row0;
+    if (row < 0 || row >= R.cap) { overflow[0] = 1; return; }
+    const YeeArgs<T>& a = args[g];
+    const int cd = code[s];
+    const int field = cd >> 1;
+    const bool point = cd & 1;
+    const long o = off[s];
+    const long sx = a.sx, sz = a.sz;
+    double v = 0.0;
+    switch (field) {
+        case OMX_EX: v = 0.5 * ((double)a.ex[o - sx] + (double)a.ex[o]); break;
+        case OMX_EY: v = 0.5 * ((double)a.ey[o - sz] + (double)a.ey[o]); break;
+        case OMX_EZ: v = point ? (double)a.ez[o] : 0.5 * ((double)a.ez[o - 1] + (double)a.ez[o]); break;
+        case OMX_HX: v = 0.25 * ((((double)a.hx[o - sz] + (double)a.hx[o]) + (double)a.hx[o - 1]) + (double)a.hx[o - sz - 1]); break;
+        case OMX_HY: v = 0.25 * ((((double)a.hy[o - sx] + (double)a.hy[o]) + (double)a.hy[o - 1]) + (double)a.hy[o - sx - 1]); break;
+        case OMX_HZ: v = 0.25 * ((((double)a.hz[o] + (double)a.hz[o - sx]) + (double)a.hz[o - sz]) + (double)a.hz[o - sx - sz]); break;
+        default: break;
+    }
+    R.rec[row * R.n_samples + local[s]] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -726,7 +846,7 @@ __global__ __launch_bounds__(256) void k_energy(const YeeArgs<T> a, const double
         double ex = a.ex[o], ey = a.ey[o], ez = a.ez[o];
         double hx = a.hx[o], hy = a.hy[o], hz = a.hz[o];
         double er[3] = {1.0, 1.0, 1.0};
-        if constexpr (MEDIA) {
+        if (MEDIA && a.tag != nullptr) {
             const int tg = a.tag[o];
             er[0] = eps_cls[3 * tg]; er[1] = eps_cls[3 * tg + 1]; er[2] = eps_cls[3 * tg + 2];
         }

@@ -7,6 +7,6 @@ solver interface (TMxll, src/FDTD/mxll_base_mod.f90:7-69) in `Grid3D`.
 """
 from .units import *  # noqa: F401,F403
 from ._lib import load_library, OmxError  # noqa: F401
-from .grid import Grid3D  # noqa: F401
+from .grid import Grid3D, Batch  # noqa: F401
 from .grid import medium as grid_medium, library_medium  # noqa: F401,E402
 from . import cases, host  # noqa: F401,E402

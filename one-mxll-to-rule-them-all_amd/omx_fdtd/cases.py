@@ -174,3 +174,35 @@ def config_dl_sphere_tfsf(n=256, dx_nm=2.0, npml=12, steps=4000, seed_no=2, reco
                 shapes=[(1, 1, [float(c[0]), float(c[1]), float(c[2]), r])], detectors=dets, tfsf=tfsf,
                 info={"sphere_centre_cells": c.tolist(), "sphere_radius_nm": r_nm, "detector_k": kdet,
                       "medium": med})
+
+
+def config_batched(n_grids=64, edges=(96, 128, 160, 192), npml=8, steps=2000, dx_nm=2.0, seed_no=4,
+                   record_every=10):
+    """Independent grids with edges drawn from `edges` per axis, CPML on every
+    face, 1-8 seeded Drude-Lorentz spheres (radius 5-30 cells, clipped to
+    the non-CPML interior) and one dipole source of seeded polarisation (three
+    point sources x, y, z scaled by the polarisation components); one point
+    detector (Ex, Ey, Ez) per grid."""
+    rng = np.random.default_rng(SEED_BASE + seed_no)
+    out = []
+    for g in range(n_grids):
+        n = tuple(int(v) for v in rng.choice(edges, size=3))
+        med = dl_sphere_poles(rng)
+        shapes = []
+        for _ in range(int(rng.integers(1, 9))):
+            rmax = min(30.0, (min(n) - 2 * npml - 4) / 2.0)
+            r = float(rng.uniform(5.0, max(5.0, rmax)))
+            c = [float(rng.uniform(npml + r + 2, n[a] - npml - r - 2)) for a in range(3)]
+            shapes.append((1, 1, c + [r]))
+        p = rng.normal(size=3)
+        p /= np.linalg.norm(p)
+        src_cell = [int(rng.integers(npml + 4, n[a] - npml - 4)) for a in range(3)]
+        pos_nm = [index_to_nm(src_cell[a], dx_nm, n[a]) for a in range(3)]
+        srcs = [(d, float(p[q]), 2.066, 4.0, 1.5, pos_nm[0], pos_nm[1], pos_nm[2], 0.0, 0.0, 1.0e6, 0.0)
+                for q, d in enumerate(("x", "y", "z"))]
+        det_cell = [int(rng.integers(npml + 2, n[a] - npml - 2)) for a in range(3)]
+        out.append(Case(name=f"batch{g}", grid=n, dx_nm=dx_nm, npml=npml, boundaries=(BC_CPML,) * 3,
+                        steps=steps, record_every=record_every, media=[med], shapes=shapes, sources=srcs,
+                        detectors=point_detectors([det_cell]),
+                        info={"polarisation": p.tolist(), "n_spheres": len(shapes)}))
+    return out

@@ -101,6 +101,12 @@ class Grid3D:
     def set_tiling(self, rows_y=0, xchunk=0):
         L.call("omx3d_set_tiling", self.h, int(rows_y), int(xchunk))
 
+    def autotune(self, steps=4):
+        """Pick the fastest tile shape; clears the fields (call before a run)."""
+        r, x, ms = C.c_int32(), C.c_int32(), C.c_double()
+        L.call("omx3d_autotune", self.h, int(steps), C.byref(r), C.byref(x), C.byref(ms))
+        return r.value, x.value, ms.value
+
     # -- coefficients ------------------------------------------------------
     def cpml_tables(self):
         p = self.npml
@@ -270,3 +276,29 @@ class Grid3D:
     @property
     def has_cpml(self):
         return any(b == BC_CPML for b in self.boundaries)
+
+
+class Batch:
+    """Independent grids advanced together: one launch per phase for all."""
+
+    def __init__(self, grids):
+        self.lib = L.load_library()
+        self.grids = list(grids)
+        arr = (C.c_void_p * len(self.grids))(*[g.h for g in self.grids])
+        h = C.c_void_p()
+        L.call("omxb_create", len(self.grids), arr, C.byref(h))
+        self.h = h
+
+    def run(self, tt_first, n_steps, record_every=0, graph=True):
+        L.call("omxb_run", self.h, int(tt_first), int(n_steps), int(record_every), 1 if graph else 0)
+
+    def kill(self):
+        if getattr(self, "h", None):
+            self.lib.omxb_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.kill()
+        except Exception:
+            pass

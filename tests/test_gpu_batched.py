@@ -1,0 +1,59 @@
+"""Batched launches: every grid of a batch computes exactly (bitwise) what
+its own single-grid run computes, with and without the captured graph."""
+import numpy as np
+import pytest
+
+import omx_fdtd
+from omx_fdtd import cases, grid as G
+
+pytestmark = pytest.mark.gpu
+
+
+def make(case, precision, tiling=None):
+    dr, dt = case.dr_dt
+    g = omx_fdtd.Grid3D(case.grid, case.npml, case.boundaries, dt, dr, precision=precision)
+    if tiling:
+        g.set_tiling(*tiling)
+    cases.apply(case, g, library=G.library_medium, make_medium=G.medium)
+    return g
+
+
+@pytest.mark.parametrize("precision", [4, 8])
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_equals_single_grid_runs(precision, graph):
+    cs = cases.config_batched(n_grids=6, edges=(24, 32, 40), npml=4, steps=60, record_every=3)
+    # a second source sharing a cell with the first of grid 0 exercises source passes
+    cs[0].sources.append(cs[0].sources[2])
+    tilings = [(4, 16), (2, 5), (1, 64), (4, 3), (4, 8), (1, 1)]
+    print(f"[batch] precision={precision} graph={graph} grids={[c.grid for c in cs]}", flush=True)
+    singles = []
+    for c, t in zip(cs, tilings):
+        g = make(c, precision)
+        g.run(1, c.steps, c.record_every)
+        singles.append(g.read_records())
+        g.kill()
+    grids = [make(c, precision, t) for c, t in zip(cs, tilings)]
+    b = omx_fdtd.Batch(grids)
+    b.run(1, 30, 3, graph=graph)
+    b.run(31, 30, 3, graph=graph)
+    for c, g, ref in zip(cs, grids, singles):
+        got = g.read_records()
+        assert got.shape == ref.shape
+        assert np.abs(ref).max() > 0
+        np.testing.assert_array_equal(got, ref)
+    b.kill()
+    for g in grids:
+        g.kill()
+
+
+def test_autotune_keeps_results_identical():
+    c = cases.config_batched(n_grids=1, edges=(40,), npml=4, steps=40, record_every=2)[0]
+    g = make(c, 4)
+    g.run(1, c.steps, c.record_every)
+    ref = g.read_records()
+    g.kill()
+    g = make(c, 4)
+    rows, xc, ms = g.autotune(steps=2)
+    print(f"autotune: rows_y={rows} xchunk={xc} {ms:.4f} ms/step")
+    g.run(1, c.steps, c.record_every)
+    np.testing.assert_array_equal(g.read_records(), ref)
