@@ -74,6 +74,8 @@ YeeArgs<T> make_args(omx3d_ctx* c) {
     a.be = tc + c->co_be; a.ced = tc + c->co_ce; a.bh = tc + c->co_bh; a.chd = tc + c->co_ch;
     a.den_ex = tc + c->co_den[0]; a.den_ey = tc + c->co_den[1]; a.den_ez = tc + c->co_den[2];
     a.den_hx = tc + c->co_den[3]; a.den_hy = tc + c->co_den[4]; a.den_hz = tc + c->co_den[5];
+    a.cdex = tc + c->co_cd[0]; a.cdey = tc + c->co_cd[1]; a.cdez = tc + c->co_cd[2];
+    a.cdhx = tc + c->co_cd[3]; a.cdhy = tc + c->co_cd[4]; a.cdhz = tc + c->co_cd[5];
     for (int q = 0; q < 6; ++q) { a.ph[q] = (T*)c->psi_h[q]; a.pe[q] = (T*)c->psi_e[q]; }
     a.tag = c->d_tag; a.cls = c->d_cls; a.med = (const MedDev<T>*)c->d_med;
     for (int q = 0; q < 3; ++q) {
@@ -811,6 +813,7 @@ int omx3d_create(const omx3d_desc* desc, omx3d_ctx** out) {
     c->co_be = 0; c->co_ce = np + 1; c->co_bh = 2 * (np + 1); c->co_ch = c->co_bh + np;
     long p = c->co_ch + np;
     for (int q = 0; q < 6; ++q) { c->co_den[q] = p; p += n[q % 3] + 2; }
+    for (int q = 0; q < 6; ++q) { c->co_cd[q] = p; p += n[q % 3] + 2; }
     c->ncoef = p;
     if (np > 64 && mask) return bail(fail(OMX_ERR_RANGE, "npml above 64 is not supported"));
     if ((rc = dalloc(c, (void**)&c->dcoef, sizeof(double) * c->ncoef))) return bail(rc);
@@ -832,6 +835,9 @@ int omx3d_create(const omx3d_desc* desc, omx3d_ctx** out) {
         return bail(fail(OMX_ERR_HIP, "coefficient readback failed"));
     for (long q = c->co_ce; q < c->co_ce + np + 1; ++q) hc[q] /= d.dr;  // ce/dr
     for (long q = c->co_ch; q < c->co_ch + np; ++q) hc[q] /= d.dr;      // ch/dr
+    for (int q = 0; q < 6; ++q)                                           // dt_eps0*den_e, dt_mu0*den_h
+        for (int k = 0; k < n[q % 3] + 2; ++k)
+            hc[c->co_cd[q] + k] = (q < 3 ? d.dt / kEps0 : d.dt / kMu0) * hc[c->co_den[q] + k];
     if (c->prec == 4) {
         std::vector<float> hf(c->ncoef);
         for (long q = 0; q < c->ncoef; ++q) hf[q] = (float)hc[q];

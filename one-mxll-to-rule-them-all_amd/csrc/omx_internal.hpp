@@ -57,6 +57,8 @@ struct YeeArgs {
     T *ex, *ey, *ez, *hx, *hy, *hz;
     const T *den_ex, *den_ey, *den_ez, *den_hx, *den_hy, *den_hz;  // 1-based
     const T *bh, *chd, *be, *ced;                                  // 1-based; FP32: b-1
+    // FP32 only: dt_eps0*den_e* and dt_mu0*den_h* folded into one coefficient (1-based)
+    const T *cdex, *cdey, *cdez, *cdhx, *cdhy, *cdhz;
     // H psi: psix_Hyx psix_Hzx psiy_Hxy psiy_Hzy psiz_Hxz psiz_Hyz
     T *ph[6];
     // E psi: psix_Eyx psix_Ezx psiy_Exy psiy_Ezy psiz_Exz psiz_Eyz
@@ -103,7 +105,7 @@ struct omx3d_ctx {
     // FP64 master coefficients: be ce (npml+1) bh ch (npml) den_e{x,y,z} den_h{x,y,z} (n+2 each)
     double* dcoef = nullptr;
     void* tcoef = nullptr;          // precision-T copies used by the kernels
-    long co_be, co_ce, co_bh, co_ch, co_den[6];
+    long co_be, co_ce, co_bh, co_ch, co_den[6], co_cd[6];
     long ncoef = 0;
     unsigned cpml_mask = 0;
     int nsh = 0, nse = 0;           // psi slots per CPML axis (H, E)

@@ -309,16 +309,26 @@ __device__ __forceinline__ void h_tile(const YeeArgs<T>& a, const int t) {
     long o = (long)i0 * sx + jk;
     const T dmu = a.dt_mu0;
     const T dhy = a.den_hy[j], dhz = a.den_hz[k];
+    const T chy = sizeof(T) == 4 ? a.cdhy[j] : (T)0, chz = sizeof(T) == 4 ? a.cdhz[k] : (T)0;
     T ezc = a.ez[o], eyc = a.ey[o];
     const int npml = a.npml;
     for (int i = i0; i <= i1; ++i, o += sx) {
         const T ezn = a.ez[o + sx], eyn = a.ey[o + sx];
         const T ezj = a.ez[o + sz], eyk = a.ey[o + 1];
         const T exc = a.ex[o], exk = a.ex[o + 1], exj = a.ex[o + sz];
-        const T dhx = a.den_hx[i];
-        T hx = a.hx[o] + dmu * ((ezc - ezj) * dhy + (eyk - eyc) * dhz);
-        T hy = a.hy[o] + dmu * ((ezn - ezc) * dhx + (exc - exk) * dhz);
-        T hz = a.hz[o] + dmu * ((eyc - eyn) * dhx + (exj - exc) * dhy);
+        T hx, hy, hz;
+        if constexpr (sizeof(T) == 4) {
+            // FP32: dt_mu0*den folded into one coefficient per axis (one rounding fewer)
+            const T chx = a.cdhx[i];
+            hx = a.hx[o] + (ezc - ezj) * chy + (eyk - eyc) * chz;
+            hy = a.hy[o] + (ezn - ezc) * chx + (exc - exk) * chz;
+            hz = a.hz[o] + (eyc - eyn) * chx + (exj - exc) * chy;
+        } else {
+            const T dhx = a.den_hx[i];
+            hx = a.hx[o] + dmu * ((ezc - ezj) * dhy + (eyk - eyc) * dhz);
+            hy = a.hy[o] + dmu * ((ezn - ezc) * dhx + (exc - exk) * dhz);
+            hz = a.hz[o] + dmu * ((eyc - eyn) * dhx + (exj - exc) * dhy);
+        }
         if constexpr (CPML) {
             const unsigned cp = a.cpml;
             const int nsh = 2 * (npml - 1);
@@ -385,10 +395,10 @@ __device__ __forceinline__ void h_tile(const YeeArgs<T>& a, const int t) {
 // Currents of embedded quantum systems are applied by k_j_correct afterwards.
 // ---------------------------------------------------------------------------
 template <typename T>
-__device__ __forceinline__ T medium_update(T e, T rot, int m, const MedDev<T>* __restrict__ med, T dt_eps0,
+__device__ __forceinline__ T medium_update(T e, T rot, T vac, int m, const MedDev<T>* __restrict__ med,
                                            long o, T* __restrict__ eold, T* __restrict__ pd,
                                            T* __restrict__ pl, T* __restrict__ plold, long ps) {
-    if (m == 0) return e + dt_eps0 * rot;
+    if (m == 0) return e + vac;   // vac = dt_eps0 * rotH
     const MedDev<T>& M = med[m];
     if (M.kind == OMX_MEDIUM_DRUDE) {
         const T p = pd[o];
@@ -449,6 +459,7 @@ __device__ __forceinline__ void e_tile(const YeeArgs<T>& a, const int t) {
     long o = (long)i0 * sx + jk;
     const T dep = a.dt_eps0;
     const T dey = a.den_ey[j], dez = a.den_ez[k];
+    const T cey = sizeof(T) == 4 ? a.cdey[j] : (T)0, cez = sizeof(T) == 4 ? a.cdez[k] : (T)0;
     T hzi = a.hz[o - sx], hyi = a.hy[o - sx];
     const int npml = a.npml;
     for (int i = i0; i <= i1; ++i, o += sx) {
@@ -459,19 +470,29 @@ __device__ __forceinline__ void e_tile(const YeeArgs<T>& a, const int t) {
         const T rx = (hzc - hzj) * dey + (hyk - hyc) * dez;
         const T ry = (hzi - hzc) * dex + (hxc - hxk) * dez;
         const T rz = (hyc - hyi) * dex + (hxj - hxc) * dey;
+        // vacuum increments; FP32 folds dt_eps0*den into one coefficient per axis
+        T vx, vy, vz;
+        if constexpr (sizeof(T) == 4) {
+            const T cex = a.cdex[i];
+            vx = (hzc - hzj) * cey + (hyk - hyc) * cez;
+            vy = (hzi - hzc) * cex + (hxc - hxk) * cez;
+            vz = (hyc - hyi) * cex + (hxj - hxc) * cey;
+        } else {
+            vx = dep * rx; vy = dep * ry; vz = dep * rz;
+        }
         T ex = a.ex[o], ey = a.ey[o], ez = a.ez[o];
         if (MEDIA && a.tag != nullptr) {
             const int tg = a.tag[o];
             if (tg == 0) {
-                ex = ex + dep * rx; ey = ey + dep * ry; ez = ez + dep * rz;
+                ex = ex + vx; ey = ey + vy; ez = ez + vz;
             } else {
                 const int8_t* c3 = a.cls + 3 * tg;
-                ex = medium_update(ex, rx, (int)c3[0], a.med, dep, o, a.eold[0], a.pd[0], a.pl[0], a.plold[0], a.pstride);
-                ey = medium_update(ey, ry, (int)c3[1], a.med, dep, o, a.eold[1], a.pd[1], a.pl[1], a.plold[1], a.pstride);
-                ez = medium_update(ez, rz, (int)c3[2], a.med, dep, o, a.eold[2], a.pd[2], a.pl[2], a.plold[2], a.pstride);
+                ex = medium_update(ex, rx, vx, (int)c3[0], a.med, o, a.eold[0], a.pd[0], a.pl[0], a.plold[0], a.pstride);
+                ey = medium_update(ey, ry, vy, (int)c3[1], a.med, o, a.eold[1], a.pd[1], a.pl[1], a.plold[1], a.pstride);
+                ez = medium_update(ez, rz, vz, (int)c3[2], a.med, o, a.eold[2], a.pd[2], a.pl[2], a.plold[2], a.pstride);
             }
         } else {
-            ex = ex + dep * rx; ey = ey + dep * ry; ez = ez + dep * rz;
+            ex = ex + vx; ey = ey + vy; ez = ez + vz;
         }
         if constexpr (CPML) {
             const unsigned cp = a.cpml;

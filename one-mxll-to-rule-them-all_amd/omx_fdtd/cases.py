@@ -206,3 +206,82 @@ def config_batched(n_grids=64, edges=(96, 128, 160, 192), npml=8, steps=2000, dx
                         detectors=point_detectors([det_cell]),
                         info={"polarisation": p.tolist(), "n_spheres": len(shapes)}))
     return out
+
+
+def random_rotation(rng):
+    """Uniform random rotation (unit quaternion) as a row-major 3x3 matrix."""
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return [1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w),
+            2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w),
+            2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]
+
+
+def config_ellipsoids(grid=(768, 768, 512), dx_nm=1.0, npml=16, steps=10000, n_ell=64, seed_no=3,
+                      record_every=100):
+    """Large grid: 64 seeded Drude-Lorentz ellipsoids (semi-axes 5-25 nm,
+    uniform random orientations, centres in the upper three quarters) above a
+    dielectric substrate (eps 2.25, lower quarter of the grid); a z-dipole
+    point source above the substrate; 32 point detectors recording Ez; the
+    volume energy integral is taken every `record_every` steps."""
+    rng = np.random.default_rng(SEED_BASE + seed_no)
+    nx, ny, nz = grid
+    med = dl_sphere_poles(rng)
+    k_sub = nz // 4
+    shapes = []
+    for _ in range(n_ell):
+        ax = rng.uniform(5.0, 25.0, size=3) / dx_nm
+        ax = np.minimum(ax, (nz - k_sub - npml - 6) / 2.0)
+        amax = float(ax.max())
+        c = [float(rng.uniform(npml + amax + 2, nx - npml - amax - 2)),
+             float(rng.uniform(npml + amax + 2, ny - npml - amax - 2)),
+             float(rng.uniform(k_sub + amax + 2, nz - npml - amax - 2))]
+        shapes.append((2, 1, c + ax.tolist() + random_rotation(rng)))
+    shapes.append((3, 2, [1, 1, 1, nx, ny, k_sub]))
+    src_cell = [nx // 2, ny // 2, int(0.6 * nz)]
+    pos_nm = [index_to_nm(src_cell[a], dx_nm, grid[a]) for a in range(3)]
+    srcs = [("z", 1.0, 2.066, 3.0, 1.0, pos_nm[0], pos_nm[1], pos_nm[2], 0.0, 0.0, 1.0e6, 0.0)]
+    det = rng.integers([npml + 2, npml + 2, npml + 2], [nx - npml - 2, ny - npml - 2, nz - npml - 2], size=(32, 3))
+    return Case(name="ellipsoids", grid=grid, dx_nm=dx_nm, npml=npml, boundaries=(BC_CPML,) * 3, steps=steps,
+                record_every=record_every, media=[med, ("dielectric", 2.25)], shapes=shapes, sources=srcs,
+                detectors=point_detectors(det.tolist(), ("Ez",)), info={"substrate_k": k_sub})
+
+
+def config_emitters(grid=(512, 512, 384), dx_nm=2.0, npml=12, steps=20000, n_emit=4096, seed_no=5,
+                    record_every=100, slab_nm=40.0):
+    """Silver (library Ag) slab with a seeded nanohole array, 4096 two-level
+    emitters (transition 2.0-2.2 eV, dipole 5-15 D, dephasing 10-50 meV) at
+    seeded cells in a 20 nm layer above the slab, local-dipole coupling; a
+    z-dipole point source above the layer excites them."""
+    rng = np.random.default_rng(SEED_BASE + seed_no)
+    nx, ny, nz = grid
+    t = int(round(slab_nm / dx_nm))
+    k0 = nz // 2 - t
+    k1 = k0 + t - 1
+    shapes = [(3, 1, [1, 1, k0, nx, ny, k1])]
+    pitch = float(rng.uniform(80.0, 120.0)) / dx_nm
+    nh = int((nx - 2 * npml) // pitch)
+    for a in range(nh):
+        for b in range(nh):
+            r = float(rng.uniform(10.0, 20.0)) / dx_nm
+            cx = npml + (a + 0.5) * pitch
+            cy = npml + (b + 0.5) * pitch
+            shapes.append((4, 0, [cx, cy, r, k0, k1]))
+    layer = int(round(20.0 / dx_nm))
+    cells = np.stack([rng.integers(npml + 2, nx - npml - 2, size=n_emit),
+                      rng.integers(npml + 2, ny - npml - 2, size=n_emit),
+                      rng.integers(k1 + 1, k1 + 1 + layer, size=n_emit)], axis=1).astype(np.int32)
+    pol = rng.normal(size=(n_emit, 3))
+    pol /= np.linalg.norm(pol, axis=1)[:, None]
+    tl = dict(omega0=rng.uniform(2.0, 2.2, n_emit) * EV_TO_AU,
+              mu=rng.uniform(5.0, 15.0, n_emit) * DEBYE_TO_AU,
+              pol=pol, gamma2=rng.uniform(10e-3, 50e-3, n_emit) * EV_TO_AU)
+    src_cell = [nx // 2, ny // 2, min(nz - npml - 4, k1 + layer + 20)]
+    pos_nm = [index_to_nm(src_cell[a], dx_nm, grid[a]) for a in range(3)]
+    srcs = [("z", 1.0, 2.1, 3.0, 1.0, pos_nm[0], pos_nm[1], pos_nm[2], 0.0, 0.0, 1.0e6, 0.0)]
+    det = cells[:8].tolist()
+    return Case(name="emitters", grid=grid, dx_nm=dx_nm, npml=npml, boundaries=(BC_CPML,) * 3, steps=steps,
+                record_every=record_every, media=[("drude-lorentz", "Ag", 1.0)], shapes=shapes, sources=srcs,
+                detectors=point_detectors(det), q_sites=cells, q_density=1.0e-3 / NM_TO_AU ** 3,
+                two_level=tl, info={"slab_k": [k0, k1], "holes": len(shapes) - 1, "pitch_cells": pitch})
