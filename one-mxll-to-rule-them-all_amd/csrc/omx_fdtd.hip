@@ -276,6 +276,109 @@ int ensure_events(omx3d_ctx* c, size_t n) {
     return OMX_OK;
 }
 
+size_t field_bytes(const omx3d_ctx* c) { return c->esz * (size_t)c->L.nalloc; }
+
+// Sizes of the H psi arrays per axis (x, y, z slabs).
+size_t psi_h_elems(const omx3d_ctx* c, int axis) {
+    const size_t A = 128 / c->esz;
+    if (axis == 0) return c->nsh * c->L.sx + A;
+    if (axis == 1) return (size_t)(c->d.nx + 2) * c->nsh * c->L.sz + A;
+    return (size_t)(c->d.nx + 2) * c->L.sy * c->nsh;
+}
+
+// Second field / H-psi buffers of the fused step: a copy of the current ones
+// (boundary values never change, interior values are rewritten each step).
+int ensure_alt(omx3d_ctx* c) {
+    if (c->f_alt[0]) return OMX_OK;
+    for (int q = 0; q < 6; ++q) {
+        int rc = dalloc(c, &c->f_alt[q], field_bytes(c));
+        if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(c->f_alt[q], c->f[q], field_bytes(c), hipMemcpyDeviceToDevice, c->s));
+    }
+    for (int q = 0; q < 6; ++q) {
+        if (!c->psi_h[q]) continue;
+        const size_t b = c->esz * psi_h_elems(c, q / 2);
+        int rc = dalloc(c, &c->psi_h_alt[q], b);
+        if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(c->psi_h_alt[q], c->psi_h[q], b, hipMemcpyDeviceToDevice, c->s));
+    }
+    return OMX_OK;
+}
+
+// Whether a source point lies in a Drude/DL component (the fused step adds
+// sources after the E update, which equals the reference order only where
+// the E update is linear in E without memory, i.e. vacuum and dielectric).
+int check_sources_dispersive(omx3d_ctx* c) {
+    if (c->src_in_dispersive >= 0) return OMX_OK;
+    c->src_in_dispersive = 0;
+    if (!c->media_set || c->n_classes <= 1 || c->sources.empty()) return OMX_OK;
+    std::vector<int8_t> cls(3 * (size_t)c->n_classes);
+    HIPCHK(hipMemcpy(cls.data(), c->d_cls, cls.size(), hipMemcpyDeviceToHost));
+    for (auto& s : c->sources) {
+        std::vector<long> off(s.npts);
+        HIPCHK(hipMemcpy(off.data(), s.d_off, sizeof(long) * s.npts, hipMemcpyDeviceToHost));
+        for (long o : off) {
+            uint8_t tg = 0;
+            HIPCHK(hipMemcpy(&tg, c->d_tag + o, 1, hipMemcpyDeviceToHost));
+            const int m = cls[3 * tg + s.dir - 1];
+            if (m && (c->media[m].kind == OMX_MEDIUM_DRUDE || c->media[m].kind == OMX_MEDIUM_DL)) {
+                c->src_in_dispersive = 1;
+                return OMX_OK;
+            }
+        }
+    }
+    return OMX_OK;
+}
+
+bool fused_applies(omx3d_ctx* c) {
+    if (c->step_mode != 0) return false;
+    for (int a = 0; a < 3; ++a)
+        if (c->d.boundaries[a] == OMX_BC_PERIODIC) return false;
+    return c->src_in_dispersive == 0;
+}
+
+template <typename T>
+int launch_fused(omx3d_ctx* c) {
+    int rc = ensure_alt(c);
+    if (rc) return rc;
+    YeeArgs<T> a = make_args<T>(c);
+    FusedArgs<T> f;
+    std::memset(&f, 0, sizeof(f));
+    f.exo = (T*)c->f_alt[0]; f.eyo = (T*)c->f_alt[1]; f.ezo = (T*)c->f_alt[2];
+    f.hxo = (T*)c->f_alt[3]; f.hyo = (T*)c->f_alt[4]; f.hzo = (T*)c->f_alt[5];
+    for (int q = 0; q < 6; ++q) f.pho[q] = (T*)c->psi_h_alt[q];
+    const int FR = c->fused_rows;
+    a.xchunk = c->fused_xchunk;
+    f.ntz = (c->L.nz + kLanes - 2) / (kLanes - 1);
+    f.nty = (c->L.ny + FR - 2) / (FR - 1);
+    const long ntx = (c->L.nx + a.xchunk - 1) / a.xchunk;
+    const unsigned nb = (unsigned)((long)f.ntz * f.nty * ntx);
+    if (c->tfsf) {
+        f.ti0 = c->tf_lo[0]; f.tj0 = c->tf_lo[1]; f.tk0 = c->tf_lo[2];
+        f.ti1 = c->tf_hi[0]; f.tj1 = c->tf_hi[1]; f.tk1 = c->tf_hi[2];
+        f.tkb = c->tf_kb; f.ex1 = c->d_tf; f.dt_mu0 = c->d.dt / kMu0;
+        hipLaunchKernelGGL(k_tfsf_1d_h, dim3(blocks_for(c->tf_n, 256)), dim3(256), 0, c->s, c->d_tf,
+                           c->d_tf + c->tf_n, c->tf_n, (c->d.dt / kMu0) * (1.0 / c->d.dr));
+    }
+    const dim3 blk(kLanes, FR);
+    const size_t lds = sizeof(T) * 2 * 4 * FR * kLanes;
+    const bool cp = c->cpml_mask != 0, md = c->media_set && c->n_classes > 1, tf = c->tfsf;
+#define OMX_FUSED(CP, MD, TF) hipLaunchKernelGGL((k_step_fused<T, CP, MD, TF>), dim3(nb), blk, lds, c->s, a, f)
+    if (cp && md && tf) OMX_FUSED(true, true, true);
+    else if (cp && md) OMX_FUSED(true, true, false);
+    else if (cp && tf) OMX_FUSED(true, false, true);
+    else if (cp) OMX_FUSED(true, false, false);
+    else if (md && tf) OMX_FUSED(false, true, true);
+    else if (md) OMX_FUSED(false, true, false);
+    else if (tf) OMX_FUSED(false, false, true);
+    else OMX_FUSED(false, false, false);
+#undef OMX_FUSED
+    HIPCHK(hipGetLastError());
+    for (int q = 0; q < 6; ++q) std::swap(c->f[q], c->f_alt[q]);
+    for (int q = 0; q < 6; ++q) std::swap(c->psi_h[q], c->psi_h_alt[q]);
+    return OMX_OK;
+}
+
 template <typename T>
 int run_impl(omx3d_ctx* c, int tt_first, int n_steps, int record_every) {
     int rc = upload_detectors(c);
@@ -286,23 +389,45 @@ int run_impl(omx3d_ctx* c, int tt_first, int n_steps, int record_every) {
         rc = ensure_events(c, (size_t)4 * n_steps);
         if (rc) return rc;
     }
+    rc = check_sources_dispersive(c);
+    if (rc) return rc;
+    const bool fused = fused_applies(c);
+    c->last_fused = fused;
     for (int tt = tt_first; tt < tt_first + n_steps; ++tt) {
         const double time = dt * ((double)tt - 0.5);
         // n_skip_steps = 1: a new quantum step every step, so t_skip = time
         // (mxim_mxll.f90:73-77,97-101).
         const double t_skip = time;
-        if (c->prof) HIPCHK(hipEventRecord(c->ev[evi++], c->s));
-        rc = launch_h<T>(c);
-        if (rc) return rc;
-        if (c->prof) HIPCHK(hipEventRecord(c->ev[evi++], c->s));
-        rc = launch_tfsf_h<T>(c);
-        if (rc) return rc;
-        rc = launch_sources<T>(c, time);
-        if (rc) return rc;
-        if (c->prof) HIPCHK(hipEventRecord(c->ev[evi++], c->s));
-        rc = launch_e<T>(c, time, t_skip);
-        if (rc) return rc;
-        if (c->prof) HIPCHK(hipEventRecord(c->ev[evi++], c->s));
+        if (fused) {
+            // H and E in one pass; sources then enter E after its update
+            // (same value as before it in vacuum/dielectric source cells).
+            // Profiling: an empty H slot, the fused kernel in the E slot.
+            if (c->prof) {
+                HIPCHK(hipEventRecord(c->ev[evi++], c->s));
+                HIPCHK(hipEventRecord(c->ev[evi++], c->s));
+                HIPCHK(hipEventRecord(c->ev[evi++], c->s));
+            }
+            rc = launch_fused<T>(c);
+            if (rc) return rc;
+            if (c->prof) HIPCHK(hipEventRecord(c->ev[evi++], c->s));
+            rc = launch_sources<T>(c, time);
+            if (rc) return rc;
+            rc = launch_j_correct<T>(c, time - t_skip);
+            if (rc) return rc;
+        } else {
+            if (c->prof) HIPCHK(hipEventRecord(c->ev[evi++], c->s));
+            rc = launch_h<T>(c);
+            if (rc) return rc;
+            if (c->prof) HIPCHK(hipEventRecord(c->ev[evi++], c->s));
+            rc = launch_tfsf_h<T>(c);
+            if (rc) return rc;
+            rc = launch_sources<T>(c, time);
+            if (rc) return rc;
+            if (c->prof) HIPCHK(hipEventRecord(c->ev[evi++], c->s));
+            rc = launch_e<T>(c, time, t_skip);
+            if (rc) return rc;
+            if (c->prof) HIPCHK(hipEventRecord(c->ev[evi++], c->s));
+        }
         rc = launch_tfsf_e<T>(c, dt * (double)tt);
         if (rc) return rc;
         rc = launch_two_level<T>(c);
@@ -383,6 +508,8 @@ int copy_field_impl(omx3d_ctx* c, int field, double* host, int to_device) {
             }
     if (to_device) {
         HIPCHK(hipMemcpyAsync(dptr, stage.data(), sizeof(T) * L.nalloc, hipMemcpyHostToDevice, c->s));
+        if (c->f_alt[field - 1])   // the fused step's second buffer holds the same boundary values
+            HIPCHK(hipMemcpyAsync(c->f_alt[field - 1], stage.data(), sizeof(T) * L.nalloc, hipMemcpyHostToDevice, c->s));
         HIPCHK(hipStreamSynchronize(c->s));
     }
     return OMX_OK;
@@ -877,6 +1004,8 @@ int omx3d_destroy(omx3d_ctx* c) {
     if (!c) return OMX_OK;
     if (c->s) (void)hipStreamSynchronize(c->s);
     for (auto& p : c->f) dfree(p);
+    for (auto& p : c->f_alt) dfree(p);
+    for (auto& p : c->psi_h_alt) dfree(p);
     dfreet(c->dcoef); dfree(c->tcoef);
     for (auto& p : c->psi_h) dfree(p);
     for (auto& p : c->psi_e) dfree(p);
@@ -1340,6 +1469,23 @@ int omx3d_run(omx3d_ctx* c, int32_t tt_first, int32_t n_steps, int32_t record_ev
                         : run_impl<double>(c, tt_first, n_steps, record_every);
 }
 
+int omx3d_set_step_mode(omx3d_ctx* c, int32_t mode, int32_t rows, int32_t xchunk) {
+    CHECK_CTX(c);
+    if (mode != 0 && mode != 1) return fail(OMX_ERR_ARG, "step mode must be 0 (fused where it applies) or 1");
+    if (rows != 0 && (rows < 2 || rows > 16)) return fail(OMX_ERR_ARG, "fused rows must be in 2..16");
+    if (xchunk < 0) return fail(OMX_ERR_ARG, "xchunk must be >= 0");
+    c->step_mode = mode;
+    if (rows) c->fused_rows = rows;
+    if (xchunk) c->fused_xchunk = xchunk;
+    return OMX_OK;
+}
+
+int omx3d_get_step_mode(omx3d_ctx* c, int32_t* fused_last_run) {
+    CHECK_CTX(c);
+    if (fused_last_run) *fused_last_run = c->last_fused ? 1 : 0;
+    return OMX_OK;
+}
+
 int omx3d_set_profiling(omx3d_ctx* c, int32_t enable) {
     CHECK_CTX(c);
     c->prof = enable != 0;
@@ -1454,6 +1600,7 @@ int omx3d_autotune(omx3d_ctx* c, int32_t steps, int32_t* rows_y, int32_t* xchunk
     // back to the initial state: zero fields, psi and dispersive state
     const size_t fb = c->esz * c->L.nalloc;
     for (auto p : c->f) HIPCHK(hipMemsetAsync(p, 0, fb, c->s));
+    for (auto p : c->f_alt) if (p) HIPCHK(hipMemsetAsync(p, 0, fb, c->s));
     const size_t A = 128 / c->esz;
     const size_t xs[2] = {c->nsh * c->L.sx + A, c->nse * c->L.sx + A};
     const size_t ys[2] = {(size_t)(c->d.nx + 2) * c->nsh * c->L.sz + A, (size_t)(c->d.nx + 2) * c->nse * c->L.sz + A};

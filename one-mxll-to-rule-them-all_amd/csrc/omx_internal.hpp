@@ -101,7 +101,14 @@ struct omx3d_ctx {
     hipStream_t s = nullptr;
     int rows_y = 4, xchunk = 16;
 
-    void* f[6] = {};    // Ex Ey Ez Hx Hy Hz
+    void* f[6] = {};    // Ex Ey Ez Hx Hy Hz (current)
+    void* f_alt[6] = {};        // second buffers of the fused step
+    void* psi_h_alt[6] = {};
+    int step_mode = 0;          // 0: fused step where it applies, 1: separate H and E kernels
+    int fused_rows = 8;         // rows per workgroup of the fused step (incl. the halo row)
+    int fused_xchunk = 8;
+    int src_in_dispersive = -1; // -1 unknown, 0 no, 1 a source point sits in a Drude/DL component
+    bool last_fused = false;
     // FP64 master coefficients: be ce (npml+1) bh ch (npml) den_e{x,y,z} den_h{x,y,z} (n+2 each)
     double* dcoef = nullptr;
     void* tcoef = nullptr;          // precision-T copies used by the kernels

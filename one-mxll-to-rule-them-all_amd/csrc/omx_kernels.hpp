@@ -292,6 +292,87 @@ __device__ __forceinline__ T psi_step(T psi, T b, T c, T d) {
     else return b * psi + c * d;
 }
 
+// H of one cell, in place in hx/hy/hz (H^{n-1/2} in, H^{n+1/2} out): bulk
+// curl of E for i,j,k in 1..n, then the CPML psi recursions of the faces the
+// cell lies in, applied per component in the reference's face order.  psi is
+// read from a.ph and written to pho (nullptr: computed but not stored).
+template <typename T, bool CPML>
+__device__ __forceinline__ void h_cell(const YeeArgs<T>& a, T* const* pho, int i, int j, int k, long jk,
+                                       T ezc, T ezn, T ezj, T eyc, T eyn, T eyk, T exc, T exk, T exj,
+                                       T dhy, T dhz, T chy, T chz, T& hx, T& hy, T& hz) {
+    const T dmu = a.dt_mu0;
+    if constexpr (sizeof(T) == 4) {
+        // FP32: dt_mu0*den folded into one coefficient per axis (one rounding fewer)
+        const T chx = a.cdhx[i];
+        hx = hx + (ezc - ezj) * chy + (eyk - eyc) * chz;
+        hy = hy + (ezn - ezc) * chx + (exc - exk) * chz;
+        hz = hz + (eyc - eyn) * chx + (exj - exc) * chy;
+    } else {
+        const T dhx = a.den_hx[i];
+        hx = hx + dmu * ((ezc - ezj) * dhy + (eyk - eyc) * dhz);
+        hy = hy + dmu * ((ezn - ezc) * dhx + (exc - exk) * dhz);
+        hz = hz + dmu * ((eyc - eyn) * dhx + (exj - exc) * dhy);
+    }
+    if constexpr (CPML) {
+        const unsigned cp = a.cpml;
+        const int npml = a.npml;
+        const int nsh = 2 * (npml - 1);
+        const long sx = a.sx;
+        const int sz = a.sz;
+        // x faces: psix_Hyx, psix_Hzx
+        if ((cp & 1u) && i <= npml - 1) {
+            const int c = i;
+            const long ps = (long)(i - 1) * sx + jk;
+            const T p0 = psi_step(a.ph[0][ps], a.bh[c], a.chd[c], ezn - ezc);
+            const T p1 = psi_step(a.ph[1][ps], a.bh[c], a.chd[c], eyc - eyn);
+            if (pho) { pho[0][ps] = p0; pho[1][ps] = p1; }
+            hy = hy + dmu * p0; hz = hz + dmu * p1;
+        }
+        if ((cp & 2u) && i >= a.nx + 1 - npml && i <= a.nx - 1) {
+            const int c = a.nx - i;
+            const long ps = (long)(npml - 2 + c) * sx + jk;
+            const T p0 = psi_step(a.ph[0][ps], a.bh[c], a.chd[c], ezn - ezc);
+            const T p1 = psi_step(a.ph[1][ps], a.bh[c], a.chd[c], eyc - eyn);
+            if (pho) { pho[0][ps] = p0; pho[1][ps] = p1; }
+            hy = hy + dmu * p0; hz = hz + dmu * p1;
+        }
+        // y faces: psiy_Hxy, psiy_Hzy
+        if ((cp & 4u) && j <= npml - 1) {
+            const int c = j;
+            const long ps = ((long)i * nsh + (j - 1)) * sz + k + a.off;
+            const T p0 = psi_step(a.ph[2][ps], a.bh[c], a.chd[c], ezc - ezj);
+            const T p1 = psi_step(a.ph[3][ps], a.bh[c], a.chd[c], exj - exc);
+            if (pho) { pho[2][ps] = p0; pho[3][ps] = p1; }
+            hx = hx + dmu * p0; hz = hz + dmu * p1;
+        }
+        if ((cp & 8u) && j >= a.ny + 1 - npml && j <= a.ny - 1) {
+            const int c = a.ny - j;
+            const long ps = ((long)i * nsh + (npml - 2 + c)) * sz + k + a.off;
+            const T p0 = psi_step(a.ph[2][ps], a.bh[c], a.chd[c], ezc - ezj);
+            const T p1 = psi_step(a.ph[3][ps], a.bh[c], a.chd[c], exj - exc);
+            if (pho) { pho[2][ps] = p0; pho[3][ps] = p1; }
+            hx = hx + dmu * p0; hz = hz + dmu * p1;
+        }
+        // z faces: psiz_Hxz, psiz_Hyz
+        if ((cp & 16u) && k <= npml - 1) {
+            const int c = k;
+            const long ps = ((long)i * a.sy + j) * nsh + (k - 1);
+            const T p0 = psi_step(a.ph[4][ps], a.bh[c], a.chd[c], eyk - eyc);
+            const T p1 = psi_step(a.ph[5][ps], a.bh[c], a.chd[c], exc - exk);
+            if (pho) { pho[4][ps] = p0; pho[5][ps] = p1; }
+            hx = hx + dmu * p0; hy = hy + dmu * p1;
+        }
+        if ((cp & 32u) && k >= a.nz + 1 - npml && k <= a.nz - 1) {
+            const int c = a.nz - k;
+            const long ps = ((long)i * a.sy + j) * nsh + (npml - 2 + c);
+            const T p0 = psi_step(a.ph[4][ps], a.bh[c], a.chd[c], eyk - eyc);
+            const T p1 = psi_step(a.ph[5][ps], a.bh[c], a.chd[c], exc - exk);
+            if (pho) { pho[4][ps] = p0; pho[5][ps] = p1; }
+            hx = hx + dmu * p0; hy = hy + dmu * p1;
+        }
+    }
+}
+
 template <typename T, bool CPML>
 __device__ __forceinline__ void h_tile(const YeeArgs<T>& a, const int t) {
     const int tz = t % a.ntz;
@@ -307,83 +388,16 @@ __device__ __forceinline__ void h_tile(const YeeArgs<T>& a, const int t) {
     const int sz = a.sz;
     const long jk = (long)j * sz + k + a.off;
     long o = (long)i0 * sx + jk;
-    const T dmu = a.dt_mu0;
     const T dhy = a.den_hy[j], dhz = a.den_hz[k];
     const T chy = sizeof(T) == 4 ? a.cdhy[j] : (T)0, chz = sizeof(T) == 4 ? a.cdhz[k] : (T)0;
     T ezc = a.ez[o], eyc = a.ey[o];
-    const int npml = a.npml;
     for (int i = i0; i <= i1; ++i, o += sx) {
         const T ezn = a.ez[o + sx], eyn = a.ey[o + sx];
         const T ezj = a.ez[o + sz], eyk = a.ey[o + 1];
         const T exc = a.ex[o], exk = a.ex[o + 1], exj = a.ex[o + sz];
-        T hx, hy, hz;
-        if constexpr (sizeof(T) == 4) {
-            // FP32: dt_mu0*den folded into one coefficient per axis (one rounding fewer)
-            const T chx = a.cdhx[i];
-            hx = a.hx[o] + (ezc - ezj) * chy + (eyk - eyc) * chz;
-            hy = a.hy[o] + (ezn - ezc) * chx + (exc - exk) * chz;
-            hz = a.hz[o] + (eyc - eyn) * chx + (exj - exc) * chy;
-        } else {
-            const T dhx = a.den_hx[i];
-            hx = a.hx[o] + dmu * ((ezc - ezj) * dhy + (eyk - eyc) * dhz);
-            hy = a.hy[o] + dmu * ((ezn - ezc) * dhx + (exc - exk) * dhz);
-            hz = a.hz[o] + dmu * ((eyc - eyn) * dhx + (exj - exc) * dhy);
-        }
-        if constexpr (CPML) {
-            const unsigned cp = a.cpml;
-            const int nsh = 2 * (npml - 1);
-            // x faces: psix_Hyx, psix_Hzx
-            if ((cp & 1u) && i <= npml - 1) {
-                const int c = i;
-                const long ps = (long)(i - 1) * sx + jk;
-                T p0 = psi_step(a.ph[0][ps], a.bh[c], a.chd[c], ezn - ezc);
-                T p1 = psi_step(a.ph[1][ps], a.bh[c], a.chd[c], eyc - eyn);
-                a.ph[0][ps] = p0; a.ph[1][ps] = p1;
-                hy = hy + dmu * p0; hz = hz + dmu * p1;
-            }
-            if ((cp & 2u) && i >= a.nx + 1 - npml && i <= a.nx - 1) {
-                const int c = a.nx - i;
-                const long ps = (long)(npml - 2 + c) * sx + jk;
-                T p0 = psi_step(a.ph[0][ps], a.bh[c], a.chd[c], ezn - ezc);
-                T p1 = psi_step(a.ph[1][ps], a.bh[c], a.chd[c], eyc - eyn);
-                a.ph[0][ps] = p0; a.ph[1][ps] = p1;
-                hy = hy + dmu * p0; hz = hz + dmu * p1;
-            }
-            // y faces: psiy_Hxy, psiy_Hzy
-            if ((cp & 4u) && j <= npml - 1) {
-                const int c = j;
-                const long ps = ((long)i * nsh + (j - 1)) * sz + k + a.off;
-                T p0 = psi_step(a.ph[2][ps], a.bh[c], a.chd[c], ezc - ezj);
-                T p1 = psi_step(a.ph[3][ps], a.bh[c], a.chd[c], exj - exc);
-                a.ph[2][ps] = p0; a.ph[3][ps] = p1;
-                hx = hx + dmu * p0; hz = hz + dmu * p1;
-            }
-            if ((cp & 8u) && j >= a.ny + 1 - npml && j <= a.ny - 1) {
-                const int c = a.ny - j;
-                const long ps = ((long)i * nsh + (npml - 2 + c)) * sz + k + a.off;
-                T p0 = psi_step(a.ph[2][ps], a.bh[c], a.chd[c], ezc - ezj);
-                T p1 = psi_step(a.ph[3][ps], a.bh[c], a.chd[c], exj - exc);
-                a.ph[2][ps] = p0; a.ph[3][ps] = p1;
-                hx = hx + dmu * p0; hz = hz + dmu * p1;
-            }
-            // z faces: psiz_Hxz, psiz_Hyz
-            if ((cp & 16u) && k <= npml - 1) {
-                const int c = k;
-                const long ps = ((long)i * a.sy + j) * nsh + (k - 1);
-                T p0 = psi_step(a.ph[4][ps], a.bh[c], a.chd[c], eyk - eyc);
-                T p1 = psi_step(a.ph[5][ps], a.bh[c], a.chd[c], exc - exk);
-                a.ph[4][ps] = p0; a.ph[5][ps] = p1;
-                hx = hx + dmu * p0; hy = hy + dmu * p1;
-            }
-            if ((cp & 32u) && k >= a.nz + 1 - npml && k <= a.nz - 1) {
-                const int c = a.nz - k;
-                const long ps = ((long)i * a.sy + j) * nsh + (npml - 2 + c);
-                T p0 = psi_step(a.ph[4][ps], a.bh[c], a.chd[c], eyk - eyc);
-                T p1 = psi_step(a.ph[5][ps], a.bh[c], a.chd[c], exc - exk);
-                a.ph[4][ps] = p0; a.ph[5][ps] = p1;
-                hx = hx + dmu * p0; hy = hy + dmu * p1;
-            }
-        }
+        T hx = a.hx[o], hy = a.hy[o], hz = a.hz[o];
+        h_cell<T, CPML>(a, a.ph, i, j, k, jk, ezc, ezn, ezj, eyc, eyn, eyk, exc, exk, exj, dhy, dhz, chy, chz,
+                        hx, hy, hz);
         a.hx[o] = hx; a.hy[o] = hy; a.hz[o] = hz;
         ezc = ezn; eyc = eyn;
     }
@@ -442,6 +456,100 @@ __device__ __forceinline__ T medium_update(T e, T rot, T vac, int m, const MedDe
     return e + M.cdt * rot;  // dielectric: dt_eps0 * (1/eps_r) * rotH
 }
 
+// E of one cell, in place in ex/ey/ez: bulk curl of H with the medium
+// update, then the CPML psi recursions (psi of E in place in a.pe).
+template <typename T, bool CPML, bool MEDIA>
+__device__ __forceinline__ void e_cell(const YeeArgs<T>& a, int i, int j, int k, long o, long jk,
+                                       T hzc, T hzj, T hzi, T hyc, T hyk, T hyi, T hxc, T hxj, T hxk,
+                                       T dey, T dez, T cey, T cez, T& ex, T& ey, T& ez) {
+    const T dep = a.dt_eps0;
+    const T dex = a.den_ex[i];
+    const T rx = (hzc - hzj) * dey + (hyk - hyc) * dez;
+    const T ry = (hzi - hzc) * dex + (hxc - hxk) * dez;
+    const T rz = (hyc - hyi) * dex + (hxj - hxc) * dey;
+    // vacuum increments; FP32 folds dt_eps0*den into one coefficient per axis
+    T vx, vy, vz;
+    if constexpr (sizeof(T) == 4) {
+        const T cex = a.cdex[i];
+        vx = (hzc - hzj) * cey + (hyk - hyc) * cez;
+        vy = (hzi - hzc) * cex + (hxc - hxk) * cez;
+        vz = (hyc - hyi) * cex + (hxj - hxc) * cey;
+    } else {
+        vx = dep * rx; vy = dep * ry; vz = dep * rz;
+    }
+    if (MEDIA && a.tag != nullptr) {
+        const int tg = a.tag[o];
+        if (tg == 0) {
+            ex = ex + vx; ey = ey + vy; ez = ez + vz;
+        } else {
+            const int8_t* c3 = a.cls + 3 * tg;
+            ex = medium_update(ex, rx, vx, (int)c3[0], a.med, o, a.eold[0], a.pd[0], a.pl[0], a.plold[0], a.pstride);
+            ey = medium_update(ey, ry, vy, (int)c3[1], a.med, o, a.eold[1], a.pd[1], a.pl[1], a.plold[1], a.pstride);
+            ez = medium_update(ez, rz, vz, (int)c3[2], a.med, o, a.eold[2], a.pd[2], a.pl[2], a.plold[2], a.pstride);
+        }
+    } else {
+        ex = ex + vx; ey = ey + vy; ez = ez + vz;
+    }
+    if constexpr (CPML) {
+        const unsigned cp = a.cpml;
+        const int npml = a.npml;
+        const int nse = 2 * npml;
+        const long sx = a.sx;
+        const int sz = a.sz;
+        // x faces: psix_Eyx, psix_Ezx (low side i = 2..npml)
+        if ((cp & 1u) && i >= 2 && i <= npml) {
+            const int c = i;
+            const long ps = (long)(i - 1) * sx + jk;
+            const T p0 = psi_step(a.pe[0][ps], a.be[c], a.ced[c], hzi - hzc);
+            const T p1 = psi_step(a.pe[1][ps], a.be[c], a.ced[c], hyc - hyi);
+            a.pe[0][ps] = p0; a.pe[1][ps] = p1;
+            ey = ey + dep * p0; ez = ez + dep * p1;
+        }
+        if ((cp & 2u) && i >= a.nx + 1 - npml && i <= a.nx - 1) {
+            const int c = a.nx + 1 - i;
+            const long ps = (long)(npml - 1 + c) * sx + jk;
+            const T p0 = psi_step(a.pe[0][ps], a.be[c], a.ced[c], hzi - hzc);
+            const T p1 = psi_step(a.pe[1][ps], a.be[c], a.ced[c], hyc - hyi);
+            a.pe[0][ps] = p0; a.pe[1][ps] = p1;
+            ey = ey + dep * p0; ez = ez + dep * p1;
+        }
+        // y faces: psiy_Exy, psiy_Ezy
+        if ((cp & 4u) && j >= 2 && j <= npml) {
+            const int c = j;
+            const long ps = ((long)i * nse + (j - 1)) * sz + k + a.off;
+            const T p0 = psi_step(a.pe[2][ps], a.be[c], a.ced[c], hzc - hzj);
+            const T p1 = psi_step(a.pe[3][ps], a.be[c], a.ced[c], hxj - hxc);
+            a.pe[2][ps] = p0; a.pe[3][ps] = p1;
+            ex = ex + dep * p0; ez = ez + dep * p1;
+        }
+        if ((cp & 8u) && j >= a.ny + 1 - npml && j <= a.ny - 1) {
+            const int c = a.ny + 1 - j;
+            const long ps = ((long)i * nse + (npml - 1 + c)) * sz + k + a.off;
+            const T p0 = psi_step(a.pe[2][ps], a.be[c], a.ced[c], hzc - hzj);
+            const T p1 = psi_step(a.pe[3][ps], a.be[c], a.ced[c], hxj - hxc);
+            a.pe[2][ps] = p0; a.pe[3][ps] = p1;
+            ex = ex + dep * p0; ez = ez + dep * p1;
+        }
+        // z faces: psiz_Exz, psiz_Eyz
+        if ((cp & 16u) && k >= 2 && k <= npml) {
+            const int c = k;
+            const long ps = ((long)i * a.sy + j) * nse + (k - 1);
+            const T p0 = psi_step(a.pe[4][ps], a.be[c], a.ced[c], hyk - hyc);
+            const T p1 = psi_step(a.pe[5][ps], a.be[c], a.ced[c], hxc - hxk);
+            a.pe[4][ps] = p0; a.pe[5][ps] = p1;
+            ex = ex + dep * p0; ey = ey + dep * p1;
+        }
+        if ((cp & 32u) && k >= a.nz + 1 - npml && k <= a.nz - 1) {
+            const int c = a.nz + 1 - k;
+            const long ps = ((long)i * a.sy + j) * nse + (npml - 1 + c);
+            const T p0 = psi_step(a.pe[4][ps], a.be[c], a.ced[c], hyk - hyc);
+            const T p1 = psi_step(a.pe[5][ps], a.be[c], a.ced[c], hxc - hxk);
+            a.pe[4][ps] = p0; a.pe[5][ps] = p1;
+            ex = ex + dep * p0; ey = ey + dep * p1;
+        }
+    }
+}
+
 template <typename T, bool CPML, bool MEDIA>
 __device__ __forceinline__ void e_tile(const YeeArgs<T>& a, const int t) {
     const int tz = t % a.ntz;
@@ -457,98 +565,16 @@ __device__ __forceinline__ void e_tile(const YeeArgs<T>& a, const int t) {
     const int sz = a.sz;
     const long jk = (long)j * sz + k + a.off;
     long o = (long)i0 * sx + jk;
-    const T dep = a.dt_eps0;
     const T dey = a.den_ey[j], dez = a.den_ez[k];
     const T cey = sizeof(T) == 4 ? a.cdey[j] : (T)0, cez = sizeof(T) == 4 ? a.cdez[k] : (T)0;
     T hzi = a.hz[o - sx], hyi = a.hy[o - sx];
-    const int npml = a.npml;
     for (int i = i0; i <= i1; ++i, o += sx) {
         const T hzc = a.hz[o], hyc = a.hy[o], hxc = a.hx[o];
         const T hzj = a.hz[o - sz], hxj = a.hx[o - sz];
         const T hyk = a.hy[o - 1], hxk = a.hx[o - 1];
-        const T dex = a.den_ex[i];
-        const T rx = (hzc - hzj) * dey + (hyk - hyc) * dez;
-        const T ry = (hzi - hzc) * dex + (hxc - hxk) * dez;
-        const T rz = (hyc - hyi) * dex + (hxj - hxc) * dey;
-        // vacuum increments; FP32 folds dt_eps0*den into one coefficient per axis
-        T vx, vy, vz;
-        if constexpr (sizeof(T) == 4) {
-            const T cex = a.cdex[i];
-            vx = (hzc - hzj) * cey + (hyk - hyc) * cez;
-            vy = (hzi - hzc) * cex + (hxc - hxk) * cez;
-            vz = (hyc - hyi) * cex + (hxj - hxc) * cey;
-        } else {
-            vx = dep * rx; vy = dep * ry; vz = dep * rz;
-        }
         T ex = a.ex[o], ey = a.ey[o], ez = a.ez[o];
-        if (MEDIA && a.tag != nullptr) {
-            const int tg = a.tag[o];
-            if (tg == 0) {
-                ex = ex + vx; ey = ey + vy; ez = ez + vz;
-            } else {
-                const int8_t* c3 = a.cls + 3 * tg;
-                ex = medium_update(ex, rx, vx, (int)c3[0], a.med, o, a.eold[0], a.pd[0], a.pl[0], a.plold[0], a.pstride);
-                ey = medium_update(ey, ry, vy, (int)c3[1], a.med, o, a.eold[1], a.pd[1], a.pl[1], a.plold[1], a.pstride);
-                ez = medium_update(ez, rz, vz, (int)c3[2], a.med, o, a.eold[2], a.pd[2], a.pl[2], a.plold[2], a.pstride);
-            }
-        } else {
-            ex = ex + vx; ey = ey + vy; ez = ez + vz;
-        }
-        if constexpr (CPML) {
-            const unsigned cp = a.cpml;
-            const int nse = 2 * npml;
-            // x faces: psix_Eyx, psix_Ezx (low side i = 2..npml)
-            if ((cp & 1u) && i >= 2 && i <= npml) {
-                const int c = i;
-                const long ps = (long)(i - 1) * sx + jk;
-                T p0 = psi_step(a.pe[0][ps], a.be[c], a.ced[c], hzi - hzc);
-                T p1 = psi_step(a.pe[1][ps], a.be[c], a.ced[c], hyc - hyi);
-                a.pe[0][ps] = p0; a.pe[1][ps] = p1;
-                ey = ey + dep * p0; ez = ez + dep * p1;
-            }
-            if ((cp & 2u) && i >= a.nx + 1 - npml && i <= a.nx - 1) {
-                const int c = a.nx + 1 - i;
-                const long ps = (long)(npml - 1 + c) * sx + jk;
-                T p0 = psi_step(a.pe[0][ps], a.be[c], a.ced[c], hzi - hzc);
-                T p1 = psi_step(a.pe[1][ps], a.be[c], a.ced[c], hyc - hyi);
-                a.pe[0][ps] = p0; a.pe[1][ps] = p1;
-                ey = ey + dep * p0; ez = ez + dep * p1;
-            }
-            // y faces: psiy_Exy, psiy_Ezy
-            if ((cp & 4u) && j >= 2 && j <= npml) {
-                const int c = j;
-                const long ps = ((long)i * nse + (j - 1)) * sz + k + a.off;
-                T p0 = psi_step(a.pe[2][ps], a.be[c], a.ced[c], hzc - hzj);
-                T p1 = psi_step(a.pe[3][ps], a.be[c], a.ced[c], hxj - hxc);
-                a.pe[2][ps] = p0; a.pe[3][ps] = p1;
-                ex = ex + dep * p0; ez = ez + dep * p1;
-            }
-            if ((cp & 8u) && j >= a.ny + 1 - npml && j <= a.ny - 1) {
-                const int c = a.ny + 1 - j;
-                const long ps = ((long)i * nse + (npml - 1 + c)) * sz + k + a.off;
-                T p0 = psi_step(a.pe[2][ps], a.be[c], a.ced[c], hzc - hzj);
-                T p1 = psi_step(a.pe[3][ps], a.be[c], a.ced[c], hxj - hxc);
-                a.pe[2][ps] = p0; a.pe[3][ps] = p1;
-                ex = ex + dep * p0; ez = ez + dep * p1;
-            }
-            // z faces: psiz_Exz, psiz_Eyz
-            if ((cp & 16u) && k >= 2 && k <= npml) {
-                const int c = k;
-                const long ps = ((long)i * a.sy + j) * nse + (k - 1);
-                T p0 = psi_step(a.pe[4][ps], a.be[c], a.ced[c], hyk - hyc);
-                T p1 = psi_step(a.pe[5][ps], a.be[c], a.ced[c], hxc - hxk);
-                a.pe[4][ps] = p0; a.pe[5][ps] = p1;
-                ex = ex + dep * p0; ey = ey + dep * p1;
-            }
-            if ((cp & 32u) && k >= a.nz + 1 - npml && k <= a.nz - 1) {
-                const int c = a.nz + 1 - k;
-                const long ps = ((long)i * a.sy + j) * nse + (npml - 1 + c);
-                T p0 = psi_step(a.pe[4][ps], a.be[c], a.ced[c], hyk - hyc);
-                T p1 = psi_step(a.pe[5][ps], a.be[c], a.ced[c], hxc - hxk);
-                a.pe[4][ps] = p0; a.pe[5][ps] = p1;
-                ex = ex + dep * p0; ey = ey + dep * p1;
-            }
-        }
+        e_cell<T, CPML, MEDIA>(a, i, j, k, o, jk, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dey, dez, cey, cez,
+                               ex, ey, ez);
         a.ex[o] = ex; a.ey[o] = ey; a.ez[o] = ez;
         hzi = hzc; hyi = hyc;
     }
@@ -562,6 +588,120 @@ __global__ __launch_bounds__(256) void k_h_update(const YeeArgs<T> a) {
 template <typename T, bool CPML, bool MEDIA>
 __global__ __launch_bounds__(256) void k_e_update(const YeeArgs<T> a) {
     e_tile<T, CPML, MEDIA>(a, xcd_tile(blockIdx.x, gridDim.x));
+}
+
+// ---------------------------------------------------------------------------
+// Fused step: H^{n+1/2} and E^{n+1} in one pass over HBM.  E and H are read
+// from the current buffers and written to the other pair, so every array is
+// read once and written once per step (12 words per cell instead of the 18
+// of separate H and E passes).  A workgroup is 64 lanes along z times FR
+// rows along y and marches an x chunk.  Lane 0 and row 0 are halos: there H
+// is recomputed (from the unmodified input buffers) so that the E update of
+// the owned 63 x (FR-1) cells has all its H neighbours; H rows travel through
+// LDS, H lanes through cross-lane shuffles, the previous x plane in
+// registers.  The psi of H is double-buffered like the fields (halo cells read
+// it); psi of E and the dispersive state are touched only by their owner.
+// The TF/SF corrections of H are applied inside, where E needs them.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct FusedArgs {
+    T *exo, *eyo, *ezo, *hxo, *hyo, *hzo;   // output buffers
+    T *pho[6];                              // output psi of H
+    int ntz, nty;                           // tiles: 63 lanes x (FR-1) rows
+    int ti0, ti1, tj0, tj1, tk0, tk1, tkb;  // TF/SF box and 1D line origin
+    const double* ex1;                      // incident Ex on the 1D line
+    double dt_mu0;
+};
+
+template <typename T, bool CPML, bool MEDIA, bool TFSF>
+__global__ __launch_bounds__(1024) void k_step_fused(const YeeArgs<T> a, const FusedArgs<T> f) {
+    extern __shared__ unsigned char omx_lds[];
+    T* lds = reinterpret_cast<T*>(omx_lds);
+    const int FR = blockDim.y;
+    const int lane = threadIdx.x, row = threadIdx.y;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int tz = t % f.ntz;
+    const int q = t / f.ntz;
+    const int ty = q % f.nty;
+    const int tx = q / f.nty;
+    const int k = tz * (kLanes - 1) + lane;        // lane 0: halo at k0-1
+    const int j = ty * (FR - 1) + row;             // row 0: halo at j0-1
+    const int i0 = tx * a.xchunk + 1;
+    const int i1 = min(i0 + a.xchunk - 1, a.nx);
+    const bool inb = k <= a.nz + 1 && j <= a.ny + 1;
+    const bool hcell = k >= 1 && k <= a.nz && j >= 1 && j <= a.ny;
+    const bool own = hcell && lane >= 1 && row >= 1;
+    const long sx = a.sx;
+    const int sz = a.sz;
+    const long jk = (long)j * sz + k + a.off;
+    long o = (long)(i0 - 1) * sx + jk;
+    const int jc = min(j, a.ny + 1), kc = min(k, a.nz + 1);
+    const T dhy = a.den_hy[jc], dhz = a.den_hz[kc], dey = a.den_ey[jc], dez = a.den_ez[kc];
+    const T chy = sizeof(T) == 4 ? a.cdhy[jc] : (T)0, chz = sizeof(T) == 4 ? a.cdhz[kc] : (T)0;
+    const T cey = sizeof(T) == 4 ? a.cdey[jc] : (T)0, cez = sizeof(T) == 4 ? a.cdez[kc] : (T)0;
+    const int plane = FR * kLanes;
+    const int me = row * kLanes + lane;
+    T ezc = inb ? a.ez[o] : (T)0, eyc = inb ? a.ey[o] : (T)0;
+    T hzi = (T)0, hyi = (T)0;
+    for (int i = i0 - 1; i <= i1; ++i, o += sx) {
+        T* L = lds + (size_t)(i & 1) * 4 * plane;
+        const T exc = inb ? a.ex[o] : (T)0;
+        const T ezn = inb ? a.ez[o + sx] : (T)0, eyn = inb ? a.ey[o + sx] : (T)0;
+        T hx = inb ? a.hx[o] : (T)0, hy = inb ? a.hy[o] : (T)0, hz = inb ? a.hz[o] : (T)0;
+        L[me] = ezc;
+        L[plane + me] = exc;
+        __syncthreads();
+        T ezj, exj;
+        if (row + 1 < FR) {
+            ezj = L[me + kLanes];
+            exj = L[plane + me + kLanes];
+        } else {
+            const bool ok = k <= a.nz + 1 && j + 1 <= a.ny + 1;
+            ezj = ok ? a.ez[o + sz] : (T)0;
+            exj = ok ? a.ex[o + sz] : (T)0;
+        }
+        T eyk = __shfl_down(eyc, 1), exk = __shfl_down(exc, 1);
+        if (lane == kLanes - 1) {
+            const bool ok = k + 1 <= a.nz + 1 && j <= a.ny + 1;
+            eyk = ok ? a.ey[o + 1] : (T)0;
+            exk = ok ? a.ex[o + 1] : (T)0;
+        }
+        const bool store = own && i >= i0;
+        if (hcell && i >= 1) {
+            h_cell<T, CPML>(a, store ? f.pho : nullptr, i, j, k, jk, ezc, ezn, ezj, eyc, eyn, eyk, exc, exk, exj,
+                            dhy, dhz, chy, chz, hx, hy, hz);
+            if constexpr (TFSF) {
+                if (i >= f.ti0 && i <= f.ti1 - 1) {
+                    if (j >= f.tj0 && j <= f.tj1) {
+                        if (k == f.tk0 - 1)
+                            hy = (T)((double)hy + f.dt_mu0 * (double)a.den_hz[k] * f.ex1[f.tk0 - f.tkb]);
+                        else if (k == f.tk1)
+                            hy = (T)((double)hy - f.dt_mu0 * (double)a.den_hz[k] * f.ex1[f.tk1 - f.tkb]);
+                    }
+                    if (k >= f.tk0 && k <= f.tk1) {
+                        if (j == f.tj0 - 1)
+                            hz = (T)((double)hz - f.dt_mu0 * (double)a.den_hy[j] * f.ex1[k - f.tkb]);
+                        else if (j == f.tj1)
+                            hz = (T)((double)hz + f.dt_mu0 * (double)a.den_hy[j] * f.ex1[k - f.tkb]);
+                    }
+                }
+            }
+        }
+        if (store) { f.hxo[o] = hx; f.hyo[o] = hy; f.hzo[o] = hz; }
+        L[2 * plane + me] = hz;
+        L[3 * plane + me] = hx;
+        __syncthreads();
+        const T hyk = __shfl_up(hy, 1), hxk = __shfl_up(hx, 1);
+        if (store) {
+            const T hzj = L[2 * plane + me - kLanes], hxj = L[3 * plane + me - kLanes];
+            T ex = exc, ey = eyc, ez = ezc;
+            e_cell<T, CPML, MEDIA>(a, i, j, k, o, jk, hz, hzj, hzi, hy, hyk, hyi, hx, hxj, hxk, dey, dez, cey, cez,
+                                   ex, ey, ez);
+            f.exo[o] = ex; f.eyo[o] = ey; f.ezo[o] = ez;
+        }
+        hzi = hz; hyi = hy;
+        ezc = ezn; eyc = eyn;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1029,6 +1169,11 @@ __global__ void k_tfsf_e(const YeeArgs<T> a, TfsfArgs t, double* __restrict__ ex
         if (m == t.ks - t.kb) v += src;
         ex1[m] = v;
     }
+}
+
+__global__ void k_tfsf_1d_h(double* __restrict__ ex1, double* __restrict__ hy1, int n1, double c) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < n1 - 1) hy1[m] = hy1[m] + c * (ex1[m] - ex1[m + 1]);
 }
 
 }  // namespace omx

@@ -101,6 +101,15 @@ class Grid3D:
     def set_tiling(self, rows_y=0, xchunk=0):
         L.call("omx3d_set_tiling", self.h, int(rows_y), int(xchunk))
 
+    def set_step_mode(self, mode=0, rows=0, xchunk=0):
+        """mode 0: fused H+E pass where it applies; 1: separate H and E kernels."""
+        L.call("omx3d_set_step_mode", self.h, int(mode), int(rows), int(xchunk))
+
+    def last_run_fused(self):
+        f = C.c_int32()
+        L.call("omx3d_get_step_mode", self.h, C.byref(f))
+        return bool(f.value)
+
     def autotune(self, steps=4):
         """Pick the fastest tile shape; clears the fields (call before a run)."""
         r, x, ms = C.c_int32(), C.c_int32(), C.c_double()

@@ -29,6 +29,7 @@ def test_batch_equals_single_grid_runs(precision, graph):
     singles = []
     for c, t in zip(cs, tilings):
         g = make(c, precision)
+        g.set_step_mode(1)   # the batch runs the separate H and E kernels
         g.run(1, c.steps, c.record_every)
         singles.append(g.read_records())
         g.kill()

@@ -149,18 +149,58 @@ def test_tiling_variants_identical():
     case = small_case("tiling", n=(30, 27, 70), media=[spec], shapes=[(1, 1, [15.0, 13.0, 30.0, 6.0])],
                       steps=40)
     ref = None
+    # separate H/E kernels under several tile shapes
     for rows, xc in ((4, 16), (1, 1), (2, 7), (4, 64)):
         dr, dt = case.dr_dt
         g = omx_fdtd.Grid3D(case.grid, case.npml, case.boundaries, dt, dr, precision=8)
+        g.set_step_mode(1)
         g.set_tiling(rows, xc)
         cases.apply(case, g, library=G.library_medium, make_medium=G.medium)
         g.run(1, case.steps, 1)
+        assert not g.last_run_fused()
         rec = g.read_records()
         if ref is None:
             ref = rec
         else:
             np.testing.assert_array_equal(rec, ref)
         g.kill()
+    # fused H+E step under several tile shapes: identical among themselves,
+    # and equal to the separate kernels up to the order in which the point
+    # source enters E at its own cell
+    fref = None
+    for rows, xc in ((8, 8), (2, 1), (16, 3), (5, 64)):
+        dr, dt = case.dr_dt
+        g = omx_fdtd.Grid3D(case.grid, case.npml, case.boundaries, dt, dr, precision=8)
+        g.set_step_mode(0, rows, xc)
+        cases.apply(case, g, library=G.library_medium, make_medium=G.medium)
+        g.run(1, case.steps, 1)
+        assert g.last_run_fused()
+        rec = g.read_records()
+        if fref is None:
+            fref = rec
+            assert rel_l2(rec, ref) <= 1e-13
+        else:
+            np.testing.assert_array_equal(rec, fref)
+        g.kill()
+
+
+@pytest.mark.parametrize("precision", [8, 4])
+def test_fused_equals_separate_kernels_tfsf(precision):
+    """No point sources: the fused pass and the separate kernels do the same
+    arithmetic in the same order, so the results are bit-identical."""
+    case = cases.config_dl_sphere_tfsf(n=48, npml=6, steps=120, record_every=4, radius_nm=12.0)
+    out = []
+    for mode in (1, 0):
+        dr, dt = case.dr_dt
+        g = omx_fdtd.Grid3D(case.grid, case.npml, case.boundaries, dt, dr, precision=precision)
+        g.set_step_mode(mode)
+        cases.apply(case, g, library=G.library_medium, make_medium=G.medium)
+        g.run(1, case.steps, case.record_every)
+        assert g.last_run_fused() == (mode == 0)
+        out.append(g.read_records())
+        g.kill()
+    assert np.abs(out[0]).max() > 0
+    np.testing.assert_array_equal(out[0], out[1])
 
 
 def test_stepwise_api_with_quantum_sites():
