@@ -636,6 +636,7 @@ int media_finish(omx3d_ctx* c) {
     HIPCHK(hipMemcpyAsync(c->d_kinds, kinds.data(), sizeof(int) * kinds.size(), hipMemcpyHostToDevice, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
     c->media_set = true;
+    c->src_in_dispersive = -1;
     return OMX_OK;
 }
 
@@ -1175,6 +1176,7 @@ int omx3d_add_source(omx3d_ctx* c, int32_t dir, double j_amp, double w0, double 
     HIPCHK(hipMemcpyAsync(s.d_kb, kb.data(), sizeof(double) * np, hipMemcpyHostToDevice, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
     c->sources.push_back(s);
+    c->src_in_dispersive = -1;
     return OMX_OK;
 }
 
