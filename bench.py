@@ -11,7 +11,8 @@ ADE, TF/SF corrections and detector recording at its cadence) over the whole
 grid, inputs resident in HBM.  The FP32 engine is timed (the headline), then
 the FP64 engine on the same steps; the FP32 detector series is compared
 with the FP64 one (relative L2).  The dominant kernel's roofline fraction
-comes from HIP events around each launch in a short profiled pass.  The
+comes from HIP events recorded around each of its launches in the timed
+steps, on the stream the kernels run on.  The
 CPU baseline is the reference's own Fortran/OpenMP solver (built from its
 sources into oracle/_ref) timed on a bounded sample on this host.
 """
@@ -128,6 +129,21 @@ def cpu_reference_baseline(case, steps):
                       f"{secs:.2f} s on {cpu_model}"}
 
 
+def copy_calibration(mib):
+    """A plain device-to-device copy of a known size: its FETCH_SIZE and
+    WRITE_SIZE counters give the factor that turns the counters of the FDTD
+    kernels into bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.arange(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    assert bool(torch.equal(dst[-16:], src[-16:]))
+    log(f"[bench] copy calibration: 5 copies of {mib} MiB ({n} float32)")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,7 +153,14 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mode", choices=("separate", "fused"), default="separate",
+                    help="separate H and E kernels, or the fused H+E pass")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB five times (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU slab "
               f"decomposition is a design, not part of this build.", file=sys.stderr)
@@ -152,15 +175,16 @@ def main():
     for prec in ((4,) if args.no_fp64 else (4, 8)):
         log(f"[bench] building FP{8 * prec} grid {case.grid}")
         g = build_grid(case, prec)
+        g.set_step_mode(1 if args.mode == "separate" else 0)
         if W:
             g.run(1, W, case.record_every)
-        secs = timed_run(g, W + 1, K, case.record_every)
-        series[prec] = g.read_records()
-        # profiled pass for per-kernel durations (continues the same run)
+        # HIP events around every H and E launch of the timed steps, recorded
+        # on the engine's stream
         g.set_profiling(True)
-        g.run(W + K + 1, 50, 0)
+        secs = timed_run(g, W + 1, K, case.record_every)
         ms_h, ms_e, n = g.profile()
         g.set_profiling(False)
+        series[prec] = g.read_records()
         b_h, b_e, b_f, dl_comp = algorithmic_bytes(case, g, prec)
         fused = g.last_run_fused()
         results[prec] = dict(secs=secs, gcells=N * K / secs / 1e9, ms_step=1e3 * secs / K,
@@ -199,7 +223,8 @@ def main():
         "config": {"workload": WORKLOAD, "grid": list(case.grid), "dx_nm": case.dx_nm, "npml": case.npml,
                    "sphere_radius_nm": round(case.info["sphere_radius_nm"], 4),
                    "dl_component_cells": r32["dl_comp"], "record_every": case.record_every,
-                   "detectors": "plane_xy Ex, Ey, Ez at k=%d" % case.info["detector_k"]},
+                   "detectors": "plane_xy Ex, Ey, Ez at k=%d" % case.info["detector_k"],
+                   "step_kernels": "fused" if r32["fused"] else "separate"},
         "roofline": {"bound": "hbm", "kernel": kname,
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,

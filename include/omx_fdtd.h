@@ -195,11 +195,11 @@ int omx3d_set_tfsf(omx3d_ctx* ctx, const int32_t lo[3], const int32_t hi[3],
 /* Steps tt = tt_first .. tt_first+n_steps-1; records detectors when
  * mod(tt, record_every) == 0 (record_every <= 0: never). */
 int omx3d_run(omx3d_ctx* ctx, int32_t tt_first, int32_t n_steps, int32_t record_every);
-/* Step kernels of omx3d_run: mode 0 (default) runs H and E as one fused
- * pass over HBM (fields double-buffered) wherever it applies -- no periodic
- * axis and no source point in a Drude/DL component; mode 1 always runs the
- * separate H and E kernels.  rows (2..16, incl. one halo row) and xchunk set
- * the fused tile; 0 keeps the current value.  No reference counterpart. */
+/* Step kernels of omx3d_run: mode 1 (default) runs the separate H and E
+ * kernels; mode 0 runs H and E as one fused pass over HBM (fields
+ * double-buffered) wherever it applies -- no periodic axis and no source
+ * point in a Drude/DL component.  rows (2..8, incl. one halo row) and xchunk
+ * set the fused tile; 0 keeps the current value.  No reference counterpart. */
 int omx3d_set_step_mode(omx3d_ctx* ctx, int32_t mode, int32_t rows, int32_t xchunk);
 /* 1 if the last omx3d_run used the fused step. */
 int omx3d_get_step_mode(omx3d_ctx* ctx, int32_t* fused_last_run);

@@ -1474,7 +1474,7 @@ int omx3d_run(omx3d_ctx* c, int32_t tt_first, int32_t n_steps, int32_t record_ev
 int omx3d_set_step_mode(omx3d_ctx* c, int32_t mode, int32_t rows, int32_t xchunk) {
     CHECK_CTX(c);
     if (mode != 0 && mode != 1) return fail(OMX_ERR_ARG, "step mode must be 0 (fused where it applies) or 1");
-    if (rows != 0 && (rows < 2 || rows > 16)) return fail(OMX_ERR_ARG, "fused rows must be in 2..16");
+    if (rows != 0 && (rows < 2 || rows > 8)) return fail(OMX_ERR_ARG, "fused rows must be in 2..8");
     if (xchunk < 0) return fail(OMX_ERR_ARG, "xchunk must be >= 0");
     c->step_mode = mode;
     if (rows) c->fused_rows = rows;

@@ -99,12 +99,12 @@ struct omx3d_ctx {
     size_t esz;
     omx::Layout L;
     hipStream_t s = nullptr;
-    int rows_y = 4, xchunk = 16;
+    int rows_y = 1, xchunk = 4;
 
     void* f[6] = {};    // Ex Ey Ez Hx Hy Hz (current)
     void* f_alt[6] = {};        // second buffers of the fused step
     void* psi_h_alt[6] = {};
-    int step_mode = 0;          // 0: fused step where it applies, 1: separate H and E kernels
+    int step_mode = 1;          // 0: fused step where it applies, 1: separate H and E kernels
     int fused_rows = 8;         // rows per workgroup of the fused step (incl. the halo row)
     int fused_xchunk = 8;
     int src_in_dispersive = -1; // -1 unknown, 0 no, 1 a source point sits in a Drude/DL component

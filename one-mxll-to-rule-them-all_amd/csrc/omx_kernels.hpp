@@ -276,7 +276,12 @@ __global__ void k_media_codes(const int8_t* __restrict__ aux, int nx, int ny, in
     int m[3];
     stagger(aux, nx, ny, nz, nm, i, j, k, m);
     const int code = m[0] + (nm + 1) * (m[1] + (nm + 1) * m[2]);
-    if (present) atomicOr(&present[code >> 5], 1u << (code & 31));
+    if (present) {
+        // nearly every cell repeats a code already seen: read before the
+        // atomic so that only first sightings contend for the word
+        const unsigned bit = 1u << (code & 31);
+        if (!(__atomic_load_n(&present[code >> 5], __ATOMIC_RELAXED) & bit)) atomicOr(&present[code >> 5], bit);
+    }
     if (lut) tag[L.at(i, j, k)] = lut[code];
     if (map_out) { map_out[c] = m[0]; map_out[c + n] = m[1]; map_out[c + 2 * n] = m[2]; }
 }
@@ -295,9 +300,10 @@ __device__ __forceinline__ T psi_step(T psi, T b, T c, T d) {
 // H of one cell, in place in hx/hy/hz (H^{n-1/2} in, H^{n+1/2} out): bulk
 // curl of E for i,j,k in 1..n, then the CPML psi recursions of the faces the
 // cell lies in, applied per component in the reference's face order.  psi is
-// read from a.ph and written to pho (nullptr: computed but not stored).
+// read from a.ph and written to pho where st is set (halo cells compute it
+// without storing).
 template <typename T, bool CPML>
-__device__ __forceinline__ void h_cell(const YeeArgs<T>& a, T* const* pho, int i, int j, int k, long jk,
+__device__ __forceinline__ void h_cell(const YeeArgs<T>& a, T* const* pho, bool st, int i, int j, int k, long jk,
                                        T ezc, T ezn, T ezj, T eyc, T eyn, T eyk, T exc, T exk, T exj,
                                        T dhy, T dhz, T chy, T chz, T& hx, T& hy, T& hz) {
     const T dmu = a.dt_mu0;
@@ -325,7 +331,7 @@ __device__ __forceinline__ void h_cell(const YeeArgs<T>& a, T* const* pho, int i
             const long ps = (long)(i - 1) * sx + jk;
             const T p0 = psi_step(a.ph[0][ps], a.bh[c], a.chd[c], ezn - ezc);
             const T p1 = psi_step(a.ph[1][ps], a.bh[c], a.chd[c], eyc - eyn);
-            if (pho) { pho[0][ps] = p0; pho[1][ps] = p1; }
+            if (st) { pho[0][ps] = p0; pho[1][ps] = p1; }
             hy = hy + dmu * p0; hz = hz + dmu * p1;
         }
         if ((cp & 2u) && i >= a.nx + 1 - npml && i <= a.nx - 1) {
@@ -333,7 +339,7 @@ __device__ __forceinline__ void h_cell(const YeeArgs<T>& a, T* const* pho, int i
             const long ps = (long)(npml - 2 + c) * sx + jk;
             const T p0 = psi_step(a.ph[0][ps], a.bh[c], a.chd[c], ezn - ezc);
             const T p1 = psi_step(a.ph[1][ps], a.bh[c], a.chd[c], eyc - eyn);
-            if (pho) { pho[0][ps] = p0; pho[1][ps] = p1; }
+            if (st) { pho[0][ps] = p0; pho[1][ps] = p1; }
             hy = hy + dmu * p0; hz = hz + dmu * p1;
         }
         // y faces: psiy_Hxy, psiy_Hzy
@@ -342,7 +348,7 @@ __device__ __forceinline__ void h_cell(const YeeArgs<T>& a, T* const* pho, int i
             const long ps = ((long)i * nsh + (j - 1)) * sz + k + a.off;
             const T p0 = psi_step(a.ph[2][ps], a.bh[c], a.chd[c], ezc - ezj);
             const T p1 = psi_step(a.ph[3][ps], a.bh[c], a.chd[c], exj - exc);
-            if (pho) { pho[2][ps] = p0; pho[3][ps] = p1; }
+            if (st) { pho[2][ps] = p0; pho[3][ps] = p1; }
             hx = hx + dmu * p0; hz = hz + dmu * p1;
         }
         if ((cp & 8u) && j >= a.ny + 1 - npml && j <= a.ny - 1) {
@@ -350,7 +356,7 @@ __device__ __forceinline__ void h_cell(const YeeArgs<T>& a, T* const* pho, int i
             const long ps = ((long)i * nsh + (npml - 2 + c)) * sz + k + a.off;
             const T p0 = psi_step(a.ph[2][ps], a.bh[c], a.chd[c], ezc - ezj);
             const T p1 = psi_step(a.ph[3][ps], a.bh[c], a.chd[c], exj - exc);
-            if (pho) { pho[2][ps] = p0; pho[3][ps] = p1; }
+            if (st) { pho[2][ps] = p0; pho[3][ps] = p1; }
             hx = hx + dmu * p0; hz = hz + dmu * p1;
         }
         // z faces: psiz_Hxz, psiz_Hyz
@@ -359,7 +365,7 @@ __device__ __forceinline__ void h_cell(const YeeArgs<T>& a, T* const* pho, int i
             const long ps = ((long)i * a.sy + j) * nsh + (k - 1);
             const T p0 = psi_step(a.ph[4][ps], a.bh[c], a.chd[c], eyk - eyc);
             const T p1 = psi_step(a.ph[5][ps], a.bh[c], a.chd[c], exc - exk);
-            if (pho) { pho[4][ps] = p0; pho[5][ps] = p1; }
+            if (st) { pho[4][ps] = p0; pho[5][ps] = p1; }
             hx = hx + dmu * p0; hy = hy + dmu * p1;
         }
         if ((cp & 32u) && k >= a.nz + 1 - npml && k <= a.nz - 1) {
@@ -367,7 +373,7 @@ __device__ __forceinline__ void h_cell(const YeeArgs<T>& a, T* const* pho, int i
             const long ps = ((long)i * a.sy + j) * nsh + (npml - 2 + c);
             const T p0 = psi_step(a.ph[4][ps], a.bh[c], a.chd[c], eyk - eyc);
             const T p1 = psi_step(a.ph[5][ps], a.bh[c], a.chd[c], exc - exk);
-            if (pho) { pho[4][ps] = p0; pho[5][ps] = p1; }
+            if (st) { pho[4][ps] = p0; pho[5][ps] = p1; }
             hx = hx + dmu * p0; hy = hy + dmu * p1;
         }
     }
@@ -396,7 +402,7 @@ __device__ __forceinline__ void h_tile(const YeeArgs<T>& a, const int t) {
         const T ezj = a.ez[o + sz], eyk = a.ey[o + 1];
         const T exc = a.ex[o], exk = a.ex[o + 1], exj = a.ex[o + sz];
         T hx = a.hx[o], hy = a.hy[o], hz = a.hz[o];
-        h_cell<T, CPML>(a, a.ph, i, j, k, jk, ezc, ezn, ezj, eyc, eyn, eyk, exc, exk, exj, dhy, dhz, chy, chz,
+        h_cell<T, CPML>(a, a.ph, true, i, j, k, jk, ezc, ezn, ezj, eyc, eyn, eyk, exc, exk, exj, dhy, dhz, chy, chz,
                         hx, hy, hz);
         a.hx[o] = hx; a.hy[o] = hy; a.hz[o] = hz;
         ezc = ezn; eyc = eyn;
@@ -457,9 +463,10 @@ __device__ __forceinline__ T medium_update(T e, T rot, T vac, int m, const MedDe
 }
 
 // E of one cell, in place in ex/ey/ez: bulk curl of H with the medium
-// update, then the CPML psi recursions (psi of E in place in a.pe).
+// update, then the CPML psi recursions (psi of E in place in a.pe).  tg is
+// the cell's class tag (0: vacuum in all three components).
 template <typename T, bool CPML, bool MEDIA>
-__device__ __forceinline__ void e_cell(const YeeArgs<T>& a, int i, int j, int k, long o, long jk,
+__device__ __forceinline__ void e_cell(const YeeArgs<T>& a, int tg, int i, int j, int k, long o, long jk,
                                        T hzc, T hzj, T hzi, T hyc, T hyk, T hyi, T hxc, T hxj, T hxk,
                                        T dey, T dez, T cey, T cez, T& ex, T& ey, T& ez) {
     const T dep = a.dt_eps0;
@@ -477,8 +484,7 @@ __device__ __forceinline__ void e_cell(const YeeArgs<T>& a, int i, int j, int k,
     } else {
         vx = dep * rx; vy = dep * ry; vz = dep * rz;
     }
-    if (MEDIA && a.tag != nullptr) {
-        const int tg = a.tag[o];
+    if (MEDIA) {
         if (tg == 0) {
             ex = ex + vx; ey = ey + vy; ez = ez + vz;
         } else {
@@ -573,8 +579,9 @@ __device__ __forceinline__ void e_tile(const YeeArgs<T>& a, const int t) {
         const T hzj = a.hz[o - sz], hxj = a.hx[o - sz];
         const T hyk = a.hy[o - 1], hxk = a.hx[o - 1];
         T ex = a.ex[o], ey = a.ey[o], ez = a.ez[o];
-        e_cell<T, CPML, MEDIA>(a, i, j, k, o, jk, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dey, dez, cey, cez,
-                               ex, ey, ez);
+        const int tg = (MEDIA && a.tag != nullptr) ? (int)a.tag[o] : 0;
+        e_cell<T, CPML, MEDIA>(a, tg, i, j, k, o, jk, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dey, dez, cey,
+                               cez, ex, ey, ez);
         a.ex[o] = ex; a.ey[o] = ey; a.ez[o] = ez;
         hzi = hzc; hyi = hyc;
     }
@@ -597,11 +604,18 @@ __global__ __launch_bounds__(256) void k_e_update(const YeeArgs<T> a) {
 // of separate H and E passes).  A workgroup is 64 lanes along z times FR
 // rows along y and marches an x chunk.  Lane 0 and row 0 are halos: there H
 // is recomputed (from the unmodified input buffers) so that the E update of
-// the owned 63 x (FR-1) cells has all its H neighbours; H rows travel through
-// LDS, H lanes through cross-lane shuffles, the previous x plane in
-// registers.  The psi of H is double-buffered like the fields (halo cells read
-// it); psi of E and the dispersive state are touched only by their owner.
-// The TF/SF corrections of H are applied inside, where E needs them.
+// the owned 63 x (FR-1) cells has all its H neighbours; the plane before the
+// chunk is an x halo in the same way.  H rows travel through LDS, H lanes
+// through cross-lane shuffles, the previous x plane in registers.  The psi of
+// H is double-buffered like the fields (halo cells read it); psi of E and the
+// dispersive state are touched only by their owner.  The TF/SF corrections
+// of H are applied inside, where E needs them.
+//
+// Pipelining: the loads of plane i+1 (E two planes ahead, H, tag, the edge
+// row and lane) are issued at the top of iteration i and consumed in
+// iteration i+1; the E rows of plane i+1 go to LDS in iteration i, so one
+// barrier per plane orders both exchanges (E rows and H rows each alternate
+// between two LDS buffers by plane parity).
 // ---------------------------------------------------------------------------
 template <typename T>
 struct FusedArgs {
@@ -613,8 +627,36 @@ struct FusedArgs {
     double dt_mu0;
 };
 
+// Values of one x plane that a fused workgroup loads ahead of use.
+template <typename T>
+struct FusedPlane {
+    T ex2, ey2, ez2;   // E two planes ahead
+    T hx, hy, hz;      // H of the next plane
+    T ezj, exj;        // E of the next plane at j+1 (last row only)
+    T eyk, exk;        // E of the next plane at k+1 (last lane only)
+    int tg;            // class tag of the next plane
+};
+
+template <typename T, bool MEDIA>
+__device__ __forceinline__ void fused_load(const YeeArgs<T>& a, FusedPlane<T>& p, long o, bool inb, bool last_row,
+                                           bool row_ok, bool last_lane, bool lane_ok, int sz) {
+    // o: offset of plane ip (the next plane) at this lane's (j, k)
+    const long sx = a.sx;
+    p.ex2 = p.ey2 = p.ez2 = (T)0;
+    p.hx = p.hy = p.hz = (T)0;
+    p.ezj = p.exj = p.eyk = p.exk = (T)0;
+    p.tg = 0;
+    if (inb) {
+        p.ex2 = a.ex[o + sx]; p.ey2 = a.ey[o + sx]; p.ez2 = a.ez[o + sx];
+        p.hx = a.hx[o]; p.hy = a.hy[o]; p.hz = a.hz[o];
+        if constexpr (MEDIA) p.tg = a.tag[o];
+    }
+    if (last_row && row_ok) { p.ezj = a.ez[o + sz]; p.exj = a.ex[o + sz]; }
+    if (last_lane && lane_ok) { p.eyk = a.ey[o + 1]; p.exk = a.ex[o + 1]; }
+}
+
 template <typename T, bool CPML, bool MEDIA, bool TFSF>
-__global__ __launch_bounds__(1024) void k_step_fused(const YeeArgs<T> a, const FusedArgs<T> f) {
+__global__ __launch_bounds__(512) void k_step_fused(const YeeArgs<T> a, const FusedArgs<T> f) {
     extern __shared__ unsigned char omx_lds[];
     T* lds = reinterpret_cast<T*>(omx_lds);
     const int FR = blockDim.y;
@@ -631,44 +673,55 @@ __global__ __launch_bounds__(1024) void k_step_fused(const YeeArgs<T> a, const F
     const bool inb = k <= a.nz + 1 && j <= a.ny + 1;
     const bool hcell = k >= 1 && k <= a.nz && j >= 1 && j <= a.ny;
     const bool own = hcell && lane >= 1 && row >= 1;
+    const bool last_row = row == FR - 1, last_lane = lane == kLanes - 1;
+    const bool row_ok = k <= a.nz + 1 && j + 1 <= a.ny + 1;
+    const bool lane_ok = k + 1 <= a.nz + 1 && j <= a.ny + 1;
     const long sx = a.sx;
     const int sz = a.sz;
     const long jk = (long)j * sz + k + a.off;
-    long o = (long)(i0 - 1) * sx + jk;
     const int jc = min(j, a.ny + 1), kc = min(k, a.nz + 1);
     const T dhy = a.den_hy[jc], dhz = a.den_hz[kc], dey = a.den_ey[jc], dez = a.den_ez[kc];
     const T chy = sizeof(T) == 4 ? a.cdhy[jc] : (T)0, chz = sizeof(T) == 4 ? a.cdhz[kc] : (T)0;
     const T cey = sizeof(T) == 4 ? a.cdey[jc] : (T)0, cez = sizeof(T) == 4 ? a.cdez[kc] : (T)0;
     const int plane = FR * kLanes;
     const int me = row * kLanes + lane;
-    T ezc = inb ? a.ez[o] : (T)0, eyc = inb ? a.ey[o] : (T)0;
+    T* const EA[2] = {lds, lds + 2 * plane};            // E rows (ez, ex) by plane parity
+    T* const HB[2] = {lds + 4 * plane, lds + 6 * plane}; // H rows (hz, hx) by plane parity
+
+    // prologue: plane i0-1 and the E of plane i0
+    long o = (long)(i0 - 1) * sx + jk;
+    T exc = (T)0, eyc = (T)0, ezc = (T)0, exn = (T)0, eyn = (T)0, ezn = (T)0;
+    T hx = (T)0, hy = (T)0, hz = (T)0, ezj = (T)0, exj = (T)0, eyk_e = (T)0, exk_e = (T)0;
+    int tg = 0;
+    if (inb) {
+        exc = a.ex[o]; eyc = a.ey[o]; ezc = a.ez[o];
+        exn = a.ex[o + sx]; eyn = a.ey[o + sx]; ezn = a.ez[o + sx];
+        hx = a.hx[o]; hy = a.hy[o]; hz = a.hz[o];
+        if constexpr (MEDIA) tg = a.tag[o];
+    }
+    if (last_row && row_ok) { ezj = a.ez[o + sz]; exj = a.ex[o + sz]; }
+    if (last_lane && lane_ok) { eyk_e = a.ey[o + 1]; exk_e = a.ex[o + 1]; }
+    {
+        T* E = EA[(i0 - 1) & 1];
+        E[me] = ezc;
+        E[plane + me] = exc;
+    }
+    __syncthreads();
     T hzi = (T)0, hyi = (T)0;
     for (int i = i0 - 1; i <= i1; ++i, o += sx) {
-        T* L = lds + (size_t)(i & 1) * 4 * plane;
-        const T exc = inb ? a.ex[o] : (T)0;
-        const T ezn = inb ? a.ez[o + sx] : (T)0, eyn = inb ? a.ey[o + sx] : (T)0;
-        T hx = inb ? a.hx[o] : (T)0, hy = inb ? a.hy[o] : (T)0, hz = inb ? a.hz[o] : (T)0;
-        L[me] = ezc;
-        L[plane + me] = exc;
-        __syncthreads();
-        T ezj, exj;
-        if (row + 1 < FR) {
-            ezj = L[me + kLanes];
-            exj = L[plane + me + kLanes];
-        } else {
-            const bool ok = k <= a.nz + 1 && j + 1 <= a.ny + 1;
-            ezj = ok ? a.ez[o + sz] : (T)0;
-            exj = ok ? a.ex[o + sz] : (T)0;
+        // loads for plane i+1, consumed next iteration
+        FusedPlane<T> nx_;
+        if (i < i1) fused_load<T, MEDIA>(a, nx_, o + sx, inb, last_row, row_ok, last_lane, lane_ok, sz);
+        // E neighbours of plane i: rows from LDS, lanes by shuffle
+        {
+            const T* E = EA[i & 1];
+            if (!last_row) { ezj = E[me + kLanes]; exj = E[plane + me + kLanes]; }
         }
         T eyk = __shfl_down(eyc, 1), exk = __shfl_down(exc, 1);
-        if (lane == kLanes - 1) {
-            const bool ok = k + 1 <= a.nz + 1 && j <= a.ny + 1;
-            eyk = ok ? a.ey[o + 1] : (T)0;
-            exk = ok ? a.ex[o + 1] : (T)0;
-        }
+        if (last_lane) { eyk = eyk_e; exk = exk_e; }
         const bool store = own && i >= i0;
         if (hcell && i >= 1) {
-            h_cell<T, CPML>(a, store ? f.pho : nullptr, i, j, k, jk, ezc, ezn, ezj, eyc, eyn, eyk, exc, exk, exj,
+            h_cell<T, CPML>(a, f.pho, store, i, j, k, jk, ezc, ezn, ezj, eyc, eyn, eyk, exc, exk, exj,
                             dhy, dhz, chy, chz, hx, hy, hz);
             if constexpr (TFSF) {
                 if (i >= f.ti0 && i <= f.ti1 - 1) {
@@ -688,19 +741,33 @@ __global__ __launch_bounds__(1024) void k_step_fused(const YeeArgs<T> a, const F
             }
         }
         if (store) { f.hxo[o] = hx; f.hyo[o] = hy; f.hzo[o] = hz; }
-        L[2 * plane + me] = hz;
-        L[3 * plane + me] = hx;
+        {
+            T* H = HB[i & 1];
+            H[me] = hz;
+            H[plane + me] = hx;
+            T* E = EA[(i + 1) & 1];
+            E[me] = ezn;
+            E[plane + me] = exn;
+        }
         __syncthreads();
         const T hyk = __shfl_up(hy, 1), hxk = __shfl_up(hx, 1);
         if (store) {
-            const T hzj = L[2 * plane + me - kLanes], hxj = L[3 * plane + me - kLanes];
+            const T* H = HB[i & 1];
+            const T hzj = H[me - kLanes], hxj = H[plane + me - kLanes];
             T ex = exc, ey = eyc, ez = ezc;
-            e_cell<T, CPML, MEDIA>(a, i, j, k, o, jk, hz, hzj, hzi, hy, hyk, hyi, hx, hxj, hxk, dey, dez, cey, cez,
-                                   ex, ey, ez);
+            e_cell<T, CPML, MEDIA>(a, tg, i, j, k, o, jk, hz, hzj, hzi, hy, hyk, hyi, hx, hxj, hxk, dey, dez, cey,
+                                   cez, ex, ey, ez);
             f.exo[o] = ex; f.eyo[o] = ey; f.ezo[o] = ez;
         }
         hzi = hz; hyi = hy;
-        ezc = ezn; eyc = eyn;
+        if (i < i1) {
+            exc = exn; eyc = eyn; ezc = ezn;
+            exn = nx_.ex2; eyn = nx_.ey2; ezn = nx_.ez2;
+            hx = nx_.hx; hy = nx_.hy; hz = nx_.hz;
+            if (last_row) { ezj = nx_.ezj; exj = nx_.exj; }
+            eyk_e = nx_.eyk; exk_e = nx_.exk;
+            tg = nx_.tg;
+        }
     }
 }
 
