@@ -334,36 +334,86 @@ bool fused_applies(omx3d_ctx* c) {
     if (c->step_mode != 0) return false;
     for (int a = 0; a < 3; ++a)
         if (c->d.boundaries[a] == OMX_BC_PERIODIC) return false;
+    // the fused kernel addresses field and z-face psi arrays by 32-bit byte
+    // offsets, and handles one z face per cell (the faces do not meet)
+    if ((unsigned long long)c->L.nalloc * c->esz >= (1ull << 32)) return false;
+    if ((unsigned long long)(c->d.nx + 2) * c->L.sy * c->nse * c->esz >= (1ull << 32)) return false;
+    if ((c->cpml_mask & 48u) && c->d.nz < 2 * c->d.npml + 1) return false;
     return c->src_in_dispersive == 0;
+}
+
+// The fused kernel's cold argument block lives in device memory, one slot
+// per buffer parity (the fields and the psi of H alternate between two
+// buffers).  A slot is rewritten only when its contents change (set-up,
+// media, TF/SF or a buffer reallocation), with a synchronous upload.
+template <typename T>
+int upload_fused_cold(omx3d_ctx* c, const FusedCold<T>& fc, const FusedCold<T>** dev) {
+    static_assert(sizeof(FusedCold<T>) <= omx3d_ctx::kColdBytes, "cold block slot too small");
+    if (!c->d_fcold) {
+        int rc = dalloc(c, &c->d_fcold, 2 * omx3d_ctx::kColdBytes);
+        if (rc) return rc;
+        c->fcold_valid[0] = c->fcold_valid[1] = false;
+    }
+    const int slot = c->fparity & 1;
+    unsigned char* h = c->h_fcold[slot];
+    char* d = (char*)c->d_fcold + (size_t)slot * omx3d_ctx::kColdBytes;
+    if (!c->fcold_valid[slot] || std::memcmp(h, &fc, sizeof(fc)) != 0) {
+        std::memcpy(h, &fc, sizeof(fc));
+        HIPCHK(hipMemcpyAsync(d, h, sizeof(fc), hipMemcpyHostToDevice, c->s));
+        HIPCHK(hipStreamSynchronize(c->s));
+        c->fcold_valid[slot] = true;
+    }
+    *dev = (const FusedCold<T>*)d;
+    return OMX_OK;
 }
 
 template <typename T>
 int launch_fused(omx3d_ctx* c) {
     int rc = ensure_alt(c);
     if (rc) return rc;
-    YeeArgs<T> a = make_args<T>(c);
+    FusedCold<T> fc;
+    std::memset(&fc, 0, sizeof(fc));
+    fc.a = make_args<T>(c);
+    for (int q = 0; q < 6; ++q) fc.pho[q] = (T*)c->psi_h_alt[q];
+    if (c->tfsf) {
+        fc.ti0 = c->tf_lo[0]; fc.tj0 = c->tf_lo[1]; fc.tk0 = c->tf_lo[2];
+        fc.ti1 = c->tf_hi[0]; fc.tj1 = c->tf_hi[1]; fc.tk1 = c->tf_hi[2];
+        fc.tkb = c->tf_kb; fc.ex1 = c->d_tf; fc.dt_mu0 = c->d.dt / kMu0;
+    }
+    const FusedCold<T>* dcold = nullptr;
+    rc = upload_fused_cold<T>(c, fc, &dcold);
+    if (rc) return rc;
+    const YeeArgs<T>& a = fc.a;
     FusedArgs<T> f;
     std::memset(&f, 0, sizeof(f));
+    f.exi = a.ex; f.eyi = a.ey; f.ezi = a.ez; f.hxi = a.hx; f.hyi = a.hy; f.hzi = a.hz;
     f.exo = (T*)c->f_alt[0]; f.eyo = (T*)c->f_alt[1]; f.ezo = (T*)c->f_alt[2];
     f.hxo = (T*)c->f_alt[3]; f.hyo = (T*)c->f_alt[4]; f.hzo = (T*)c->f_alt[5];
-    for (int q = 0; q < 6; ++q) f.pho[q] = (T*)c->psi_h_alt[q];
+    f.tag = a.tag;
+    f.cx = sizeof(T) == 4 ? a.cdhx : a.den_hx;
+    f.dex = a.den_ex;
+    f.cex = a.cdex;
+    f.phz[0] = a.ph[4]; f.phz[1] = a.ph[5];
+    f.phzo[0] = fc.pho[4]; f.phzo[1] = fc.pho[5];
+    f.pez[0] = a.pe[4]; f.pez[1] = a.pe[5];
+    f.dt_mu0 = a.dt_mu0; f.dt_eps0 = a.dt_eps0;
+    f.sx = a.sx;
+    f.nx = a.nx; f.ny = a.ny; f.nz = a.nz; f.sy = a.sy; f.sz = a.sz; f.off = a.off;
+    f.npml = a.npml; f.cpml = a.cpml;
     const int FR = c->fused_rows;
-    a.xchunk = c->fused_xchunk;
+    f.xchunk = c->fused_xchunk;
     f.ntz = (c->L.nz + kLanes - 2) / (kLanes - 1);
     f.nty = (c->L.ny + FR - 2) / (FR - 1);
-    const long ntx = (c->L.nx + a.xchunk - 1) / a.xchunk;
+    const long ntx = (c->L.nx + f.xchunk - 1) / f.xchunk;
     const unsigned nb = (unsigned)((long)f.ntz * f.nty * ntx);
-    if (c->tfsf) {
-        f.ti0 = c->tf_lo[0]; f.tj0 = c->tf_lo[1]; f.tk0 = c->tf_lo[2];
-        f.ti1 = c->tf_hi[0]; f.tj1 = c->tf_hi[1]; f.tk1 = c->tf_hi[2];
-        f.tkb = c->tf_kb; f.ex1 = c->d_tf; f.dt_mu0 = c->d.dt / kMu0;
+    if (c->tfsf)
         hipLaunchKernelGGL(k_tfsf_1d_h, dim3(blocks_for(c->tf_n, 256)), dim3(256), 0, c->s, c->d_tf,
                            c->d_tf + c->tf_n, c->tf_n, (c->d.dt / kMu0) * (1.0 / c->d.dr));
-    }
     const dim3 blk(kLanes, FR);
-    const size_t lds = sizeof(T) * 2 * 4 * FR * kLanes;
+    const size_t lds = sizeof(T) * 10 * FR * kLanes;   // E rows 2 x 3 planes, H rows 2 x 2
     const bool cp = c->cpml_mask != 0, md = c->media_set && c->n_classes > 1, tf = c->tfsf;
-#define OMX_FUSED(CP, MD, TF) hipLaunchKernelGGL((k_step_fused<T, CP, MD, TF>), dim3(nb), blk, lds, c->s, a, f)
+#define OMX_FUSED(CP, MD, TF) \
+    hipLaunchKernelGGL((k_step_fused<T, CP, MD, TF>), dim3(nb), blk, lds, c->s, f, dcold)
     if (cp && md && tf) OMX_FUSED(true, true, true);
     else if (cp && md) OMX_FUSED(true, true, false);
     else if (cp && tf) OMX_FUSED(true, false, true);
@@ -376,6 +426,7 @@ int launch_fused(omx3d_ctx* c) {
     HIPCHK(hipGetLastError());
     for (int q = 0; q < 6; ++q) std::swap(c->f[q], c->f_alt[q]);
     for (int q = 0; q < 6; ++q) std::swap(c->psi_h[q], c->psi_h_alt[q]);
+    c->fparity ^= 1;
     return OMX_OK;
 }
 
@@ -1007,6 +1058,7 @@ int omx3d_destroy(omx3d_ctx* c) {
     for (auto& p : c->f) dfree(p);
     for (auto& p : c->f_alt) dfree(p);
     for (auto& p : c->psi_h_alt) dfree(p);
+    dfree(c->d_fcold);
     dfreet(c->dcoef); dfree(c->tcoef);
     for (auto& p : c->psi_h) dfree(p);
     for (auto& p : c->psi_e) dfree(p);

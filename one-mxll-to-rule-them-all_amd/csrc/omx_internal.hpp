@@ -105,10 +105,16 @@ struct omx3d_ctx {
     void* f_alt[6] = {};        // second buffers of the fused step
     void* psi_h_alt[6] = {};
     int step_mode = 1;          // 0: fused step where it applies, 1: separate H and E kernels
-    int fused_rows = 8;         // rows per workgroup of the fused step (incl. the halo row)
-    int fused_xchunk = 8;
+    int fused_rows = 4;         // rows per workgroup of the fused step (incl. the halo row)
+    int fused_xchunk = 16;
     int src_in_dispersive = -1; // -1 unknown, 0 no, 1 a source point sits in a Drude/DL component
     bool last_fused = false;
+    // fused step: device-resident cold argument blocks, one per buffer parity
+    static constexpr size_t kColdBytes = 1024;
+    void* d_fcold = nullptr;
+    unsigned char h_fcold[2][kColdBytes] = {};
+    bool fcold_valid[2] = {false, false};
+    int fparity = 0;
     // FP64 master coefficients: be ce (npml+1) bh ch (npml) den_e{x,y,z} den_h{x,y,z} (n+2 each)
     double* dcoef = nullptr;
     void* tcoef = nullptr;          // precision-T copies used by the kernels

@@ -299,83 +299,109 @@ __device__ __forceinline__ T psi_step(T psi, T b, T c, T d) {
 
 // H of one cell, in place in hx/hy/hz (H^{n-1/2} in, H^{n+1/2} out): bulk
 // curl of E for i,j,k in 1..n, then the CPML psi recursions of the faces the
-// cell lies in, applied per component in the reference's face order.  psi is
-// read from a.ph and written to pho where st is set (halo cells compute it
-// without storing).
+// cell lies in, applied per component in the reference's face order (x, y,
+// z).  psi is read from a.ph and written to pho where st is set (halo cells
+// compute it without storing).  cx is the plane's coefficient: cdhx[i] in
+// FP32, den_hx[i] in FP64.
+template <typename T>
+__device__ __forceinline__ void h_bulk(T dmu, T cx, T dhy, T dhz, T chy, T chz, T ezc, T ezn, T ezj, T eyc, T eyn,
+                                       T eyk, T exc, T exk, T exj, T& hx, T& hy, T& hz) {
+    if constexpr (sizeof(T) == 4) {
+        // FP32: dt_mu0*den folded into one coefficient per axis (one rounding fewer)
+        hx = hx + (ezc - ezj) * chy + (eyk - eyc) * chz;
+        hy = hy + (ezn - ezc) * cx + (exc - exk) * chz;
+        hz = hz + (eyc - eyn) * cx + (exj - exc) * chy;
+    } else {
+        hx = hx + dmu * ((ezc - ezj) * dhy + (eyk - eyc) * dhz);
+        hy = hy + dmu * ((ezn - ezc) * cx + (exc - exk) * dhz);
+        hz = hz + dmu * ((eyc - eyn) * cx + (exj - exc) * dhy);
+    }
+}
+
+// x faces: psix_Hyx, psix_Hzx
+template <typename T>
+__device__ __forceinline__ void h_cpml_x(const YeeArgs<T>& a, T* const* pho, bool st, int i, long jk, T dmu,
+                                         T ezc, T ezn, T eyc, T eyn, T& hy, T& hz) {
+    const unsigned cp = a.cpml;
+    const int npml = a.npml;
+    if ((cp & 1u) && i <= npml - 1) {
+        const int c = i;
+        const long ps = (long)(i - 1) * a.sx + jk;
+        const T p0 = psi_step(a.ph[0][ps], a.bh[c], a.chd[c], ezn - ezc);
+        const T p1 = psi_step(a.ph[1][ps], a.bh[c], a.chd[c], eyc - eyn);
+        if (st) { pho[0][ps] = p0; pho[1][ps] = p1; }
+        hy = hy + dmu * p0; hz = hz + dmu * p1;
+    }
+    if ((cp & 2u) && i >= a.nx + 1 - npml && i <= a.nx - 1) {
+        const int c = a.nx - i;
+        const long ps = (long)(npml - 2 + c) * a.sx + jk;
+        const T p0 = psi_step(a.ph[0][ps], a.bh[c], a.chd[c], ezn - ezc);
+        const T p1 = psi_step(a.ph[1][ps], a.bh[c], a.chd[c], eyc - eyn);
+        if (st) { pho[0][ps] = p0; pho[1][ps] = p1; }
+        hy = hy + dmu * p0; hz = hz + dmu * p1;
+    }
+}
+
+// y faces: psiy_Hxy, psiy_Hzy
+template <typename T>
+__device__ __forceinline__ void h_cpml_y(const YeeArgs<T>& a, T* const* pho, bool st, int i, int j, int k, T dmu,
+                                         T ezc, T ezj, T exc, T exj, T& hx, T& hz) {
+    const unsigned cp = a.cpml;
+    const int npml = a.npml;
+    const int nsh = 2 * (npml - 1);
+    if ((cp & 4u) && j <= npml - 1) {
+        const int c = j;
+        const long ps = ((long)i * nsh + (j - 1)) * a.sz + k + a.off;
+        const T p0 = psi_step(a.ph[2][ps], a.bh[c], a.chd[c], ezc - ezj);
+        const T p1 = psi_step(a.ph[3][ps], a.bh[c], a.chd[c], exj - exc);
+        if (st) { pho[2][ps] = p0; pho[3][ps] = p1; }
+        hx = hx + dmu * p0; hz = hz + dmu * p1;
+    }
+    if ((cp & 8u) && j >= a.ny + 1 - npml && j <= a.ny - 1) {
+        const int c = a.ny - j;
+        const long ps = ((long)i * nsh + (npml - 2 + c)) * a.sz + k + a.off;
+        const T p0 = psi_step(a.ph[2][ps], a.bh[c], a.chd[c], ezc - ezj);
+        const T p1 = psi_step(a.ph[3][ps], a.bh[c], a.chd[c], exj - exc);
+        if (st) { pho[2][ps] = p0; pho[3][ps] = p1; }
+        hx = hx + dmu * p0; hz = hz + dmu * p1;
+    }
+}
+
+// z faces: psiz_Hxz, psiz_Hyz
+template <typename T>
+__device__ __forceinline__ void h_cpml_z(const YeeArgs<T>& a, T* const* pho, bool st, int i, int j, int k, T dmu,
+                                         T eyc, T eyk, T exc, T exk, T& hx, T& hy) {
+    const unsigned cp = a.cpml;
+    const int npml = a.npml;
+    const int nsh = 2 * (npml - 1);
+    if ((cp & 16u) && k <= npml - 1) {
+        const int c = k;
+        const long ps = ((long)i * a.sy + j) * nsh + (k - 1);
+        const T p0 = psi_step(a.ph[4][ps], a.bh[c], a.chd[c], eyk - eyc);
+        const T p1 = psi_step(a.ph[5][ps], a.bh[c], a.chd[c], exc - exk);
+        if (st) { pho[4][ps] = p0; pho[5][ps] = p1; }
+        hx = hx + dmu * p0; hy = hy + dmu * p1;
+    }
+    if ((cp & 32u) && k >= a.nz + 1 - npml && k <= a.nz - 1) {
+        const int c = a.nz - k;
+        const long ps = ((long)i * a.sy + j) * nsh + (npml - 2 + c);
+        const T p0 = psi_step(a.ph[4][ps], a.bh[c], a.chd[c], eyk - eyc);
+        const T p1 = psi_step(a.ph[5][ps], a.bh[c], a.chd[c], exc - exk);
+        if (st) { pho[4][ps] = p0; pho[5][ps] = p1; }
+        hx = hx + dmu * p0; hy = hy + dmu * p1;
+    }
+}
+
 template <typename T, bool CPML>
 __device__ __forceinline__ void h_cell(const YeeArgs<T>& a, T* const* pho, bool st, int i, int j, int k, long jk,
                                        T ezc, T ezn, T ezj, T eyc, T eyn, T eyk, T exc, T exk, T exj,
-                                       T dhy, T dhz, T chy, T chz, T& hx, T& hy, T& hz) {
+                                       T cx, T dhy, T dhz, T chy, T chz, T& hx, T& hy, T& hz) {
     const T dmu = a.dt_mu0;
-    if constexpr (sizeof(T) == 4) {
-        // FP32: dt_mu0*den folded into one coefficient per axis (one rounding fewer)
-        const T chx = a.cdhx[i];
-        hx = hx + (ezc - ezj) * chy + (eyk - eyc) * chz;
-        hy = hy + (ezn - ezc) * chx + (exc - exk) * chz;
-        hz = hz + (eyc - eyn) * chx + (exj - exc) * chy;
-    } else {
-        const T dhx = a.den_hx[i];
-        hx = hx + dmu * ((ezc - ezj) * dhy + (eyk - eyc) * dhz);
-        hy = hy + dmu * ((ezn - ezc) * dhx + (exc - exk) * dhz);
-        hz = hz + dmu * ((eyc - eyn) * dhx + (exj - exc) * dhy);
-    }
+    h_bulk(dmu, cx, dhy, dhz, chy, chz, ezc, ezn, ezj, eyc, eyn, eyk, exc, exk, exj, hx, hy, hz);
     if constexpr (CPML) {
-        const unsigned cp = a.cpml;
-        const int npml = a.npml;
-        const int nsh = 2 * (npml - 1);
-        const long sx = a.sx;
-        const int sz = a.sz;
-        // x faces: psix_Hyx, psix_Hzx
-        if ((cp & 1u) && i <= npml - 1) {
-            const int c = i;
-            const long ps = (long)(i - 1) * sx + jk;
-            const T p0 = psi_step(a.ph[0][ps], a.bh[c], a.chd[c], ezn - ezc);
-            const T p1 = psi_step(a.ph[1][ps], a.bh[c], a.chd[c], eyc - eyn);
-            if (st) { pho[0][ps] = p0; pho[1][ps] = p1; }
-            hy = hy + dmu * p0; hz = hz + dmu * p1;
-        }
-        if ((cp & 2u) && i >= a.nx + 1 - npml && i <= a.nx - 1) {
-            const int c = a.nx - i;
-            const long ps = (long)(npml - 2 + c) * sx + jk;
-            const T p0 = psi_step(a.ph[0][ps], a.bh[c], a.chd[c], ezn - ezc);
-            const T p1 = psi_step(a.ph[1][ps], a.bh[c], a.chd[c], eyc - eyn);
-            if (st) { pho[0][ps] = p0; pho[1][ps] = p1; }
-            hy = hy + dmu * p0; hz = hz + dmu * p1;
-        }
-        // y faces: psiy_Hxy, psiy_Hzy
-        if ((cp & 4u) && j <= npml - 1) {
-            const int c = j;
-            const long ps = ((long)i * nsh + (j - 1)) * sz + k + a.off;
-            const T p0 = psi_step(a.ph[2][ps], a.bh[c], a.chd[c], ezc - ezj);
-            const T p1 = psi_step(a.ph[3][ps], a.bh[c], a.chd[c], exj - exc);
-            if (st) { pho[2][ps] = p0; pho[3][ps] = p1; }
-            hx = hx + dmu * p0; hz = hz + dmu * p1;
-        }
-        if ((cp & 8u) && j >= a.ny + 1 - npml && j <= a.ny - 1) {
-            const int c = a.ny - j;
-            const long ps = ((long)i * nsh + (npml - 2 + c)) * sz + k + a.off;
-            const T p0 = psi_step(a.ph[2][ps], a.bh[c], a.chd[c], ezc - ezj);
-            const T p1 = psi_step(a.ph[3][ps], a.bh[c], a.chd[c], exj - exc);
-            if (st) { pho[2][ps] = p0; pho[3][ps] = p1; }
-            hx = hx + dmu * p0; hz = hz + dmu * p1;
-        }
-        // z faces: psiz_Hxz, psiz_Hyz
-        if ((cp & 16u) && k <= npml - 1) {
-            const int c = k;
-            const long ps = ((long)i * a.sy + j) * nsh + (k - 1);
-            const T p0 = psi_step(a.ph[4][ps], a.bh[c], a.chd[c], eyk - eyc);
-            const T p1 = psi_step(a.ph[5][ps], a.bh[c], a.chd[c], exc - exk);
-            if (st) { pho[4][ps] = p0; pho[5][ps] = p1; }
-            hx = hx + dmu * p0; hy = hy + dmu * p1;
-        }
-        if ((cp & 32u) && k >= a.nz + 1 - npml && k <= a.nz - 1) {
-            const int c = a.nz - k;
-            const long ps = ((long)i * a.sy + j) * nsh + (npml - 2 + c);
-            const T p0 = psi_step(a.ph[4][ps], a.bh[c], a.chd[c], eyk - eyc);
-            const T p1 = psi_step(a.ph[5][ps], a.bh[c], a.chd[c], exc - exk);
-            if (st) { pho[4][ps] = p0; pho[5][ps] = p1; }
-            hx = hx + dmu * p0; hy = hy + dmu * p1;
-        }
+        h_cpml_x(a, pho, st, i, jk, dmu, ezc, ezn, eyc, eyn, hy, hz);
+        h_cpml_y(a, pho, st, i, j, k, dmu, ezc, ezj, exc, exj, hx, hz);
+        h_cpml_z(a, pho, st, i, j, k, dmu, eyc, eyk, exc, exk, hx, hy);
     }
 }
 
@@ -402,8 +428,9 @@ __device__ __forceinline__ void h_tile(const YeeArgs<T>& a, const int t) {
         const T ezj = a.ez[o + sz], eyk = a.ey[o + 1];
         const T exc = a.ex[o], exk = a.ex[o + 1], exj = a.ex[o + sz];
         T hx = a.hx[o], hy = a.hy[o], hz = a.hz[o];
-        h_cell<T, CPML>(a, a.ph, true, i, j, k, jk, ezc, ezn, ezj, eyc, eyn, eyk, exc, exk, exj, dhy, dhz, chy, chz,
-                        hx, hy, hz);
+        const T cx = sizeof(T) == 4 ? a.cdhx[i] : a.den_hx[i];
+        h_cell<T, CPML>(a, a.ph, true, i, j, k, jk, ezc, ezn, ezj, eyc, eyn, eyk, exc, exk, exj, cx, dhy, dhz, chy,
+                        chz, hx, hy, hz);
         a.hx[o] = hx; a.hy[o] = hy; a.hz[o] = hz;
         ezc = ezn; eyc = eyn;
     }
@@ -464,20 +491,19 @@ __device__ __forceinline__ T medium_update(T e, T rot, T vac, int m, const MedDe
 
 // E of one cell, in place in ex/ey/ez: bulk curl of H with the medium
 // update, then the CPML psi recursions (psi of E in place in a.pe).  tg is
-// the cell's class tag (0: vacuum in all three components).
-template <typename T, bool CPML, bool MEDIA>
-__device__ __forceinline__ void e_cell(const YeeArgs<T>& a, int tg, int i, int j, int k, long o, long jk,
-                                       T hzc, T hzj, T hzi, T hyc, T hyk, T hyi, T hxc, T hxj, T hxk,
-                                       T dey, T dez, T cey, T cez, T& ex, T& ey, T& ez) {
-    const T dep = a.dt_eps0;
-    const T dex = a.den_ex[i];
+// the cell's class tag (0: vacuum in all three components); dex = den_ex[i],
+// cex_ = cdex[i] (FP32 only).
+template <typename T, bool MEDIA>
+__device__ __forceinline__ void e_bulk(const YeeArgs<T>& a, int tg, long o, T dep, T hzc, T hzj, T hzi, T hyc,
+                                       T hyk, T hyi, T hxc, T hxj, T hxk, T dex, T cex_, T dey, T dez, T cey, T cez,
+                                       T& ex, T& ey, T& ez) {
     const T rx = (hzc - hzj) * dey + (hyk - hyc) * dez;
     const T ry = (hzi - hzc) * dex + (hxc - hxk) * dez;
     const T rz = (hyc - hyi) * dex + (hxj - hxc) * dey;
     // vacuum increments; FP32 folds dt_eps0*den into one coefficient per axis
     T vx, vy, vz;
     if constexpr (sizeof(T) == 4) {
-        const T cex = a.cdex[i];
+        const T cex = cex_;
         vx = (hzc - hzj) * cey + (hyk - hyc) * cez;
         vy = (hzi - hzc) * cex + (hxc - hxk) * cez;
         vz = (hyc - hyi) * cex + (hxj - hxc) * cey;
@@ -496,63 +522,93 @@ __device__ __forceinline__ void e_cell(const YeeArgs<T>& a, int tg, int i, int j
     } else {
         ex = ex + vx; ey = ey + vy; ez = ez + vz;
     }
+}
+
+// x faces: psix_Eyx, psix_Ezx (low side i = 2..npml)
+template <typename T>
+__device__ __forceinline__ void e_cpml_x(const YeeArgs<T>& a, int i, long jk, T dep, T hzc, T hzi, T hyc, T hyi,
+                                         T& ey, T& ez) {
+    const unsigned cp = a.cpml;
+    const int npml = a.npml;
+    if ((cp & 1u) && i >= 2 && i <= npml) {
+        const int c = i;
+        const long ps = (long)(i - 1) * a.sx + jk;
+        const T p0 = psi_step(a.pe[0][ps], a.be[c], a.ced[c], hzi - hzc);
+        const T p1 = psi_step(a.pe[1][ps], a.be[c], a.ced[c], hyc - hyi);
+        a.pe[0][ps] = p0; a.pe[1][ps] = p1;
+        ey = ey + dep * p0; ez = ez + dep * p1;
+    }
+    if ((cp & 2u) && i >= a.nx + 1 - npml && i <= a.nx - 1) {
+        const int c = a.nx + 1 - i;
+        const long ps = (long)(npml - 1 + c) * a.sx + jk;
+        const T p0 = psi_step(a.pe[0][ps], a.be[c], a.ced[c], hzi - hzc);
+        const T p1 = psi_step(a.pe[1][ps], a.be[c], a.ced[c], hyc - hyi);
+        a.pe[0][ps] = p0; a.pe[1][ps] = p1;
+        ey = ey + dep * p0; ez = ez + dep * p1;
+    }
+}
+
+// y faces: psiy_Exy, psiy_Ezy
+template <typename T>
+__device__ __forceinline__ void e_cpml_y(const YeeArgs<T>& a, int i, int j, int k, T dep, T hzc, T hzj, T hxc, T hxj,
+                                         T& ex, T& ez) {
+    const unsigned cp = a.cpml;
+    const int npml = a.npml;
+    const int nse = 2 * npml;
+    if ((cp & 4u) && j >= 2 && j <= npml) {
+        const int c = j;
+        const long ps = ((long)i * nse + (j - 1)) * a.sz + k + a.off;
+        const T p0 = psi_step(a.pe[2][ps], a.be[c], a.ced[c], hzc - hzj);
+        const T p1 = psi_step(a.pe[3][ps], a.be[c], a.ced[c], hxj - hxc);
+        a.pe[2][ps] = p0; a.pe[3][ps] = p1;
+        ex = ex + dep * p0; ez = ez + dep * p1;
+    }
+    if ((cp & 8u) && j >= a.ny + 1 - npml && j <= a.ny - 1) {
+        const int c = a.ny + 1 - j;
+        const long ps = ((long)i * nse + (npml - 1 + c)) * a.sz + k + a.off;
+        const T p0 = psi_step(a.pe[2][ps], a.be[c], a.ced[c], hzc - hzj);
+        const T p1 = psi_step(a.pe[3][ps], a.be[c], a.ced[c], hxj - hxc);
+        a.pe[2][ps] = p0; a.pe[3][ps] = p1;
+        ex = ex + dep * p0; ez = ez + dep * p1;
+    }
+}
+
+// z faces: psiz_Exz, psiz_Eyz
+template <typename T>
+__device__ __forceinline__ void e_cpml_z(const YeeArgs<T>& a, int i, int j, int k, T dep, T hyc, T hyk, T hxc, T hxk,
+                                         T& ex, T& ey) {
+    const unsigned cp = a.cpml;
+    const int npml = a.npml;
+    const int nse = 2 * npml;
+    if ((cp & 16u) && k >= 2 && k <= npml) {
+        const int c = k;
+        const long ps = ((long)i * a.sy + j) * nse + (k - 1);
+        const T p0 = psi_step(a.pe[4][ps], a.be[c], a.ced[c], hyk - hyc);
+        const T p1 = psi_step(a.pe[5][ps], a.be[c], a.ced[c], hxc - hxk);
+        a.pe[4][ps] = p0; a.pe[5][ps] = p1;
+        ex = ex + dep * p0; ey = ey + dep * p1;
+    }
+    if ((cp & 32u) && k >= a.nz + 1 - npml && k <= a.nz - 1) {
+        const int c = a.nz + 1 - k;
+        const long ps = ((long)i * a.sy + j) * nse + (npml - 1 + c);
+        const T p0 = psi_step(a.pe[4][ps], a.be[c], a.ced[c], hyk - hyc);
+        const T p1 = psi_step(a.pe[5][ps], a.be[c], a.ced[c], hxc - hxk);
+        a.pe[4][ps] = p0; a.pe[5][ps] = p1;
+        ex = ex + dep * p0; ey = ey + dep * p1;
+    }
+}
+
+template <typename T, bool CPML, bool MEDIA>
+__device__ __forceinline__ void e_cell(const YeeArgs<T>& a, int tg, int i, int j, int k, long o, long jk,
+                                       T hzc, T hzj, T hzi, T hyc, T hyk, T hyi, T hxc, T hxj, T hxk,
+                                       T dex, T cex_, T dey, T dez, T cey, T cez, T& ex, T& ey, T& ez) {
+    const T dep = a.dt_eps0;
+    e_bulk<T, MEDIA>(a, tg, o, dep, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dex, cex_, dey, dez, cey, cez, ex, ey,
+                     ez);
     if constexpr (CPML) {
-        const unsigned cp = a.cpml;
-        const int npml = a.npml;
-        const int nse = 2 * npml;
-        const long sx = a.sx;
-        const int sz = a.sz;
-        // x faces: psix_Eyx, psix_Ezx (low side i = 2..npml)
-        if ((cp & 1u) && i >= 2 && i <= npml) {
-            const int c = i;
-            const long ps = (long)(i - 1) * sx + jk;
-            const T p0 = psi_step(a.pe[0][ps], a.be[c], a.ced[c], hzi - hzc);
-            const T p1 = psi_step(a.pe[1][ps], a.be[c], a.ced[c], hyc - hyi);
-            a.pe[0][ps] = p0; a.pe[1][ps] = p1;
-            ey = ey + dep * p0; ez = ez + dep * p1;
-        }
-        if ((cp & 2u) && i >= a.nx + 1 - npml && i <= a.nx - 1) {
-            const int c = a.nx + 1 - i;
-            const long ps = (long)(npml - 1 + c) * sx + jk;
-            const T p0 = psi_step(a.pe[0][ps], a.be[c], a.ced[c], hzi - hzc);
-            const T p1 = psi_step(a.pe[1][ps], a.be[c], a.ced[c], hyc - hyi);
-            a.pe[0][ps] = p0; a.pe[1][ps] = p1;
-            ey = ey + dep * p0; ez = ez + dep * p1;
-        }
-        // y faces: psiy_Exy, psiy_Ezy
-        if ((cp & 4u) && j >= 2 && j <= npml) {
-            const int c = j;
-            const long ps = ((long)i * nse + (j - 1)) * sz + k + a.off;
-            const T p0 = psi_step(a.pe[2][ps], a.be[c], a.ced[c], hzc - hzj);
-            const T p1 = psi_step(a.pe[3][ps], a.be[c], a.ced[c], hxj - hxc);
-            a.pe[2][ps] = p0; a.pe[3][ps] = p1;
-            ex = ex + dep * p0; ez = ez + dep * p1;
-        }
-        if ((cp & 8u) && j >= a.ny + 1 - npml && j <= a.ny - 1) {
-            const int c = a.ny + 1 - j;
-            const long ps = ((long)i * nse + (npml - 1 + c)) * sz + k + a.off;
-            const T p0 = psi_step(a.pe[2][ps], a.be[c], a.ced[c], hzc - hzj);
-            const T p1 = psi_step(a.pe[3][ps], a.be[c], a.ced[c], hxj - hxc);
-            a.pe[2][ps] = p0; a.pe[3][ps] = p1;
-            ex = ex + dep * p0; ez = ez + dep * p1;
-        }
-        // z faces: psiz_Exz, psiz_Eyz
-        if ((cp & 16u) && k >= 2 && k <= npml) {
-            const int c = k;
-            const long ps = ((long)i * a.sy + j) * nse + (k - 1);
-            const T p0 = psi_step(a.pe[4][ps], a.be[c], a.ced[c], hyk - hyc);
-            const T p1 = psi_step(a.pe[5][ps], a.be[c], a.ced[c], hxc - hxk);
-            a.pe[4][ps] = p0; a.pe[5][ps] = p1;
-            ex = ex + dep * p0; ey = ey + dep * p1;
-        }
-        if ((cp & 32u) && k >= a.nz + 1 - npml && k <= a.nz - 1) {
-            const int c = a.nz + 1 - k;
-            const long ps = ((long)i * a.sy + j) * nse + (npml - 1 + c);
-            const T p0 = psi_step(a.pe[4][ps], a.be[c], a.ced[c], hyk - hyc);
-            const T p1 = psi_step(a.pe[5][ps], a.be[c], a.ced[c], hxc - hxk);
-            a.pe[4][ps] = p0; a.pe[5][ps] = p1;
-            ex = ex + dep * p0; ey = ey + dep * p1;
-        }
+        e_cpml_x(a, i, jk, dep, hzc, hzi, hyc, hyi, ey, ez);
+        e_cpml_y(a, i, j, k, dep, hzc, hzj, hxc, hxj, ex, ez);
+        e_cpml_z(a, i, j, k, dep, hyc, hyk, hxc, hxk, ex, ey);
     }
 }
 
@@ -580,8 +636,9 @@ __device__ __forceinline__ void e_tile(const YeeArgs<T>& a, const int t) {
         const T hyk = a.hy[o - 1], hxk = a.hx[o - 1];
         T ex = a.ex[o], ey = a.ey[o], ez = a.ez[o];
         const int tg = (MEDIA && a.tag != nullptr) ? (int)a.tag[o] : 0;
-        e_cell<T, CPML, MEDIA>(a, tg, i, j, k, o, jk, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dey, dez, cey,
-                               cez, ex, ey, ez);
+        const T dex = a.den_ex[i], cex = sizeof(T) == 4 ? a.cdex[i] : (T)0;
+        e_cell<T, CPML, MEDIA>(a, tg, i, j, k, o, jk, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dex, cex, dey, dez,
+                               cey, cez, ex, ey, ez);
         a.ex[o] = ex; a.ey[o] = ey; a.ez[o] = ez;
         hzi = hzc; hyi = hyc;
     }
@@ -602,172 +659,296 @@ __global__ __launch_bounds__(256) void k_e_update(const YeeArgs<T> a) {
 // from the current buffers and written to the other pair, so every array is
 // read once and written once per step (12 words per cell instead of the 18
 // of separate H and E passes).  A workgroup is 64 lanes along z times FR
-// rows along y and marches an x chunk.  Lane 0 and row 0 are halos: there H
-// is recomputed (from the unmodified input buffers) so that the E update of
-// the owned 63 x (FR-1) cells has all its H neighbours; the plane before the
-// chunk is an x halo in the same way.  H rows travel through LDS, H lanes
-// through cross-lane shuffles, the previous x plane in registers.  The psi of
-// H is double-buffered like the fields (halo cells read it); psi of E and the
-// dispersive state are touched only by their owner.  The TF/SF corrections
-// of H are applied inside, where E needs them.
+// rows along y (one wavefront per row) and marches an x chunk.  Lane 0 and
+// row 0 are halos: there H is recomputed (from the unmodified input buffers)
+// so that the E update of the owned 63 x (FR-1) cells has all its H
+// neighbours; the plane before the chunk is an x halo in the same way.  The
+// psi of H is double-buffered like the fields (halo cells read it); psi of E
+// and the dispersive state are touched only by their owner.  The TF/SF
+// corrections of H are applied inside, where E needs them.
 //
-// Pipelining: the loads of plane i+1 (E two planes ahead, H, tag, the edge
-// row and lane) are issued at the top of iteration i and consumed in
-// iteration i+1; the E rows of plane i+1 go to LDS in iteration i, so one
-// barrier per plane orders both exchanges (E rows and H rows each alternate
-// between two LDS buffers by plane parity).
+// Data movement per plane i: E of plane i and its j+1 / k+1 neighbours come
+// from LDS (written one plane earlier), H rows of plane i go through LDS for
+// the E update, H lanes through cross-lane shuffles, the previous plane in
+// registers.  Everything of plane i+1 that comes from HBM (E two planes
+// ahead, H, tag, edge row and lane, z-face psi) is loaded at the top of
+// iteration i into the idle one of two register sets; the loop is unrolled
+// by two so that the sets alternate without copies and the loads stay in
+// flight across the plane's barrier.
+//
+// Arguments: the per-plane hot set is passed by value; everything the
+// x/y-face CPML, the dispersive media and the TF/SF corrections need sits in
+// a FusedCold block in device memory and is read with scalar loads on those
+// paths only, which keeps the hot loop's scalar registers from spilling.
 // ---------------------------------------------------------------------------
 template <typename T>
 struct FusedArgs {
-    T *exo, *eyo, *ezo, *hxo, *hyo, *hzo;   // output buffers
-    T *pho[6];                              // output psi of H
-    int ntz, nty;                           // tiles: 63 lanes x (FR-1) rows
+    const T *exi, *eyi, *ezi, *hxi, *hyi, *hzi;   // current fields
+    T *exo, *eyo, *ezo, *hxo, *hyo, *hzo;         // fields after the step
+    const uint8_t* tag;
+    const T *cx, *dex, *cex;                      // plane tables: cdhx|den_hx, den_ex, cdex|unused
+    const T* phz[2];                              // psiz_Hxz, psiz_Hyz (current)
+    T* phzo[2];                                   // psiz_Hxz, psiz_Hyz (after the step)
+    T* pez[2];                                    // psiz_Exz, psiz_Eyz (in place)
+    T dt_mu0, dt_eps0;
+    long sx;
+    int nx, ny, nz, sy, sz, off, npml, xchunk, ntz, nty;
+    unsigned cpml;
+};
+
+template <typename T>
+struct FusedCold {
+    YeeArgs<T> a;                           // current fields / psi, tables, media
+    T* pho[6];                              // output psi of H
     int ti0, ti1, tj0, tj1, tk0, tk1, tkb;  // TF/SF box and 1D line origin
     const double* ex1;                      // incident Ex on the 1D line
     double dt_mu0;
 };
 
-// Values of one x plane that a fused workgroup loads ahead of use.
+// Field access by 32-bit byte offset from a uniform base pointer (the host
+// only takes the fused path when every field array is below 4 GiB).
+template <typename T>
+__device__ __forceinline__ T ldb(const T* p, uint32_t b) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + b);
+}
+template <typename T>
+__device__ __forceinline__ void stb(T* p, uint32_t b, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + b) = v;
+}
+// Uniform read of a table the kernel never writes (scalar load).
+template <typename T>
+__device__ __forceinline__ T ldc(const T* p, int i) {
+    return ((const __attribute__((address_space(4))) T*)p)[i];
+}
+
+// Values of one x plane p that a fused workgroup loads one plane ahead.
 template <typename T>
 struct FusedPlane {
-    T ex2, ey2, ez2;   // E two planes ahead
-    T hx, hy, hz;      // H of the next plane
-    T ezj, exj;        // E of the next plane at j+1 (last row only)
-    T eyk, exk;        // E of the next plane at k+1 (last lane only)
-    int tg;            // class tag of the next plane
+    T ex2, ey2, ez2;   // E of plane p+1
+    T hx, hy, hz;      // H of plane p
+    T ezj, exj;        // E of plane p at j+1 (last row; others: own cell)
+    T eyk, exk;        // E of plane p at k+1 (last lane; others: own cell)
+    T pzh0, pzh1;      // psiz_Hxz, psiz_Hyz of plane p (z-face lanes)
+    T pze0, pze1;      // psiz_Exz, psiz_Eyz of plane p (z-face lanes)
+    int tg;            // class tag of plane p
 };
 
-template <typename T, bool MEDIA>
-__device__ __forceinline__ void fused_load(const YeeArgs<T>& a, FusedPlane<T>& p, long o, bool inb, bool last_row,
-                                           bool row_ok, bool last_lane, bool lane_ok, int sz) {
-    // o: offset of plane ip (the next plane) at this lane's (j, k)
-    const long sx = a.sx;
-    p.ex2 = p.ey2 = p.ez2 = (T)0;
-    p.hx = p.hy = p.hz = (T)0;
-    p.ezj = p.exj = p.eyk = p.exk = (T)0;
-    p.tg = 0;
-    if (inb) {
-        p.ex2 = a.ex[o + sx]; p.ey2 = a.ey[o + sx]; p.ez2 = a.ez[o + sx];
-        p.hx = a.hx[o]; p.hy = a.hy[o]; p.hz = a.hz[o];
-        if constexpr (MEDIA) p.tg = a.tag[o];
+// Per-thread constants of the fused step.
+template <typename T>
+struct FusedLane {
+    int i0, i1, j, k, lane, row, me, plane;
+    bool hcell, own, last_row, last_lane, zh, ze;
+    long jk;                 // element offset of (j, k) in a plane, clamped into the grid
+    uint32_t sxb, side_rb, side_lb;
+    uint32_t zhb, zeb;       // byte offsets of this lane's z-face psi of H / E in plane i0-1
+    uint32_t zhs, zes;       // their plane strides in bytes
+    T dhy, dhz, dey, dez, chy, chz, cey, cez;
+    T zbh, zch, zbe, zce;    // z-face CPML coefficients of this lane (H, E)
+};
+
+// All loads are unconditional (no value depends on a branch, so the two
+// register sets of the pipeline need no copies): lanes and rows past the
+// grid load a clamped in-bounds cell, the edge-neighbour loads of the other
+// lanes and rows read their own cell, lanes outside the z faces read psi at
+// offset 0 of their (i, j) row, and the last plane of a chunk reloads itself.
+template <typename T, bool CPML, bool MEDIA>
+__device__ __forceinline__ void fused_load(const FusedArgs<T>& f, const FusedLane<T>& c, FusedPlane<T>& p,
+                                           uint32_t ob, long o, uint32_t zh, uint32_t ze) {
+    // ob / o: byte / element offset of plane p at this lane's (clamped) (j, k)
+    p.ex2 = ldb(f.exi, ob + c.sxb); p.ey2 = ldb(f.eyi, ob + c.sxb); p.ez2 = ldb(f.ezi, ob + c.sxb);
+    p.hx = ldb(f.hxi, ob); p.hy = ldb(f.hyi, ob); p.hz = ldb(f.hzi, ob);
+    if constexpr (MEDIA) p.tg = f.tag[o];
+    else p.tg = 0;
+    p.ezj = ldb(f.ezi, ob + c.side_rb); p.exj = ldb(f.exi, ob + c.side_rb);
+    p.eyk = ldb(f.eyi, ob + c.side_lb); p.exk = ldb(f.exi, ob + c.side_lb);
+    if constexpr (CPML) {
+        p.pzh0 = ldb(f.phz[0], zh); p.pzh1 = ldb(f.phz[1], zh);
+        p.pze0 = ldb(f.pez[0], ze); p.pze1 = ldb(f.pez[1], ze);
+    } else {
+        p.pzh0 = p.pzh1 = p.pze0 = p.pze1 = (T)0;
     }
-    if (last_row && row_ok) { p.ezj = a.ez[o + sz]; p.exj = a.ex[o + sz]; }
-    if (last_lane && lane_ok) { p.eyk = a.ey[o + 1]; p.exk = a.ex[o + 1]; }
+}
+
+// One plane i of the fused step.  cur holds plane i (loaded one plane
+// earlier), nxt receives plane i+1.  LDS parity P: E rows of plane i in
+// buffer P, E rows of plane i+1 go to 1-P, H rows of plane i to P.
+template <typename T, bool CPML, bool MEDIA, bool TFSF, int P>
+__device__ __forceinline__ void fused_plane(const FusedArgs<T>& f, const FusedCold<T>* __restrict__ cold,
+                                            const FusedLane<T>& c, T* lds, int i, long o, uint32_t ob, uint32_t zh,
+                                            uint32_t ze, const FusedPlane<T>& cur, FusedPlane<T>& nxt, T& hzi,
+                                            T& hyi) {
+    const int pl = c.plane;
+    {
+        const bool more = i < c.i1;
+        fused_load<T, CPML, MEDIA>(f, c, nxt, ob + (more ? c.sxb : 0u), o + (more ? f.sx : 0),
+                                   zh + (more ? c.zhs : 0u), ze + (more ? c.zes : 0u));
+    }
+    // E of plane i and its j+1 / k+1 neighbours from LDS (edges from cur)
+    const int e0 = P * 3 * pl + c.me;
+    const T ezc = lds[e0], exc = lds[e0 + pl], eyc = lds[e0 + 2 * pl];
+    T ezj, exj, eyk, exk;
+    if (!c.last_row) { ezj = lds[e0 + kLanes]; exj = lds[e0 + pl + kLanes]; }
+    else { ezj = cur.ezj; exj = cur.exj; }
+    if (!c.last_lane) { eyk = lds[e0 + 2 * pl + 1]; exk = lds[e0 + pl + 1]; }
+    else { eyk = cur.eyk; exk = cur.exk; }
+    T hx = cur.hx, hy = cur.hy, hz = cur.hz;
+    const bool store = c.own && i >= c.i0;
+    const int j = c.j, k = c.k;
+    const T dmu = f.dt_mu0;
+    if (c.hcell && i >= 1) {
+        h_bulk(dmu, ldc(f.cx, i), c.dhy, c.dhz, c.chy, c.chz, ezc, cur.ez2, ezj, eyc, cur.ey2, eyk, exc, exk, exj,
+               hx, hy, hz);
+        if constexpr (CPML) {
+            if (f.cpml & 15u) {
+                const YeeArgs<T>& a = cold->a;
+                h_cpml_x(a, cold->pho, store, i, c.jk, dmu, ezc, cur.ez2, eyc, cur.ey2, hy, hz);
+                h_cpml_y(a, cold->pho, store, i, j, k, dmu, ezc, ezj, exc, exj, hx, hz);
+            }
+            if (c.zh) {
+                const T p0 = psi_step(cur.pzh0, c.zbh, c.zch, eyk - eyc);
+                const T p1 = psi_step(cur.pzh1, c.zbh, c.zch, exc - exk);
+                if (store) { stb(f.phzo[0], zh, p0); stb(f.phzo[1], zh, p1); }
+                hx = hx + dmu * p0; hy = hy + dmu * p1;
+            }
+        }
+        if constexpr (TFSF) {
+            const FusedCold<T>& q = *cold;
+            if (i >= q.ti0 && i <= q.ti1 - 1) {
+                if (j >= q.tj0 && j <= q.tj1) {
+                    if (k == q.tk0 - 1)
+                        hy = (T)((double)hy + q.dt_mu0 * (double)q.a.den_hz[k] * q.ex1[q.tk0 - q.tkb]);
+                    else if (k == q.tk1)
+                        hy = (T)((double)hy - q.dt_mu0 * (double)q.a.den_hz[k] * q.ex1[q.tk1 - q.tkb]);
+                }
+                if (k >= q.tk0 && k <= q.tk1) {
+                    if (j == q.tj0 - 1)
+                        hz = (T)((double)hz - q.dt_mu0 * (double)q.a.den_hy[j] * q.ex1[k - q.tkb]);
+                    else if (j == q.tj1)
+                        hz = (T)((double)hz + q.dt_mu0 * (double)q.a.den_hy[j] * q.ex1[k - q.tkb]);
+                }
+            }
+        }
+    }
+    if (store) { stb(f.hxo, ob, hx); stb(f.hyo, ob, hy); stb(f.hzo, ob, hz); }
+    {
+        const int h0 = 6 * pl + P * 2 * pl + c.me;
+        lds[h0] = hz;
+        lds[h0 + pl] = hx;
+        const int e1 = (1 - P) * 3 * pl + c.me;
+        lds[e1] = cur.ez2;
+        lds[e1 + pl] = cur.ex2;
+        lds[e1 + 2 * pl] = cur.ey2;
+    }
+    __syncthreads();
+    const T hyk = __shfl_up(hy, 1), hxk = __shfl_up(hx, 1);
+    if (store) {
+        const int h0 = 6 * pl + P * 2 * pl + c.me - kLanes;
+        const T hzj = lds[h0], hxj = lds[h0 + pl];
+        T ex = exc, ey = eyc, ez = ezc;
+        const T dep = f.dt_eps0;
+        e_bulk<T, MEDIA>(cold->a, cur.tg, o, dep, hz, hzj, hzi, hy, hyk, hyi, hx, hxj, hxk, ldc(f.dex, i),
+                         sizeof(T) == 4 ? ldc(f.cex, i) : (T)0, c.dey, c.dez, c.cey, c.cez, ex, ey, ez);
+        if constexpr (CPML) {
+            if (f.cpml & 15u) {
+                const YeeArgs<T>& a = cold->a;
+                e_cpml_x(a, i, c.jk, dep, hz, hzi, hy, hyi, ey, ez);
+                e_cpml_y(a, i, j, k, dep, hz, hzj, hx, hxj, ex, ez);
+            }
+            if (c.ze) {
+                const T p0 = psi_step(cur.pze0, c.zbe, c.zce, hyk - hy);
+                const T p1 = psi_step(cur.pze1, c.zbe, c.zce, hx - hxk);
+                stb(f.pez[0], ze, p0); stb(f.pez[1], ze, p1);
+                ex = ex + dep * p0; ey = ey + dep * p1;
+            }
+        }
+        stb(f.exo, ob, ex); stb(f.eyo, ob, ey); stb(f.ezo, ob, ez);
+    }
+    hzi = hz; hyi = hy;
 }
 
 template <typename T, bool CPML, bool MEDIA, bool TFSF>
-__global__ __launch_bounds__(512) void k_step_fused(const YeeArgs<T> a, const FusedArgs<T> f) {
+__global__ __launch_bounds__(512) void k_step_fused(const FusedArgs<T> f, const FusedCold<T>* __restrict__ cold) {
     extern __shared__ unsigned char omx_lds[];
     T* lds = reinterpret_cast<T*>(omx_lds);
+    FusedLane<T> c;
     const int FR = blockDim.y;
-    const int lane = threadIdx.x, row = threadIdx.y;
+    c.lane = threadIdx.x; c.row = threadIdx.y;
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     const int tz = t % f.ntz;
     const int q = t / f.ntz;
     const int ty = q % f.nty;
     const int tx = q / f.nty;
-    const int k = tz * (kLanes - 1) + lane;        // lane 0: halo at k0-1
-    const int j = ty * (FR - 1) + row;             // row 0: halo at j0-1
-    const int i0 = tx * a.xchunk + 1;
-    const int i1 = min(i0 + a.xchunk - 1, a.nx);
-    const bool inb = k <= a.nz + 1 && j <= a.ny + 1;
-    const bool hcell = k >= 1 && k <= a.nz && j >= 1 && j <= a.ny;
-    const bool own = hcell && lane >= 1 && row >= 1;
-    const bool last_row = row == FR - 1, last_lane = lane == kLanes - 1;
-    const bool row_ok = k <= a.nz + 1 && j + 1 <= a.ny + 1;
-    const bool lane_ok = k + 1 <= a.nz + 1 && j <= a.ny + 1;
-    const long sx = a.sx;
-    const int sz = a.sz;
-    const long jk = (long)j * sz + k + a.off;
-    const int jc = min(j, a.ny + 1), kc = min(k, a.nz + 1);
-    const T dhy = a.den_hy[jc], dhz = a.den_hz[kc], dey = a.den_ey[jc], dez = a.den_ez[kc];
-    const T chy = sizeof(T) == 4 ? a.cdhy[jc] : (T)0, chz = sizeof(T) == 4 ? a.cdhz[kc] : (T)0;
-    const T cey = sizeof(T) == 4 ? a.cdey[jc] : (T)0, cez = sizeof(T) == 4 ? a.cdez[kc] : (T)0;
-    const int plane = FR * kLanes;
-    const int me = row * kLanes + lane;
-    T* const EA[2] = {lds, lds + 2 * plane};            // E rows (ez, ex) by plane parity
-    T* const HB[2] = {lds + 4 * plane, lds + 6 * plane}; // H rows (hz, hx) by plane parity
-
-    // prologue: plane i0-1 and the E of plane i0
-    long o = (long)(i0 - 1) * sx + jk;
-    T exc = (T)0, eyc = (T)0, ezc = (T)0, exn = (T)0, eyn = (T)0, ezn = (T)0;
-    T hx = (T)0, hy = (T)0, hz = (T)0, ezj = (T)0, exj = (T)0, eyk_e = (T)0, exk_e = (T)0;
-    int tg = 0;
-    if (inb) {
-        exc = a.ex[o]; eyc = a.ey[o]; ezc = a.ez[o];
-        exn = a.ex[o + sx]; eyn = a.ey[o + sx]; ezn = a.ez[o + sx];
-        hx = a.hx[o]; hy = a.hy[o]; hz = a.hz[o];
-        if constexpr (MEDIA) tg = a.tag[o];
-    }
-    if (last_row && row_ok) { ezj = a.ez[o + sz]; exj = a.ex[o + sz]; }
-    if (last_lane && lane_ok) { eyk_e = a.ey[o + 1]; exk_e = a.ex[o + 1]; }
+    c.k = tz * (kLanes - 1) + c.lane;        // lane 0: halo at k0-1
+    c.j = ty * (FR - 1) + c.row;             // row 0: halo at j0-1
+    c.i0 = tx * f.xchunk + 1;
+    c.i1 = min(c.i0 + f.xchunk - 1, f.nx);
+    c.hcell = c.k >= 1 && c.k <= f.nz && c.j >= 1 && c.j <= f.ny;
+    c.own = c.hcell && c.lane >= 1 && c.row >= 1;
+    c.last_row = c.row == FR - 1;
+    c.last_lane = c.lane == kLanes - 1;
+    // lanes/rows past the grid work on a clamped cell: their values are
+    // never used by an owned cell, and every address stays in bounds
+    const int jc = min(c.j, f.ny + 1), kc = min(c.k, f.nz + 1);
+    c.jk = (long)jc * f.sz + kc + f.off;
+    c.sxb = (uint32_t)(f.sx * (long)sizeof(T));
+    c.side_rb = (c.last_row && jc + 1 <= f.ny + 1) ? (uint32_t)(f.sz * sizeof(T)) : 0u;
+    c.side_lb = (c.last_lane && kc + 1 <= f.nz + 1) ? (uint32_t)sizeof(T) : 0u;
     {
-        T* E = EA[(i0 - 1) & 1];
-        E[me] = ezc;
-        E[plane + me] = exc;
+        const YeeArgs<T>& a = cold->a;
+        c.dhy = a.den_hy[jc]; c.dhz = a.den_hz[kc]; c.dey = a.den_ey[jc]; c.dez = a.den_ez[kc];
+        c.chy = sizeof(T) == 4 ? a.cdhy[jc] : (T)0; c.chz = sizeof(T) == 4 ? a.cdhz[kc] : (T)0;
+        c.cey = sizeof(T) == 4 ? a.cdey[jc] : (T)0; c.cez = sizeof(T) == 4 ? a.cdez[kc] : (T)0;
+        c.zh = c.ze = false;
+        c.zbh = c.zch = c.zbe = c.zce = (T)0;
+        c.zhb = c.zeb = c.zhs = c.zes = 0u;
+        if constexpr (CPML) {
+            // z faces: H for k in [1, npml-1] and [nz+1-npml, nz-1] (coefficient
+            // index k resp. nz-k, psi slot k-1 resp. npml-2+nz-k); E for k in
+            // [2, npml] and [nz+1-npml, nz-1] (index k resp. nz+1-k, slot k-1
+            // resp. npml-1+nz+1-k)
+            const int npml = f.npml, nz = f.nz, k = c.k;
+            const int nsh = 2 * (npml - 1), nse = 2 * npml;
+            int ch = -1, sh = 0, ce = -1, se = 0;
+            if (c.hcell) {
+                if ((f.cpml & 16u) && k <= npml - 1) { ch = k; sh = k - 1; }
+                else if ((f.cpml & 32u) && k >= nz + 1 - npml && k <= nz - 1) { ch = nz - k; sh = npml - 2 + nz - k; }
+                if ((f.cpml & 16u) && k >= 2 && k <= npml) { ce = k; se = k - 1; }
+                else if ((f.cpml & 32u) && k >= nz + 1 - npml && k <= nz - 1) { ce = nz + 1 - k; se = npml - 1 + nz + 1 - k; }
+            }
+            c.zh = ch >= 0; c.ze = ce >= 0;
+            if (c.zh) { c.zbh = a.bh[ch]; c.zch = a.chd[ch]; }
+            if (c.ze) { c.zbe = a.be[ce]; c.zce = a.ced[ce]; }
+            const long row_h = ((long)(c.i0 - 1) * f.sy + jc) * nsh + sh;
+            const long row_e = ((long)(c.i0 - 1) * f.sy + jc) * nse + se;
+            c.zhb = (uint32_t)(row_h * (long)sizeof(T));
+            c.zeb = (uint32_t)(row_e * (long)sizeof(T));
+            c.zhs = (uint32_t)((long)f.sy * nsh * (long)sizeof(T));
+            c.zes = (uint32_t)((long)f.sy * nse * (long)sizeof(T));
+        }
     }
+    c.plane = FR * kLanes;
+    c.me = c.row * kLanes + c.lane;
+    // LDS: E rows (ez, ex, ey) of a plane at 3*plane*parity, H rows (hz, hx) at
+    // 6*plane + 2*plane*parity.  Integer offsets keep every access a ds_ op.
+
+    // prologue: E of plane i0-1 into LDS buffer 0, plane i0-1 into A
+    long o = (long)(c.i0 - 1) * f.sx + c.jk;
+    uint32_t ob = (uint32_t)(o * (long)sizeof(T));
+    uint32_t zh = c.zhb, ze = c.zeb;
+    FusedPlane<T> A, B;
+    lds[c.me] = ldb(f.ezi, ob);
+    lds[c.me + c.plane] = ldb(f.exi, ob);
+    lds[c.me + 2 * c.plane] = ldb(f.eyi, ob);
+    fused_load<T, CPML, MEDIA>(f, c, A, ob, o, zh, ze);
     __syncthreads();
     T hzi = (T)0, hyi = (T)0;
-    for (int i = i0 - 1; i <= i1; ++i, o += sx) {
-        // loads for plane i+1, consumed next iteration
-        FusedPlane<T> nx_;
-        if (i < i1) fused_load<T, MEDIA>(a, nx_, o + sx, inb, last_row, row_ok, last_lane, lane_ok, sz);
-        // E neighbours of plane i: rows from LDS, lanes by shuffle
-        {
-            const T* E = EA[i & 1];
-            if (!last_row) { ezj = E[me + kLanes]; exj = E[plane + me + kLanes]; }
-        }
-        T eyk = __shfl_down(eyc, 1), exk = __shfl_down(exc, 1);
-        if (last_lane) { eyk = eyk_e; exk = exk_e; }
-        const bool store = own && i >= i0;
-        if (hcell && i >= 1) {
-            h_cell<T, CPML>(a, f.pho, store, i, j, k, jk, ezc, ezn, ezj, eyc, eyn, eyk, exc, exk, exj,
-                            dhy, dhz, chy, chz, hx, hy, hz);
-            if constexpr (TFSF) {
-                if (i >= f.ti0 && i <= f.ti1 - 1) {
-                    if (j >= f.tj0 && j <= f.tj1) {
-                        if (k == f.tk0 - 1)
-                            hy = (T)((double)hy + f.dt_mu0 * (double)a.den_hz[k] * f.ex1[f.tk0 - f.tkb]);
-                        else if (k == f.tk1)
-                            hy = (T)((double)hy - f.dt_mu0 * (double)a.den_hz[k] * f.ex1[f.tk1 - f.tkb]);
-                    }
-                    if (k >= f.tk0 && k <= f.tk1) {
-                        if (j == f.tj0 - 1)
-                            hz = (T)((double)hz - f.dt_mu0 * (double)a.den_hy[j] * f.ex1[k - f.tkb]);
-                        else if (j == f.tj1)
-                            hz = (T)((double)hz + f.dt_mu0 * (double)a.den_hy[j] * f.ex1[k - f.tkb]);
-                    }
-                }
-            }
-        }
-        if (store) { f.hxo[o] = hx; f.hyo[o] = hy; f.hzo[o] = hz; }
-        {
-            T* H = HB[i & 1];
-            H[me] = hz;
-            H[plane + me] = hx;
-            T* E = EA[(i + 1) & 1];
-            E[me] = ezn;
-            E[plane + me] = exn;
-        }
-        __syncthreads();
-        const T hyk = __shfl_up(hy, 1), hxk = __shfl_up(hx, 1);
-        if (store) {
-            const T* H = HB[i & 1];
-            const T hzj = H[me - kLanes], hxj = H[plane + me - kLanes];
-            T ex = exc, ey = eyc, ez = ezc;
-            e_cell<T, CPML, MEDIA>(a, tg, i, j, k, o, jk, hz, hzj, hzi, hy, hyk, hyi, hx, hxj, hxk, dey, dez, cey,
-                                   cez, ex, ey, ez);
-            f.exo[o] = ex; f.eyo[o] = ey; f.ezo[o] = ez;
-        }
-        hzi = hz; hyi = hy;
-        if (i < i1) {
-            exc = exn; eyc = eyn; ezc = ezn;
-            exn = nx_.ex2; eyn = nx_.ey2; ezn = nx_.ez2;
-            hx = nx_.hx; hy = nx_.hy; hz = nx_.hz;
-            if (last_row) { ezj = nx_.ezj; exj = nx_.exj; }
-            eyk_e = nx_.eyk; exk_e = nx_.exk;
-            tg = nx_.tg;
-        }
+    for (int i = c.i0 - 1;; i += 2) {
+        fused_plane<T, CPML, MEDIA, TFSF, 0>(f, cold, c, lds, i, o, ob, zh, ze, A, B, hzi, hyi);
+        if (i + 1 > c.i1) break;
+        o += f.sx; ob += c.sxb; zh += c.zhs; ze += c.zes;
+        fused_plane<T, CPML, MEDIA, TFSF, 1>(f, cold, c, lds, i + 1, o, ob, zh, ze, B, A, hzi, hyi);
+        if (i + 2 > c.i1) break;
+        o += f.sx; ob += c.sxb; zh += c.zhs; ze += c.zes;
     }
 }
 
