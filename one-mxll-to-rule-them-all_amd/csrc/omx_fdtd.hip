@@ -89,6 +89,14 @@ YeeArgs<T> make_args(omx3d_ctx* c) {
     a.sx = c->L.sx; a.off = c->L.off;
     a.npml = c->d.npml;
     a.cpml = c->cpml_mask;
+    a.disp_tag0 = c->disp_tag0;
+    a.cpml_sep = 1;
+    {
+        const int n3[3] = {c->d.nx, c->d.ny, c->d.nz};
+        for (int q = 0; q < 3; ++q)
+            if ((c->cpml_mask >> (2 * q)) & 3u)
+                if (n3[q] < 2 * c->d.npml) a.cpml_sep = 0;
+    }
     a.xchunk = c->xchunk;
     a.ntz = (c->L.nz + kLanes - 1) / kLanes;
     a.nty = (c->L.ny + c->rows_y - 1) / c->rows_y;
@@ -158,6 +166,21 @@ int launch_j_correct(omx3d_ctx* c, double tau) {
     return OMX_OK;
 }
 
+// E of the cells of dispersive classes: a's E arrays are the input, the
+// output arrays get the result (the same arrays for the separate kernels).
+template <typename T>
+int launch_e_dispersive(omx3d_ctx* c, const YeeArgs<T>& a, T* exo, T* eyo, T* ezo) {
+    const unsigned nb = blocks_for(c->n_disp, 256);
+    if (c->cpml_mask)
+        hipLaunchKernelGGL((k_e_dispersive<T, true>), dim3(nb), dim3(256), 0, c->s, a, exo, eyo, ezo, c->d_disp,
+                           c->n_disp);
+    else
+        hipLaunchKernelGGL((k_e_dispersive<T, false>), dim3(nb), dim3(256), 0, c->s, a, exo, eyo, ezo, c->d_disp,
+                           c->n_disp);
+    HIPCHK(hipGetLastError());
+    return OMX_OK;
+}
+
 template <typename T>
 int launch_e(omx3d_ctx* c, double time, double t_skip) {
     int rc = wrap_h<T>(c, c->s);
@@ -166,11 +189,15 @@ int launch_e(omx3d_ctx* c, double time, double t_skip) {
     const dim3 blk(kLanes, c->rows_y);
     const unsigned nb = yee_blocks(c, a);
     const bool cp = c->cpml_mask != 0, md = c->media_set && c->n_classes > 1;
-    if (cp && md) hipLaunchKernelGGL((k_e_update<T, true, true>), dim3(nb), blk, 0, c->s, a);
-    else if (cp) hipLaunchKernelGGL((k_e_update<T, true, false>), dim3(nb), blk, 0, c->s, a);
-    else if (md) hipLaunchKernelGGL((k_e_update<T, false, true>), dim3(nb), blk, 0, c->s, a);
-    else hipLaunchKernelGGL((k_e_update<T, false, false>), dim3(nb), blk, 0, c->s, a);
+    if (cp && md) hipLaunchKernelGGL((k_e_update<T, true, 1>), dim3(nb), blk, 0, c->s, a);
+    else if (cp) hipLaunchKernelGGL((k_e_update<T, true, 0>), dim3(nb), blk, 0, c->s, a);
+    else if (md) hipLaunchKernelGGL((k_e_update<T, false, 1>), dim3(nb), blk, 0, c->s, a);
+    else hipLaunchKernelGGL((k_e_update<T, false, 0>), dim3(nb), blk, 0, c->s, a);
     HIPCHK(hipGetLastError());
+    if (md && c->n_disp > 0) {
+        rc = launch_e_dispersive<T>(c, a, a.ex, a.ey, a.ez);
+        if (rc) return rc;
+    }
     return launch_j_correct<T>(c, time - t_skip);
 }
 
@@ -375,11 +402,7 @@ int launch_fused(omx3d_ctx* c) {
     std::memset(&fc, 0, sizeof(fc));
     fc.a = make_args<T>(c);
     for (int q = 0; q < 6; ++q) fc.pho[q] = (T*)c->psi_h_alt[q];
-    if (c->tfsf) {
-        fc.ti0 = c->tf_lo[0]; fc.tj0 = c->tf_lo[1]; fc.tk0 = c->tf_lo[2];
-        fc.ti1 = c->tf_hi[0]; fc.tj1 = c->tf_hi[1]; fc.tk1 = c->tf_hi[2];
-        fc.tkb = c->tf_kb; fc.ex1 = c->d_tf; fc.dt_mu0 = c->d.dt / kMu0;
-    }
+    if (c->tfsf) { fc.tkb = c->tf_kb; fc.ex1 = c->d_tf; fc.dt_mu0 = c->d.dt / kMu0; }
     const FusedCold<T>* dcold = nullptr;
     rc = upload_fused_cold<T>(c, fc, &dcold);
     if (rc) return rc;
@@ -400,6 +423,11 @@ int launch_fused(omx3d_ctx* c) {
     f.sx = a.sx;
     f.nx = a.nx; f.ny = a.ny; f.nz = a.nz; f.sy = a.sy; f.sz = a.sz; f.off = a.off;
     f.npml = a.npml; f.cpml = a.cpml;
+    f.disp_tag0 = c->disp_tag0;
+    if (c->tfsf) {
+        f.ti0 = c->tf_lo[0]; f.tj0 = c->tf_lo[1]; f.tk0 = c->tf_lo[2];
+        f.ti1 = c->tf_hi[0]; f.tj1 = c->tf_hi[1]; f.tk1 = c->tf_hi[2];
+    }
     const int FR = c->fused_rows;
     f.xchunk = c->fused_xchunk;
     f.ntz = (c->L.nz + kLanes - 2) / (kLanes - 1);
@@ -427,6 +455,14 @@ int launch_fused(omx3d_ctx* c) {
     for (int q = 0; q < 6; ++q) std::swap(c->f[q], c->f_alt[q]);
     for (int q = 0; q < 6; ++q) std::swap(c->psi_h[q], c->psi_h_alt[q]);
     c->fparity ^= 1;
+    if (md && c->n_disp > 0) {
+        // cells of dispersive classes: H^{n+1/2} now current, E^n in the
+        // other buffer, E^{n+1} into the current one
+        YeeArgs<T> an = make_args<T>(c);
+        an.ex = (T*)c->f_alt[0]; an.ey = (T*)c->f_alt[1]; an.ez = (T*)c->f_alt[2];
+        rc = launch_e_dispersive<T>(c, an, (T*)c->f[0], (T*)c->f[1], (T*)c->f[2]);
+        if (rc) return rc;
+    }
     return OMX_OK;
 }
 
@@ -586,17 +622,29 @@ int build_tags(omx3d_ctx* c) {
     std::vector<int8_t> cls = {0, 0, 0};
     std::vector<double> epsc = {1.0, 1.0, 1.0};
     int ncls = 1;
-    for (int code = 1; code < ncode; ++code) {
-        if (!((present[code >> 5] >> (code & 31)) & 1u)) continue;
-        if (ncls > 255) return fail(OMX_ERR_RANGE, "more than 255 distinct per-cell medium combinations");
-        lut[code] = (uint8_t)ncls++;
-        const int m0 = code % (nm + 1), m1 = (code / (nm + 1)) % (nm + 1), m2 = code / ((nm + 1) * (nm + 1));
-        const int mm[3] = {m0, m1, m2};
-        for (int q = 0; q < 3; ++q) {
-            cls.push_back((int8_t)mm[q]);
-            epsc.push_back(mm[q] ? c->media[mm[q]].eps_r : 1.0);
+    // classes without a Drude / Drude-Lorentz component first, so that the
+    // dispersive ones are the tags >= disp_tag0
+    c->disp_tag0 = 256;
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1) c->disp_tag0 = ncls;
+        for (int code = 1; code < ncode; ++code) {
+            if (!((present[code >> 5] >> (code & 31)) & 1u)) continue;
+            const int m0 = code % (nm + 1), m1 = (code / (nm + 1)) % (nm + 1), m2 = code / ((nm + 1) * (nm + 1));
+            const int mm[3] = {m0, m1, m2};
+            bool disp = false;
+            for (int q = 0; q < 3; ++q)
+                if (mm[q] && (c->media[mm[q]].kind == OMX_MEDIUM_DRUDE || c->media[mm[q]].kind == OMX_MEDIUM_DL))
+                    disp = true;
+            if (disp != (pass == 1)) continue;
+            if (ncls > 255) return fail(OMX_ERR_RANGE, "more than 255 distinct per-cell medium combinations");
+            lut[code] = (uint8_t)ncls++;
+            for (int q = 0; q < 3; ++q) {
+                cls.push_back((int8_t)mm[q]);
+                epsc.push_back(mm[q] ? c->media[mm[q]].eps_r : 1.0);
+            }
         }
     }
+    if (c->disp_tag0 >= ncls) c->disp_tag0 = 256;
     c->n_classes = ncls;
     rc = dalloc(c, (void**)&d_lut, ncode);
     if (rc) return rc;
@@ -616,6 +664,29 @@ int build_tags(omx3d_ctx* c) {
     c->device_bytes -= (long)(sizeof(unsigned) * ((ncode + 31) / 32) + ncode);
     dfreet(d_present);
     dfreet(d_lut);
+    // the cells of dispersive classes, in memory order, for k_e_dispersive
+    dfreet(c->d_disp);
+    c->n_disp = 0;
+    if (c->disp_tag0 < 256) {
+        std::vector<uint8_t> tags(L.nalloc);
+        HIPCHK(hipMemcpyAsync(tags.data(), c->d_tag, L.nalloc, hipMemcpyDeviceToHost, c->s));
+        HIPCHK(hipStreamSynchronize(c->s));
+        std::vector<long> cells;
+        for (int i = 1; i <= L.nx; ++i)
+            for (int j = 1; j <= L.ny; ++j)
+                for (int k = 1; k <= L.nz; ++k) {
+                    const long o = L.at(i, j, k);
+                    if (tags[o] >= c->disp_tag0) cells.push_back(o);
+                }
+        if (cells.size() > (size_t)INT32_MAX) return fail(OMX_ERR_RANGE, "too many dispersive cells");
+        if (!cells.empty()) {
+            rc = dalloc(c, (void**)&c->d_disp, sizeof(long) * cells.size());
+            if (rc) return rc;
+            HIPCHK(hipMemcpyAsync(c->d_disp, cells.data(), sizeof(long) * cells.size(), hipMemcpyHostToDevice, c->s));
+            HIPCHK(hipStreamSynchronize(c->s));
+            c->n_disp = (int)cells.size();
+        }
+    }
     return OMX_OK;
 }
 
@@ -828,6 +899,17 @@ int batch_step_launches(omxb_ctx* b) {
         for (auto* c : b->g) { int rc = wrap_h<T>(c, b->s); if (rc) return rc; }
     hipLaunchKernelGGL(k_e_update_batched<T>, dim3(b->total_tiles), blk, 0, b->s, (const YeeArgs<T>*)b->d_args,
                        b->d_tile_start, b->n);
+    for (auto* c : b->g) {
+        if (!(c->media_set && c->n_classes > 1 && c->n_disp > 0)) continue;
+        const YeeArgs<T> a = make_args<T>(c);
+        const unsigned nb = blocks_for(c->n_disp, 256);
+        if (c->cpml_mask)
+            hipLaunchKernelGGL((k_e_dispersive<T, true>), dim3(nb), dim3(256), 0, b->s, a, a.ex, a.ey, a.ez,
+                               c->d_disp, c->n_disp);
+        else
+            hipLaunchKernelGGL((k_e_dispersive<T, false>), dim3(nb), dim3(256), 0, b->s, a, a.ex, a.ey, a.ez,
+                               c->d_disp, c->n_disp);
+    }
     if (b->n_samp > 0)
         hipLaunchKernelGGL(k_sample_batched<T>, dim3(blocks_for(b->n_samp, 256)), dim3(256), 0, b->s,
                            (const YeeArgs<T>*)b->d_args, b->d_br, b->d_grid_of, b->d_local, b->d_code, b->d_off,
@@ -1062,7 +1144,7 @@ int omx3d_destroy(omx3d_ctx* c) {
     dfreet(c->dcoef); dfree(c->tcoef);
     for (auto& p : c->psi_h) dfree(p);
     for (auto& p : c->psi_e) dfree(p);
-    dfreet(c->d_aux); dfreet(c->d_tag); dfreet(c->d_cls); dfree(c->d_med); dfreet(c->d_med_in); dfreet(c->d_med_co);
+    dfreet(c->d_aux); dfreet(c->d_tag); dfreet(c->d_cls); dfreet(c->d_disp); dfree(c->d_med); dfreet(c->d_med_in); dfreet(c->d_med_co);
     dfreet(c->d_eps_class); dfreet(c->d_kinds);
     for (int q = 0; q < 3; ++q) {
         dfree(c->disp_eold[q]); dfree(c->disp_pd[q]); dfree(c->disp_pl[q]); dfree(c->disp_plold[q]);

@@ -74,6 +74,8 @@ struct YeeArgs {
     int off;
     int npml;
     unsigned cpml;             // bit f set: CPML on face f (x-lo, x-hi, y-lo, y-hi, z-lo, z-hi)
+    int cpml_sep;              // 1: on every CPML axis the two faces do not meet (n >= 2 npml)
+    int disp_tag0;             // tags >= disp_tag0: classes with a Drude / Drude-Lorentz component
     int ntz, nty, xchunk, rows;
 };
 
@@ -133,6 +135,9 @@ struct omx3d_ctx {
     uint8_t* d_tag = nullptr;
     int8_t* d_cls = nullptr;        // 3 * n_classes
     int n_classes = 0;
+    int disp_tag0 = 256;            // tags >= disp_tag0: classes with a Drude / DL component
+    long* d_disp = nullptr;         // element offsets of the cells of those classes (ascending)
+    int n_disp = 0;
     void* d_med = nullptr;          // MedDev<T>[n_media+1]
     omx_medium* d_med_in = nullptr;
     omx_medium_coeffs* d_med_co = nullptr;

@@ -405,6 +405,30 @@ __device__ __forceinline__ void h_cell(const YeeArgs<T>& a, T* const* pho, bool 
     }
 }
 
+// CPML face of one axis for a cell: psi slot and coefficient index (slot
+// < 0: the cell is in no face of that axis).  H faces: low 1..npml-1 (slot
+// n-1, coefficient n), high n_a+1-npml..n_a-1 (slot npml-2+c, c = n_a-n).  E
+// faces: low 2..npml (slot n-1, coefficient n), high n_a+1-npml..n_a-1 (slot
+// npml-1+c, c = n_a+1-n).  Valid when the two faces of an axis do not meet.
+struct Face {
+    int slot, c;
+};
+__device__ __forceinline__ Face face_h(bool lo_on, bool hi_on, int n, int na, int npml) {
+    if (lo_on && n <= npml - 1) return {n - 1, n};
+    if (hi_on && n >= na + 1 - npml && n <= na - 1) return {npml - 2 + na - n, na - n};
+    return {-1, 0};
+}
+__device__ __forceinline__ Face face_e(bool lo_on, bool hi_on, int n, int na, int npml) {
+    if (lo_on && n >= 2 && n <= npml) return {n - 1, n};
+    if (hi_on && n >= na + 1 - npml && n <= na - 1) return {npml - 1 + na + 1 - n, na + 1 - n};
+    return {-1, 0};
+}
+
+// A tile: 64 lanes along z, rows along y, marching xchunk planes along x.
+// With CPML, the face slots of the row (y) and lane (z) are fixed over the
+// tile and those of the plane (x) are uniform; the psi of all faces a cell
+// lies in are loaded together with its fields, ahead of the arithmetic.
+// Grids whose two faces of an axis meet (n < 2 npml) take the per-cell path.
 template <typename T, bool CPML>
 __device__ __forceinline__ void h_tile(const YeeArgs<T>& a, const int t) {
     const int tz = t % a.ntz;
@@ -423,14 +447,58 @@ __device__ __forceinline__ void h_tile(const YeeArgs<T>& a, const int t) {
     const T dhy = a.den_hy[j], dhz = a.den_hz[k];
     const T chy = sizeof(T) == 4 ? a.cdhy[j] : (T)0, chz = sizeof(T) == 4 ? a.cdhz[k] : (T)0;
     T ezc = a.ez[o], eyc = a.ey[o];
+    if (!CPML || !a.cpml_sep) {
+        for (int i = i0; i <= i1; ++i, o += sx) {
+            const T ezn = a.ez[o + sx], eyn = a.ey[o + sx];
+            const T ezj = a.ez[o + sz], eyk = a.ey[o + 1];
+            const T exc = a.ex[o], exk = a.ex[o + 1], exj = a.ex[o + sz];
+            T hx = a.hx[o], hy = a.hy[o], hz = a.hz[o];
+            const T cx = sizeof(T) == 4 ? a.cdhx[i] : a.den_hx[i];
+            h_cell<T, CPML>(a, a.ph, true, i, j, k, jk, ezc, ezn, ezj, eyc, eyn, eyk, exc, exk, exj, cx, dhy, dhz,
+                            chy, chz, hx, hy, hz);
+            a.hx[o] = hx; a.hy[o] = hy; a.hz[o] = hz;
+            ezc = ezn; eyc = eyn;
+        }
+        return;
+    }
+    const unsigned cp = a.cpml;
+    const int npml = a.npml, nsh = 2 * (npml - 1);
+    const Face fy = face_h(cp & 4u, cp & 8u, j, a.ny, npml);
+    const Face fz = face_h(cp & 16u, cp & 32u, k, a.nz, npml);
+    T yb = (T)0, yc = (T)0, zb = (T)0, zc = (T)0;
+    if (fy.slot >= 0) { yb = a.bh[fy.c]; yc = a.chd[fy.c]; }
+    if (fz.slot >= 0) { zb = a.bh[fz.c]; zc = a.chd[fz.c]; }
+    const T dmu = a.dt_mu0;
     for (int i = i0; i <= i1; ++i, o += sx) {
         const T ezn = a.ez[o + sx], eyn = a.ey[o + sx];
         const T ezj = a.ez[o + sz], eyk = a.ey[o + 1];
         const T exc = a.ex[o], exk = a.ex[o + 1], exj = a.ex[o + sz];
         T hx = a.hx[o], hy = a.hy[o], hz = a.hz[o];
         const T cx = sizeof(T) == 4 ? a.cdhx[i] : a.den_hx[i];
-        h_cell<T, CPML>(a, a.ph, true, i, j, k, jk, ezc, ezn, ezj, eyc, eyn, eyk, exc, exk, exj, cx, dhy, dhz, chy,
-                        chz, hx, hy, hz);
+        const Face fx = face_h(cp & 1u, cp & 2u, i, a.nx, npml);
+        const long psx = (long)fx.slot * sx + jk;
+        const long psy = ((long)i * nsh + fy.slot) * sz + k + a.off;
+        const long psz = ((long)i * a.sy + j) * nsh + fz.slot;
+        T px0 = (T)0, px1 = (T)0, xb = (T)0, xc = (T)0, py0 = (T)0, py1 = (T)0, pz0 = (T)0, pz1 = (T)0;
+        if (fx.slot >= 0) { px0 = a.ph[0][psx]; px1 = a.ph[1][psx]; xb = a.bh[fx.c]; xc = a.chd[fx.c]; }
+        if (fy.slot >= 0) { py0 = a.ph[2][psy]; py1 = a.ph[3][psy]; }
+        if (fz.slot >= 0) { pz0 = a.ph[4][psz]; pz1 = a.ph[5][psz]; }
+        h_bulk(dmu, cx, dhy, dhz, chy, chz, ezc, ezn, ezj, eyc, eyn, eyk, exc, exk, exj, hx, hy, hz);
+        if (fx.slot >= 0) {
+            const T p0 = psi_step(px0, xb, xc, ezn - ezc), p1 = psi_step(px1, xb, xc, eyc - eyn);
+            a.ph[0][psx] = p0; a.ph[1][psx] = p1;
+            hy = hy + dmu * p0; hz = hz + dmu * p1;
+        }
+        if (fy.slot >= 0) {
+            const T p0 = psi_step(py0, yb, yc, ezc - ezj), p1 = psi_step(py1, yb, yc, exj - exc);
+            a.ph[2][psy] = p0; a.ph[3][psy] = p1;
+            hx = hx + dmu * p0; hz = hz + dmu * p1;
+        }
+        if (fz.slot >= 0) {
+            const T p0 = psi_step(pz0, zb, zc, eyk - eyc), p1 = psi_step(pz1, zb, zc, exc - exk);
+            a.ph[4][psz] = p0; a.ph[5][psz] = p1;
+            hx = hx + dmu * p0; hy = hy + dmu * p1;
+        }
         a.hx[o] = hx; a.hy[o] = hy; a.hz[o] = hz;
         ezc = ezn; eyc = eyn;
     }
@@ -493,7 +561,10 @@ __device__ __forceinline__ T medium_update(T e, T rot, T vac, int m, const MedDe
 // update, then the CPML psi recursions (psi of E in place in a.pe).  tg is
 // the cell's class tag (0: vacuum in all three components); dex = den_ex[i],
 // cex_ = cdex[i] (FP32 only).
-template <typename T, bool MEDIA>
+// MED: 0 no media; 1 vacuum and dielectric components only (the caller
+// leaves cells of the Drude/Drude-Lorentz classes, tag >= disp_tag0, to the
+// dispersive-cell pass); 2 every medium kind.
+template <typename T, int MED>
 __device__ __forceinline__ void e_bulk(const YeeArgs<T>& a, int tg, long o, T dep, T hzc, T hzj, T hzi, T hyc,
                                        T hyk, T hyi, T hxc, T hxj, T hxk, T dex, T cex_, T dey, T dez, T cey, T cez,
                                        T& ex, T& ey, T& ez) {
@@ -510,7 +581,18 @@ __device__ __forceinline__ void e_bulk(const YeeArgs<T>& a, int tg, long o, T de
     } else {
         vx = dep * rx; vy = dep * ry; vz = dep * rz;
     }
-    if (MEDIA) {
+    if (MED == 1) {
+        if (tg == 0) {
+            ex = ex + vx; ey = ey + vy; ez = ez + vz;
+        } else {
+            // vacuum: + dt_eps0 rotH; dielectric: + dt_eps0/eps_r rotH
+            const int8_t* c3 = a.cls + 3 * tg;
+            const int m0 = c3[0], m1 = c3[1], m2 = c3[2];
+            ex = m0 == 0 ? ex + vx : ex + a.med[m0].cdt * rx;
+            ey = m1 == 0 ? ey + vy : ey + a.med[m1].cdt * ry;
+            ez = m2 == 0 ? ez + vz : ez + a.med[m2].cdt * rz;
+        }
+    } else if (MED == 2) {
         if (tg == 0) {
             ex = ex + vx; ey = ey + vy; ez = ez + vz;
         } else {
@@ -603,8 +685,8 @@ __device__ __forceinline__ void e_cell(const YeeArgs<T>& a, int tg, int i, int j
                                        T hzc, T hzj, T hzi, T hyc, T hyk, T hyi, T hxc, T hxj, T hxk,
                                        T dex, T cex_, T dey, T dez, T cey, T cez, T& ex, T& ey, T& ez) {
     const T dep = a.dt_eps0;
-    e_bulk<T, MEDIA>(a, tg, o, dep, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dex, cex_, dey, dez, cey, cez, ex, ey,
-                     ez);
+    e_bulk<T, MEDIA ? 2 : 0>(a, tg, o, dep, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dex, cex_, dey, dez, cey, cez,
+                             ex, ey, ez);
     if constexpr (CPML) {
         e_cpml_x(a, i, jk, dep, hzc, hzi, hyc, hyi, ey, ez);
         e_cpml_y(a, i, j, k, dep, hzc, hzj, hxc, hxj, ex, ez);
@@ -612,7 +694,9 @@ __device__ __forceinline__ void e_cell(const YeeArgs<T>& a, int tg, int i, int j
     }
 }
 
-template <typename T, bool CPML, bool MEDIA>
+// MED as for e_bulk; with MED 1 the tile skips the cells of dispersive
+// classes entirely (E and psi of E), which k_e_dispersive then updates.
+template <typename T, bool CPML, int MED>
 __device__ __forceinline__ void e_tile(const YeeArgs<T>& a, const int t) {
     const int tz = t % a.ntz;
     const int r = t / a.ntz;
@@ -630,15 +714,67 @@ __device__ __forceinline__ void e_tile(const YeeArgs<T>& a, const int t) {
     const T dey = a.den_ey[j], dez = a.den_ez[k];
     const T cey = sizeof(T) == 4 ? a.cdey[j] : (T)0, cez = sizeof(T) == 4 ? a.cdez[k] : (T)0;
     T hzi = a.hz[o - sx], hyi = a.hy[o - sx];
+    if (!CPML || !a.cpml_sep) {
+        for (int i = i0; i <= i1; ++i, o += sx) {
+            const T hzc = a.hz[o], hyc = a.hy[o], hxc = a.hx[o];
+            const T hzj = a.hz[o - sz], hxj = a.hx[o - sz];
+            const T hyk = a.hy[o - 1], hxk = a.hx[o - 1];
+            T ex = a.ex[o], ey = a.ey[o], ez = a.ez[o];
+            const int tg = (MED != 0 && a.tag != nullptr) ? (int)a.tag[o] : 0;
+            const T dex = a.den_ex[i], cex = sizeof(T) == 4 ? a.cdex[i] : (T)0;
+            if (MED != 1 || tg < a.disp_tag0) {
+                e_cell<T, CPML, MED != 0>(a, tg, i, j, k, o, jk, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dex,
+                                          cex, dey, dez, cey, cez, ex, ey, ez);
+                a.ex[o] = ex; a.ey[o] = ey; a.ez[o] = ez;
+            }
+            hzi = hzc; hyi = hyc;
+        }
+        return;
+    }
+    const unsigned cp = a.cpml;
+    const int npml = a.npml, nse = 2 * npml;
+    const Face fy = face_e(cp & 4u, cp & 8u, j, a.ny, npml);
+    const Face fz = face_e(cp & 16u, cp & 32u, k, a.nz, npml);
+    T yb = (T)0, yc = (T)0, zb = (T)0, zc = (T)0;
+    if (fy.slot >= 0) { yb = a.be[fy.c]; yc = a.ced[fy.c]; }
+    if (fz.slot >= 0) { zb = a.be[fz.c]; zc = a.ced[fz.c]; }
+    const T dep = a.dt_eps0;
     for (int i = i0; i <= i1; ++i, o += sx) {
         const T hzc = a.hz[o], hyc = a.hy[o], hxc = a.hx[o];
         const T hzj = a.hz[o - sz], hxj = a.hx[o - sz];
         const T hyk = a.hy[o - 1], hxk = a.hx[o - 1];
         T ex = a.ex[o], ey = a.ey[o], ez = a.ez[o];
-        const int tg = (MEDIA && a.tag != nullptr) ? (int)a.tag[o] : 0;
+        const int tg = (MED != 0 && a.tag != nullptr) ? (int)a.tag[o] : 0;
         const T dex = a.den_ex[i], cex = sizeof(T) == 4 ? a.cdex[i] : (T)0;
-        e_cell<T, CPML, MEDIA>(a, tg, i, j, k, o, jk, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dex, cex, dey, dez,
-                               cey, cez, ex, ey, ez);
+        if (MED == 1 && tg >= a.disp_tag0) {
+            hzi = hzc; hyi = hyc;
+            continue;
+        }
+        const Face fx = face_e(cp & 1u, cp & 2u, i, a.nx, npml);
+        const long psx = (long)fx.slot * sx + jk;
+        const long psy = ((long)i * nse + fy.slot) * sz + k + a.off;
+        const long psz = ((long)i * a.sy + j) * nse + fz.slot;
+        T px0 = (T)0, px1 = (T)0, xb = (T)0, xc = (T)0, py0 = (T)0, py1 = (T)0, pz0 = (T)0, pz1 = (T)0;
+        if (fx.slot >= 0) { px0 = a.pe[0][psx]; px1 = a.pe[1][psx]; xb = a.be[fx.c]; xc = a.ced[fx.c]; }
+        if (fy.slot >= 0) { py0 = a.pe[2][psy]; py1 = a.pe[3][psy]; }
+        if (fz.slot >= 0) { pz0 = a.pe[4][psz]; pz1 = a.pe[5][psz]; }
+        e_bulk<T, MED>(a, tg, o, dep, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dex, cex, dey, dez, cey, cez,
+                       ex, ey, ez);
+        if (fx.slot >= 0) {
+            const T p0 = psi_step(px0, xb, xc, hzi - hzc), p1 = psi_step(px1, xb, xc, hyc - hyi);
+            a.pe[0][psx] = p0; a.pe[1][psx] = p1;
+            ey = ey + dep * p0; ez = ez + dep * p1;
+        }
+        if (fy.slot >= 0) {
+            const T p0 = psi_step(py0, yb, yc, hzc - hzj), p1 = psi_step(py1, yb, yc, hxj - hxc);
+            a.pe[2][psy] = p0; a.pe[3][psy] = p1;
+            ex = ex + dep * p0; ez = ez + dep * p1;
+        }
+        if (fz.slot >= 0) {
+            const T p0 = psi_step(pz0, zb, zc, hyk - hyc), p1 = psi_step(pz1, zb, zc, hxc - hxk);
+            a.pe[4][psz] = p0; a.pe[5][psz] = p1;
+            ex = ex + dep * p0; ey = ey + dep * p1;
+        }
         a.ex[o] = ex; a.ey[o] = ey; a.ez[o] = ez;
         hzi = hzc; hyi = hyc;
     }
@@ -649,9 +785,42 @@ __global__ __launch_bounds__(256) void k_h_update(const YeeArgs<T> a) {
     h_tile<T, CPML>(a, xcd_tile(blockIdx.x, gridDim.x));
 }
 
-template <typename T, bool CPML, bool MEDIA>
+template <typename T, bool CPML, int MED>
 __global__ __launch_bounds__(256) void k_e_update(const YeeArgs<T> a) {
-    e_tile<T, CPML, MEDIA>(a, xcd_tile(blockIdx.x, gridDim.x));
+    e_tile<T, CPML, MED>(a, xcd_tile(blockIdx.x, gridDim.x));
+}
+
+// E of the cells of dispersive classes (Drude / Drude-Lorentz components),
+// after the tile pass has updated every other cell: the per-cell update of
+// e_cell with the full medium path, from E in a.ex/a.ey/a.ez to the output
+// arrays (the same arrays in the separate H/E path, the other buffer after
+// the fused step).  list holds the cells' element offsets.
+template <typename T, bool CPML>
+__global__ __launch_bounds__(256) void k_e_dispersive(const YeeArgs<T> a, T* __restrict__ exo, T* __restrict__ eyo,
+                                                      T* __restrict__ ezo, const long* __restrict__ list, int n) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const long o = list[p];
+    // o = i*sx + j*sz + k + off with 0 <= k < sz and j*sz + k < sx
+    const long u = o - a.off;
+    const int i = (int)(u / a.sx);
+    const long r = u - (long)i * a.sx;
+    const int j = (int)(r / a.sz);
+    const int k = (int)(r - (long)j * a.sz);
+    const long jk = (long)j * a.sz + k + a.off;
+    const long sx = a.sx;
+    const int sz = a.sz;
+    const T hzc = a.hz[o], hyc = a.hy[o], hxc = a.hx[o];
+    const T hzj = a.hz[o - sz], hxj = a.hx[o - sz];
+    const T hyk = a.hy[o - 1], hxk = a.hx[o - 1];
+    const T hzi = a.hz[o - sx], hyi = a.hy[o - sx];
+    T ex = a.ex[o], ey = a.ey[o], ez = a.ez[o];
+    const T dex = a.den_ex[i], cex = sizeof(T) == 4 ? a.cdex[i] : (T)0;
+    const T dey = a.den_ey[j], dez = a.den_ez[k];
+    const T cey = sizeof(T) == 4 ? a.cdey[j] : (T)0, cez = sizeof(T) == 4 ? a.cdez[k] : (T)0;
+    e_cell<T, CPML, true>(a, (int)a.tag[o], i, j, k, o, jk, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dex, cex, dey,
+                          dez, cey, cez, ex, ey, ez);
+    exo[o] = ex; eyo[o] = ey; ezo[o] = ez;
 }
 
 // ---------------------------------------------------------------------------
@@ -694,13 +863,15 @@ struct FusedArgs {
     long sx;
     int nx, ny, nz, sy, sz, off, npml, xchunk, ntz, nty;
     unsigned cpml;
+    int ti0, ti1, tj0, tj1, tk0, tk1;       // TF/SF box
+    int disp_tag0;                          // first tag of the dispersive classes
 };
 
 template <typename T>
 struct FusedCold {
     YeeArgs<T> a;                           // current fields / psi, tables, media
     T* pho[6];                              // output psi of H
-    int ti0, ti1, tj0, tj1, tk0, tk1, tkb;  // TF/SF box and 1D line origin
+    int tkb;                                // TF/SF 1D line origin
     const double* ex1;                      // incident Ex on the 1D line
     double dt_mu0;
 };
@@ -744,6 +915,10 @@ struct FusedLane {
     uint32_t zhs, zes;       // their plane strides in bytes
     T dhy, dhz, dey, dez, chy, chz, cey, cez;
     T zbh, zch, zbe, zce;    // z-face CPML coefficients of this lane (H, E)
+    // TF/SF: this lane's Hy (z face) and Hz (y face) corrections, constant
+    // over the step; applied in the planes ti0 <= i <= ti1-1
+    bool tf_y, tf_z;
+    double tfy, tfz;
 };
 
 // All loads are unconditional (no value depends on a branch, so the two
@@ -789,8 +964,9 @@ __device__ __forceinline__ void fused_plane(const FusedArgs<T>& f, const FusedCo
     T ezj, exj, eyk, exk;
     if (!c.last_row) { ezj = lds[e0 + kLanes]; exj = lds[e0 + pl + kLanes]; }
     else { ezj = cur.ezj; exj = cur.exj; }
-    if (!c.last_lane) { eyk = lds[e0 + 2 * pl + 1]; exk = lds[e0 + pl + 1]; }
-    else { eyk = cur.eyk; exk = cur.exk; }
+    // lane 63 reads past its row (inside the LDS block) and keeps cur's value
+    eyk = lds[e0 + 2 * pl + 1]; exk = lds[e0 + pl + 1];
+    if (c.last_lane) { eyk = cur.eyk; exk = cur.exk; }
     T hx = cur.hx, hy = cur.hy, hz = cur.hz;
     const bool store = c.own && i >= c.i0;
     const int j = c.j, k = c.k;
@@ -812,20 +988,9 @@ __device__ __forceinline__ void fused_plane(const FusedArgs<T>& f, const FusedCo
             }
         }
         if constexpr (TFSF) {
-            const FusedCold<T>& q = *cold;
-            if (i >= q.ti0 && i <= q.ti1 - 1) {
-                if (j >= q.tj0 && j <= q.tj1) {
-                    if (k == q.tk0 - 1)
-                        hy = (T)((double)hy + q.dt_mu0 * (double)q.a.den_hz[k] * q.ex1[q.tk0 - q.tkb]);
-                    else if (k == q.tk1)
-                        hy = (T)((double)hy - q.dt_mu0 * (double)q.a.den_hz[k] * q.ex1[q.tk1 - q.tkb]);
-                }
-                if (k >= q.tk0 && k <= q.tk1) {
-                    if (j == q.tj0 - 1)
-                        hz = (T)((double)hz - q.dt_mu0 * (double)q.a.den_hy[j] * q.ex1[k - q.tkb]);
-                    else if (j == q.tj1)
-                        hz = (T)((double)hz + q.dt_mu0 * (double)q.a.den_hy[j] * q.ex1[k - q.tkb]);
-                }
+            if (i >= f.ti0 && i <= f.ti1 - 1) {
+                if (c.tf_y) hy = (T)((double)hy + c.tfy);
+                if (c.tf_z) hz = (T)((double)hz + c.tfz);
             }
         }
     }
@@ -841,12 +1006,12 @@ __device__ __forceinline__ void fused_plane(const FusedArgs<T>& f, const FusedCo
     }
     __syncthreads();
     const T hyk = __shfl_up(hy, 1), hxk = __shfl_up(hx, 1);
-    if (store) {
+    if (store && (!MEDIA || cur.tg < f.disp_tag0)) {
         const int h0 = 6 * pl + P * 2 * pl + c.me - kLanes;
         const T hzj = lds[h0], hxj = lds[h0 + pl];
         T ex = exc, ey = eyc, ez = ezc;
         const T dep = f.dt_eps0;
-        e_bulk<T, MEDIA>(cold->a, cur.tg, o, dep, hz, hzj, hzi, hy, hyk, hyi, hx, hxj, hxk, ldc(f.dex, i),
+        e_bulk<T, MEDIA ? 1 : 0>(cold->a, cur.tg, o, dep, hz, hzj, hzi, hy, hyk, hyi, hx, hxj, hxk, ldc(f.dex, i),
                          sizeof(T) == 4 ? ldc(f.cex, i) : (T)0, c.dey, c.dez, c.cey, c.cez, ex, ey, ez);
         if constexpr (CPML) {
             if (f.cpml & 15u) {
@@ -872,7 +1037,9 @@ __global__ __launch_bounds__(512) void k_step_fused(const FusedArgs<T> f, const 
     T* lds = reinterpret_cast<T*>(omx_lds);
     FusedLane<T> c;
     const int FR = blockDim.y;
-    c.lane = threadIdx.x; c.row = threadIdx.y;
+    // one wavefront per row: the row index is uniform, which turns the row
+    // conditions into scalar branches
+    c.lane = threadIdx.x; c.row = __builtin_amdgcn_readfirstlane(threadIdx.y);
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     const int tz = t % f.ntz;
     const int q = t / f.ntz;
@@ -924,6 +1091,23 @@ __global__ __launch_bounds__(512) void k_step_fused(const FusedArgs<T> f, const 
             c.zeb = (uint32_t)(row_e * (long)sizeof(T));
             c.zhs = (uint32_t)((long)f.sy * nsh * (long)sizeof(T));
             c.zes = (uint32_t)((long)f.sy * nse * (long)sizeof(T));
+        }
+    }
+    c.tf_y = c.tf_z = false;
+    c.tfy = c.tfz = 0.0;
+    if constexpr (TFSF) {
+        // the corrections of k_tfsf_h: Hy -/+ dt/mu0 den_hz(k) Ex_inc on the
+        // z faces k = tk0-1 / tk1 (tj0 <= j <= tj1), Hz -/+ dt/mu0 den_hy(j)
+        // Ex_inc(k) on the y faces j = tj0-1 / tj1 (tk0 <= k <= tk1)
+        const FusedCold<T>& q = *cold;
+        const int j = c.j, k = c.k;
+        if (c.hcell && j >= f.tj0 && j <= f.tj1) {
+            if (k == f.tk0 - 1) { c.tf_y = true; c.tfy = q.dt_mu0 * (double)q.a.den_hz[k] * q.ex1[f.tk0 - q.tkb]; }
+            else if (k == f.tk1) { c.tf_y = true; c.tfy = -(q.dt_mu0 * (double)q.a.den_hz[k] * q.ex1[f.tk1 - q.tkb]); }
+        }
+        if (c.hcell && k >= f.tk0 && k <= f.tk1) {
+            if (j == f.tj0 - 1) { c.tf_z = true; c.tfz = -(q.dt_mu0 * (double)q.a.den_hy[j] * q.ex1[k - q.tkb]); }
+            else if (j == f.tj1) { c.tf_z = true; c.tfz = q.dt_mu0 * (double)q.a.den_hy[j] * q.ex1[k - q.tkb]; }
         }
     }
     c.plane = FR * kLanes;
@@ -986,8 +1170,8 @@ __global__ __launch_bounds__(256) void k_e_update_batched(const YeeArgs<T>* __re
     const int g = find_grid(tile_start, n, t);
     const YeeArgs<T>& a = args[g];
     if ((int)threadIdx.y >= a.rows) return;
-    if (a.cpml) e_tile<T, true, true>(a, t - tile_start[g]);
-    else e_tile<T, false, true>(a, t - tile_start[g]);
+    if (a.cpml) e_tile<T, true, 1>(a, t - tile_start[g]);
+    else e_tile<T, false, 1>(a, t - tile_start[g]);
 }
 
 // ---------------------------------------------------------------------------
