@@ -1,10 +1,11 @@
-"""Benchmark of the MI355X FDTD engine on the 3D dispersive-media workload:
-256^3 Yee grid, dx = 2 nm, CPML 12 cells on every face, one seeded
-Drude-Lorentz sphere (eps_inf 5, Drude 9 eV / 0.07 eV, two seeded Lorentz
-poles) in vacuum, x-polarised TF/SF plane wave along +z (600 nm carrier),
-plane detectors Ex, Ey, Ez on the transmitted side.
+"""Benchmark of the MI355X FDTD engine.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload dl_sphere_tfsf_256]
+
+Default workload (the headline): 256^3 Yee grid, dx = 2 nm, CPML 12 cells on
+every face, one seeded Drude-Lorentz sphere (eps_inf 5, Drude 9 eV / 0.07
+eV, two seeded Lorentz poles) in vacuum, x-polarised TF/SF plane wave along
++z (600 nm carrier), plane detectors Ex, Ey, Ez on the transmitted side.
 
 A step is one full Yee step (H and E updates with CPML, the Drude-Lorentz
 ADE, TF/SF corrections and detector recording at its cadence) over the whole
@@ -12,9 +13,19 @@ grid, inputs resident in HBM.  The FP32 engine is timed (the headline), then
 the FP64 engine on the same steps; the FP32 detector series is compared
 with the FP64 one (relative L2).  The dominant kernel's roofline fraction
 comes from HIP events recorded around each of its launches in the timed
-steps, on the stream the kernels run on.  The
-CPU baseline is the reference's own Fortran/OpenMP solver (built from its
-sources into oracle/_ref) timed on a bounded sample on this host.
+steps, on the stream the kernels run on; its HBM traffic from the committed
+rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the same kernel and workload.
+The CPU baseline is the reference's own Fortran/OpenMP solver (built from
+its sources into oracle/_ref) timed on a bounded sample on this host.
+
+Other workloads (each prints its own JSON line):
+  ellipsoids_768x768x512   64 DL ellipsoids over a dielectric substrate,
+                           energy integral every 100 steps
+  batched_64               64 independent grids (edges 96-192) in one batch,
+                           against the same grids one by one and against one
+                           large grid of the same cell count
+  emitters_512x512x384     Ag slab with a nanohole array, 4096 two-level
+                           emitters coupled to the field
 """
 import argparse
 import json
@@ -32,43 +43,81 @@ sys.path[:0] = [os.path.join(ROOT, "one-mxll-to-rule-them-all_amd"), ROOT]
 
 import omx_fdtd  # noqa: E402
 from omx_fdtd import cases, grid as G  # noqa: E402
-from omx_fdtd.units import index_to_nm, MEDIUM_DL  # noqa: E402
+from omx_fdtd.units import index_to_nm, MEDIUM_DRUDE, MEDIUM_DL, BC_CPML  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-WORKLOAD = "dl_sphere_tfsf_256"
+HEADLINE = "dl_sphere_tfsf_256"
+METRIC = "Gcell-updates/s, FP32 3D Drude-Lorentz + CPML FDTD (TF/SF plane wave)"
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_grid(case, precision):
+def build_grid(case, precision, mode="separate"):
     dr, dt = case.dr_dt
     g = omx_fdtd.Grid3D(case.grid, case.npml, case.boundaries, dt, dr, precision=precision)
+    g.set_step_mode(1 if mode == "separate" else 0)
     cases.apply(case, g, library=G.library_medium, make_medium=G.medium)
     return g
 
 
+def resolved_media(case):
+    out = []
+    for spec in case.media:
+        m = cases.medium_struct(spec, G.medium)
+        if isinstance(m, tuple) and m[0] == "library":
+            m = G.library_medium(m[1], m[2])
+        out.append(m)
+    return out
+
+
 def algorithmic_bytes(case, g, w):
     """Bytes per H and per E launch: every live array read once and written
-    once (per cell: H 9 words, E 9 words), plus 4 words per CPML slab cell
-    (two psi arrays, read and written), plus per Drude-Lorentz component
-    cell 4 + 4P words (E_old, PD, PL_k, PL_old_k read and written), plus the
+    once (per cell: H 9 words, E 9 words), plus 4 words per CPML slab cell and
+    half step (two psi arrays, read and written), plus per dispersive
+    component cell 2 words (Drude: PD) or 4 + 4P words (Drude-Lorentz with P
+    poles: E_old, PD, PL_k, PL_old_k), all read and written, plus the
     one-byte cell tag when the grid holds media."""
     nx, ny, nz = case.grid
     N = nx * ny * nz
     p = case.npml
-    slab_h = 2 * (p - 1) * (ny * nz + nx * nz + nx * ny)   # cells in H CPML slabs, per axis pair
-    slab_e = 2 * (p - 1) * (ny * nz + nx * nz + nx * ny)   # E slabs: 2..npml and n+1-npml..n-1
-    mm = g.media_map()
-    P = case.media[0][4] and len(case.media[0][4]) or 0
-    dl_comp = int((mm > 0).sum())
-    b_h = w * (9 * N + 4 * slab_h)
-    b_e = w * (9 * N + 4 * slab_e + (4 + 4 * P) * dl_comp) + N
-    # fused pass: E and H each read once and written once (12 words), psi of
-    # both, dispersive state, tag
-    b_f = w * (12 * N + 4 * slab_h + 4 * slab_e + (4 + 4 * P) * dl_comp) + N
-    return b_h, b_e, b_f, dl_comp
+    slab = 0
+    for n_a, bc in zip(case.grid, case.boundaries):
+        if bc == BC_CPML:
+            slab += 2 * (p - 1) * (N // n_a)
+    disp = 0
+    comp_cells = 0
+    if case.media:
+        mm = g.media_map()
+        for m_idx, m in enumerate(resolved_media(case), start=1):
+            cnt = int((mm == m_idx).sum())
+            if m.kind == MEDIUM_DRUDE:
+                disp += 2 * cnt
+                comp_cells += cnt
+            elif m.kind == MEDIUM_DL:
+                disp += (4 + 4 * m.n_poles) * cnt
+                comp_cells += cnt
+    tag = N if case.media else 0
+    b_h = w * (9 * N + 4 * slab)
+    b_e = w * (9 * N + 4 * slab + disp) + tag
+    return b_h, b_e, comp_cells
+
+
+def committed_traffic(workload, dtype, step_kernels, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary of this workload (FETCH_SIZE and WRITE_SIZE, each scaled by the
+    factor a plain copy of known size gives), or None."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}_{dtype}.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"```json\n(.*?)\n```", open(path).read(), re.S)
+    if not m:
+        return None
+    d = json.loads(m.group(1))
+    if d.get("step_kernels") != step_kernels or kernel not in d.get("kernels", {}):
+        return None
+    return d["kernels"][kernel]["bytes"]
 
 
 def timed_run(g, first, steps, record_every):
@@ -77,6 +126,11 @@ def timed_run(g, first, steps, record_every):
     g.run(first, steps, record_every)
     g.synchronize()
     return time.perf_counter() - t0
+
+
+def rel_l2(a, b):
+    nb = float(np.linalg.norm(b))
+    return float(np.linalg.norm(a - b) / nb) if nb > 0 else None
 
 
 def cpu_reference_baseline(case, steps):
@@ -144,10 +198,235 @@ def copy_calibration(mib):
     log(f"[bench] copy calibration: 5 copies of {mib} MiB ({n} float32)")
 
 
+def roofline(case, g, prec, res, workload):
+    """Roofline object of the dominant kernel of a run (res: ms_h, ms_e,
+    fused), and the algorithmic bytes of a whole step."""
+    b_h, b_e, _ = algorithmic_bytes(case, g, prec)
+    dtype = "fp32" if prec == 4 else "fp64"
+    if res["fused"]:
+        kname, b_dom, ms_dom = "k_step_fused", b_h + b_e, res["ms_e"]
+    elif res["ms_e"] >= res["ms_h"]:
+        kname, b_dom, ms_dom = "k_e_update", b_e, res["ms_e"]
+    else:
+        kname, b_dom, ms_dom = "k_h_update", b_h, res["ms_h"]
+    achieved = b_dom / (ms_dom * 1e-3) / 1e9
+    traffic = committed_traffic(workload, dtype, "fused" if res["fused"] else "separate", kname)
+    return {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "algorithmic_bytes_per_launch": b_dom, "avg_launch_ms": round(ms_dom, 5)}, b_h + b_e
+
+
+def run_single(case, prec, W, K, mode, energy_every=0):
+    """Warm-up W steps, then K timed steps with HIP events around the H and E
+    launches.  With energy_every > 0 the volume energy integral is taken
+    every energy_every steps inside the timed region."""
+    log(f"[bench] building FP{8 * prec} grid {case.grid}")
+    g = build_grid(case, prec, mode)
+    if W:
+        g.run(1, W, case.record_every)
+    g.set_profiling(True)
+    energies = []
+    if energy_every > 0:
+        g.synchronize()
+        t0 = time.perf_counter()
+        tt = W + 1
+        while tt < W + 1 + K:
+            n = min(energy_every, W + 1 + K - tt)
+            g.run(tt, n, case.record_every)
+            tt += n
+            energies.append(g.energy()[0])
+            if len(energies) % 10 == 0:
+                log(f"[bench] FP{8 * prec}: step {tt - 1} of {W + K}")
+        g.synchronize()
+        secs = time.perf_counter() - t0
+    else:
+        secs = timed_run(g, W + 1, K, case.record_every)
+    ms_h, ms_e, n = g.profile()
+    g.set_profiling(False)
+    res = dict(secs=secs, gcells=int(np.prod(case.grid)) * K / secs / 1e9, ms_step=1e3 * secs / K,
+               ms_h=ms_h / max(n, 1), ms_e=ms_e / max(n, 1), fused=g.last_run_fused(),
+               series=g.read_records(), energies=np.array(energies), dev_bytes=g.device_bytes())
+    log(f"[bench] FP{8 * prec}: {res['gcells']:.2f} Gcell/s, {res['ms_step']:.3f} ms/step, "
+        f"H {res['ms_h']:.3f} ms, E {res['ms_e']:.3f} ms")
+    return g, res
+
+
+def base_record(workload, K, W, value, ms_step, dtype, config):
+    return {"metric": METRIC if workload == HEADLINE else f"Gcell-updates/s, FP32 3D FDTD ({workload})",
+            "value": round(value, 4), "unit": "Gcell-updates/s", "n_gpus": 1, "steps": K, "warmup": W,
+            "ms_per_step": round(ms_step, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": dtype, "data": "synthetic", "config": dict({"workload": workload}, **config)}
+
+
+def precisions(args):
+    return (4,) if args.no_fp64 else (4, 8)
+
+
+def add_fp64(out, results, r32):
+    if 8 in results:
+        r64 = results[8]
+        out["fp64"] = {"value": round(r64["gcells"], 4), "ms_per_step": round(r64["ms_step"], 5),
+                       "step_roofline_frac": round(r64["step_bytes"] / (r64["ms_step"] * 1e-3) / 1e9
+                                                   / HBM_PEAK_GBS, 4)}
+        out["fp32_over_fp64"] = round(r32["gcells"] / r64["gcells"], 3)
+        out["fp32_vs_fp64_detector_relL2"] = rel_l2(r32["series"], r64["series"])
+
+
+def single_grid_runs(case, args, W, K, workload, energy_every=0, extra=None):
+    results = {}
+    for prec in precisions(args):
+        g, res = run_single(case, prec, W, K, args.mode, energy_every)
+        res["roof"], res["step_bytes"] = roofline(case, g, prec, res, workload)
+        res["comp_cells"] = algorithmic_bytes(case, g, prec)[2]
+        if extra:
+            res.update(extra(g))
+        g.kill()
+        results[prec] = res
+    return results
+
+
+def finish(out, results):
+    r32 = results[4]
+    out["roofline"] = r32["roof"]
+    out["step_roofline_frac"] = round(r32["step_bytes"] / (r32["ms_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    add_fp64(out, results, r32)
+    return out
+
+
+def bench_headline(args):
+    W, K = args.warmup, args.steps
+    case = cases.config_dl_sphere_tfsf(n=args.n, steps=W + K)
+    results = single_grid_runs(case, args, W, K, HEADLINE)
+    r32 = results[4]
+    cpu = None
+    if not args.no_cpu:
+        log(f"[bench] CPU reference baseline: {args.cpu_steps} steps")
+        cpu = cpu_reference_baseline(case, args.cpu_steps)
+    out = base_record(HEADLINE, K, W, r32["gcells"], r32["ms_step"], "fp32",
+                      {"grid": list(case.grid), "dx_nm": case.dx_nm, "npml": case.npml,
+                       "sphere_radius_nm": round(case.info["sphere_radius_nm"], 4),
+                       "dl_component_cells": r32["comp_cells"], "record_every": case.record_every,
+                       "detectors": "plane_xy Ex, Ey, Ez at k=%d" % case.info["detector_k"],
+                       "step_kernels": "fused" if r32["fused"] else "separate"})
+    finish(out, results)
+    out["cpu_baseline"] = cpu
+    if cpu:
+        out["gpu_over_cpu"] = round(r32["gcells"] / cpu["value"], 1)
+    return out
+
+
+def bench_ellipsoids(args):
+    W = args.warmup
+    K = args.steps if args.steps_given else 10000
+    case = cases.config_ellipsoids(steps=W + K)
+    results = single_grid_runs(case, args, W, K, "ellipsoids_768x768x512", energy_every=100)
+    r32 = results[4]
+    out = base_record("ellipsoids_768x768x512", K, W, r32["gcells"], r32["ms_step"], "fp32",
+                      {"grid": list(case.grid), "dx_nm": case.dx_nm, "npml": case.npml, "ellipsoids": 64,
+                       "dl_component_cells": r32["comp_cells"], "substrate_k": case.info["substrate_k"],
+                       "detectors": "32 point Ez, every 100 steps", "energy_every": 100,
+                       "step_kernels": "fused" if r32["fused"] else "separate"})
+    finish(out, results)
+    out["device_bytes_fp32"] = r32["dev_bytes"]
+    if 8 in results:
+        out["fp32_vs_fp64_energy_relL2"] = rel_l2(r32["energies"], results[8]["energies"])
+    return out
+
+
+def bench_emitters(args):
+    W = args.warmup
+    K = args.steps if args.steps_given else 20000
+    case = cases.config_emitters(steps=W + K)
+
+    def tls(g):
+        dip, rho = g.two_level_state()
+        return {"tls": np.concatenate([dip.ravel(), rho.ravel()])}
+
+    results = single_grid_runs(case, args, W, K, "emitters_512x512x384", extra=tls)
+    r32 = results[4]
+    out = base_record("emitters_512x512x384", K, W, r32["gcells"], r32["ms_step"], "fp32",
+                      {"grid": list(case.grid), "dx_nm": case.dx_nm, "npml": case.npml, "emitters": 4096,
+                       "slab_k": case.info["slab_k"], "holes": case.info["holes"],
+                       "dl_component_cells": r32["comp_cells"], "detectors": "8 point Ex, Ey, Ez, every 100 steps",
+                       "step_kernels": "fused" if r32["fused"] else "separate"})
+    finish(out, results)
+    if 8 in results:
+        out["fp32_vs_fp64_emitter_state_relL2"] = rel_l2(r32["tls"], results[8]["tls"])
+    return out
+
+
+def bench_batched(args):
+    W = args.warmup
+    K = args.steps if args.steps_given else 2000
+    cs = cases.config_batched(steps=W + K)
+    cells = sum(int(np.prod(c.grid)) for c in cs)
+    out_prec = {}
+    for prec in precisions(args):
+        log(f"[bench] batch of {len(cs)} grids, {cells} cells, FP{8 * prec}")
+        grids = [build_grid(c, prec) for c in cs]
+        b = omx_fdtd.Batch(grids)
+        if W:
+            b.run(1, W, cs[0].record_every)
+        for g in grids:
+            g.synchronize()
+        t0 = time.perf_counter()
+        b.run(W + 1, K, cs[0].record_every)
+        for g in grids:
+            g.synchronize()
+        secs = time.perf_counter() - t0
+        series = [g.read_records() for g in grids]
+        b.kill()
+        for g in grids:
+            g.kill()
+        out_prec[prec] = dict(gcells=cells * K / secs / 1e9, ms_step=1e3 * secs / K, series=series)
+        log(f"[bench] batched FP{8 * prec}: {out_prec[prec]['gcells']:.2f} Gcell/s")
+    # the same grids one after another (FP32, a bounded number of steps)
+    Ks = min(K, 200)
+    t_seq = 0.0
+    for c in cs:
+        g = build_grid(c, 4)
+        g.run(1, 2, 0)
+        t_seq += timed_run(g, 3, Ks, 0)
+        g.kill()
+    seq = cells * Ks / t_seq / 1e9
+    log(f"[bench] one by one FP32: {seq:.2f} Gcell/s")
+    # one large grid of the same cell count (CPML 8, one Drude-Lorentz sphere)
+    edge = int(round(cells ** (1.0 / 3.0)))
+    big = cases.config_dl_sphere_tfsf(n=edge, npml=8, steps=Ks + 2, record_every=0)
+    big.tfsf = None
+    g = build_grid(big, 4)
+    g.run(1, 2, 0)
+    t_big = timed_run(g, 3, Ks, 0)
+    g.kill()
+    single = edge ** 3 * Ks / t_big / 1e9
+    log(f"[bench] single {edge}^3 grid FP32: {single:.2f} Gcell/s")
+    r32 = out_prec[4]
+    out = base_record("batched_64", K, W, r32["gcells"], r32["ms_step"], "fp32",
+                      {"grids": len(cs), "edges": [96, 128, 160, 192], "total_cells": cells, "npml": 8,
+                       "record_every": cs[0].record_every, "graph": True})
+    out["one_by_one_gcells"] = round(seq, 4)
+    out["single_grid_gcells"] = round(single, 4)
+    out["single_grid_edge"] = edge
+    out["batched_over_single_grid"] = round(r32["gcells"] / single, 4)
+    out["batched_over_one_by_one"] = round(r32["gcells"] / seq, 4)
+    if 8 in out_prec:
+        r64 = out_prec[8]
+        out["fp64"] = {"value": round(r64["gcells"], 4), "ms_per_step": round(r64["ms_step"], 5)}
+        out["fp32_over_fp64"] = round(r32["gcells"] / r64["gcells"], 3)
+        rels = [rel_l2(a, b) for a, b in zip(r32["series"], r64["series"])]
+        rels = [r for r in rels if r is not None]
+        out["fp32_vs_fp64_detector_relL2_max"] = max(rels) if rels else None
+    return out
+
+
+WORKLOADS = {HEADLINE: bench_headline, "ellipsoids_768x768x512": bench_ellipsoids,
+             "batched_64": bench_batched, "emitters_512x512x384": bench_emitters}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=20)
@@ -155,6 +434,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", choices=("separate", "fused"), default="separate",
                     help="separate H and E kernels, or the fused H+E pass")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default=HEADLINE)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (HBM counter calibration)")
     args = ap.parse_args()
@@ -165,82 +445,10 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU slab "
               f"decomposition is a design, not part of this build.", file=sys.stderr)
         sys.exit(2)
-    W, K = args.warmup, args.steps
-    case = cases.config_dl_sphere_tfsf(n=args.n, steps=W + K)
-    nx, ny, nz = case.grid
-    N = nx * ny * nz
-
-    results = {}
-    series = {}
-    for prec in ((4,) if args.no_fp64 else (4, 8)):
-        log(f"[bench] building FP{8 * prec} grid {case.grid}")
-        g = build_grid(case, prec)
-        g.set_step_mode(1 if args.mode == "separate" else 0)
-        if W:
-            g.run(1, W, case.record_every)
-        # HIP events around every H and E launch of the timed steps, recorded
-        # on the engine's stream
-        g.set_profiling(True)
-        secs = timed_run(g, W + 1, K, case.record_every)
-        ms_h, ms_e, n = g.profile()
-        g.set_profiling(False)
-        series[prec] = g.read_records()
-        b_h, b_e, b_f, dl_comp = algorithmic_bytes(case, g, prec)
-        fused = g.last_run_fused()
-        results[prec] = dict(secs=secs, gcells=N * K / secs / 1e9, ms_step=1e3 * secs / K,
-                             ms_h=ms_h / n, ms_e=ms_e / n, b_h=b_h, b_e=b_e, b_f=b_f, dl_comp=dl_comp,
-                             fused=fused, dev_bytes=g.device_bytes())
-        log(f"[bench] FP{8 * prec}: {results[prec]['gcells']:.2f} Gcell/s, {results[prec]['ms_step']:.3f} ms/step, "
-            f"H {results[prec]['ms_h']:.3f} ms, E {results[prec]['ms_e']:.3f} ms")
-        g.kill()
-
-    r32 = results[4]
-    rel = None
-    if 8 in series:
-        a, b = series[4], series[8]
-        rel = float(np.linalg.norm(a - b) / np.linalg.norm(b))
-    if r32["fused"]:
-        dom, kname, b_dom, ms_dom = "e", "k_step_fused", r32["b_f"], r32["ms_e"]
-        step_bytes = r32["b_f"]
-    else:
-        dom = "e" if r32["ms_e"] >= r32["ms_h"] else "h"
-        kname = "k_e_update" if dom == "e" else "k_h_update"
-        b_dom, ms_dom = r32["b_" + dom], r32["ms_" + dom]
-        step_bytes = r32["b_h"] + r32["b_e"]
-    achieved = b_dom / (ms_dom * 1e-3) / 1e9
-    cpu = None
-    if not args.no_cpu:
-        log(f"[bench] CPU reference baseline: {args.cpu_steps} steps")
-        cpu = cpu_reference_baseline(case, args.cpu_steps)
-    out = {
-        "metric": "Gcell-updates/s, FP32 3D Drude-Lorentz + CPML FDTD (TF/SF plane wave)",
-        "value": round(r32["gcells"], 4),
-        "unit": "Gcell-updates/s",
-        "n_gpus": 1, "steps": K, "warmup": W,
-        "ms_per_step": round(r32["ms_step"], 5),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32", "data": "synthetic",
-        "config": {"workload": WORKLOAD, "grid": list(case.grid), "dx_nm": case.dx_nm, "npml": case.npml,
-                   "sphere_radius_nm": round(case.info["sphere_radius_nm"], 4),
-                   "dl_component_cells": r32["dl_comp"], "record_every": case.record_every,
-                   "detectors": "plane_xy Ex, Ey, Ez at k=%d" % case.info["detector_k"],
-                   "step_kernels": "fused" if r32["fused"] else "separate"},
-        "roofline": {"bound": "hbm", "kernel": kname,
-                     "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "algorithmic_bytes_per_launch": b_dom, "avg_launch_ms": round(ms_dom, 5)},
-        "step_roofline_frac": round(step_bytes / (r32["ms_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-        "cpu_baseline": cpu,
-    }
-    if 8 in results:
-        r64 = results[8]
-        b64 = r64["b_f"] if r64["fused"] else r64["b_h"] + r64["b_e"]
-        out["fp64"] = {"value": round(r64["gcells"], 4), "ms_per_step": round(r64["ms_step"], 5),
-                       "step_roofline_frac": round(b64 / (r64["ms_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-        out["fp32_over_fp64"] = round(r32["gcells"] / r64["gcells"], 3)
-        out["fp32_vs_fp64_detector_relL2"] = rel
-    if cpu:
-        out["gpu_over_cpu"] = round(r32["gcells"] / cpu["value"], 1)
+    args.steps_given = args.steps is not None
+    if args.steps is None:
+        args.steps = 4000
+    out = WORKLOADS[args.workload](args)
     print(json.dumps(out), flush=True)
 
 

@@ -139,14 +139,17 @@ int wrap_h(omx3d_ctx* c, hipStream_t st) {
 }
 
 template <typename T>
-int launch_h(omx3d_ctx* c) {
+// ev0/ev1 (profiling, may be null) bracket the H kernel itself.
+int launch_h(omx3d_ctx* c, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
     int rc = wrap_e<T>(c, c->s);
     if (rc) return rc;
     YeeArgs<T> a = make_args<T>(c);
     const dim3 blk(kLanes, c->rows_y);
     const unsigned nb = yee_blocks(c, a);
+    if (ev0) HIPCHK(hipEventRecord(ev0, c->s));
     if (c->cpml_mask) hipLaunchKernelGGL((k_h_update<T, true>), dim3(nb), blk, 0, c->s, a);
     else hipLaunchKernelGGL((k_h_update<T, false>), dim3(nb), blk, 0, c->s, a);
+    if (ev1) HIPCHK(hipEventRecord(ev1, c->s));
     HIPCHK(hipGetLastError());
     return OMX_OK;
 }
@@ -169,33 +172,36 @@ int launch_j_correct(omx3d_ctx* c, double tau) {
 // E of the cells of dispersive classes: a's E arrays are the input, the
 // output arrays get the result (the same arrays for the separate kernels).
 template <typename T>
-int launch_e_dispersive(omx3d_ctx* c, const YeeArgs<T>& a, T* exo, T* eyo, T* ezo) {
+int launch_e_dispersive(omx3d_ctx* c, const YeeArgs<T>& a, T* exo, T* eyo, T* ezo, hipStream_t st) {
     const unsigned nb = blocks_for(c->n_disp, 256);
     if (c->cpml_mask)
-        hipLaunchKernelGGL((k_e_dispersive<T, true>), dim3(nb), dim3(256), 0, c->s, a, exo, eyo, ezo, c->d_disp,
+        hipLaunchKernelGGL((k_e_dispersive<T, true>), dim3(nb), dim3(256), 0, st, a, exo, eyo, ezo, c->d_disp,
                            c->n_disp);
     else
-        hipLaunchKernelGGL((k_e_dispersive<T, false>), dim3(nb), dim3(256), 0, c->s, a, exo, eyo, ezo, c->d_disp,
+        hipLaunchKernelGGL((k_e_dispersive<T, false>), dim3(nb), dim3(256), 0, st, a, exo, eyo, ezo, c->d_disp,
                            c->n_disp);
     HIPCHK(hipGetLastError());
     return OMX_OK;
 }
 
+// ev0/ev1 (profiling, may be null) bracket the E tile kernel itself.
 template <typename T>
-int launch_e(omx3d_ctx* c, double time, double t_skip) {
+int launch_e(omx3d_ctx* c, double time, double t_skip, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
     int rc = wrap_h<T>(c, c->s);
     if (rc) return rc;
     YeeArgs<T> a = make_args<T>(c);
     const dim3 blk(kLanes, c->rows_y);
     const unsigned nb = yee_blocks(c, a);
     const bool cp = c->cpml_mask != 0, md = c->media_set && c->n_classes > 1;
+    if (ev0) HIPCHK(hipEventRecord(ev0, c->s));
     if (cp && md) hipLaunchKernelGGL((k_e_update<T, true, 1>), dim3(nb), blk, 0, c->s, a);
     else if (cp) hipLaunchKernelGGL((k_e_update<T, true, 0>), dim3(nb), blk, 0, c->s, a);
     else if (md) hipLaunchKernelGGL((k_e_update<T, false, 1>), dim3(nb), blk, 0, c->s, a);
     else hipLaunchKernelGGL((k_e_update<T, false, 0>), dim3(nb), blk, 0, c->s, a);
+    if (ev1) HIPCHK(hipEventRecord(ev1, c->s));
     HIPCHK(hipGetLastError());
     if (md && c->n_disp > 0) {
-        rc = launch_e_dispersive<T>(c, a, a.ex, a.ey, a.ez);
+        rc = launch_e_dispersive<T>(c, a, a.ex, a.ey, a.ez, c->s);
         if (rc) return rc;
     }
     return launch_j_correct<T>(c, time - t_skip);
@@ -395,7 +401,7 @@ int upload_fused_cold(omx3d_ctx* c, const FusedCold<T>& fc, const FusedCold<T>**
 }
 
 template <typename T>
-int launch_fused(omx3d_ctx* c) {
+int launch_fused(omx3d_ctx* c, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
     int rc = ensure_alt(c);
     if (rc) return rc;
     FusedCold<T> fc;
@@ -442,6 +448,7 @@ int launch_fused(omx3d_ctx* c) {
     const bool cp = c->cpml_mask != 0, md = c->media_set && c->n_classes > 1, tf = c->tfsf;
 #define OMX_FUSED(CP, MD, TF) \
     hipLaunchKernelGGL((k_step_fused<T, CP, MD, TF>), dim3(nb), blk, lds, c->s, f, dcold)
+    if (ev0) HIPCHK(hipEventRecord(ev0, c->s));
     if (cp && md && tf) OMX_FUSED(true, true, true);
     else if (cp && md) OMX_FUSED(true, true, false);
     else if (cp && tf) OMX_FUSED(true, false, true);
@@ -451,6 +458,7 @@ int launch_fused(omx3d_ctx* c) {
     else if (tf) OMX_FUSED(false, false, true);
     else OMX_FUSED(false, false, false);
 #undef OMX_FUSED
+    if (ev1) HIPCHK(hipEventRecord(ev1, c->s));
     HIPCHK(hipGetLastError());
     for (int q = 0; q < 6; ++q) std::swap(c->f[q], c->f_alt[q]);
     for (int q = 0; q < 6; ++q) std::swap(c->psi_h[q], c->psi_h_alt[q]);
@@ -460,7 +468,7 @@ int launch_fused(omx3d_ctx* c) {
         // other buffer, E^{n+1} into the current one
         YeeArgs<T> an = make_args<T>(c);
         an.ex = (T*)c->f_alt[0]; an.ey = (T*)c->f_alt[1]; an.ez = (T*)c->f_alt[2];
-        rc = launch_e_dispersive<T>(c, an, (T*)c->f[0], (T*)c->f[1], (T*)c->f[2]);
+        rc = launch_e_dispersive<T>(c, an, (T*)c->f[0], (T*)c->f[1], (T*)c->f[2], c->s);
         if (rc) return rc;
     }
     return OMX_OK;
@@ -485,35 +493,32 @@ int run_impl(omx3d_ctx* c, int tt_first, int n_steps, int record_every) {
         // n_skip_steps = 1: a new quantum step every step, so t_skip = time
         // (mxim_mxll.f90:73-77,97-101).
         const double t_skip = time;
+        // profiling: events bracket the H and E kernels themselves (the
+        // fused kernel takes the E slot, the H slot stays empty)
+        hipEvent_t* ev = c->prof ? &c->ev[evi] : nullptr;
+        if (c->prof) evi += 4;
         if (fused) {
             // H and E in one pass; sources then enter E after its update
             // (same value as before it in vacuum/dielectric source cells).
-            // Profiling: an empty H slot, the fused kernel in the E slot.
-            if (c->prof) {
-                HIPCHK(hipEventRecord(c->ev[evi++], c->s));
-                HIPCHK(hipEventRecord(c->ev[evi++], c->s));
-                HIPCHK(hipEventRecord(c->ev[evi++], c->s));
+            if (ev) {
+                HIPCHK(hipEventRecord(ev[0], c->s));
+                HIPCHK(hipEventRecord(ev[1], c->s));
             }
-            rc = launch_fused<T>(c);
+            rc = launch_fused<T>(c, ev ? ev[2] : nullptr, ev ? ev[3] : nullptr);
             if (rc) return rc;
-            if (c->prof) HIPCHK(hipEventRecord(c->ev[evi++], c->s));
             rc = launch_sources<T>(c, time);
             if (rc) return rc;
             rc = launch_j_correct<T>(c, time - t_skip);
             if (rc) return rc;
         } else {
-            if (c->prof) HIPCHK(hipEventRecord(c->ev[evi++], c->s));
-            rc = launch_h<T>(c);
+            rc = launch_h<T>(c, ev ? ev[0] : nullptr, ev ? ev[1] : nullptr);
             if (rc) return rc;
-            if (c->prof) HIPCHK(hipEventRecord(c->ev[evi++], c->s));
             rc = launch_tfsf_h<T>(c);
             if (rc) return rc;
             rc = launch_sources<T>(c, time);
             if (rc) return rc;
-            if (c->prof) HIPCHK(hipEventRecord(c->ev[evi++], c->s));
-            rc = launch_e<T>(c, time, t_skip);
+            rc = launch_e<T>(c, time, t_skip, ev ? ev[2] : nullptr, ev ? ev[3] : nullptr);
             if (rc) return rc;
-            if (c->prof) HIPCHK(hipEventRecord(c->ev[evi++], c->s));
         }
         rc = launch_tfsf_e<T>(c, dt * (double)tt);
         if (rc) return rc;
@@ -676,7 +681,11 @@ int build_tags(omx3d_ctx* c) {
             for (int j = 1; j <= L.ny; ++j)
                 for (int k = 1; k <= L.nz; ++k) {
                     const long o = L.at(i, j, k);
-                    if (tags[o] >= c->disp_tag0) cells.push_back(o);
+                    const int tg = tags[o];
+                    if (tg >= c->disp_tag0) {
+                        const long m0 = cls[3 * tg], m1 = cls[3 * tg + 1], m2 = cls[3 * tg + 2];
+                        cells.push_back(o | (m0 << 48) | (m1 << 52) | (m2 << 56));
+                    }
                 }
         if (cells.size() > (size_t)INT32_MAX) return fail(OMX_ERR_RANGE, "too many dispersive cells");
         if (!cells.empty()) {

@@ -564,23 +564,29 @@ __device__ __forceinline__ T medium_update(T e, T rot, T vac, int m, const MedDe
 // MED: 0 no media; 1 vacuum and dielectric components only (the caller
 // leaves cells of the Drude/Drude-Lorentz classes, tag >= disp_tag0, to the
 // dispersive-cell pass); 2 every medium kind.
-template <typename T, int MED>
-__device__ __forceinline__ void e_bulk(const YeeArgs<T>& a, int tg, long o, T dep, T hzc, T hzj, T hzi, T hyc,
-                                       T hyk, T hyi, T hxc, T hxj, T hxk, T dex, T cex_, T dey, T dez, T cey, T cez,
-                                       T& ex, T& ey, T& ez) {
-    const T rx = (hzc - hzj) * dey + (hyk - hyc) * dez;
-    const T ry = (hzi - hzc) * dex + (hxc - hxk) * dez;
-    const T rz = (hyc - hyi) * dex + (hxj - hxc) * dey;
-    // vacuum increments; FP32 folds dt_eps0*den into one coefficient per axis
-    T vx, vy, vz;
+// Curl of H at an E cell (rot, for the medium updates) and the vacuum
+// increments dt_eps0 * rot (FP32: folded coefficients dt_eps0*den).
+template <typename T>
+__device__ __forceinline__ void e_curl(T dep, T hzc, T hzj, T hzi, T hyc, T hyk, T hyi, T hxc, T hxj, T hxk, T dex,
+                                       T cex, T dey, T dez, T cey, T cez, T& rx, T& ry, T& rz, T& vx, T& vy, T& vz) {
+    rx = (hzc - hzj) * dey + (hyk - hyc) * dez;
+    ry = (hzi - hzc) * dex + (hxc - hxk) * dez;
+    rz = (hyc - hyi) * dex + (hxj - hxc) * dey;
     if constexpr (sizeof(T) == 4) {
-        const T cex = cex_;
         vx = (hzc - hzj) * cey + (hyk - hyc) * cez;
         vy = (hzi - hzc) * cex + (hxc - hxk) * cez;
         vz = (hyc - hyi) * cex + (hxj - hxc) * cey;
     } else {
         vx = dep * rx; vy = dep * ry; vz = dep * rz;
     }
+}
+
+template <typename T, int MED>
+__device__ __forceinline__ void e_bulk(const YeeArgs<T>& a, int tg, long o, T dep, T hzc, T hzj, T hzi, T hyc,
+                                       T hyk, T hyi, T hxc, T hxj, T hxk, T dex, T cex_, T dey, T dez, T cey, T cez,
+                                       T& ex, T& ey, T& ez) {
+    T rx, ry, rz, vx, vy, vz;
+    e_curl(dep, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dex, cex_, dey, dez, cey, cez, rx, ry, rz, vx, vy, vz);
     if (MED == 1) {
         if (tg == 0) {
             ex = ex + vx; ey = ey + vy; ez = ez + vz;
@@ -794,13 +800,18 @@ __global__ __launch_bounds__(256) void k_e_update(const YeeArgs<T> a) {
 // after the tile pass has updated every other cell: the per-cell update of
 // e_cell with the full medium path, from E in a.ex/a.ey/a.ez to the output
 // arrays (the same arrays in the separate H/E path, the other buffer after
-// the fused step).  list holds the cells' element offsets.
+// the fused step).  list holds the cells' element offsets with the media of
+// their three components (the same arithmetic as e_cell).
 template <typename T, bool CPML>
 __global__ __launch_bounds__(256) void k_e_dispersive(const YeeArgs<T> a, T* __restrict__ exo, T* __restrict__ eyo,
                                                       T* __restrict__ ezo, const long* __restrict__ list, int n) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
-    const long o = list[p];
+    // entry: element offset in bits 0-47, the media of the x, y, z
+    // components in bits 48-51, 52-55, 56-59
+    const long e = list[p];
+    const long o = e & ((1L << 48) - 1);
+    const int m0 = (int)((e >> 48) & 15), m1 = (int)((e >> 52) & 15), m2 = (int)((e >> 56) & 15);
     // o = i*sx + j*sz + k + off with 0 <= k < sz and j*sz + k < sx
     const long u = o - a.off;
     const int i = (int)(u / a.sx);
@@ -818,8 +829,17 @@ __global__ __launch_bounds__(256) void k_e_dispersive(const YeeArgs<T> a, T* __r
     const T dex = a.den_ex[i], cex = sizeof(T) == 4 ? a.cdex[i] : (T)0;
     const T dey = a.den_ey[j], dez = a.den_ez[k];
     const T cey = sizeof(T) == 4 ? a.cdey[j] : (T)0, cez = sizeof(T) == 4 ? a.cdez[k] : (T)0;
-    e_cell<T, CPML, true>(a, (int)a.tag[o], i, j, k, o, jk, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dex, cex, dey,
-                          dez, cey, cez, ex, ey, ez);
+    const T dep = a.dt_eps0;
+    T rx, ry, rz, vx, vy, vz;
+    e_curl(dep, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dex, cex, dey, dez, cey, cez, rx, ry, rz, vx, vy, vz);
+    ex = medium_update(ex, rx, vx, m0, a.med, o, a.eold[0], a.pd[0], a.pl[0], a.plold[0], a.pstride);
+    ey = medium_update(ey, ry, vy, m1, a.med, o, a.eold[1], a.pd[1], a.pl[1], a.plold[1], a.pstride);
+    ez = medium_update(ez, rz, vz, m2, a.med, o, a.eold[2], a.pd[2], a.pl[2], a.plold[2], a.pstride);
+    if constexpr (CPML) {
+        e_cpml_x(a, i, jk, dep, hzc, hzi, hyc, hyi, ey, ez);
+        e_cpml_y(a, i, j, k, dep, hzc, hzj, hxc, hxj, ex, ez);
+        e_cpml_z(a, i, j, k, dep, hyc, hyk, hxc, hxk, ex, ey);
+    }
     exo[o] = ex; eyo[o] = ey; ezo[o] = ez;
 }
 
