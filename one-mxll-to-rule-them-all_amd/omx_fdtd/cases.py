@@ -224,7 +224,13 @@ def config_ellipsoids(grid=(768, 768, 512), dx_nm=1.0, npml=16, steps=10000, n_e
     uniform random orientations, centres in the upper three quarters) above a
     dielectric substrate (eps 2.25, lower quarter of the grid); a z-dipole
     point source above the substrate; 32 point detectors recording Ez; the
-    volume energy integral is taken every `record_every` steps."""
+    volume energy integral is taken every `record_every` steps.
+
+    The substrate stops where the CPML begins: the reference adds the CPML
+    term dt/eps0 * psi to E without the 1/eps_r of the medium
+    (mxll_3D_mod.f90:826, :897 and the other faces), and with eps_r != 1
+    inside the CPML that update grows without bound (max |Ez| 1e24 after 800
+    steps of a 64x64x48 version on the oracle)."""
     rng = np.random.default_rng(SEED_BASE + seed_no)
     nx, ny, nz = grid
     med = dl_sphere_poles(rng)
@@ -238,7 +244,7 @@ def config_ellipsoids(grid=(768, 768, 512), dx_nm=1.0, npml=16, steps=10000, n_e
              float(rng.uniform(npml + amax + 2, ny - npml - amax - 2)),
              float(rng.uniform(k_sub + amax + 2, nz - npml - amax - 2))]
         shapes.append((2, 1, c + ax.tolist() + random_rotation(rng)))
-    shapes.append((3, 2, [1, 1, 1, nx, ny, k_sub]))
+    shapes.append((3, 2, [npml + 1, npml + 1, npml + 1, nx - npml, ny - npml, k_sub]))
     src_cell = [nx // 2, ny // 2, int(0.6 * nz)]
     pos_nm = [index_to_nm(src_cell[a], dx_nm, grid[a]) for a in range(3)]
     srcs = [("z", 1.0, 2.066, 3.0, 1.0, pos_nm[0], pos_nm[1], pos_nm[2], 0.0, 0.0, 1.0e6, 0.0)]
@@ -253,13 +259,15 @@ def config_emitters(grid=(512, 512, 384), dx_nm=2.0, npml=12, steps=20000, n_emi
     """Silver (library Ag) slab with a seeded nanohole array, 4096 two-level
     emitters (transition 2.0-2.2 eV, dipole 5-15 D, dephasing 10-50 meV) at
     seeded cells in a 20 nm layer above the slab, local-dipole coupling; a
-    z-dipole point source above the layer excites them."""
+    z-dipole point source above the layer excites them.  The slab stops where
+    the x and y CPML begin (a medium inside the CPML is unstable under the
+    reference's CPML update, see config_ellipsoids)."""
     rng = np.random.default_rng(SEED_BASE + seed_no)
     nx, ny, nz = grid
     t = int(round(slab_nm / dx_nm))
     k0 = nz // 2 - t
     k1 = k0 + t - 1
-    shapes = [(3, 1, [1, 1, k0, nx, ny, k1])]
+    shapes = [(3, 1, [npml + 1, npml + 1, k0, nx - npml, ny - npml, k1])]
     pitch = float(rng.uniform(80.0, 120.0)) / dx_nm
     nh = int((nx - 2 * npml) // pitch)
     for a in range(nh):

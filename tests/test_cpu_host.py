@@ -12,7 +12,7 @@ from scipy.integrate import solve_ivp
 from scipy.linalg import expm
 
 import omx_fdtd
-from omx_fdtd import host, units
+from omx_fdtd import cases, host, units
 from oracle import oracle
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -141,3 +141,32 @@ def test_engine_errors_are_reported():
     assert omx_fdtd.load_library().omx3d_create(C.byref(d), C.byref(h)) == 1
     with pytest.raises(omx_fdtd.OmxError):
         omx_fdtd.Grid3D((10, 10, 10), 8, (3, 3, 3), 0.1, 1.0)
+
+
+def test_workload_media_stay_out_of_the_cpml():
+    """The reference adds dt/eps0 * psi to E inside the CPML whatever the
+    medium (mxll_3D_mod.f90:826), which grows without bound where a medium
+    with eps_r != 1 reaches into the CPML.  The workload builders keep their
+    substrate and slab out of it; on the oracle the scaled ellipsoid case
+    stays bounded, and the same case with the substrate reaching into the
+    CPML does not."""
+    import dataclasses
+    from oracle import oracle as O
+    base = cases.config_ellipsoids(grid=(48, 48, 40), n_ell=4, npml=6, steps=600, record_every=0)
+    nx, ny, nz = base.grid
+    sub = [s for s in base.shapes if s[0] == 3][0]
+    assert sub[2][:3] == [base.npml + 1] * 3 and sub[2][3:5] == [nx - base.npml, ny - base.npml]
+    slab = cases.config_emitters(grid=(64, 64, 48), n_emit=4, npml=6, steps=10).shapes[0]
+    assert slab[2][:2] == [7, 7] and slab[2][3:5] == [58, 58]
+    into = dataclasses.replace(base, shapes=[(3, 2, [1, 1, 1, nx, ny, base.info["substrate_k"]])])
+    peaks = []
+    for c in (base, into):
+        dr, dt = c.dr_dt
+        g = O.OracleGrid(c.grid, c.npml, c.boundaries, dt, dr)
+        cases.apply(c, g, library=O.library_medium, make_medium=O.make_medium, rasterize=O.rasterize)
+        g.run(1, 600, 0)
+        peaks.append(float(np.abs(g.get_field("Ez")).max()))
+        g.kill()
+    print(f"max |Ez| after 600 steps: substrate outside the CPML {peaks[0]:.3e}, inside {peaks[1]:.3e}")
+    assert peaks[0] < 1e-3
+    assert peaks[1] > 1e3
