@@ -73,12 +73,15 @@ def resolved_media(case):
 
 
 def algorithmic_bytes(case, g, w):
-    """Bytes per H and per E launch: every live array read once and written
-    once (per cell: H 9 words, E 9 words), plus 4 words per CPML slab cell and
-    half step (two psi arrays, read and written), plus per dispersive
-    component cell 2 words (Drude: PD) or 4 + 4P words (Drude-Lorentz with P
-    poles: E_old, PD, PL_k, PL_old_k), all read and written, plus the
-    one-byte cell tag when the grid holds media."""
+    """Bytes per H and per E half step: every live array read once and
+    written once (per cell: H 9 words, E 9 words), plus 4 words per CPML slab
+    cell and half step (two psi arrays, read and written), plus per
+    dispersive component cell 2 words (Drude: PD) or 4 + 4P words
+    (Drude-Lorentz with P poles: E_old, PD, PL_k, PL_old_k), all read and
+    written, plus the one-byte cell tag when the grid holds media.  Returns
+    (H bytes, E bytes, dispersive-state bytes, dispersive component cells);
+    the E tile kernel moves E bytes minus the dispersive state, which the
+    dispersive-cell pass moves."""
     nx, ny, nz = case.grid
     N = nx * ny * nz
     p = case.npml
@@ -101,7 +104,7 @@ def algorithmic_bytes(case, g, w):
     tag = N if case.media else 0
     b_h = w * (9 * N + 4 * slab)
     b_e = w * (9 * N + 4 * slab + disp) + tag
-    return b_h, b_e, comp_cells
+    return b_h, b_e, w * disp, comp_cells
 
 
 def committed_traffic(workload, dtype, step_kernels, kernel):
@@ -201,12 +204,12 @@ def copy_calibration(mib):
 def roofline(case, g, prec, res, workload):
     """Roofline object of the dominant kernel of a run (res: ms_h, ms_e,
     fused), and the algorithmic bytes of a whole step."""
-    b_h, b_e, _ = algorithmic_bytes(case, g, prec)
+    b_h, b_e, b_disp, _ = algorithmic_bytes(case, g, prec)
     dtype = "fp32" if prec == 4 else "fp64"
     if res["fused"]:
-        kname, b_dom, ms_dom = "k_step_fused", b_h + b_e, res["ms_e"]
+        kname, b_dom, ms_dom = "k_step_fused", b_h + b_e - b_disp, res["ms_e"]
     elif res["ms_e"] >= res["ms_h"]:
-        kname, b_dom, ms_dom = "k_e_update", b_e, res["ms_e"]
+        kname, b_dom, ms_dom = "k_e_update", b_e - b_disp, res["ms_e"]
     else:
         kname, b_dom, ms_dom = "k_h_update", b_h, res["ms_h"]
     achieved = b_dom / (ms_dom * 1e-3) / 1e9
@@ -277,7 +280,7 @@ def single_grid_runs(case, args, W, K, workload, energy_every=0, extra=None):
     for prec in precisions(args):
         g, res = run_single(case, prec, W, K, args.mode, energy_every)
         res["roof"], res["step_bytes"] = roofline(case, g, prec, res, workload)
-        res["comp_cells"] = algorithmic_bytes(case, g, prec)[2]
+        res["comp_cells"] = algorithmic_bytes(case, g, prec)[3]
         if extra:
             res.update(extra(g))
         g.kill()
@@ -355,6 +358,32 @@ def bench_emitters(args):
     return out
 
 
+def batched_as_one_grid(cs):
+    """One cubic grid with the cell count of the batch and the same media:
+    block q of a 4x4x4 arrangement carries the spheres of grid q (positions
+    scaled into the block), one dipole source per block, one point detector
+    per block."""
+    cells = sum(int(np.prod(c.grid)) for c in cs)
+    edge = int(round(cells ** (1.0 / 3.0)))
+    nb = 4
+    blk = edge / nb
+    shapes, srcs, dets = [], [], []
+    c0 = cs[0]
+    for q, c in enumerate(cs[: nb ** 3]):
+        o = np.array([q % nb, (q // nb) % nb, q // (nb * nb)]) * blk
+        sc = blk / np.array(c.grid, dtype=float)
+        for (typ, med, p) in c.shapes:
+            ctr = o + np.array(p[:3]) * sc
+            shapes.append((typ, med, ctr.tolist() + [float(p[3]) * float(sc.min())]))
+        cell = (o + blk / 2).astype(int)
+        pos = [index_to_nm(int(cell[a]), c.dx_nm, edge) for a in range(3)]
+        srcs.append((c.sources[2][0], c.sources[2][1]) + c.sources[2][2:5] + tuple(pos) + c.sources[2][8:])
+        dets.append(tuple(int(v) + 3 for v in cell))
+    return cases.Case(name="batch_as_one", grid=(edge, edge, edge), dx_nm=c0.dx_nm, npml=c0.npml,
+                      boundaries=c0.boundaries, steps=c0.steps, record_every=c0.record_every, media=c0.media[:1],
+                      shapes=shapes, sources=srcs, detectors=cases.point_detectors(dets))
+
+
 def bench_batched(args):
     W = args.warmup
     K = args.steps if args.steps_given else 2000
@@ -385,18 +414,18 @@ def bench_batched(args):
     t_seq = 0.0
     for c in cs:
         g = build_grid(c, 4)
-        g.run(1, 2, 0)
-        t_seq += timed_run(g, 3, Ks, 0)
+        g.run(1, 2, c.record_every)
+        t_seq += timed_run(g, 3, Ks, c.record_every)
         g.kill()
     seq = cells * Ks / t_seq / 1e9
     log(f"[bench] one by one FP32: {seq:.2f} Gcell/s")
-    # one large grid of the same cell count (CPML 8, one Drude-Lorentz sphere)
-    edge = int(round(cells ** (1.0 / 3.0)))
-    big = cases.config_dl_sphere_tfsf(n=edge, npml=8, steps=Ks + 2, record_every=0)
-    big.tfsf = None
+    # one large grid of the same cell count holding the same content: a 4x4x4
+    # arrangement of blocks, block q carrying grid q's spheres (CPML 8 outside)
+    big = batched_as_one_grid(cs)
+    edge = big.grid[0]
     g = build_grid(big, 4)
-    g.run(1, 2, 0)
-    t_big = timed_run(g, 3, Ks, 0)
+    g.run(1, 2, big.record_every)
+    t_big = timed_run(g, 3, Ks, big.record_every)
     g.kill()
     single = edge ** 3 * Ks / t_big / 1e9
     log(f"[bench] single {edge}^3 grid FP32: {single:.2f} Gcell/s")

@@ -801,6 +801,11 @@ struct omxb_ctx {
     int* d_overflow = nullptr;
     long* d_step = nullptr;
     bool periodic = false;
+    int rows = 1;                   // workgroup height: the largest rows_y of the grids
+    long* d_disp = nullptr;         // cells of dispersive classes of every grid (packed as per grid)
+    int* d_disp_grid = nullptr;     // their grid
+    int n_disp = 0;
+    bool disp_cpml = false;
 };
 
 namespace {
@@ -827,9 +832,31 @@ int batch_setup(omxb_ctx* b) {
     }
     ts.push_back(tot);
     b->total_tiles = tot;
+    b->rows = 1;
+    for (auto* c : b->g) b->rows = std::max(b->rows, c->rows_y);
     int rc = bupload(b->s, (YeeArgs<T>**)&b->d_args, args);
     if (!rc) rc = bupload(b->s, &b->d_tile_start, ts);
     if (rc) return rc;
+    // cells of dispersive classes of all grids, for one batched pass
+    {
+        std::vector<long> cells;
+        std::vector<int> gof;
+        b->disp_cpml = false;
+        for (int q = 0; q < b->n; ++q) {
+            omx3d_ctx* c = b->g[q];
+            if (!(c->media_set && c->n_classes > 1 && c->n_disp > 0)) continue;
+            std::vector<long> h(c->n_disp);
+            HIPCHK(hipMemcpy(h.data(), c->d_disp, sizeof(long) * h.size(), hipMemcpyDeviceToHost));
+            cells.insert(cells.end(), h.begin(), h.end());
+            gof.insert(gof.end(), h.size(), q);
+            if (c->cpml_mask) b->disp_cpml = true;
+        }
+        if (cells.size() > (size_t)INT32_MAX) return fail(OMX_ERR_RANGE, "too many dispersive cells in the batch");
+        b->n_disp = (int)cells.size();
+        rc = bupload(b->s, &b->d_disp, cells);
+        if (!rc) rc = bupload(b->s, &b->d_disp_grid, gof);
+        if (rc) return rc;
+    }
     // sources: one pass per repeated (field, cell), in source order
     std::map<std::pair<void*, long>, int> seen;
     std::vector<std::vector<void*>> pf;
@@ -893,7 +920,7 @@ int batch_setup(omxb_ctx* b) {
 
 template <typename T>
 int batch_step_launches(omxb_ctx* b) {
-    const dim3 blk(kLanes, 4);
+    const dim3 blk(kLanes, b->rows);
     if (b->periodic)
         for (auto* c : b->g) { int rc = wrap_e<T>(c, b->s); if (rc) return rc; }
     hipLaunchKernelGGL(k_h_update_batched<T>, dim3(b->total_tiles), blk, 0, b->s, (const YeeArgs<T>*)b->d_args,
@@ -908,17 +935,9 @@ int batch_step_launches(omxb_ctx* b) {
         for (auto* c : b->g) { int rc = wrap_h<T>(c, b->s); if (rc) return rc; }
     hipLaunchKernelGGL(k_e_update_batched<T>, dim3(b->total_tiles), blk, 0, b->s, (const YeeArgs<T>*)b->d_args,
                        b->d_tile_start, b->n);
-    for (auto* c : b->g) {
-        if (!(c->media_set && c->n_classes > 1 && c->n_disp > 0)) continue;
-        const YeeArgs<T> a = make_args<T>(c);
-        const unsigned nb = blocks_for(c->n_disp, 256);
-        if (c->cpml_mask)
-            hipLaunchKernelGGL((k_e_dispersive<T, true>), dim3(nb), dim3(256), 0, b->s, a, a.ex, a.ey, a.ez,
-                               c->d_disp, c->n_disp);
-        else
-            hipLaunchKernelGGL((k_e_dispersive<T, false>), dim3(nb), dim3(256), 0, b->s, a, a.ex, a.ey, a.ez,
-                               c->d_disp, c->n_disp);
-    }
+    if (b->n_disp > 0)
+        hipLaunchKernelGGL(k_e_dispersive_batched<T>, dim3(blocks_for(b->n_disp, 256)), dim3(256), 0, b->s,
+                           (const YeeArgs<T>*)b->d_args, b->d_disp, b->d_disp_grid, b->n_disp);
     if (b->n_samp > 0)
         hipLaunchKernelGGL(k_sample_batched<T>, dim3(blocks_for(b->n_samp, 256)), dim3(256), 0, b->s,
                            (const YeeArgs<T>*)b->d_args, b->d_br, b->d_grid_of, b->d_local, b->d_code, b->d_off,
@@ -1700,7 +1719,7 @@ int omxb_destroy(omxb_ctx* b) {
     dfree(b->d_args); dfreet(b->d_tile_start);
     dfreet(b->d_src_field); dfreet(b->d_src_off); dfreet(b->d_src_ka); dfreet(b->d_src_kb); dfreet(b->d_src_of);
     dfreet(b->d_sp); dfreet(b->d_br); dfreet(b->d_grid_of); dfreet(b->d_local); dfreet(b->d_code); dfreet(b->d_off);
-    dfreet(b->d_overflow); dfreet(b->d_step);
+    dfreet(b->d_overflow); dfreet(b->d_step); dfreet(b->d_disp); dfreet(b->d_disp_grid);
     if (b->s) (void)hipStreamDestroy(b->s);
     delete b;
     return OMX_OK;

@@ -803,13 +803,10 @@ __global__ __launch_bounds__(256) void k_e_update(const YeeArgs<T> a) {
 // the fused step).  list holds the cells' element offsets with the media of
 // their three components (the same arithmetic as e_cell).
 template <typename T, bool CPML>
-__global__ __launch_bounds__(256) void k_e_dispersive(const YeeArgs<T> a, T* __restrict__ exo, T* __restrict__ eyo,
-                                                      T* __restrict__ ezo, const long* __restrict__ list, int n) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
+__device__ __forceinline__ void e_dispersive_cell(const YeeArgs<T>& a, T* __restrict__ exo, T* __restrict__ eyo,
+                                                  T* __restrict__ ezo, long e) {
     // entry: element offset in bits 0-47, the media of the x, y, z
     // components in bits 48-51, 52-55, 56-59
-    const long e = list[p];
     const long o = e & ((1L << 48) - 1);
     const int m0 = (int)((e >> 48) & 15), m1 = (int)((e >> 52) & 15), m2 = (int)((e >> 56) & 15);
     // o = i*sx + j*sz + k + off with 0 <= k < sz and j*sz + k < sx
@@ -841,6 +838,27 @@ __global__ __launch_bounds__(256) void k_e_dispersive(const YeeArgs<T> a, T* __r
         e_cpml_z(a, i, j, k, dep, hyc, hyk, hxc, hxk, ex, ey);
     }
     exo[o] = ex; eyo[o] = ey; ezo[o] = ez;
+}
+
+template <typename T, bool CPML>
+__global__ __launch_bounds__(256) void k_e_dispersive(const YeeArgs<T> a, T* __restrict__ exo, T* __restrict__ eyo,
+                                                      T* __restrict__ ezo, const long* __restrict__ list, int n) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    e_dispersive_cell<T, CPML>(a, exo, eyo, ezo, list[p]);
+}
+
+// The dispersive cells of every grid of a batch (in place, like the
+// separate kernels); grid_of[p] selects the argument block.
+template <typename T>
+__global__ __launch_bounds__(256) void k_e_dispersive_batched(const YeeArgs<T>* __restrict__ args,
+                                                              const long* __restrict__ list,
+                                                              const int* __restrict__ grid_of, int n) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const YeeArgs<T>& a = args[grid_of[p]];
+    if (a.cpml) e_dispersive_cell<T, true>(a, a.ex, a.ey, a.ez, list[p]);
+    else e_dispersive_cell<T, false>(a, a.ex, a.ey, a.ez, list[p]);
 }
 
 // ---------------------------------------------------------------------------
