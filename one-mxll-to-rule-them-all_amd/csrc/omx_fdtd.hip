@@ -436,15 +436,17 @@ int launch_fused(omx3d_ctx* c, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullpt
     }
     const int FR = c->fused_rows;
     f.xchunk = c->fused_xchunk;
-    f.ntz = (c->L.nz + kLanes - 2) / (kLanes - 1);
+    f.ntz = (c->L.nz + kLanes - 1) / kLanes;        // walked inside each workgroup
     f.nty = (c->L.ny + FR - 2) / (FR - 1);
     const long ntx = (c->L.nx + f.xchunk - 1) / f.xchunk;
-    const unsigned nb = (unsigned)((long)f.ntz * f.nty * ntx);
+    const unsigned nb = (unsigned)((long)f.nty * ntx);
     if (c->tfsf)
         hipLaunchKernelGGL(k_tfsf_1d_h, dim3(blocks_for(c->tf_n, 256)), dim3(256), 0, c->s, c->d_tf,
                            c->d_tf + c->tf_n, c->tf_n, (c->d.dt / kMu0) * (1.0 / c->d.dr));
     const dim3 blk(kLanes, FR);
-    const size_t lds = sizeof(T) * 10 * FR * kLanes;   // E rows 2 x 3 planes, H rows 2 x 2
+    // E rows 2 x 3 planes, H rows 2 x 2 planes, H handed between z tiles
+    const size_t lds = sizeof(T) * (10 * FR * kLanes + 4 * (f.xchunk + 1) * FR);
+    if (lds > 160 * 1024) return fail(OMX_ERR_ARG, "fused step tile needs more than 160 KiB of LDS");
     const bool cp = c->cpml_mask != 0, md = c->media_set && c->n_classes > 1, tf = c->tfsf;
 #define OMX_FUSED(CP, MD, TF) \
     hipLaunchKernelGGL((k_step_fused<T, CP, MD, TF>), dim3(nb), blk, lds, c->s, f, dcold)

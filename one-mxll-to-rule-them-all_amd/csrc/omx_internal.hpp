@@ -108,7 +108,7 @@ struct omx3d_ctx {
     void* psi_h_alt[6] = {};
     int step_mode = 1;          // 0: fused step where it applies, 1: separate H and E kernels
     int fused_rows = 4;         // rows per workgroup of the fused step (incl. the halo row)
-    int fused_xchunk = 16;
+    int fused_xchunk = 4;
     int src_in_dispersive = -1; // -1 unknown, 0 no, 1 a source point sits in a Drude/DL component
     bool last_fused = false;
     // fused step: device-resident cold argument blocks, one per buffer parity

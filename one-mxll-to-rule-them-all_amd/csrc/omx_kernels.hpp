@@ -865,14 +865,17 @@ __global__ __launch_bounds__(256) void k_e_dispersive_batched(const YeeArgs<T>* 
 // Fused step: H^{n+1/2} and E^{n+1} in one pass over HBM.  E and H are read
 // from the current buffers and written to the other pair, so every array is
 // read once and written once per step (12 words per cell instead of the 18
-// of separate H and E passes).  A workgroup is 64 lanes along z times FR
-// rows along y (one wavefront per row) and marches an x chunk.  Lane 0 and
-// row 0 are halos: there H is recomputed (from the unmodified input buffers)
-// so that the E update of the owned 63 x (FR-1) cells has all its H
-// neighbours; the plane before the chunk is an x halo in the same way.  The
-// psi of H is double-buffered like the fields (halo cells read it); psi of E
-// and the dispersive state are touched only by their owner.  The TF/SF
-// corrections of H are applied inside, where E needs them.
+// of separate H and E passes).  A workgroup is FR wavefronts, one per row
+// along y, of 64 lanes along z; it owns rows j0..j0+FR-2 of an x chunk and
+// walks the z tiles of those rows one after another (64 owned cells per
+// wave, aligned stores).  Row 0 is a halo: there H is recomputed (from the
+// unmodified input buffers) so that the E update of the owned rows has all
+// its H neighbours; the plane before the chunk is an x halo in the same way.
+// The H at k0-1 that lane 0 needs comes from lane 63 of the previous z tile
+// of the same row, kept in LDS per plane (the k = 0 boundary for the first
+// tile).  The psi of H is double-buffered like the fields (halo cells read
+// it); psi of E and the dispersive state are touched only by their owner.
+// The TF/SF corrections of H are applied inside, where E needs them.
 //
 // Data movement per plane i: E of plane i and its j+1 / k+1 neighbours come
 // from LDS (written one plane earlier), H rows of plane i go through LDS for
@@ -957,6 +960,10 @@ struct FusedLane {
     // over the step; applied in the planes ti0 <= i <= ti1-1
     bool tf_y, tf_z;
     double tfy, tfz;
+    // H at k0-1 handed from lane 63 of one z tile to lane 0 of the next:
+    // LDS at zbuf + ((parity * (xchunk+1) + plane) * FR + row) * 2
+    int zbuf, zin, zout, zstride;
+    bool zpass;              // a next z tile exists
 };
 
 // All loads are unconditional (no value depends on a branch, so the two
@@ -1033,6 +1040,7 @@ __device__ __forceinline__ void fused_plane(const FusedArgs<T>& f, const FusedCo
         }
     }
     if (store) { stb(f.hxo, ob, hx); stb(f.hyo, ob, hy); stb(f.hzo, ob, hz); }
+    const int zp = i - c.i0 + 1;   // plane index within the chunk (x halo = 0)
     {
         const int h0 = 6 * pl + P * 2 * pl + c.me;
         lds[h0] = hz;
@@ -1041,9 +1049,19 @@ __device__ __forceinline__ void fused_plane(const FusedArgs<T>& f, const FusedCo
         lds[e1] = cur.ez2;
         lds[e1 + pl] = cur.ex2;
         lds[e1 + 2 * pl] = cur.ey2;
+        if (c.last_lane && c.zpass) {
+            const int zo = c.zout + zp * c.zstride;
+            lds[zo] = hy;
+            lds[zo + 1] = hx;
+        }
     }
     __syncthreads();
-    const T hyk = __shfl_up(hy, 1), hxk = __shfl_up(hx, 1);
+    T hyk = __shfl_up(hy, 1), hxk = __shfl_up(hx, 1);
+    if (c.lane == 0) {
+        const int zi = c.zin + zp * c.zstride;
+        hyk = lds[zi];
+        hxk = lds[zi + 1];
+    }
     if (store && (!MEDIA || cur.tg < f.disp_tag0)) {
         const int h0 = 6 * pl + P * 2 * pl + c.me - kLanes;
         const T hzj = lds[h0], hxj = lds[h0 + pl];
@@ -1069,67 +1087,48 @@ __device__ __forceinline__ void fused_plane(const FusedArgs<T>& f, const FusedCo
     hzi = hz; hyi = hy;
 }
 
-template <typename T, bool CPML, bool MEDIA, bool TFSF>
-__global__ __launch_bounds__(512) void k_step_fused(const FusedArgs<T> f, const FusedCold<T>* __restrict__ cold) {
-    extern __shared__ unsigned char omx_lds[];
-    T* lds = reinterpret_cast<T*>(omx_lds);
-    FusedLane<T> c;
-    const int FR = blockDim.y;
-    // one wavefront per row: the row index is uniform, which turns the row
-    // conditions into scalar branches
-    c.lane = threadIdx.x; c.row = __builtin_amdgcn_readfirstlane(threadIdx.y);
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
-    const int tz = t % f.ntz;
-    const int q = t / f.ntz;
-    const int ty = q % f.nty;
-    const int tx = q / f.nty;
-    c.k = tz * (kLanes - 1) + c.lane;        // lane 0: halo at k0-1
-    c.j = ty * (FR - 1) + c.row;             // row 0: halo at j0-1
-    c.i0 = tx * f.xchunk + 1;
-    c.i1 = min(c.i0 + f.xchunk - 1, f.nx);
-    c.hcell = c.k >= 1 && c.k <= f.nz && c.j >= 1 && c.j <= f.ny;
-    c.own = c.hcell && c.lane >= 1 && c.row >= 1;
-    c.last_row = c.row == FR - 1;
-    c.last_lane = c.lane == kLanes - 1;
+// Lane constants of one z tile of the fused step.
+template <typename T, bool CPML, bool TFSF>
+__device__ __forceinline__ void fused_tile_lanes(const FusedArgs<T>& f, const FusedCold<T>* __restrict__ cold,
+                                                 FusedLane<T>& c, int tz) {
+    c.k = tz * kLanes + c.lane + 1;
+    c.hcell = c.k <= f.nz && c.j >= 1 && c.j <= f.ny;
+    c.own = c.hcell && c.row >= 1;
     // lanes/rows past the grid work on a clamped cell: their values are
     // never used by an owned cell, and every address stays in bounds
     const int jc = min(c.j, f.ny + 1), kc = min(c.k, f.nz + 1);
     c.jk = (long)jc * f.sz + kc + f.off;
-    c.sxb = (uint32_t)(f.sx * (long)sizeof(T));
-    c.side_rb = (c.last_row && jc + 1 <= f.ny + 1) ? (uint32_t)(f.sz * sizeof(T)) : 0u;
     c.side_lb = (c.last_lane && kc + 1 <= f.nz + 1) ? (uint32_t)sizeof(T) : 0u;
-    {
-        const YeeArgs<T>& a = cold->a;
-        c.dhy = a.den_hy[jc]; c.dhz = a.den_hz[kc]; c.dey = a.den_ey[jc]; c.dez = a.den_ez[kc];
-        c.chy = sizeof(T) == 4 ? a.cdhy[jc] : (T)0; c.chz = sizeof(T) == 4 ? a.cdhz[kc] : (T)0;
-        c.cey = sizeof(T) == 4 ? a.cdey[jc] : (T)0; c.cez = sizeof(T) == 4 ? a.cdez[kc] : (T)0;
-        c.zh = c.ze = false;
-        c.zbh = c.zch = c.zbe = c.zce = (T)0;
-        c.zhb = c.zeb = c.zhs = c.zes = 0u;
-        if constexpr (CPML) {
-            // z faces: H for k in [1, npml-1] and [nz+1-npml, nz-1] (coefficient
-            // index k resp. nz-k, psi slot k-1 resp. npml-2+nz-k); E for k in
-            // [2, npml] and [nz+1-npml, nz-1] (index k resp. nz+1-k, slot k-1
-            // resp. npml-1+nz+1-k)
-            const int npml = f.npml, nz = f.nz, k = c.k;
-            const int nsh = 2 * (npml - 1), nse = 2 * npml;
-            int ch = -1, sh = 0, ce = -1, se = 0;
-            if (c.hcell) {
-                if ((f.cpml & 16u) && k <= npml - 1) { ch = k; sh = k - 1; }
-                else if ((f.cpml & 32u) && k >= nz + 1 - npml && k <= nz - 1) { ch = nz - k; sh = npml - 2 + nz - k; }
-                if ((f.cpml & 16u) && k >= 2 && k <= npml) { ce = k; se = k - 1; }
-                else if ((f.cpml & 32u) && k >= nz + 1 - npml && k <= nz - 1) { ce = nz + 1 - k; se = npml - 1 + nz + 1 - k; }
-            }
-            c.zh = ch >= 0; c.ze = ce >= 0;
-            if (c.zh) { c.zbh = a.bh[ch]; c.zch = a.chd[ch]; }
-            if (c.ze) { c.zbe = a.be[ce]; c.zce = a.ced[ce]; }
-            const long row_h = ((long)(c.i0 - 1) * f.sy + jc) * nsh + sh;
-            const long row_e = ((long)(c.i0 - 1) * f.sy + jc) * nse + se;
-            c.zhb = (uint32_t)(row_h * (long)sizeof(T));
-            c.zeb = (uint32_t)(row_e * (long)sizeof(T));
-            c.zhs = (uint32_t)((long)f.sy * nsh * (long)sizeof(T));
-            c.zes = (uint32_t)((long)f.sy * nse * (long)sizeof(T));
+    const YeeArgs<T>& a = cold->a;
+    c.dhz = a.den_hz[kc]; c.dez = a.den_ez[kc];
+    c.chz = sizeof(T) == 4 ? a.cdhz[kc] : (T)0;
+    c.cez = sizeof(T) == 4 ? a.cdez[kc] : (T)0;
+    c.zh = c.ze = false;
+    c.zbh = c.zch = c.zbe = c.zce = (T)0;
+    c.zhb = c.zeb = c.zhs = c.zes = 0u;
+    if constexpr (CPML) {
+        // z faces: H for k in [1, npml-1] and [nz+1-npml, nz-1] (coefficient
+        // index k resp. nz-k, psi slot k-1 resp. npml-2+nz-k); E for k in
+        // [2, npml] and [nz+1-npml, nz-1] (index k resp. nz+1-k, slot k-1
+        // resp. npml-1+nz+1-k)
+        const int npml = f.npml, nz = f.nz, k = c.k;
+        const int nsh = 2 * (npml - 1), nse = 2 * npml;
+        int ch = -1, sh = 0, ce = -1, se = 0;
+        if (c.hcell) {
+            if ((f.cpml & 16u) && k <= npml - 1) { ch = k; sh = k - 1; }
+            else if ((f.cpml & 32u) && k >= nz + 1 - npml && k <= nz - 1) { ch = nz - k; sh = npml - 2 + nz - k; }
+            if ((f.cpml & 16u) && k >= 2 && k <= npml) { ce = k; se = k - 1; }
+            else if ((f.cpml & 32u) && k >= nz + 1 - npml && k <= nz - 1) { ce = nz + 1 - k; se = npml - 1 + nz + 1 - k; }
         }
+        c.zh = ch >= 0; c.ze = ce >= 0;
+        if (c.zh) { c.zbh = a.bh[ch]; c.zch = a.chd[ch]; }
+        if (c.ze) { c.zbe = a.be[ce]; c.zce = a.ced[ce]; }
+        const long row_h = ((long)(c.i0 - 1) * f.sy + jc) * nsh + sh;
+        const long row_e = ((long)(c.i0 - 1) * f.sy + jc) * nse + se;
+        c.zhb = (uint32_t)(row_h * (long)sizeof(T));
+        c.zeb = (uint32_t)(row_e * (long)sizeof(T));
+        c.zhs = (uint32_t)((long)f.sy * nsh * (long)sizeof(T));
+        c.zes = (uint32_t)((long)f.sy * nse * (long)sizeof(T));
     }
     c.tf_y = c.tf_z = false;
     c.tfy = c.tfz = 0.0;
@@ -1148,29 +1147,74 @@ __global__ __launch_bounds__(512) void k_step_fused(const FusedArgs<T> f, const 
             else if (j == f.tj1) { c.tf_z = true; c.tfz = q.dt_mu0 * (double)q.a.den_hy[j] * q.ex1[k - q.tkb]; }
         }
     }
+    // H at k0-1: read from parity tz&1, written for the next tile to 1-(tz&1)
+    c.zin = c.zbuf + (tz & 1) * (f.xchunk + 1) * c.zstride + c.row * 2;
+    c.zout = c.zbuf + ((tz + 1) & 1) * (f.xchunk + 1) * c.zstride + c.row * 2;
+    c.zpass = tz + 1 < f.ntz;
+}
+
+template <typename T, bool CPML, bool MEDIA, bool TFSF>
+__global__ __launch_bounds__(512) void k_step_fused(const FusedArgs<T> f, const FusedCold<T>* __restrict__ cold) {
+    extern __shared__ unsigned char omx_lds[];
+    T* lds = reinterpret_cast<T*>(omx_lds);
+    FusedLane<T> c;
+    const int FR = blockDim.y;
+    // one wavefront per row: the row index is uniform, which turns the row
+    // conditions into scalar branches
+    c.lane = threadIdx.x; c.row = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int ty = t % f.nty;
+    const int tx = t / f.nty;
+    c.j = ty * (FR - 1) + c.row;             // row 0: halo at j0-1
+    c.i0 = tx * f.xchunk + 1;
+    c.i1 = min(c.i0 + f.xchunk - 1, f.nx);
+    c.last_row = c.row == FR - 1;
+    c.last_lane = c.lane == kLanes - 1;
+    const int jc = min(c.j, f.ny + 1);
+    c.sxb = (uint32_t)(f.sx * (long)sizeof(T));
+    c.side_rb = (c.last_row && jc + 1 <= f.ny + 1) ? (uint32_t)(f.sz * sizeof(T)) : 0u;
+    {
+        const YeeArgs<T>& a = cold->a;
+        c.dhy = a.den_hy[jc]; c.dey = a.den_ey[jc];
+        c.chy = sizeof(T) == 4 ? a.cdhy[jc] : (T)0;
+        c.cey = sizeof(T) == 4 ? a.cdey[jc] : (T)0;
+    }
     c.plane = FR * kLanes;
     c.me = c.row * kLanes + c.lane;
+    c.zbuf = 10 * c.plane;
+    c.zstride = FR * 2;
     // LDS: E rows (ez, ex, ey) of a plane at 3*plane*parity, H rows (hz, hx) at
-    // 6*plane + 2*plane*parity.  Integer offsets keep every access a ds_ op.
-
-    // prologue: E of plane i0-1 into LDS buffer 0, plane i0-1 into A
-    long o = (long)(c.i0 - 1) * f.sx + c.jk;
-    uint32_t ob = (uint32_t)(o * (long)sizeof(T));
-    uint32_t zh = c.zhb, ze = c.zeb;
-    FusedPlane<T> A, B;
-    lds[c.me] = ldb(f.ezi, ob);
-    lds[c.me + c.plane] = ldb(f.exi, ob);
-    lds[c.me + 2 * c.plane] = ldb(f.eyi, ob);
-    fused_load<T, CPML, MEDIA>(f, c, A, ob, o, zh, ze);
-    __syncthreads();
-    T hzi = (T)0, hyi = (T)0;
-    for (int i = c.i0 - 1;; i += 2) {
-        fused_plane<T, CPML, MEDIA, TFSF, 0>(f, cold, c, lds, i, o, ob, zh, ze, A, B, hzi, hyi);
-        if (i + 1 > c.i1) break;
-        o += f.sx; ob += c.sxb; zh += c.zhs; ze += c.zes;
-        fused_plane<T, CPML, MEDIA, TFSF, 1>(f, cold, c, lds, i + 1, o, ob, zh, ze, B, A, hzi, hyi);
-        if (i + 2 > c.i1) break;
-        o += f.sx; ob += c.sxb; zh += c.zhs; ze += c.zes;
+    // 6*plane + 2*plane*parity, the H handed between z tiles at zbuf.  Integer
+    // offsets keep every access a ds_ op.
+    for (int tz = 0; tz < f.ntz; ++tz) {
+        fused_tile_lanes<T, CPML, TFSF>(f, cold, c, tz);
+        long o = (long)(c.i0 - 1) * f.sx + c.jk;
+        uint32_t ob = (uint32_t)(o * (long)sizeof(T));
+        uint32_t zh = c.zhb, ze = c.zeb;
+        // prologue: E of plane i0-1 into LDS buffer 0, plane i0-1 into A;
+        // for the first tile lane 0 stores the boundary H at k = 0
+        FusedPlane<T> A, B;
+        lds[c.me] = ldb(f.ezi, ob);
+        lds[c.me + c.plane] = ldb(f.exi, ob);
+        lds[c.me + 2 * c.plane] = ldb(f.eyi, ob);
+        if (tz == 0 && c.lane == 0) {
+            for (int p = 0; p <= c.i1 - c.i0 + 1; ++p) {
+                const uint32_t b0 = ob + (uint32_t)p * c.sxb - (uint32_t)sizeof(T);
+                lds[c.zin + p * c.zstride] = ldb(f.hyi, b0);
+                lds[c.zin + p * c.zstride + 1] = ldb(f.hxi, b0);
+            }
+        }
+        fused_load<T, CPML, MEDIA>(f, c, A, ob, o, zh, ze);
+        __syncthreads();
+        T hzi = (T)0, hyi = (T)0;
+        for (int i = c.i0 - 1;; i += 2) {
+            fused_plane<T, CPML, MEDIA, TFSF, 0>(f, cold, c, lds, i, o, ob, zh, ze, A, B, hzi, hyi);
+            if (i + 1 > c.i1) break;
+            o += f.sx; ob += c.sxb; zh += c.zhs; ze += c.zes;
+            fused_plane<T, CPML, MEDIA, TFSF, 1>(f, cold, c, lds, i + 1, o, ob, zh, ze, B, A, hzi, hyi);
+            if (i + 2 > c.i1) break;
+            o += f.sx; ob += c.sxb; zh += c.zhs; ze += c.zes;
+        }
     }
 }
 
