@@ -127,9 +127,14 @@ int omx_medium_library(const char* name, double eps_r, omx_medium* out);
 /* ---- lifecycle (TMxll%init / TMxll%kill) --------------------------------- */
 int omx3d_create(const omx3d_desc* desc, omx3d_ctx** out);
 int omx3d_destroy(omx3d_ctx* ctx);
-/* Tile shape of the H/E kernels: y rows per workgroup (1, 2 or 4) and x planes
- * marched per workgroup (>= 1).  0 keeps the current value. */
-int omx3d_set_tiling(omx3d_ctx* ctx, int32_t rows_y, int32_t xchunk);
+/* Tile shape of the H/E kernels: y rows per workgroup (1, 2 or 4), x planes
+ * marched per workgroup (>= 1; 0 keeps the current values) and the order in
+ * which workgroups take tiles: 0 block order (consecutive workgroups,
+ * spread over the XCDs, cover consecutive memory), 1 one contiguous slab of
+ * tiles per XCD (-1 keeps it).  A batch uses the tile order of its first
+ * grid.  No reference counterpart (engine tuning). */
+int omx3d_set_tiling(omx3d_ctx* ctx, int32_t rows_y, int32_t xchunk, int32_t tile_order);
+int omx3d_get_tiling(omx3d_ctx* ctx, int32_t* rows_y, int32_t* xchunk, int32_t* tile_order);
 /* CPML profiles and den arrays (mxll_3D_mod.f90:282-420), device FP64.
  * be,ce: npml; bh,ch: npml-1; den: den_ex(nx) den_ey(ny) den_ez(nz) den_hx(nx) den_hy(ny) den_hz(nz). */
 int omx3d_get_cpml(omx3d_ctx* ctx, double* be, double* ce, double* bh, double* ch, double* den);
@@ -210,10 +215,12 @@ int omx3d_get_profile(omx3d_ctx* ctx, double* ms_h, double* ms_e, int32_t* n_lau
 int omx3d_synchronize(omx3d_ctx* ctx);
 /* Bytes of device memory the context holds. */
 int omx3d_device_bytes(omx3d_ctx* ctx, int64_t* bytes);
-/* Tile-shape autotuner: times rows_y x xchunk candidates with `steps` H+E
- * steps each, keeps the fastest, then clears fields, CPML and dispersive
- * state (call before a run).  No reference counterpart (engine tuning). */
-int omx3d_autotune(omx3d_ctx* ctx, int32_t steps, int32_t* rows_y, int32_t* xchunk, double* ms_per_step);
+/* Tile-shape autotuner: times rows_y x xchunk x tile order candidates with
+ * `steps` H+E steps each, keeps the fastest, then clears fields, CPML and
+ * dispersive state (call before a run).  No reference counterpart (engine
+ * tuning). */
+int omx3d_autotune(omx3d_ctx* ctx, int32_t steps, int32_t* rows_y, int32_t* xchunk, int32_t* tile_order,
+                   double* ms_per_step);
 
 /* ---- batches of independent grids (no reference counterpart) ------------ */
 /* One launch per phase (H, sources, E, detectors) covers every grid of the

@@ -90,17 +90,11 @@ YeeArgs<T> make_args(omx3d_ctx* c) {
     a.npml = c->d.npml;
     a.cpml = c->cpml_mask;
     a.disp_tag0 = c->disp_tag0;
-    a.cpml_sep = 1;
-    {
-        const int n3[3] = {c->d.nx, c->d.ny, c->d.nz};
-        for (int q = 0; q < 3; ++q)
-            if ((c->cpml_mask >> (2 * q)) & 3u)
-                if (n3[q] < 2 * c->d.npml) a.cpml_sep = 0;
-    }
     a.xchunk = c->xchunk;
     a.ntz = (c->L.nz + kLanes - 1) / kLanes;
     a.nty = (c->L.ny + c->rows_y - 1) / c->rows_y;
     a.rows = c->rows_y;
+    a.tile_order = c->tile_order;
     return a;
 }
 
@@ -138,6 +132,12 @@ int wrap_h(omx3d_ctx* c, hipStream_t st) {
     return OMX_OK;
 }
 
+// Whether every array of the grid (fields, psi, tags, dispersive state) is
+// addressed by 32-bit byte offsets: the field arrays are the largest.
+bool offsets32(const omx3d_ctx* c) {
+    return (unsigned long long)c->L.nalloc * c->esz < (1ull << 32);
+}
+
 template <typename T>
 // ev0/ev1 (profiling, may be null) bracket the H kernel itself.
 int launch_h(omx3d_ctx* c, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
@@ -146,9 +146,12 @@ int launch_h(omx3d_ctx* c, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
     YeeArgs<T> a = make_args<T>(c);
     const dim3 blk(kLanes, c->rows_y);
     const unsigned nb = yee_blocks(c, a);
+    const bool cp = c->cpml_mask != 0, b32 = offsets32(c);
     if (ev0) HIPCHK(hipEventRecord(ev0, c->s));
-    if (c->cpml_mask) hipLaunchKernelGGL((k_h_update<T, true>), dim3(nb), blk, 0, c->s, a);
-    else hipLaunchKernelGGL((k_h_update<T, false>), dim3(nb), blk, 0, c->s, a);
+    if (cp && b32) hipLaunchKernelGGL((k_h_update<T, true, true>), dim3(nb), blk, 0, c->s, a);
+    else if (cp) hipLaunchKernelGGL((k_h_update<T, true, false>), dim3(nb), blk, 0, c->s, a);
+    else if (b32) hipLaunchKernelGGL((k_h_update<T, false, true>), dim3(nb), blk, 0, c->s, a);
+    else hipLaunchKernelGGL((k_h_update<T, false, false>), dim3(nb), blk, 0, c->s, a);
     if (ev1) HIPCHK(hipEventRecord(ev1, c->s));
     HIPCHK(hipGetLastError());
     return OMX_OK;
@@ -194,10 +197,17 @@ int launch_e(omx3d_ctx* c, double time, double t_skip, hipEvent_t ev0 = nullptr,
     const unsigned nb = yee_blocks(c, a);
     const bool cp = c->cpml_mask != 0, md = c->media_set && c->n_classes > 1;
     if (ev0) HIPCHK(hipEventRecord(ev0, c->s));
-    if (cp && md) hipLaunchKernelGGL((k_e_update<T, true, 1>), dim3(nb), blk, 0, c->s, a);
-    else if (cp) hipLaunchKernelGGL((k_e_update<T, true, 0>), dim3(nb), blk, 0, c->s, a);
-    else if (md) hipLaunchKernelGGL((k_e_update<T, false, 1>), dim3(nb), blk, 0, c->s, a);
-    else hipLaunchKernelGGL((k_e_update<T, false, 0>), dim3(nb), blk, 0, c->s, a);
+    if (offsets32(c)) {
+        if (cp && md) hipLaunchKernelGGL((k_e_update<T, true, 1, true>), dim3(nb), blk, 0, c->s, a);
+        else if (cp) hipLaunchKernelGGL((k_e_update<T, true, 0, true>), dim3(nb), blk, 0, c->s, a);
+        else if (md) hipLaunchKernelGGL((k_e_update<T, false, 1, true>), dim3(nb), blk, 0, c->s, a);
+        else hipLaunchKernelGGL((k_e_update<T, false, 0, true>), dim3(nb), blk, 0, c->s, a);
+    } else {
+        if (cp && md) hipLaunchKernelGGL((k_e_update<T, true, 1, false>), dim3(nb), blk, 0, c->s, a);
+        else if (cp) hipLaunchKernelGGL((k_e_update<T, true, 0, false>), dim3(nb), blk, 0, c->s, a);
+        else if (md) hipLaunchKernelGGL((k_e_update<T, false, 1, false>), dim3(nb), blk, 0, c->s, a);
+        else hipLaunchKernelGGL((k_e_update<T, false, 0, false>), dim3(nb), blk, 0, c->s, a);
+    }
     if (ev1) HIPCHK(hipEventRecord(ev1, c->s));
     HIPCHK(hipGetLastError());
     if (md && c->n_disp > 0) {
@@ -804,6 +814,8 @@ struct omxb_ctx {
     long* d_step = nullptr;
     bool periodic = false;
     int rows = 1;                   // workgroup height: the largest rows_y of the grids
+    int tile_order = 0;             // tile order of the batched H/E kernels: the first grid's
+    bool b32 = true;                // every grid addressed by 32-bit byte offsets
     long* d_disp = nullptr;         // cells of dispersive classes of every grid (packed as per grid)
     int* d_disp_grid = nullptr;     // their grid
     int n_disp = 0;
@@ -836,6 +848,8 @@ int batch_setup(omxb_ctx* b) {
     b->total_tiles = tot;
     b->rows = 1;
     for (auto* c : b->g) b->rows = std::max(b->rows, c->rows_y);
+    b->tile_order = b->g[0]->tile_order;
+    for (auto* c : b->g) b->b32 = b->b32 && offsets32(c);
     int rc = bupload(b->s, (YeeArgs<T>**)&b->d_args, args);
     if (!rc) rc = bupload(b->s, &b->d_tile_start, ts);
     if (rc) return rc;
@@ -925,8 +939,12 @@ int batch_step_launches(omxb_ctx* b) {
     const dim3 blk(kLanes, b->rows);
     if (b->periodic)
         for (auto* c : b->g) { int rc = wrap_e<T>(c, b->s); if (rc) return rc; }
-    hipLaunchKernelGGL(k_h_update_batched<T>, dim3(b->total_tiles), blk, 0, b->s, (const YeeArgs<T>*)b->d_args,
-                       b->d_tile_start, b->n);
+    if (b->b32)
+        hipLaunchKernelGGL((k_h_update_batched<T, true>), dim3(b->total_tiles), blk, 0, b->s,
+                           (const YeeArgs<T>*)b->d_args, b->d_tile_start, b->n, b->tile_order);
+    else
+        hipLaunchKernelGGL((k_h_update_batched<T, false>), dim3(b->total_tiles), blk, 0, b->s,
+                           (const YeeArgs<T>*)b->d_args, b->d_tile_start, b->n, b->tile_order);
     for (size_t q = 0; q < b->pass_first.size(); ++q) {
         const int f0 = b->pass_first[q], nq = b->pass_count[q];
         hipLaunchKernelGGL(k_source_batched<T>, dim3(blocks_for(nq, 256)), dim3(256), 0, b->s,
@@ -935,8 +953,12 @@ int batch_step_launches(omxb_ctx* b) {
     }
     if (b->periodic)
         for (auto* c : b->g) { int rc = wrap_h<T>(c, b->s); if (rc) return rc; }
-    hipLaunchKernelGGL(k_e_update_batched<T>, dim3(b->total_tiles), blk, 0, b->s, (const YeeArgs<T>*)b->d_args,
-                       b->d_tile_start, b->n);
+    if (b->b32)
+        hipLaunchKernelGGL((k_e_update_batched<T, true>), dim3(b->total_tiles), blk, 0, b->s,
+                           (const YeeArgs<T>*)b->d_args, b->d_tile_start, b->n, b->tile_order);
+    else
+        hipLaunchKernelGGL((k_e_update_batched<T, false>), dim3(b->total_tiles), blk, 0, b->s,
+                           (const YeeArgs<T>*)b->d_args, b->d_tile_start, b->n, b->tile_order);
     if (b->n_disp > 0)
         hipLaunchKernelGGL(k_e_dispersive_batched<T>, dim3(blocks_for(b->n_disp, 256)), dim3(256), 0, b->s,
                            (const YeeArgs<T>*)b->d_args, b->d_disp, b->d_disp_grid, b->n_disp);
@@ -1190,8 +1212,9 @@ int omx3d_destroy(omx3d_ctx* c) {
     return OMX_OK;
 }
 
-int omx3d_set_tiling(omx3d_ctx* c, int32_t rows_y, int32_t xchunk) {
+int omx3d_set_tiling(omx3d_ctx* c, int32_t rows_y, int32_t xchunk, int32_t tile_order) {
     CHECK_CTX(c);
+    if (tile_order < -1 || tile_order > 1) return fail(OMX_ERR_ARG, "tile_order must be -1, 0 or 1");
     if (rows_y != 0) {
         if (rows_y != 1 && rows_y != 2 && rows_y != 4) return fail(OMX_ERR_ARG, "rows_y must be 1, 2 or 4");
         c->rows_y = rows_y;
@@ -1200,6 +1223,15 @@ int omx3d_set_tiling(omx3d_ctx* c, int32_t rows_y, int32_t xchunk) {
         if (xchunk < 1) return fail(OMX_ERR_ARG, "xchunk must be >= 1");
         c->xchunk = xchunk;
     }
+    if (tile_order >= 0) c->tile_order = tile_order;
+    return OMX_OK;
+}
+
+int omx3d_get_tiling(omx3d_ctx* c, int32_t* rows_y, int32_t* xchunk, int32_t* tile_order) {
+    CHECK_CTX(c);
+    if (rows_y) *rows_y = c->rows_y;
+    if (xchunk) *xchunk = c->xchunk;
+    if (tile_order) *tile_order = c->tile_order;
     return OMX_OK;
 }
 
@@ -1727,22 +1759,24 @@ int omxb_destroy(omxb_ctx* b) {
     return OMX_OK;
 }
 
-// Times the tile shapes (rows_y in {1,2,4} x xchunk in {4,8,16,32,64}) with
-// `steps` H+E steps each and keeps the fastest.  The fields, CPML and
-// dispersive state are cleared afterwards, so call it before a run.
-int omx3d_autotune(omx3d_ctx* c, int32_t steps, int32_t* rows_y, int32_t* xchunk, double* ms_per_step) {
+// Times the tile shapes (rows_y in {1,2,4} x xchunk in {1,2,4,8,16} x tile
+// order 0/1) with `steps` H+E steps each and keeps the fastest.  The fields,
+// CPML and dispersive state are cleared afterwards, so call it before a run.
+int omx3d_autotune(omx3d_ctx* c, int32_t steps, int32_t* rows_y, int32_t* xchunk, int32_t* tile_order,
+                   double* ms_per_step) {
     CHECK_CTX(c);
     if (steps < 1) return fail(OMX_ERR_ARG, "steps must be >= 1");
     const int rows_c[3] = {1, 2, 4};
-    const int xc_c[5] = {4, 8, 16, 32, 64};
+    const int xc_c[5] = {1, 2, 4, 8, 16};
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     double best = 1e30;
-    int br = c->rows_y, bx = c->xchunk;
+    int br = c->rows_y, bx = c->xchunk, bo = c->tile_order;
+    for (int order = 0; order < 2; ++order)
     for (int r : rows_c)
         for (int x : xc_c) {
-            c->rows_y = r; c->xchunk = x;
+            c->rows_y = r; c->xchunk = x; c->tile_order = order;
             for (int w = 0; w < 2; ++w) {   // warm-up launch pair
                 int rc = c->prec == 4 ? launch_h<float>(c) : launch_h<double>(c);
                 if (!rc) rc = c->prec == 4 ? launch_e<float>(c, 0.0, 0.0) : launch_e<double>(c, 0.0, 0.0);
@@ -1758,11 +1792,11 @@ int omx3d_autotune(omx3d_ctx* c, int32_t steps, int32_t* rows_y, int32_t* xchunk
             HIPCHK(hipEventSynchronize(e1));
             float ms = 0.f;
             HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-            if (ms / steps < best) { best = ms / steps; br = r; bx = x; }
+            if (ms / steps < best) { best = ms / steps; br = r; bx = x; bo = order; }
         }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    c->rows_y = br; c->xchunk = bx;
+    c->rows_y = br; c->xchunk = bx; c->tile_order = bo;
     // back to the initial state: zero fields, psi and dispersive state
     const size_t fb = c->esz * c->L.nalloc;
     for (auto p : c->f) HIPCHK(hipMemsetAsync(p, 0, fb, c->s));
@@ -1787,6 +1821,7 @@ int omx3d_autotune(omx3d_ctx* c, int32_t steps, int32_t* rows_y, int32_t* xchunk
     HIPCHK(hipStreamSynchronize(c->s));
     if (rows_y) *rows_y = br;
     if (xchunk) *xchunk = bx;
+    if (tile_order) *tile_order = bo;
     if (ms_per_step) *ms_per_step = best;
     return OMX_OK;
 }

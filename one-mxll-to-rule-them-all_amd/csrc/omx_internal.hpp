@@ -74,9 +74,9 @@ struct YeeArgs {
     int off;
     int npml;
     unsigned cpml;             // bit f set: CPML on face f (x-lo, x-hi, y-lo, y-hi, z-lo, z-hi)
-    int cpml_sep;              // 1: on every CPML axis the two faces do not meet (n >= 2 npml)
     int disp_tag0;             // tags >= disp_tag0: classes with a Drude / Drude-Lorentz component
     int ntz, nty, xchunk, rows;
+    int tile_order;            // 0: tiles in block order, 1: one contiguous slab of tiles per XCD
 };
 
 struct SourceDev {
@@ -102,6 +102,7 @@ struct omx3d_ctx {
     omx::Layout L;
     hipStream_t s = nullptr;
     int rows_y = 1, xchunk = 4;
+    int tile_order = 0;         // tile order of the H/E kernels (YeeArgs::tile_order)
 
     void* f[6] = {};    // Ex Ey Ez Hx Hy Hz (current)
     void* f_alt[6] = {};        // second buffers of the fused step

@@ -22,6 +22,54 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
     return x * per + (x < rem ? x : rem) + q;
 }
 
+// Tile of this workgroup.  Order 0 keeps the block order: consecutive
+// workgroups, which the dispatcher deals round-robin to the XCDs, cover
+// consecutive memory, so the whole chip streams one region of every array at
+// a time (the neighbour rows another XCD read are found in the memory-side
+// cache).  Order 1 gives every XCD one contiguous slab of tiles, which keeps
+// the neighbour rows in its own L2 but streams eight regions at once.
+__device__ __forceinline__ int tile_of_block(int order) {
+    return order ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+}
+
+// Field access by 32-bit byte offset from a uniform base pointer (callers
+// take these paths only when every array of the grid is below 4 GiB).
+template <typename T>
+__device__ __forceinline__ T ldb(const T* p, uint32_t b) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + b);
+}
+template <typename T>
+__device__ __forceinline__ void stb(T* p, uint32_t b, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + b) = v;
+}
+// Uniform read of a table the kernel never writes (scalar load).
+template <typename T>
+__device__ __forceinline__ T ldc(const T* p, int i) {
+    return ((const __attribute__((address_space(4))) T*)p)[i];
+}
+
+// Offsets of a cell into the field, psi and tag arrays of the H/E tiles.
+// B32: 32-bit byte offsets (every array of the grid below 4 GiB), so that a
+// load takes a scalar base and one 32-bit vector offset with no 64-bit
+// address arithmetic, which leaves vector registers for loads in flight;
+// otherwise 64-bit element indices.
+template <typename T, bool B32>
+struct Off {
+    using I = long;
+    static __device__ __forceinline__ I of(long e) { return e; }
+    static __device__ __forceinline__ T ld(const T* p, I o) { return p[o]; }
+    static __device__ __forceinline__ void st(T* p, I o, T v) { p[o] = v; }
+    static __device__ __forceinline__ int tag(const uint8_t* p, I o) { return p[o]; }
+};
+template <typename T>
+struct Off<T, true> {
+    using I = uint32_t;
+    static __device__ __forceinline__ I of(long e) { return (uint32_t)(e * (long)sizeof(T)); }
+    static __device__ __forceinline__ T ld(const T* p, I o) { return ldb(p, o); }
+    static __device__ __forceinline__ void st(T* p, I o, T v) { stb(p, o, v); }
+    static __device__ __forceinline__ int tag(const uint8_t* p, I o) { return p[o / (uint32_t)sizeof(T)]; }
+};
+
 // ---------------------------------------------------------------------------
 // K8: FP64 coefficient precompute on the device.
 // CPML grading follows init_3Dgrid (src/FDTD/mxll_3D_mod.f90:294-318): the
@@ -392,19 +440,6 @@ __device__ __forceinline__ void h_cpml_z(const YeeArgs<T>& a, T* const* pho, boo
     }
 }
 
-template <typename T, bool CPML>
-__device__ __forceinline__ void h_cell(const YeeArgs<T>& a, T* const* pho, bool st, int i, int j, int k, long jk,
-                                       T ezc, T ezn, T ezj, T eyc, T eyn, T eyk, T exc, T exk, T exj,
-                                       T cx, T dhy, T dhz, T chy, T chz, T& hx, T& hy, T& hz) {
-    const T dmu = a.dt_mu0;
-    h_bulk(dmu, cx, dhy, dhz, chy, chz, ezc, ezn, ezj, eyc, eyn, eyk, exc, exk, exj, hx, hy, hz);
-    if constexpr (CPML) {
-        h_cpml_x(a, pho, st, i, jk, dmu, ezc, ezn, eyc, eyn, hy, hz);
-        h_cpml_y(a, pho, st, i, j, k, dmu, ezc, ezj, exc, exj, hx, hz);
-        h_cpml_z(a, pho, st, i, j, k, dmu, eyc, eyk, exc, exk, hx, hy);
-    }
-}
-
 // CPML face of one axis for a cell: psi slot and coefficient index (slot
 // < 0: the cell is in no face of that axis).  H faces: low 1..npml-1 (slot
 // n-1, coefficient n), high n_a+1-npml..n_a-1 (slot npml-2+c, c = n_a-n).  E
@@ -424,39 +459,61 @@ __device__ __forceinline__ Face face_e(bool lo_on, bool hi_on, int n, int na, in
     return {-1, 0};
 }
 
-// A tile: 64 lanes along z, rows along y, marching xchunk planes along x.
-// With CPML, the face slots of the row (y) and lane (z) are fixed over the
-// tile and those of the plane (x) are uniform; the psi of all faces a cell
-// lies in are loaded together with its fields, ahead of the arithmetic.
-// Grids whose two faces of an axis meet (n < 2 npml) take the per-cell path.
-template <typename T, bool CPML>
+// Whether index range [a0, a1] meets [f0, f1].
+__device__ __forceinline__ bool meets(int a0, int a1, int f0, int f1) { return a0 <= f1 && a1 >= f0; }
+// Whether [a0, a1] holds a cell of an H face (1..npml-1, n+1-npml..n-1) or
+// an E face (2..npml, n+1-npml..n-1) of an axis.
+__device__ __forceinline__ bool in_faces_h(bool lo_on, bool hi_on, int a0, int a1, int n, int npml) {
+    return (lo_on && meets(a0, a1, 1, npml - 1)) || (hi_on && meets(a0, a1, n + 1 - npml, n - 1));
+}
+__device__ __forceinline__ bool in_faces_e(bool lo_on, bool hi_on, int a0, int a1, int n, int npml) {
+    return (lo_on && meets(a0, a1, 2, npml)) || (hi_on && meets(a0, a1, n + 1 - npml, n - 1));
+}
+
+// A tile: 64 lanes along z (one wavefront), one row along y per wavefront,
+// marching xchunk planes along x.  A wavefront whose cells (its row, its z
+// tile, the planes of its chunk) meet no CPML face runs the bulk loop.  In
+// the face loop the face slots of the row (y) and lane (z) are fixed over
+// the tile and those of the plane (x) are uniform; the psi of all faces a
+// cell lies in are loaded together with its fields, ahead of the
+// arithmetic.  A cell lies in at most one face per axis: grids are at least
+// 2 npml cells along every CPML axis (omx3d_create rejects thinner ones).
+template <typename T, bool CPML, bool B32>
 __device__ __forceinline__ void h_tile(const YeeArgs<T>& a, const int t) {
+    using O = Off<T, B32>;
+    using I = typename O::I;
     const int tz = t % a.ntz;
     const int r = t / a.ntz;
     const int ty = r % a.nty;
     const int tx = r / a.nty;
     const int k = tz * kLanes + threadIdx.x + 1;
-    const int j = ty * a.rows + threadIdx.y + 1;
+    const int j = ty * a.rows + __builtin_amdgcn_readfirstlane(threadIdx.y) + 1;
     if (k > a.nz || j > a.ny) return;
     const int i0 = tx * a.xchunk + 1;
     const int i1 = min(i0 + a.xchunk - 1, a.nx);
-    const long sx = a.sx;
-    const int sz = a.sz;
-    const long jk = (long)j * sz + k + a.off;
-    long o = (long)i0 * sx + jk;
+    const I sx = O::of(a.sx), sz = O::of(a.sz), s1 = O::of(1);
+    const long jke = (long)j * a.sz + k + a.off;
+    I o = O::of((long)i0 * a.sx + jke);
     const T dhy = a.den_hy[j], dhz = a.den_hz[k];
     const T chy = sizeof(T) == 4 ? a.cdhy[j] : (T)0, chz = sizeof(T) == 4 ? a.cdhz[k] : (T)0;
-    T ezc = a.ez[o], eyc = a.ey[o];
-    if (!CPML || !a.cpml_sep) {
+    const T dmu = a.dt_mu0;
+    T ezc = O::ld(a.ez, o), eyc = O::ld(a.ey, o);
+    bool faces = false;
+    if (CPML) {
+        const unsigned cp = a.cpml;
+        const int k0 = tz * kLanes + 1, k1 = min(k0 + kLanes - 1, a.nz);
+        faces = in_faces_h(cp & 1u, cp & 2u, i0, i1, a.nx, a.npml) || in_faces_h(cp & 4u, cp & 8u, j, j, a.ny, a.npml) ||
+                in_faces_h(cp & 16u, cp & 32u, k0, k1, a.nz, a.npml);
+    }
+    if (!faces) {
         for (int i = i0; i <= i1; ++i, o += sx) {
-            const T ezn = a.ez[o + sx], eyn = a.ey[o + sx];
-            const T ezj = a.ez[o + sz], eyk = a.ey[o + 1];
-            const T exc = a.ex[o], exk = a.ex[o + 1], exj = a.ex[o + sz];
-            T hx = a.hx[o], hy = a.hy[o], hz = a.hz[o];
+            const T ezn = O::ld(a.ez, o + sx), eyn = O::ld(a.ey, o + sx);
+            const T ezj = O::ld(a.ez, o + sz), eyk = O::ld(a.ey, o + s1);
+            const T exc = O::ld(a.ex, o), exk = O::ld(a.ex, o + s1), exj = O::ld(a.ex, o + sz);
+            T hx = O::ld(a.hx, o), hy = O::ld(a.hy, o), hz = O::ld(a.hz, o);
             const T cx = sizeof(T) == 4 ? a.cdhx[i] : a.den_hx[i];
-            h_cell<T, CPML>(a, a.ph, true, i, j, k, jk, ezc, ezn, ezj, eyc, eyn, eyk, exc, exk, exj, cx, dhy, dhz,
-                            chy, chz, hx, hy, hz);
-            a.hx[o] = hx; a.hy[o] = hy; a.hz[o] = hz;
+            h_bulk(dmu, cx, dhy, dhz, chy, chz, ezc, ezn, ezj, eyc, eyn, eyk, exc, exk, exj, hx, hy, hz);
+            O::st(a.hx, o, hx); O::st(a.hy, o, hy); O::st(a.hz, o, hz);
             ezc = ezn; eyc = eyn;
         }
         return;
@@ -468,38 +525,41 @@ __device__ __forceinline__ void h_tile(const YeeArgs<T>& a, const int t) {
     T yb = (T)0, yc = (T)0, zb = (T)0, zc = (T)0;
     if (fy.slot >= 0) { yb = a.bh[fy.c]; yc = a.chd[fy.c]; }
     if (fz.slot >= 0) { zb = a.bh[fz.c]; zc = a.chd[fz.c]; }
-    const T dmu = a.dt_mu0;
-    for (int i = i0; i <= i1; ++i, o += sx) {
-        const T ezn = a.ez[o + sx], eyn = a.ey[o + sx];
-        const T ezj = a.ez[o + sz], eyk = a.ey[o + 1];
-        const T exc = a.ex[o], exk = a.ex[o + 1], exj = a.ex[o + sz];
-        T hx = a.hx[o], hy = a.hy[o], hz = a.hz[o];
+    // psi of plane i: x (fx.slot*sx + jk), y (i*nsh*sz + slot*sz + k + off),
+    // z (i*sy*nsh + j*nsh + slot); offsets of faces a cell is not in are unused
+    const I jk = O::of(jke);
+    const I ystep = O::of((long)nsh * a.sz), zstep = O::of((long)a.sy * nsh);
+    I py = O::of((long)i0 * nsh * a.sz + (long)fy.slot * a.sz + k + a.off);
+    I pz = O::of(((long)i0 * a.sy + j) * nsh + fz.slot);
+    for (int i = i0; i <= i1; ++i, o += sx, py += ystep, pz += zstep) {
+        const T ezn = O::ld(a.ez, o + sx), eyn = O::ld(a.ey, o + sx);
+        const T ezj = O::ld(a.ez, o + sz), eyk = O::ld(a.ey, o + s1);
+        const T exc = O::ld(a.ex, o), exk = O::ld(a.ex, o + s1), exj = O::ld(a.ex, o + sz);
+        T hx = O::ld(a.hx, o), hy = O::ld(a.hy, o), hz = O::ld(a.hz, o);
         const T cx = sizeof(T) == 4 ? a.cdhx[i] : a.den_hx[i];
         const Face fx = face_h(cp & 1u, cp & 2u, i, a.nx, npml);
-        const long psx = (long)fx.slot * sx + jk;
-        const long psy = ((long)i * nsh + fy.slot) * sz + k + a.off;
-        const long psz = ((long)i * a.sy + j) * nsh + fz.slot;
+        const I px = O::of((long)fx.slot * a.sx) + jk;
         T px0 = (T)0, px1 = (T)0, xb = (T)0, xc = (T)0, py0 = (T)0, py1 = (T)0, pz0 = (T)0, pz1 = (T)0;
-        if (fx.slot >= 0) { px0 = a.ph[0][psx]; px1 = a.ph[1][psx]; xb = a.bh[fx.c]; xc = a.chd[fx.c]; }
-        if (fy.slot >= 0) { py0 = a.ph[2][psy]; py1 = a.ph[3][psy]; }
-        if (fz.slot >= 0) { pz0 = a.ph[4][psz]; pz1 = a.ph[5][psz]; }
+        if (fx.slot >= 0) { px0 = O::ld(a.ph[0], px); px1 = O::ld(a.ph[1], px); xb = a.bh[fx.c]; xc = a.chd[fx.c]; }
+        if (fy.slot >= 0) { py0 = O::ld(a.ph[2], py); py1 = O::ld(a.ph[3], py); }
+        if (fz.slot >= 0) { pz0 = O::ld(a.ph[4], pz); pz1 = O::ld(a.ph[5], pz); }
         h_bulk(dmu, cx, dhy, dhz, chy, chz, ezc, ezn, ezj, eyc, eyn, eyk, exc, exk, exj, hx, hy, hz);
         if (fx.slot >= 0) {
             const T p0 = psi_step(px0, xb, xc, ezn - ezc), p1 = psi_step(px1, xb, xc, eyc - eyn);
-            a.ph[0][psx] = p0; a.ph[1][psx] = p1;
+            O::st(a.ph[0], px, p0); O::st(a.ph[1], px, p1);
             hy = hy + dmu * p0; hz = hz + dmu * p1;
         }
         if (fy.slot >= 0) {
             const T p0 = psi_step(py0, yb, yc, ezc - ezj), p1 = psi_step(py1, yb, yc, exj - exc);
-            a.ph[2][psy] = p0; a.ph[3][psy] = p1;
+            O::st(a.ph[2], py, p0); O::st(a.ph[3], py, p1);
             hx = hx + dmu * p0; hz = hz + dmu * p1;
         }
         if (fz.slot >= 0) {
             const T p0 = psi_step(pz0, zb, zc, eyk - eyc), p1 = psi_step(pz1, zb, zc, exc - exk);
-            a.ph[4][psz] = p0; a.ph[5][psz] = p1;
+            O::st(a.ph[4], pz, p0); O::st(a.ph[5], pz, p1);
             hx = hx + dmu * p0; hy = hy + dmu * p1;
         }
-        a.hx[o] = hx; a.hy[o] = hy; a.hz[o] = hz;
+        O::st(a.hx, o, hx); O::st(a.hy, o, hy); O::st(a.hz, o, hz);
         ezc = ezn; eyc = eyn;
     }
 }
@@ -557,13 +617,6 @@ __device__ __forceinline__ T medium_update(T e, T rot, T vac, int m, const MedDe
     return e + M.cdt * rot;  // dielectric: dt_eps0 * (1/eps_r) * rotH
 }
 
-// E of one cell, in place in ex/ey/ez: bulk curl of H with the medium
-// update, then the CPML psi recursions (psi of E in place in a.pe).  tg is
-// the cell's class tag (0: vacuum in all three components); dex = den_ex[i],
-// cex_ = cdex[i] (FP32 only).
-// MED: 0 no media; 1 vacuum and dielectric components only (the caller
-// leaves cells of the Drude/Drude-Lorentz classes, tag >= disp_tag0, to the
-// dispersive-cell pass); 2 every medium kind.
 // Curl of H at an E cell (rot, for the medium updates) and the vacuum
 // increments dt_eps0 * rot (FP32: folded coefficients dt_eps0*den).
 template <typename T>
@@ -581,32 +634,25 @@ __device__ __forceinline__ void e_curl(T dep, T hzc, T hzj, T hzi, T hyc, T hyk,
     }
 }
 
+// Bulk E update of one cell in ex/ey/ez (no CPML).  tg is the cell's class
+// tag (0: vacuum in all three components); dex = den_ex[i], cex_ = cdex[i]
+// (FP32 only).  MED: 0 no media; 1 vacuum and dielectric components (the
+// caller leaves cells of the Drude/Drude-Lorentz classes, tag >= disp_tag0,
+// to the dispersive-cell pass).
 template <typename T, int MED>
-__device__ __forceinline__ void e_bulk(const YeeArgs<T>& a, int tg, long o, T dep, T hzc, T hzj, T hzi, T hyc,
-                                       T hyk, T hyi, T hxc, T hxj, T hxk, T dex, T cex_, T dey, T dez, T cey, T cez,
-                                       T& ex, T& ey, T& ez) {
+__device__ __forceinline__ void e_bulk(const YeeArgs<T>& a, int tg, T dep, T hzc, T hzj, T hzi, T hyc, T hyk, T hyi,
+                                       T hxc, T hxj, T hxk, T dex, T cex_, T dey, T dez, T cey, T cez, T& ex, T& ey,
+                                       T& ez) {
+    static_assert(MED == 0 || MED == 1, "dispersive media are updated per cell by e_dispersive_cell");
     T rx, ry, rz, vx, vy, vz;
     e_curl(dep, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dex, cex_, dey, dez, cey, cez, rx, ry, rz, vx, vy, vz);
-    if (MED == 1) {
-        if (tg == 0) {
-            ex = ex + vx; ey = ey + vy; ez = ez + vz;
-        } else {
-            // vacuum: + dt_eps0 rotH; dielectric: + dt_eps0/eps_r rotH
-            const int8_t* c3 = a.cls + 3 * tg;
-            const int m0 = c3[0], m1 = c3[1], m2 = c3[2];
-            ex = m0 == 0 ? ex + vx : ex + a.med[m0].cdt * rx;
-            ey = m1 == 0 ? ey + vy : ey + a.med[m1].cdt * ry;
-            ez = m2 == 0 ? ez + vz : ez + a.med[m2].cdt * rz;
-        }
-    } else if (MED == 2) {
-        if (tg == 0) {
-            ex = ex + vx; ey = ey + vy; ez = ez + vz;
-        } else {
-            const int8_t* c3 = a.cls + 3 * tg;
-            ex = medium_update(ex, rx, vx, (int)c3[0], a.med, o, a.eold[0], a.pd[0], a.pl[0], a.plold[0], a.pstride);
-            ey = medium_update(ey, ry, vy, (int)c3[1], a.med, o, a.eold[1], a.pd[1], a.pl[1], a.plold[1], a.pstride);
-            ez = medium_update(ez, rz, vz, (int)c3[2], a.med, o, a.eold[2], a.pd[2], a.pl[2], a.plold[2], a.pstride);
-        }
+    if (MED == 1 && tg != 0) {
+        // vacuum: + dt_eps0 rotH; dielectric: + dt_eps0/eps_r rotH
+        const int8_t* c3 = a.cls + 3 * tg;
+        const int m0 = c3[0], m1 = c3[1], m2 = c3[2];
+        ex = m0 == 0 ? ex + vx : ex + a.med[m0].cdt * rx;
+        ey = m1 == 0 ? ey + vy : ey + a.med[m1].cdt * ry;
+        ez = m2 == 0 ? ez + vz : ez + a.med[m2].cdt * rz;
     } else {
         ex = ex + vx; ey = ey + vy; ez = ez + vz;
     }
@@ -686,52 +732,51 @@ __device__ __forceinline__ void e_cpml_z(const YeeArgs<T>& a, int i, int j, int 
     }
 }
 
-template <typename T, bool CPML, bool MEDIA>
-__device__ __forceinline__ void e_cell(const YeeArgs<T>& a, int tg, int i, int j, int k, long o, long jk,
-                                       T hzc, T hzj, T hzi, T hyc, T hyk, T hyi, T hxc, T hxj, T hxk,
-                                       T dex, T cex_, T dey, T dez, T cey, T cez, T& ex, T& ey, T& ez) {
-    const T dep = a.dt_eps0;
-    e_bulk<T, MEDIA ? 2 : 0>(a, tg, o, dep, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dex, cex_, dey, dez, cey, cez,
-                             ex, ey, ez);
-    if constexpr (CPML) {
-        e_cpml_x(a, i, jk, dep, hzc, hzi, hyc, hyi, ey, ez);
-        e_cpml_y(a, i, j, k, dep, hzc, hzj, hxc, hxj, ex, ez);
-        e_cpml_z(a, i, j, k, dep, hyc, hyk, hxc, hxk, ex, ey);
-    }
-}
-
-// MED as for e_bulk; with MED 1 the tile skips the cells of dispersive
-// classes entirely (E and psi of E), which k_e_dispersive then updates.
-template <typename T, bool CPML, int MED>
+// E tile (the tile and the bulk / face loops as in h_tile).  MED 0: no
+// media; 1: vacuum and dielectric components, the cells of dispersive
+// classes (tag >= disp_tag0) left entirely (E and psi of E) to
+// k_e_dispersive.
+template <typename T, bool CPML, int MED, bool B32>
 __device__ __forceinline__ void e_tile(const YeeArgs<T>& a, const int t) {
+    static_assert(MED == 0 || MED == 1, "the tiles take vacuum and dielectric cells");
+    using O = Off<T, B32>;
+    using I = typename O::I;
     const int tz = t % a.ntz;
     const int r = t / a.ntz;
     const int ty = r % a.nty;
     const int tx = r / a.nty;
     const int k = tz * kLanes + threadIdx.x + 1;
-    const int j = ty * a.rows + threadIdx.y + 1;
+    const int j = ty * a.rows + __builtin_amdgcn_readfirstlane(threadIdx.y) + 1;
     if (k > a.nz || j > a.ny) return;
     const int i0 = tx * a.xchunk + 1;
     const int i1 = min(i0 + a.xchunk - 1, a.nx);
-    const long sx = a.sx;
-    const int sz = a.sz;
-    const long jk = (long)j * sz + k + a.off;
-    long o = (long)i0 * sx + jk;
+    const I sx = O::of(a.sx), sz = O::of(a.sz), s1 = O::of(1);
+    const long jke = (long)j * a.sz + k + a.off;
+    I o = O::of((long)i0 * a.sx + jke);
     const T dey = a.den_ey[j], dez = a.den_ez[k];
     const T cey = sizeof(T) == 4 ? a.cdey[j] : (T)0, cez = sizeof(T) == 4 ? a.cdez[k] : (T)0;
-    T hzi = a.hz[o - sx], hyi = a.hy[o - sx];
-    if (!CPML || !a.cpml_sep) {
+    const T dep = a.dt_eps0;
+    const bool tags = MED != 0 && a.tag != nullptr;
+    T hzi = O::ld(a.hz, o - sx), hyi = O::ld(a.hy, o - sx);
+    bool faces = false;
+    if (CPML) {
+        const unsigned cp = a.cpml;
+        const int k0 = tz * kLanes + 1, k1 = min(k0 + kLanes - 1, a.nz);
+        faces = in_faces_e(cp & 1u, cp & 2u, i0, i1, a.nx, a.npml) || in_faces_e(cp & 4u, cp & 8u, j, j, a.ny, a.npml) ||
+                in_faces_e(cp & 16u, cp & 32u, k0, k1, a.nz, a.npml);
+    }
+    if (!faces) {
         for (int i = i0; i <= i1; ++i, o += sx) {
-            const T hzc = a.hz[o], hyc = a.hy[o], hxc = a.hx[o];
-            const T hzj = a.hz[o - sz], hxj = a.hx[o - sz];
-            const T hyk = a.hy[o - 1], hxk = a.hx[o - 1];
-            T ex = a.ex[o], ey = a.ey[o], ez = a.ez[o];
-            const int tg = (MED != 0 && a.tag != nullptr) ? (int)a.tag[o] : 0;
+            const T hzc = O::ld(a.hz, o), hyc = O::ld(a.hy, o), hxc = O::ld(a.hx, o);
+            const T hzj = O::ld(a.hz, o - sz), hxj = O::ld(a.hx, o - sz);
+            const T hyk = O::ld(a.hy, o - s1), hxk = O::ld(a.hx, o - s1);
+            T ex = O::ld(a.ex, o), ey = O::ld(a.ey, o), ez = O::ld(a.ez, o);
+            const int tg = tags ? O::tag(a.tag, o) : 0;
             const T dex = a.den_ex[i], cex = sizeof(T) == 4 ? a.cdex[i] : (T)0;
             if (MED != 1 || tg < a.disp_tag0) {
-                e_cell<T, CPML, MED != 0>(a, tg, i, j, k, o, jk, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dex,
-                                          cex, dey, dez, cey, cez, ex, ey, ez);
-                a.ex[o] = ex; a.ey[o] = ey; a.ez[o] = ez;
+                e_bulk<T, MED>(a, tg, dep, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dex, cex, dey, dez, cey,
+                               cez, ex, ey, ez);
+                O::st(a.ex, o, ex); O::st(a.ey, o, ey); O::st(a.ez, o, ez);
             }
             hzi = hzc; hyi = hyc;
         }
@@ -744,64 +789,65 @@ __device__ __forceinline__ void e_tile(const YeeArgs<T>& a, const int t) {
     T yb = (T)0, yc = (T)0, zb = (T)0, zc = (T)0;
     if (fy.slot >= 0) { yb = a.be[fy.c]; yc = a.ced[fy.c]; }
     if (fz.slot >= 0) { zb = a.be[fz.c]; zc = a.ced[fz.c]; }
-    const T dep = a.dt_eps0;
-    for (int i = i0; i <= i1; ++i, o += sx) {
-        const T hzc = a.hz[o], hyc = a.hy[o], hxc = a.hx[o];
-        const T hzj = a.hz[o - sz], hxj = a.hx[o - sz];
-        const T hyk = a.hy[o - 1], hxk = a.hx[o - 1];
-        T ex = a.ex[o], ey = a.ey[o], ez = a.ez[o];
-        const int tg = (MED != 0 && a.tag != nullptr) ? (int)a.tag[o] : 0;
+    const I jk = O::of(jke);
+    const I ystep = O::of((long)nse * a.sz), zstep = O::of((long)a.sy * nse);
+    I py = O::of((long)i0 * nse * a.sz + (long)fy.slot * a.sz + k + a.off);
+    I pz = O::of(((long)i0 * a.sy + j) * nse + fz.slot);
+    for (int i = i0; i <= i1; ++i, o += sx, py += ystep, pz += zstep) {
+        const T hzc = O::ld(a.hz, o), hyc = O::ld(a.hy, o), hxc = O::ld(a.hx, o);
+        const T hzj = O::ld(a.hz, o - sz), hxj = O::ld(a.hx, o - sz);
+        const T hyk = O::ld(a.hy, o - s1), hxk = O::ld(a.hx, o - s1);
+        T ex = O::ld(a.ex, o), ey = O::ld(a.ey, o), ez = O::ld(a.ez, o);
+        const int tg = tags ? O::tag(a.tag, o) : 0;
         const T dex = a.den_ex[i], cex = sizeof(T) == 4 ? a.cdex[i] : (T)0;
         if (MED == 1 && tg >= a.disp_tag0) {
             hzi = hzc; hyi = hyc;
             continue;
         }
         const Face fx = face_e(cp & 1u, cp & 2u, i, a.nx, npml);
-        const long psx = (long)fx.slot * sx + jk;
-        const long psy = ((long)i * nse + fy.slot) * sz + k + a.off;
-        const long psz = ((long)i * a.sy + j) * nse + fz.slot;
+        const I px = O::of((long)fx.slot * a.sx) + jk;
         T px0 = (T)0, px1 = (T)0, xb = (T)0, xc = (T)0, py0 = (T)0, py1 = (T)0, pz0 = (T)0, pz1 = (T)0;
-        if (fx.slot >= 0) { px0 = a.pe[0][psx]; px1 = a.pe[1][psx]; xb = a.be[fx.c]; xc = a.ced[fx.c]; }
-        if (fy.slot >= 0) { py0 = a.pe[2][psy]; py1 = a.pe[3][psy]; }
-        if (fz.slot >= 0) { pz0 = a.pe[4][psz]; pz1 = a.pe[5][psz]; }
-        e_bulk<T, MED>(a, tg, o, dep, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dex, cex, dey, dez, cey, cez,
+        if (fx.slot >= 0) { px0 = O::ld(a.pe[0], px); px1 = O::ld(a.pe[1], px); xb = a.be[fx.c]; xc = a.ced[fx.c]; }
+        if (fy.slot >= 0) { py0 = O::ld(a.pe[2], py); py1 = O::ld(a.pe[3], py); }
+        if (fz.slot >= 0) { pz0 = O::ld(a.pe[4], pz); pz1 = O::ld(a.pe[5], pz); }
+        e_bulk<T, MED>(a, tg, dep, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dex, cex, dey, dez, cey, cez,
                        ex, ey, ez);
         if (fx.slot >= 0) {
             const T p0 = psi_step(px0, xb, xc, hzi - hzc), p1 = psi_step(px1, xb, xc, hyc - hyi);
-            a.pe[0][psx] = p0; a.pe[1][psx] = p1;
+            O::st(a.pe[0], px, p0); O::st(a.pe[1], px, p1);
             ey = ey + dep * p0; ez = ez + dep * p1;
         }
         if (fy.slot >= 0) {
             const T p0 = psi_step(py0, yb, yc, hzc - hzj), p1 = psi_step(py1, yb, yc, hxj - hxc);
-            a.pe[2][psy] = p0; a.pe[3][psy] = p1;
+            O::st(a.pe[2], py, p0); O::st(a.pe[3], py, p1);
             ex = ex + dep * p0; ez = ez + dep * p1;
         }
         if (fz.slot >= 0) {
             const T p0 = psi_step(pz0, zb, zc, hyk - hyc), p1 = psi_step(pz1, zb, zc, hxc - hxk);
-            a.pe[4][psz] = p0; a.pe[5][psz] = p1;
+            O::st(a.pe[4], pz, p0); O::st(a.pe[5], pz, p1);
             ex = ex + dep * p0; ey = ey + dep * p1;
         }
-        a.ex[o] = ex; a.ey[o] = ey; a.ez[o] = ez;
+        O::st(a.ex, o, ex); O::st(a.ey, o, ey); O::st(a.ez, o, ez);
         hzi = hzc; hyi = hyc;
     }
 }
 
-template <typename T, bool CPML>
+template <typename T, bool CPML, bool B32>
 __global__ __launch_bounds__(256) void k_h_update(const YeeArgs<T> a) {
-    h_tile<T, CPML>(a, xcd_tile(blockIdx.x, gridDim.x));
+    h_tile<T, CPML, B32>(a, tile_of_block(a.tile_order));
 }
 
-template <typename T, bool CPML, int MED>
+template <typename T, bool CPML, int MED, bool B32>
 __global__ __launch_bounds__(256) void k_e_update(const YeeArgs<T> a) {
-    e_tile<T, CPML, MED>(a, xcd_tile(blockIdx.x, gridDim.x));
+    e_tile<T, CPML, MED, B32>(a, tile_of_block(a.tile_order));
 }
 
 // E of the cells of dispersive classes (Drude / Drude-Lorentz components),
-// after the tile pass has updated every other cell: the per-cell update of
-// e_cell with the full medium path, from E in a.ex/a.ey/a.ez to the output
-// arrays (the same arrays in the separate H/E path, the other buffer after
-// the fused step).  list holds the cells' element offsets with the media of
-// their three components (the same arithmetic as e_cell).
+// after the tile pass has updated every other cell: curl of H, the full
+// medium update and the CPML psi recursions of the cell, from E in
+// a.ex/a.ey/a.ez to the output arrays (the same arrays in the separate H/E
+// path, the other buffer after the fused step).  list holds the cells'
+// element offsets with the media of their three components.
 template <typename T, bool CPML>
 __device__ __forceinline__ void e_dispersive_cell(const YeeArgs<T>& a, T* __restrict__ exo, T* __restrict__ eyo,
                                                   T* __restrict__ ezo, long e) {
@@ -917,21 +963,6 @@ struct FusedCold {
     double dt_mu0;
 };
 
-// Field access by 32-bit byte offset from a uniform base pointer (the host
-// only takes the fused path when every field array is below 4 GiB).
-template <typename T>
-__device__ __forceinline__ T ldb(const T* p, uint32_t b) {
-    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + b);
-}
-template <typename T>
-__device__ __forceinline__ void stb(T* p, uint32_t b, T v) {
-    *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + b) = v;
-}
-// Uniform read of a table the kernel never writes (scalar load).
-template <typename T>
-__device__ __forceinline__ T ldc(const T* p, int i) {
-    return ((const __attribute__((address_space(4))) T*)p)[i];
-}
 
 // Values of one x plane p that a fused workgroup loads one plane ahead.
 template <typename T>
@@ -1067,7 +1098,7 @@ __device__ __forceinline__ void fused_plane(const FusedArgs<T>& f, const FusedCo
         const T hzj = lds[h0], hxj = lds[h0 + pl];
         T ex = exc, ey = eyc, ez = ezc;
         const T dep = f.dt_eps0;
-        e_bulk<T, MEDIA ? 1 : 0>(cold->a, cur.tg, o, dep, hz, hzj, hzi, hy, hyk, hyi, hx, hxj, hxk, ldc(f.dex, i),
+        e_bulk<T, MEDIA ? 1 : 0>(cold->a, cur.tg, dep, hz, hzj, hzi, hy, hyk, hyi, hx, hxj, hxk, ldc(f.dex, i),
                          sizeof(T) == 4 ? ldc(f.cex, i) : (T)0, c.dey, c.dez, c.cey, c.cez, ex, ey, ez);
         if constexpr (CPML) {
             if (f.cpml & 15u) {
@@ -1234,26 +1265,26 @@ __device__ __forceinline__ int find_grid(const int* __restrict__ tile_start, int
     return lo;
 }
 
-template <typename T>
+template <typename T, bool B32>
 __global__ __launch_bounds__(256) void k_h_update_batched(const YeeArgs<T>* __restrict__ args,
-                                                          const int* __restrict__ tile_start, int n) {
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
+                                                          const int* __restrict__ tile_start, int n, int order) {
+    const int t = tile_of_block(order);
     const int g = find_grid(tile_start, n, t);
     const YeeArgs<T>& a = args[g];
     if ((int)threadIdx.y >= a.rows) return;
-    if (a.cpml) h_tile<T, true>(a, t - tile_start[g]);
-    else h_tile<T, false>(a, t - tile_start[g]);
+    if (a.cpml) h_tile<T, true, B32>(a, t - tile_start[g]);
+    else h_tile<T, false, B32>(a, t - tile_start[g]);
 }
 
-template <typename T>
+template <typename T, bool B32>
 __global__ __launch_bounds__(256) void k_e_update_batched(const YeeArgs<T>* __restrict__ args,
-                                                          const int* __restrict__ tile_start, int n) {
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
+                                                          const int* __restrict__ tile_start, int n, int order) {
+    const int t = tile_of_block(order);
     const int g = find_grid(tile_start, n, t);
     const YeeArgs<T>& a = args[g];
     if ((int)threadIdx.y >= a.rows) return;
-    if (a.cpml) e_tile<T, true, 1>(a, t - tile_start[g]);
-    else e_tile<T, false, 1>(a, t - tile_start[g]);
+    if (a.cpml) e_tile<T, true, 1, B32>(a, t - tile_start[g]);
+    else e_tile<T, false, 1, B32>(a, t - tile_start[g]);
 }
 
 // ---------------------------------------------------------------------------

@@ -61,7 +61,8 @@ _SIGS = {
     "omx_medium_library": ([C.c_char_p, _D, C.POINTER(Medium)], C.c_int),
     "omx3d_create": ([C.POINTER(Desc), C.POINTER(_P)], C.c_int),
     "omx3d_destroy": ([_P], C.c_int),
-    "omx3d_set_tiling": ([_P, _I, _I], C.c_int),
+    "omx3d_set_tiling": ([_P, _I, _I, _I], C.c_int),
+    "omx3d_get_tiling": ([_P, _IP, _IP, _IP], C.c_int),
     "omx3d_get_cpml": ([_P, _DP, _DP, _DP, _DP, _DP], C.c_int),
     "omx3d_set_media": ([_P, _I, C.POINTER(Medium), C.POINTER(C.c_int8)], C.c_int),
     "omx3d_set_media_shapes": ([_P, _I, C.POINTER(Medium), _I, C.POINTER(Shape)], C.c_int),
@@ -90,7 +91,7 @@ _SIGS = {
     "omx3d_get_profile": ([_P, _DP, _DP, _IP], C.c_int),
     "omx3d_synchronize": ([_P], C.c_int),
     "omx3d_device_bytes": ([_P, C.POINTER(C.c_int64)], C.c_int),
-    "omx3d_autotune": ([_P, _I, _IP, _IP, _DP], C.c_int),
+    "omx3d_autotune": ([_P, _I, _IP, _IP, _IP, _DP], C.c_int),
     "omx3d_set_step_mode": ([_P, _I, _I, _I], C.c_int),
     "omx3d_get_step_mode": ([_P, _IP], C.c_int),
     "omxb_create": ([_I, C.POINTER(_P), C.POINTER(_P)], C.c_int),

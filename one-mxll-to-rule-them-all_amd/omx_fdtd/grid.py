@@ -98,8 +98,14 @@ class Grid3D:
     def __exit__(self, *exc):
         self.kill()
 
-    def set_tiling(self, rows_y=0, xchunk=0):
-        L.call("omx3d_set_tiling", self.h, int(rows_y), int(xchunk))
+    def set_tiling(self, rows_y=0, xchunk=0, tile_order=-1):
+        """rows_y / xchunk 0 and tile_order -1 keep the current value."""
+        L.call("omx3d_set_tiling", self.h, int(rows_y), int(xchunk), int(tile_order))
+
+    def tiling(self):
+        r, x, o = C.c_int32(), C.c_int32(), C.c_int32()
+        L.call("omx3d_get_tiling", self.h, C.byref(r), C.byref(x), C.byref(o))
+        return r.value, x.value, o.value
 
     def set_step_mode(self, mode=0, rows=0, xchunk=0):
         """mode 0: fused H+E pass where it applies; 1: separate H and E kernels."""
@@ -112,9 +118,9 @@ class Grid3D:
 
     def autotune(self, steps=4):
         """Pick the fastest tile shape; clears the fields (call before a run)."""
-        r, x, ms = C.c_int32(), C.c_int32(), C.c_double()
-        L.call("omx3d_autotune", self.h, int(steps), C.byref(r), C.byref(x), C.byref(ms))
-        return r.value, x.value, ms.value
+        r, x, o, ms = C.c_int32(), C.c_int32(), C.c_int32(), C.c_double()
+        L.call("omx3d_autotune", self.h, int(steps), C.byref(r), C.byref(x), C.byref(o), C.byref(ms))
+        return r.value, x.value, o.value, ms.value
 
     # -- coefficients ------------------------------------------------------
     def cpml_tables(self):

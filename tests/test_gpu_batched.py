@@ -24,7 +24,7 @@ def test_batch_equals_single_grid_runs(precision, graph):
     cs = cases.config_batched(n_grids=6, edges=(24, 32, 40), npml=4, steps=60, record_every=3)
     # a second source sharing a cell with the first of grid 0 exercises source passes
     cs[0].sources.append(cs[0].sources[2])
-    tilings = [(4, 16), (2, 5), (1, 64), (4, 3), (4, 8), (1, 1)]
+    tilings = [(4, 16, 1), (2, 5, 0), (1, 64, 1), (4, 3, 0), (4, 8, 1), (1, 1, 0)]
     print(f"[batch] precision={precision} graph={graph} grids={[c.grid for c in cs]}", flush=True)
     singles = []
     for c, t in zip(cs, tilings):
@@ -54,7 +54,8 @@ def test_autotune_keeps_results_identical():
     ref = g.read_records()
     g.kill()
     g = make(c, 4)
-    rows, xc, ms = g.autotune(steps=2)
-    print(f"autotune: rows_y={rows} xchunk={xc} {ms:.4f} ms/step")
+    rows, xc, order, ms = g.autotune(steps=2)
+    print(f"autotune: rows_y={rows} xchunk={xc} tile order={order} {ms:.4f} ms/step")
+    assert g.tiling() == (rows, xc, order)
     g.run(1, c.steps, c.record_every)
     np.testing.assert_array_equal(g.read_records(), ref)

@@ -134,6 +134,22 @@ def test_media_fp64(medium):
     assert r <= TOL64
 
 
+@pytest.mark.parametrize("precision", [8, 4])
+def test_cpml_faces_adjacent(precision):
+    """Grids exactly 2 npml cells along x and z: the low and high CPML faces
+    touch, every cell of those axes lies in one of them and a z tile holds
+    both; vacuum, dielectric and Drude-Lorentz cells."""
+    spec = ("drude-lorentz", "Ag", 1.0)
+    shapes = [(1, 1, [8.0, 9.0, 8.0, 3.5]), (3, 2, [2, 2, 2, 15, 17, 3])]
+    case = small_case("faces-adjacent", n=(16, 18, 16), npml=8, media=[spec, ("dielectric", 2.25)],
+                      shapes=shapes, steps=80)
+    g, o, got, want = run_both(case, precision=precision)
+    assert np.abs(want).max() > 0
+    r = rel_l2(got, want)
+    print(f"FP{8 * precision} relL2 = {r:.3e}")
+    assert r <= (TOL64 if precision == 8 else TOL32)
+
+
 def test_dl_sphere_fp32_against_fp64_oracle():
     spec = ("dl-custom", 5.0, 9.0, 0.07, [(1.2, 3.1, 0.3), (0.7, 2.7, 0.15)])
     case = small_case("fp32-dl", n=(28, 26, 24), media=[spec], shapes=[(1, 1, [14.0, 13.0, 12.0, 5.0])],
@@ -150,11 +166,12 @@ def test_tiling_variants_identical():
                       steps=40)
     ref = None
     # separate H/E kernels under several tile shapes
-    for rows, xc in ((4, 16), (1, 1), (2, 7), (4, 64)):
+    for rows, xc, order in ((4, 16, 0), (1, 1, 1), (2, 7, 0), (4, 64, 1), (1, 4, 0), (1, 4, 1)):
         dr, dt = case.dr_dt
         g = omx_fdtd.Grid3D(case.grid, case.npml, case.boundaries, dt, dr, precision=8)
         g.set_step_mode(1)
-        g.set_tiling(rows, xc)
+        g.set_tiling(rows, xc, order)
+        assert g.tiling() == (rows, xc, order)
         cases.apply(case, g, library=G.library_medium, make_medium=G.medium)
         g.run(1, case.steps, 1)
         assert not g.last_run_fused()
