@@ -79,9 +79,11 @@ def algorithmic_bytes(case, g, w):
     dispersive component cell 2 words (Drude: PD) or 4 + 4P words
     (Drude-Lorentz with P poles: E_old, PD, PL_k, PL_old_k), all read and
     written, plus the one-byte cell tag when the grid holds media.  Returns
-    (H bytes, E bytes, dispersive-state bytes, dispersive component cells);
-    the E tile kernel moves E bytes minus the dispersive state, which the
-    dispersive-cell pass moves."""
+    (H bytes, E bytes, dispersive-state bytes, dispersive component cells).
+    The separate E launch (tiles and dispersive cells) moves the E bytes;
+    the fused step moves H and E bytes but its dispersive cells run in a
+    launch of their own, so the fused kernel moves H + E minus the
+    dispersive state."""
     nx, ny, nz = case.grid
     N = nx * ny * nz
     p = case.npml
@@ -209,7 +211,7 @@ def roofline(case, g, prec, res, workload):
     if res["fused"]:
         kname, b_dom, ms_dom = "k_step_fused", b_h + b_e - b_disp, res["ms_e"]
     elif res["ms_e"] >= res["ms_h"]:
-        kname, b_dom, ms_dom = "k_e_update", b_e - b_disp, res["ms_e"]
+        kname, b_dom, ms_dom = "k_e_update", b_e, res["ms_e"]
     else:
         kname, b_dom, ms_dom = "k_h_update", b_h, res["ms_h"]
     achieved = b_dom / (ms_dom * 1e-3) / 1e9

@@ -177,43 +177,50 @@ int launch_j_correct(omx3d_ctx* c, double tau) {
 template <typename T>
 int launch_e_dispersive(omx3d_ctx* c, const YeeArgs<T>& a, T* exo, T* eyo, T* ezo, hipStream_t st) {
     const unsigned nb = blocks_for(c->n_disp, 256);
-    if (c->cpml_mask)
-        hipLaunchKernelGGL((k_e_dispersive<T, true>), dim3(nb), dim3(256), 0, st, a, exo, eyo, ezo, c->d_disp,
+    const bool b32 = offsets32(c);
+    if (c->cpml_mask && b32)
+        hipLaunchKernelGGL((k_e_dispersive<T, true, true>), dim3(nb), dim3(256), 0, st, a, exo, eyo, ezo, c->d_disp,
                            c->n_disp);
+    else if (c->cpml_mask)
+        hipLaunchKernelGGL((k_e_dispersive<T, true, false>), dim3(nb), dim3(256), 0, st, a, exo, eyo, ezo,
+                           c->d_disp, c->n_disp);
+    else if (b32)
+        hipLaunchKernelGGL((k_e_dispersive<T, false, true>), dim3(nb), dim3(256), 0, st, a, exo, eyo, ezo,
+                           c->d_disp, c->n_disp);
     else
-        hipLaunchKernelGGL((k_e_dispersive<T, false>), dim3(nb), dim3(256), 0, st, a, exo, eyo, ezo, c->d_disp,
-                           c->n_disp);
+        hipLaunchKernelGGL((k_e_dispersive<T, false, false>), dim3(nb), dim3(256), 0, st, a, exo, eyo, ezo,
+                           c->d_disp, c->n_disp);
     HIPCHK(hipGetLastError());
     return OMX_OK;
 }
 
-// ev0/ev1 (profiling, may be null) bracket the E tile kernel itself.
+// The E tiles and, in the same launch, the cells of dispersive classes.
+// ev0/ev1 (profiling, may be null) bracket that kernel.
 template <typename T>
 int launch_e(omx3d_ctx* c, double time, double t_skip, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
     int rc = wrap_h<T>(c, c->s);
     if (rc) return rc;
     YeeArgs<T> a = make_args<T>(c);
     const dim3 blk(kLanes, c->rows_y);
-    const unsigned nb = yee_blocks(c, a);
     const bool cp = c->cpml_mask != 0, md = c->media_set && c->n_classes > 1;
+    const int nd = md ? c->n_disp : 0;
+    const int db = (nd + kLanes * c->rows_y - 1) / (kLanes * c->rows_y);
+    const dim3 grid(yee_blocks(c, a) + db);
+    const long* dl = c->d_disp;
     if (ev0) HIPCHK(hipEventRecord(ev0, c->s));
     if (offsets32(c)) {
-        if (cp && md) hipLaunchKernelGGL((k_e_update<T, true, 1, true>), dim3(nb), blk, 0, c->s, a);
-        else if (cp) hipLaunchKernelGGL((k_e_update<T, true, 0, true>), dim3(nb), blk, 0, c->s, a);
-        else if (md) hipLaunchKernelGGL((k_e_update<T, false, 1, true>), dim3(nb), blk, 0, c->s, a);
-        else hipLaunchKernelGGL((k_e_update<T, false, 0, true>), dim3(nb), blk, 0, c->s, a);
+        if (cp && md) hipLaunchKernelGGL((k_e_update<T, true, 1, true>), grid, blk, 0, c->s, a, dl, nd, db);
+        else if (cp) hipLaunchKernelGGL((k_e_update<T, true, 0, true>), grid, blk, 0, c->s, a, dl, 0, 0);
+        else if (md) hipLaunchKernelGGL((k_e_update<T, false, 1, true>), grid, blk, 0, c->s, a, dl, nd, db);
+        else hipLaunchKernelGGL((k_e_update<T, false, 0, true>), grid, blk, 0, c->s, a, dl, 0, 0);
     } else {
-        if (cp && md) hipLaunchKernelGGL((k_e_update<T, true, 1, false>), dim3(nb), blk, 0, c->s, a);
-        else if (cp) hipLaunchKernelGGL((k_e_update<T, true, 0, false>), dim3(nb), blk, 0, c->s, a);
-        else if (md) hipLaunchKernelGGL((k_e_update<T, false, 1, false>), dim3(nb), blk, 0, c->s, a);
-        else hipLaunchKernelGGL((k_e_update<T, false, 0, false>), dim3(nb), blk, 0, c->s, a);
+        if (cp && md) hipLaunchKernelGGL((k_e_update<T, true, 1, false>), grid, blk, 0, c->s, a, dl, nd, db);
+        else if (cp) hipLaunchKernelGGL((k_e_update<T, true, 0, false>), grid, blk, 0, c->s, a, dl, 0, 0);
+        else if (md) hipLaunchKernelGGL((k_e_update<T, false, 1, false>), grid, blk, 0, c->s, a, dl, nd, db);
+        else hipLaunchKernelGGL((k_e_update<T, false, 0, false>), grid, blk, 0, c->s, a, dl, 0, 0);
     }
     if (ev1) HIPCHK(hipEventRecord(ev1, c->s));
     HIPCHK(hipGetLastError());
-    if (md && c->n_disp > 0) {
-        rc = launch_e_dispersive<T>(c, a, a.ex, a.ey, a.ez, c->s);
-        if (rc) return rc;
-    }
     return launch_j_correct<T>(c, time - t_skip);
 }
 
@@ -681,10 +688,13 @@ int build_tags(omx3d_ctx* c) {
     c->device_bytes -= (long)(sizeof(unsigned) * ((ncode + 31) / 32) + ncode);
     dfreet(d_present);
     dfreet(d_lut);
-    // the cells of dispersive classes, in memory order, for k_e_dispersive
+    // the cells of dispersive classes, in memory order, for the dispersive
+    // part of the E launch: i | j << 16 | k << 32 | media << 48
     dfreet(c->d_disp);
     c->n_disp = 0;
     if (c->disp_tag0 < 256) {
+        if (L.nx > 65535 || L.ny > 65535 || L.nz > 65535)
+            return fail(OMX_ERR_RANGE, "Drude / Drude-Lorentz media need at most 65535 cells per axis");
         std::vector<uint8_t> tags(L.nalloc);
         HIPCHK(hipMemcpyAsync(tags.data(), c->d_tag, L.nalloc, hipMemcpyDeviceToHost, c->s));
         HIPCHK(hipStreamSynchronize(c->s));
@@ -696,7 +706,8 @@ int build_tags(omx3d_ctx* c) {
                     const int tg = tags[o];
                     if (tg >= c->disp_tag0) {
                         const long m0 = cls[3 * tg], m1 = cls[3 * tg + 1], m2 = cls[3 * tg + 2];
-                        cells.push_back(o | (m0 << 48) | (m1 << 52) | (m2 << 56));
+                        cells.push_back((long)i | ((long)j << 16) | ((long)k << 32) | (m0 << 48) | (m1 << 52) |
+                                        (m2 << 56));
                     }
                 }
         if (cells.size() > (size_t)INT32_MAX) return fail(OMX_ERR_RANGE, "too many dispersive cells");
@@ -817,9 +828,9 @@ struct omxb_ctx {
     int tile_order = 0;             // tile order of the batched H/E kernels: the first grid's
     bool b32 = true;                // every grid addressed by 32-bit byte offsets
     long* d_disp = nullptr;         // cells of dispersive classes of every grid (packed as per grid)
-    int* d_disp_grid = nullptr;     // their grid
-    int n_disp = 0;
-    bool disp_cpml = false;
+    int* d_disp_grid = nullptr;     // grid of each workgroup of dispersive cells
+    int n_disp = 0;                 // list entries (whole workgroups, padding -1)
+    int disp_blocks = 0;
 };
 
 namespace {
@@ -853,24 +864,28 @@ int batch_setup(omxb_ctx* b) {
     int rc = bupload(b->s, (YeeArgs<T>**)&b->d_args, args);
     if (!rc) rc = bupload(b->s, &b->d_tile_start, ts);
     if (rc) return rc;
-    // cells of dispersive classes of all grids, for one batched pass
+    // cells of dispersive classes of all grids, for the first workgroups of
+    // the batched E launch: each grid's cells fill whole workgroups (padded
+    // with -1), so that a workgroup's grid, and with it its argument block,
+    // is uniform
     {
+        const size_t wg = (size_t)kLanes * b->rows;
         std::vector<long> cells;
-        std::vector<int> gof;
-        b->disp_cpml = false;
+        std::vector<int> gob;
         for (int q = 0; q < b->n; ++q) {
             omx3d_ctx* c = b->g[q];
             if (!(c->media_set && c->n_classes > 1 && c->n_disp > 0)) continue;
             std::vector<long> h(c->n_disp);
             HIPCHK(hipMemcpy(h.data(), c->d_disp, sizeof(long) * h.size(), hipMemcpyDeviceToHost));
             cells.insert(cells.end(), h.begin(), h.end());
-            gof.insert(gof.end(), h.size(), q);
-            if (c->cpml_mask) b->disp_cpml = true;
+            cells.resize((cells.size() + wg - 1) / wg * wg, -1L);
+            gob.resize(cells.size() / wg, q);
         }
         if (cells.size() > (size_t)INT32_MAX) return fail(OMX_ERR_RANGE, "too many dispersive cells in the batch");
         b->n_disp = (int)cells.size();
+        b->disp_blocks = (int)gob.size();
         rc = bupload(b->s, &b->d_disp, cells);
-        if (!rc) rc = bupload(b->s, &b->d_disp_grid, gof);
+        if (!rc) rc = bupload(b->s, &b->d_disp_grid, gob);
         if (rc) return rc;
     }
     // sources: one pass per repeated (field, cell), in source order
@@ -953,15 +968,16 @@ int batch_step_launches(omxb_ctx* b) {
     }
     if (b->periodic)
         for (auto* c : b->g) { int rc = wrap_h<T>(c, b->s); if (rc) return rc; }
+    // the dispersive cells of every grid ride in the first db workgroups
+    const int db = b->disp_blocks;
     if (b->b32)
-        hipLaunchKernelGGL((k_e_update_batched<T, true>), dim3(b->total_tiles), blk, 0, b->s,
-                           (const YeeArgs<T>*)b->d_args, b->d_tile_start, b->n, b->tile_order);
+        hipLaunchKernelGGL((k_e_update_batched<T, true>), dim3(b->total_tiles + db), blk, 0, b->s,
+                           (const YeeArgs<T>*)b->d_args, b->d_tile_start, b->n, b->tile_order, b->d_disp,
+                           b->d_disp_grid, b->n_disp, db);
     else
-        hipLaunchKernelGGL((k_e_update_batched<T, false>), dim3(b->total_tiles), blk, 0, b->s,
-                           (const YeeArgs<T>*)b->d_args, b->d_tile_start, b->n, b->tile_order);
-    if (b->n_disp > 0)
-        hipLaunchKernelGGL(k_e_dispersive_batched<T>, dim3(blocks_for(b->n_disp, 256)), dim3(256), 0, b->s,
-                           (const YeeArgs<T>*)b->d_args, b->d_disp, b->d_disp_grid, b->n_disp);
+        hipLaunchKernelGGL((k_e_update_batched<T, false>), dim3(b->total_tiles + db), blk, 0, b->s,
+                           (const YeeArgs<T>*)b->d_args, b->d_tile_start, b->n, b->tile_order, b->d_disp,
+                           b->d_disp_grid, b->n_disp, db);
     if (b->n_samp > 0)
         hipLaunchKernelGGL(k_sample_batched<T>, dim3(blocks_for(b->n_samp, 256)), dim3(256), 0, b->s,
                            (const YeeArgs<T>*)b->d_args, b->d_br, b->d_grid_of, b->d_local, b->d_code, b->d_off,
@@ -1105,6 +1121,10 @@ int omx3d_create(const omx3d_desc* desc, omx3d_ctx** out) {
     c->d = d;
     c->prec = d.precision;
     c->esz = (size_t)d.precision;
+    // default tile shapes, measured fastest on 256^3 CPML grids: FP32 4 rows
+    // x 2 planes, FP64 1 row x 4 planes (block order)
+    c->rows_y = d.precision == OMX_FP32 ? 4 : 1;
+    c->xchunk = d.precision == OMX_FP32 ? 2 : 4;
     c->cpml_mask = mask;
     Layout& L = c->L;
     const int A = 128 / (int)c->esz;

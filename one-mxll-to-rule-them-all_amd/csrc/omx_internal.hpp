@@ -137,7 +137,7 @@ struct omx3d_ctx {
     int8_t* d_cls = nullptr;        // 3 * n_classes
     int n_classes = 0;
     int disp_tag0 = 256;            // tags >= disp_tag0: classes with a Drude / DL component
-    long* d_disp = nullptr;         // cells of those classes (ascending): offset | media << 48
+    long* d_disp = nullptr;         // cells of those classes (memory order): i | j << 16 | k << 32 | media << 48
     int n_disp = 0;
     void* d_med = nullptr;          // MedDev<T>[n_media+1]
     omx_medium* d_med_in = nullptr;

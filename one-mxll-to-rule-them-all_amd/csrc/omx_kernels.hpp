@@ -22,14 +22,15 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
     return x * per + (x < rem ? x : rem) + q;
 }
 
-// Tile of this workgroup.  Order 0 keeps the block order: consecutive
-// workgroups, which the dispatcher deals round-robin to the XCDs, cover
-// consecutive memory, so the whole chip streams one region of every array at
-// a time (the neighbour rows another XCD read are found in the memory-side
-// cache).  Order 1 gives every XCD one contiguous slab of tiles, which keeps
-// the neighbour rows in its own L2 but streams eight regions at once.
-__device__ __forceinline__ int tile_of_block(int order) {
-    return order ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+// Tile of workgroup b of nb tile workgroups.  Order 0 keeps the block
+// order: consecutive workgroups, which the dispatcher deals round-robin to
+// the XCDs, cover consecutive memory, so the whole chip streams one region
+// of every array at a time (the neighbour rows another XCD read are found in
+// the memory-side cache).  Order 1 gives every XCD one contiguous slab of
+// tiles, which keeps the neighbour rows in its own L2 but streams eight
+// regions at once.
+__device__ __forceinline__ int tile_of_block(int order, int b, int nb) {
+    return order ? xcd_tile(b, nb) : b;
 }
 
 // Field access by 32-bit byte offset from a uniform base pointer (callers
@@ -569,49 +570,55 @@ __device__ __forceinline__ void h_tile(const YeeArgs<T>& a, const int t) {
 // medium update of get_medium_polarization (classical_medium_mod.f90:773-833).
 // Currents of embedded quantum systems are applied by k_j_correct afterwards.
 // ---------------------------------------------------------------------------
-template <typename T>
+// o: the cell's offset (Off<T, B32>) into eold, pd and into every pole plane
+// of pl / plold, which lie ps elements apart.
+template <typename T, bool B32>
 __device__ __forceinline__ T medium_update(T e, T rot, T vac, int m, const MedDev<T>* __restrict__ med,
-                                           long o, T* __restrict__ eold, T* __restrict__ pd,
+                                           typename Off<T, B32>::I o, T* __restrict__ eold, T* __restrict__ pd,
                                            T* __restrict__ pl, T* __restrict__ plold, long ps) {
+    using O = Off<T, B32>;
     if (m == 0) return e + vac;   // vac = dt_eps0 * rotH
     const MedDev<T>& M = med[m];
     if (M.kind == OMX_MEDIUM_DRUDE) {
-        const T p = pd[o];
+        const T p = O::ld(pd, o);
         if constexpr (sizeof(T) == 8) {
             const T t = M.C1 * e + M.C3 * rot - M.C4 * p;
-            pd[o] = M.A1 * p + M.A2 * (t + e);
+            O::st(pd, o, M.A1 * p + M.A2 * (t + e));
             return t;
         } else {
             const T t = e + (M.C1 * e + M.C3 * rot - M.C4 * p);
-            pd[o] = p + (M.A1 * p + M.A2 * (t + e));
+            O::st(pd, o, p + (M.A1 * p + M.A2 * (t + e)));
             return t;
         }
     }
     if (M.kind == OMX_MEDIUM_DL) {
-        const T eo = eold[o];
-        const T p = pd[o];
+        const T eo = O::ld(eold, o);
+        const T p = O::ld(pd, o);
         const int np = M.np;
         T t;
         if constexpr (sizeof(T) == 8) {
             t = M.C1 * e + M.C2 * eo + M.C3 * rot - M.C4 * p;
-            for (int q = 0; q < np; ++q) t = t - M.B1[q] * pl[o + q * ps] - M.B2[q] * plold[o + q * ps];
-            pd[o] = M.A1 * p + M.A2 * (t + e);
+            for (int q = 0; q < np; ++q) t = t - M.B1[q] * O::ld(pl + q * ps, o) - M.B2[q] * O::ld(plold + q * ps, o);
+            O::st(pd, o, M.A1 * p + M.A2 * (t + e));
         } else {
             T dlt = M.C1 * e + M.C2 * eo + M.C3 * rot - M.C4 * p;
-            for (int q = 0; q < np; ++q) dlt = dlt - M.B1[q] * pl[o + q * ps] - M.B2[q] * plold[o + q * ps];
+            for (int q = 0; q < np; ++q)
+                dlt = dlt - M.B1[q] * O::ld(pl + q * ps, o) - M.B2[q] * O::ld(plold + q * ps, o);
             t = e + dlt;
-            pd[o] = p + (M.A1 * p + M.A2 * (t + e));
+            O::st(pd, o, p + (M.A1 * p + M.A2 * (t + e)));
         }
         const T de = t - eo;
         for (int q = 0; q < np; ++q) {
-            const T L = pl[o + q * ps], Lo = plold[o + q * ps];
+            T* const lq = pl + q * ps;
+            T* const loq = plold + q * ps;
+            const T L = O::ld(lq, o), Lo = O::ld(loq, o);
             T nl;
             if constexpr (sizeof(T) == 8) nl = M.al[q] * L + M.bt[q] * Lo + M.ga[q] * de;
             else nl = L + (((L - Lo) + M.al[q] * L) + M.bt[q] * Lo + M.ga[q] * de);
-            plold[o + q * ps] = L;
-            pl[o + q * ps] = nl;
+            O::st(loq, o, L);
+            O::st(lq, o, nl);
         }
-        eold[o] = e;
+        O::st(eold, o, e);
         return t;
     }
     return e + M.cdt * rot;  // dielectric: dt_eps0 * (1/eps_r) * rotH
@@ -832,79 +839,74 @@ __device__ __forceinline__ void e_tile(const YeeArgs<T>& a, const int t) {
     }
 }
 
-template <typename T, bool CPML, bool B32>
-__global__ __launch_bounds__(256) void k_h_update(const YeeArgs<T> a) {
-    h_tile<T, CPML, B32>(a, tile_of_block(a.tile_order));
-}
-
-template <typename T, bool CPML, int MED, bool B32>
-__global__ __launch_bounds__(256) void k_e_update(const YeeArgs<T> a) {
-    e_tile<T, CPML, MED, B32>(a, tile_of_block(a.tile_order));
-}
-
 // E of the cells of dispersive classes (Drude / Drude-Lorentz components),
 // after the tile pass has updated every other cell: curl of H, the full
 // medium update and the CPML psi recursions of the cell, from E in
 // a.ex/a.ey/a.ez to the output arrays (the same arrays in the separate H/E
 // path, the other buffer after the fused step).  list holds the cells'
-// element offsets with the media of their three components.
-template <typename T, bool CPML>
+// indices (i, j, k) with the media of their three components.
+template <typename T, bool CPML, bool B32>
 __device__ __forceinline__ void e_dispersive_cell(const YeeArgs<T>& a, T* __restrict__ exo, T* __restrict__ eyo,
                                                   T* __restrict__ ezo, long e) {
-    // entry: element offset in bits 0-47, the media of the x, y, z
+    using O = Off<T, B32>;
+    using I = typename O::I;
+    // entry: i, j, k in bits 0-15, 16-31, 32-47; the media of the x, y, z
     // components in bits 48-51, 52-55, 56-59
-    const long o = e & ((1L << 48) - 1);
+    const int i = (int)(e & 0xffff), j = (int)((e >> 16) & 0xffff), k = (int)((e >> 32) & 0xffff);
     const int m0 = (int)((e >> 48) & 15), m1 = (int)((e >> 52) & 15), m2 = (int)((e >> 56) & 15);
-    // o = i*sx + j*sz + k + off with 0 <= k < sz and j*sz + k < sx
-    const long u = o - a.off;
-    const int i = (int)(u / a.sx);
-    const long r = u - (long)i * a.sx;
-    const int j = (int)(r / a.sz);
-    const int k = (int)(r - (long)j * a.sz);
     const long jk = (long)j * a.sz + k + a.off;
-    const long sx = a.sx;
-    const int sz = a.sz;
-    const T hzc = a.hz[o], hyc = a.hy[o], hxc = a.hx[o];
-    const T hzj = a.hz[o - sz], hxj = a.hx[o - sz];
-    const T hyk = a.hy[o - 1], hxk = a.hx[o - 1];
-    const T hzi = a.hz[o - sx], hyi = a.hy[o - sx];
-    T ex = a.ex[o], ey = a.ey[o], ez = a.ez[o];
+    const I o = O::of((long)i * a.sx + jk);
+    const I sx = O::of(a.sx), sz = O::of(a.sz), s1 = O::of(1);
+    const T hzc = O::ld(a.hz, o), hyc = O::ld(a.hy, o), hxc = O::ld(a.hx, o);
+    const T hzj = O::ld(a.hz, o - sz), hxj = O::ld(a.hx, o - sz);
+    const T hyk = O::ld(a.hy, o - s1), hxk = O::ld(a.hx, o - s1);
+    const T hzi = O::ld(a.hz, o - sx), hyi = O::ld(a.hy, o - sx);
+    T ex = O::ld(a.ex, o), ey = O::ld(a.ey, o), ez = O::ld(a.ez, o);
     const T dex = a.den_ex[i], cex = sizeof(T) == 4 ? a.cdex[i] : (T)0;
     const T dey = a.den_ey[j], dez = a.den_ez[k];
     const T cey = sizeof(T) == 4 ? a.cdey[j] : (T)0, cez = sizeof(T) == 4 ? a.cdez[k] : (T)0;
     const T dep = a.dt_eps0;
     T rx, ry, rz, vx, vy, vz;
     e_curl(dep, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dex, cex, dey, dez, cey, cez, rx, ry, rz, vx, vy, vz);
-    ex = medium_update(ex, rx, vx, m0, a.med, o, a.eold[0], a.pd[0], a.pl[0], a.plold[0], a.pstride);
-    ey = medium_update(ey, ry, vy, m1, a.med, o, a.eold[1], a.pd[1], a.pl[1], a.plold[1], a.pstride);
-    ez = medium_update(ez, rz, vz, m2, a.med, o, a.eold[2], a.pd[2], a.pl[2], a.plold[2], a.pstride);
+    ex = medium_update<T, B32>(ex, rx, vx, m0, a.med, o, a.eold[0], a.pd[0], a.pl[0], a.plold[0], a.pstride);
+    ey = medium_update<T, B32>(ey, ry, vy, m1, a.med, o, a.eold[1], a.pd[1], a.pl[1], a.plold[1], a.pstride);
+    ez = medium_update<T, B32>(ez, rz, vz, m2, a.med, o, a.eold[2], a.pd[2], a.pl[2], a.plold[2], a.pstride);
     if constexpr (CPML) {
         e_cpml_x(a, i, jk, dep, hzc, hzi, hyc, hyi, ey, ez);
         e_cpml_y(a, i, j, k, dep, hzc, hzj, hxc, hxj, ex, ez);
         e_cpml_z(a, i, j, k, dep, hyc, hyk, hxc, hxk, ex, ey);
     }
-    exo[o] = ex; eyo[o] = ey; ezo[o] = ez;
+    O::st(exo, o, ex); O::st(eyo, o, ey); O::st(ezo, o, ez);
 }
 
-template <typename T, bool CPML>
+template <typename T, bool CPML, bool B32>
 __global__ __launch_bounds__(256) void k_e_dispersive(const YeeArgs<T> a, T* __restrict__ exo, T* __restrict__ eyo,
                                                       T* __restrict__ ezo, const long* __restrict__ list, int n) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
-    e_dispersive_cell<T, CPML>(a, exo, eyo, ezo, list[p]);
+    e_dispersive_cell<T, CPML, B32>(a, exo, eyo, ezo, list[p]);
 }
 
-// The dispersive cells of every grid of a batch (in place, like the
-// separate kernels); grid_of[p] selects the argument block.
-template <typename T>
-__global__ __launch_bounds__(256) void k_e_dispersive_batched(const YeeArgs<T>* __restrict__ args,
-                                                              const long* __restrict__ list,
-                                                              const int* __restrict__ grid_of, int n) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const YeeArgs<T>& a = args[grid_of[p]];
-    if (a.cpml) e_dispersive_cell<T, true>(a, a.ex, a.ey, a.ez, list[p]);
-    else e_dispersive_cell<T, false>(a, a.ex, a.ey, a.ez, list[p]);
+template <typename T, bool CPML, bool B32>
+__global__ __launch_bounds__(256) void k_h_update(const YeeArgs<T> a) {
+    h_tile<T, CPML, B32>(a, tile_of_block(a.tile_order, blockIdx.x, gridDim.x));
+}
+
+// E tiles; with MED 1 the first db workgroups take the cells of dispersive
+// classes instead (one cell per thread, list as for k_e_dispersive): the
+// tiles leave those cells alone, so both parts of the launch are
+// independent, and the latency-bound per-cell work runs beside the
+// streaming tiles.
+template <typename T, bool CPML, int MED, bool B32>
+__global__ __launch_bounds__(256) void k_e_update(const YeeArgs<T> a, const long* __restrict__ disp, int n_disp,
+                                                  int db) {
+    const int b = blockIdx.x;
+    if (MED == 1 && b < db) {
+        const int p = (b * (int)blockDim.y + (int)threadIdx.y) * kLanes + (int)threadIdx.x;
+        if (p < n_disp) e_dispersive_cell<T, CPML, B32>(a, a.ex, a.ey, a.ez, disp[p]);
+        return;
+    }
+    e_tile<T, CPML, MED, B32>(a, tile_of_block(a.tile_order, b - db, (int)gridDim.x - db));
 }
 
 // ---------------------------------------------------------------------------
@@ -1268,7 +1270,7 @@ __device__ __forceinline__ int find_grid(const int* __restrict__ tile_start, int
 template <typename T, bool B32>
 __global__ __launch_bounds__(256) void k_h_update_batched(const YeeArgs<T>* __restrict__ args,
                                                           const int* __restrict__ tile_start, int n, int order) {
-    const int t = tile_of_block(order);
+    const int t = tile_of_block(order, blockIdx.x, gridDim.x);
     const int g = find_grid(tile_start, n, t);
     const YeeArgs<T>& a = args[g];
     if ((int)threadIdx.y >= a.rows) return;
@@ -1276,10 +1278,26 @@ __global__ __launch_bounds__(256) void k_h_update_batched(const YeeArgs<T>* __re
     else h_tile<T, false, B32>(a, t - tile_start[g]);
 }
 
+// Batched E tiles; the first db workgroups take the dispersive cells of
+// every grid (disp: the grids' lists one after the other, each padded with
+// -1 to whole workgroups; grid_of[b]: the grid of workgroup b).
 template <typename T, bool B32>
 __global__ __launch_bounds__(256) void k_e_update_batched(const YeeArgs<T>* __restrict__ args,
-                                                          const int* __restrict__ tile_start, int n, int order) {
-    const int t = tile_of_block(order);
+                                                          const int* __restrict__ tile_start, int n, int order,
+                                                          const long* __restrict__ disp,
+                                                          const int* __restrict__ grid_of, int n_disp, int db) {
+    const int b = blockIdx.x;
+    if (b < db) {
+        const int p = (b * (int)blockDim.y + (int)threadIdx.y) * kLanes + (int)threadIdx.x;
+        const long e = p < n_disp ? disp[p] : -1L;
+        if (e < 0) return;
+        // the CPML terms test each face's bit of a.cpml, so one instance
+        // serves grids with and without CPML
+        const YeeArgs<T>& a = args[grid_of[b]];
+        e_dispersive_cell<T, true, B32>(a, a.ex, a.ey, a.ez, e);
+        return;
+    }
+    const int t = tile_of_block(order, b - db, (int)gridDim.x - db);
     const int g = find_grid(tile_start, n, t);
     const YeeArgs<T>& a = args[g];
     if ((int)threadIdx.y >= a.rows) return;
