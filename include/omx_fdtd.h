@@ -131,9 +131,10 @@ int omx3d_destroy(omx3d_ctx* ctx);
  * marched per workgroup (>= 1; 0 keeps the current values) and the order in
  * which workgroups take tiles: 0 block order (consecutive workgroups,
  * spread over the XCDs, cover consecutive memory), 1 one contiguous slab of
- * tiles per XCD (-1 keeps it).  A batch uses the tile order of its first
- * grid.  Defaults: FP32 4 rows x 2 planes, FP64 1 row x 4 planes, tile
- * order 0.  No reference counterpart (engine tuning). */
+ * tiles per XCD, 2 block order by whole row blocks (the z tiles of a row
+ * block on one XCD) (-1 keeps it).  A batch uses the tile order of its first
+ * grid.  Defaults: FP32 4 rows x 4 planes, FP64 2 rows x 4 planes, tile
+ * order 2.  No reference counterpart (engine tuning). */
 int omx3d_set_tiling(omx3d_ctx* ctx, int32_t rows_y, int32_t xchunk, int32_t tile_order);
 int omx3d_get_tiling(omx3d_ctx* ctx, int32_t* rows_y, int32_t* xchunk, int32_t* tile_order);
 /* CPML profiles and den arrays (mxll_3D_mod.f90:282-420), device FP64.

@@ -826,6 +826,7 @@ struct omxb_ctx {
     bool periodic = false;
     int rows = 1;                   // workgroup height: the largest rows_y of the grids
     int tile_order = 0;             // tile order of the batched H/E kernels: the first grid's
+    int ntz = 1;                    // z tiles per row block of the first grid (tile order 2)
     bool b32 = true;                // every grid addressed by 32-bit byte offsets
     long* d_disp = nullptr;         // cells of dispersive classes of every grid (packed as per grid)
     int* d_disp_grid = nullptr;     // grid of each workgroup of dispersive cells
@@ -860,6 +861,7 @@ int batch_setup(omxb_ctx* b) {
     b->rows = 1;
     for (auto* c : b->g) b->rows = std::max(b->rows, c->rows_y);
     b->tile_order = b->g[0]->tile_order;
+    b->ntz = args[0].ntz;
     for (auto* c : b->g) b->b32 = b->b32 && offsets32(c);
     int rc = bupload(b->s, (YeeArgs<T>**)&b->d_args, args);
     if (!rc) rc = bupload(b->s, &b->d_tile_start, ts);
@@ -956,10 +958,10 @@ int batch_step_launches(omxb_ctx* b) {
         for (auto* c : b->g) { int rc = wrap_e<T>(c, b->s); if (rc) return rc; }
     if (b->b32)
         hipLaunchKernelGGL((k_h_update_batched<T, true>), dim3(b->total_tiles), blk, 0, b->s,
-                           (const YeeArgs<T>*)b->d_args, b->d_tile_start, b->n, b->tile_order);
+                           (const YeeArgs<T>*)b->d_args, b->d_tile_start, b->n, b->tile_order, b->ntz);
     else
         hipLaunchKernelGGL((k_h_update_batched<T, false>), dim3(b->total_tiles), blk, 0, b->s,
-                           (const YeeArgs<T>*)b->d_args, b->d_tile_start, b->n, b->tile_order);
+                           (const YeeArgs<T>*)b->d_args, b->d_tile_start, b->n, b->tile_order, b->ntz);
     for (size_t q = 0; q < b->pass_first.size(); ++q) {
         const int f0 = b->pass_first[q], nq = b->pass_count[q];
         hipLaunchKernelGGL(k_source_batched<T>, dim3(blocks_for(nq, 256)), dim3(256), 0, b->s,
@@ -972,11 +974,11 @@ int batch_step_launches(omxb_ctx* b) {
     const int db = b->disp_blocks;
     if (b->b32)
         hipLaunchKernelGGL((k_e_update_batched<T, true>), dim3(b->total_tiles + db), blk, 0, b->s,
-                           (const YeeArgs<T>*)b->d_args, b->d_tile_start, b->n, b->tile_order, b->d_disp,
+                           (const YeeArgs<T>*)b->d_args, b->d_tile_start, b->n, b->tile_order, b->ntz, b->d_disp,
                            b->d_disp_grid, b->n_disp, db);
     else
         hipLaunchKernelGGL((k_e_update_batched<T, false>), dim3(b->total_tiles + db), blk, 0, b->s,
-                           (const YeeArgs<T>*)b->d_args, b->d_tile_start, b->n, b->tile_order, b->d_disp,
+                           (const YeeArgs<T>*)b->d_args, b->d_tile_start, b->n, b->tile_order, b->ntz, b->d_disp,
                            b->d_disp_grid, b->n_disp, db);
     if (b->n_samp > 0)
         hipLaunchKernelGGL(k_sample_batched<T>, dim3(blocks_for(b->n_samp, 256)), dim3(256), 0, b->s,
@@ -1122,9 +1124,10 @@ int omx3d_create(const omx3d_desc* desc, omx3d_ctx** out) {
     c->prec = d.precision;
     c->esz = (size_t)d.precision;
     // default tile shapes, measured fastest on 256^3 CPML grids: FP32 4 rows
-    // x 2 planes, FP64 1 row x 4 planes (block order)
-    c->rows_y = d.precision == OMX_FP32 ? 4 : 1;
-    c->xchunk = d.precision == OMX_FP32 ? 2 : 4;
+    // x 4 planes, FP64 2 rows x 4 planes, whole row blocks per XCD
+    c->rows_y = d.precision == OMX_FP32 ? 4 : 2;
+    c->xchunk = 4;
+    c->tile_order = 2;
     c->cpml_mask = mask;
     Layout& L = c->L;
     const int A = 128 / (int)c->esz;
@@ -1234,7 +1237,7 @@ int omx3d_destroy(omx3d_ctx* c) {
 
 int omx3d_set_tiling(omx3d_ctx* c, int32_t rows_y, int32_t xchunk, int32_t tile_order) {
     CHECK_CTX(c);
-    if (tile_order < -1 || tile_order > 1) return fail(OMX_ERR_ARG, "tile_order must be -1, 0 or 1");
+    if (tile_order < -1 || tile_order > 2) return fail(OMX_ERR_ARG, "tile_order must be -1, 0, 1 or 2");
     if (rows_y != 0) {
         if (rows_y != 1 && rows_y != 2 && rows_y != 4) return fail(OMX_ERR_ARG, "rows_y must be 1, 2 or 4");
         c->rows_y = rows_y;
@@ -1780,7 +1783,7 @@ int omxb_destroy(omxb_ctx* b) {
 }
 
 // Times the tile shapes (rows_y in {1,2,4} x xchunk in {1,2,4,8,16} x tile
-// order 0/1) with `steps` H+E steps each and keeps the fastest.  The fields,
+// order 0/1/2) with `steps` H+E steps each and keeps the fastest.  The fields,
 // CPML and dispersive state are cleared afterwards, so call it before a run.
 int omx3d_autotune(omx3d_ctx* c, int32_t steps, int32_t* rows_y, int32_t* xchunk, int32_t* tile_order,
                    double* ms_per_step) {
@@ -1793,7 +1796,7 @@ int omx3d_autotune(omx3d_ctx* c, int32_t steps, int32_t* rows_y, int32_t* xchunk
     HIPCHK(hipEventCreate(&e1));
     double best = 1e30;
     int br = c->rows_y, bx = c->xchunk, bo = c->tile_order;
-    for (int order = 0; order < 2; ++order)
+    for (int order = 0; order < 3; ++order)
     for (int r : rows_c)
         for (int x : xc_c) {
             c->rows_y = r; c->xchunk = x; c->tile_order = order;

@@ -22,15 +22,26 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
     return x * per + (x < rem ? x : rem) + q;
 }
 
-// Tile of workgroup b of nb tile workgroups.  Order 0 keeps the block
-// order: consecutive workgroups, which the dispatcher deals round-robin to
-// the XCDs, cover consecutive memory, so the whole chip streams one region
-// of every array at a time (the neighbour rows another XCD read are found in
-// the memory-side cache).  Order 1 gives every XCD one contiguous slab of
-// tiles, which keeps the neighbour rows in its own L2 but streams eight
-// regions at once.
-__device__ __forceinline__ int tile_of_block(int order, int b, int nb) {
-    return order ? xcd_tile(b, nb) : b;
+// Tile of workgroup b of nb tile workgroups (tiles numbered z tile fastest,
+// then row block, then x chunk; ntz z tiles per row block).  The dispatcher
+// deals consecutive workgroups round-robin to the 8 XCDs.  Order 0 keeps
+// the block order: consecutive workgroups cover consecutive memory, so the
+// whole chip streams one region of every array at a time, and the halo
+// lines of the neighbouring z tile and row, read by another XCD, come from
+// the memory-side cache.  Order 1 gives every XCD one contiguous slab of
+// tiles, which keeps the halos in its own L2 but streams eight regions at
+// once.  Order 2 streams like order 0 but deals whole row blocks: the z
+// tiles of a row block run on one XCD, whose L2 then serves their shared
+// k-1 / k+1 halo lines.
+__device__ __forceinline__ int tile_of_block(int order, int b, int nb, int ntz) {
+    if (order == 1) return xcd_tile(b, nb);
+    if (order == 2) {
+        const int G = 8 * ntz;                 // a group: 8 row blocks of ntz z tiles
+        if (b >= nb / G * G) return b;         // the tail keeps the block order
+        const int g = b / G, r = b - g * G;
+        return ((g << 3) + (r & 7)) * ntz + (r >> 3);
+    }
+    return b;
 }
 
 // Field access by 32-bit byte offset from a uniform base pointer (callers
@@ -889,7 +900,7 @@ __global__ __launch_bounds__(256) void k_e_dispersive(const YeeArgs<T> a, T* __r
 
 template <typename T, bool CPML, bool B32>
 __global__ __launch_bounds__(256) void k_h_update(const YeeArgs<T> a) {
-    h_tile<T, CPML, B32>(a, tile_of_block(a.tile_order, blockIdx.x, gridDim.x));
+    h_tile<T, CPML, B32>(a, tile_of_block(a.tile_order, blockIdx.x, gridDim.x, a.ntz));
 }
 
 // E tiles; with MED 1 the first db workgroups take the cells of dispersive
@@ -906,7 +917,7 @@ __global__ __launch_bounds__(256) void k_e_update(const YeeArgs<T> a, const long
         if (p < n_disp) e_dispersive_cell<T, CPML, B32>(a, a.ex, a.ey, a.ez, disp[p]);
         return;
     }
-    e_tile<T, CPML, MED, B32>(a, tile_of_block(a.tile_order, b - db, (int)gridDim.x - db));
+    e_tile<T, CPML, MED, B32>(a, tile_of_block(a.tile_order, b - db, (int)gridDim.x - db, a.ntz));
 }
 
 // ---------------------------------------------------------------------------
@@ -1269,8 +1280,9 @@ __device__ __forceinline__ int find_grid(const int* __restrict__ tile_start, int
 
 template <typename T, bool B32>
 __global__ __launch_bounds__(256) void k_h_update_batched(const YeeArgs<T>* __restrict__ args,
-                                                          const int* __restrict__ tile_start, int n, int order) {
-    const int t = tile_of_block(order, blockIdx.x, gridDim.x);
+                                                          const int* __restrict__ tile_start, int n, int order,
+                                                          int ntz) {
+    const int t = tile_of_block(order, blockIdx.x, gridDim.x, ntz);
     const int g = find_grid(tile_start, n, t);
     const YeeArgs<T>& a = args[g];
     if ((int)threadIdx.y >= a.rows) return;
@@ -1284,7 +1296,7 @@ __global__ __launch_bounds__(256) void k_h_update_batched(const YeeArgs<T>* __re
 template <typename T, bool B32>
 __global__ __launch_bounds__(256) void k_e_update_batched(const YeeArgs<T>* __restrict__ args,
                                                           const int* __restrict__ tile_start, int n, int order,
-                                                          const long* __restrict__ disp,
+                                                          int ntz, const long* __restrict__ disp,
                                                           const int* __restrict__ grid_of, int n_disp, int db) {
     const int b = blockIdx.x;
     if (b < db) {
@@ -1297,7 +1309,7 @@ __global__ __launch_bounds__(256) void k_e_update_batched(const YeeArgs<T>* __re
         e_dispersive_cell<T, true, B32>(a, a.ex, a.ey, a.ez, e);
         return;
     }
-    const int t = tile_of_block(order, b - db, (int)gridDim.x - db);
+    const int t = tile_of_block(order, b - db, (int)gridDim.x - db, ntz);
     const int g = find_grid(tile_start, n, t);
     const YeeArgs<T>& a = args[g];
     if ((int)threadIdx.y >= a.rows) return;

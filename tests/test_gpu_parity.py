@@ -166,7 +166,7 @@ def test_tiling_variants_identical():
                       steps=40)
     ref = None
     # separate H/E kernels under several tile shapes
-    for rows, xc, order in ((4, 16, 0), (1, 1, 1), (2, 7, 0), (4, 64, 1), (1, 4, 0), (1, 4, 1)):
+    for rows, xc, order in ((4, 16, 0), (1, 1, 1), (2, 7, 2), (4, 64, 1), (1, 4, 0), (4, 2, 2)):
         dr, dt = case.dr_dt
         g = omx_fdtd.Grid3D(case.grid, case.npml, case.boundaries, dt, dr, precision=8)
         g.set_step_mode(1)
