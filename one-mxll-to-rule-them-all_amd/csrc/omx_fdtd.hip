@@ -132,26 +132,111 @@ int wrap_h(omx3d_ctx* c, hipStream_t st) {
     return OMX_OK;
 }
 
+TfsfArgs tfsf_args(omx3d_ctx* c) {
+    TfsfArgs t;
+    t.i0 = c->tf_lo[0]; t.j0 = c->tf_lo[1]; t.k0 = c->tf_lo[2];
+    t.i1 = c->tf_hi[0]; t.j1 = c->tf_hi[1]; t.k1 = c->tf_hi[2];
+    t.kb = c->tf_kb; t.n1 = c->tf_n; t.ks = c->tf_ks;
+    t.dt_mu0 = c->d.dt / kMu0; t.dt_eps0 = c->d.dt / kEps0; t.inv_dr = 1.0 / c->d.dr;
+    return t;
+}
+
+double tfsf_source(const omx3d_ctx* c, double t_e) {
+    const double x = (t_e - c->tf_t0) / c->tf_tau;
+    return c->tf_e0 * std::exp(-(x * x)) * std::cos(c->tf_w0 * (t_e - c->tf_t0));
+}
+
+// Whether the TF/SF E corrections can ride in the E tiles: no cell of a
+// dispersive class lies on the E correction faces (those cells are updated
+// by the dispersive workgroups of the E launch, so their corrections stay in
+// k_tfsf_e, after the launch).
+int tfsf_fold_check(omx3d_ctx* c) {
+    if (c->tf_fold_e >= 0) return OMX_OK;
+    c->tf_fold_e = 0;
+    if (!c->tfsf) return OMX_OK;
+    if (c->media_set && c->n_classes > 1 && c->n_disp > 0) {
+        int* d_flag = nullptr;
+        int rc = dalloc(c, (void**)&d_flag, sizeof(int));
+        if (rc) return rc;
+        HIPCHK(hipMemsetAsync(d_flag, 0, sizeof(int), c->s));
+        hipLaunchKernelGGL(k_disp_on_tfsf, dim3(blocks_for(c->n_disp, 256)), dim3(256), 0, c->s, c->d_disp,
+                           c->n_disp, tfsf_args(c), d_flag);
+        HIPCHK(hipGetLastError());
+        int flag = 0;
+        HIPCHK(hipMemcpyAsync(&flag, d_flag, sizeof(int), hipMemcpyDeviceToHost, c->s));
+        HIPCHK(hipStreamSynchronize(c->s));
+        dfreet(d_flag);
+        if (flag) return OMX_OK;
+    }
+    c->tf_fold_e = 1;
+    return OMX_OK;
+}
+
+// TF/SF part of a separate H or E launch (workgroups of 64 x rows_y threads):
+// the line's workgroups start at b0 (0 in the H launch).
+TfsfLaunch tfsf_launch(omx3d_ctx* c, bool on) {
+    TfsfLaunch L{};
+    L.on = on ? 1 : 0;
+    if (!on) return L;
+    L.t = tfsf_args(c);
+    L.ex1 = c->d_tf;
+    L.hy1 = c->d_tf + c->tf_n;
+    const int wg = kLanes * c->rows_y;
+    L.nb = (c->tf_n + wg - 1) / wg;
+    return L;
+}
+
 // Whether every array of the grid (fields, psi, tags, dispersive state) is
 // addressed by 32-bit byte offsets: the field arrays are the largest.
 bool offsets32(const omx3d_ctx* c) {
     return (unsigned long long)c->L.nalloc * c->esz < (1ull << 32);
 }
 
+// The H / E tile kernel instance for a grid: CPML, media, 32-bit offsets and
+// TF/SF are template switches, so that a grid without one of them carries
+// none of its code or registers.
 template <typename T>
-// ev0/ev1 (profiling, may be null) bracket the H kernel itself.
-int launch_h(omx3d_ctx* c, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
+using H_kernel = void (*)(const YeeArgs<T>, const TfsfLaunch);
+template <typename T>
+using E_kernel = void (*)(const YeeArgs<T>, const long*, int, int, const TfsfLaunch);
+
+template <typename T, bool CP, bool B32>
+H_kernel<T> pick_h_tf(bool tf) {
+    return tf ? k_h_update<T, CP, B32, true> : k_h_update<T, CP, B32, false>;
+}
+template <typename T>
+H_kernel<T> pick_h(bool cp, bool b32, bool tf) {
+    if (cp) return b32 ? pick_h_tf<T, true, true>(tf) : pick_h_tf<T, true, false>(tf);
+    return b32 ? pick_h_tf<T, false, true>(tf) : pick_h_tf<T, false, false>(tf);
+}
+template <typename T, bool CP, int MED, bool B32>
+E_kernel<T> pick_e_tf(bool tf) {
+    return tf ? k_e_update<T, CP, MED, B32, true> : k_e_update<T, CP, MED, B32, false>;
+}
+template <typename T, bool CP, int MED>
+E_kernel<T> pick_e_b32(bool b32, bool tf) {
+    return b32 ? pick_e_tf<T, CP, MED, true>(tf) : pick_e_tf<T, CP, MED, false>(tf);
+}
+template <typename T>
+E_kernel<T> pick_e(bool cp, bool md, bool b32, bool tf) {
+    if (cp) return md ? pick_e_b32<T, true, 1>(b32, tf) : pick_e_b32<T, true, 0>(b32, tf);
+    return md ? pick_e_b32<T, false, 1>(b32, tf) : pick_e_b32<T, false, 0>(b32, tf);
+}
+
+template <typename T>
+// tf: the TF/SF H corrections and the 1D line's H half step ride in the
+// launch.  ev0/ev1 (profiling, may be null) bracket the H kernel itself.
+int launch_h(omx3d_ctx* c, bool tf, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
     int rc = wrap_e<T>(c, c->s);
     if (rc) return rc;
     YeeArgs<T> a = make_args<T>(c);
     const dim3 blk(kLanes, c->rows_y);
-    const unsigned nb = yee_blocks(c, a);
+    const TfsfLaunch L = tfsf_launch(c, tf);
+    const unsigned nb = yee_blocks(c, a) + (L.on ? L.nb : 0);
     const bool cp = c->cpml_mask != 0, b32 = offsets32(c);
     if (ev0) HIPCHK(hipEventRecord(ev0, c->s));
-    if (cp && b32) hipLaunchKernelGGL((k_h_update<T, true, true>), dim3(nb), blk, 0, c->s, a);
-    else if (cp) hipLaunchKernelGGL((k_h_update<T, true, false>), dim3(nb), blk, 0, c->s, a);
-    else if (b32) hipLaunchKernelGGL((k_h_update<T, false, true>), dim3(nb), blk, 0, c->s, a);
-    else hipLaunchKernelGGL((k_h_update<T, false, false>), dim3(nb), blk, 0, c->s, a);
+    const H_kernel<T> k = pick_h<T>(cp, b32, L.on != 0);
+    hipLaunchKernelGGL(k, dim3(nb), blk, 0, c->s, a, L);
     if (ev1) HIPCHK(hipEventRecord(ev1, c->s));
     HIPCHK(hipGetLastError());
     return OMX_OK;
@@ -194,31 +279,27 @@ int launch_e_dispersive(omx3d_ctx* c, const YeeArgs<T>& a, T* exo, T* eyo, T* ez
     return OMX_OK;
 }
 
-// The E tiles and, in the same launch, the cells of dispersive classes.
-// ev0/ev1 (profiling, may be null) bracket that kernel.
+// The E tiles and, in the same launch, the cells of dispersive classes; tf:
+// the TF/SF E corrections and the 1D line's E half step (source at time
+// t_tf) ride in the launch too.  ev0/ev1 (profiling, may be null) bracket
+// that kernel.
 template <typename T>
-int launch_e(omx3d_ctx* c, double time, double t_skip, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
+int launch_e(omx3d_ctx* c, double time, double t_skip, bool tf, double t_tf, hipEvent_t ev0 = nullptr,
+             hipEvent_t ev1 = nullptr) {
     int rc = wrap_h<T>(c, c->s);
     if (rc) return rc;
     YeeArgs<T> a = make_args<T>(c);
     const dim3 blk(kLanes, c->rows_y);
     const bool cp = c->cpml_mask != 0, md = c->media_set && c->n_classes > 1;
     const int nd = md ? c->n_disp : 0;
-    const int db = (nd + kLanes * c->rows_y - 1) / (kLanes * c->rows_y);
-    const dim3 grid(yee_blocks(c, a) + db);
-    const long* dl = c->d_disp;
+    const int db = md ? (nd + kLanes * c->rows_y - 1) / (kLanes * c->rows_y) : 0;
+    TfsfLaunch L = tfsf_launch(c, tf);
+    L.b0 = db;
+    if (tf) L.src = tfsf_source(c, t_tf);
+    const dim3 grid(yee_blocks(c, a) + db + (L.on ? L.nb : 0));
     if (ev0) HIPCHK(hipEventRecord(ev0, c->s));
-    if (offsets32(c)) {
-        if (cp && md) hipLaunchKernelGGL((k_e_update<T, true, 1, true>), grid, blk, 0, c->s, a, dl, nd, db);
-        else if (cp) hipLaunchKernelGGL((k_e_update<T, true, 0, true>), grid, blk, 0, c->s, a, dl, 0, 0);
-        else if (md) hipLaunchKernelGGL((k_e_update<T, false, 1, true>), grid, blk, 0, c->s, a, dl, nd, db);
-        else hipLaunchKernelGGL((k_e_update<T, false, 0, true>), grid, blk, 0, c->s, a, dl, 0, 0);
-    } else {
-        if (cp && md) hipLaunchKernelGGL((k_e_update<T, true, 1, false>), grid, blk, 0, c->s, a, dl, nd, db);
-        else if (cp) hipLaunchKernelGGL((k_e_update<T, true, 0, false>), grid, blk, 0, c->s, a, dl, 0, 0);
-        else if (md) hipLaunchKernelGGL((k_e_update<T, false, 1, false>), grid, blk, 0, c->s, a, dl, nd, db);
-        else hipLaunchKernelGGL((k_e_update<T, false, 0, false>), grid, blk, 0, c->s, a, dl, 0, 0);
-    }
+    const E_kernel<T> k = pick_e<T>(cp, md, offsets32(c), L.on != 0);
+    hipLaunchKernelGGL(k, grid, blk, 0, c->s, a, (const long*)c->d_disp, nd, db, L);
     if (ev1) HIPCHK(hipEventRecord(ev1, c->s));
     HIPCHK(hipGetLastError());
     return launch_j_correct<T>(c, time - t_skip);
@@ -244,35 +325,13 @@ int launch_sources(omx3d_ctx* c, double time) {
     return OMX_OK;
 }
 
-TfsfArgs tfsf_args(omx3d_ctx* c) {
-    TfsfArgs t;
-    t.i0 = c->tf_lo[0]; t.j0 = c->tf_lo[1]; t.k0 = c->tf_lo[2];
-    t.i1 = c->tf_hi[0]; t.j1 = c->tf_hi[1]; t.k1 = c->tf_hi[2];
-    t.kb = c->tf_kb; t.n1 = c->tf_n; t.ks = c->tf_ks;
-    t.dt_mu0 = c->d.dt / kMu0; t.dt_eps0 = c->d.dt / kEps0; t.inv_dr = 1.0 / c->d.dr;
-    return t;
-}
-
-template <typename T>
-int launch_tfsf_h(omx3d_ctx* c) {
-    if (!c->tfsf) return OMX_OK;
-    TfsfArgs t = tfsf_args(c);
-    const long nxi = t.i1 - t.i0, nyj = t.j1 - t.j0 + 1, nzk = t.k1 - t.k0 + 1;
-    const long n = 2 * nxi * nyj + 2 * nxi * nzk + t.n1;
-    hipLaunchKernelGGL(k_tfsf_h<T>, dim3(blocks_for(n, 256)), dim3(256), 0, c->s, make_args<T>(c), t,
-                       c->d_tf, c->d_tf + c->tf_n);
-    HIPCHK(hipGetLastError());
-    return OMX_OK;
-}
-
 template <typename T>
 int launch_tfsf_e(omx3d_ctx* c, double t_e) {
     if (!c->tfsf) return OMX_OK;
     TfsfArgs t = tfsf_args(c);
     const long nxi = t.i1 - t.i0, nyj = t.j1 - t.j0 + 1, nzk = t.k1 - t.k0;
     const long n = 2 * nxi * nyj + 2 * nyj * nzk + t.n1;
-    const double x = (t_e - c->tf_t0) / c->tf_tau;
-    const double src = c->tf_e0 * std::exp(-(x * x)) * std::cos(c->tf_w0 * (t_e - c->tf_t0));
+    const double src = tfsf_source(c, t_e);
     hipLaunchKernelGGL(k_tfsf_e<T>, dim3(blocks_for(n, 256)), dim3(256), 0, c->s, make_args<T>(c), t,
                        c->d_tf, c->d_tf + c->tf_n, src);
     HIPCHK(hipGetLastError());
@@ -507,6 +566,12 @@ int run_impl(omx3d_ctx* c, int tt_first, int n_steps, int record_every) {
     if (rc) return rc;
     const bool fused = fused_applies(c);
     c->last_fused = fused;
+    // TF/SF inside the separate H and E launches (the E side only when no
+    // cell of a dispersive class lies on the E correction faces: the tiles
+    // leave such cells to the dispersive workgroups)
+    rc = tfsf_fold_check(c);
+    if (rc) return rc;
+    const bool fold = !fused && c->tfsf, fold_e = fold && c->tf_fold_e == 1;
     for (int tt = tt_first; tt < tt_first + n_steps; ++tt) {
         const double time = dt * ((double)tt - 0.5);
         // n_skip_steps = 1: a new quantum step every step, so t_skip = time
@@ -530,17 +595,17 @@ int run_impl(omx3d_ctx* c, int tt_first, int n_steps, int record_every) {
             rc = launch_j_correct<T>(c, time - t_skip);
             if (rc) return rc;
         } else {
-            rc = launch_h<T>(c, ev ? ev[0] : nullptr, ev ? ev[1] : nullptr);
-            if (rc) return rc;
-            rc = launch_tfsf_h<T>(c);
+            rc = launch_h<T>(c, fold, ev ? ev[0] : nullptr, ev ? ev[1] : nullptr);
             if (rc) return rc;
             rc = launch_sources<T>(c, time);
             if (rc) return rc;
-            rc = launch_e<T>(c, time, t_skip, ev ? ev[2] : nullptr, ev ? ev[3] : nullptr);
+            rc = launch_e<T>(c, time, t_skip, fold_e, dt * (double)tt, ev ? ev[2] : nullptr, ev ? ev[3] : nullptr);
             if (rc) return rc;
         }
-        rc = launch_tfsf_e<T>(c, dt * (double)tt);
-        if (rc) return rc;
+        if (!fold_e) {
+            rc = launch_tfsf_e<T>(c, dt * (double)tt);
+            if (rc) return rc;
+        }
         rc = launch_two_level<T>(c);
         if (rc) return rc;
         if (record_every > 0 && tt % record_every == 0 && c->n_samples > 0) {
@@ -791,6 +856,7 @@ int media_finish(omx3d_ctx* c) {
     HIPCHK(hipStreamSynchronize(c->s));
     c->media_set = true;
     c->src_in_dispersive = -1;
+    c->tf_fold_e = -1;
     return OMX_OK;
 }
 
@@ -1350,12 +1416,12 @@ int omx3d_get_medium_coeffs(omx3d_ctx* c, int32_t m, omx_medium_coeffs* out) {
 
 int omx3d_step_h(omx3d_ctx* c) {
     CHECK_CTX(c);
-    return c->prec == 4 ? launch_h<float>(c) : launch_h<double>(c);
+    return c->prec == 4 ? launch_h<float>(c, false) : launch_h<double>(c, false);
 }
 
 int omx3d_step_e(omx3d_ctx* c, double time, double t_skip) {
     CHECK_CTX(c);
-    return c->prec == 4 ? launch_e<float>(c, time, t_skip) : launch_e<double>(c, time, t_skip);
+    return c->prec == 4 ? launch_e<float>(c, time, t_skip, false, 0.0) : launch_e<double>(c, time, t_skip, false, 0.0);
 }
 
 int omx3d_add_source(omx3d_ctx* c, int32_t dir, double j_amp, double w0, double t0, double tau, double t_init,
@@ -1680,6 +1746,7 @@ int omx3d_set_tfsf(omx3d_ctx* c, const int32_t lo[3], const int32_t hi[3], doubl
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(c->s));
     c->tfsf = true;
+    c->tf_fold_e = -1;
     return OMX_OK;
 }
 
@@ -1801,14 +1868,16 @@ int omx3d_autotune(omx3d_ctx* c, int32_t steps, int32_t* rows_y, int32_t* xchunk
         for (int x : xc_c) {
             c->rows_y = r; c->xchunk = x; c->tile_order = order;
             for (int w = 0; w < 2; ++w) {   // warm-up launch pair
-                int rc = c->prec == 4 ? launch_h<float>(c) : launch_h<double>(c);
-                if (!rc) rc = c->prec == 4 ? launch_e<float>(c, 0.0, 0.0) : launch_e<double>(c, 0.0, 0.0);
+                int rc = c->prec == 4 ? launch_h<float>(c, false) : launch_h<double>(c, false);
+                if (!rc)
+                    rc = c->prec == 4 ? launch_e<float>(c, 0.0, 0.0, false, 0.0) : launch_e<double>(c, 0.0, 0.0, false, 0.0);
                 if (rc) return rc;
             }
             HIPCHK(hipEventRecord(e0, c->s));
             for (int t = 0; t < steps; ++t) {
-                int rc = c->prec == 4 ? launch_h<float>(c) : launch_h<double>(c);
-                if (!rc) rc = c->prec == 4 ? launch_e<float>(c, 0.0, 0.0) : launch_e<double>(c, 0.0, 0.0);
+                int rc = c->prec == 4 ? launch_h<float>(c, false) : launch_h<double>(c, false);
+                if (!rc)
+                    rc = c->prec == 4 ? launch_e<float>(c, 0.0, 0.0, false, 0.0) : launch_e<double>(c, 0.0, 0.0, false, 0.0);
                 if (rc) return rc;
             }
             HIPCHK(hipEventRecord(e1, c->s));

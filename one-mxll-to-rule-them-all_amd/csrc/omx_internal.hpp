@@ -187,6 +187,7 @@ struct omx3d_ctx {
     int tf_kb = 0, tf_n = 0, tf_ks = 0;
     double tf_e0, tf_w0, tf_t0, tf_tau;
     double* d_tf = nullptr;         // ex1 (n) hy1 (n)
+    int tf_fold_e = -1;             // TF/SF E corrections inside the E tiles: -1 unknown, 0 no, 1 yes
 
     // energy scratch
     double* d_red = nullptr;

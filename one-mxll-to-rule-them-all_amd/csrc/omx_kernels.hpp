@@ -471,6 +471,32 @@ __device__ __forceinline__ Face face_e(bool lo_on, bool hi_on, int n, int na, in
     return {-1, 0};
 }
 
+// TF/SF plane wave, x-polarised, travelling along +z.  The incident Ex (at
+// z = k) and Hy (at z = k + 1/2) come from a 1D Yee line with the 3D grid's
+// dt and dr, so the incident wave has the 3D grid's numerical dispersion
+// along z.  Total-field region: Ex i in [i0,i1-1], j in [j0,j1], k in [k0,k1];
+// Hy i in [i0,i1-1], j in [j0,j1], k in [k0,k1-1]; Hz i in [i0,i1-1],
+// j in [j0,j1-1], k in [k0,k1]; Ez i in [i0,i1], j in [j0,j1], k in [k0,k1-1].
+struct TfsfArgs {
+    int i0, i1, j0, j1, k0, k1;
+    int kb, n1, ks;     // 1D line covers k = kb .. kb+n1-1; source node ks
+    double dt_mu0, dt_eps0, inv_dr;
+};
+
+// TF/SF inside the separate H / E launches: the tile that updates a cell
+// adds the cell's correction before storing it, and workgroups [b0, b0 + nb)
+// of the launch advance the 1D line (H launch: hy1 from ex1; E launch: ex1
+// from hy1 plus the source).  The tiles read only the half of the line the
+// launch does not write.
+struct TfsfLaunch {
+    TfsfArgs t;
+    double* ex1;
+    double* hy1;
+    double src;         // E launch: value added at node ks this step
+    int on;
+    int b0, nb;
+};
+
 // Whether index range [a0, a1] meets [f0, f1].
 __device__ __forceinline__ bool meets(int a0, int a1, int f0, int f1) { return a0 <= f1 && a1 >= f0; }
 // Whether [a0, a1] holds a cell of an H face (1..npml-1, n+1-npml..n-1) or
@@ -482,6 +508,72 @@ __device__ __forceinline__ bool in_faces_e(bool lo_on, bool hi_on, int a0, int a
     return (lo_on && meets(a0, a1, 2, npml)) || (hi_on && meets(a0, a1, n + 1 - npml, n - 1));
 }
 
+// TF/SF corrections of the H cells of a wavefront (row j, lanes k of z tile
+// [kt0, kt1], planes i0..i1): Hy on the z faces k = k0-1 (+ dt/mu0
+// den_hz(k) Ex_inc(k0)) and k = k1 (- ... Ex_inc(k1)), j in [j0, j1]; Hz on
+// the y faces j = j0-1 (- dt/mu0 den_hy(j) Ex_inc(k)) and j = j1 (+), k in
+// [k0, k1]; both for the planes i in [ia, ib] = [i0, i1-1] of the box.  The
+// tile adds them before storing: H = (T)((double)H + c) with the lane's cy
+// / cz (0 for lanes without a correction).  Returns false (uniform) when the
+// wavefront has none.
+template <typename T>
+__device__ __forceinline__ bool tfsf_h_lane(const YeeArgs<T>& a, const TfsfLaunch& L, int i0, int i1, int j, int k,
+                                            int kt0, int kt1, double& cy, double& cz, int& ia, int& ib) {
+    const TfsfArgs& t = L.t;
+    cy = 0.0;
+    cz = 0.0;
+    ia = max(i0, t.i0);
+    ib = min(i1, t.i1 - 1);
+    if (ia > ib) return false;
+    const bool yrow = j >= t.j0 && j <= t.j1 && (meets(kt0, kt1, t.k0 - 1, t.k0 - 1) || meets(kt0, kt1, t.k1, t.k1));
+    const bool zrow = (j == t.j0 - 1 || j == t.j1) && meets(kt0, kt1, t.k0, t.k1);
+    if (!yrow && !zrow) return false;
+    if (yrow && (k == t.k0 - 1 || k == t.k1)) {
+        const bool hi = k == t.k1;
+        const double c = t.dt_mu0 * (double)a.den_hz[k] * L.ex1[(hi ? t.k1 : t.k0) - t.kb];
+        cy = hi ? -c : c;
+    }
+    if (zrow && k >= t.k0 && k <= t.k1) {
+        const double c = t.dt_mu0 * (double)a.den_hy[j] * L.ex1[k - t.kb];
+        cz = j == t.j1 ? c : -c;
+    }
+    return true;
+}
+
+// TF/SF corrections of the E cells of a wavefront (row j in [j0, j1], lanes
+// k of z tile [kt0, kt1], planes i0..i1): Ex on the z faces k = k0 (+ dt/eps0
+// den_ez(k) Hy_inc(k0-1)) and k = k1 (- ... Hy_inc(k1)) for the planes i in
+// [xa, xb] = [i0, i1-1] of the box; Ez on the x faces i = i0 (cl = - dt/eps0
+// den_ex(i0) Hy_inc(k)) and i = i1 (ch = + ...), k in [k0, k1-1].  The tile
+// adds them before storing, as for H.  Returns false (uniform) when the
+// wavefront has none.
+template <typename T>
+__device__ __forceinline__ bool tfsf_e_lane(const YeeArgs<T>& a, const TfsfLaunch& L, int i0, int i1, int j, int k,
+                                            int kt0, int kt1, double& cx, double& cl, double& ch, int& xa, int& xb) {
+    const TfsfArgs& t = L.t;
+    cx = 0.0;
+    cl = 0.0;
+    ch = 0.0;
+    xa = max(i0, t.i0);
+    xb = min(i1, t.i1 - 1);
+    if (j < t.j0 || j > t.j1) return false;
+    const bool xrow = xa <= xb && (meets(kt0, kt1, t.k0, t.k0) || meets(kt0, kt1, t.k1, t.k1));
+    const bool zrow = (meets(i0, i1, t.i0, t.i0) || meets(i0, i1, t.i1, t.i1)) && meets(kt0, kt1, t.k0, t.k1 - 1);
+    if (!xrow && !zrow) return false;
+    if (xrow && (k == t.k0 || k == t.k1)) {
+        const bool hi = k == t.k1;
+        const double c = t.dt_eps0 * (double)a.den_ez[k] * (hi ? L.hy1[t.k1 - t.kb] : L.hy1[t.k0 - 1 - t.kb]);
+        cx = hi ? -c : c;
+    }
+    if (!xrow) { xa = 1; xb = 0; }
+    if (zrow && k >= t.k0 && k <= t.k1 - 1) {
+        const double inc = L.hy1[k - t.kb];
+        cl = -(t.dt_eps0 * (double)a.den_ex[t.i0] * inc);
+        ch = t.dt_eps0 * (double)a.den_ex[t.i1] * inc;
+    }
+    return true;
+}
+
 // A tile: 64 lanes along z (one wavefront), one row along y per wavefront,
 // marching xchunk planes along x.  A wavefront whose cells (its row, its z
 // tile, the planes of its chunk) meet no CPML face runs the bulk loop.  In
@@ -490,8 +582,8 @@ __device__ __forceinline__ bool in_faces_e(bool lo_on, bool hi_on, int a0, int a
 // cell lies in are loaded together with its fields, ahead of the
 // arithmetic.  A cell lies in at most one face per axis: grids are at least
 // 2 npml cells along every CPML axis (omx3d_create rejects thinner ones).
-template <typename T, bool CPML, bool B32>
-__device__ __forceinline__ void h_tile(const YeeArgs<T>& a, const int t) {
+template <typename T, bool CPML, bool B32, bool TF>
+__device__ __forceinline__ void h_tile(const YeeArgs<T>& a, const TfsfLaunch& tf, const int t) {
     using O = Off<T, B32>;
     using I = typename O::I;
     const int tz = t % a.ntz;
@@ -510,10 +602,14 @@ __device__ __forceinline__ void h_tile(const YeeArgs<T>& a, const int t) {
     const T chy = sizeof(T) == 4 ? a.cdhy[j] : (T)0, chz = sizeof(T) == 4 ? a.cdhz[k] : (T)0;
     const T dmu = a.dt_mu0;
     T ezc = O::ld(a.ez, o), eyc = O::ld(a.ey, o);
+    const int kt0 = tz * kLanes + 1, kt1 = min(kt0 + kLanes - 1, a.nz);
+    double cy = 0.0, cz = 0.0;
+    int ia = 0, ib = -1;
+    const bool tfw = TF && tfsf_h_lane(a, tf, i0, i1, j, k, kt0, kt1, cy, cz, ia, ib);
     bool faces = false;
     if (CPML) {
         const unsigned cp = a.cpml;
-        const int k0 = tz * kLanes + 1, k1 = min(k0 + kLanes - 1, a.nz);
+        const int k0 = kt0, k1 = kt1;
         faces = in_faces_h(cp & 1u, cp & 2u, i0, i1, a.nx, a.npml) || in_faces_h(cp & 4u, cp & 8u, j, j, a.ny, a.npml) ||
                 in_faces_h(cp & 16u, cp & 32u, k0, k1, a.nz, a.npml);
     }
@@ -525,6 +621,7 @@ __device__ __forceinline__ void h_tile(const YeeArgs<T>& a, const int t) {
             T hx = O::ld(a.hx, o), hy = O::ld(a.hy, o), hz = O::ld(a.hz, o);
             const T cx = sizeof(T) == 4 ? a.cdhx[i] : a.den_hx[i];
             h_bulk(dmu, cx, dhy, dhz, chy, chz, ezc, ezn, ezj, eyc, eyn, eyk, exc, exk, exj, hx, hy, hz);
+            if (tfw && i >= ia && i <= ib) { hy = (T)((double)hy + cy); hz = (T)((double)hz + cz); }
             O::st(a.hx, o, hx); O::st(a.hy, o, hy); O::st(a.hz, o, hz);
             ezc = ezn; eyc = eyn;
         }
@@ -571,6 +668,7 @@ __device__ __forceinline__ void h_tile(const YeeArgs<T>& a, const int t) {
             O::st(a.ph[4], pz, p0); O::st(a.ph[5], pz, p1);
             hx = hx + dmu * p0; hy = hy + dmu * p1;
         }
+        if (tfw && i >= ia && i <= ib) { hy = (T)((double)hy + cy); hz = (T)((double)hz + cz); }
         O::st(a.hx, o, hx); O::st(a.hy, o, hy); O::st(a.hz, o, hz);
         ezc = ezn; eyc = eyn;
     }
@@ -754,8 +852,8 @@ __device__ __forceinline__ void e_cpml_z(const YeeArgs<T>& a, int i, int j, int 
 // media; 1: vacuum and dielectric components, the cells of dispersive
 // classes (tag >= disp_tag0) left entirely (E and psi of E) to
 // k_e_dispersive.
-template <typename T, bool CPML, int MED, bool B32>
-__device__ __forceinline__ void e_tile(const YeeArgs<T>& a, const int t) {
+template <typename T, bool CPML, int MED, bool B32, bool TF>
+__device__ __forceinline__ void e_tile(const YeeArgs<T>& a, const TfsfLaunch& tf, const int t) {
     static_assert(MED == 0 || MED == 1, "the tiles take vacuum and dielectric cells");
     using O = Off<T, B32>;
     using I = typename O::I;
@@ -776,10 +874,15 @@ __device__ __forceinline__ void e_tile(const YeeArgs<T>& a, const int t) {
     const T dep = a.dt_eps0;
     const bool tags = MED != 0 && a.tag != nullptr;
     T hzi = O::ld(a.hz, o - sx), hyi = O::ld(a.hy, o - sx);
+    const int kt0 = tz * kLanes + 1, kt1 = min(kt0 + kLanes - 1, a.nz);
+    double tcx = 0.0, tcl = 0.0, tch = 0.0;
+    int txa = 1, txb = 0;
+    const bool tfw = TF && tfsf_e_lane(a, tf, i0, i1, j, k, kt0, kt1, tcx, tcl, tch, txa, txb);
+    const int ti0 = TF ? tf.t.i0 : 0, ti1 = TF ? tf.t.i1 : 0;
     bool faces = false;
     if (CPML) {
         const unsigned cp = a.cpml;
-        const int k0 = tz * kLanes + 1, k1 = min(k0 + kLanes - 1, a.nz);
+        const int k0 = kt0, k1 = kt1;
         faces = in_faces_e(cp & 1u, cp & 2u, i0, i1, a.nx, a.npml) || in_faces_e(cp & 4u, cp & 8u, j, j, a.ny, a.npml) ||
                 in_faces_e(cp & 16u, cp & 32u, k0, k1, a.nz, a.npml);
     }
@@ -794,6 +897,11 @@ __device__ __forceinline__ void e_tile(const YeeArgs<T>& a, const int t) {
             if (MED != 1 || tg < a.disp_tag0) {
                 e_bulk<T, MED>(a, tg, dep, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dex, cex, dey, dez, cey,
                                cez, ex, ey, ez);
+                if (tfw) {
+                    if (i >= txa && i <= txb) ex = (T)((double)ex + tcx);
+                    if (i == ti0) ez = (T)((double)ez + tcl);
+                    if (i == ti1) ez = (T)((double)ez + tch);
+                }
                 O::st(a.ex, o, ex); O::st(a.ey, o, ey); O::st(a.ez, o, ez);
             }
             hzi = hzc; hyi = hyc;
@@ -844,6 +952,11 @@ __device__ __forceinline__ void e_tile(const YeeArgs<T>& a, const int t) {
             const T p0 = psi_step(pz0, zb, zc, hyk - hyc), p1 = psi_step(pz1, zb, zc, hxc - hxk);
             O::st(a.pe[4], pz, p0); O::st(a.pe[5], pz, p1);
             ex = ex + dep * p0; ey = ey + dep * p1;
+        }
+        if (tfw) {
+            if (i >= txa && i <= txb) ex = (T)((double)ex + tcx);
+            if (i == ti0) ez = (T)((double)ez + tcl);
+            if (i == ti1) ez = (T)((double)ez + tch);
         }
         O::st(a.ex, o, ex); O::st(a.ey, o, ey); O::st(a.ez, o, ez);
         hzi = hzc; hyi = hyc;
@@ -898,9 +1011,19 @@ __global__ __launch_bounds__(256) void k_e_dispersive(const YeeArgs<T> a, T* __r
     e_dispersive_cell<T, CPML, B32>(a, exo, eyo, ezo, list[p]);
 }
 
-template <typename T, bool CPML, bool B32>
-__global__ __launch_bounds__(256) void k_h_update(const YeeArgs<T> a) {
-    h_tile<T, CPML, B32>(a, tile_of_block(a.tile_order, blockIdx.x, gridDim.x, a.ntz));
+// H tiles; with TF the tiles apply the TF/SF H corrections and the first
+// tf.nb workgroups advance the 1D TF/SF line (hy1 from ex1) instead.
+template <typename T, bool CPML, bool B32, bool TF>
+__global__ __launch_bounds__(256) void k_h_update(const YeeArgs<T> a, const TfsfLaunch tf) {
+    const int b = blockIdx.x;
+    const int ns = TF ? tf.nb : 0;
+    if (b < ns) {
+        const int m = b * (int)(blockDim.x * blockDim.y) + (int)(threadIdx.y * blockDim.x + threadIdx.x);
+        const TfsfArgs& t = tf.t;
+        if (m < t.n1 - 1) tf.hy1[m] = tf.hy1[m] + t.dt_mu0 * t.inv_dr * (tf.ex1[m] - tf.ex1[m + 1]);
+        return;
+    }
+    h_tile<T, CPML, B32, TF>(a, tf, tile_of_block(a.tile_order, b - ns, (int)gridDim.x - ns, a.ntz));
 }
 
 // E tiles; with MED 1 the first db workgroups take the cells of dispersive
@@ -908,16 +1031,30 @@ __global__ __launch_bounds__(256) void k_h_update(const YeeArgs<T> a) {
 // tiles leave those cells alone, so both parts of the launch are
 // independent, and the latency-bound per-cell work runs beside the
 // streaming tiles.
-template <typename T, bool CPML, int MED, bool B32>
+// With TF the tiles add the TF/SF E corrections and workgroups
+// [db, db + tf.nb) advance the 1D TF/SF line's E (ex1 from hy1, plus the
+// source at node ks); the tiles read only hy1.
+template <typename T, bool CPML, int MED, bool B32, bool TF>
 __global__ __launch_bounds__(256) void k_e_update(const YeeArgs<T> a, const long* __restrict__ disp, int n_disp,
-                                                  int db) {
+                                                  int db, const TfsfLaunch tf) {
     const int b = blockIdx.x;
     if (MED == 1 && b < db) {
         const int p = (b * (int)blockDim.y + (int)threadIdx.y) * kLanes + (int)threadIdx.x;
         if (p < n_disp) e_dispersive_cell<T, CPML, B32>(a, a.ex, a.ey, a.ez, disp[p]);
         return;
     }
-    e_tile<T, CPML, MED, B32>(a, tile_of_block(a.tile_order, b - db, (int)gridDim.x - db, a.ntz));
+    const int ns = db + (TF ? tf.nb : 0);
+    if (b < ns) {
+        const int m = (b - db) * (int)(blockDim.x * blockDim.y) + (int)(threadIdx.y * blockDim.x + threadIdx.x);
+        const TfsfArgs& t = tf.t;
+        if (m >= 1 && m < t.n1 - 1) {
+            double v = tf.ex1[m] + t.dt_eps0 * t.inv_dr * (tf.hy1[m - 1] - tf.hy1[m]);
+            if (m == t.ks - t.kb) v += tf.src;
+            tf.ex1[m] = v;
+        }
+        return;
+    }
+    e_tile<T, CPML, MED, B32, TF>(a, tf, tile_of_block(a.tile_order, b - ns, (int)gridDim.x - ns, a.ntz));
 }
 
 // ---------------------------------------------------------------------------
@@ -1177,7 +1314,7 @@ __device__ __forceinline__ void fused_tile_lanes(const FusedArgs<T>& f, const Fu
     c.tf_y = c.tf_z = false;
     c.tfy = c.tfz = 0.0;
     if constexpr (TFSF) {
-        // the corrections of k_tfsf_h: Hy -/+ dt/mu0 den_hz(k) Ex_inc on the
+        // the TF/SF H corrections: Hy -/+ dt/mu0 den_hz(k) Ex_inc on the
         // z faces k = tk0-1 / tk1 (tj0 <= j <= tj1), Hz -/+ dt/mu0 den_hy(j)
         // Ex_inc(k) on the y faces j = tj0-1 / tj1 (tk0 <= k <= tk1)
         const FusedCold<T>& q = *cold;
@@ -1286,8 +1423,9 @@ __global__ __launch_bounds__(256) void k_h_update_batched(const YeeArgs<T>* __re
     const int g = find_grid(tile_start, n, t);
     const YeeArgs<T>& a = args[g];
     if ((int)threadIdx.y >= a.rows) return;
-    if (a.cpml) h_tile<T, true, B32>(a, t - tile_start[g]);
-    else h_tile<T, false, B32>(a, t - tile_start[g]);
+    const TfsfLaunch none{};
+    if (a.cpml) h_tile<T, true, B32, false>(a, none, t - tile_start[g]);
+    else h_tile<T, false, B32, false>(a, none, t - tile_start[g]);
 }
 
 // Batched E tiles; the first db workgroups take the dispersive cells of
@@ -1313,8 +1451,9 @@ __global__ __launch_bounds__(256) void k_e_update_batched(const YeeArgs<T>* __re
     const int g = find_grid(tile_start, n, t);
     const YeeArgs<T>& a = args[g];
     if ((int)threadIdx.y >= a.rows) return;
-    if (a.cpml) e_tile<T, true, 1, B32>(a, t - tile_start[g]);
-    else e_tile<T, false, 1, B32>(a, t - tile_start[g]);
+    const TfsfLaunch none{};
+    if (a.cpml) e_tile<T, true, 1, B32, false>(a, none, t - tile_start[g]);
+    else e_tile<T, false, 1, B32, false>(a, none, t - tile_start[g]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1647,59 +1786,11 @@ __global__ __launch_bounds__(256) void k_energy_final(const double* __restrict__
 }
 
 // ---------------------------------------------------------------------------
-// TF/SF plane wave, x-polarised, travelling along +z.  The incident Ex (at
-// z = k) and Hy (at z = k + 1/2) come from a 1D Yee line with the 3D grid's
-// dt and dr, so the incident wave has the 3D grid's numerical dispersion
-// along z.  Total-field region: Ex i in [i0,i1-1], j in [j0,j1], k in [k0,k1];
-// Hy i in [i0,i1-1], j in [j0,j1], k in [k0,k1-1]; Hz i in [i0,i1-1],
-// j in [j0,j1-1], k in [k0,k1]; Ez i in [i0,i1], j in [j0,j1], k in [k0,k1-1].
+// TF/SF plane wave (TfsfArgs above): the E corrections and the 1D line's E
+// half step after the E launch (both step modes), and the 1D line's H half
+// step of the fused step (the separate H launch carries the H corrections and
+// the line's H half step itself).
 // ---------------------------------------------------------------------------
-struct TfsfArgs {
-    int i0, i1, j0, j1, k0, k1;
-    int kb, n1, ks;     // 1D line covers k = kb .. kb+n1-1; source node ks
-    double dt_mu0, dt_eps0, inv_dr;
-};
-
-template <typename T>
-__global__ void k_tfsf_h(const YeeArgs<T> a, TfsfArgs t, double* __restrict__ ex1, double* __restrict__ hy1) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    const int nxi = t.i1 - t.i0;            // Hy/Hz x range size
-    const int nyj = t.j1 - t.j0 + 1;        // Hy y range
-    const int nzk = t.k1 - t.k0 + 1;        // Hz z range
-    const int sa = nxi * nyj, sb = nxi * nzk;
-    if (q < 2 * sa) {                        // Hy on the z faces
-        const int side = q / sa, r = q % sa;
-        const int i = t.i0 + r % nxi, j = t.j0 + r / nxi;
-        if (side == 0) {
-            const int k = t.k0 - 1;
-            const long o = a.sx * i + (long)a.sz * j + k + a.off;
-            a.hy[o] = (T)((double)a.hy[o] + t.dt_mu0 * (double)a.den_hz[k] * ex1[t.k0 - t.kb]);
-        } else {
-            const int k = t.k1;
-            const long o = a.sx * i + (long)a.sz * j + k + a.off;
-            a.hy[o] = (T)((double)a.hy[o] - t.dt_mu0 * (double)a.den_hz[k] * ex1[t.k1 - t.kb]);
-        }
-        return;
-    }
-    const int q2 = q - 2 * sa;
-    if (q2 < 2 * sb) {                       // Hz on the y faces
-        const int side = q2 / sb, r = q2 % sb;
-        const int i = t.i0 + r % nxi, k = t.k0 + r / nxi;
-        const double inc = ex1[k - t.kb];
-        if (side == 0) {
-            const int j = t.j0 - 1;
-            const long o = a.sx * i + (long)a.sz * j + k + a.off;
-            a.hz[o] = (T)((double)a.hz[o] - t.dt_mu0 * (double)a.den_hy[j] * inc);
-        } else {
-            const int j = t.j1;
-            const long o = a.sx * i + (long)a.sz * j + k + a.off;
-            a.hz[o] = (T)((double)a.hz[o] + t.dt_mu0 * (double)a.den_hy[j] * inc);
-        }
-        return;
-    }
-    const int m = q2 - 2 * sb;               // 1D H update
-    if (m < t.n1 - 1) hy1[m] = hy1[m] + t.dt_mu0 * t.inv_dr * (ex1[m] - ex1[m + 1]);
-}
 
 template <typename T>
 __global__ void k_tfsf_e(const YeeArgs<T> a, TfsfArgs t, double* __restrict__ ex1, const double* __restrict__ hy1,
@@ -1723,9 +1814,9 @@ __global__ void k_tfsf_e(const YeeArgs<T> a, TfsfArgs t, double* __restrict__ ex
         return;
     }
     const int q2 = q - 2 * sa;
-    if (q2 < 2 * sb) {                       // Ez on the x faces
+    if (q2 < 2 * sb) {                       // Ez on the x faces (consecutive threads along z)
         const int side = q2 / sb, r = q2 % sb;
-        const int j = t.j0 + r % nyz, k = t.k0 + r / nyz;
+        const int k = t.k0 + r % nzk, j = t.j0 + r / nzk;
         const double inc = hy1[k - t.kb];
         if (side == 0) {
             const int i = t.i0;
@@ -1744,6 +1835,18 @@ __global__ void k_tfsf_e(const YeeArgs<T> a, TfsfArgs t, double* __restrict__ ex
         if (m == t.ks - t.kb) v += src;
         ex1[m] = v;
     }
+}
+
+// Sets *flag when a listed dispersive cell (entries as for
+// e_dispersive_cell) lies on the E correction faces of the TF/SF box.
+__global__ void k_disp_on_tfsf(const long* __restrict__ list, int n, TfsfArgs t, int* __restrict__ flag) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const long e = list[p];
+    const int i = (int)(e & 0xffff), j = (int)((e >> 16) & 0xffff), k = (int)((e >> 32) & 0xffff);
+    const bool ex_face = (k == t.k0 || k == t.k1) && i >= t.i0 && i <= t.i1 - 1 && j >= t.j0 && j <= t.j1;
+    const bool ez_face = (i == t.i0 || i == t.i1) && j >= t.j0 && j <= t.j1 && k >= t.k0 && k <= t.k1 - 1;
+    if (ex_face || ez_face) flag[0] = 1;
 }
 
 __global__ void k_tfsf_1d_h(double* __restrict__ ex1, double* __restrict__ hy1, int n1, double c) {
