@@ -46,13 +46,28 @@ __device__ __forceinline__ int tile_of_block(int order, int b, int nb, int ntz) 
 
 // Field access by 32-bit byte offset from a uniform base pointer (callers
 // take these paths only when every array of the grid is below 4 GiB).
+// The device arrays all live in global memory; stating it lets the
+// compiler use global (not flat) loads and stores also where a pointer comes
+// from memory (the argument blocks of a batch), with a scalar base and a
+// 32-bit vector offset.
+#define OMX_GLOBAL __attribute__((address_space(1)))
+// (C-style casts: the address-space conversion of a generic pointer)
 template <typename T>
 __device__ __forceinline__ T ldb(const T* p, uint32_t b) {
-    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + b);
+    return *(const OMX_GLOBAL T*)((const OMX_GLOBAL char*)p + b);
 }
 template <typename T>
 __device__ __forceinline__ void stb(T* p, uint32_t b, T v) {
-    *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + b) = v;
+    *(OMX_GLOBAL T*)((OMX_GLOBAL char*)p + b) = v;
+}
+// Element i of a global array.
+template <typename T>
+__device__ __forceinline__ T gld(const T* p, long i) {
+    return ((const OMX_GLOBAL T*)p)[i];
+}
+template <typename T>
+__device__ __forceinline__ void gst(T* p, long i, T v) {
+    ((OMX_GLOBAL T*)p)[i] = v;
 }
 // Uniform read of a table the kernel never writes (scalar load).
 template <typename T>
@@ -69,9 +84,9 @@ template <typename T, bool B32>
 struct Off {
     using I = long;
     static __device__ __forceinline__ I of(long e) { return e; }
-    static __device__ __forceinline__ T ld(const T* p, I o) { return p[o]; }
-    static __device__ __forceinline__ void st(T* p, I o, T v) { p[o] = v; }
-    static __device__ __forceinline__ int tag(const uint8_t* p, I o) { return p[o]; }
+    static __device__ __forceinline__ T ld(const T* p, I o) { return gld(p, o); }
+    static __device__ __forceinline__ void st(T* p, I o, T v) { gst(p, o, v); }
+    static __device__ __forceinline__ int tag(const uint8_t* p, I o) { return gld(p, o); }
 };
 template <typename T>
 struct Off<T, true> {
@@ -79,7 +94,7 @@ struct Off<T, true> {
     static __device__ __forceinline__ I of(long e) { return (uint32_t)(e * (long)sizeof(T)); }
     static __device__ __forceinline__ T ld(const T* p, I o) { return ldb(p, o); }
     static __device__ __forceinline__ void st(T* p, I o, T v) { stb(p, o, v); }
-    static __device__ __forceinline__ int tag(const uint8_t* p, I o) { return p[o / (uint32_t)sizeof(T)]; }
+    static __device__ __forceinline__ int tag(const uint8_t* p, I o) { return gld(p, (long)(o / (uint32_t)sizeof(T))); }
 };
 
 // ---------------------------------------------------------------------------
@@ -530,11 +545,11 @@ __device__ __forceinline__ bool tfsf_h_lane(const YeeArgs<T>& a, const TfsfLaunc
     if (!yrow && !zrow) return false;
     if (yrow && (k == t.k0 - 1 || k == t.k1)) {
         const bool hi = k == t.k1;
-        const double c = t.dt_mu0 * (double)a.den_hz[k] * L.ex1[(hi ? t.k1 : t.k0) - t.kb];
+        const double c = t.dt_mu0 * (double)gld(a.den_hz, k) * L.ex1[(hi ? t.k1 : t.k0) - t.kb];
         cy = hi ? -c : c;
     }
     if (zrow && k >= t.k0 && k <= t.k1) {
-        const double c = t.dt_mu0 * (double)a.den_hy[j] * L.ex1[k - t.kb];
+        const double c = t.dt_mu0 * (double)gld(a.den_hy, j) * L.ex1[k - t.kb];
         cz = j == t.j1 ? c : -c;
     }
     return true;
@@ -562,14 +577,14 @@ __device__ __forceinline__ bool tfsf_e_lane(const YeeArgs<T>& a, const TfsfLaunc
     if (!xrow && !zrow) return false;
     if (xrow && (k == t.k0 || k == t.k1)) {
         const bool hi = k == t.k1;
-        const double c = t.dt_eps0 * (double)a.den_ez[k] * (hi ? L.hy1[t.k1 - t.kb] : L.hy1[t.k0 - 1 - t.kb]);
+        const double c = t.dt_eps0 * (double)gld(a.den_ez, k) * (hi ? L.hy1[t.k1 - t.kb] : L.hy1[t.k0 - 1 - t.kb]);
         cx = hi ? -c : c;
     }
     if (!xrow) { xa = 1; xb = 0; }
     if (zrow && k >= t.k0 && k <= t.k1 - 1) {
         const double inc = L.hy1[k - t.kb];
-        cl = -(t.dt_eps0 * (double)a.den_ex[t.i0] * inc);
-        ch = t.dt_eps0 * (double)a.den_ex[t.i1] * inc;
+        cl = -(t.dt_eps0 * (double)gld(a.den_ex, t.i0) * inc);
+        ch = t.dt_eps0 * (double)gld(a.den_ex, t.i1) * inc;
     }
     return true;
 }
@@ -598,8 +613,8 @@ __device__ __forceinline__ void h_tile(const YeeArgs<T>& a, const TfsfLaunch& tf
     const I sx = O::of(a.sx), sz = O::of(a.sz), s1 = O::of(1);
     const long jke = (long)j * a.sz + k + a.off;
     I o = O::of((long)i0 * a.sx + jke);
-    const T dhy = a.den_hy[j], dhz = a.den_hz[k];
-    const T chy = sizeof(T) == 4 ? a.cdhy[j] : (T)0, chz = sizeof(T) == 4 ? a.cdhz[k] : (T)0;
+    const T dhy = gld(a.den_hy, j), dhz = gld(a.den_hz, k);
+    const T chy = sizeof(T) == 4 ? gld(a.cdhy, j) : (T)0, chz = sizeof(T) == 4 ? gld(a.cdhz, k) : (T)0;
     const T dmu = a.dt_mu0;
     T ezc = O::ld(a.ez, o), eyc = O::ld(a.ey, o);
     const int kt0 = tz * kLanes + 1, kt1 = min(kt0 + kLanes - 1, a.nz);
@@ -619,7 +634,7 @@ __device__ __forceinline__ void h_tile(const YeeArgs<T>& a, const TfsfLaunch& tf
             const T ezj = O::ld(a.ez, o + sz), eyk = O::ld(a.ey, o + s1);
             const T exc = O::ld(a.ex, o), exk = O::ld(a.ex, o + s1), exj = O::ld(a.ex, o + sz);
             T hx = O::ld(a.hx, o), hy = O::ld(a.hy, o), hz = O::ld(a.hz, o);
-            const T cx = sizeof(T) == 4 ? a.cdhx[i] : a.den_hx[i];
+            const T cx = sizeof(T) == 4 ? gld(a.cdhx, i) : gld(a.den_hx, i);
             h_bulk(dmu, cx, dhy, dhz, chy, chz, ezc, ezn, ezj, eyc, eyn, eyk, exc, exk, exj, hx, hy, hz);
             if (tfw && i >= ia && i <= ib) { hy = (T)((double)hy + cy); hz = (T)((double)hz + cz); }
             O::st(a.hx, o, hx); O::st(a.hy, o, hy); O::st(a.hz, o, hz);
@@ -632,8 +647,8 @@ __device__ __forceinline__ void h_tile(const YeeArgs<T>& a, const TfsfLaunch& tf
     const Face fy = face_h(cp & 4u, cp & 8u, j, a.ny, npml);
     const Face fz = face_h(cp & 16u, cp & 32u, k, a.nz, npml);
     T yb = (T)0, yc = (T)0, zb = (T)0, zc = (T)0;
-    if (fy.slot >= 0) { yb = a.bh[fy.c]; yc = a.chd[fy.c]; }
-    if (fz.slot >= 0) { zb = a.bh[fz.c]; zc = a.chd[fz.c]; }
+    if (fy.slot >= 0) { yb = gld(a.bh, fy.c); yc = gld(a.chd, fy.c); }
+    if (fz.slot >= 0) { zb = gld(a.bh, fz.c); zc = gld(a.chd, fz.c); }
     // psi of plane i: x (fx.slot*sx + jk), y (i*nsh*sz + slot*sz + k + off),
     // z (i*sy*nsh + j*nsh + slot); offsets of faces a cell is not in are unused
     const I jk = O::of(jke);
@@ -645,11 +660,11 @@ __device__ __forceinline__ void h_tile(const YeeArgs<T>& a, const TfsfLaunch& tf
         const T ezj = O::ld(a.ez, o + sz), eyk = O::ld(a.ey, o + s1);
         const T exc = O::ld(a.ex, o), exk = O::ld(a.ex, o + s1), exj = O::ld(a.ex, o + sz);
         T hx = O::ld(a.hx, o), hy = O::ld(a.hy, o), hz = O::ld(a.hz, o);
-        const T cx = sizeof(T) == 4 ? a.cdhx[i] : a.den_hx[i];
+        const T cx = sizeof(T) == 4 ? gld(a.cdhx, i) : gld(a.den_hx, i);
         const Face fx = face_h(cp & 1u, cp & 2u, i, a.nx, npml);
         const I px = O::of((long)fx.slot * a.sx) + jk;
         T px0 = (T)0, px1 = (T)0, xb = (T)0, xc = (T)0, py0 = (T)0, py1 = (T)0, pz0 = (T)0, pz1 = (T)0;
-        if (fx.slot >= 0) { px0 = O::ld(a.ph[0], px); px1 = O::ld(a.ph[1], px); xb = a.bh[fx.c]; xc = a.chd[fx.c]; }
+        if (fx.slot >= 0) { px0 = O::ld(a.ph[0], px); px1 = O::ld(a.ph[1], px); xb = gld(a.bh, fx.c); xc = gld(a.chd, fx.c); }
         if (fy.slot >= 0) { py0 = O::ld(a.ph[2], py); py1 = O::ld(a.ph[3], py); }
         if (fz.slot >= 0) { pz0 = O::ld(a.ph[4], pz); pz1 = O::ld(a.ph[5], pz); }
         h_bulk(dmu, cx, dhy, dhz, chy, chz, ezc, ezn, ezj, eyc, eyn, eyk, exc, exk, exj, hx, hy, hz);
@@ -687,7 +702,7 @@ __device__ __forceinline__ T medium_update(T e, T rot, T vac, int m, const MedDe
                                            T* __restrict__ pl, T* __restrict__ plold, long ps) {
     using O = Off<T, B32>;
     if (m == 0) return e + vac;   // vac = dt_eps0 * rotH
-    const MedDev<T>& M = med[m];
+    const OMX_GLOBAL MedDev<T>& M = *((const OMX_GLOBAL MedDev<T>*)med + m);
     if (M.kind == OMX_MEDIUM_DRUDE) {
         const T p = O::ld(pd, o);
         if constexpr (sizeof(T) == 8) {
@@ -764,11 +779,10 @@ __device__ __forceinline__ void e_bulk(const YeeArgs<T>& a, int tg, T dep, T hzc
     e_curl(dep, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dex, cex_, dey, dez, cey, cez, rx, ry, rz, vx, vy, vz);
     if (MED == 1 && tg != 0) {
         // vacuum: + dt_eps0 rotH; dielectric: + dt_eps0/eps_r rotH
-        const int8_t* c3 = a.cls + 3 * tg;
-        const int m0 = c3[0], m1 = c3[1], m2 = c3[2];
-        ex = m0 == 0 ? ex + vx : ex + a.med[m0].cdt * rx;
-        ey = m1 == 0 ? ey + vy : ey + a.med[m1].cdt * ry;
-        ez = m2 == 0 ? ez + vz : ez + a.med[m2].cdt * rz;
+        const int m0 = gld(a.cls, 3 * tg), m1 = gld(a.cls, 3 * tg + 1), m2 = gld(a.cls, 3 * tg + 2);
+        ex = m0 == 0 ? ex + vx : ex + gld(&a.med[m0].cdt, 0) * rx;
+        ey = m1 == 0 ? ey + vy : ey + gld(&a.med[m1].cdt, 0) * ry;
+        ez = m2 == 0 ? ez + vz : ez + gld(&a.med[m2].cdt, 0) * rz;
     } else {
         ex = ex + vx; ey = ey + vy; ez = ez + vz;
     }
@@ -783,17 +797,17 @@ __device__ __forceinline__ void e_cpml_x(const YeeArgs<T>& a, int i, long jk, T 
     if ((cp & 1u) && i >= 2 && i <= npml) {
         const int c = i;
         const long ps = (long)(i - 1) * a.sx + jk;
-        const T p0 = psi_step(a.pe[0][ps], a.be[c], a.ced[c], hzi - hzc);
-        const T p1 = psi_step(a.pe[1][ps], a.be[c], a.ced[c], hyc - hyi);
-        a.pe[0][ps] = p0; a.pe[1][ps] = p1;
+        const T p0 = psi_step(gld(a.pe[0], ps), gld(a.be, c), gld(a.ced, c), hzi - hzc);
+        const T p1 = psi_step(gld(a.pe[1], ps), gld(a.be, c), gld(a.ced, c), hyc - hyi);
+        gst(a.pe[0], ps, p0); gst(a.pe[1], ps, p1);
         ey = ey + dep * p0; ez = ez + dep * p1;
     }
     if ((cp & 2u) && i >= a.nx + 1 - npml && i <= a.nx - 1) {
         const int c = a.nx + 1 - i;
         const long ps = (long)(npml - 1 + c) * a.sx + jk;
-        const T p0 = psi_step(a.pe[0][ps], a.be[c], a.ced[c], hzi - hzc);
-        const T p1 = psi_step(a.pe[1][ps], a.be[c], a.ced[c], hyc - hyi);
-        a.pe[0][ps] = p0; a.pe[1][ps] = p1;
+        const T p0 = psi_step(gld(a.pe[0], ps), gld(a.be, c), gld(a.ced, c), hzi - hzc);
+        const T p1 = psi_step(gld(a.pe[1], ps), gld(a.be, c), gld(a.ced, c), hyc - hyi);
+        gst(a.pe[0], ps, p0); gst(a.pe[1], ps, p1);
         ey = ey + dep * p0; ez = ez + dep * p1;
     }
 }
@@ -808,17 +822,17 @@ __device__ __forceinline__ void e_cpml_y(const YeeArgs<T>& a, int i, int j, int 
     if ((cp & 4u) && j >= 2 && j <= npml) {
         const int c = j;
         const long ps = ((long)i * nse + (j - 1)) * a.sz + k + a.off;
-        const T p0 = psi_step(a.pe[2][ps], a.be[c], a.ced[c], hzc - hzj);
-        const T p1 = psi_step(a.pe[3][ps], a.be[c], a.ced[c], hxj - hxc);
-        a.pe[2][ps] = p0; a.pe[3][ps] = p1;
+        const T p0 = psi_step(gld(a.pe[2], ps), gld(a.be, c), gld(a.ced, c), hzc - hzj);
+        const T p1 = psi_step(gld(a.pe[3], ps), gld(a.be, c), gld(a.ced, c), hxj - hxc);
+        gst(a.pe[2], ps, p0); gst(a.pe[3], ps, p1);
         ex = ex + dep * p0; ez = ez + dep * p1;
     }
     if ((cp & 8u) && j >= a.ny + 1 - npml && j <= a.ny - 1) {
         const int c = a.ny + 1 - j;
         const long ps = ((long)i * nse + (npml - 1 + c)) * a.sz + k + a.off;
-        const T p0 = psi_step(a.pe[2][ps], a.be[c], a.ced[c], hzc - hzj);
-        const T p1 = psi_step(a.pe[3][ps], a.be[c], a.ced[c], hxj - hxc);
-        a.pe[2][ps] = p0; a.pe[3][ps] = p1;
+        const T p0 = psi_step(gld(a.pe[2], ps), gld(a.be, c), gld(a.ced, c), hzc - hzj);
+        const T p1 = psi_step(gld(a.pe[3], ps), gld(a.be, c), gld(a.ced, c), hxj - hxc);
+        gst(a.pe[2], ps, p0); gst(a.pe[3], ps, p1);
         ex = ex + dep * p0; ez = ez + dep * p1;
     }
 }
@@ -833,17 +847,17 @@ __device__ __forceinline__ void e_cpml_z(const YeeArgs<T>& a, int i, int j, int 
     if ((cp & 16u) && k >= 2 && k <= npml) {
         const int c = k;
         const long ps = ((long)i * a.sy + j) * nse + (k - 1);
-        const T p0 = psi_step(a.pe[4][ps], a.be[c], a.ced[c], hyk - hyc);
-        const T p1 = psi_step(a.pe[5][ps], a.be[c], a.ced[c], hxc - hxk);
-        a.pe[4][ps] = p0; a.pe[5][ps] = p1;
+        const T p0 = psi_step(gld(a.pe[4], ps), gld(a.be, c), gld(a.ced, c), hyk - hyc);
+        const T p1 = psi_step(gld(a.pe[5], ps), gld(a.be, c), gld(a.ced, c), hxc - hxk);
+        gst(a.pe[4], ps, p0); gst(a.pe[5], ps, p1);
         ex = ex + dep * p0; ey = ey + dep * p1;
     }
     if ((cp & 32u) && k >= a.nz + 1 - npml && k <= a.nz - 1) {
         const int c = a.nz + 1 - k;
         const long ps = ((long)i * a.sy + j) * nse + (npml - 1 + c);
-        const T p0 = psi_step(a.pe[4][ps], a.be[c], a.ced[c], hyk - hyc);
-        const T p1 = psi_step(a.pe[5][ps], a.be[c], a.ced[c], hxc - hxk);
-        a.pe[4][ps] = p0; a.pe[5][ps] = p1;
+        const T p0 = psi_step(gld(a.pe[4], ps), gld(a.be, c), gld(a.ced, c), hyk - hyc);
+        const T p1 = psi_step(gld(a.pe[5], ps), gld(a.be, c), gld(a.ced, c), hxc - hxk);
+        gst(a.pe[4], ps, p0); gst(a.pe[5], ps, p1);
         ex = ex + dep * p0; ey = ey + dep * p1;
     }
 }
@@ -869,8 +883,8 @@ __device__ __forceinline__ void e_tile(const YeeArgs<T>& a, const TfsfLaunch& tf
     const I sx = O::of(a.sx), sz = O::of(a.sz), s1 = O::of(1);
     const long jke = (long)j * a.sz + k + a.off;
     I o = O::of((long)i0 * a.sx + jke);
-    const T dey = a.den_ey[j], dez = a.den_ez[k];
-    const T cey = sizeof(T) == 4 ? a.cdey[j] : (T)0, cez = sizeof(T) == 4 ? a.cdez[k] : (T)0;
+    const T dey = gld(a.den_ey, j), dez = gld(a.den_ez, k);
+    const T cey = sizeof(T) == 4 ? gld(a.cdey, j) : (T)0, cez = sizeof(T) == 4 ? gld(a.cdez, k) : (T)0;
     const T dep = a.dt_eps0;
     const bool tags = MED != 0 && a.tag != nullptr;
     T hzi = O::ld(a.hz, o - sx), hyi = O::ld(a.hy, o - sx);
@@ -893,7 +907,7 @@ __device__ __forceinline__ void e_tile(const YeeArgs<T>& a, const TfsfLaunch& tf
             const T hyk = O::ld(a.hy, o - s1), hxk = O::ld(a.hx, o - s1);
             T ex = O::ld(a.ex, o), ey = O::ld(a.ey, o), ez = O::ld(a.ez, o);
             const int tg = tags ? O::tag(a.tag, o) : 0;
-            const T dex = a.den_ex[i], cex = sizeof(T) == 4 ? a.cdex[i] : (T)0;
+            const T dex = gld(a.den_ex, i), cex = sizeof(T) == 4 ? gld(a.cdex, i) : (T)0;
             if (MED != 1 || tg < a.disp_tag0) {
                 e_bulk<T, MED>(a, tg, dep, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dex, cex, dey, dez, cey,
                                cez, ex, ey, ez);
@@ -913,8 +927,8 @@ __device__ __forceinline__ void e_tile(const YeeArgs<T>& a, const TfsfLaunch& tf
     const Face fy = face_e(cp & 4u, cp & 8u, j, a.ny, npml);
     const Face fz = face_e(cp & 16u, cp & 32u, k, a.nz, npml);
     T yb = (T)0, yc = (T)0, zb = (T)0, zc = (T)0;
-    if (fy.slot >= 0) { yb = a.be[fy.c]; yc = a.ced[fy.c]; }
-    if (fz.slot >= 0) { zb = a.be[fz.c]; zc = a.ced[fz.c]; }
+    if (fy.slot >= 0) { yb = gld(a.be, fy.c); yc = gld(a.ced, fy.c); }
+    if (fz.slot >= 0) { zb = gld(a.be, fz.c); zc = gld(a.ced, fz.c); }
     const I jk = O::of(jke);
     const I ystep = O::of((long)nse * a.sz), zstep = O::of((long)a.sy * nse);
     I py = O::of((long)i0 * nse * a.sz + (long)fy.slot * a.sz + k + a.off);
@@ -925,7 +939,7 @@ __device__ __forceinline__ void e_tile(const YeeArgs<T>& a, const TfsfLaunch& tf
         const T hyk = O::ld(a.hy, o - s1), hxk = O::ld(a.hx, o - s1);
         T ex = O::ld(a.ex, o), ey = O::ld(a.ey, o), ez = O::ld(a.ez, o);
         const int tg = tags ? O::tag(a.tag, o) : 0;
-        const T dex = a.den_ex[i], cex = sizeof(T) == 4 ? a.cdex[i] : (T)0;
+        const T dex = gld(a.den_ex, i), cex = sizeof(T) == 4 ? gld(a.cdex, i) : (T)0;
         if (MED == 1 && tg >= a.disp_tag0) {
             hzi = hzc; hyi = hyc;
             continue;
@@ -933,7 +947,7 @@ __device__ __forceinline__ void e_tile(const YeeArgs<T>& a, const TfsfLaunch& tf
         const Face fx = face_e(cp & 1u, cp & 2u, i, a.nx, npml);
         const I px = O::of((long)fx.slot * a.sx) + jk;
         T px0 = (T)0, px1 = (T)0, xb = (T)0, xc = (T)0, py0 = (T)0, py1 = (T)0, pz0 = (T)0, pz1 = (T)0;
-        if (fx.slot >= 0) { px0 = O::ld(a.pe[0], px); px1 = O::ld(a.pe[1], px); xb = a.be[fx.c]; xc = a.ced[fx.c]; }
+        if (fx.slot >= 0) { px0 = O::ld(a.pe[0], px); px1 = O::ld(a.pe[1], px); xb = gld(a.be, fx.c); xc = gld(a.ced, fx.c); }
         if (fy.slot >= 0) { py0 = O::ld(a.pe[2], py); py1 = O::ld(a.pe[3], py); }
         if (fz.slot >= 0) { pz0 = O::ld(a.pe[4], pz); pz1 = O::ld(a.pe[5], pz); }
         e_bulk<T, MED>(a, tg, dep, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dex, cex, dey, dez, cey, cez,
@@ -986,9 +1000,9 @@ __device__ __forceinline__ void e_dispersive_cell(const YeeArgs<T>& a, T* __rest
     const T hyk = O::ld(a.hy, o - s1), hxk = O::ld(a.hx, o - s1);
     const T hzi = O::ld(a.hz, o - sx), hyi = O::ld(a.hy, o - sx);
     T ex = O::ld(a.ex, o), ey = O::ld(a.ey, o), ez = O::ld(a.ez, o);
-    const T dex = a.den_ex[i], cex = sizeof(T) == 4 ? a.cdex[i] : (T)0;
-    const T dey = a.den_ey[j], dez = a.den_ez[k];
-    const T cey = sizeof(T) == 4 ? a.cdey[j] : (T)0, cez = sizeof(T) == 4 ? a.cdez[k] : (T)0;
+    const T dex = gld(a.den_ex, i), cex = sizeof(T) == 4 ? gld(a.cdex, i) : (T)0;
+    const T dey = gld(a.den_ey, j), dez = gld(a.den_ez, k);
+    const T cey = sizeof(T) == 4 ? gld(a.cdey, j) : (T)0, cez = sizeof(T) == 4 ? gld(a.cdez, k) : (T)0;
     const T dep = a.dt_eps0;
     T rx, ry, rz, vx, vy, vz;
     e_curl(dep, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dex, cex, dey, dez, cey, cez, rx, ry, rz, vx, vy, vz);
