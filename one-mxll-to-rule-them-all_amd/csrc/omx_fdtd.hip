@@ -82,7 +82,8 @@ YeeArgs<T> make_args(omx3d_ctx* c) {
         a.eold[q] = (T*)c->disp_eold[q]; a.pd[q] = (T*)c->disp_pd[q];
         a.pl[q] = (T*)c->disp_pl[q]; a.plold[q] = (T*)c->disp_plold[q];
     }
-    a.pstride = c->L.nalloc;
+    a.pstride = c->n_disp;
+    a.disp0 = 0;
     a.dt_mu0 = (T)(c->d.dt / kMu0);
     a.dt_eps0 = (T)(c->d.dt / kEps0);
     a.nx = c->L.nx; a.ny = c->L.ny; a.nz = c->L.nz; a.sy = c->L.sy; a.sz = c->L.sz;
@@ -830,7 +831,8 @@ int media_common(omx3d_ctx* c, int n_media, const omx_medium* table) {
 int media_finish(omx3d_ctx* c) {
     int rc = build_tags(c);
     if (rc) return rc;
-    const size_t bytes = c->esz * (size_t)c->L.nalloc;
+    // the dispersive state is kept for the listed cells only, in list order
+    const size_t bytes = c->esz * (size_t)std::max(c->n_disp, 1);
     if (c->has_drude || c->has_dl) {
         for (int q = 0; q < 3; ++q) {
             rc = dalloc(c, &c->disp_pd[q], bytes);
@@ -929,13 +931,11 @@ int batch_setup(omxb_ctx* b) {
     b->tile_order = b->g[0]->tile_order;
     b->ntz = args[0].ntz;
     for (auto* c : b->g) b->b32 = b->b32 && offsets32(c);
-    int rc = bupload(b->s, (YeeArgs<T>**)&b->d_args, args);
-    if (!rc) rc = bupload(b->s, &b->d_tile_start, ts);
-    if (rc) return rc;
     // cells of dispersive classes of all grids, for the first workgroups of
     // the batched E launch: each grid's cells fill whole workgroups (padded
     // with -1), so that a workgroup's grid, and with it its argument block,
-    // is uniform
+    // is uniform; disp0 locates a grid's cells (and so its state) in the list
+    int rc = OMX_OK;
     {
         const size_t wg = (size_t)kLanes * b->rows;
         std::vector<long> cells;
@@ -945,6 +945,7 @@ int batch_setup(omxb_ctx* b) {
             if (!(c->media_set && c->n_classes > 1 && c->n_disp > 0)) continue;
             std::vector<long> h(c->n_disp);
             HIPCHK(hipMemcpy(h.data(), c->d_disp, sizeof(long) * h.size(), hipMemcpyDeviceToHost));
+            args[q].disp0 = (long)cells.size();
             cells.insert(cells.end(), h.begin(), h.end());
             cells.resize((cells.size() + wg - 1) / wg * wg, -1L);
             gob.resize(cells.size() / wg, q);
@@ -956,6 +957,9 @@ int batch_setup(omxb_ctx* b) {
         if (!rc) rc = bupload(b->s, &b->d_disp_grid, gob);
         if (rc) return rc;
     }
+    rc = bupload(b->s, (YeeArgs<T>**)&b->d_args, args);
+    if (!rc) rc = bupload(b->s, &b->d_tile_start, ts);
+    if (rc) return rc;
     // sources: one pass per repeated (field, cell), in source order
     std::map<std::pair<void*, long>, int> seen;
     std::vector<std::vector<void*>> pf;
@@ -1904,11 +1908,12 @@ int omx3d_autotune(omx3d_ctx* c, int32_t steps, int32_t* rows_y, int32_t* xchunk
             if (c->psi_e[2 * a + q]) HIPCHK(hipMemsetAsync(c->psi_e[2 * a + q], 0, c->esz * sz[1], c->s));
         }
     }
+    const size_t db = c->esz * (size_t)std::max(c->n_disp, 1);
     for (int q = 0; q < 3; ++q) {
-        if (c->disp_pd[q]) HIPCHK(hipMemsetAsync(c->disp_pd[q], 0, fb, c->s));
-        if (c->disp_eold[q]) HIPCHK(hipMemsetAsync(c->disp_eold[q], 0, fb, c->s));
-        if (c->disp_pl[q]) HIPCHK(hipMemsetAsync(c->disp_pl[q], 0, fb * c->maxp, c->s));
-        if (c->disp_plold[q]) HIPCHK(hipMemsetAsync(c->disp_plold[q], 0, fb * c->maxp, c->s));
+        if (c->disp_pd[q]) HIPCHK(hipMemsetAsync(c->disp_pd[q], 0, db, c->s));
+        if (c->disp_eold[q]) HIPCHK(hipMemsetAsync(c->disp_eold[q], 0, db, c->s));
+        if (c->disp_pl[q]) HIPCHK(hipMemsetAsync(c->disp_pl[q], 0, db * c->maxp, c->s));
+        if (c->disp_plold[q]) HIPCHK(hipMemsetAsync(c->disp_plold[q], 0, db * c->maxp, c->s));
     }
     HIPCHK(hipStreamSynchronize(c->s));
     if (rows_y) *rows_y = br;

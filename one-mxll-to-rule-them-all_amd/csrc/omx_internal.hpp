@@ -67,7 +67,8 @@ struct YeeArgs {
     const int8_t* cls;
     const MedDev<T>* med;
     T *eold[3], *pd[3], *pl[3], *plold[3];
-    long pstride;              // elements between pole planes
+    long pstride;              // elements between pole planes of the dispersive state (the list's length)
+    long disp0;                // position of the grid's first dispersive cell in the list the kernel reads
     T dt_mu0, dt_eps0;
     int nx, ny, nz, sy, sz;
     long sx;
@@ -145,6 +146,8 @@ struct omx3d_ctx {
     double* d_eps_class = nullptr;  // eps_r per class and component (energy)
     int maxp = 0;
     bool has_drude = false, has_dl = false;
+    // Drude / DL state per component, compact: entry p belongs to the
+    // dispersive list's cell p (PL and PL_old: maxp planes of n_disp)
     void* disp_eold[3] = {};
     void* disp_pd[3] = {};
     void* disp_pl[3] = {};

@@ -694,55 +694,53 @@ __device__ __forceinline__ void h_tile(const YeeArgs<T>& a, const TfsfLaunch& tf
 // medium update of get_medium_polarization (classical_medium_mod.f90:773-833).
 // Currents of embedded quantum systems are applied by k_j_correct afterwards.
 // ---------------------------------------------------------------------------
-// o: the cell's offset (Off<T, B32>) into eold, pd and into every pole plane
-// of pl / plold, which lie ps elements apart.
-template <typename T, bool B32>
-__device__ __forceinline__ T medium_update(T e, T rot, T vac, int m, const MedDev<T>* __restrict__ med,
-                                           typename Off<T, B32>::I o, T* __restrict__ eold, T* __restrict__ pd,
-                                           T* __restrict__ pl, T* __restrict__ plold, long ps) {
-    using O = Off<T, B32>;
+// p: the cell's position in the dispersive list, its index into the compact
+// state eold, pd and into every pole plane of pl / plold (planes ps apart).
+template <typename T>
+__device__ __forceinline__ T medium_update(T e, T rot, T vac, int m, const MedDev<T>* __restrict__ med, long p,
+                                           T* __restrict__ eold, T* __restrict__ pd, T* __restrict__ pl,
+                                           T* __restrict__ plold, long ps) {
     if (m == 0) return e + vac;   // vac = dt_eps0 * rotH
     const OMX_GLOBAL MedDev<T>& M = *((const OMX_GLOBAL MedDev<T>*)med + m);
     if (M.kind == OMX_MEDIUM_DRUDE) {
-        const T p = O::ld(pd, o);
+        const T pv = gld(pd, p);
         if constexpr (sizeof(T) == 8) {
-            const T t = M.C1 * e + M.C3 * rot - M.C4 * p;
-            O::st(pd, o, M.A1 * p + M.A2 * (t + e));
+            const T t = M.C1 * e + M.C3 * rot - M.C4 * pv;
+            gst(pd, p, M.A1 * pv + M.A2 * (t + e));
             return t;
         } else {
-            const T t = e + (M.C1 * e + M.C3 * rot - M.C4 * p);
-            O::st(pd, o, p + (M.A1 * p + M.A2 * (t + e)));
+            const T t = e + (M.C1 * e + M.C3 * rot - M.C4 * pv);
+            gst(pd, p, pv + (M.A1 * pv + M.A2 * (t + e)));
             return t;
         }
     }
     if (M.kind == OMX_MEDIUM_DL) {
-        const T eo = O::ld(eold, o);
-        const T p = O::ld(pd, o);
+        const T eo = gld(eold, p);
+        const T pv = gld(pd, p);
         const int np = M.np;
         T t;
         if constexpr (sizeof(T) == 8) {
-            t = M.C1 * e + M.C2 * eo + M.C3 * rot - M.C4 * p;
-            for (int q = 0; q < np; ++q) t = t - M.B1[q] * O::ld(pl + q * ps, o) - M.B2[q] * O::ld(plold + q * ps, o);
-            O::st(pd, o, M.A1 * p + M.A2 * (t + e));
+            t = M.C1 * e + M.C2 * eo + M.C3 * rot - M.C4 * pv;
+            for (int q = 0; q < np; ++q) t = t - M.B1[q] * gld(pl, p + q * ps) - M.B2[q] * gld(plold, p + q * ps);
+            gst(pd, p, M.A1 * pv + M.A2 * (t + e));
         } else {
-            T dlt = M.C1 * e + M.C2 * eo + M.C3 * rot - M.C4 * p;
+            T dlt = M.C1 * e + M.C2 * eo + M.C3 * rot - M.C4 * pv;
             for (int q = 0; q < np; ++q)
-                dlt = dlt - M.B1[q] * O::ld(pl + q * ps, o) - M.B2[q] * O::ld(plold + q * ps, o);
+                dlt = dlt - M.B1[q] * gld(pl, p + q * ps) - M.B2[q] * gld(plold, p + q * ps);
             t = e + dlt;
-            O::st(pd, o, p + (M.A1 * p + M.A2 * (t + e)));
+            gst(pd, p, pv + (M.A1 * pv + M.A2 * (t + e)));
         }
         const T de = t - eo;
         for (int q = 0; q < np; ++q) {
-            T* const lq = pl + q * ps;
-            T* const loq = plold + q * ps;
-            const T L = O::ld(lq, o), Lo = O::ld(loq, o);
+            const long pq = p + q * ps;
+            const T L = gld(pl, pq), Lo = gld(plold, pq);
             T nl;
             if constexpr (sizeof(T) == 8) nl = M.al[q] * L + M.bt[q] * Lo + M.ga[q] * de;
             else nl = L + (((L - Lo) + M.al[q] * L) + M.bt[q] * Lo + M.ga[q] * de);
-            O::st(loq, o, L);
-            O::st(lq, o, nl);
+            gst(plold, pq, L);
+            gst(pl, pq, nl);
         }
-        O::st(eold, o, e);
+        gst(eold, p, e);
         return t;
     }
     return e + M.cdt * rot;  // dielectric: dt_eps0 * (1/eps_r) * rotH
@@ -982,10 +980,11 @@ __device__ __forceinline__ void e_tile(const YeeArgs<T>& a, const TfsfLaunch& tf
 // medium update and the CPML psi recursions of the cell, from E in
 // a.ex/a.ey/a.ez to the output arrays (the same arrays in the separate H/E
 // path, the other buffer after the fused step).  list holds the cells'
-// indices (i, j, k) with the media of their three components.
+// indices (i, j, k) with the media of their three components; p, the cell's
+// position in its grid's list, indexes its compact Drude / DL state.
 template <typename T, bool CPML, bool B32>
 __device__ __forceinline__ void e_dispersive_cell(const YeeArgs<T>& a, T* __restrict__ exo, T* __restrict__ eyo,
-                                                  T* __restrict__ ezo, long e) {
+                                                  T* __restrict__ ezo, long e, long p) {
     using O = Off<T, B32>;
     using I = typename O::I;
     // entry: i, j, k in bits 0-15, 16-31, 32-47; the media of the x, y, z
@@ -1006,9 +1005,9 @@ __device__ __forceinline__ void e_dispersive_cell(const YeeArgs<T>& a, T* __rest
     const T dep = a.dt_eps0;
     T rx, ry, rz, vx, vy, vz;
     e_curl(dep, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dex, cex, dey, dez, cey, cez, rx, ry, rz, vx, vy, vz);
-    ex = medium_update<T, B32>(ex, rx, vx, m0, a.med, o, a.eold[0], a.pd[0], a.pl[0], a.plold[0], a.pstride);
-    ey = medium_update<T, B32>(ey, ry, vy, m1, a.med, o, a.eold[1], a.pd[1], a.pl[1], a.plold[1], a.pstride);
-    ez = medium_update<T, B32>(ez, rz, vz, m2, a.med, o, a.eold[2], a.pd[2], a.pl[2], a.plold[2], a.pstride);
+    ex = medium_update(ex, rx, vx, m0, a.med, p, a.eold[0], a.pd[0], a.pl[0], a.plold[0], a.pstride);
+    ey = medium_update(ey, ry, vy, m1, a.med, p, a.eold[1], a.pd[1], a.pl[1], a.plold[1], a.pstride);
+    ez = medium_update(ez, rz, vz, m2, a.med, p, a.eold[2], a.pd[2], a.pl[2], a.plold[2], a.pstride);
     if constexpr (CPML) {
         e_cpml_x(a, i, jk, dep, hzc, hzi, hyc, hyi, ey, ez);
         e_cpml_y(a, i, j, k, dep, hzc, hzj, hxc, hxj, ex, ez);
@@ -1022,7 +1021,7 @@ __global__ __launch_bounds__(256) void k_e_dispersive(const YeeArgs<T> a, T* __r
                                                       T* __restrict__ ezo, const long* __restrict__ list, int n) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
-    e_dispersive_cell<T, CPML, B32>(a, exo, eyo, ezo, list[p]);
+    e_dispersive_cell<T, CPML, B32>(a, exo, eyo, ezo, list[p], p);
 }
 
 // H tiles; with TF the tiles apply the TF/SF H corrections and the first
@@ -1054,7 +1053,7 @@ __global__ __launch_bounds__(256) void k_e_update(const YeeArgs<T> a, const long
     const int b = blockIdx.x;
     if (MED == 1 && b < db) {
         const int p = (b * (int)blockDim.y + (int)threadIdx.y) * kLanes + (int)threadIdx.x;
-        if (p < n_disp) e_dispersive_cell<T, CPML, B32>(a, a.ex, a.ey, a.ez, disp[p]);
+        if (p < n_disp) e_dispersive_cell<T, CPML, B32>(a, a.ex, a.ey, a.ez, disp[p], p);
         return;
     }
     const int ns = db + (TF ? tf.nb : 0);
@@ -1458,7 +1457,7 @@ __global__ __launch_bounds__(256) void k_e_update_batched(const YeeArgs<T>* __re
         // the CPML terms test each face's bit of a.cpml, so one instance
         // serves grids with and without CPML
         const YeeArgs<T>& a = args[grid_of[b]];
-        e_dispersive_cell<T, true, B32>(a, a.ex, a.ey, a.ez, e);
+        e_dispersive_cell<T, true, B32>(a, a.ex, a.ey, a.ez, e, p - a.disp0);
         return;
     }
     const int t = tile_of_block(order, b - db, (int)gridDim.x - db, ntz);
