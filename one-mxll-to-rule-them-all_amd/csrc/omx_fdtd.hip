@@ -63,6 +63,11 @@ void dfreet(P*& p) {
 
 inline unsigned blocks_for(long n, int b) { return (unsigned)((n + b - 1) / b); }
 
+// Bytes of the compact Drude / DL state (YeeArgs::dstate).
+size_t disp_state_bytes(const omx3d_ctx* c) {
+    return c->esz * 3 * (size_t)(2 + 2 * c->maxp) * (size_t)std::max(c->n_disp, 1);
+}
+
 // Kernel argument block for precision T.
 template <typename T>
 YeeArgs<T> make_args(omx3d_ctx* c) {
@@ -78,11 +83,10 @@ YeeArgs<T> make_args(omx3d_ctx* c) {
     a.cdhx = tc + c->co_cd[3]; a.cdhy = tc + c->co_cd[4]; a.cdhz = tc + c->co_cd[5];
     for (int q = 0; q < 6; ++q) { a.ph[q] = (T*)c->psi_h[q]; a.pe[q] = (T*)c->psi_e[q]; }
     a.tag = c->d_tag; a.cls = c->d_cls; a.med = (const MedDev<T>*)c->d_med;
-    for (int q = 0; q < 3; ++q) {
-        a.eold[q] = (T*)c->disp_eold[q]; a.pd[q] = (T*)c->disp_pd[q];
-        a.pl[q] = (T*)c->disp_pl[q]; a.plold[q] = (T*)c->disp_plold[q];
-    }
-    a.pstride = c->n_disp;
+    a.dstate = (T*)c->disp_state;
+    a.pstride = std::max(c->n_disp, 1);
+    a.maxp = c->maxp;
+    a.dcomp = (long)(2 + 2 * c->maxp) * a.pstride;
     a.disp0 = 0;
     a.dt_mu0 = (T)(c->d.dt / kMu0);
     a.dt_eps0 = (T)(c->d.dt / kEps0);
@@ -832,22 +836,9 @@ int media_finish(omx3d_ctx* c) {
     int rc = build_tags(c);
     if (rc) return rc;
     // the dispersive state is kept for the listed cells only, in list order
-    const size_t bytes = c->esz * (size_t)std::max(c->n_disp, 1);
     if (c->has_drude || c->has_dl) {
-        for (int q = 0; q < 3; ++q) {
-            rc = dalloc(c, &c->disp_pd[q], bytes);
-            if (rc) return rc;
-        }
-    }
-    if (c->has_dl) {
-        for (int q = 0; q < 3; ++q) {
-            rc = dalloc(c, &c->disp_eold[q], bytes);
-            if (rc) return rc;
-            rc = dalloc(c, &c->disp_pl[q], bytes * c->maxp);
-            if (rc) return rc;
-            rc = dalloc(c, &c->disp_plold[q], bytes * c->maxp);
-            if (rc) return rc;
-        }
+        rc = dalloc(c, &c->disp_state, disp_state_bytes(c));
+        if (rc) return rc;
     }
     // medium kinds for the J gate of k_j_correct
     std::vector<int> kinds(OMX_MAX_MEDIA + 1, 0);
@@ -1290,9 +1281,8 @@ int omx3d_destroy(omx3d_ctx* c) {
     for (auto& p : c->psi_h) dfree(p);
     for (auto& p : c->psi_e) dfree(p);
     dfreet(c->d_aux); dfreet(c->d_tag); dfreet(c->d_cls); dfreet(c->d_disp); dfree(c->d_med); dfreet(c->d_med_in); dfreet(c->d_med_co);
-    dfreet(c->d_eps_class); dfreet(c->d_kinds);
+    dfreet(c->d_eps_class); dfreet(c->d_kinds); dfree(c->disp_state);
     for (int q = 0; q < 3; ++q) {
-        dfree(c->disp_eold[q]); dfree(c->disp_pd[q]); dfree(c->disp_pl[q]); dfree(c->disp_plold[q]);
         dfreet(c->d_aff_off[q]); dfreet(c->d_aff_a[q]); dfreet(c->d_aff_b[q]);
     }
     for (auto& s : c->sources) { dfreet(s.d_off); dfreet(s.d_ka); dfreet(s.d_kb); }
@@ -1908,13 +1898,7 @@ int omx3d_autotune(omx3d_ctx* c, int32_t steps, int32_t* rows_y, int32_t* xchunk
             if (c->psi_e[2 * a + q]) HIPCHK(hipMemsetAsync(c->psi_e[2 * a + q], 0, c->esz * sz[1], c->s));
         }
     }
-    const size_t db = c->esz * (size_t)std::max(c->n_disp, 1);
-    for (int q = 0; q < 3; ++q) {
-        if (c->disp_pd[q]) HIPCHK(hipMemsetAsync(c->disp_pd[q], 0, db, c->s));
-        if (c->disp_eold[q]) HIPCHK(hipMemsetAsync(c->disp_eold[q], 0, db, c->s));
-        if (c->disp_pl[q]) HIPCHK(hipMemsetAsync(c->disp_pl[q], 0, db * c->maxp, c->s));
-        if (c->disp_plold[q]) HIPCHK(hipMemsetAsync(c->disp_plold[q], 0, db * c->maxp, c->s));
-    }
+    if (c->disp_state) HIPCHK(hipMemsetAsync(c->disp_state, 0, disp_state_bytes(c), c->s));
     HIPCHK(hipStreamSynchronize(c->s));
     if (rows_y) *rows_y = br;
     if (xchunk) *xchunk = bx;

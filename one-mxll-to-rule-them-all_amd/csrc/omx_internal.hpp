@@ -66,8 +66,13 @@ struct YeeArgs {
     const uint8_t* tag;
     const int8_t* cls;
     const MedDev<T>* med;
-    T *eold[3], *pd[3], *pl[3], *plold[3];
-    long pstride;              // elements between pole planes of the dispersive state (the list's length)
+    // Drude / DL state of the listed dispersive cells, one block: component c
+    // at dstate + c * dcomp holds E_old, PD, then maxp PL planes and maxp
+    // PL_old planes, each pstride (the list's length) elements
+    T* dstate;
+    long dcomp;
+    long pstride;
+    int maxp;
     long disp0;                // position of the grid's first dispersive cell in the list the kernel reads
     T dt_mu0, dt_eps0;
     int nx, ny, nz, sy, sz;
@@ -146,12 +151,10 @@ struct omx3d_ctx {
     double* d_eps_class = nullptr;  // eps_r per class and component (energy)
     int maxp = 0;
     bool has_drude = false, has_dl = false;
-    // Drude / DL state per component, compact: entry p belongs to the
-    // dispersive list's cell p (PL and PL_old: maxp planes of n_disp)
-    void* disp_eold[3] = {};
-    void* disp_pd[3] = {};
-    void* disp_pl[3] = {};
-    void* disp_plold[3] = {};
+    // Drude / DL state, compact: entry p belongs to the dispersive list's
+    // cell p; per component E_old, PD, maxp PL and maxp PL_old planes of
+    // n_disp entries (YeeArgs::dstate)
+    void* disp_state = nullptr;
 
     // sources
     std::vector<omx::SourceDev> sources;

@@ -1005,9 +1005,13 @@ __device__ __forceinline__ void e_dispersive_cell(const YeeArgs<T>& a, T* __rest
     const T dep = a.dt_eps0;
     T rx, ry, rz, vx, vy, vz;
     e_curl(dep, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dex, cex, dey, dez, cey, cez, rx, ry, rz, vx, vy, vz);
-    ex = medium_update(ex, rx, vx, m0, a.med, p, a.eold[0], a.pd[0], a.pl[0], a.plold[0], a.pstride);
-    ey = medium_update(ey, ry, vy, m1, a.med, p, a.eold[1], a.pd[1], a.pl[1], a.plold[1], a.pstride);
-    ez = medium_update(ez, rz, vz, m2, a.med, p, a.eold[2], a.pd[2], a.pl[2], a.plold[2], a.pstride);
+    const long ps = a.pstride, pl0 = 2 * ps, plo0 = (2 + (long)a.maxp) * ps;
+    T* const st0 = a.dstate;
+    T* const st1 = st0 + a.dcomp;
+    T* const st2 = st1 + a.dcomp;
+    ex = medium_update(ex, rx, vx, m0, a.med, p, st0, st0 + ps, st0 + pl0, st0 + plo0, ps);
+    ey = medium_update(ey, ry, vy, m1, a.med, p, st1, st1 + ps, st1 + pl0, st1 + plo0, ps);
+    ez = medium_update(ez, rz, vz, m2, a.med, p, st2, st2 + ps, st2 + pl0, st2 + plo0, ps);
     if constexpr (CPML) {
         e_cpml_x(a, i, jk, dep, hzc, hzi, hyc, hyi, ey, ez);
         e_cpml_y(a, i, j, k, dep, hzc, hzj, hxc, hxj, ex, ez);
