@@ -137,6 +137,11 @@ int omx3d_destroy(omx3d_ctx* ctx);
  * order 2.  No reference counterpart (engine tuning). */
 int omx3d_set_tiling(omx3d_ctx* ctx, int32_t rows_y, int32_t xchunk, int32_t tile_order);
 int omx3d_get_tiling(omx3d_ctx* ctx, int32_t* rows_y, int32_t* xchunk, int32_t* tile_order);
+/* Addressing of the step kernels: 32 (default) uses 32-bit byte offsets when
+ * every array of the grid is below 4 GiB, 64-bit element indices otherwise;
+ * 64 always uses 64-bit element indices (same results).  No reference
+ * counterpart (engine tuning). */
+int omx3d_set_addressing(omx3d_ctx* ctx, int32_t offset_bits);
 /* CPML profiles and den arrays (mxll_3D_mod.f90:282-420), device FP64.
  * be,ce: npml; bh,ch: npml-1; den: den_ex(nx) den_ey(ny) den_ez(nz) den_hx(nx) den_hy(ny) den_hz(nz). */
 int omx3d_get_cpml(omx3d_ctx* ctx, double* be, double* ce, double* bh, double* ch, double* den);

@@ -194,7 +194,7 @@ TfsfLaunch tfsf_launch(omx3d_ctx* c, bool on) {
 // Whether every array of the grid (fields, psi, tags, dispersive state) is
 // addressed by 32-bit byte offsets: the field arrays are the largest.
 bool offsets32(const omx3d_ctx* c) {
-    return (unsigned long long)c->L.nalloc * c->esz < (1ull << 32);
+    return c->offset_bits == 32 && (unsigned long long)c->L.nalloc * c->esz < (1ull << 32);
 }
 
 // The H / E tile kernel instance for a grid: CPML, media, 32-bit offsets and
@@ -1307,6 +1307,13 @@ int omx3d_set_tiling(omx3d_ctx* c, int32_t rows_y, int32_t xchunk, int32_t tile_
         c->xchunk = xchunk;
     }
     if (tile_order >= 0) c->tile_order = tile_order;
+    return OMX_OK;
+}
+
+int omx3d_set_addressing(omx3d_ctx* c, int32_t offset_bits) {
+    CHECK_CTX(c);
+    if (offset_bits != 32 && offset_bits != 64) return fail(OMX_ERR_ARG, "offset_bits must be 32 or 64");
+    c->offset_bits = offset_bits;
     return OMX_OK;
 }
 

@@ -109,6 +109,7 @@ struct omx3d_ctx {
     hipStream_t s = nullptr;
     int rows_y = 1, xchunk = 4;
     int tile_order = 0;         // tile order of the H/E kernels (YeeArgs::tile_order)
+    int offset_bits = 32;       // 32: byte offsets where every array is below 4 GiB; 64: element indices
 
     void* f[6] = {};    // Ex Ey Ez Hx Hy Hz (current)
     void* f_alt[6] = {};        // second buffers of the fused step

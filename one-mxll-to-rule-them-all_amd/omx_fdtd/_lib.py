@@ -63,6 +63,7 @@ _SIGS = {
     "omx3d_destroy": ([_P], C.c_int),
     "omx3d_set_tiling": ([_P, _I, _I, _I], C.c_int),
     "omx3d_get_tiling": ([_P, _IP, _IP, _IP], C.c_int),
+    "omx3d_set_addressing": ([_P, _I], C.c_int),
     "omx3d_get_cpml": ([_P, _DP, _DP, _DP, _DP, _DP], C.c_int),
     "omx3d_set_media": ([_P, _I, C.POINTER(Medium), C.POINTER(C.c_int8)], C.c_int),
     "omx3d_set_media_shapes": ([_P, _I, C.POINTER(Medium), _I, C.POINTER(Shape)], C.c_int),

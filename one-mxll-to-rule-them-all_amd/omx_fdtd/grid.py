@@ -102,6 +102,10 @@ class Grid3D:
         """rows_y / xchunk 0 and tile_order -1 keep the current value."""
         L.call("omx3d_set_tiling", self.h, int(rows_y), int(xchunk), int(tile_order))
 
+    def set_addressing(self, offset_bits=32):
+        """32: byte offsets where every array is below 4 GiB (default); 64: element indices."""
+        L.call("omx3d_set_addressing", self.h, int(offset_bits))
+
     def tiling(self):
         r, x, o = C.c_int32(), C.c_int32(), C.c_int32()
         L.call("omx3d_get_tiling", self.h, C.byref(r), C.byref(x), C.byref(o))

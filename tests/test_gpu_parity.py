@@ -202,6 +202,32 @@ def test_tiling_variants_identical():
 
 
 @pytest.mark.parametrize("precision", [8, 4])
+def test_tiling_addressing_variants_identical_tfsf(precision):
+    """TF/SF corrections, CPML and a Drude-Lorentz sphere under several tile
+    shapes and orders, with 32-bit byte offsets and with 64-bit indices: the
+    same arithmetic on every cell, so bitwise equal records."""
+    case = cases.config_dl_sphere_tfsf(n=40, npml=6, steps=80, record_every=2, radius_nm=10.0)
+    ref = None
+    for rows, xc, order, bits in ((4, 4, 2, 32), (4, 4, 2, 64), (1, 3, 0, 32), (2, 16, 1, 64), (4, 1, 2, 32),
+                                  (2, 5, 0, 64)):
+        dr, dt = case.dr_dt
+        g = omx_fdtd.Grid3D(case.grid, case.npml, case.boundaries, dt, dr, precision=precision)
+        g.set_step_mode(1)
+        g.set_tiling(rows, xc, order)
+        g.set_addressing(bits)
+        cases.apply(case, g, library=G.library_medium, make_medium=G.medium)
+        g.run(1, case.steps, case.record_every)
+        rec = g.read_records()
+        print(f"rows {rows} xchunk {xc} order {order} offsets {bits}: max |rec| {np.abs(rec).max():.3e}")
+        if ref is None:
+            ref = rec
+            assert np.abs(ref).max() > 0
+        else:
+            np.testing.assert_array_equal(rec, ref)
+        g.kill()
+
+
+@pytest.mark.parametrize("precision", [8, 4])
 def test_fused_equals_separate_kernels_tfsf(precision):
     """No point sources: the fused pass and the separate kernels evaluate the
     same expressions in the same order; they differ only where the compiler
