@@ -1303,7 +1303,8 @@ int omx3d_set_tiling(omx3d_ctx* c, int32_t rows_y, int32_t xchunk, int32_t tile_
         c->rows_y = rows_y;
     }
     if (xchunk != 0) {
-        if (xchunk < 1) return fail(OMX_ERR_ARG, "xchunk must be >= 1");
+        // the tiles hold the chunk's per-plane coefficients one per lane
+        if (xchunk < 1 || xchunk > 64) return fail(OMX_ERR_ARG, "xchunk must be 1..64");
         c->xchunk = xchunk;
     }
     if (tile_order >= 0) c->tile_order = tile_order;

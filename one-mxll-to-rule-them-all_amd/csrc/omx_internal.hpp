@@ -82,7 +82,8 @@ struct YeeArgs {
     unsigned cpml;             // bit f set: CPML on face f (x-lo, x-hi, y-lo, y-hi, z-lo, z-hi)
     int disp_tag0;             // tags >= disp_tag0: classes with a Drude / Drude-Lorentz component
     int ntz, nty, xchunk, rows;
-    int tile_order;            // 0: tiles in block order, 1: one contiguous slab of tiles per XCD
+    int tile_order;            // 0: tiles in block order, 1: one contiguous slab of tiles per XCD,
+                               // 2: whole row blocks (all x chunks) dealt to the XCDs
 };
 
 struct SourceDev {

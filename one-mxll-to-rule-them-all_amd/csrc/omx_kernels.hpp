@@ -75,6 +75,27 @@ __device__ __forceinline__ T ldc(const T* p, int i) {
     return ((const __attribute__((address_space(4))) T*)p)[i];
 }
 
+// Per-plane coefficients of a tile's x chunk, one plane per lane: lane l
+// holds c[min(i0 + l, i1)].  Loaded once per tile while all 64 lanes of the
+// wavefront are active (chunks are at most 64 planes), then read for plane
+// i with plane_coef(v, i - i0), a v_readlane: the plane loop issues no
+// memory access for them, and nothing waits for them after a branch.
+template <typename T>
+__device__ __forceinline__ T plane_table(const T* c, int i0, int i1) {
+    return gld(c, min(i0 + (int)threadIdx.x, i1));
+}
+template <typename T>
+__device__ __forceinline__ T plane_coef(T v, int l) {
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+    } else {
+        const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+        const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+        return __builtin_bit_cast(T, ((unsigned long long)hi << 32) | lo);
+    }
+}
+
 // Offsets of a cell into the field, psi and tag arrays of the H/E tiles.
 // B32: 32-bit byte offsets (every array of the grid below 4 GiB), so that a
 // load takes a scalar base and one 32-bit vector offset with no 64-bit
@@ -607,9 +628,10 @@ __device__ __forceinline__ void h_tile(const YeeArgs<T>& a, const TfsfLaunch& tf
     const int tx = r / a.nty;
     const int k = tz * kLanes + threadIdx.x + 1;
     const int j = ty * a.rows + __builtin_amdgcn_readfirstlane(threadIdx.y) + 1;
-    if (k > a.nz || j > a.ny) return;
     const int i0 = tx * a.xchunk + 1;
     const int i1 = min(i0 + a.xchunk - 1, a.nx);
+    const T tcx = plane_table(sizeof(T) == 4 ? a.cdhx : a.den_hx, i0, i1);
+    if (k > a.nz || j > a.ny) return;
     const I sx = O::of(a.sx), sz = O::of(a.sz), s1 = O::of(1);
     const long jke = (long)j * a.sz + k + a.off;
     I o = O::of((long)i0 * a.sx + jke);
@@ -634,7 +656,7 @@ __device__ __forceinline__ void h_tile(const YeeArgs<T>& a, const TfsfLaunch& tf
             const T ezj = O::ld(a.ez, o + sz), eyk = O::ld(a.ey, o + s1);
             const T exc = O::ld(a.ex, o), exk = O::ld(a.ex, o + s1), exj = O::ld(a.ex, o + sz);
             T hx = O::ld(a.hx, o), hy = O::ld(a.hy, o), hz = O::ld(a.hz, o);
-            const T cx = sizeof(T) == 4 ? gld(a.cdhx, i) : gld(a.den_hx, i);
+            const T cx = plane_coef(tcx, i - i0);
             h_bulk(dmu, cx, dhy, dhz, chy, chz, ezc, ezn, ezj, eyc, eyn, eyk, exc, exk, exj, hx, hy, hz);
             if (tfw && i >= ia && i <= ib) { hy = (T)((double)hy + cy); hz = (T)((double)hz + cz); }
             O::st(a.hx, o, hx); O::st(a.hy, o, hy); O::st(a.hz, o, hz);
@@ -660,7 +682,7 @@ __device__ __forceinline__ void h_tile(const YeeArgs<T>& a, const TfsfLaunch& tf
         const T ezj = O::ld(a.ez, o + sz), eyk = O::ld(a.ey, o + s1);
         const T exc = O::ld(a.ex, o), exk = O::ld(a.ex, o + s1), exj = O::ld(a.ex, o + sz);
         T hx = O::ld(a.hx, o), hy = O::ld(a.hy, o), hz = O::ld(a.hz, o);
-        const T cx = sizeof(T) == 4 ? gld(a.cdhx, i) : gld(a.den_hx, i);
+        const T cx = plane_coef(tcx, i - i0);
         const Face fx = face_h(cp & 1u, cp & 2u, i, a.nx, npml);
         const I px = O::of((long)fx.slot * a.sx) + jk;
         T px0 = (T)0, px1 = (T)0, xb = (T)0, xc = (T)0, py0 = (T)0, py1 = (T)0, pz0 = (T)0, pz1 = (T)0;
@@ -875,9 +897,11 @@ __device__ __forceinline__ void e_tile(const YeeArgs<T>& a, const TfsfLaunch& tf
     const int tx = r / a.nty;
     const int k = tz * kLanes + threadIdx.x + 1;
     const int j = ty * a.rows + __builtin_amdgcn_readfirstlane(threadIdx.y) + 1;
-    if (k > a.nz || j > a.ny) return;
     const int i0 = tx * a.xchunk + 1;
     const int i1 = min(i0 + a.xchunk - 1, a.nx);
+    const T tdex = plane_table(a.den_ex, i0, i1);
+    const T tcex = sizeof(T) == 4 ? plane_table(a.cdex, i0, i1) : (T)0;
+    if (k > a.nz || j > a.ny) return;
     const I sx = O::of(a.sx), sz = O::of(a.sz), s1 = O::of(1);
     const long jke = (long)j * a.sz + k + a.off;
     I o = O::of((long)i0 * a.sx + jke);
@@ -905,7 +929,7 @@ __device__ __forceinline__ void e_tile(const YeeArgs<T>& a, const TfsfLaunch& tf
             const T hyk = O::ld(a.hy, o - s1), hxk = O::ld(a.hx, o - s1);
             T ex = O::ld(a.ex, o), ey = O::ld(a.ey, o), ez = O::ld(a.ez, o);
             const int tg = tags ? O::tag(a.tag, o) : 0;
-            const T dex = gld(a.den_ex, i), cex = sizeof(T) == 4 ? gld(a.cdex, i) : (T)0;
+            const T dex = plane_coef(tdex, i - i0), cex = sizeof(T) == 4 ? plane_coef(tcex, i - i0) : (T)0;
             if (MED != 1 || tg < a.disp_tag0) {
                 e_bulk<T, MED>(a, tg, dep, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dex, cex, dey, dez, cey,
                                cez, ex, ey, ez);
@@ -937,32 +961,32 @@ __device__ __forceinline__ void e_tile(const YeeArgs<T>& a, const TfsfLaunch& tf
         const T hyk = O::ld(a.hy, o - s1), hxk = O::ld(a.hx, o - s1);
         T ex = O::ld(a.ex, o), ey = O::ld(a.ey, o), ez = O::ld(a.ez, o);
         const int tg = tags ? O::tag(a.tag, o) : 0;
-        const T dex = gld(a.den_ex, i), cex = sizeof(T) == 4 ? gld(a.cdex, i) : (T)0;
-        if (MED == 1 && tg >= a.disp_tag0) {
-            hzi = hzc; hyi = hyc;
-            continue;
-        }
+        const T dex = plane_coef(tdex, i - i0), cex = sizeof(T) == 4 ? plane_coef(tcex, i - i0) : (T)0;
         const Face fx = face_e(cp & 1u, cp & 2u, i, a.nx, npml);
         const I px = O::of((long)fx.slot * a.sx) + jk;
         T px0 = (T)0, px1 = (T)0, xb = (T)0, xc = (T)0, py0 = (T)0, py1 = (T)0, pz0 = (T)0, pz1 = (T)0;
         if (fx.slot >= 0) { px0 = O::ld(a.pe[0], px); px1 = O::ld(a.pe[1], px); xb = gld(a.be, fx.c); xc = gld(a.ced, fx.c); }
         if (fy.slot >= 0) { py0 = O::ld(a.pe[2], py); py1 = O::ld(a.pe[3], py); }
         if (fz.slot >= 0) { pz0 = O::ld(a.pe[4], pz); pz1 = O::ld(a.pe[5], pz); }
+        // cells of dispersive classes belong to the dispersive workgroups:
+        // their loads above are unused and nothing of theirs is stored; the
+        // psi loads are issued with the fields, ahead of the tag test
+        const bool own = MED != 1 || tg < a.disp_tag0;
         e_bulk<T, MED>(a, tg, dep, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dex, cex, dey, dez, cey, cez,
                        ex, ey, ez);
         if (fx.slot >= 0) {
             const T p0 = psi_step(px0, xb, xc, hzi - hzc), p1 = psi_step(px1, xb, xc, hyc - hyi);
-            O::st(a.pe[0], px, p0); O::st(a.pe[1], px, p1);
+            if (own) { O::st(a.pe[0], px, p0); O::st(a.pe[1], px, p1); }
             ey = ey + dep * p0; ez = ez + dep * p1;
         }
         if (fy.slot >= 0) {
             const T p0 = psi_step(py0, yb, yc, hzc - hzj), p1 = psi_step(py1, yb, yc, hxj - hxc);
-            O::st(a.pe[2], py, p0); O::st(a.pe[3], py, p1);
+            if (own) { O::st(a.pe[2], py, p0); O::st(a.pe[3], py, p1); }
             ex = ex + dep * p0; ez = ez + dep * p1;
         }
         if (fz.slot >= 0) {
             const T p0 = psi_step(pz0, zb, zc, hyk - hyc), p1 = psi_step(pz1, zb, zc, hxc - hxk);
-            O::st(a.pe[4], pz, p0); O::st(a.pe[5], pz, p1);
+            if (own) { O::st(a.pe[4], pz, p0); O::st(a.pe[5], pz, p1); }
             ex = ex + dep * p0; ey = ey + dep * p1;
         }
         if (tfw) {
@@ -970,7 +994,7 @@ __device__ __forceinline__ void e_tile(const YeeArgs<T>& a, const TfsfLaunch& tf
             if (i == ti0) ez = (T)((double)ez + tcl);
             if (i == ti1) ez = (T)((double)ez + tch);
         }
-        O::st(a.ex, o, ex); O::st(a.ey, o, ey); O::st(a.ez, o, ez);
+        if (own) { O::st(a.ex, o, ex); O::st(a.ey, o, ey); O::st(a.ez, o, ez); }
         hzi = hzc; hyi = hyc;
     }
 }
