@@ -128,7 +128,7 @@ int omx_medium_library(const char* name, double eps_r, omx_medium* out);
 int omx3d_create(const omx3d_desc* desc, omx3d_ctx** out);
 int omx3d_destroy(omx3d_ctx* ctx);
 /* Tile shape of the H/E kernels: y rows per workgroup (1, 2 or 4), x planes
- * marched per workgroup (1..64; 0 keeps the current values) and the order in
+ * marched per workgroup (>= 1; 0 keeps the current values) and the order in
  * which workgroups take tiles: 0 block order (consecutive workgroups,
  * spread over the XCDs, cover consecutive memory), 1 one contiguous slab of
  * tiles per XCD, 2 block order by whole row blocks (the z tiles of a row

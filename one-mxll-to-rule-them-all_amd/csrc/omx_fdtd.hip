@@ -397,7 +397,7 @@ size_t psi_h_elems(const omx3d_ctx* c, int axis) {
     const size_t A = 128 / c->esz;
     if (axis == 0) return c->nsh * c->L.sx + A;
     if (axis == 1) return (size_t)(c->d.nx + 2) * c->nsh * c->L.sz + A;
-    return (size_t)(c->d.nx + 2) * c->L.sy * c->nsh;
+    return (size_t)zpsi_elems(c->L.sy, c->d.nx, c->d.npml);
 }
 
 // Second field / H-psi buffers of the fused step: a copy of the current ones
@@ -451,7 +451,7 @@ bool fused_applies(omx3d_ctx* c) {
     // the fused kernel addresses field and z-face psi arrays by 32-bit byte
     // offsets, and handles one z face per cell (the faces do not meet)
     if ((unsigned long long)c->L.nalloc * c->esz >= (1ull << 32)) return false;
-    if ((unsigned long long)(c->d.nx + 2) * c->L.sy * c->nse * c->esz >= (1ull << 32)) return false;
+    if ((unsigned long long)zpsi_elems(c->L.sy, c->d.nx, c->d.npml) * c->esz >= (1ull << 32)) return false;
     if ((c->cpml_mask & 48u) && c->d.nz < 2 * c->d.npml + 1) return false;
     return c->src_in_dispersive == 0;
 }
@@ -1255,7 +1255,7 @@ int omx3d_create(const omx3d_desc* desc, omx3d_ctx** out) {
         const size_t A2 = (size_t)A;
         const size_t xs[2] = {c->nsh * L.sx + A2, c->nse * L.sx + A2};
         const size_t ys[2] = {(size_t)(d.nx + 2) * c->nsh * L.sz + A2, (size_t)(d.nx + 2) * c->nse * L.sz + A2};
-        const size_t zs[2] = {(size_t)(d.nx + 2) * L.sy * c->nsh, (size_t)(d.nx + 2) * L.sy * c->nse};
+        const size_t zs[2] = {(size_t)zpsi_elems(L.sy, d.nx, d.npml), (size_t)zpsi_elems(L.sy, d.nx, d.npml)};
         for (int a = 0; a < 3; ++a) {
             if (!((mask >> (2 * a)) & 3u)) continue;
             const size_t* sz = a == 0 ? xs : (a == 1 ? ys : zs);
@@ -1303,8 +1303,7 @@ int omx3d_set_tiling(omx3d_ctx* c, int32_t rows_y, int32_t xchunk, int32_t tile_
         c->rows_y = rows_y;
     }
     if (xchunk != 0) {
-        // the tiles hold the chunk's per-plane coefficients one per lane
-        if (xchunk < 1 || xchunk > 64) return fail(OMX_ERR_ARG, "xchunk must be 1..64");
+        if (xchunk < 1) return fail(OMX_ERR_ARG, "xchunk must be >= 1");
         c->xchunk = xchunk;
     }
     if (tile_order >= 0) c->tile_order = tile_order;
@@ -1898,7 +1897,7 @@ int omx3d_autotune(omx3d_ctx* c, int32_t steps, int32_t* rows_y, int32_t* xchunk
     const size_t A = 128 / c->esz;
     const size_t xs[2] = {c->nsh * c->L.sx + A, c->nse * c->L.sx + A};
     const size_t ys[2] = {(size_t)(c->d.nx + 2) * c->nsh * c->L.sz + A, (size_t)(c->d.nx + 2) * c->nse * c->L.sz + A};
-    const size_t zs[2] = {(size_t)(c->d.nx + 2) * c->L.sy * c->nsh, (size_t)(c->d.nx + 2) * c->L.sy * c->nse};
+    const size_t zs[2] = {(size_t)zpsi_elems(c->L.sy, c->d.nx, c->d.npml), (size_t)zpsi_elems(c->L.sy, c->d.nx, c->d.npml)};
     for (int a = 0; a < 3; ++a) {
         const size_t* sz = a == 0 ? xs : (a == 1 ? ys : zs);
         for (int q = 0; q < 2; ++q) {

@@ -7,6 +7,8 @@
 // consecutive tiles (which share halo rows) land on the same XCD.
 #pragma once
 
+#include <type_traits>
+
 #include "omx_internal.hpp"
 
 namespace omx {
@@ -69,31 +71,13 @@ template <typename T>
 __device__ __forceinline__ void gst(T* p, long i, T v) {
     ((OMX_GLOBAL T*)p)[i] = v;
 }
-// Uniform read of a table the kernel never writes (scalar load).
+// Uniform read of a table the kernel never writes (scalar load).  The tiles
+// read their per-plane x coefficients this way: a scalar load does not wait
+// on the vector memory counter, so a load the compiler places after a
+// branch costs a scalar-cache round trip, not a memory round trip.
 template <typename T>
 __device__ __forceinline__ T ldc(const T* p, int i) {
     return ((const __attribute__((address_space(4))) T*)p)[i];
-}
-
-// Per-plane coefficients of a tile's x chunk, one plane per lane: lane l
-// holds c[min(i0 + l, i1)].  Loaded once per tile while all 64 lanes of the
-// wavefront are active (chunks are at most 64 planes), then read for plane
-// i with plane_coef(v, i - i0), a v_readlane: the plane loop issues no
-// memory access for them, and nothing waits for them after a branch.
-template <typename T>
-__device__ __forceinline__ T plane_table(const T* c, int i0, int i1) {
-    return gld(c, min(i0 + (int)threadIdx.x, i1));
-}
-template <typename T>
-__device__ __forceinline__ T plane_coef(T v, int l) {
-    if constexpr (sizeof(T) == 4) {
-        return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
-    } else {
-        const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
-        const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
-        return __builtin_bit_cast(T, ((unsigned long long)hi << 32) | lo);
-    }
 }
 
 // Offsets of a cell into the field, psi and tag arrays of the H/E tiles.
@@ -463,31 +447,6 @@ __device__ __forceinline__ void h_cpml_y(const YeeArgs<T>& a, T* const* pho, boo
     }
 }
 
-// z faces: psiz_Hxz, psiz_Hyz
-template <typename T>
-__device__ __forceinline__ void h_cpml_z(const YeeArgs<T>& a, T* const* pho, bool st, int i, int j, int k, T dmu,
-                                         T eyc, T eyk, T exc, T exk, T& hx, T& hy) {
-    const unsigned cp = a.cpml;
-    const int npml = a.npml;
-    const int nsh = 2 * (npml - 1);
-    if ((cp & 16u) && k <= npml - 1) {
-        const int c = k;
-        const long ps = ((long)i * a.sy + j) * nsh + (k - 1);
-        const T p0 = psi_step(a.ph[4][ps], a.bh[c], a.chd[c], eyk - eyc);
-        const T p1 = psi_step(a.ph[5][ps], a.bh[c], a.chd[c], exc - exk);
-        if (st) { pho[4][ps] = p0; pho[5][ps] = p1; }
-        hx = hx + dmu * p0; hy = hy + dmu * p1;
-    }
-    if ((cp & 32u) && k >= a.nz + 1 - npml && k <= a.nz - 1) {
-        const int c = a.nz - k;
-        const long ps = ((long)i * a.sy + j) * nsh + (npml - 2 + c);
-        const T p0 = psi_step(a.ph[4][ps], a.bh[c], a.chd[c], eyk - eyc);
-        const T p1 = psi_step(a.ph[5][ps], a.bh[c], a.chd[c], exc - exk);
-        if (st) { pho[4][ps] = p0; pho[5][ps] = p1; }
-        hx = hx + dmu * p0; hy = hy + dmu * p1;
-    }
-}
-
 // CPML face of one axis for a cell: psi slot and coefficient index (slot
 // < 0: the cell is in no face of that axis).  H faces: low 1..npml-1 (slot
 // n-1, coefficient n), high n_a+1-npml..n_a-1 (slot npml-2+c, c = n_a-n).  E
@@ -505,6 +464,71 @@ __device__ __forceinline__ Face face_e(bool lo_on, bool hi_on, int n, int na, in
     if (lo_on && n >= 2 && n <= npml) return {n - 1, n};
     if (hi_on && n >= na + 1 - npml && n <= na - 1) return {npml - 1 + na + 1 - n, na + 1 - n};
     return {-1, 0};
+}
+
+// z-face psi (psiz_Hxz/Hyz, psiz_Exz/Eyz).  Per side (0 low, 1 high), row j
+// and plane i, the side's npml-1 cells m are contiguous, and the planes of a
+// row follow one another:
+//   index = ((side * sy + j) * (nx + 2) + i) * (npml - 1) + m.
+// H faces: low k = 1..npml-1 (m = k-1), high k = nz+1-npml..nz-1 (m =
+// nz-1-k); E faces: low k = 2..npml (m = k-2), high as for H.  The
+// coefficient index is m+1 (H) and m+2 (E) on both sides.
+__host__ __device__ inline long zpsi_index(int side, int j, int i, int m, int sy, int nx, int npml) {
+    return (((long)side * sy + j) * (nx + 2) + i) * (npml - 1) + m;
+}
+// Elements of one z-face psi array.
+__host__ __device__ inline long zpsi_elems(int sy, int nx, int npml) { return 2L * sy * (nx + 2) * (npml - 1); }
+
+// Value of v in lane byte_addr/4 (ds_bpermute; both lanes active).
+template <typename T>
+__device__ __forceinline__ T lane_pull(int byte_addr, T v) {
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_ds_bpermute(byte_addr, __builtin_bit_cast(int, v)));
+    } else {
+        const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+        const unsigned lo = (unsigned)__builtin_amdgcn_ds_bpermute(byte_addr, (int)(unsigned)b);
+        const unsigned hi = (unsigned)__builtin_amdgcn_ds_bpermute(byte_addr, (int)(unsigned)(b >> 32));
+        return __builtin_bit_cast(T, ((unsigned long long)hi << 32) | lo);
+    }
+}
+
+// z-face psi of a wavefront, kept in registers a batch of planes at a time.
+// The wavefront's face lanes (cells of its tile [kt0, kt1] in the low face
+// [zl0, zl1] or the high face [zh0, zh1]; nf of them, ranked q = 0..nf-1 in
+// lane order) are served by storage lanes: lane l < P * nf (P = nact / nf
+// planes per batch, nact the tile's active lanes) holds rank l % nf of
+// batch plane l / nf.  A batch is loaded and stored with one access per
+// array, the face lanes read and return their values by ds_bpermute each
+// plane.  (One access of nf lanes per plane and array costs the memory
+// system more than its bytes; the batched access does not.)
+struct ZBatch {
+    int nf, P;          // face lanes of the wavefront and planes per batch (uniform)
+    int q;              // this lane's face rank, -1: not a face lane
+    int m, side;        // this face lane's cell in its side
+    int sp, slane;      // storage: batch plane held by this lane (-1: none), its face lane
+    long sidx;          // psi index of this storage lane in the first batch
+};
+__device__ __forceinline__ ZBatch z_batch(int lane, int kt0, int kt1, int zl0, int zl1, int zh0, int zh1, int klo0,
+                                          int j, int i0, int nz, int sy, int nx, int npml) {
+    ZBatch z;
+    const int a0 = max(kt0, zl0), a1 = min(kt1, zl1), b0 = max(kt0, zh0), b1 = min(kt1, zh1);
+    const int nlo = max(0, a1 - a0 + 1), nhi = max(0, b1 - b0 + 1);
+    const int la = a0 - kt0, ha = b0 - kt0;
+    z.nf = nlo + nhi;
+    z.P = z.nf > 0 ? (kt1 - kt0 + 1) / z.nf : 0;
+    z.q = -1; z.m = 0; z.side = 0;
+    if (lane >= la && lane < la + nlo) { z.q = lane - la; z.side = 0; z.m = kt0 + lane - klo0; }
+    else if (lane >= ha && lane < ha + nhi) { z.q = nlo + lane - ha; z.side = 1; z.m = nz - 1 - (kt0 + lane); }
+    z.sp = -1; z.slane = 0; z.sidx = 0;
+    if (z.nf > 0 && lane < z.P * z.nf) {
+        z.sp = lane / z.nf;
+        const int sq = lane - z.sp * z.nf;
+        z.slane = sq < nlo ? la + sq : ha + (sq - nlo);
+        const int ks = kt0 + z.slane;
+        const int side = sq < nlo ? 0 : 1;
+        z.sidx = zpsi_index(side, j, i0 + z.sp, side == 0 ? ks - klo0 : nz - 1 - ks, sy, nx, npml);
+    }
+    return z;
 }
 
 // TF/SF plane wave, x-polarised, travelling along +z.  The incident Ex (at
@@ -630,7 +654,7 @@ __device__ __forceinline__ void h_tile(const YeeArgs<T>& a, const TfsfLaunch& tf
     const int j = ty * a.rows + __builtin_amdgcn_readfirstlane(threadIdx.y) + 1;
     const int i0 = tx * a.xchunk + 1;
     const int i1 = min(i0 + a.xchunk - 1, a.nx);
-    const T tcx = plane_table(sizeof(T) == 4 ? a.cdhx : a.den_hx, i0, i1);
+    const T* const cxs = sizeof(T) == 4 ? a.cdhx : a.den_hx;
     if (k > a.nz || j > a.ny) return;
     const I sx = O::of(a.sx), sz = O::of(a.sz), s1 = O::of(1);
     const long jke = (long)j * a.sz + k + a.off;
@@ -643,20 +667,19 @@ __device__ __forceinline__ void h_tile(const YeeArgs<T>& a, const TfsfLaunch& tf
     double cy = 0.0, cz = 0.0;
     int ia = 0, ib = -1;
     const bool tfw = TF && tfsf_h_lane(a, tf, i0, i1, j, k, kt0, kt1, cy, cz, ia, ib);
-    bool faces = false;
+    bool xyf = false, zf = false;
     if (CPML) {
         const unsigned cp = a.cpml;
-        const int k0 = kt0, k1 = kt1;
-        faces = in_faces_h(cp & 1u, cp & 2u, i0, i1, a.nx, a.npml) || in_faces_h(cp & 4u, cp & 8u, j, j, a.ny, a.npml) ||
-                in_faces_h(cp & 16u, cp & 32u, k0, k1, a.nz, a.npml);
+        xyf = in_faces_h(cp & 1u, cp & 2u, i0, i1, a.nx, a.npml) || in_faces_h(cp & 4u, cp & 8u, j, j, a.ny, a.npml);
+        zf = in_faces_h(cp & 16u, cp & 32u, kt0, kt1, a.nz, a.npml);
     }
-    if (!faces) {
+    if (!xyf && !zf) {
         for (int i = i0; i <= i1; ++i, o += sx) {
             const T ezn = O::ld(a.ez, o + sx), eyn = O::ld(a.ey, o + sx);
             const T ezj = O::ld(a.ez, o + sz), eyk = O::ld(a.ey, o + s1);
             const T exc = O::ld(a.ex, o), exk = O::ld(a.ex, o + s1), exj = O::ld(a.ex, o + sz);
             T hx = O::ld(a.hx, o), hy = O::ld(a.hy, o), hz = O::ld(a.hz, o);
-            const T cx = plane_coef(tcx, i - i0);
+            const T cx = ldc(cxs, i);
             h_bulk(dmu, cx, dhy, dhz, chy, chz, ezc, ezn, ezj, eyc, eyn, eyk, exc, exk, exj, hx, hy, hz);
             if (tfw && i >= ia && i <= ib) { hy = (T)((double)hy + cy); hz = (T)((double)hz + cz); }
             O::st(a.hx, o, hx); O::st(a.hy, o, hy); O::st(a.hz, o, hz);
@@ -664,51 +687,84 @@ __device__ __forceinline__ void h_tile(const YeeArgs<T>& a, const TfsfLaunch& tf
         }
         return;
     }
-    const unsigned cp = a.cpml;
-    const int npml = a.npml, nsh = 2 * (npml - 1);
-    const Face fy = face_h(cp & 4u, cp & 8u, j, a.ny, npml);
-    const Face fz = face_h(cp & 16u, cp & 32u, k, a.nz, npml);
-    T yb = (T)0, yc = (T)0, zb = (T)0, zc = (T)0;
-    if (fy.slot >= 0) { yb = gld(a.bh, fy.c); yc = gld(a.chd, fy.c); }
-    if (fz.slot >= 0) { zb = gld(a.bh, fz.c); zc = gld(a.chd, fz.c); }
-    // psi of plane i: x (fx.slot*sx + jk), y (i*nsh*sz + slot*sz + k + off),
-    // z (i*sy*nsh + j*nsh + slot); offsets of faces a cell is not in are unused
-    const I jk = O::of(jke);
-    const I ystep = O::of((long)nsh * a.sz), zstep = O::of((long)a.sy * nsh);
-    I py = O::of((long)i0 * nsh * a.sz + (long)fy.slot * a.sz + k + a.off);
-    I pz = O::of(((long)i0 * a.sy + j) * nsh + fz.slot);
-    for (int i = i0; i <= i1; ++i, o += sx, py += ystep, pz += zstep) {
-        const T ezn = O::ld(a.ez, o + sx), eyn = O::ld(a.ey, o + sx);
-        const T ezj = O::ld(a.ez, o + sz), eyk = O::ld(a.ey, o + s1);
-        const T exc = O::ld(a.ex, o), exk = O::ld(a.ex, o + s1), exj = O::ld(a.ex, o + sz);
-        T hx = O::ld(a.hx, o), hy = O::ld(a.hy, o), hz = O::ld(a.hz, o);
-        const T cx = plane_coef(tcx, i - i0);
-        const Face fx = face_h(cp & 1u, cp & 2u, i, a.nx, npml);
-        const I px = O::of((long)fx.slot * a.sx) + jk;
-        T px0 = (T)0, px1 = (T)0, xb = (T)0, xc = (T)0, py0 = (T)0, py1 = (T)0, pz0 = (T)0, pz1 = (T)0;
-        if (fx.slot >= 0) { px0 = O::ld(a.ph[0], px); px1 = O::ld(a.ph[1], px); xb = gld(a.bh, fx.c); xc = gld(a.chd, fx.c); }
-        if (fy.slot >= 0) { py0 = O::ld(a.ph[2], py); py1 = O::ld(a.ph[3], py); }
-        if (fz.slot >= 0) { pz0 = O::ld(a.ph[4], pz); pz1 = O::ld(a.ph[5], pz); }
-        h_bulk(dmu, cx, dhy, dhz, chy, chz, ezc, ezn, ezj, eyc, eyn, eyk, exc, exk, exj, hx, hy, hz);
-        if (fx.slot >= 0) {
-            const T p0 = psi_step(px0, xb, xc, ezn - ezc), p1 = psi_step(px1, xb, xc, eyc - eyn);
-            O::st(a.ph[0], px, p0); O::st(a.ph[1], px, p1);
-            hy = hy + dmu * p0; hz = hz + dmu * p1;
+    // XY false: the wavefront meets only z faces (per lane), so the x and y
+    // face code is compiled out of its loop
+    const auto face_loop = [&](auto xy_const) {
+        constexpr bool XY = decltype(xy_const)::value;
+        const unsigned cp = a.cpml;
+        const int npml = a.npml, nsh = 2 * (npml - 1);
+        const Face fy = XY ? face_h(cp & 4u, cp & 8u, j, a.ny, npml) : Face{-1, 0};
+        T yb = (T)0, yc = (T)0, zbc = (T)0, zcc = (T)0;
+        if (fy.slot >= 0) { yb = gld(a.bh, fy.c); yc = gld(a.chd, fy.c); }
+        // z faces: low k = 1..npml-1, high k = nz+1-npml..nz-1
+        const ZBatch zw = z_batch((int)threadIdx.x, kt0, kt1, 1, (cp & 16u) ? npml - 1 : 0, a.nz + 1 - npml,
+                                  (cp & 32u) ? a.nz - 1 : a.nz - npml, 1, j, i0, a.nz, a.sy, a.nx, npml);
+        if (zw.q >= 0) { zbc = gld(a.bh, zw.m + 1); zcc = gld(a.chd, zw.m + 1); }
+        const long zadv = (long)zw.P * (npml - 1);
+        long zidx = zw.sidx;
+        T zs0 = (T)0, zs1 = (T)0;
+        int zb0 = i0;
+        // psi of plane i: x (fx.slot*sx + jk), y (i*nsh*sz + slot*sz + k + off);
+        // offsets of faces a cell is not in are unused
+        const I jk = O::of(jke);
+        const I ystep = O::of((long)nsh * a.sz);
+        I py = O::of((long)i0 * nsh * a.sz + (long)fy.slot * a.sz + k + a.off);
+        for (int i = i0; i <= i1; ++i, o += sx, py += ystep) {
+            int zp = i - zb0;
+            if (zw.nf > 0) {
+                if (zp == zw.P) {
+                    // the previous batch is complete: store it, start the next
+                    if (zw.sp >= 0) { O::st(a.ph[4], O::of(zidx), zs0); O::st(a.ph[5], O::of(zidx), zs1); }
+                    zb0 = i; zp = 0; zidx += zadv;
+                }
+                if (zp == 0 && zw.sp >= 0 && zb0 + zw.sp <= i1) {
+                    zs0 = O::ld(a.ph[4], O::of(zidx)); zs1 = O::ld(a.ph[5], O::of(zidx));
+                }
+            }
+            const T ezn = O::ld(a.ez, o + sx), eyn = O::ld(a.ey, o + sx);
+            const T ezj = O::ld(a.ez, o + sz), eyk = O::ld(a.ey, o + s1);
+            const T exc = O::ld(a.ex, o), exk = O::ld(a.ex, o + s1), exj = O::ld(a.ex, o + sz);
+            T hx = O::ld(a.hx, o), hy = O::ld(a.hy, o), hz = O::ld(a.hz, o);
+            const T cx = ldc(cxs, i);
+            const Face fx = XY ? face_h(cp & 1u, cp & 2u, i, a.nx, npml) : Face{-1, 0};
+            const I px = O::of((long)fx.slot * a.sx) + jk;
+            T px0 = (T)0, px1 = (T)0, xb = (T)0, xc = (T)0, py0 = (T)0, py1 = (T)0;
+            if (fx.slot >= 0) {
+                px0 = O::ld(a.ph[0], px); px1 = O::ld(a.ph[1], px);
+                xb = gld(a.bh, fx.c); xc = gld(a.chd, fx.c);
+            }
+            if (fy.slot >= 0) { py0 = O::ld(a.ph[2], py); py1 = O::ld(a.ph[3], py); }
+            h_bulk(dmu, cx, dhy, dhz, chy, chz, ezc, ezn, ezj, eyc, eyn, eyk, exc, exk, exj, hx, hy, hz);
+            if (fx.slot >= 0) {
+                const T p0 = psi_step(px0, xb, xc, ezn - ezc), p1 = psi_step(px1, xb, xc, eyc - eyn);
+                O::st(a.ph[0], px, p0); O::st(a.ph[1], px, p1);
+                hy = hy + dmu * p0; hz = hz + dmu * p1;
+            }
+            if (fy.slot >= 0) {
+                const T p0 = psi_step(py0, yb, yc, ezc - ezj), p1 = psi_step(py1, yb, yc, exj - exc);
+                O::st(a.ph[2], py, p0); O::st(a.ph[3], py, p1);
+                hx = hx + dmu * p0; hz = hz + dmu * p1;
+            }
+            if (zw.nf > 0) {
+                // face lanes take their psi from the storage lanes and return
+                // the new values to them (every active lane takes part)
+                const int from = (zp * zw.nf + max(zw.q, 0)) * 4;
+                const T p0 = psi_step(lane_pull(from, zs0), zbc, zcc, eyk - eyc);
+                const T p1 = psi_step(lane_pull(from, zs1), zbc, zcc, exc - exk);
+                if (zw.q >= 0) { hx = hx + dmu * p0; hy = hy + dmu * p1; }
+                const T n0 = lane_pull(zw.slane * 4, p0), n1 = lane_pull(zw.slane * 4, p1);
+                if (zw.sp == zp) { zs0 = n0; zs1 = n1; }
+            }
+            if (tfw && i >= ia && i <= ib) { hy = (T)((double)hy + cy); hz = (T)((double)hz + cz); }
+            O::st(a.hx, o, hx); O::st(a.hy, o, hy); O::st(a.hz, o, hz);
+            ezc = ezn; eyc = eyn;
         }
-        if (fy.slot >= 0) {
-            const T p0 = psi_step(py0, yb, yc, ezc - ezj), p1 = psi_step(py1, yb, yc, exj - exc);
-            O::st(a.ph[2], py, p0); O::st(a.ph[3], py, p1);
-            hx = hx + dmu * p0; hz = hz + dmu * p1;
+        if (zw.nf > 0 && zw.sp >= 0 && zb0 + zw.sp <= i1) {
+            O::st(a.ph[4], O::of(zidx), zs0); O::st(a.ph[5], O::of(zidx), zs1);
         }
-        if (fz.slot >= 0) {
-            const T p0 = psi_step(pz0, zb, zc, eyk - eyc), p1 = psi_step(pz1, zb, zc, exc - exk);
-            O::st(a.ph[4], pz, p0); O::st(a.ph[5], pz, p1);
-            hx = hx + dmu * p0; hy = hy + dmu * p1;
-        }
-        if (tfw && i >= ia && i <= ib) { hy = (T)((double)hy + cy); hz = (T)((double)hz + cz); }
-        O::st(a.hx, o, hx); O::st(a.hy, o, hy); O::st(a.hz, o, hz);
-        ezc = ezn; eyc = eyn;
-    }
+    };
+    if (xyf) face_loop(std::true_type{});
+    else face_loop(std::false_type{});
 }
 
 // ---------------------------------------------------------------------------
@@ -863,10 +919,9 @@ __device__ __forceinline__ void e_cpml_z(const YeeArgs<T>& a, int i, int j, int 
                                          T& ex, T& ey) {
     const unsigned cp = a.cpml;
     const int npml = a.npml;
-    const int nse = 2 * npml;
     if ((cp & 16u) && k >= 2 && k <= npml) {
         const int c = k;
-        const long ps = ((long)i * a.sy + j) * nse + (k - 1);
+        const long ps = zpsi_index(0, j, i, k - 2, a.sy, a.nx, npml);
         const T p0 = psi_step(gld(a.pe[4], ps), gld(a.be, c), gld(a.ced, c), hyk - hyc);
         const T p1 = psi_step(gld(a.pe[5], ps), gld(a.be, c), gld(a.ced, c), hxc - hxk);
         gst(a.pe[4], ps, p0); gst(a.pe[5], ps, p1);
@@ -874,7 +929,7 @@ __device__ __forceinline__ void e_cpml_z(const YeeArgs<T>& a, int i, int j, int 
     }
     if ((cp & 32u) && k >= a.nz + 1 - npml && k <= a.nz - 1) {
         const int c = a.nz + 1 - k;
-        const long ps = ((long)i * a.sy + j) * nse + (npml - 1 + c);
+        const long ps = zpsi_index(1, j, i, a.nz - 1 - k, a.sy, a.nx, npml);
         const T p0 = psi_step(gld(a.pe[4], ps), gld(a.be, c), gld(a.ced, c), hyk - hyc);
         const T p1 = psi_step(gld(a.pe[5], ps), gld(a.be, c), gld(a.ced, c), hxc - hxk);
         gst(a.pe[4], ps, p0); gst(a.pe[5], ps, p1);
@@ -899,8 +954,6 @@ __device__ __forceinline__ void e_tile(const YeeArgs<T>& a, const TfsfLaunch& tf
     const int j = ty * a.rows + __builtin_amdgcn_readfirstlane(threadIdx.y) + 1;
     const int i0 = tx * a.xchunk + 1;
     const int i1 = min(i0 + a.xchunk - 1, a.nx);
-    const T tdex = plane_table(a.den_ex, i0, i1);
-    const T tcex = sizeof(T) == 4 ? plane_table(a.cdex, i0, i1) : (T)0;
     if (k > a.nz || j > a.ny) return;
     const I sx = O::of(a.sx), sz = O::of(a.sz), s1 = O::of(1);
     const long jke = (long)j * a.sz + k + a.off;
@@ -915,21 +968,20 @@ __device__ __forceinline__ void e_tile(const YeeArgs<T>& a, const TfsfLaunch& tf
     int txa = 1, txb = 0;
     const bool tfw = TF && tfsf_e_lane(a, tf, i0, i1, j, k, kt0, kt1, tcx, tcl, tch, txa, txb);
     const int ti0 = TF ? tf.t.i0 : 0, ti1 = TF ? tf.t.i1 : 0;
-    bool faces = false;
+    bool xyf = false, zf = false;
     if (CPML) {
         const unsigned cp = a.cpml;
-        const int k0 = kt0, k1 = kt1;
-        faces = in_faces_e(cp & 1u, cp & 2u, i0, i1, a.nx, a.npml) || in_faces_e(cp & 4u, cp & 8u, j, j, a.ny, a.npml) ||
-                in_faces_e(cp & 16u, cp & 32u, k0, k1, a.nz, a.npml);
+        xyf = in_faces_e(cp & 1u, cp & 2u, i0, i1, a.nx, a.npml) || in_faces_e(cp & 4u, cp & 8u, j, j, a.ny, a.npml);
+        zf = in_faces_e(cp & 16u, cp & 32u, kt0, kt1, a.nz, a.npml);
     }
-    if (!faces) {
+    if (!xyf && !zf) {
         for (int i = i0; i <= i1; ++i, o += sx) {
             const T hzc = O::ld(a.hz, o), hyc = O::ld(a.hy, o), hxc = O::ld(a.hx, o);
             const T hzj = O::ld(a.hz, o - sz), hxj = O::ld(a.hx, o - sz);
             const T hyk = O::ld(a.hy, o - s1), hxk = O::ld(a.hx, o - s1);
             T ex = O::ld(a.ex, o), ey = O::ld(a.ey, o), ez = O::ld(a.ez, o);
             const int tg = tags ? O::tag(a.tag, o) : 0;
-            const T dex = plane_coef(tdex, i - i0), cex = sizeof(T) == 4 ? plane_coef(tcex, i - i0) : (T)0;
+            const T dex = ldc(a.den_ex, i), cex = sizeof(T) == 4 ? ldc(a.cdex, i) : (T)0;
             if (MED != 1 || tg < a.disp_tag0) {
                 e_bulk<T, MED>(a, tg, dep, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dex, cex, dey, dez, cey,
                                cez, ex, ey, ez);
@@ -944,59 +996,94 @@ __device__ __forceinline__ void e_tile(const YeeArgs<T>& a, const TfsfLaunch& tf
         }
         return;
     }
-    const unsigned cp = a.cpml;
-    const int npml = a.npml, nse = 2 * npml;
-    const Face fy = face_e(cp & 4u, cp & 8u, j, a.ny, npml);
-    const Face fz = face_e(cp & 16u, cp & 32u, k, a.nz, npml);
-    T yb = (T)0, yc = (T)0, zb = (T)0, zc = (T)0;
-    if (fy.slot >= 0) { yb = gld(a.be, fy.c); yc = gld(a.ced, fy.c); }
-    if (fz.slot >= 0) { zb = gld(a.be, fz.c); zc = gld(a.ced, fz.c); }
-    const I jk = O::of(jke);
-    const I ystep = O::of((long)nse * a.sz), zstep = O::of((long)a.sy * nse);
-    I py = O::of((long)i0 * nse * a.sz + (long)fy.slot * a.sz + k + a.off);
-    I pz = O::of(((long)i0 * a.sy + j) * nse + fz.slot);
-    for (int i = i0; i <= i1; ++i, o += sx, py += ystep, pz += zstep) {
-        const T hzc = O::ld(a.hz, o), hyc = O::ld(a.hy, o), hxc = O::ld(a.hx, o);
-        const T hzj = O::ld(a.hz, o - sz), hxj = O::ld(a.hx, o - sz);
-        const T hyk = O::ld(a.hy, o - s1), hxk = O::ld(a.hx, o - s1);
-        T ex = O::ld(a.ex, o), ey = O::ld(a.ey, o), ez = O::ld(a.ez, o);
-        const int tg = tags ? O::tag(a.tag, o) : 0;
-        const T dex = plane_coef(tdex, i - i0), cex = sizeof(T) == 4 ? plane_coef(tcex, i - i0) : (T)0;
-        const Face fx = face_e(cp & 1u, cp & 2u, i, a.nx, npml);
-        const I px = O::of((long)fx.slot * a.sx) + jk;
-        T px0 = (T)0, px1 = (T)0, xb = (T)0, xc = (T)0, py0 = (T)0, py1 = (T)0, pz0 = (T)0, pz1 = (T)0;
-        if (fx.slot >= 0) { px0 = O::ld(a.pe[0], px); px1 = O::ld(a.pe[1], px); xb = gld(a.be, fx.c); xc = gld(a.ced, fx.c); }
-        if (fy.slot >= 0) { py0 = O::ld(a.pe[2], py); py1 = O::ld(a.pe[3], py); }
-        if (fz.slot >= 0) { pz0 = O::ld(a.pe[4], pz); pz1 = O::ld(a.pe[5], pz); }
-        // cells of dispersive classes belong to the dispersive workgroups:
-        // their loads above are unused and nothing of theirs is stored; the
-        // psi loads are issued with the fields, ahead of the tag test
-        const bool own = MED != 1 || tg < a.disp_tag0;
-        e_bulk<T, MED>(a, tg, dep, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dex, cex, dey, dez, cey, cez,
-                       ex, ey, ez);
-        if (fx.slot >= 0) {
-            const T p0 = psi_step(px0, xb, xc, hzi - hzc), p1 = psi_step(px1, xb, xc, hyc - hyi);
-            if (own) { O::st(a.pe[0], px, p0); O::st(a.pe[1], px, p1); }
-            ey = ey + dep * p0; ez = ez + dep * p1;
+    // XY false: the wavefront meets only z faces (per lane), so the x and y
+    // face code is compiled out of its loop
+    const auto face_loop = [&](auto xy_const) {
+        constexpr bool XY = decltype(xy_const)::value;
+        const unsigned cp = a.cpml;
+        const int npml = a.npml, nse = 2 * npml;
+        const Face fy = XY ? face_e(cp & 4u, cp & 8u, j, a.ny, npml) : Face{-1, 0};
+        T yb = (T)0, yc = (T)0, zbc = (T)0, zcc = (T)0;
+        if (fy.slot >= 0) { yb = gld(a.be, fy.c); yc = gld(a.ced, fy.c); }
+        // z faces: low k = 2..npml, high k = nz+1-npml..nz-1
+        const ZBatch zw = z_batch((int)threadIdx.x, kt0, kt1, 2, (cp & 16u) ? npml : 1, a.nz + 1 - npml,
+                                  (cp & 32u) ? a.nz - 1 : a.nz - npml, 2, j, i0, a.nz, a.sy, a.nx, npml);
+        if (zw.q >= 0) { zbc = gld(a.be, zw.m + 2); zcc = gld(a.ced, zw.m + 2); }
+        const long zadv = (long)zw.P * (npml - 1);
+        long zidx = zw.sidx;
+        T zs0 = (T)0, zs1 = (T)0;
+        int sown = 1;       // storage lane: its cell is not one of the dispersive workgroups'
+        int zb0 = i0;
+        const I jk = O::of(jke);
+        const I ystep = O::of((long)nse * a.sz);
+        I py = O::of((long)i0 * nse * a.sz + (long)fy.slot * a.sz + k + a.off);
+        for (int i = i0; i <= i1; ++i, o += sx, py += ystep) {
+            int zp = i - zb0;
+            if (zw.nf > 0) {
+                if (zp == zw.P) {
+                    // the previous batch is complete: store it, start the next
+                    if (zw.sp >= 0 && sown) { O::st(a.pe[4], O::of(zidx), zs0); O::st(a.pe[5], O::of(zidx), zs1); }
+                    zb0 = i; zp = 0; zidx += zadv;
+                }
+                if (zp == 0 && zw.sp >= 0 && zb0 + zw.sp <= i1) {
+                    zs0 = O::ld(a.pe[4], O::of(zidx)); zs1 = O::ld(a.pe[5], O::of(zidx));
+                }
+            }
+            const T hzc = O::ld(a.hz, o), hyc = O::ld(a.hy, o), hxc = O::ld(a.hx, o);
+            const T hzj = O::ld(a.hz, o - sz), hxj = O::ld(a.hx, o - sz);
+            const T hyk = O::ld(a.hy, o - s1), hxk = O::ld(a.hx, o - s1);
+            T ex = O::ld(a.ex, o), ey = O::ld(a.ey, o), ez = O::ld(a.ez, o);
+            const int tg = tags ? O::tag(a.tag, o) : 0;
+            const T dex = ldc(a.den_ex, i), cex = sizeof(T) == 4 ? ldc(a.cdex, i) : (T)0;
+            const Face fx = XY ? face_e(cp & 1u, cp & 2u, i, a.nx, npml) : Face{-1, 0};
+            const I px = O::of((long)fx.slot * a.sx) + jk;
+            T px0 = (T)0, px1 = (T)0, xb = (T)0, xc = (T)0, py0 = (T)0, py1 = (T)0;
+            if (fx.slot >= 0) {
+                px0 = O::ld(a.pe[0], px); px1 = O::ld(a.pe[1], px);
+                xb = gld(a.be, fx.c); xc = gld(a.ced, fx.c);
+            }
+            if (fy.slot >= 0) { py0 = O::ld(a.pe[2], py); py1 = O::ld(a.pe[3], py); }
+            // cells of dispersive classes belong to the dispersive workgroups:
+            // their loads above are unused and nothing of theirs is stored; the
+            // psi loads are issued with the fields, ahead of the tag test
+            const bool own = MED != 1 || tg < a.disp_tag0;
+            e_bulk<T, MED>(a, tg, dep, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dex, cex, dey, dez, cey, cez,
+                           ex, ey, ez);
+            if (fx.slot >= 0) {
+                const T p0 = psi_step(px0, xb, xc, hzi - hzc), p1 = psi_step(px1, xb, xc, hyc - hyi);
+                if (own) { O::st(a.pe[0], px, p0); O::st(a.pe[1], px, p1); }
+                ey = ey + dep * p0; ez = ez + dep * p1;
+            }
+            if (fy.slot >= 0) {
+                const T p0 = psi_step(py0, yb, yc, hzc - hzj), p1 = psi_step(py1, yb, yc, hxj - hxc);
+                if (own) { O::st(a.pe[2], py, p0); O::st(a.pe[3], py, p1); }
+                ex = ex + dep * p0; ez = ez + dep * p1;
+            }
+            if (zw.nf > 0) {
+                // face lanes take their psi from the storage lanes and return
+                // the new values (and whether they own the cell) to them
+                const int from = (zp * zw.nf + max(zw.q, 0)) * 4;
+                const T p0 = psi_step(lane_pull(from, zs0), zbc, zcc, hyk - hyc);
+                const T p1 = psi_step(lane_pull(from, zs1), zbc, zcc, hxc - hxk);
+                if (zw.q >= 0) { ex = ex + dep * p0; ey = ey + dep * p1; }
+                const T n0 = lane_pull(zw.slane * 4, p0), n1 = lane_pull(zw.slane * 4, p1);
+                const int no = __builtin_amdgcn_ds_bpermute(zw.slane * 4, own ? 1 : 0);
+                if (zw.sp == zp) { zs0 = n0; zs1 = n1; sown = no; }
+            }
+            if (tfw) {
+                if (i >= txa && i <= txb) ex = (T)((double)ex + tcx);
+                if (i == ti0) ez = (T)((double)ez + tcl);
+                if (i == ti1) ez = (T)((double)ez + tch);
+            }
+            if (own) { O::st(a.ex, o, ex); O::st(a.ey, o, ey); O::st(a.ez, o, ez); }
+            hzi = hzc; hyi = hyc;
         }
-        if (fy.slot >= 0) {
-            const T p0 = psi_step(py0, yb, yc, hzc - hzj), p1 = psi_step(py1, yb, yc, hxj - hxc);
-            if (own) { O::st(a.pe[2], py, p0); O::st(a.pe[3], py, p1); }
-            ex = ex + dep * p0; ez = ez + dep * p1;
+        if (zw.nf > 0 && zw.sp >= 0 && zb0 + zw.sp <= i1 && sown) {
+            O::st(a.pe[4], O::of(zidx), zs0); O::st(a.pe[5], O::of(zidx), zs1);
         }
-        if (fz.slot >= 0) {
-            const T p0 = psi_step(pz0, zb, zc, hyk - hyc), p1 = psi_step(pz1, zb, zc, hxc - hxk);
-            if (own) { O::st(a.pe[4], pz, p0); O::st(a.pe[5], pz, p1); }
-            ex = ex + dep * p0; ey = ey + dep * p1;
-        }
-        if (tfw) {
-            if (i >= txa && i <= txb) ex = (T)((double)ex + tcx);
-            if (i == ti0) ez = (T)((double)ez + tcl);
-            if (i == ti1) ez = (T)((double)ez + tch);
-        }
-        if (own) { O::st(a.ex, o, ex); O::st(a.ey, o, ey); O::st(a.ez, o, ez); }
-        hzi = hzc; hyi = hyc;
-    }
+    };
+    if (xyf) face_loop(std::true_type{});
+    else face_loop(std::false_type{});
 }
 
 // E of the cells of dispersive classes (Drude / Drude-Lorentz components),
@@ -1330,27 +1417,26 @@ __device__ __forceinline__ void fused_tile_lanes(const FusedArgs<T>& f, const Fu
     c.zhb = c.zeb = c.zhs = c.zes = 0u;
     if constexpr (CPML) {
         // z faces: H for k in [1, npml-1] and [nz+1-npml, nz-1] (coefficient
-        // index k resp. nz-k, psi slot k-1 resp. npml-2+nz-k); E for k in
-        // [2, npml] and [nz+1-npml, nz-1] (index k resp. nz+1-k, slot k-1
-        // resp. npml-1+nz+1-k)
+        // index k resp. nz-k, cell m = k-1 resp. nz-1-k of zpsi_index); E for
+        // k in [2, npml] and [nz+1-npml, nz-1] (index k resp. nz+1-k, cell
+        // m = k-2 resp. nz-1-k)
         const int npml = f.npml, nz = f.nz, k = c.k;
-        const int nsh = 2 * (npml - 1), nse = 2 * npml;
-        int ch = -1, sh = 0, ce = -1, se = 0;
+        int ch = -1, sdh = 0, mh = 0, ce = -1, sde = 0, me = 0;
         if (c.hcell) {
-            if ((f.cpml & 16u) && k <= npml - 1) { ch = k; sh = k - 1; }
-            else if ((f.cpml & 32u) && k >= nz + 1 - npml && k <= nz - 1) { ch = nz - k; sh = npml - 2 + nz - k; }
-            if ((f.cpml & 16u) && k >= 2 && k <= npml) { ce = k; se = k - 1; }
-            else if ((f.cpml & 32u) && k >= nz + 1 - npml && k <= nz - 1) { ce = nz + 1 - k; se = npml - 1 + nz + 1 - k; }
+            if ((f.cpml & 16u) && k <= npml - 1) { ch = k; mh = k - 1; }
+            else if ((f.cpml & 32u) && k >= nz + 1 - npml && k <= nz - 1) { ch = nz - k; sdh = 1; mh = nz - 1 - k; }
+            if ((f.cpml & 16u) && k >= 2 && k <= npml) { ce = k; me = k - 2; }
+            else if ((f.cpml & 32u) && k >= nz + 1 - npml && k <= nz - 1) { ce = nz + 1 - k; sde = 1; me = nz - 1 - k; }
         }
         c.zh = ch >= 0; c.ze = ce >= 0;
         if (c.zh) { c.zbh = a.bh[ch]; c.zch = a.chd[ch]; }
         if (c.ze) { c.zbe = a.be[ce]; c.zce = a.ced[ce]; }
-        const long row_h = ((long)(c.i0 - 1) * f.sy + jc) * nsh + sh;
-        const long row_e = ((long)(c.i0 - 1) * f.sy + jc) * nse + se;
+        const long row_h = zpsi_index(sdh, jc, c.i0 - 1, mh, f.sy, f.nx, npml);
+        const long row_e = zpsi_index(sde, jc, c.i0 - 1, me, f.sy, f.nx, npml);
         c.zhb = (uint32_t)(row_h * (long)sizeof(T));
         c.zeb = (uint32_t)(row_e * (long)sizeof(T));
-        c.zhs = (uint32_t)((long)f.sy * nsh * (long)sizeof(T));
-        c.zes = (uint32_t)((long)f.sy * nse * (long)sizeof(T));
+        c.zhs = (uint32_t)((long)(npml - 1) * (long)sizeof(T));
+        c.zes = c.zhs;
     }
     c.tf_y = c.tf_z = false;
     c.tfy = c.tfz = 0.0;

@@ -150,6 +150,49 @@ def test_cpml_faces_adjacent(precision):
     assert r <= (TOL64 if precision == 8 else TOL32)
 
 
+@pytest.mark.parametrize("precision", [8, 4])
+def test_zface_psi_batches(precision):
+    """z-face CPML psi held in registers a batch of planes at a time: nz =
+    133 with npml 6 puts the high H face (k = 128..132) across the z tiles
+    ending at 128 and 129, chunks of 13 and 40 planes take several batches
+    (12 planes per batch in the first z tile, 2 in the last), xchunk 1 one
+    plane per batch; a Drude-Lorentz sphere inside the low z face and a
+    dielectric box inside the high one (the dispersive workgroups own the
+    sphere's psi).  Every tiling against the FP64 oracle."""
+    n = (14, 13, 133)
+    spec = ("drude-lorentz", "Ag", 1.0)
+    shapes = [(1, 1, [7.0, 6.0, 4.0, 3.5]), (3, 2, [2, 2, 126, 13, 12, 132])]
+    pts = [(7, 6, 3), (7, 6, 130), (5, 7, 128), (8, 5, 66), (3, 3, 131), (11, 10, 2)]
+    dets = cases.point_detectors(pts, ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"))
+    # point sources 8 cells inside each z face (positions in nm from the centre cell 66)
+    sources = [("x", 1.0, 1.549802, 0.6, 0.3, 0.0, 0.0, -232.0, 0.0, 0.0, 1.0e6, 0.0),
+               ("y", 1.0, 1.549802, 0.6, 0.3, 4.0, 0.0, 232.0, 0.0, 0.0, 1.0e6, 0.0)]
+    case = small_case("zface-batches", n=n, npml=6, media=[spec, ("dielectric", 2.25)], shapes=shapes,
+                      steps=70, sources=sources, dets=dets, dx=4.0)
+    dr, dt = case.dr_dt
+    o = oracle.OracleGrid(case.grid, case.npml, case.boundaries, dt, dr)
+    cases.apply(case, o, library=oracle.library_medium, make_medium=oracle.make_medium,
+                rasterize=oracle.rasterize)
+    want = o.run(1, case.steps, case.record_every)
+    assert np.abs(want).max() > 0
+    for rows, xc in ((4, 4), (1, 1), (2, 13), (4, 40)):
+        g = omx_fdtd.Grid3D(case.grid, case.npml, case.boundaries, dt, dr, precision=precision)
+        g.set_step_mode(1)
+        g.set_tiling(rows, xc)
+        cases.apply(case, g, library=G.library_medium, make_medium=G.medium)
+        g.run(1, case.steps, case.record_every)
+        got = g.read_records()
+        r = rel_l2(got, want)
+        # each detector point on its own (six components): the weak points
+        # near the faces count as much as the strong ones
+        rp = max(rel_l2(got[:, 6 * q:6 * q + 6], want[:, 6 * q:6 * q + 6]) for q in range(len(pts)))
+        print(f"FP{8 * precision} rows {rows} xchunk {xc}: relL2 = {r:.3e}, worst point {rp:.3e}", flush=True)
+        assert r <= (TOL64 if precision == 8 else TOL32)
+        if precision == 8:
+            assert rp <= 1e-10
+        g.kill()
+
+
 def test_dl_sphere_fp32_against_fp64_oracle():
     spec = ("dl-custom", 5.0, 9.0, 0.07, [(1.2, 3.1, 0.3), (0.7, 2.7, 0.15)])
     case = small_case("fp32-dl", n=(28, 26, 24), media=[spec], shapes=[(1, 1, [14.0, 13.0, 12.0, 5.0])],
