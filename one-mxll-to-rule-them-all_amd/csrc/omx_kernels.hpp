@@ -1139,6 +1139,16 @@ __global__ __launch_bounds__(256) void k_e_dispersive(const YeeArgs<T> a, T* __r
     e_dispersive_cell<T, CPML, B32>(a, exo, eyo, ezo, list[p], p);
 }
 
+// Minimum waves per SIMD the compiler must fit the E kernel into.  The
+// FP32 E kernels on 32-bit offsets need a few more scalar registers than
+// 8 waves leave; asked for 8, the compiler keeps the excess in vector-register
+// lanes (no memory), and the launch runs 8 waves per SIMD instead of 7:
+// E -3 % on the emitter slab, -5.5 % on the ellipsoids.  With TF/SF (and in
+// the batched kernel) the step measured 0.5 % slower, and the FP64 and
+// 64-bit-index kernels would spill to memory, so those keep 1 (no demand).
+template <typename T, bool B32, bool TF>
+inline constexpr int kEWaves = (sizeof(T) == 4 && B32 && !TF) ? 8 : 1;
+
 // H tiles; with TF the tiles apply the TF/SF H corrections and the first
 // tf.nb workgroups advance the 1D TF/SF line (hy1 from ex1) instead.
 template <typename T, bool CPML, bool B32, bool TF>
@@ -1163,8 +1173,8 @@ __global__ __launch_bounds__(256) void k_h_update(const YeeArgs<T> a, const Tfsf
 // [db, db + tf.nb) advance the 1D TF/SF line's E (ex1 from hy1, plus the
 // source at node ks); the tiles read only hy1.
 template <typename T, bool CPML, int MED, bool B32, bool TF>
-__global__ __launch_bounds__(256) void k_e_update(const YeeArgs<T> a, const long* __restrict__ disp, int n_disp,
-                                                  int db, const TfsfLaunch tf) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kEWaves<T, B32, TF>, 8)))
+void k_e_update(const YeeArgs<T> a, const long* __restrict__ disp, int n_disp, int db, const TfsfLaunch tf) {
     const int b = blockIdx.x;
     if (MED == 1 && b < db) {
         const int p = (b * (int)blockDim.y + (int)threadIdx.y) * kLanes + (int)threadIdx.x;
