@@ -504,7 +504,7 @@ __device__ __forceinline__ T lane_pull(int byte_addr, T v) {
 struct ZBatch {
     int nf, P;          // face lanes of the wavefront and planes per batch (uniform)
     int q;              // this lane's face rank, -1: not a face lane
-    int m, side;        // this face lane's cell in its side
+    int m;              // this face lane's cell in its side
     int sp, slane;      // storage: batch plane held by this lane (-1: none), its face lane
     long sidx;          // psi index of this storage lane in the first batch
 };
@@ -516,9 +516,9 @@ __device__ __forceinline__ ZBatch z_batch(int lane, int kt0, int kt1, int zl0, i
     const int la = a0 - kt0, ha = b0 - kt0;
     z.nf = nlo + nhi;
     z.P = z.nf > 0 ? (kt1 - kt0 + 1) / z.nf : 0;
-    z.q = -1; z.m = 0; z.side = 0;
-    if (lane >= la && lane < la + nlo) { z.q = lane - la; z.side = 0; z.m = kt0 + lane - klo0; }
-    else if (lane >= ha && lane < ha + nhi) { z.q = nlo + lane - ha; z.side = 1; z.m = nz - 1 - (kt0 + lane); }
+    z.q = -1; z.m = 0;
+    if (lane >= la && lane < la + nlo) { z.q = lane - la; z.m = kt0 + lane - klo0; }
+    else if (lane >= ha && lane < ha + nhi) { z.q = nlo + lane - ha; z.m = nz - 1 - (kt0 + lane); }
     z.sp = -1; z.slane = 0; z.sidx = 0;
     if (z.nf > 0 && lane < z.P * z.nf) {
         z.sp = lane / z.nf;
