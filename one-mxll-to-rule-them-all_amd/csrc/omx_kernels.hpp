@@ -638,10 +638,11 @@ __device__ __forceinline__ bool tfsf_e_lane(const YeeArgs<T>& a, const TfsfLaunc
 // marching xchunk planes along x.  A wavefront whose cells (its row, its z
 // tile, the planes of its chunk) meet no CPML face runs the bulk loop.  In
 // the face loop the face slots of the row (y) and lane (z) are fixed over
-// the tile and those of the plane (x) are uniform; the psi of all faces a
-// cell lies in are loaded together with its fields, ahead of the
-// arithmetic.  A cell lies in at most one face per axis: grids are at least
-// 2 npml cells along every CPML axis (omx3d_create rejects thinner ones).
+// the tile and those of the plane (x) are uniform; the x and y psi of the
+// faces a cell lies in are loaded together with its fields, ahead of the
+// arithmetic, and the z psi a batch of planes at a time (ZBatch).  A cell
+// lies in at most one face per axis: grids are at least 2 npml cells along
+// every CPML axis (omx3d_create rejects thinner ones).
 template <typename T, bool CPML, bool B32, bool TF>
 __device__ __forceinline__ void h_tile(const YeeArgs<T>& a, const TfsfLaunch& tf, const int t) {
     using O = Off<T, B32>;
