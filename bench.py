@@ -12,9 +12,11 @@ ADE, TF/SF corrections and detector recording at its cadence) over the whole
 grid, inputs resident in HBM.  The FP32 engine is timed (the headline), then
 the FP64 engine on the same steps; the FP32 detector series is compared
 with the FP64 one (relative L2).  The dominant kernel's roofline fraction
-comes from HIP events recorded around each of its launches in the timed
-steps, on the stream the kernels run on; its HBM traffic from the committed
-rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the same kernel and workload.
+comes from HIP events recorded around its launches in every PROF_EVERY-th
+timed step, on the stream the kernels run on (an event pair costs the
+stream a few microseconds, so events on every step would slow the timed
+run by several percent); its HBM traffic from the committed rocprofv3
+FETCH_SIZE / WRITE_SIZE summary of the same kernel and workload.
 The CPU baseline is the reference's own Fortran/OpenMP solver (built from
 its sources into oracle/_ref) timed on a bounded sample on this host.
 
@@ -46,6 +48,7 @@ from omx_fdtd import cases, grid as G  # noqa: E402
 from omx_fdtd.units import index_to_nm, MEDIUM_DRUDE, MEDIUM_DL, BC_CPML  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+PROF_EVERY = 8          # timed steps per profiled step (HIP events around its H and E launches)
 HEADLINE = "dl_sphere_tfsf_256"
 METRIC = "Gcell-updates/s, FP32 3D Drude-Lorentz + CPML FDTD (TF/SF plane wave)"
 
@@ -223,13 +226,13 @@ def roofline(case, g, prec, res, workload):
 
 def run_single(case, prec, W, K, mode, energy_every=0):
     """Warm-up W steps, then K timed steps with HIP events around the H and E
-    launches.  With energy_every > 0 the volume energy integral is taken
+    launches of every PROF_EVERY-th step.  With energy_every > 0 the volume energy integral is taken
     every energy_every steps inside the timed region."""
     log(f"[bench] building FP{8 * prec} grid {case.grid}")
     g = build_grid(case, prec, mode)
     if W:
         g.run(1, W, case.record_every)
-    g.set_profiling(True)
+    g.set_profiling(True, every=PROF_EVERY)
     energies = []
     if energy_every > 0:
         g.synchronize()

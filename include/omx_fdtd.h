@@ -215,9 +215,12 @@ int omx3d_run(omx3d_ctx* ctx, int32_t tt_first, int32_t n_steps, int32_t record_
 int omx3d_set_step_mode(omx3d_ctx* ctx, int32_t mode, int32_t rows, int32_t xchunk);
 /* 1 if the last omx3d_run used the fused step. */
 int omx3d_get_step_mode(omx3d_ctx* ctx, int32_t* fused_last_run);
-/* Per-kernel timing (HIP events around each H and E launch of omx3d_run;
- * with the fused step, ms_h is 0 and ms_e times the fused kernel). */
-int omx3d_set_profiling(omx3d_ctx* ctx, int32_t enable);
+/* Per-kernel timing: HIP events around the H and E launches of every
+ * every-th step of omx3d_run (1: every step, 0: off); omx3d_get_profile
+ * sums them.  With the fused step, ms_h is 0 and ms_e times the fused
+ * kernel.  Each event pair costs the stream a few microseconds, so a
+ * stride above 1 keeps the timed run close to an unprofiled one. */
+int omx3d_set_profiling(omx3d_ctx* ctx, int32_t every);
 int omx3d_get_profile(omx3d_ctx* ctx, double* ms_h, double* ms_e, int32_t* n_launches);
 int omx3d_synchronize(omx3d_ctx* ctx);
 /* Bytes of device memory the context holds. */

@@ -563,8 +563,9 @@ int run_impl(omx3d_ctx* c, int tt_first, int n_steps, int record_every) {
     if (rc) return rc;
     const double dt = c->d.dt;
     size_t evi = 0;
-    if (c->prof) {
-        rc = ensure_events(c, (size_t)4 * n_steps);
+    const int pe = c->prof_every;
+    if (pe > 0) {
+        rc = ensure_events(c, (size_t)4 * ((n_steps + pe - 1) / pe));
         if (rc) return rc;
     }
     rc = check_sources_dispersive(c);
@@ -584,8 +585,9 @@ int run_impl(omx3d_ctx* c, int tt_first, int n_steps, int record_every) {
         const double t_skip = time;
         // profiling: events bracket the H and E kernels themselves (the
         // fused kernel takes the E slot, the H slot stays empty)
-        hipEvent_t* ev = c->prof ? &c->ev[evi] : nullptr;
-        if (c->prof) evi += 4;
+        const bool prof = pe > 0 && (tt - tt_first) % pe == 0;
+        hipEvent_t* ev = prof ? &c->ev[evi] : nullptr;
+        if (prof) evi += 4;
         if (fused) {
             // H and E in one pass; sources then enter E after its update
             // (same value as before it in vacuum/dielectric source cells).
@@ -620,7 +622,7 @@ int run_impl(omx3d_ctx* c, int tt_first, int n_steps, int record_every) {
             c->rec_rows++;
         }
     }
-    if (c->prof) {
+    if (pe > 0) {
         HIPCHK(hipStreamSynchronize(c->s));
         for (size_t q = 0; q + 3 < evi; q += 4) {
             float mh = 0.f, me = 0.f;
@@ -1775,9 +1777,10 @@ int omx3d_get_step_mode(omx3d_ctx* c, int32_t* fused_last_run) {
     return OMX_OK;
 }
 
-int omx3d_set_profiling(omx3d_ctx* c, int32_t enable) {
+int omx3d_set_profiling(omx3d_ctx* c, int32_t every) {
     CHECK_CTX(c);
-    c->prof = enable != 0;
+    if (every < 0) return fail(OMX_ERR_ARG, "profiling stride must be >= 0");
+    c->prof_every = every;
     c->prof_ms_h = c->prof_ms_e = 0.0;
     c->prof_n = 0;
     return OMX_OK;

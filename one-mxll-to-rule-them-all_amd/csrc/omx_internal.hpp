@@ -202,7 +202,7 @@ struct omx3d_ctx {
     int red_blocks = 0;
 
     // profiling
-    bool prof = false;
+    int prof_every = 0;     // events around the H/E launches of every prof_every-th step (0: none)
     std::vector<hipEvent_t> ev;
     double prof_ms_h = 0.0, prof_ms_e = 0.0;
     int prof_n = 0;

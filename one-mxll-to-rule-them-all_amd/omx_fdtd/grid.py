@@ -279,8 +279,9 @@ class Grid3D:
         L.call("omx3d_set_tfsf", self.h, L.iptr(lo), L.iptr(hi), e0, w0, t0, tau, int(max_steps))
 
     # -- profiling -----------------------------------------------------------------
-    def set_profiling(self, on):
-        L.call("omx3d_set_profiling", self.h, 1 if on else 0)
+    def set_profiling(self, on, every=1):
+        """Events around the H and E launches of every `every`-th step."""
+        L.call("omx3d_set_profiling", self.h, int(every) if on else 0)
 
     def profile(self):
         mh, me, n = C.c_double(), C.c_double(), C.c_int32()

@@ -353,6 +353,27 @@ def test_two_level_emitters_fp64():
     np.testing.assert_allclose(dg, do, rtol=1e-9, atol=1e-12 * np.abs(do).max())
 
 
+def test_profiling_stride():
+    """Events around the H and E launches of every n-th step: the launch
+    count and per-launch times of the profiled steps."""
+    case = small_case("profiling", steps=10)
+    dr, dt = case.dr_dt
+    g = omx_fdtd.Grid3D(case.grid, case.npml, case.boundaries, dt, dr, precision=8)
+    cases.apply(case, g, library=G.library_medium, make_medium=G.medium)
+    for every, launches in ((1, 10), (3, 4), (8, 2), (20, 1)):
+        g.set_profiling(True, every=every)
+        g.run(1, 10, 0)
+        mh, me, n = g.profile()
+        print(f"every {every}: {n} profiled steps, H {mh / n:.4f} ms, E {me / n:.4f} ms")
+        assert n == launches and mh > 0 and me > 0
+    g.set_profiling(False)
+    g.run(11, 5, 0)
+    assert g.profile()[2] == 0
+    with pytest.raises(omx_fdtd.OmxError):
+        g.set_profiling(True, every=-1)
+    g.kill()
+
+
 def test_field_copy_roundtrip_and_energy():
     case = small_case("fields", n=(14, 12, 10), steps=0)
     dr, dt = case.dr_dt
