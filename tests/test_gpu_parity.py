@@ -193,6 +193,46 @@ def test_zface_psi_batches(precision):
         g.kill()
 
 
+@pytest.mark.parametrize("seed", [7, 11])
+def test_seeded_random_grids(seed):
+    """Seeded random grids (sizes, CPML thickness, tile shape and order,
+    Drude-Lorentz sphere, dielectric layer, two point sources, detectors
+    near both z faces): FP64 against the oracle, per detector point."""
+    rng = np.random.default_rng(seed)
+    for it in range(4):
+        npml = int(rng.integers(3, 9))
+        n = (int(rng.integers(2 * npml + 1, 2 * npml + 50)), int(rng.integers(2 * npml + 1, 2 * npml + 50)),
+             int(rng.integers(max(2 * npml + 1, 20), 150)))
+        dx = 4.0
+        pts = [tuple(int(rng.integers(1, n[a] + 1)) for a in range(3)) for _ in range(4)]
+        pts += [(n[0] // 2, n[1] // 2, 2), (n[0] // 2, n[1] // 2, n[2] - 1)]
+        dets = cases.point_detectors(pts, ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"))
+        srcs = []
+        for _ in range(2):
+            pos = [float((rng.integers(1, n[a] + 1) - n[a] // 2) * dx) for a in range(3)]
+            srcs.append((str(rng.choice(["x", "y", "z"])), 1.0, 1.549802, 0.6, 0.3, *pos, 0.0, 0.0, 1.0e6, 0.0))
+        centre = [float(rng.uniform(1, n[a])) for a in range(3)]
+        shapes = [(1, 1, centre + [float(rng.uniform(2, 6))]), (3, 2, [1, 1, 1, n[0], n[1], int(rng.integers(1, 5))])]
+        media = [("drude-lorentz", str(rng.choice(["Ag", "Au", "Al"])), 1.0), ("dielectric", 2.25)]
+        case = small_case(f"random-{seed}-{it}", n=n, npml=npml, media=media, shapes=shapes, steps=50,
+                          sources=srcs, dets=dets, dx=dx)
+        rows, xc, order = int(rng.choice([1, 2, 4])), int(rng.integers(1, 20)), int(rng.integers(0, 3))
+        g, o = both(case)
+        g.set_step_mode(1)
+        g.set_tiling(rows, xc, order)
+        g.run(1, case.steps, case.record_every)
+        got = g.read_records()
+        want = o.run(1, case.steps, case.record_every)
+        r = rel_l2(got, want)
+        rp = max(rel_l2(got[:, 6 * q:6 * q + 6], want[:, 6 * q:6 * q + 6]) for q in range(len(pts))
+                 if np.abs(want[:, 6 * q:6 * q + 6]).max() > 1e-30)
+        print(f"grid {n} npml {npml} tiling {rows}x{xc} order {order}: relL2 {r:.2e}, worst point {rp:.2e}",
+              flush=True)
+        assert np.abs(want).max() > 0
+        assert r <= TOL64 and rp <= 1e-10
+        g.kill()
+
+
 def test_dl_sphere_fp32_against_fp64_oracle():
     spec = ("dl-custom", 5.0, 9.0, 0.07, [(1.2, 3.1, 0.3), (0.7, 2.7, 0.15)])
     case = small_case("fp32-dl", n=(28, 26, 24), media=[spec], shapes=[(1, 1, [14.0, 13.0, 12.0, 5.0])],
