@@ -1193,7 +1193,11 @@ void k_e_update(const YeeArgs<T> a, const long* __restrict__ disp, int n_disp, i
         }
         return;
     }
-    e_tile<T, CPML, MED, B32, TF>(a, tf, tile_of_block(a.tile_order, b - ns, (int)gridDim.x - ns, a.ntz));
+    // the E sweep runs backwards through the tiles: it starts where the H
+    // sweep ended, so the fields H wrote last (and read last) are still in
+    // the memory-side cache, and it ends where the next H sweep starts
+    const int nt = (int)gridDim.x - ns;
+    e_tile<T, CPML, MED, B32, TF>(a, tf, nt - 1 - tile_of_block(a.tile_order, b - ns, nt, a.ntz));
 }
 
 // ---------------------------------------------------------------------------
@@ -1585,7 +1589,9 @@ __global__ __launch_bounds__(256) void k_e_update_batched(const YeeArgs<T>* __re
         e_dispersive_cell<T, true, B32>(a, a.ex, a.ey, a.ez, e, p - a.disp0);
         return;
     }
-    const int t = tile_of_block(order, b - db, (int)gridDim.x - db, ntz);
+    // backwards, as in k_e_update
+    const int nt = (int)gridDim.x - db;
+    const int t = nt - 1 - tile_of_block(order, b - db, nt, ntz);
     const int g = find_grid(tile_start, n, t);
     const YeeArgs<T>& a = args[g];
     if ((int)threadIdx.y >= a.rows) return;
