@@ -797,27 +797,43 @@ __device__ __forceinline__ T medium_update(T e, T rot, T vac, int m, const MedDe
         const T eo = gld(eold, p);
         const T pv = gld(pd, p);
         const int np = M.np;
+        // the poles' PL / PL_old of the cell, loaded together (the loops are
+        // unrolled to the maximum pole count, guarded by np) and kept in
+        // registers for both passes: one memory round trip per component
+        // instead of two per pole
+        T Lq[OMX_MAX_POLES], Loq[OMX_MAX_POLES];
+#pragma unroll
+        for (int q = 0; q < OMX_MAX_POLES; ++q) {
+            Lq[q] = (T)0; Loq[q] = (T)0;
+            if (q < np) { Lq[q] = gld(pl, p + q * ps); Loq[q] = gld(plold, p + q * ps); }
+        }
         T t;
         if constexpr (sizeof(T) == 8) {
             t = M.C1 * e + M.C2 * eo + M.C3 * rot - M.C4 * pv;
-            for (int q = 0; q < np; ++q) t = t - M.B1[q] * gld(pl, p + q * ps) - M.B2[q] * gld(plold, p + q * ps);
+#pragma unroll
+            for (int q = 0; q < OMX_MAX_POLES; ++q)
+                if (q < np) t = t - M.B1[q] * Lq[q] - M.B2[q] * Loq[q];
             gst(pd, p, M.A1 * pv + M.A2 * (t + e));
         } else {
             T dlt = M.C1 * e + M.C2 * eo + M.C3 * rot - M.C4 * pv;
-            for (int q = 0; q < np; ++q)
-                dlt = dlt - M.B1[q] * gld(pl, p + q * ps) - M.B2[q] * gld(plold, p + q * ps);
+#pragma unroll
+            for (int q = 0; q < OMX_MAX_POLES; ++q)
+                if (q < np) dlt = dlt - M.B1[q] * Lq[q] - M.B2[q] * Loq[q];
             t = e + dlt;
             gst(pd, p, pv + (M.A1 * pv + M.A2 * (t + e)));
         }
         const T de = t - eo;
-        for (int q = 0; q < np; ++q) {
-            const long pq = p + q * ps;
-            const T L = gld(pl, pq), Lo = gld(plold, pq);
-            T nl;
-            if constexpr (sizeof(T) == 8) nl = M.al[q] * L + M.bt[q] * Lo + M.ga[q] * de;
-            else nl = L + (((L - Lo) + M.al[q] * L) + M.bt[q] * Lo + M.ga[q] * de);
-            gst(plold, pq, L);
-            gst(pl, pq, nl);
+#pragma unroll
+        for (int q = 0; q < OMX_MAX_POLES; ++q) {
+            if (q < np) {
+                const long pq = p + q * ps;
+                const T L = Lq[q], Lo = Loq[q];
+                T nl;
+                if constexpr (sizeof(T) == 8) nl = M.al[q] * L + M.bt[q] * Lo + M.ga[q] * de;
+                else nl = L + (((L - Lo) + M.al[q] * L) + M.bt[q] * Lo + M.ga[q] * de);
+                gst(plold, pq, L);
+                gst(pl, pq, nl);
+            }
         }
         gst(eold, p, e);
         return t;
