@@ -141,17 +141,30 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / nb) if nb > 0 else None
 
 
+def host_cpu():
+    """Host cores this process may use, the machine's logical CPUs and the CPU model."""
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    model = ""
+    try:
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, text=True).stdout
+        model = re.search(r"Model name:\s*(.*)", out).group(1).strip()
+    except Exception:
+        pass
+    return affinity, os.cpu_count() or affinity, model
+
+
 def cpu_reference_baseline(case, steps):
-    """Times the reference's own solver (oracle/_ref/omx_ref_driver) on
-    `steps` steps of the same grid on this host's cores."""
-    drv = os.path.join(ROOT, "oracle", "_ref", "omx_ref_driver")
+    """Times the reference's own solver (oracle/_ref/o3/omx_ref_driver, its
+    Fortran sources built with amdflang -O3 -fopenmp) on `steps` steps of the
+    same grid, with one OpenMP thread per host core this process may use."""
+    drv = os.path.join(ROOT, "oracle", "_ref", "o3", "omx_ref_driver")
     if not os.path.exists(drv):
         return None
-    try:
-        ncpu = len(os.sched_getaffinity(0))
-    except AttributeError:
-        ncpu = os.cpu_count() or 1
-    threads = max(1, min(ncpu, 16))
+    affinity, logical, cpu_model = host_cpu()
+    threads = affinity
     nx, ny, nz = case.grid
     c = case.info["sphere_centre_cells"]
     r = case.info["sphere_radius_nm"] / case.dx_nm
@@ -177,17 +190,13 @@ def cpu_reference_baseline(case, steps):
         log("cpu baseline failed:", p.stdout[-2000:])
         return None
     secs = float(m.group(2))
-    cpu_model = ""
-    try:
-        cpu_model = subprocess.run(["lscpu"], stdout=subprocess.PIPE, text=True).stdout
-        cpu_model = re.search(r"Model name:\s*(.*)", cpu_model).group(1).strip()
-    except Exception:
-        pass
     return {"value": nx * ny * nz * steps / secs / 1e9, "unit": "Gcell-updates/s", "cores": int(m.group(3)),
-            "kind": "reference",
+            "kind": "reference", "threads": int(m.group(3)), "host_cores": affinity, "host_logical_cpus": logical,
+            "cpu_model": cpu_model, "build": "amdflang -O3 -fopenmp",
             "sample": f"{steps} steps of the {nx}x{ny}x{nz} grid (CPML {case.npml}, same sphere cells as a "
                       f"library Ag Drude-Lorentz medium, one point source, no TF/SF, no detector I/O): "
-                      f"the reference's Fortran/OpenMP td_propagate_H/E_3Dfield built with amdflang -O2, "
+                      f"the reference's Fortran/OpenMP td_propagate_H/E_3Dfield built with amdflang -O3, "
+                      f"{int(m.group(3))} OpenMP threads (OMP_PLACES=cores, OMP_PROC_BIND=spread), "
                       f"{secs:.2f} s on {cpu_model}"}
 
 
@@ -277,7 +286,46 @@ def add_fp64(out, results, r32):
                        "step_roofline_frac": round(r64["step_bytes"] / (r64["ms_step"] * 1e-3) / 1e9
                                                    / HBM_PEAK_GBS, 4)}
         out["fp32_over_fp64"] = round(r32["gcells"] / r64["gcells"], 3)
-        out["fp32_vs_fp64_detector_relL2"] = rel_l2(r32["series"], r64["series"])
+
+
+FIELDS = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
+
+
+def validation_run(case, prec, mode, fields=True):
+    """The whole workload (case.steps steps from step 1) at one precision:
+    detector series, the final fields (all six, reference bounds) and the
+    engine for further reads."""
+    g = build_grid(case, prec, mode)
+    g.run(1, case.steps, case.record_every)
+    out = {"series": g.read_records()}
+    if fields:
+        out["fields"] = np.concatenate([g.get_field(f).ravel() for f in FIELDS])
+    return g, out
+
+
+def fp32_error(case, mode, fields=True, series=None):
+    """FP32 against FP64 over the workload's full length (case.steps steps,
+    whatever the timed window is): relative L2 of the detector series and of
+    the six final fields.  `series` (prec -> recorded series) reuses timed
+    runs that already covered the full length.  Raises when the detectors
+    saw no signal, so that no result goes out without a stated error."""
+    res = {}
+    for prec in (4, 8):
+        if series is not None and not fields:
+            res[prec] = {"series": series[prec]}
+            continue
+        log(f"[bench] validation: FP{8 * prec}, {case.steps} steps")
+        g, res[prec] = validation_run(case, prec, mode, fields)
+        g.kill()
+    det = rel_l2(res[4]["series"], res[8]["series"])
+    if det is None:
+        raise RuntimeError(f"validation window of {case.steps} steps recorded no signal at the detectors")
+    out = {"steps": case.steps, "detector_relL2": det, "detector_samples": int(res[8]["series"].size)}
+    if fields:
+        out["field_relL2"] = rel_l2(res[4]["fields"], res[8]["fields"])
+    log(f"[bench] validation: FP32 vs FP64 detector relL2 {det:.3e}"
+        + (f", final-field relL2 {out['field_relL2']:.3e}" if fields else ""))
+    return out
 
 
 def single_grid_runs(case, args, W, K, workload, energy_every=0, extra=None):
@@ -301,11 +349,16 @@ def finish(out, results):
     return out
 
 
+HEADLINE_STEPS = 4000   # the workload's length: the pulse crosses the detector plane within it
+
+
 def bench_headline(args):
     W, K = args.warmup, args.steps
     case = cases.config_dl_sphere_tfsf(n=args.n, steps=W + K)
     results = single_grid_runs(case, args, W, K, HEADLINE)
     r32 = results[4]
+    full = cases.config_dl_sphere_tfsf(n=args.n, steps=max(HEADLINE_STEPS, W + K))
+    val = fp32_error(full, args.mode)
     cpu = None
     if not args.no_cpu:
         log(f"[bench] CPU reference baseline: {args.cpu_steps} steps")
@@ -317,10 +370,24 @@ def bench_headline(args):
                        "detectors": "plane_xy Ex, Ey, Ez at k=%d" % case.info["detector_k"],
                        "step_kernels": "fused" if r32["fused"] else "separate"})
     finish(out, results)
+    out["fp32_vs_fp64_detector_relL2"] = val["detector_relL2"]
+    out["fp32_vs_fp64_validation"] = val
     out["cpu_baseline"] = cpu
     if cpu:
         out["gpu_over_cpu"] = round(r32["gcells"] / cpu["value"], 1)
     return out
+
+
+def workload_error(args, W, K, results, full_case):
+    """FP32 vs FP64 detector error over the workload's full length: from the
+    timed runs when they covered it (warm-up and timed steps record from step
+    1), otherwise from validation runs of the full length."""
+    if 8 in results and W + K >= full_case.steps:
+        det = rel_l2(results[4]["series"], results[8]["series"])
+        if det is None:
+            raise RuntimeError("the detectors recorded no signal")
+        return {"steps": W + K, "detector_relL2": det, "detector_samples": int(results[8]["series"].size)}
+    return fp32_error(full_case, args.mode, fields=False)
 
 
 def bench_ellipsoids(args):
@@ -335,6 +402,9 @@ def bench_ellipsoids(args):
                        "detectors": "32 point Ez, every 100 steps", "energy_every": 100,
                        "step_kernels": "fused" if r32["fused"] else "separate"})
     finish(out, results)
+    val = workload_error(args, W, K, results, cases.config_ellipsoids(steps=max(10000, W + K)))
+    out["fp32_vs_fp64_detector_relL2"] = val["detector_relL2"]
+    out["fp32_vs_fp64_validation"] = val
     out["device_bytes_fp32"] = r32["dev_bytes"]
     if 8 in results:
         out["fp32_vs_fp64_energy_relL2"] = rel_l2(r32["energies"], results[8]["energies"])
@@ -358,6 +428,9 @@ def bench_emitters(args):
                        "dl_component_cells": r32["comp_cells"], "detectors": "8 point Ex, Ey, Ez, every 100 steps",
                        "step_kernels": "fused" if r32["fused"] else "separate"})
     finish(out, results)
+    val = workload_error(args, W, K, results, cases.config_emitters(steps=max(20000, W + K)))
+    out["fp32_vs_fp64_detector_relL2"] = val["detector_relL2"]
+    out["fp32_vs_fp64_validation"] = val
     if 8 in results:
         out["fp32_vs_fp64_emitter_state_relL2"] = rel_l2(r32["tls"], results[8]["tls"])
     return out

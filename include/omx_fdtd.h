@@ -181,7 +181,11 @@ int omx3d_read_records(omx3d_ctx* ctx, double* out, int32_t* rows);
 int omx3d_sample(omx3d_ctx* ctx, double* out);
 
 /* ---- embedded quantum systems, "material" groups ------------------------ */
-/* Sites: map(n,3:5) of q_group_mod.f90:505-509 as an (n,3) Fortran array. */
+/* Sites: map(n,3:5) of q_group_mod.f90:505-509 as an (n,3) Fortran array.
+ * dt_q (au): the requested quantum step.  As the driver loop does
+ * (mxim_mxll.f90:73-79), the systems move every n_skip = NINT(dt_q/dt) steps
+ * when dt_q > dt (else every step) and are propagated over n_skip*dt;
+ * omx3d_run sets t_skip at each new quantum step (mxim_mxll.f90:97-104). */
 int omx3d_set_q_sites(omx3d_ctx* ctx, int32_t n, const int32_t* ijk, double density, double dt_q);
 int omx3d_gather_e(omx3d_ctx* ctx, double* e_sites);              /* (n,3) */
 int omx3d_scatter_j(omx3d_ctx* ctx, const double* dpdt_sites);    /* (n,3) */
@@ -236,7 +240,11 @@ int omx3d_autotune(omx3d_ctx* ctx, int32_t steps, int32_t* rows_y, int32_t* xchu
 /* One launch per phase (H, sources, E, detectors) covers every grid of the
  * batch; each grid computes exactly what its own omx3d_run computes.  The
  * grids are set up with the single-grid calls first (same precision, no
- * TF/SF or quantum sites); use_graph replays a captured step graph. */
+ * TF/SF or quantum sites); use_graph replays a captured step graph.  The
+ * grids stay owned by the caller and must outlive the batch; omxb_run picks
+ * up changes made to them since the last call (buffers, sources, detectors,
+ * tiling, record buffer).  Recording into a full detector buffer fails as in
+ * omx3d_run. */
 typedef struct omxb_ctx omxb_ctx;
 int omxb_create(int32_t n, omx3d_ctx* const* grids, omxb_ctx** out);
 int omxb_run(omxb_ctx* batch, int32_t tt_first, int32_t n_steps, int32_t record_every, int32_t use_graph);

@@ -578,11 +578,20 @@ int run_impl(omx3d_ctx* c, int tt_first, int n_steps, int record_every) {
     rc = tfsf_fold_check(c);
     if (rc) return rc;
     const bool fold = !fused && c->tfsf, fold_e = fold && c->tf_fold_e == 1;
+    const int tt_0 = c->n_skip == 1 ? 1 : 0;
     for (int tt = tt_first; tt < tt_first + n_steps; ++tt) {
         const double time = dt * ((double)tt - 0.5);
-        // n_skip_steps = 1: a new quantum step every step, so t_skip = time
-        // (mxim_mxll.f90:73-77,97-101).
-        const double t_skip = time;
+        // a new quantum step starts when tq_step = 1 + int((tt - tt_0)/n_skip)
+        // grows: t_skip = time and the quantum systems move this step
+        // (mxim_mxll.f90:76-77,97-104); with n_skip = 1 that is every step
+        const int tq_step = 1 + (tt - tt_0) / c->n_skip;
+        bool move = false;
+        if (tq_step > c->tq_step_old) {
+            c->t_skip = time;
+            c->tq_step_old = tq_step;
+            move = c->nq > 0;
+        }
+        const double t_skip = c->t_skip;
         // profiling: events bracket the H and E kernels themselves (the
         // fused kernel takes the E slot, the H slot stays empty)
         const bool prof = pe > 0 && (tt - tt_first) % pe == 0;
@@ -613,8 +622,10 @@ int run_impl(omx3d_ctx* c, int tt_first, int n_steps, int record_every) {
             rc = launch_tfsf_e<T>(c, dt * (double)tt);
             if (rc) return rc;
         }
-        rc = launch_two_level<T>(c);
-        if (rc) return rc;
+        if (move) {
+            rc = launch_two_level<T>(c);
+            if (rc) return rc;
+        }
         if (record_every > 0 && tt % record_every == 0 && c->n_samples > 0) {
             if (c->rec_rows >= c->rec_cap) return fail(OMX_ERR_RANGE, "detector record buffer full");
             rc = launch_record<T>(c, c->d_rec + (long)c->rec_rows * c->n_samples);
@@ -893,6 +904,7 @@ struct omxb_ctx {
     int* d_disp_grid = nullptr;     // grid of each workgroup of dispersive cells
     int n_disp = 0;                 // list entries (whole workgroups, padding -1)
     int disp_blocks = 0;
+    std::vector<std::vector<long>> sig;   // per grid: its state when the argument blocks were built
 };
 
 namespace {
@@ -906,8 +918,25 @@ int bupload(hipStream_t s, P** d, const std::vector<P>& h) {
     return OMX_OK;
 }
 
+// What the batch's argument blocks, source and detector lists are built from:
+// the current field / psi buffers (the fused step swaps them), tiling,
+// addressing, media, sources, detectors and the record buffer.
+std::vector<long> batch_signature(const omx3d_ctx* c) {
+    std::vector<long> v;
+    for (int q = 0; q < 6; ++q) { v.push_back((long)(uintptr_t)c->f[q]); v.push_back((long)(uintptr_t)c->psi_h[q]); }
+    v.push_back((long)(uintptr_t)c->disp_state);
+    v.push_back((long)(uintptr_t)c->d_rec);
+    v.push_back((long)(uintptr_t)c->d_disp);
+    v.push_back(c->rows_y); v.push_back(c->xchunk); v.push_back(c->tile_order); v.push_back(c->offset_bits);
+    v.push_back((long)c->sources.size()); v.push_back(c->n_samples); v.push_back(c->n_disp);
+    v.push_back(c->rec_cap); v.push_back(c->media_set ? 1 : 0);
+    return v;
+}
+
 template <typename T>
 int batch_setup(omxb_ctx* b) {
+    b->sig.clear();
+    for (auto* c : b->g) b->sig.push_back(batch_signature(c));
     std::vector<YeeArgs<T>> args;
     std::vector<int> ts;
     int tot = 0;
@@ -923,6 +952,7 @@ int batch_setup(omxb_ctx* b) {
     for (auto* c : b->g) b->rows = std::max(b->rows, c->rows_y);
     b->tile_order = b->g[0]->tile_order;
     b->ntz = args[0].ntz;
+    b->b32 = true;
     for (auto* c : b->g) b->b32 = b->b32 && offsets32(c);
     // cells of dispersive classes of all grids, for the first workgroups of
     // the batched E launch: each grid's cells fill whole workgroups (padded
@@ -1054,26 +1084,42 @@ int batch_step_launches(omxb_ctx* b) {
 
 template <typename T>
 int batch_run(omxb_ctx* b, int tt_first, int n_steps, int every, int use_graph) {
-    std::vector<BatchRecord> br(b->n);
-    int first_tt = -1;
-    if (every > 0)
-        for (int tt = tt_first; tt < tt_first + n_steps; ++tt)
-            if (tt % every == 0) { first_tt = tt; break; }
-    int nrec = 0;
-    if (every > 0)
-        for (int tt = tt_first; tt < tt_first + n_steps; ++tt) nrec += (tt % every == 0);
+    // the grids' state may have changed since the argument blocks were built
+    // (a fused single-grid run swaps the field buffers; sources, detectors or
+    // the record buffer may have been added): build them again then
     for (int q = 0; q < b->n; ++q) {
         omx3d_ctx* c = b->g[q];
         int rc = upload_detectors(c);
         if (rc) return rc;
         HIPCHK(hipStreamSynchronize(c->s));
+    }
+    bool stale = false;
+    for (int q = 0; q < b->n; ++q) stale = stale || batch_signature(b->g[q]) != b->sig[q];
+    if (stale) {
+        int rc = batch_setup<T>(b);
+        if (rc) return rc;
+    }
+    std::vector<BatchRecord> br(b->n);
+    bool found = false;
+    int first_tt = 0;
+    if (every > 0)
+        for (int tt = tt_first; tt < tt_first + n_steps; ++tt)
+            if (tt % every == 0) { first_tt = tt; found = true; break; }
+    int nrec = 0;
+    if (every > 0)
+        for (int tt = tt_first; tt < tt_first + n_steps; ++tt) nrec += (tt % every == 0);
+    for (int q = 0; q < b->n; ++q) {
+        omx3d_ctx* c = b->g[q];
         br[q].rec = c->d_rec;
         br[q].n_samples = c->n_samples;
-        br[q].every = (c->n_samples > 0 && c->rec_cap > 0) ? every : 0;
-        br[q].row0 = first_tt > 0 ? first_tt / every - c->rec_rows : 0;
+        br[q].every = c->n_samples > 0 ? every : 0;
+        // row of step tt: tt/every - row0, so that the first recorded step
+        // of this call lands on the grid's next free row
+        br[q].row0 = found ? first_tt / every - c->rec_rows : 0;
         br[q].cap = c->rec_cap;
-        if (br[q].every > 0 && c->rec_rows + nrec > c->rec_cap)
-            return fail(OMX_ERR_RANGE, "batch grid " + std::to_string(q) + ": detector record buffer too small");
+        // as omx3d_run: recording into a full (or absent) buffer is an error
+        if (br[q].every > 0 && nrec > 0 && c->rec_rows + nrec > c->rec_cap)
+            return fail(OMX_ERR_RANGE, "batch grid " + std::to_string(q) + ": detector record buffer full");
     }
     int rc = bupload(b->s, &b->d_br, br);
     if (rc) return rc;
@@ -1107,7 +1153,7 @@ int batch_run(omxb_ctx* b, int tt_first, int n_steps, int every, int use_graph) 
     HIPCHK(hipMemcpy(&ov, b->d_overflow, sizeof(int), hipMemcpyDeviceToHost));
     if (ov) return fail(OMX_ERR_RANGE, "batch detector rows out of range");
     for (auto* c : b->g)
-        if (c->n_samples > 0 && c->rec_cap > 0) c->rec_rows += nrec;
+        if (c->n_samples > 0) c->rec_rows += nrec;
     return OMX_OK;
 }
 
@@ -1626,7 +1672,11 @@ int omx3d_set_q_sites(omx3d_ctx* c, int32_t n, const int32_t* ijk, double densit
     c->nq = n;
     c->nslot = ns;
     c->q_density = density;
-    c->q_dt = dt_q;
+    // mxim_mxll.f90:73-79: n_skip_steps = NINT(dt_q/dt) when dt_q > dt, else
+    // 1; the quantum step is then n_skip_steps * dt
+    c->n_skip = dt_q > c->d.dt ? (int)std::lround(dt_q / c->d.dt) : 1;
+    c->q_dt = (double)c->n_skip * c->d.dt;
+    c->tq_step_old = 0;
     return OMX_OK;
 }
 

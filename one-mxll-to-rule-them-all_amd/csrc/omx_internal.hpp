@@ -176,6 +176,11 @@ struct omx3d_ctx {
     // quantum sites ("material" group)
     int nq = 0, nslot = 0;
     double q_density = 0.0, q_dt = 0.0;
+    // quantum sub-stepping of the driver loop (mxim_mxll.f90:73-104): the
+    // systems move every n_skip steps, q_dt = n_skip * dt; tq_step_old and
+    // t_skip carry over from one omx3d_run call to the next
+    int n_skip = 1, tq_step_old = 0;
+    double t_skip = 0.0;
     long* d_q_off = nullptr;        // per site: cell offset
     int* d_q_winner = nullptr;      // per slot: last site mapping to it
     double* d_J = nullptr;          // per slot: Jx Jy Jz Jold(3) dJ(3) -> 9*nslot

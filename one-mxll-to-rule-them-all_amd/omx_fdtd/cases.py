@@ -36,6 +36,7 @@ class Case:
     detectors: list = field(default_factory=list) # (type, field, indx_list)
     q_sites: object = None                        # (n,3) int indices
     q_density: float = 0.0
+    q_skip: int = 1                               # quantum step dt_q = q_skip * dt (n_skip_steps)
     two_level: dict = None                        # omega0, mu, pol, gamma2 (au)
     tfsf: dict = None                             # lo, hi, e0, w0, t0, tau
     info: dict = field(default_factory=dict)
@@ -93,7 +94,7 @@ def apply(case, g, library=None, make_medium=None, rasterize=None, record=True):
     for (dtype, fld, idx) in case.detectors:
         g.add_detector(dtype, fld, idx)
     if case.q_sites is not None:
-        g.set_q_sites(case.q_sites, case.q_density, dt)
+        g.set_q_sites(case.q_sites, case.q_density, case.q_skip * dt)
         if case.two_level is not None:
             t = case.two_level
             g.set_two_level(t["omega0"], t["mu"], t["pol"], t["gamma2"])

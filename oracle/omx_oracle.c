@@ -98,6 +98,7 @@ typedef struct orc_sim {
     long n_samples;
     /* quantum sites */
     int nq; int *q_ijk; double q_density, q_dt;
+    int n_skip, tq_step_old; double t_skip;   /* quantum sub-stepping (mxim_mxll.f90:73-104) */
     double *tls;             /* 16 per site, same packing as the engine */
     /* tf/sf */
     int tfsf, tf_lo[3], tf_hi[3], tf_kb, tf_n, tf_ks;
@@ -134,6 +135,7 @@ orc_sim* orc_create(int nx, int ny, int nz, int npml, const int* bc, double dt, 
     s->dt = dt; s->dr = dr;
     s->dt_eps0 = dt / EPS0;
     s->dt_mu0 = dt / MU0;
+    s->n_skip = 1;
     arr_init(&s->Ex, 0, nx, 1, ny + 1, 1, nz + 1);
     arr_init(&s->Ey, 1, nx + 1, 0, ny, 1, nz + 1);
     arr_init(&s->Ez, 1, nx + 1, 1, ny + 1, 0, nz);
@@ -410,41 +412,47 @@ void orc_step_h(orc_sim* s) {
         for (int j = 1; j <= ny + 1; ++j) for (int i = 0; i <= nx; ++i) AT(s->Ex, i, j, nz + 1) = AT(s->Ex, i, j, 1);
         for (int j = 0; j <= ny; ++j) for (int i = 1; i <= nx + 1; ++i) AT(s->Ey, i, j, nz + 1) = AT(s->Ey, i, j, 1);
     }
+    /* the reference runs i, j, k with k innermost (mxll_3D_mod.f90:548-550);
+     * every cell is independent, so the order here (i innermost, the arrays'
+     * fastest index) gives the same values */
 #pragma omp parallel
     {
-#pragma omp for collapse(3) schedule(static) nowait
-        for (int i = 1; i <= nx; ++i)
+#pragma omp for collapse(2) schedule(static) nowait
+        for (int k = 1; k <= nz; ++k)
             for (int j = 1; j <= ny; ++j)
-                for (int k = 1; k <= nz; ++k) {
+                for (int i = 1; i <= nx; ++i) {
                     double rotE = (AT(s->Ez, i, j, k) - AT(s->Ez, i, j + 1, k)) * s->den_hy[j] +
                                   (AT(s->Ey, i, j, k + 1) - AT(s->Ey, i, j, k)) * s->den_hz[k];
                     AT(s->Hx, i, j, k) = AT(s->Hx, i, j, k) + dt_mu0 * rotE;
                 }
-#pragma omp for collapse(3) schedule(static) nowait
-        for (int i = 1; i <= nx; ++i)
+#pragma omp for collapse(2) schedule(static) nowait
+        for (int k = 1; k <= nz; ++k)
             for (int j = 1; j <= ny; ++j)
-                for (int k = 1; k <= nz; ++k) {
+                for (int i = 1; i <= nx; ++i) {
                     double rotE = (AT(s->Ez, i + 1, j, k) - AT(s->Ez, i, j, k)) * s->den_hx[i] +
                                   (AT(s->Ex, i, j, k) - AT(s->Ex, i, j, k + 1)) * s->den_hz[k];
                     AT(s->Hy, i, j, k) = AT(s->Hy, i, j, k) + dt_mu0 * rotE;
                 }
-#pragma omp for collapse(3) schedule(static)
-        for (int i = 1; i <= nx; ++i)
+#pragma omp for collapse(2) schedule(static)
+        for (int k = 1; k <= nz; ++k)
             for (int j = 1; j <= ny; ++j)
-                for (int k = 1; k <= nz; ++k) {
+                for (int i = 1; i <= nx; ++i) {
                     double rotE = (AT(s->Ey, i, j, k) - AT(s->Ey, i + 1, j, k)) * s->den_hx[i] +
                                   (AT(s->Ex, i, j + 1, k) - AT(s->Ex, i, j, k)) * s->den_hy[j];
                     AT(s->Hz, i, j, k) = AT(s->Hz, i, j, k) + dt_mu0 * rotE;
                 }
     }
-    if (s->cpml_pos[0])
+    if (s->cpml_pos[0]) {
+        #pragma omp parallel for schedule(static)
         for (int j = 1; j <= ny; ++j) for (int k = 1; k <= nz; ++k) for (int i = 1; i <= npml - 1; ++i) {
             AT(s->psix_Hyx, i, j, k) = s->bh[i] * AT(s->psix_Hyx, i, j, k) + s->ch[i] * (AT(s->Ez, i + 1, j, k) - AT(s->Ez, i, j, k)) / dr;
             AT(s->psix_Hzx, i, j, k) = s->bh[i] * AT(s->psix_Hzx, i, j, k) + s->ch[i] * (AT(s->Ey, i, j, k) - AT(s->Ey, i + 1, j, k)) / dr;
             AT(s->Hy, i, j, k) = AT(s->Hy, i, j, k) + dt_mu0 * AT(s->psix_Hyx, i, j, k);
             AT(s->Hz, i, j, k) = AT(s->Hz, i, j, k) + dt_mu0 * AT(s->psix_Hzx, i, j, k);
         }
-    if (s->cpml_pos[1])
+    }
+    if (s->cpml_pos[1]) {
+        #pragma omp parallel for schedule(static)
         for (int j = 1; j <= ny; ++j) for (int k = 1; k <= nz; ++k) {
             int ii = 2 * (npml - 1);
             for (int i = nx + 1 - npml; i <= nx - 1; ++i) {
@@ -455,14 +463,18 @@ void orc_step_h(orc_sim* s) {
                 ii = ii - 1;
             }
         }
-    if (s->cpml_pos[2])
+    }
+    if (s->cpml_pos[2]) {
+        #pragma omp parallel for schedule(static)
         for (int k = 1; k <= nz; ++k) for (int i = 1; i <= nx; ++i) for (int j = 1; j <= npml - 1; ++j) {
             AT(s->psiy_Hxy, i, j, k) = s->bh[j] * AT(s->psiy_Hxy, i, j, k) + s->ch[j] * (AT(s->Ez, i, j, k) - AT(s->Ez, i, j + 1, k)) / dr;
             AT(s->psiy_Hzy, i, j, k) = s->bh[j] * AT(s->psiy_Hzy, i, j, k) + s->ch[j] * (AT(s->Ex, i, j + 1, k) - AT(s->Ex, i, j, k)) / dr;
             AT(s->Hx, i, j, k) = AT(s->Hx, i, j, k) + dt_mu0 * AT(s->psiy_Hxy, i, j, k);
             AT(s->Hz, i, j, k) = AT(s->Hz, i, j, k) + dt_mu0 * AT(s->psiy_Hzy, i, j, k);
         }
-    if (s->cpml_pos[3])
+    }
+    if (s->cpml_pos[3]) {
+        #pragma omp parallel for schedule(static)
         for (int k = 1; k <= nz; ++k) for (int i = 1; i <= nx; ++i) {
             int jj = 2 * (npml - 1);
             for (int j = ny + 1 - npml; j <= ny - 1; ++j) {
@@ -473,14 +485,18 @@ void orc_step_h(orc_sim* s) {
                 jj = jj - 1;
             }
         }
-    if (s->cpml_pos[4])
+    }
+    if (s->cpml_pos[4]) {
+        #pragma omp parallel for schedule(static)
         for (int i = 1; i <= nx; ++i) for (int j = 1; j <= ny; ++j) for (int k = 1; k <= npml - 1; ++k) {
             AT(s->psiz_Hxz, i, j, k) = s->bh[k] * AT(s->psiz_Hxz, i, j, k) + s->ch[k] * (AT(s->Ey, i, j, k + 1) - AT(s->Ey, i, j, k)) / dr;
             AT(s->psiz_Hyz, i, j, k) = s->bh[k] * AT(s->psiz_Hyz, i, j, k) + s->ch[k] * (AT(s->Ex, i, j, k) - AT(s->Ex, i, j, k + 1)) / dr;
             AT(s->Hx, i, j, k) = AT(s->Hx, i, j, k) + dt_mu0 * AT(s->psiz_Hxz, i, j, k);
             AT(s->Hy, i, j, k) = AT(s->Hy, i, j, k) + dt_mu0 * AT(s->psiz_Hyz, i, j, k);
         }
-    if (s->cpml_pos[5])
+    }
+    if (s->cpml_pos[5]) {
+        #pragma omp parallel for schedule(static)
         for (int i = 1; i <= nx; ++i) for (int j = 1; j <= ny; ++j) {
             int kk = 2 * (npml - 1);
             for (int k = nz + 1 - npml; k <= nz - 1; ++k) {
@@ -491,6 +507,7 @@ void orc_step_h(orc_sim* s) {
                 kk = kk - 1;
             }
         }
+    }
 }
 
 /* td_propagate_E_3Dfield (mxll_3D_mod.f90:700-923) */
@@ -564,14 +581,17 @@ void orc_step_e(orc_sim* s, double time, double t_skip) {
     }
 #undef MM
 #undef PIDX
-    if (s->cpml_pos[0])
+    if (s->cpml_pos[0]) {
+        #pragma omp parallel for schedule(static)
         for (int j = 1; j <= ny; ++j) for (int k = 1; k <= nz; ++k) for (int i = 2; i <= npml; ++i) {
             AT(s->psix_Eyx, i, j, k) = s->be[i] * AT(s->psix_Eyx, i, j, k) + s->ce[i] * (AT(s->Hz, i - 1, j, k) - AT(s->Hz, i, j, k)) / dr;
             AT(s->psix_Ezx, i, j, k) = s->be[i] * AT(s->psix_Ezx, i, j, k) + s->ce[i] * (AT(s->Hy, i, j, k) - AT(s->Hy, i - 1, j, k)) / dr;
             AT(s->Ey, i, j, k) = AT(s->Ey, i, j, k) + dt_eps0 * AT(s->psix_Eyx, i, j, k);
             AT(s->Ez, i, j, k) = AT(s->Ez, i, j, k) + dt_eps0 * AT(s->psix_Ezx, i, j, k);
         }
-    if (s->cpml_pos[1])
+    }
+    if (s->cpml_pos[1]) {
+        #pragma omp parallel for schedule(static)
         for (int j = 1; j <= ny; ++j) for (int k = 1; k <= nz; ++k) {
             int ii = 2 * npml;
             for (int i = nx + 1 - npml; i <= nx - 1; ++i) {
@@ -582,14 +602,18 @@ void orc_step_e(orc_sim* s, double time, double t_skip) {
                 ii = ii - 1;
             }
         }
-    if (s->cpml_pos[2])
+    }
+    if (s->cpml_pos[2]) {
+        #pragma omp parallel for schedule(static)
         for (int k = 1; k <= nz; ++k) for (int i = 1; i <= nx; ++i) for (int j = 2; j <= npml; ++j) {
             AT(s->psiy_Exy, i, j, k) = s->be[j] * AT(s->psiy_Exy, i, j, k) + s->ce[j] * (AT(s->Hz, i, j, k) - AT(s->Hz, i, j - 1, k)) / dr;
             AT(s->psiy_Ezy, i, j, k) = s->be[j] * AT(s->psiy_Ezy, i, j, k) + s->ce[j] * (AT(s->Hx, i, j - 1, k) - AT(s->Hx, i, j, k)) / dr;
             AT(s->Ex, i, j, k) = AT(s->Ex, i, j, k) + dt_eps0 * AT(s->psiy_Exy, i, j, k);
             AT(s->Ez, i, j, k) = AT(s->Ez, i, j, k) + dt_eps0 * AT(s->psiy_Ezy, i, j, k);
         }
-    if (s->cpml_pos[3])
+    }
+    if (s->cpml_pos[3]) {
+        #pragma omp parallel for schedule(static)
         for (int k = 1; k <= nz; ++k) for (int i = 1; i <= nx; ++i) {
             int jj = 2 * npml;
             for (int j = ny + 1 - npml; j <= ny - 1; ++j) {
@@ -600,14 +624,18 @@ void orc_step_e(orc_sim* s, double time, double t_skip) {
                 jj = jj - 1;
             }
         }
-    if (s->cpml_pos[4])
+    }
+    if (s->cpml_pos[4]) {
+        #pragma omp parallel for schedule(static)
         for (int i = 1; i <= nx; ++i) for (int j = 1; j <= ny; ++j) for (int k = 2; k <= npml; ++k) {
             AT(s->psiz_Exz, i, j, k) = s->be[k] * AT(s->psiz_Exz, i, j, k) + s->ce[k] * (AT(s->Hy, i, j, k - 1) - AT(s->Hy, i, j, k)) / dr;
             AT(s->psiz_Eyz, i, j, k) = s->be[k] * AT(s->psiz_Eyz, i, j, k) + s->ce[k] * (AT(s->Hx, i, j, k) - AT(s->Hx, i, j, k - 1)) / dr;
             AT(s->Ex, i, j, k) = AT(s->Ex, i, j, k) + dt_eps0 * AT(s->psiz_Exz, i, j, k);
             AT(s->Ey, i, j, k) = AT(s->Ey, i, j, k) + dt_eps0 * AT(s->psiz_Eyz, i, j, k);
         }
-    if (s->cpml_pos[5])
+    }
+    if (s->cpml_pos[5]) {
+        #pragma omp parallel for schedule(static)
         for (int i = 1; i <= nx; ++i) for (int j = 1; j <= ny; ++j) {
             int kk = 2 * npml;
             for (int k = nz + 1 - npml; k <= nz - 1; ++k) {
@@ -618,6 +646,7 @@ void orc_step_e(orc_sim* s, double time, double t_skip) {
                 kk = kk - 1;
             }
         }
+    }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -738,7 +767,10 @@ void orc_set_q_sites(orc_sim* s, int n, const int* ijk, double density, double d
     s->nq = n;
     s->q_ijk = (int*)malloc(sizeof(int) * 3 * n);
     memcpy(s->q_ijk, ijk, sizeof(int) * 3 * n);
-    s->q_density = density; s->q_dt = dt_q;
+    /* mxim_mxll.f90:73-79: n_skip_steps = NINT(dt_q/dt) when dt_q > dt, else 1,
+     * and the quantum step becomes n_skip_steps*dt */
+    s->n_skip = dt_q > s->dt ? (int)lround(dt_q / s->dt) : 1;
+    s->q_density = density; s->q_dt = (double)s->n_skip * s->dt;
     /* J arrays (mxll_3D_mod.f90:132-143) */
     arr_init(&s->Jx, 1, nx + 1, 1, ny, 1, nz); arr_init(&s->Jy, 1, nx, 1, ny + 1, 1, nz); arr_init(&s->Jz, 1, nx, 1, ny, 1, nz + 1);
     arr_init(&s->Jx_old, 1, nx + 1, 1, ny, 1, nz); arr_init(&s->Jy_old, 1, nx, 1, ny + 1, 1, nz); arr_init(&s->Jz_old, 1, nx, 1, ny, 1, nz + 1);
@@ -891,20 +923,30 @@ static void tfsf_e(orc_sim* s, double t_e) {
 }
 
 /* ------------------------------------------------------------------------ */
-/* the driver loop of mxim_mxll.f90:92-132 with n_skip_steps = 1 */
+/* the driver loop of mxim_mxll.f90:92-132: the quantum systems move, and
+ * t_skip is set, on the steps that start a new quantum step,
+ * tq_step = 1 + int((tt - tt_0)/n_skip_steps) with tt_0 = 1 when
+ * n_skip_steps = 1 and 0 otherwise (mxim_mxll.f90:76-77,97-104) */
 long orc_n_samples(const orc_sim* s) { return s->n_samples; }
 
 void orc_run(orc_sim* s, int tt_first, int n_steps, int record_every, double* rec, int* rows) {
     int r = 0;
+    const int tt_0 = s->n_skip == 1 ? 1 : 0;
     for (int tt = tt_first; tt < tt_first + n_steps; ++tt) {
         const double time = s->dt * ((double)tt - 0.5);
-        const double t_skip = time;
+        const int tq_step = 1 + (tt - tt_0) / s->n_skip;
+        int move = 0;
+        if (tq_step > s->tq_step_old) {
+            s->t_skip = time;
+            s->tq_step_old = tq_step;
+            move = s->nq > 0;
+        }
         orc_step_h(s);
         if (s->tfsf) tfsf_h(s);
         orc_inject_sources(s, time);
-        orc_step_e(s, time, t_skip);
+        orc_step_e(s, time, s->t_skip);
         if (s->tfsf) tfsf_e(s, s->dt * (double)tt);
-        if (s->tls) orc_two_level_step(s);
+        if (s->tls && move) orc_two_level_step(s);
         if (record_every > 0 && tt % record_every == 0 && s->n_samples > 0) {
             if (rec) orc_sample(s, rec + (long)r * s->n_samples);
             ++r;

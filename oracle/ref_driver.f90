@@ -37,11 +37,11 @@ program omx_ref_driver
 
     integer  :: nx = 16, ny = 16, nz = 16, npml = 4, bc(3) = 3
     integer  :: n_steps = 10, record_every = 1, n_media = 0, n_src = 0, n_det = 0, n_q = 0
-    integer  :: timing = 0
+    integer  :: timing = 0, n_skip = 1
     real(dp) :: dx_nm = 5.0d0, q_density_nm3 = 1.0d-3
     real(dp) :: diel_eps(15) = 1.0d0
     namelist /case/ nx, ny, nz, npml, bc, n_steps, record_every, n_media, n_src, n_det, n_q, &
-                    timing, dx_nm, q_density_nm3, diel_eps
+                    timing, dx_nm, q_density_nm3, diel_eps, n_skip
 
     type(TMxll_3D)               :: mxll
     type(TSource), allocatable   :: source_list(:)
@@ -57,6 +57,9 @@ program omx_ref_driver
     integer,  allocatable :: q_ijk(:, :)
     real(dp), allocatable :: q_amp(:, :), e_sites(:, :), dpdt(:, :)
     integer(8) :: c0_clock, c1_clock, rate
+    integer  :: tt_0
+    logical  :: move_q_system
+    real(dp) :: dt_q
 
     open(newunit=funit, file="case.nml", action="read", iostat=ierr)
     if (ierr /= 0) error stop "case.nml missing"
@@ -144,21 +147,30 @@ program omx_ref_driver
         close(funit)
         if (timing == 0) then
             open(unit=77, file="qsites_e.bin", access="stream", form="unformatted", status="replace")
+            open(unit=78, file="qmoves.bin", access="stream", form="unformatted", status="replace")
         end if
     end if
 
-    ! ---- time loop (mxim_mxll.f90:92-132, n_skip_steps = 1) ----------------
-    mxll%n_skip_steps = 1
+    ! ---- time loop (mxim_mxll.f90:73-132): the quantum sub-step bookkeeping
+    ! with n_skip_steps given directly (the reference derives it from dt_q,
+    ! mxim_mxll.f90:73-74), the quantum step dt_q = n_skip_steps*dt (:79)
+    mxll%n_skip_steps = n_skip
+    tt_0 = 0
+    if (mxll%n_skip_steps == 1) tt_0 = 1
     mxll%tq_step_old = 0
+    dt_q = dble(mxll%n_skip_steps) * dt
     print_det_step = 0
     call system_clock(c0_clock, rate)
     do tt = 1, n_steps
         time = dt * (DBLE(tt) - 0.5d0)
         mxll%time = time
-        mxll%tq_step = 1 + (tt - 1)
+        mxll%tq_step = 1 + int((tt - tt_0) / mxll%n_skip_steps)
+        move_q_system = .false.
         if (mxll%tq_step > mxll%tq_step_old) then
             mxll%t_skip = time
             mxll%tq_step_old = mxll%tq_step
+            move_q_system = .true.
+            if (n_q == 0) move_q_system = .false.
         end if
 
         call mxll%td_propagate_H_field()
@@ -166,9 +178,10 @@ program omx_ref_driver
         call inject_sources()
         call mxll%td_propagate_E_field(tt)
 
-        if (n_q > 0) then
+        if (move_q_system) then
             call gather_e()
             if (timing == 0) write(77) e_sites
+            if (timing == 0) write(78) tt
             do c = 1, 3
                 dpdt(:, c) = q_amp(:, c) * sin(0.3d0 * tt + (c - 1))
             end do
@@ -186,6 +199,7 @@ program omx_ref_driver
     end do
     call system_clock(c1_clock)
     if (n_q > 0 .and. timing == 0) close(77)
+    if (n_q > 0 .and. timing == 0) close(78)
     write(*, '("REF_TIMING steps=", I0, " seconds=", ES16.8, " threads=", I0, " cells=", I0)') &
         n_steps, dble(c1_clock - c0_clock) / dble(rate), omp_get_max_threads(), nx * ny * nz
     call mxll%kill()
@@ -251,9 +265,9 @@ contains
             mxll%Jy(ii, jj, kk) = density * dpdt(q, 2)
             mxll%Jz(ii, jj, kk) = density * dpdt(q, 3)
         end do
-        mxll%dJx = (mxll%Jx - mxll%Jx_old) / dt
-        mxll%dJy = (mxll%Jy - mxll%Jy_old) / dt
-        mxll%dJz = (mxll%Jz - mxll%Jz_old) / dt
+        mxll%dJx = (mxll%Jx - mxll%Jx_old) / dt_q
+        mxll%dJy = (mxll%Jy - mxll%Jy_old) / dt_q
+        mxll%dJz = (mxll%Jz - mxll%Jz_old) / dt_q
     end subroutine scatter_j
 
 end program omx_ref_driver

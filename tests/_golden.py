@@ -9,7 +9,7 @@ from omx_fdtd import cases, host
 from omx_fdtd.units import DETECTOR_CODES, FIELD_CODES, NM_TO_AU, index_to_nm, grid_dr_dt
 
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-NAMES = ("vacuum64", "media_mix", "au_al_periodic", "qsites")
+NAMES = ("vacuum64", "media_mix", "au_al_periodic", "qsites", "qsites_skip3")
 
 
 def load(name):
@@ -52,9 +52,25 @@ def to_case(desc, fx):
                       detectors=dets)
 
 
+def q_moves(n_skip, steps):
+    """Steps on which the quantum systems move in the driver loop of
+    mxim_mxll.f90:76-77,97-104: tq_step = 1 + int((tt - tt_0)/n_skip_steps),
+    tt_0 = 1 for n_skip_steps = 1 and 0 otherwise, a move whenever tq_step
+    grows."""
+    tt_0 = 1 if n_skip == 1 else 0
+    old, out = 0, []
+    for tt in range(1, steps + 1):
+        tq = 1 + (tt - tt_0) // n_skip
+        if tq > old:
+            old = tq
+            out.append(tt)
+    return out
+
+
 def drive(g, desc, case):
     """Runs the case; with quantum sites the reference loop is driven call by
-    call with the driver's prescribed dPt/dt.  Returns (series, E at sites)."""
+    call with the driver's prescribed dPt/dt, moving the quantum sites every
+    n_skip steps.  Returns (series, E at sites on the moving steps)."""
     dr, dt = case.dr_dt
     if not desc["qsites"]:
         if hasattr(g, "read_records"):
@@ -64,18 +80,22 @@ def drive(g, desc, case):
     q = np.array(desc["qsites"], dtype=float)
     ijk = q[:, :3].astype(np.int32)
     amp = q[:, 3:]
-    g.set_q_sites(ijk, 1.0e-3 / NM_TO_AU ** 3, dt)
+    n_skip = desc.get("n_skip", 1)
+    g.set_q_sites(ijk, 1.0e-3 / NM_TO_AU ** 3, n_skip * dt)
+    moves = set(q_moves(n_skip, case.steps))
     rows, es = [], []
     for tt in range(1, case.steps + 1):
         time = dt * (tt - 0.5)
         g.time = time
-        g.t_skip = time
+        if tt in moves:
+            g.t_skip = time
         g.td_propagate_H_field()
         g.inject_sources(time)
         g.td_propagate_E_field(tt)
-        es.append(g.gather_E())
-        dpdt = amp * np.sin(0.3 * tt + np.arange(3))[None, :]
-        g.scatter_J(dpdt)
+        if tt in moves:
+            es.append(g.gather_E())
+            dpdt = amp * np.sin(0.3 * tt + np.arange(3))[None, :]
+            g.scatter_J(dpdt)
         if tt % case.record_every == 0:
             rows.append(g.sample())
     return np.array(rows), np.array(es)

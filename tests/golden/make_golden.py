@@ -88,6 +88,10 @@ def golden_cases():
         detectors=dets,
         qsites=[(8, 9, 8, 1e-2, -2e-2, 5e-3), (9, 9, 8, 3e-3, 1e-2, -1e-2), (12, 6, 10, -4e-3, 2e-3, 7e-3),
                 (8, 9, 8, 2e-2, 1e-3, -3e-3), (14, 7, 12, 1e-2, 1e-2, 1e-2)])
+
+    # the same sites with the quantum systems moved every 3rd step
+    # (n_skip_steps = 3: J interpolated over tau = time - t_skip in between)
+    out["qsites_skip3"] = dict(out["qsites"], steps=61, n_skip=3)
     return out
 
 
@@ -133,6 +137,7 @@ def write_inputs(case, d):
         f.write(f" nx={nx}, ny={ny}, nz={nz}, npml={case['npml']}, bc={case['bc'][0]},{case['bc'][1]},{case['bc'][2]},\n")
         f.write(f" n_steps={case['steps']}, record_every={case['record_every']}, n_media={len(case['media'])},\n")
         f.write(f" n_src={len(case['sources'])}, n_det={len(case['detectors'])}, n_q={nq}, timing=0,\n")
+        f.write(f" n_skip={case.get('n_skip', 1)},\n")
         f.write(f" dx_nm={dx!r}, diel_eps=" + ",".join(repr(v) for v in diel) + "\n/\n")
     return aux
 
@@ -180,7 +185,9 @@ def run_case(name, case):
         out["series"] = parse_series(d, case, nrec)
         if case["qsites"]:
             e = np.fromfile(os.path.join(d, "qsites_e.bin"), dtype=np.float64)
-            out["qsites_e"] = e.reshape((case["steps"], 3, len(case["qsites"]))).transpose(0, 2, 1)
+            moves = np.fromfile(os.path.join(d, "qmoves.bin"), dtype=np.int32)
+            out["qsites_e"] = e.reshape((moves.size, 3, len(case["qsites"]))).transpose(0, 2, 1)
+            out["qmoves"] = moves
         desc = {k: v for k, v in case.items() if k != "media"}
         desc["media"] = [[h, p] for (h, p, _) in case["media"]]
         out["case_json"] = np.array(json.dumps(desc))

@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 import _golden
-from omx_fdtd import cases
+from omx_fdtd import cases, units
 from oracle import oracle
 
 
@@ -68,6 +68,54 @@ def test_time_series_match_reference(name):
     assert r <= 1e-13
     if es is not None:
         assert rel_l2(es, fx["qsites_e"]) <= 1e-13
+
+
+def test_quantum_move_schedule_matches_reference():
+    """The steps on which the reference driver loop moved the quantum sites
+    (recorded by the reference build) against the schedule the tests drive."""
+    for name in ("qsites", "qsites_skip3"):
+        desc, fx = _golden.load(name)
+        want = fx["qmoves"].tolist()
+        assert _golden.q_moves(desc.get("n_skip", 1), desc["steps"]) == want, name
+    assert _golden.q_moves(3, 10) == [1, 3, 6, 9]
+
+
+def test_oracle_run_loop_moves_emitters_every_n_skip_steps():
+    """orc_run's own sub-step bookkeeping against the schedule driven call by
+    call: two-level emitters with dt_q = 3 dt."""
+    desc, fx = _golden.load("qsites")
+    case = _golden.to_case(desc, fx)
+    dr, dt = case.dr_dt
+    ijk = np.array([q[:3] for q in desc["qsites"][1:]], dtype=np.int32)
+    n = len(ijk)
+    tl = dict(omega0=np.full(n, 0.08), mu=np.linspace(2.0, 4.0, n), pol=np.eye(3)[np.arange(n) % 3] + 0.1,
+              gamma2=np.full(n, 1e-3))
+    res = []
+    for manual in (False, True):
+        o = oracle.OracleGrid(case.grid, case.npml, case.boundaries, dt, dr)
+        cases.apply(case, o, library=oracle.library_medium, make_medium=oracle.make_medium,
+                    rasterize=oracle.rasterize)
+        o.set_q_sites(ijk, 1.0e-3 / units.NM_TO_AU ** 3, 3 * dt)
+        o.set_two_level(tl["omega0"], tl["mu"], tl["pol"], tl["gamma2"])
+        if manual:
+            moves = set(_golden.q_moves(3, case.steps))
+            rows = []
+            for tt in range(1, case.steps + 1):
+                o.time = dt * (tt - 0.5)
+                if tt in moves:
+                    o.t_skip = o.time
+                o.td_propagate_H_field()
+                o.inject_sources(o.time)
+                o.td_propagate_E_field(tt)
+                if tt in moves:
+                    o.two_level_step()
+                rows.append(o.sample())
+            res.append(np.array(rows))
+        else:
+            res.append(o.run(1, case.steps, 1))
+        o.kill()
+    assert np.abs(res[0]).max() > 0
+    np.testing.assert_array_equal(res[0], res[1])
 
 
 def test_planted_mistake_is_caught():

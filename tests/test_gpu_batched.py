@@ -62,3 +62,34 @@ def test_autotune_keeps_results_identical():
     assert g.tiling() == (rows, xc, order)
     g.run(1, c.steps, c.record_every)
     np.testing.assert_array_equal(g.read_records(), ref)
+
+
+def test_batch_follows_grid_changes_and_full_buffers():
+    """A grid advanced on its own between batch runs (the fused step swaps its
+    field buffers) is picked up by the next batch run; recording into a grid
+    whose record buffer is full (or absent) fails as omx3d_run does."""
+    cs = cases.config_batched(n_grids=2, edges=(32, 40), npml=4, steps=40, record_every=4)
+    refs = []
+    for c in cs:
+        g = make(c, 8)
+        g.run(1, c.steps, c.record_every)
+        refs.append(g.read_records())
+        g.kill()
+    grids = [make(c, 8) for c in cs]
+    print(f"[batch] grids={[c.grid for c in cs]} with a fused single-grid run between batch runs", flush=True)
+    b = omx_fdtd.Batch(grids)
+    b.run(1, 12, 4)
+    grids[0].set_step_mode(0)       # grid 0's source is in vacuum: the fused step applies
+    grids[0].run(13, 8, 4)
+    assert grids[0].last_run_fused()
+    grids[1].run(13, 8, 4)
+    b.run(21, 20, 4)
+    for g, ref in zip(grids, refs):
+        got = g.read_records()
+        assert got.shape == ref.shape
+        np.testing.assert_allclose(got, ref, rtol=0, atol=1e-13 * np.abs(ref).max())
+    with pytest.raises(omx_fdtd.OmxError, match="record buffer full"):
+        b.run(41, 8, 4)
+    b.kill()
+    for g in grids:
+        g.kill()

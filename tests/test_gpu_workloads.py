@@ -71,9 +71,13 @@ def test_ellipsoids_scaled(precision, mode):
     check(case, precision, mode)
 
 
+@pytest.mark.parametrize("q_skip", [1, 3])
 @pytest.mark.parametrize("mode", [1, 0])
-def test_emitters_scaled(mode):
+def test_emitters_scaled(mode, q_skip):
+    """q_skip 3: the emitters move every third step over dt_q = 3 dt, the
+    current interpolated over time - t_skip in between (mxim_mxll.f90:73-104)."""
     case = cases.config_emitters(grid=(96, 96, 64), n_emit=16, npml=8, steps=300, record_every=10)
+    case.q_skip = q_skip
 
     def tls(g, o, precision):
         (dg, rg), (do, ro) = g.two_level_state(), o.two_level_state()
