@@ -226,6 +226,19 @@ int omx3d_get_step_mode(omx3d_ctx* ctx, int32_t* fused_last_run);
  * stride above 1 keeps the timed run close to an unprofiled one. */
 int omx3d_set_profiling(omx3d_ctx* ctx, int32_t every);
 int omx3d_get_profile(omx3d_ctx* ctx, double* ms_h, double* ms_e, int32_t* n_launches);
+/* Near-field patches of FP32 grids.  The quasi-static near field of a
+ * point source is (c/(omega dr))^2 times its radiated field one cell away;
+ * rounding it to FP32 every half step adds noise that reaches the far
+ * detectors at ~1e-4 of their signal.  An FP32 grid therefore keeps FP64
+ * copies of E and H in a box of `radius` cells around each source (boxes
+ * outside the CPML, merged where they touch), recomputes the box in FP64
+ * after each H and E update and writes the rounded result to the fields;
+ * sources and quantum-site currents inside a box act on the FP64 copy.
+ * Default radius 8; 0 turns the patches off; FP64 grids and grids with a
+ * TF/SF plane wave have none.  omx3d_get_source_patch reports the radius and
+ * the boxes and cells in use.  No reference counterpart (FP32 accuracy). */
+int omx3d_set_source_patch(omx3d_ctx* ctx, int32_t radius);
+int omx3d_get_source_patch(omx3d_ctx* ctx, int32_t* radius, int32_t* n_boxes, int64_t* n_cells);
 int omx3d_synchronize(omx3d_ctx* ctx);
 /* Bytes of device memory the context holds. */
 int omx3d_device_bytes(omx3d_ctx* ctx, int64_t* bytes);

@@ -228,11 +228,228 @@ E_kernel<T> pick_e(bool cp, bool md, bool b32, bool tf) {
     return md ? pick_e_b32<T, false, 1>(b32, tf) : pick_e_b32<T, false, 0>(b32, tf);
 }
 
+// ---------------------------------------------------------------------------
+// Near-field patches (FP32 grids; kernels: k_patch).  The boxes follow the
+// sources: each source's points widened by patch_radius cells on every side,
+// clipped to the cells outside the CPML faces and off the outer planes
+// (2..n-1), then merged while two boxes overlap or touch.
+// ---------------------------------------------------------------------------
+void free_patches(omx3d_ctx* c) {
+    c->device_bytes -= c->patch_bytes;
+    c->patch_bytes = 0;
+    dfreet(c->d_boxes);
+    dfreet(c->d_patch);
+    dfreet(c->d_patch_cdt);
+    dfreet(c->d_patch_dl);
+    dfreet(c->d_patch_ds);
+    for (auto& s : c->sources) {
+        dfreet(s.d_sh);
+        s.h_sh.assign(s.npts, nullptr);
+    }
+    for (int q = 0; q < 3; ++q) dfreet(c->d_aff_sh[q]);
+    c->h_boxes.clear();
+    c->patch_cells = 0;
+}
+
+// The shadow element of component comp of cell (i,j,k), or null.
+double* patch_element(const omx3d_ctx* c, int comp, int i, int j, int k) {
+    for (const PatchBox& B : c->h_boxes) {
+        const int di = i - B.lo[0], dj = j - B.lo[1], dk = k - B.lo[2];
+        if (di < 0 || dj < 0 || dk < 0 || di >= B.w[0] || dj >= B.w[1] || dk >= B.w[2]) continue;
+        return B.pe + comp * B.cs + ((long)di * B.w[1] + dj) * B.w[2] + dk;
+    }
+    return nullptr;
+}
+
+template <typename P>
+int upload_ptrs(omx3d_ctx* c, P** d, const std::vector<P>& h) {
+    dfreet(*d);
+    bool any = false;
+    for (auto v : h) any = any || v != nullptr;
+    if (!any) return OMX_OK;
+    int rc = dalloc(c, (void**)d, sizeof(P) * h.size());
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(*d, h.data(), sizeof(P) * h.size(), hipMemcpyHostToDevice, c->s));
+    return OMX_OK;
+}
+
+int build_patches_impl(omx3d_ctx* c) {
+    if (c->prec != 4 || c->patch_radius <= 0 || c->tfsf || c->sources.empty()) return OMX_OK;
+    const int n[3] = {c->d.nx, c->d.ny, c->d.nz};
+    int amin[3], amax[3];
+    for (int a = 0; a < 3; ++a) {
+        const bool cp = c->d.boundaries[a] == OMX_BC_CPML;
+        amin[a] = cp ? c->d.npml + 1 : 2;
+        amax[a] = cp ? n[a] - c->d.npml : n[a] - 1;
+    }
+    struct Bx { int lo[3], hi[3]; };
+    std::vector<Bx> bx;
+    const int R = c->patch_radius;
+    for (const auto& src : c->sources) {
+        Bx b;
+        for (int a = 0; a < 3; ++a) { b.lo[a] = INT32_MAX; b.hi[a] = INT32_MIN; }
+        for (int p = 0; p < src.npts; ++p)
+            for (int a = 0; a < 3; ++a) {
+                b.lo[a] = std::min(b.lo[a], src.h_ijk[3 * p + a]);
+                b.hi[a] = std::max(b.hi[a], src.h_ijk[3 * p + a]);
+            }
+        bool ok = true;
+        for (int a = 0; a < 3; ++a) {
+            b.lo[a] = std::max(b.lo[a] - R, amin[a]);
+            b.hi[a] = std::min(b.hi[a] + R, amax[a]);
+            ok = ok && b.lo[a] <= b.hi[a];
+        }
+        if (ok) bx.push_back(b);
+    }
+    for (bool merged = true; merged;) {
+        merged = false;
+        for (size_t u = 0; u < bx.size() && !merged; ++u)
+            for (size_t v = u + 1; v < bx.size() && !merged; ++v) {
+                bool touch = true;
+                for (int a = 0; a < 3; ++a)
+                    touch = touch && bx[u].lo[a] <= bx[v].hi[a] + 1 && bx[v].lo[a] <= bx[u].hi[a] + 1;
+                if (!touch) continue;
+                for (int a = 0; a < 3; ++a) {
+                    bx[u].lo[a] = std::min(bx[u].lo[a], bx[v].lo[a]);
+                    bx[u].hi[a] = std::max(bx[u].hi[a], bx[v].hi[a]);
+                }
+                bx.erase(bx.begin() + v);
+                merged = true;
+            }
+    }
+    if (bx.empty()) return OMX_OK;
+    long cells = 0;
+    for (const Bx& b : bx) cells += (long)(b.hi[0] - b.lo[0] + 1) * (b.hi[1] - b.lo[1] + 1) * (b.hi[2] - b.lo[2] + 1);
+    int rc = dalloc(c, (void**)&c->d_patch, sizeof(double) * 6 * cells);
+    if (rc) return rc;
+    c->patch_cells = cells;
+    // dt_eps0 / eps_r per class and component (dt_eps0 in vacuum), as the
+    // FP64 engine's dielectric coefficient dt_eps0 * (1 / eps_r)
+    const double dt_eps0 = c->d.dt / kEps0;
+    const int ncls = c->media_set ? std::max(c->n_classes, 1) : 1;
+    std::vector<int8_t> cls(3 * (size_t)ncls, 0);
+    if (c->media_set && c->d_cls) HIPCHK(hipMemcpy(cls.data(), c->d_cls, cls.size(), hipMemcpyDeviceToHost));
+    std::vector<double> cdt(3 * (size_t)ncls, dt_eps0);
+    for (size_t q = 0; q < cls.size(); ++q) {
+        const int m = cls[q];
+        if (m > 0 && c->media[m].kind == OMX_MEDIUM_DIELECTRIC) cdt[q] = dt_eps0 * (1.0 / c->media[m].eps_r);
+    }
+    rc = dalloc(c, (void**)&c->d_patch_cdt, sizeof(double) * cdt.size());
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(c->d_patch_cdt, cdt.data(), sizeof(double) * cdt.size(), hipMemcpyHostToDevice, c->s));
+    long first = 0;
+    for (const Bx& b : bx) {
+        PatchBox B;
+        std::memset(&B, 0, sizeof(B));
+        for (int a = 0; a < 3; ++a) { B.lo[a] = b.lo[a]; B.w[a] = b.hi[a] - b.lo[a] + 1; }
+        B.first = first;
+        B.pe = c->d_patch + first;
+        B.ph = c->d_patch + 3 * cells + first;
+        B.cs = cells;
+        for (int q = 0; q < 6; ++q) B.den[q] = c->dcoef + c->co_den[q];
+        B.cdt = c->d_patch_cdt;
+        B.dt_mu0 = c->d.dt / kMu0;
+        B.dt_eps0 = dt_eps0;
+        B.g = 0;
+        c->h_boxes.push_back(B);
+        first += (long)B.w[0] * B.w[1] * B.w[2];
+    }
+    // cells of Drude / Drude-Lorentz classes in the boxes: their position in
+    // the dispersive list, and an FP64 copy of their ADE state
+    if (c->media_set && c->n_disp > 0 && (c->has_drude || c->has_dl)) {
+        std::vector<long> list(c->n_disp);
+        HIPCHK(hipMemcpy(list.data(), c->d_disp, sizeof(long) * list.size(), hipMemcpyDeviceToHost));
+        std::vector<int> dl(cells, -1);
+        bool any = false;
+        for (int p = 0; p < c->n_disp; ++p) {
+            const long e = list[p];
+            const int i = (int)(e & 0xffff), j = (int)((e >> 16) & 0xffff), k = (int)((e >> 32) & 0xffff);
+            for (const PatchBox& B : c->h_boxes) {
+                const int di = i - B.lo[0], dj = j - B.lo[1], dk = k - B.lo[2];
+                if (di < 0 || dj < 0 || dk < 0 || di >= B.w[0] || dj >= B.w[1] || dk >= B.w[2]) continue;
+                dl[B.first + ((long)di * B.w[1] + dj) * B.w[2] + dk] = p;
+                any = true;
+            }
+        }
+        if (any) {
+            const int nv = 2 + 2 * c->maxp;
+            rc = dalloc(c, (void**)&c->d_patch_dl, sizeof(int) * cells);
+            if (!rc) rc = dalloc(c, (void**)&c->d_patch_ds, sizeof(double) * 3 * nv * cells);
+            if (rc) return rc;
+            HIPCHK(hipMemcpyAsync(c->d_patch_dl, dl.data(), sizeof(int) * cells, hipMemcpyHostToDevice, c->s));
+            for (PatchBox& B : c->h_boxes) {
+                B.dl = c->d_patch_dl + B.first;
+                B.ds = c->d_patch_ds + B.first;
+                B.nv = nv;
+                B.mco = c->d_med_co;
+            }
+            HIPCHK(hipStreamSynchronize(c->s));
+        }
+    }
+    rc = dalloc(c, (void**)&c->d_boxes, sizeof(PatchBox) * c->h_boxes.size());
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(c->d_boxes, c->h_boxes.data(), sizeof(PatchBox) * c->h_boxes.size(),
+                          hipMemcpyHostToDevice, c->s));
+    for (auto& src : c->sources) {
+        for (int p = 0; p < src.npts; ++p)
+            src.h_sh[p] = patch_element(c, src.dir - 1, src.h_ijk[3 * p], src.h_ijk[3 * p + 1], src.h_ijk[3 * p + 2]);
+        rc = upload_ptrs(c, &src.d_sh, src.h_sh);
+        if (rc) return rc;
+    }
+    for (int comp = 0; comp < 3; ++comp) {
+        const std::vector<int>& v = c->h_aff_ijk[comp];
+        std::vector<double*> sh(v.size() / 3);
+        for (size_t q = 0; q < sh.size(); ++q) sh[q] = patch_element(c, comp, v[3 * q], v[3 * q + 1], v[3 * q + 2]);
+        rc = upload_ptrs(c, &c->d_aff_sh[comp], sh);
+        if (rc) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(c->s));
+    c->patch_dirty = true;
+    return OMX_OK;
+}
+
+int build_patches(omx3d_ctx* c) {
+    free_patches(c);
+    c->patches_built = true;
+    c->patch_dirty = false;
+    const long before = c->device_bytes;
+    const int rc = build_patches_impl(c);
+    c->patch_bytes = c->device_bytes - before;
+    return rc;
+}
+
+template <typename T>
+int launch_patch(omx3d_ctx* c, const YeeArgs<T>& a, int phase, hipStream_t st) {
+    if (c->patch_cells == 0) return OMX_OK;
+    hipLaunchKernelGGL(k_patch<T>, dim3(blocks_for(c->patch_cells, 256)), dim3(256), 0, st, a, c->d_boxes,
+                       (int)c->h_boxes.size(), c->patch_cells, phase);
+    HIPCHK(hipGetLastError());
+    return OMX_OK;
+}
+
+// Boxes up to date with the sources, and shadows equal to the fields after
+// a set-up or a write from the host.
+template <typename T>
+int ensure_patches(omx3d_ctx* c) {
+    if (!c->patches_built) {
+        int rc = build_patches(c);
+        if (rc) return rc;
+    }
+    if (c->patch_dirty) {
+        int rc = launch_patch<T>(c, make_args<T>(c), 2, c->s);
+        if (rc) return rc;
+        c->patch_dirty = false;
+    }
+    return OMX_OK;
+}
+
 template <typename T>
 // tf: the TF/SF H corrections and the 1D line's H half step ride in the
 // launch.  ev0/ev1 (profiling, may be null) bracket the H kernel itself.
 int launch_h(omx3d_ctx* c, bool tf, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
-    int rc = wrap_e<T>(c, c->s);
+    int rc = ensure_patches<T>(c);
+    if (rc) return rc;
+    rc = wrap_e<T>(c, c->s);
     if (rc) return rc;
     YeeArgs<T> a = make_args<T>(c);
     const dim3 blk(kLanes, c->rows_y);
@@ -244,7 +461,7 @@ int launch_h(omx3d_ctx* c, bool tf, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = n
     hipLaunchKernelGGL(k, dim3(nb), blk, 0, c->s, a, L);
     if (ev1) HIPCHK(hipEventRecord(ev1, c->s));
     HIPCHK(hipGetLastError());
-    return OMX_OK;
+    return launch_patch<T>(c, a, 0, c->s);
 }
 
 template <typename T>
@@ -256,7 +473,8 @@ int launch_j_correct(omx3d_ctx* c, double tau) {
         hipLaunchKernelGGL(k_j_correct<T>, dim3(blocks_for(c->n_aff[comp], 256)), dim3(256), 0, c->s,
                            (T*)c->f[comp], c->d_aff_off[comp], c->d_aff_a[comp], c->d_aff_b[comp],
                            c->n_aff[comp], comp, c->d_J, tau, dt_eps0,
-                           c->media_set ? c->d_tag : (const uint8_t*)nullptr, c->d_cls, c->d_kinds);
+                           c->media_set ? c->d_tag : (const uint8_t*)nullptr, c->d_cls, c->d_kinds,
+                           (double* const*)c->d_aff_sh[comp]);
     }
     HIPCHK(hipGetLastError());
     return OMX_OK;
@@ -291,7 +509,9 @@ int launch_e_dispersive(omx3d_ctx* c, const YeeArgs<T>& a, T* exo, T* eyo, T* ez
 template <typename T>
 int launch_e(omx3d_ctx* c, double time, double t_skip, bool tf, double t_tf, hipEvent_t ev0 = nullptr,
              hipEvent_t ev1 = nullptr) {
-    int rc = wrap_h<T>(c, c->s);
+    int rc = ensure_patches<T>(c);
+    if (rc) return rc;
+    rc = wrap_h<T>(c, c->s);
     if (rc) return rc;
     YeeArgs<T> a = make_args<T>(c);
     const dim3 blk(kLanes, c->rows_y);
@@ -307,6 +527,8 @@ int launch_e(omx3d_ctx* c, double time, double t_skip, bool tf, double t_tf, hip
     hipLaunchKernelGGL(k, grid, blk, 0, c->s, a, (const long*)c->d_disp, nd, db, L);
     if (ev1) HIPCHK(hipEventRecord(ev1, c->s));
     HIPCHK(hipGetLastError());
+    rc = launch_patch<T>(c, a, 1, c->s);
+    if (rc) return rc;
     return launch_j_correct<T>(c, time - t_skip);
 }
 
@@ -321,10 +543,13 @@ SourceParams source_params(const omx3d_ctx* c, const SourceDev& s) {
 
 template <typename T>
 int launch_sources(omx3d_ctx* c, double time) {
+    int rc = ensure_patches<T>(c);
+    if (rc) return rc;
     for (auto& s : c->sources) {
         if (!(time >= s.t_init && time <= s.t_final)) continue;  // J_mat = 0 outside the window
         hipLaunchKernelGGL(k_source<T>, dim3(blocks_for(s.npts, 256)), dim3(256), 0, c->s,
-                           (T*)c->f[s.dir - 1], s.d_off, s.d_ka, s.d_kb, s.npts, source_params(c, s), time);
+                           (T*)c->f[s.dir - 1], s.d_off, s.d_ka, s.d_kb, s.npts, source_params(c, s), time,
+                           (double* const*)s.d_sh);
     }
     HIPCHK(hipGetLastError());
     return OMX_OK;
@@ -483,7 +708,9 @@ int upload_fused_cold(omx3d_ctx* c, const FusedCold<T>& fc, const FusedCold<T>**
 
 template <typename T>
 int launch_fused(omx3d_ctx* c, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
-    int rc = ensure_alt(c);
+    int rc = ensure_patches<T>(c);
+    if (rc) return rc;
+    rc = ensure_alt(c);
     if (rc) return rc;
     FusedCold<T> fc;
     std::memset(&fc, 0, sizeof(fc));
@@ -552,6 +779,16 @@ int launch_fused(omx3d_ctx* c, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullpt
         YeeArgs<T> an = make_args<T>(c);
         an.ex = (T*)c->f_alt[0]; an.ey = (T*)c->f_alt[1]; an.ez = (T*)c->f_alt[2];
         rc = launch_e_dispersive<T>(c, an, (T*)c->f[0], (T*)c->f[1], (T*)c->f[2], c->s);
+        if (rc) return rc;
+    }
+    if (c->patch_cells) {
+        // near-field patches: H^{n+1/2} from E^n (the other buffer) into the
+        // current H, then E^{n+1} from it into the current E
+        YeeArgs<T> ah = make_args<T>(c);
+        ah.ex = (T*)c->f_alt[0]; ah.ey = (T*)c->f_alt[1]; ah.ez = (T*)c->f_alt[2];
+        rc = launch_patch<T>(c, ah, 0, c->s);
+        if (rc) return rc;
+        rc = launch_patch<T>(c, make_args<T>(c), 1, c->s);
         if (rc) return rc;
     }
     return OMX_OK;
@@ -705,6 +942,7 @@ int copy_field_impl(omx3d_ctx* c, int field, double* host, int to_device) {
         if (c->f_alt[field - 1])   // the fused step's second buffer holds the same boundary values
             HIPCHK(hipMemcpyAsync(c->f_alt[field - 1], stage.data(), sizeof(T) * L.nalloc, hipMemcpyHostToDevice, c->s));
         HIPCHK(hipStreamSynchronize(c->s));
+        c->patch_dirty = true;
     }
     return OMX_OK;
 }
@@ -863,6 +1101,7 @@ int media_finish(omx3d_ctx* c) {
     c->media_set = true;
     c->src_in_dispersive = -1;
     c->tf_fold_e = -1;
+    c->patches_built = false;
     return OMX_OK;
 }
 
@@ -905,6 +1144,10 @@ struct omxb_ctx {
     int n_disp = 0;                 // list entries (whole workgroups, padding -1)
     int disp_blocks = 0;
     std::vector<std::vector<long>> sig;   // per grid: its state when the argument blocks were built
+    PatchBox* d_pbox = nullptr;     // near-field patch boxes of every grid (PatchBox::g: the grid)
+    int n_pbox = 0;
+    long n_pcell = 0;
+    double** d_src_sh = nullptr;    // per source point: its shadow element or null
 };
 
 namespace {
@@ -930,11 +1173,17 @@ std::vector<long> batch_signature(const omx3d_ctx* c) {
     v.push_back(c->rows_y); v.push_back(c->xchunk); v.push_back(c->tile_order); v.push_back(c->offset_bits);
     v.push_back((long)c->sources.size()); v.push_back(c->n_samples); v.push_back(c->n_disp);
     v.push_back(c->rec_cap); v.push_back(c->media_set ? 1 : 0);
+    v.push_back((long)(uintptr_t)c->d_patch); v.push_back(c->patch_cells);
     return v;
 }
 
 template <typename T>
 int batch_setup(omxb_ctx* b) {
+    for (auto* c : b->g) {
+        int rc = ensure_patches<T>(c);
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(c->s));
+    }
     b->sig.clear();
     for (auto* c : b->g) b->sig.push_back(batch_signature(c));
     std::vector<YeeArgs<T>> args;
@@ -983,12 +1232,29 @@ int batch_setup(omxb_ctx* b) {
     rc = bupload(b->s, (YeeArgs<T>**)&b->d_args, args);
     if (!rc) rc = bupload(b->s, &b->d_tile_start, ts);
     if (rc) return rc;
+    // near-field patch boxes of all grids, one flat index over them
+    {
+        std::vector<PatchBox> pb;
+        long first = 0;
+        for (int q = 0; q < b->n; ++q)
+            for (PatchBox B : b->g[q]->h_boxes) {
+                B.g = q;
+                B.first = first;
+                first += (long)B.w[0] * B.w[1] * B.w[2];
+                pb.push_back(B);
+            }
+        b->n_pbox = (int)pb.size();
+        b->n_pcell = first;
+        rc = bupload(b->s, &b->d_pbox, pb);
+        if (rc) return rc;
+    }
     // sources: one pass per repeated (field, cell), in source order
     std::map<std::pair<void*, long>, int> seen;
     std::vector<std::vector<void*>> pf;
     std::vector<std::vector<long>> po;
     std::vector<std::vector<double>> pa, pb;
     std::vector<std::vector<int>> ps;
+    std::vector<std::vector<double*>> psh;
     std::vector<SourceParams> sp;
     for (auto* c : b->g) {
         for (auto& src : c->sources) {
@@ -1002,20 +1268,24 @@ int batch_setup(omxb_ctx* b) {
             void* fp = c->f[src.dir - 1];
             for (int q = 0; q < src.npts; ++q) {
                 const int pass = seen[std::make_pair(fp, off[q])]++;
-                if ((int)pf.size() <= pass) { pf.resize(pass + 1); po.resize(pass + 1); pa.resize(pass + 1); pb.resize(pass + 1); ps.resize(pass + 1); }
+                if ((int)pf.size() <= pass) {
+                    pf.resize(pass + 1); po.resize(pass + 1); pa.resize(pass + 1); pb.resize(pass + 1);
+                    ps.resize(pass + 1); psh.resize(pass + 1);
+                }
                 pf[pass].push_back(fp); po[pass].push_back(off[q]); pa[pass].push_back(ka[q]);
-                pb[pass].push_back(kb[q]); ps[pass].push_back(sid);
+                pb[pass].push_back(kb[q]); ps[pass].push_back(sid); psh[pass].push_back(src.h_sh[q]);
             }
         }
     }
     std::vector<void*> F; std::vector<long> O; std::vector<double> A, B; std::vector<int> S;
+    std::vector<double*> SH;
     b->pass_first.clear(); b->pass_count.clear();
     for (size_t q = 0; q < pf.size(); ++q) {
         b->pass_first.push_back((int)F.size());
         b->pass_count.push_back((int)pf[q].size());
         F.insert(F.end(), pf[q].begin(), pf[q].end()); O.insert(O.end(), po[q].begin(), po[q].end());
         A.insert(A.end(), pa[q].begin(), pa[q].end()); B.insert(B.end(), pb[q].begin(), pb[q].end());
-        S.insert(S.end(), ps[q].begin(), ps[q].end());
+        S.insert(S.end(), ps[q].begin(), ps[q].end()); SH.insert(SH.end(), psh[q].begin(), psh[q].end());
     }
     b->n_src_pts = (int)F.size();
     rc = bupload(b->s, &b->d_src_field, F);
@@ -1023,6 +1293,7 @@ int batch_setup(omxb_ctx* b) {
     if (!rc) rc = bupload(b->s, &b->d_src_ka, A);
     if (!rc) rc = bupload(b->s, &b->d_src_kb, B);
     if (!rc) rc = bupload(b->s, &b->d_src_of, S);
+    if (!rc) rc = bupload(b->s, &b->d_src_sh, SH);
     if (!rc) rc = bupload(b->s, &b->d_sp, sp);
     if (rc) return rc;
     // detectors
@@ -1055,11 +1326,15 @@ int batch_step_launches(omxb_ctx* b) {
     else
         hipLaunchKernelGGL((k_h_update_batched<T, false>), dim3(b->total_tiles), blk, 0, b->s,
                            (const YeeArgs<T>*)b->d_args, b->d_tile_start, b->n, b->tile_order, b->ntz);
+    if (b->n_pcell)
+        hipLaunchKernelGGL(k_patch_batched<T>, dim3(blocks_for(b->n_pcell, 256)), dim3(256), 0, b->s,
+                           (const YeeArgs<T>*)b->d_args, b->d_pbox, b->n_pbox, b->n_pcell, 0);
     for (size_t q = 0; q < b->pass_first.size(); ++q) {
         const int f0 = b->pass_first[q], nq = b->pass_count[q];
         hipLaunchKernelGGL(k_source_batched<T>, dim3(blocks_for(nq, 256)), dim3(256), 0, b->s,
                            (T* const*)(b->d_src_field + f0), b->d_src_off + f0, b->d_src_ka + f0, b->d_src_kb + f0,
-                           b->d_src_of + f0, b->d_sp, nq, (const long*)b->d_step);
+                           b->d_src_of + f0, b->d_sp, nq, (const long*)b->d_step,
+                           (double* const*)(b->d_src_sh + f0));
     }
     if (b->periodic)
         for (auto* c : b->g) { int rc = wrap_h<T>(c, b->s); if (rc) return rc; }
@@ -1073,6 +1348,9 @@ int batch_step_launches(omxb_ctx* b) {
         hipLaunchKernelGGL((k_e_update_batched<T, false>), dim3(b->total_tiles + db), blk, 0, b->s,
                            (const YeeArgs<T>*)b->d_args, b->d_tile_start, b->n, b->tile_order, b->ntz, b->d_disp,
                            b->d_disp_grid, b->n_disp, db);
+    if (b->n_pcell)
+        hipLaunchKernelGGL(k_patch_batched<T>, dim3(blocks_for(b->n_pcell, 256)), dim3(256), 0, b->s,
+                           (const YeeArgs<T>*)b->d_args, b->d_pbox, b->n_pbox, b->n_pcell, 1);
     if (b->n_samp > 0)
         hipLaunchKernelGGL(k_sample_batched<T>, dim3(blocks_for(b->n_samp, 256)), dim3(256), 0, b->s,
                            (const YeeArgs<T>*)b->d_args, b->d_br, b->d_grid_of, b->d_local, b->d_code, b->d_off,
@@ -1090,6 +1368,7 @@ int batch_run(omxb_ctx* b, int tt_first, int n_steps, int every, int use_graph) 
     for (int q = 0; q < b->n; ++q) {
         omx3d_ctx* c = b->g[q];
         int rc = upload_detectors(c);
+        if (!rc) rc = ensure_patches<T>(c);
         if (rc) return rc;
         HIPCHK(hipStreamSynchronize(c->s));
     }
@@ -1321,6 +1600,7 @@ int omx3d_create(const omx3d_desc* desc, omx3d_ctx** out) {
 int omx3d_destroy(omx3d_ctx* c) {
     if (!c) return OMX_OK;
     if (c->s) (void)hipStreamSynchronize(c->s);
+    free_patches(c);
     for (auto& p : c->f) dfree(p);
     for (auto& p : c->f_alt) dfree(p);
     for (auto& p : c->psi_h_alt) dfree(p);
@@ -1500,7 +1780,11 @@ int omx3d_add_source(omx3d_ctx* c, int32_t dir, double j_amp, double w0, double 
     SourceDev s;
     s.dir = dir; s.j_amp = j_amp; s.w0 = w0; s.t0 = t0; s.tau = tau; s.t_init = t_init; s.t_final = t_final;
     s.phase = phase; s.npts = (int)np;
-    s.d_off = nullptr; s.d_ka = nullptr; s.d_kb = nullptr;
+    s.d_off = nullptr; s.d_ka = nullptr; s.d_kb = nullptr; s.d_sh = nullptr;
+    s.h_ijk.resize(3 * (size_t)np);
+    for (long p = 0; p < np; ++p)
+        for (int a = 0; a < 3; ++a) s.h_ijk[3 * p + a] = ind[p + a * np];
+    s.h_sh.assign(np, nullptr);
     int rc = dalloc(c, (void**)&s.d_off, sizeof(long) * np);
     if (!rc) rc = dalloc(c, (void**)&s.d_ka, sizeof(double) * np);
     if (!rc) rc = dalloc(c, (void**)&s.d_kb, sizeof(double) * np);
@@ -1511,6 +1795,7 @@ int omx3d_add_source(omx3d_ctx* c, int32_t dir, double j_amp, double w0, double 
     HIPCHK(hipStreamSynchronize(c->s));
     c->sources.push_back(s);
     c->src_in_dispersive = -1;
+    c->patches_built = false;
     return OMX_OK;
 }
 
@@ -1660,6 +1945,12 @@ int omx3d_set_q_sites(omx3d_ctx* c, int32_t n, const int32_t* ijk, double densit
             B.push_back(ib == slot_of.end() ? -1 : ib->second);
         }
         c->n_aff[comp] = (int)o.size();
+        c->h_aff_ijk[comp].clear();
+        for (auto& kv : aff) {
+            c->h_aff_ijk[comp].push_back(std::get<0>(kv.first));
+            c->h_aff_ijk[comp].push_back(std::get<1>(kv.first));
+            c->h_aff_ijk[comp].push_back(std::get<2>(kv.first));
+        }
         rc = dalloc(c, (void**)&c->d_aff_off[comp], sizeof(long) * o.size());
         if (!rc) rc = dalloc(c, (void**)&c->d_aff_a[comp], sizeof(int) * A.size());
         if (!rc) rc = dalloc(c, (void**)&c->d_aff_b[comp], sizeof(int) * B.size());
@@ -1677,6 +1968,7 @@ int omx3d_set_q_sites(omx3d_ctx* c, int32_t n, const int32_t* ijk, double densit
     c->n_skip = dt_q > c->d.dt ? (int)std::lround(dt_q / c->d.dt) : 1;
     c->q_dt = (double)c->n_skip * c->d.dt;
     c->tq_step_old = 0;
+    c->patches_built = false;
     return OMX_OK;
 }
 
@@ -1800,6 +2092,7 @@ int omx3d_set_tfsf(omx3d_ctx* c, const int32_t lo[3], const int32_t hi[3], doubl
     HIPCHK(hipStreamSynchronize(c->s));
     c->tfsf = true;
     c->tf_fold_e = -1;
+    c->patches_built = false;
     return OMX_OK;
 }
 
@@ -1841,6 +2134,32 @@ int omx3d_get_profile(omx3d_ctx* c, double* ms_h, double* ms_e, int32_t* n) {
     if (ms_h) *ms_h = c->prof_ms_h;
     if (ms_e) *ms_e = c->prof_ms_e;
     if (n) *n = c->prof_n;
+    return OMX_OK;
+}
+
+int omx3d_set_source_patch(omx3d_ctx* c, int32_t radius) {
+    CHECK_CTX(c);
+    if (radius < 0 || radius > 64) return fail(OMX_ERR_ARG, "patch radius must be in 0..64");
+    HIPCHK(hipStreamSynchronize(c->s));
+    if (c->patch_cells) {
+        // fields hold the rounded shadows already; the new boxes start from them
+        free_patches(c);
+    }
+    c->patch_radius = radius;
+    c->patches_built = false;
+    return OMX_OK;
+}
+
+int omx3d_get_source_patch(omx3d_ctx* c, int32_t* radius, int32_t* n_boxes, int64_t* n_cells) {
+    CHECK_CTX(c);
+    if (!c->patches_built) {
+        HIPCHK(hipStreamSynchronize(c->s));
+        int rc = build_patches(c);
+        if (rc) return rc;
+    }
+    if (radius) *radius = c->patch_radius;
+    if (n_boxes) *n_boxes = (int32_t)c->h_boxes.size();
+    if (n_cells) *n_cells = c->patch_cells;
     return OMX_OK;
 }
 
@@ -1898,6 +2217,7 @@ int omxb_destroy(omxb_ctx* b) {
     dfreet(b->d_src_field); dfreet(b->d_src_off); dfreet(b->d_src_ka); dfreet(b->d_src_kb); dfreet(b->d_src_of);
     dfreet(b->d_sp); dfreet(b->d_br); dfreet(b->d_grid_of); dfreet(b->d_local); dfreet(b->d_code); dfreet(b->d_off);
     dfreet(b->d_overflow); dfreet(b->d_step); dfreet(b->d_disp); dfreet(b->d_disp_grid);
+    dfreet(b->d_pbox); dfreet(b->d_src_sh);
     if (b->s) (void)hipStreamDestroy(b->s);
     delete b;
     return OMX_OK;
@@ -1960,6 +2280,7 @@ int omx3d_autotune(omx3d_ctx* c, int32_t steps, int32_t* rows_y, int32_t* xchunk
     }
     if (c->disp_state) HIPCHK(hipMemsetAsync(c->disp_state, 0, disp_state_bytes(c), c->s));
     HIPCHK(hipStreamSynchronize(c->s));
+    c->patch_dirty = true;
     if (rows_y) *rows_y = br;
     if (xchunk) *xchunk = bx;
     if (tile_order) *tile_order = bo;

@@ -93,6 +93,30 @@ struct SourceDev {
     long* d_off;        // field offsets of the kernel points
     double* d_ka;       // ker_mat at the point
     double* d_kb;       // ker_mat one step along dir (0 past the kernel edge)
+    std::vector<int> h_ijk;          // the points' cells (i, j, k), 3 per point
+    std::vector<double*> h_sh;       // per point: its near-field shadow element, or null
+    double** d_sh;                   // device copy of h_sh (null when no point has one)
+};
+
+// A near-field patch box of an FP32 grid (kernels: k_patch).
+struct PatchBox {
+    int lo[3], w[3];            // first cell (1-based) and extent per axis
+    long first;                 // first cell of the box in the launch's flat index
+    double* pe;                 // shadow of E: component c at pe + c * cs, cell ((i*w1)+j)*w2+k
+    double* ph;                 // shadow of H, same layout
+    long cs;
+    const double* den[6];       // FP64 den_ex den_ey den_ez den_hx den_hy den_hz (1-based)
+    const double* cdt;          // per class and component: dt_eps0 / eps_r (dt_eps0 in vacuum)
+    double dt_mu0, dt_eps0;
+    int g;                      // argument block of the box's grid (batched launches)
+    // cells of Drude / Drude-Lorentz classes in the box: their position in the
+    // grid's dispersive list (-1: other cells), per box cell; null if none
+    const int* dl;
+    // FP64 Drude / DL state of the box cells: variable v (E_old, PD, maxp PL,
+    // maxp PL_old) of component c at ds + (c * nv + v) * cs, cell as for pe
+    double* ds;
+    int nv;                     // 2 + 2 * maxp
+    const omx_medium_coeffs* mco;   // FP64 update coefficients per medium
 };
 
 struct DetectorHost {
@@ -193,6 +217,22 @@ struct omx3d_ctx {
     int* d_kinds = nullptr;         // medium kind per medium index (gate of the J term)
     bool tls = false;
     double* d_tls = nullptr;        // per site: omega0 mu pol(3) gamma2 | rho(4) dip(3) dip_old(3)
+
+    // near-field patches (FP32 grids with point sources): FP64 shadows of E
+    // and H in boxes of patch_radius cells around the sources
+    int patch_radius = 8;
+    bool patches_built = false;     // boxes match the current sources / sites / media
+    bool patch_dirty = false;       // shadows must be read again from the fields
+    std::vector<omx::PatchBox> h_boxes;
+    omx::PatchBox* d_boxes = nullptr;
+    long patch_cells = 0;
+    long patch_bytes = 0;           // device bytes the patches hold
+    double* d_patch = nullptr;      // E shadow (3 * cells) then H shadow (3 * cells)
+    double* d_patch_cdt = nullptr;  // dt_eps0 / eps_r per class and component
+    int* d_patch_dl = nullptr;      // per box cell: position in the dispersive list or -1
+    double* d_patch_ds = nullptr;   // FP64 Drude / DL state of the box cells
+    std::vector<int> h_aff_ijk[3];  // cells whose E component sees J (3 ints each)
+    double** d_aff_sh[3] = {};      // their shadow elements (null: none)
 
     // TF/SF
     bool tfsf = false;

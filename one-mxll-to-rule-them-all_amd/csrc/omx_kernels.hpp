@@ -1655,14 +1655,29 @@ __device__ inline double source_amp(const SourceParams& s, double time) {
     return (s.j_amp * exp(-(x * x))) * cos(s.w0 * (time - s.t0) + s.phase);
 }
 
+// E += add at one field point; a point inside a near-field patch (sh != null)
+// is accumulated in its FP64 shadow and the field gets the rounded shadow.
+template <typename T>
+__device__ __forceinline__ void add_to_e(T* f, long o, double* sh, double add) {
+    if (sh) {
+        const double v = *sh + add;
+        *sh = v;
+        f[o] = (T)v;
+    } else {
+        f[o] = (T)((double)f[o] + add);
+    }
+}
+
+// sh: per point, its FP64 shadow element or null (sh itself may be null).
 template <typename T>
 __global__ void k_source(T* __restrict__ f, const long* __restrict__ off, const double* __restrict__ ka,
-                         const double* __restrict__ kb, int n, SourceParams sp, double time) {
+                         const double* __restrict__ kb, int n, SourceParams sp, double time,
+                         double* const* __restrict__ sh) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const double s = source_amp(sp, time);
     const double jav = 0.5 * (s * ka[p] + s * kb[p]);
-    f[off[p]] = (T)((double)f[off[p]] + sp.c_src * jav);
+    add_to_e(f, off[p], sh ? sh[p] : nullptr, sp.c_src * jav);
 }
 
 // All points of all sources of a batch; the step index comes from device
@@ -1673,15 +1688,14 @@ template <typename T>
 __global__ void k_source_batched(T* const* __restrict__ field, const long* __restrict__ off,
                                  const double* __restrict__ ka, const double* __restrict__ kb,
                                  const int* __restrict__ src_of, const SourceParams* __restrict__ sp, int n,
-                                 const long* __restrict__ step) {
+                                 const long* __restrict__ step, double* const* __restrict__ sh) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const SourceParams& q = sp[src_of[p]];
     const double time = q.dt * ((double)step[0] - 0.5);
     const double s = source_amp(q, time);
     const double jav = 0.5 * (s * ka[p] + s * kb[p]);
-    T* f = field[p];
-    f[off[p]] = (T)((double)f[off[p]] + q.c_src * jav);
+    add_to_e(field[p], off[p], sh[p], q.c_src * jav);
 }
 
 __global__ void k_step_advance(long* step) { step[0] += 1; }
@@ -1793,7 +1807,8 @@ template <typename T>
 __global__ void k_j_correct(T* __restrict__ f, const long* __restrict__ off, const int* __restrict__ sa,
                             const int* __restrict__ sb, int n, int comp, const double* __restrict__ J,
                             double tau, double dt_eps0, const uint8_t* __restrict__ tag,
-                            const int8_t* __restrict__ cls, const int* __restrict__ kind) {
+                            const int8_t* __restrict__ cls, const int* __restrict__ kind,
+                            double* const* __restrict__ sh) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
     const long o = off[q];
@@ -1807,7 +1822,7 @@ __global__ void k_j_correct(T* __restrict__ f, const long* __restrict__ off, con
     const double jb = B >= 0 ? J[9 * B + 3 + comp] : 0.0;
     const double db = B >= 0 ? J[9 * B + 6 + comp] : 0.0;
     const double jav = 0.5 * (((ja + tau * da) + jb) + tau * db);
-    f[o] = (T)((double)f[o] - dt_eps0 * jav);
+    add_to_e(f, o, sh ? sh[q] : nullptr, -(dt_eps0 * jav));
 }
 
 // Two-level emitter, FP64 RK4 over one quantum step with the gathered field
@@ -1943,6 +1958,211 @@ __global__ __launch_bounds__(256) void k_energy_final(const double* __restrict__
         out[0] = 0.5 * scale * sd[0];
         out[1] = 0.5 * scale * (double)(ss[0] + sc[0]);
     }
+}
+
+// ---------------------------------------------------------------------------
+// Near-field patches of FP32 grids.  Next to a point source the field is
+// dominated by its quasi-static near field, some (c/(omega dr))^2 times the
+// radiated field at one cell (1e3-1e4 at the workloads' resolutions);
+// rounding that field to FP32 every half step injects noise that radiates
+// to the detectors at ~1e-4 of their signal.  A box of cells around each
+// source keeps FP64 shadows of E and H: after the FP32 H (E) launch the box's
+// H (E) is computed again in FP64 from the shadows inside the box and the
+// FP32 fields outside it, and both the shadow and the FP32 field get the
+// result.  Boxes lie outside the CPML and away from the outer planes, and
+// two boxes never touch (the host merges them), so a neighbour of a box cell
+// is either in the same box or in no box.  Cells of Drude / Drude-Lorentz
+// classes in a box keep their ADE state in FP64 as well.  The arithmetic is
+// that of the FP64 engine (h_bulk, e_curl, e_bulk, medium_update).
+// ---------------------------------------------------------------------------
+// (PatchBox: omx_internal.hpp)
+
+__device__ __forceinline__ int find_box(const PatchBox* __restrict__ b, int n, long q) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (b[mid].first <= q) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Field value at (i,j,k): the shadow when the cell is in the box, else the
+// FP32 field (neighbours of box cells are in the box or in no box).
+template <typename T>
+__device__ __forceinline__ double patch_val(const PatchBox& B, const double* sh, const T* f, const YeeArgs<T>& a,
+                                            int i, int j, int k) {
+    const int di = i - B.lo[0], dj = j - B.lo[1], dk = k - B.lo[2];
+    if ((unsigned)di < (unsigned)B.w[0] && (unsigned)dj < (unsigned)B.w[1] && (unsigned)dk < (unsigned)B.w[2])
+        return sh[((long)di * B.w[1] + dj) * B.w[2] + dk];
+    return (double)f[(long)i * a.sx + (long)j * a.sz + k + a.off];
+}
+
+__device__ __forceinline__ void patch_cell(const PatchBox& B, long q, int& i, int& j, int& k, long& r) {
+    r = q - B.first;
+    k = B.lo[2] + (int)(r % B.w[2]);
+    j = B.lo[1] + (int)((r / B.w[2]) % B.w[1]);
+    i = B.lo[0] + (int)(r / ((long)B.w[2] * B.w[1]));
+}
+
+// H of a box cell from the E of a.ex/a.ey/a.ez (E^n) into the shadow and
+// a.hx/a.hy/a.hz.
+template <typename T>
+__device__ __forceinline__ void patch_h_cell(const YeeArgs<T>& a, const PatchBox& B, long q) {
+    int i, j, k;
+    long r;
+    patch_cell(B, q, i, j, k, r);
+    const double* ex = B.pe;
+    const double* ey = B.pe + B.cs;
+    const double* ez = B.pe + 2 * B.cs;
+    const double ezc = patch_val(B, ez, a.ez, a, i, j, k), ezj = patch_val(B, ez, a.ez, a, i, j + 1, k);
+    const double ezn = patch_val(B, ez, a.ez, a, i + 1, j, k);
+    const double eyc = patch_val(B, ey, a.ey, a, i, j, k), eyk = patch_val(B, ey, a.ey, a, i, j, k + 1);
+    const double eyn = patch_val(B, ey, a.ey, a, i + 1, j, k);
+    const double exc = patch_val(B, ex, a.ex, a, i, j, k), exk = patch_val(B, ex, a.ex, a, i, j, k + 1);
+    const double exj = patch_val(B, ex, a.ex, a, i, j + 1, k);
+    const double dhx = B.den[3][i], dhy = B.den[4][j], dhz = B.den[5][k], dmu = B.dt_mu0;
+    const double hx = B.ph[r] + dmu * ((ezc - ezj) * dhy + (eyk - eyc) * dhz);
+    const double hy = B.ph[B.cs + r] + dmu * ((ezn - ezc) * dhx + (exc - exk) * dhz);
+    const double hz = B.ph[2 * B.cs + r] + dmu * ((eyc - eyn) * dhx + (exj - exc) * dhy);
+    B.ph[r] = hx; B.ph[B.cs + r] = hy; B.ph[2 * B.cs + r] = hz;
+    const long o = (long)i * a.sx + (long)j * a.sz + k + a.off;
+    a.hx[o] = (T)hx; a.hy[o] = (T)hy; a.hz[o] = (T)hz;
+}
+
+// One E component of a box cell of a dispersive class: the update of
+// medium_update (get_medium_polarization, classical_medium_mod.f90:773-833)
+// in FP64 on the box's state (c: component, m: its medium, r: box cell, dp:
+// the cell's position in the dispersive list); the FP32 compact state of the
+// cell takes the rounded values.  vac_c: the vacuum / dielectric coefficient.
+template <typename T>
+__device__ __forceinline__ double patch_medium(const YeeArgs<T>& a, const PatchBox& B, int c, int m, long r, int dp,
+                                               double e, double rot, double vac_c) {
+    if (m == 0) return e + vac_c * rot;
+    const int kind = a.med[m].kind;
+    if (kind != OMX_MEDIUM_DRUDE && kind != OMX_MEDIUM_DL) return e + vac_c * rot;
+    const omx_medium_coeffs& M = B.mco[m];
+    const long cs = B.cs;
+    double* const s0 = B.ds + (long)c * B.nv * cs + r;     // E_old, PD, PL_q, PL_old_q at s0 + v * cs
+    T* const f0 = a.dstate + c * a.dcomp + dp;             // the same in the FP32 compact state
+    const long ps = a.pstride;
+    const int maxp = a.maxp;
+    const double pv = s0[cs];
+    double t;
+    if (kind == OMX_MEDIUM_DRUDE) {
+        t = M.C1 * e + M.C3 * rot - M.C4 * pv;
+        const double pd = M.A1 * pv + M.A2 * (t + e);
+        s0[cs] = pd;
+        f0[ps] = (T)pd;
+        return t;
+    }
+    const int np = a.med[m].np;
+    const double eo = s0[0];
+    t = M.C1 * e + M.C2 * eo + M.C3 * rot - M.C4 * pv;
+    for (int q = 0; q < np; ++q) t = t - M.B1_k[q] * s0[(2 + q) * cs] - M.B2_k[q] * s0[(2 + maxp + q) * cs];
+    const double pd = M.A1 * pv + M.A2 * (t + e);
+    s0[cs] = pd;
+    f0[ps] = (T)pd;
+    const double de = t - eo;
+    for (int q = 0; q < np; ++q) {
+        const double L = s0[(2 + q) * cs], Lo = s0[(2 + maxp + q) * cs];
+        const double nl = M.alpha_k[q] * L + M.beta_k[q] * Lo + M.gamma_k[q] * de;
+        s0[(2 + maxp + q) * cs] = L;
+        s0[(2 + q) * cs] = nl;
+        f0[(2 + maxp + q) * ps] = (T)L;
+        f0[(2 + q) * ps] = (T)nl;
+    }
+    s0[0] = e;
+    f0[0] = (T)e;
+    return t;
+}
+
+// E of a box cell from the H of a.hx/a.hy/a.hz (H^{n+1/2}) into the shadow
+// and a.ex/a.ey/a.ez; a cell of a dispersive class keeps the FP32 result of
+// the dispersive update and its shadow takes that value.
+template <typename T>
+__device__ __forceinline__ void patch_e_cell(const YeeArgs<T>& a, const PatchBox& B, long q) {
+    int i, j, k;
+    long r;
+    patch_cell(B, q, i, j, k, r);
+    const long o = (long)i * a.sx + (long)j * a.sz + k + a.off;
+    const int tg = a.tag ? (int)a.tag[o] : 0;
+    const int dp = tg >= a.disp_tag0 && B.dl ? B.dl[r] : -1;
+    if (tg >= a.disp_tag0 && dp < 0) {
+        B.pe[r] = (double)a.ex[o]; B.pe[B.cs + r] = (double)a.ey[o]; B.pe[2 * B.cs + r] = (double)a.ez[o];
+        return;
+    }
+    const double* hx = B.ph;
+    const double* hy = B.ph + B.cs;
+    const double* hz = B.ph + 2 * B.cs;
+    const double hzc = patch_val(B, hz, a.hz, a, i, j, k), hzj = patch_val(B, hz, a.hz, a, i, j - 1, k);
+    const double hzi = patch_val(B, hz, a.hz, a, i - 1, j, k);
+    const double hyc = patch_val(B, hy, a.hy, a, i, j, k), hyk = patch_val(B, hy, a.hy, a, i, j, k - 1);
+    const double hyi = patch_val(B, hy, a.hy, a, i - 1, j, k);
+    const double hxc = patch_val(B, hx, a.hx, a, i, j, k), hxk = patch_val(B, hx, a.hx, a, i, j, k - 1);
+    const double hxj = patch_val(B, hx, a.hx, a, i, j - 1, k);
+    const double dex = B.den[0][i], dey = B.den[1][j], dez = B.den[2][k];
+    const double rx = (hzc - hzj) * dey + (hyk - hyc) * dez;
+    const double ry = (hzi - hzc) * dex + (hxc - hxk) * dez;
+    const double rz = (hyc - hyi) * dex + (hxj - hxc) * dey;
+    double ex, ey, ez;
+    if (dp < 0) {
+        ex = B.pe[r] + B.cdt[3 * tg] * rx;
+        ey = B.pe[B.cs + r] + B.cdt[3 * tg + 1] * ry;
+        ez = B.pe[2 * B.cs + r] + B.cdt[3 * tg + 2] * rz;
+    } else {
+        // a cell of a dispersive class: the full per-cell medium update in
+        // FP64 on the box's state, which the FP32 state then takes rounded
+        const int m0 = a.cls[3 * tg], m1 = a.cls[3 * tg + 1], m2 = a.cls[3 * tg + 2];
+        ex = patch_medium(a, B, 0, m0, r, dp, B.pe[r], rx, B.cdt[3 * tg]);
+        ey = patch_medium(a, B, 1, m1, r, dp, B.pe[B.cs + r], ry, B.cdt[3 * tg + 1]);
+        ez = patch_medium(a, B, 2, m2, r, dp, B.pe[2 * B.cs + r], rz, B.cdt[3 * tg + 2]);
+    }
+    B.pe[r] = ex; B.pe[B.cs + r] = ey; B.pe[2 * B.cs + r] = ez;
+    a.ex[o] = (T)ex; a.ey[o] = (T)ey; a.ez[o] = (T)ez;
+}
+
+// Shadows from the FP32 fields (set-up, and after fields were written from
+// the host).
+template <typename T>
+__device__ __forceinline__ void patch_sync_cell(const YeeArgs<T>& a, const PatchBox& B, long q) {
+    int i, j, k;
+    long r;
+    patch_cell(B, q, i, j, k, r);
+    const long o = (long)i * a.sx + (long)j * a.sz + k + a.off;
+    B.pe[r] = (double)a.ex[o]; B.pe[B.cs + r] = (double)a.ey[o]; B.pe[2 * B.cs + r] = (double)a.ez[o];
+    B.ph[r] = (double)a.hx[o]; B.ph[B.cs + r] = (double)a.hy[o]; B.ph[2 * B.cs + r] = (double)a.hz[o];
+    const int dp = B.dl ? B.dl[r] : -1;
+    if (dp >= 0)
+        for (int c = 0; c < 3; ++c)
+            for (int v = 0; v < B.nv; ++v)
+                B.ds[((long)c * B.nv + v) * B.cs + r] = (double)a.dstate[c * a.dcomp + (long)v * a.pstride + dp];
+}
+
+// phase 0: H, 1: E, 2: shadows from the fields.  One grid: the argument
+// block by value; ncell: cells of all boxes.
+template <typename T>
+__global__ __launch_bounds__(256) void k_patch(const YeeArgs<T> a, const PatchBox* __restrict__ boxes, int nbox,
+                                               long ncell, int phase) {
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= ncell) return;
+    const PatchBox& B = boxes[find_box(boxes, nbox, q)];
+    if (phase == 0) patch_h_cell(a, B, q);
+    else if (phase == 1) patch_e_cell(a, B, q);
+    else patch_sync_cell(a, B, q);
+}
+
+// The boxes of every grid of a batch: box B belongs to grid B.g.
+template <typename T>
+__global__ __launch_bounds__(256) void k_patch_batched(const YeeArgs<T>* __restrict__ args,
+                                                       const PatchBox* __restrict__ boxes, int nbox, long ncell,
+                                                       int phase) {
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= ncell) return;
+    const PatchBox& B = boxes[find_box(boxes, nbox, q)];
+    const YeeArgs<T>& a = args[B.g];
+    if (phase == 0) patch_h_cell(a, B, q);
+    else if (phase == 1) patch_e_cell(a, B, q);
+    else patch_sync_cell(a, B, q);
 }
 
 // ---------------------------------------------------------------------------

@@ -90,6 +90,8 @@ _SIGS = {
     "omx3d_run": ([_P, _I, _I, _I], C.c_int),
     "omx3d_set_profiling": ([_P, _I], C.c_int),
     "omx3d_get_profile": ([_P, _DP, _DP, _IP], C.c_int),
+    "omx3d_set_source_patch": ([_P, _I], C.c_int),
+    "omx3d_get_source_patch": ([_P, _IP, _IP, C.POINTER(C.c_int64)], C.c_int),
     "omx3d_synchronize": ([_P], C.c_int),
     "omx3d_device_bytes": ([_P, C.POINTER(C.c_int64)], C.c_int),
     "omx3d_autotune": ([_P, _I, _IP, _IP, _IP, _DP], C.c_int),

@@ -248,11 +248,9 @@ def config_ellipsoids(grid=(768, 768, 512), dx_nm=1.0, npml=16, steps=10000, n_e
     shapes.append((3, 2, [npml + 1, npml + 1, npml + 1, nx - npml, ny - npml, k_sub]))
     src_cell = [nx // 2, ny // 2, int(0.6 * nz)]
     pos_nm = [index_to_nm(src_cell[a], dx_nm, grid[a]) for a in range(3)]
-    # soft z dipole with a 6 nm Blackman-Harris kernel: a one-cell source
-    # drives a near field some 1e4 times the far field at the detectors, and
-    # FP32 storage noise of that near field alone puts the far-field FP32
-    # series 3e-4 (relL2) off the FP64 one; the 6 nm kernel brings it to 1e-5
-    srcs = [("z", 1.0, 2.066, 3.0, 1.0, pos_nm[0], pos_nm[1], pos_nm[2], 6.0, 0.0, 1.0e6, 0.0)]
+    # one-cell soft z dipole (its near field is kept in FP64 by the engine's
+    # near-field patch on FP32 grids)
+    srcs = [("z", 1.0, 2.066, 3.0, 1.0, pos_nm[0], pos_nm[1], pos_nm[2], 0.0, 0.0, 1.0e6, 0.0)]
     det = rng.integers([npml + 2, npml + 2, npml + 2], [nx - npml - 2, ny - npml - 2, nz - npml - 2], size=(32, 3))
     return Case(name="ellipsoids", grid=grid, dx_nm=dx_nm, npml=npml, boundaries=(BC_CPML,) * 3, steps=steps,
                 record_every=record_every, media=[med, ("dielectric", 2.25)], shapes=shapes, sources=srcs,
@@ -292,7 +290,7 @@ def config_emitters(grid=(512, 512, 384), dx_nm=2.0, npml=12, steps=20000, n_emi
               pol=pol, gamma2=rng.uniform(10e-3, 50e-3, n_emit) * EV_TO_AU)
     src_cell = [nx // 2, ny // 2, min(nz - npml - 4, k1 + layer + 20)]
     pos_nm = [index_to_nm(src_cell[a], dx_nm, grid[a]) for a in range(3)]
-    srcs = [("z", 1.0, 2.1, 3.0, 1.0, pos_nm[0], pos_nm[1], pos_nm[2], 6.0, 0.0, 1.0e6, 0.0)]   # 6 nm kernel
+    srcs = [("z", 1.0, 2.1, 3.0, 1.0, pos_nm[0], pos_nm[1], pos_nm[2], 0.0, 0.0, 1.0e6, 0.0)]   # one cell
     det = cells[:8].tolist()
     return Case(name="emitters", grid=grid, dx_nm=dx_nm, npml=npml, boundaries=(BC_CPML,) * 3, steps=steps,
                 record_every=record_every, media=[("drude-lorentz", "Ag", 1.0)], shapes=shapes, sources=srcs,

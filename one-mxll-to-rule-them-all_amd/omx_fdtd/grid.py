@@ -111,6 +111,16 @@ class Grid3D:
         L.call("omx3d_get_tiling", self.h, C.byref(r), C.byref(x), C.byref(o))
         return r.value, x.value, o.value
 
+    def set_source_patch(self, radius):
+        """FP64 near-field boxes of `radius` cells around point sources (FP32 grids); 0: off."""
+        L.call("omx3d_set_source_patch", self.h, int(radius))
+
+    def source_patch(self):
+        """(radius, boxes, cells) of the near-field patches in use."""
+        r, nb, nc = C.c_int32(), C.c_int32(), C.c_int64()
+        L.call("omx3d_get_source_patch", self.h, C.byref(r), C.byref(nb), C.byref(nc))
+        return r.value, nb.value, nc.value
+
     def set_step_mode(self, mode=0, rows=0, xchunk=0):
         """mode 0: fused H+E pass where it applies; 1: separate H and E kernels."""
         L.call("omx3d_set_step_mode", self.h, int(mode), int(rows), int(xchunk))

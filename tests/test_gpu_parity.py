@@ -3,7 +3,8 @@
 Bar: FP64 detector time series within relL2 <= 1e-12 of the oracle (the
 engine contracts multiply-adds, the oracle does not, so bitwise equality is
 not expected); integer/index work (media maps, classes) bit-exact; FP32 runs
-within relL2 <= 2e-5 of the FP64 oracle on these short runs.
+within relL2 <= 1e-5 of the FP64 oracle (point sources with the engine's
+FP64 near-field patches, the default of FP32 grids).
 """
 import numpy as np
 import pytest
@@ -17,7 +18,7 @@ from oracle import oracle
 pytestmark = pytest.mark.gpu
 
 TOL64 = 1e-12
-TOL32 = 2e-5
+TOL32 = 1e-5
 
 
 def rel_l2(a, b):
@@ -460,7 +461,7 @@ def test_tfsf_dispersive_cells_on_the_faces(precision):
     assert np.abs(want).max() > 0
     r = rel_l2(got, want)
     print(f"FP{8 * precision} relL2 vs oracle = {r:.3e}")
-    assert r <= (TOL64 if precision == 8 else 5e-5)
+    assert r <= (TOL64 if precision == 8 else TOL32)
 
 
 def test_tfsf_vacuum_leakage():

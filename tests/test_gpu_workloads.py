@@ -1,9 +1,8 @@
 """The four measured workloads, scaled down so that the oracle finishes in
 seconds, against the oracle: the same case builders, media, sources,
 detectors and (emitters) two-level systems as the full-size runs of the
-bench.  FP64 detector series within relL2 <= 1e-12, FP32 within 5e-5
-(FP32 storage noise of the fields, see test_gpu_golden); the separate H/E
-kernels and the fused step both."""
+bench.  FP64 detector series within relL2 <= 1e-12, FP32 within 1e-5; the
+separate H/E kernels and the fused step both."""
 import numpy as np
 import pytest
 
@@ -14,7 +13,7 @@ from oracle import oracle
 pytestmark = pytest.mark.gpu
 
 TOL64 = 1e-12
-TOL32 = 5e-5
+TOL32 = 1e-5
 
 
 def rel_l2(a, b):
@@ -71,9 +70,10 @@ def test_ellipsoids_scaled(precision, mode):
     check(case, precision, mode)
 
 
+@pytest.mark.parametrize("precision", [8, 4])
 @pytest.mark.parametrize("q_skip", [1, 3])
 @pytest.mark.parametrize("mode", [1, 0])
-def test_emitters_scaled(mode, q_skip):
+def test_emitters_scaled(mode, q_skip, precision):
     """q_skip 3: the emitters move every third step over dt_q = 3 dt, the
     current interpolated over time - t_skip in between (mxim_mxll.f90:73-104)."""
     case = cases.config_emitters(grid=(96, 96, 64), n_emit=16, npml=8, steps=300, record_every=10)
@@ -82,17 +82,25 @@ def test_emitters_scaled(mode, q_skip):
     def tls(g, o, precision):
         (dg, rg), (do, ro) = g.two_level_state(), o.two_level_state()
         assert np.abs(do).max() > 0
-        np.testing.assert_allclose(rg, ro, rtol=1e-10, atol=1e-14)
-        np.testing.assert_allclose(dg, do, rtol=1e-8, atol=1e-11 * np.abs(do).max())
+        if precision == 8:
+            np.testing.assert_allclose(rg, ro, rtol=1e-10, atol=1e-14)
+            np.testing.assert_allclose(dg, do, rtol=1e-8, atol=1e-11 * np.abs(do).max())
+        else:
+            # FP32 fields, FP64 density matrices: the emitters' dipoles within
+            # relL2 1e-5 of the FP64 oracle's
+            r = rel_l2(dg, do)
+            print(f"emitter dipoles FP32 vs oracle relL2 = {r:.3e}", flush=True)
+            assert r <= TOL32
 
-    check(case, 8, mode, extra=tls)
+    check(case, precision, mode, extra=tls)
 
 
+@pytest.mark.parametrize("precision", [8, 4])
 @pytest.mark.parametrize("graph", [True, False])
-def test_batched_scaled(graph):
+def test_batched_scaled(graph, precision):
     cs = cases.config_batched(n_grids=4, edges=(32, 40, 48), npml=4, steps=200, record_every=4)
-    print(f"[workload] batched grids={[c.grid for c in cs]} graph={graph}", flush=True)
-    grids = [engine(c, 8, 1) for c in cs]
+    print(f"[workload] batched grids={[c.grid for c in cs]} graph={graph} precision={precision}", flush=True)
+    grids = [engine(c, precision, 1) for c in cs]
     b = omx_fdtd.Batch(grids)
     b.run(1, cs[0].steps, cs[0].record_every, graph=graph)
     for c, g in zip(cs, grids):
@@ -102,7 +110,7 @@ def test_batched_scaled(graph):
         assert np.abs(want).max() > 0
         r = rel_l2(got, want)
         print(f"{c.name} {c.grid}: relL2 = {r:.3e}", flush=True)
-        assert r <= TOL64
+        assert r <= (TOL64 if precision == 8 else TOL32)
         o.kill()
     b.kill()
     for g in grids:
