@@ -202,6 +202,17 @@ int omx3d_copy_field(omx3d_ctx* ctx, int32_t field, double* host, int32_t to_dev
 /* W = 1/2 dr^3 sum(eps0 eps_r E^2 + mu0 H^2) over the interior cells; w64 is
  * accumulated in FP64, w32c with compensated FP32 sums. */
 int omx3d_energy(omx3d_ctx* ctx, double* w64, double* w32c);
+/* omx3d_energy for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops): no reference counterpart.  Allocates
+ * nothing: the reduction's partial sums go to a fixed 24 KiB device array of
+ * the library (one per GPU, for the life of the process), and (w64, w32c)
+ * are written to w_dev, 2 doubles of device memory on the context's GPU,
+ * asynchronously on the context's stream (omx3d_synchronize before reading).
+ * Same kernels and launch shape as omx3d_energy: the same values, bit for
+ * bit.  The scratch does not grow with the grid (the reduction strides over
+ * every cell with a fixed number of workgroups).  Calls that overlap on one
+ * GPU share that array and race: issue them from one stream at a time. */
+int omx3d_energy_noalloc(omx3d_ctx* ctx, double* w_dev);
 
 /* ---- total-field / scattered-field plane wave (x-polarised, along +z) ---- */
 int omx3d_set_tfsf(omx3d_ctx* ctx, const int32_t lo[3], const int32_t hi[3],

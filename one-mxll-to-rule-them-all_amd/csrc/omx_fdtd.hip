@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -15,6 +16,16 @@
 #include "omx_kernels.hpp"
 
 using namespace omx;
+
+// Workgroups of the energy reduction (k_energy): each leaves three partial
+// sums (FP64, compensated FP32 and its compensation) for k_energy_final.
+constexpr int kEnergyBlocks = 1024;
+
+// Scratch of omx3d_energy_noalloc: the partial sums of the energy reduction,
+// 3 * kEnergyBlocks doubles (24 KiB) on every GPU the library runs on, held
+// until the process ends.  Host code never uses this name as an address: the
+// device address of each GPU's copy comes from hipGetSymbolAddress (below).
+static __device__ __align__(256) double scratch[3 * kEnergyBlocks];
 
 namespace {
 
@@ -884,25 +895,56 @@ int run_impl(omx3d_ctx* c, int tt_first, int n_steps, int record_every) {
     return OMX_OK;
 }
 
+// The energy reduction on the context's stream: partial sums of
+// kEnergyBlocks workgroups into part (3 * kEnergyBlocks doubles), the result
+// (w64, w32c) into out (2 doubles, device memory).
+template <typename T>
+int launch_energy(omx3d_ctx* c, double* part, double* out) {
+    const int nb = kEnergyBlocks;
+    YeeArgs<T> a = make_args<T>(c);
+    const bool md = c->media_set && c->n_classes > 1;
+    if (md) hipLaunchKernelGGL((k_energy<T, true>), dim3(nb), dim3(256), 0, c->s, a, c->d_eps_class, part);
+    else hipLaunchKernelGGL((k_energy<T, false>), dim3(nb), dim3(256), 0, c->s, a, c->d_eps_class, part);
+    const double dr = c->d.dr;
+    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, c->s, part, nb, dr * dr * dr, out);
+    HIPCHK(hipGetLastError());
+    return OMX_OK;
+}
+
 template <typename T>
 int energy_impl(omx3d_ctx* c, double* w64, double* w32c) {
-    const int nb = 1024;
+    const int nb = kEnergyBlocks;
     if (!c->d_red) {
         int rc = dalloc(c, (void**)&c->d_red, sizeof(double) * (3 * nb + 2));
         if (rc) return rc;
     }
-    YeeArgs<T> a = make_args<T>(c);
-    const bool md = c->media_set && c->n_classes > 1;
-    if (md) hipLaunchKernelGGL((k_energy<T, true>), dim3(nb), dim3(256), 0, c->s, a, c->d_eps_class, c->d_red);
-    else hipLaunchKernelGGL((k_energy<T, false>), dim3(nb), dim3(256), 0, c->s, a, c->d_eps_class, c->d_red);
-    const double dr = c->d.dr;
-    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, c->s, c->d_red, nb, dr * dr * dr, c->d_red + 3 * nb);
-    HIPCHK(hipGetLastError());
+    int rc = launch_energy<T>(c, c->d_red, c->d_red + 3 * nb);
+    if (rc) return rc;
     double out[2];
     HIPCHK(hipMemcpyAsync(out, c->d_red + 3 * nb, sizeof(out), hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
     if (w64) *w64 = out[0];
     if (w32c) *w32c = out[1];
+    return OMX_OK;
+}
+
+// Device address of `scratch` on each GPU, taken once per device with that
+// device current (each GPU holds its own copy of the array).
+constexpr int kMaxDevices = 64;
+double* g_scratch[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+int scratch_on(int dev, double** out) {
+    if (dev < 0 || dev >= kMaxDevices) return fail(OMX_ERR_RANGE, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(OMX_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch[dev] = (double*)p;
+    }
+    *out = g_scratch[dev];
     return OMX_OK;
 }
 
@@ -1509,6 +1551,10 @@ int omx3d_create(const omx3d_desc* desc, omx3d_ctx** out) {
     }
     omx3d_ctx* c = new omx3d_ctx();
     c->d = d;
+    if (hipGetDevice(&c->device) != hipSuccess) {
+        delete c;
+        return fail(OMX_ERR_HIP, "hipGetDevice failed");
+    }
     c->prec = d.precision;
     c->esz = (size_t)d.precision;
     // default tile shapes, measured fastest on 256^3 CPML grids: FP32 4 rows
@@ -2064,6 +2110,30 @@ int omx3d_copy_field(omx3d_ctx* c, int32_t field, double* host, int32_t to_devic
 int omx3d_energy(omx3d_ctx* c, double* w64, double* w32c) {
     CHECK_CTX(c);
     return c->prec == 4 ? energy_impl<float>(c, w64, w32c) : energy_impl<double>(c, w64, w32c);
+}
+
+int omx3d_energy_noalloc(omx3d_ctx* c, double* w_dev) {
+    CHECK_CTX(c);
+    if (!w_dev) return fail(OMX_ERR_ARG, "null output");
+    hipPointerAttribute_t attr;
+    std::memset(&attr, 0, sizeof(attr));
+    if (hipPointerGetAttributes(&attr, w_dev) != hipSuccess || attr.type != hipMemoryTypeDevice) {
+        (void)hipGetLastError();
+        return fail(OMX_ERR_ARG, "the output of omx3d_energy_noalloc must be device memory (2 doubles)");
+    }
+    if (attr.device != c->device)
+        return fail(OMX_ERR_ARG, "the output must live on the context's GPU");
+    int cur = 0;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != c->device) HIPCHK(hipSetDevice(c->device));
+    double* part = nullptr;
+    int rc = scratch_on(c->device, &part);
+    if (!rc) rc = c->prec == 4 ? launch_energy<float>(c, part, w_dev) : launch_energy<double>(c, part, w_dev);
+    if (cur != c->device) {
+        const hipError_t e = hipSetDevice(cur);
+        if (!rc && e != hipSuccess) rc = fail(OMX_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    }
+    return rc;
 }
 
 int omx3d_set_tfsf(omx3d_ctx* c, const int32_t lo[3], const int32_t hi[3], double e0, double w0, double t0,

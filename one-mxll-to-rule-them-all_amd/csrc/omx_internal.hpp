@@ -128,6 +128,7 @@ struct DetectorHost {
 
 struct omx3d_ctx {
     omx3d_desc d;
+    int device = 0;     // the GPU current at omx3d_create: every buffer of the context lives there
     int prec;           // 4 or 8
     size_t esz;
     omx::Layout L;

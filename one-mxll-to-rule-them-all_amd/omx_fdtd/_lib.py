@@ -86,6 +86,7 @@ _SIGS = {
     "omx3d_get_two_level": ([_P, _DP, _DP], C.c_int),
     "omx3d_copy_field": ([_P, _I, _DP, _I], C.c_int),
     "omx3d_energy": ([_P, _DP, _DP], C.c_int),
+    "omx3d_energy_noalloc": ([_P, _P], C.c_int),
     "omx3d_set_tfsf": ([_P, _IP, _IP, _D, _D, _D, _D, _I], C.c_int),
     "omx3d_run": ([_P, _I, _I, _I], C.c_int),
     "omx3d_set_profiling": ([_P, _I], C.c_int),
