@@ -225,8 +225,9 @@ int omx3d_run(omx3d_ctx* ctx, int32_t tt_first, int32_t n_steps, int32_t record_
 /* Step kernels of omx3d_run: mode 1 (default) runs the separate H and E
  * kernels; mode 0 runs H and E as one fused pass over HBM (fields
  * double-buffered) wherever it applies -- no periodic axis and no source
- * point in a Drude/DL component.  rows (2..8, incl. one halo row) and xchunk
- * set the fused tile; 0 keeps the current value.  No reference counterpart. */
+ * point in a Drude/DL component.  rows (1, 2 or 4 rows of 64 z cells per
+ * wavefront) and xchunk (planes per wavefront) set the fused tile; 0 keeps
+ * the current value.  No reference counterpart. */
 int omx3d_set_step_mode(omx3d_ctx* ctx, int32_t mode, int32_t rows, int32_t xchunk);
 /* 1 if the last omx3d_run used the fused step. */
 int omx3d_get_step_mode(omx3d_ctx* ctx, int32_t* fused_last_run);
@@ -245,7 +246,11 @@ int omx3d_get_profile(omx3d_ctx* ctx, double* ms_h, double* ms_e, int32_t* n_lau
  * outside the CPML, merged where they touch), recomputes the box in FP64
  * after each H and E update and writes the rounded result to the fields;
  * sources and quantum-site currents inside a box act on the FP64 copy.
- * Default radius 8; 0 turns the patches off; FP64 grids and grids with a
+ * Default radius 16 (at dx = 1 nm and a 600 nm carrier, the FP32 error of
+ * far detectors over 10000 steps is 1.6e-5 with radius 8 and 6e-6 with 16:
+ * rounding the near field while the source runs leaves small spurious
+ * charges whose static field persists); 0 turns the patches off; FP64
+ * grids and grids with a
  * TF/SF plane wave have none.  omx3d_get_source_patch reports the radius and
  * the boxes and cells in use.  No reference counterpart (FP32 accuracy). */
 int omx3d_set_source_patch(omx3d_ctx* ctx, int32_t radius);

@@ -687,103 +687,124 @@ bool fused_applies(omx3d_ctx* c) {
     // the fused kernel addresses field and z-face psi arrays by 32-bit byte
     // offsets, and handles one z face per cell (the faces do not meet)
     if ((unsigned long long)c->L.nalloc * c->esz >= (1ull << 32)) return false;
-    if ((unsigned long long)zpsi_elems(c->L.sy, c->d.nx, c->d.npml) * c->esz >= (1ull << 32)) return false;
+    for (int a = 0; a < 3; ++a) {
+        // E psi arrays have npml slots per side to the npml-1 of H
+        const unsigned long long e = a == 2 ? 0ull : (unsigned long long)(c->nse - c->nsh) * (a == 0 ? c->L.sx : (c->d.nx + 2) * c->L.sz);
+        if (((unsigned long long)psi_h_elems(c, a) + e) * c->esz >= (1ull << 32)) return false;
+    }
     if ((c->cpml_mask & 48u) && c->d.nz < 2 * c->d.npml + 1) return false;
     return c->src_in_dispersive == 0;
 }
 
-// The fused kernel's cold argument block lives in device memory, one slot
-// per buffer parity (the fields and the psi of H alternate between two
-// buffers).  A slot is rewritten only when its contents change (set-up,
-// media, TF/SF or a buffer reallocation), with a synchronous upload.
-template <typename T>
-int upload_fused_cold(omx3d_ctx* c, const FusedCold<T>& fc, const FusedCold<T>** dev) {
-    static_assert(sizeof(FusedCold<T>) <= omx3d_ctx::kColdBytes, "cold block slot too small");
-    if (!c->d_fcold) {
-        int rc = dalloc(c, &c->d_fcold, 2 * omx3d_ctx::kColdBytes);
-        if (rc) return rc;
-        c->fcold_valid[0] = c->fcold_valid[1] = false;
-    }
-    const int slot = c->fparity & 1;
-    unsigned char* h = c->h_fcold[slot];
-    char* d = (char*)c->d_fcold + (size_t)slot * omx3d_ctx::kColdBytes;
-    if (!c->fcold_valid[slot] || std::memcmp(h, &fc, sizeof(fc)) != 0) {
-        std::memcpy(h, &fc, sizeof(fc));
-        HIPCHK(hipMemcpyAsync(d, h, sizeof(fc), hipMemcpyHostToDevice, c->s));
-        HIPCHK(hipStreamSynchronize(c->s));
-        c->fcold_valid[slot] = true;
-    }
-    *dev = (const FusedCold<T>*)d;
-    return OMX_OK;
+template <typename T, int R, int CP>
+void launch_fused_kernel(omx3d_ctx* c, bool md, bool tf, unsigned nb, const YeeArgs<T>& a, const FusedOut<T>& fo) {
+#define OMX_FUSED(MD, TF) \
+    hipLaunchKernelGGL((k_step_fused<T, R, CP, MD, TF>), dim3(nb), dim3(256), 0, c->s, a, fo)
+    if (md && tf) OMX_FUSED(1, true);
+    else if (md) OMX_FUSED(1, false);
+    else if (tf) OMX_FUSED(0, true);
+    else OMX_FUSED(0, false);
+#undef OMX_FUSED
 }
 
+template <typename T, int CP>
+void launch_fused_rows(omx3d_ctx* c, bool md, bool tf, unsigned nb, const YeeArgs<T>& a, const FusedOut<T>& fo) {
+    if (c->fused_rows == 4) launch_fused_kernel<T, 4, CP>(c, md, tf, nb, a, fo);
+    else if (c->fused_rows == 2) launch_fused_kernel<T, 2, CP>(c, md, tf, nb, a, fo);
+    else launch_fused_kernel<T, 1, CP>(c, md, tf, nb, a, fo);
+}
+
+// The fused step's regions.  Planes xa..xb and rows ya..yb hold no cell of
+// an x or y CPML face (nor, one plane / row below, a halo cell of one); z
+// tiles tza..tzb-1 likewise for the z faces.  Three launches over disjoint
+// tiles: that interior without CPML code, its z-face tiles with the z-face
+// code only, and the x and y slabs with all of it (so the few CPML cells do
+// not cost the interior its occupancy).  The slabs are thin, so their
+// wavefronts march short chunks (kSlabChunk planes) to give the launch
+// enough wavefronts to fill the chip.
+struct FusedPlan {
+    FusedOut<double>::Box box[3][4];   // per launch (CPML 0, 4, 7)
+    int nbox[3];
+};
+
+FusedPlan fused_plan(const omx3d_ctx* c) {
+    FusedPlan P;
+    std::memset(&P, 0, sizeof(P));
+    const int R = c->fused_rows, X = c->fused_xchunk, np = c->d.npml;
+    const int nx = c->L.nx, ny = c->L.ny, nz = c->L.nz;
+    const unsigned m = c->cpml_mask;
+    const int ntz = (nz + kLanes - 1) / kLanes;
+    int xa = (m & 1u) ? np + 1 : 1, xb = (m & 2u) ? nx - np : nx;
+    int ya = (m & 4u) ? np + 1 : 1, yb = (m & 8u) ? ny - np : ny;
+    if (xa > xb || ya > yb) { xa = 1; xb = 0; ya = 1; yb = 0; }   // no interior: all of it slabs
+    // z tiles k0..k0+63 (k0 = 64 tz + 1) free of z faces, k0-1 included
+    int tza = 0, tzb = ntz;
+    if (m & 16u) while (tza < ntz && 64 * tza + 1 < np + 1) ++tza;
+    if (m & 32u) while (tzb > tza && 64 * (tzb - 1) + 64 > nz - np) --tzb;
+    constexpr int kSlabChunk = 4;
+    auto add = [&](int l, int i_lo, int i_hi, int j_lo, int j_hi, int tz_lo, int tz_hi) {
+        const int xc = l == 2 ? std::min(X, kSlabChunk) : X;
+        if (i_lo > i_hi || j_lo > j_hi || tz_lo >= tz_hi) return;
+        FusedOut<double>::Box& B = P.box[l][P.nbox[l]];
+        const int prev = P.nbox[l] ? P.box[l][P.nbox[l] - 1].t_end : 0;
+        B.i_lo = i_lo; B.i_hi = i_hi; B.j_lo = j_lo; B.j_hi = j_hi;
+        B.tz_lo = tz_lo; B.ntz = tz_hi - tz_lo;
+        B.nty = (j_hi - j_lo + R) / R;
+        B.ntx = (i_hi - i_lo + xc) / xc;
+        B.xc = xc;
+        B.t_end = prev + B.ntz * B.nty * B.ntx;
+        ++P.nbox[l];
+    };
+    add(0, xa, xb, ya, yb, tza, tzb);
+    add(1, xa, xb, ya, yb, 0, tza);
+    add(1, xa, xb, ya, yb, tzb, ntz);
+    if (xb < xa) {
+        add(2, 1, nx, 1, ny, 0, ntz);
+    } else {
+        add(2, 1, xa - 1, 1, ny, 0, ntz);
+        add(2, xb + 1, nx, 1, ny, 0, ntz);
+        add(2, xa, xb, 1, ya - 1, 0, ntz);
+        add(2, xa, xb, yb + 1, ny, 0, ntz);
+    }
+    return P;
+}
+
+// One fused step (k_step_fused): reads the current buffers, writes the other
+// set, then swaps them; the cells of dispersive classes and the near-field
+// patches follow in their own launches.
 template <typename T>
 int launch_fused(omx3d_ctx* c, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
     int rc = ensure_patches<T>(c);
     if (rc) return rc;
     rc = ensure_alt(c);
     if (rc) return rc;
-    FusedCold<T> fc;
-    std::memset(&fc, 0, sizeof(fc));
-    fc.a = make_args<T>(c);
-    for (int q = 0; q < 6; ++q) fc.pho[q] = (T*)c->psi_h_alt[q];
-    if (c->tfsf) { fc.tkb = c->tf_kb; fc.ex1 = c->d_tf; fc.dt_mu0 = c->d.dt / kMu0; }
-    const FusedCold<T>* dcold = nullptr;
-    rc = upload_fused_cold<T>(c, fc, &dcold);
-    if (rc) return rc;
-    const YeeArgs<T>& a = fc.a;
-    FusedArgs<T> f;
-    std::memset(&f, 0, sizeof(f));
-    f.exi = a.ex; f.eyi = a.ey; f.ezi = a.ez; f.hxi = a.hx; f.hyi = a.hy; f.hzi = a.hz;
-    f.exo = (T*)c->f_alt[0]; f.eyo = (T*)c->f_alt[1]; f.ezo = (T*)c->f_alt[2];
-    f.hxo = (T*)c->f_alt[3]; f.hyo = (T*)c->f_alt[4]; f.hzo = (T*)c->f_alt[5];
-    f.tag = a.tag;
-    f.cx = sizeof(T) == 4 ? a.cdhx : a.den_hx;
-    f.dex = a.den_ex;
-    f.cex = a.cdex;
-    f.phz[0] = a.ph[4]; f.phz[1] = a.ph[5];
-    f.phzo[0] = fc.pho[4]; f.phzo[1] = fc.pho[5];
-    f.pez[0] = a.pe[4]; f.pez[1] = a.pe[5];
-    f.dt_mu0 = a.dt_mu0; f.dt_eps0 = a.dt_eps0;
-    f.sx = a.sx;
-    f.nx = a.nx; f.ny = a.ny; f.nz = a.nz; f.sy = a.sy; f.sz = a.sz; f.off = a.off;
-    f.npml = a.npml; f.cpml = a.cpml;
-    f.disp_tag0 = c->disp_tag0;
-    if (c->tfsf) {
-        f.ti0 = c->tf_lo[0]; f.tj0 = c->tf_lo[1]; f.tk0 = c->tf_lo[2];
-        f.ti1 = c->tf_hi[0]; f.tj1 = c->tf_hi[1]; f.tk1 = c->tf_hi[2];
-    }
-    const int FR = c->fused_rows;
-    f.xchunk = c->fused_xchunk;
-    f.ntz = (c->L.nz + kLanes - 1) / kLanes;        // walked inside each workgroup
-    f.nty = (c->L.ny + FR - 2) / (FR - 1);
-    const long ntx = (c->L.nx + f.xchunk - 1) / f.xchunk;
-    const unsigned nb = (unsigned)((long)f.nty * ntx);
-    if (c->tfsf)
+    const YeeArgs<T> a = make_args<T>(c);
+    FusedOut<T> fo;
+    std::memset(&fo, 0, sizeof(fo));
+    fo.ex = (T*)c->f_alt[0]; fo.ey = (T*)c->f_alt[1]; fo.ez = (T*)c->f_alt[2];
+    fo.hx = (T*)c->f_alt[3]; fo.hy = (T*)c->f_alt[4]; fo.hz = (T*)c->f_alt[5];
+    for (int q = 0; q < 6; ++q) fo.ph[q] = (T*)c->psi_h_alt[q];
+    if (c->tfsf) { fo.ex1 = c->d_tf; fo.t = tfsf_args(c); }
+    fo.tile_order = c->tile_order == 0 ? 0 : 1;
+    if (c->tfsf)   // the incident line's H half step (the kernel reads its E only)
         hipLaunchKernelGGL(k_tfsf_1d_h, dim3(blocks_for(c->tf_n, 256)), dim3(256), 0, c->s, c->d_tf,
                            c->d_tf + c->tf_n, c->tf_n, (c->d.dt / kMu0) * (1.0 / c->d.dr));
-    const dim3 blk(kLanes, FR);
-    // E rows 2 x 3 planes, H rows 2 x 2 planes, H handed between z tiles
-    const size_t lds = sizeof(T) * (10 * FR * kLanes + 4 * (f.xchunk + 1) * FR);
-    if (lds > 160 * 1024) return fail(OMX_ERR_ARG, "fused step tile needs more than 160 KiB of LDS");
-    const bool cp = c->cpml_mask != 0, md = c->media_set && c->n_classes > 1, tf = c->tfsf;
-#define OMX_FUSED(CP, MD, TF) \
-    hipLaunchKernelGGL((k_step_fused<T, CP, MD, TF>), dim3(nb), blk, lds, c->s, f, dcold)
+    const bool md = c->media_set && c->n_classes > 1, tf = c->tfsf;
+    const FusedPlan P = fused_plan(c);
     if (ev0) HIPCHK(hipEventRecord(ev0, c->s));
-    if (cp && md && tf) OMX_FUSED(true, true, true);
-    else if (cp && md) OMX_FUSED(true, true, false);
-    else if (cp && tf) OMX_FUSED(true, false, true);
-    else if (cp) OMX_FUSED(true, false, false);
-    else if (md && tf) OMX_FUSED(false, true, true);
-    else if (md) OMX_FUSED(false, true, false);
-    else if (tf) OMX_FUSED(false, false, true);
-    else OMX_FUSED(false, false, false);
-#undef OMX_FUSED
+    for (int l = 0; l < 3; ++l) {
+        if (!P.nbox[l]) continue;
+        fo.nbox = P.nbox[l];
+        for (int q = 0; q < 4; ++q) std::memcpy(&fo.box[q], &P.box[l][q], sizeof(fo.box[q]));
+        const unsigned nb = (unsigned)((P.box[l][P.nbox[l] - 1].t_end + 3) / 4);   // four wavefronts per workgroup
+        if (l == 0) launch_fused_rows<T, 0>(c, md, tf, nb, a, fo);
+        else if (l == 1) launch_fused_rows<T, 4>(c, md, tf, nb, a, fo);
+        else launch_fused_rows<T, 7>(c, md, tf, nb, a, fo);
+    }
     if (ev1) HIPCHK(hipEventRecord(ev1, c->s));
     HIPCHK(hipGetLastError());
     for (int q = 0; q < 6; ++q) std::swap(c->f[q], c->f_alt[q]);
     for (int q = 0; q < 6; ++q) std::swap(c->psi_h[q], c->psi_h_alt[q]);
-    c->fparity ^= 1;
     if (md && c->n_disp > 0) {
         // cells of dispersive classes: H^{n+1/2} now current, E^n in the
         // other buffer, E^{n+1} into the current one
@@ -1562,6 +1583,9 @@ int omx3d_create(const omx3d_desc* desc, omx3d_ctx** out) {
     c->rows_y = d.precision == OMX_FP32 ? 4 : 2;
     c->xchunk = 4;
     c->tile_order = 2;
+    // fused step: FP32 4 rows x 16 planes per wavefront, FP64 2 x 8
+    c->fused_rows = d.precision == OMX_FP32 ? 4 : 2;
+    c->fused_xchunk = d.precision == OMX_FP32 ? 16 : 8;
     c->cpml_mask = mask;
     Layout& L = c->L;
     const int A = 128 / (int)c->esz;
@@ -1650,7 +1674,6 @@ int omx3d_destroy(omx3d_ctx* c) {
     for (auto& p : c->f) dfree(p);
     for (auto& p : c->f_alt) dfree(p);
     for (auto& p : c->psi_h_alt) dfree(p);
-    dfree(c->d_fcold);
     dfreet(c->dcoef); dfree(c->tcoef);
     for (auto& p : c->psi_h) dfree(p);
     for (auto& p : c->psi_e) dfree(p);
@@ -2176,7 +2199,7 @@ int omx3d_run(omx3d_ctx* c, int32_t tt_first, int32_t n_steps, int32_t record_ev
 int omx3d_set_step_mode(omx3d_ctx* c, int32_t mode, int32_t rows, int32_t xchunk) {
     CHECK_CTX(c);
     if (mode != 0 && mode != 1) return fail(OMX_ERR_ARG, "step mode must be 0 (fused where it applies) or 1");
-    if (rows != 0 && (rows < 2 || rows > 8)) return fail(OMX_ERR_ARG, "fused rows must be in 2..8");
+    if (rows != 0 && rows != 1 && rows != 2 && rows != 4) return fail(OMX_ERR_ARG, "fused rows must be 1, 2 or 4");
     if (xchunk < 0) return fail(OMX_ERR_ARG, "xchunk must be >= 0");
     c->step_mode = mode;
     if (rows) c->fused_rows = rows;

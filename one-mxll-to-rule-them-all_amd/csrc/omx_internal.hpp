@@ -141,16 +141,10 @@ struct omx3d_ctx {
     void* f_alt[6] = {};        // second buffers of the fused step
     void* psi_h_alt[6] = {};
     int step_mode = 1;          // 0: fused step where it applies, 1: separate H and E kernels
-    int fused_rows = 4;         // rows per workgroup of the fused step (incl. the halo row)
-    int fused_xchunk = 4;
+    int fused_rows = 4;         // rows per wavefront of the fused step (2 or 4)
+    int fused_xchunk = 16;      // planes per wavefront of the fused step
     int src_in_dispersive = -1; // -1 unknown, 0 no, 1 a source point sits in a Drude/DL component
     bool last_fused = false;
-    // fused step: device-resident cold argument blocks, one per buffer parity
-    static constexpr size_t kColdBytes = 1024;
-    void* d_fcold = nullptr;
-    unsigned char h_fcold[2][kColdBytes] = {};
-    bool fcold_valid[2] = {false, false};
-    int fparity = 0;
     // FP64 master coefficients: be ce (npml+1) bh ch (npml) den_e{x,y,z} den_h{x,y,z} (n+2 each)
     double* dcoef = nullptr;
     void* tcoef = nullptr;          // precision-T copies used by the kernels
@@ -221,7 +215,7 @@ struct omx3d_ctx {
 
     // near-field patches (FP32 grids with point sources): FP64 shadows of E
     // and H in boxes of patch_radius cells around the sources
-    int patch_radius = 8;
+    int patch_radius = 16;
     bool patches_built = false;     // boxes match the current sources / sites / media
     bool patch_dirty = false;       // shadows must be read again from the fields
     std::vector<omx::PatchBox> h_boxes;

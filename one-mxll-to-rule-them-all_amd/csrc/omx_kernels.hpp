@@ -1217,347 +1217,6 @@ void k_e_update(const YeeArgs<T> a, const long* __restrict__ disp, int n_disp, i
 }
 
 // ---------------------------------------------------------------------------
-// Fused step: H^{n+1/2} and E^{n+1} in one pass over HBM.  E and H are read
-// from the current buffers and written to the other pair, so every array is
-// read once and written once per step (12 words per cell instead of the 18
-// of separate H and E passes).  A workgroup is FR wavefronts, one per row
-// along y, of 64 lanes along z; it owns rows j0..j0+FR-2 of an x chunk and
-// walks the z tiles of those rows one after another (64 owned cells per
-// wave, aligned stores).  Row 0 is a halo: there H is recomputed (from the
-// unmodified input buffers) so that the E update of the owned rows has all
-// its H neighbours; the plane before the chunk is an x halo in the same way.
-// The H at k0-1 that lane 0 needs comes from lane 63 of the previous z tile
-// of the same row, kept in LDS per plane (the k = 0 boundary for the first
-// tile).  The psi of H is double-buffered like the fields (halo cells read
-// it); psi of E and the dispersive state are touched only by their owner.
-// The TF/SF corrections of H are applied inside, where E needs them.
-//
-// Data movement per plane i: E of plane i and its j+1 / k+1 neighbours come
-// from LDS (written one plane earlier), H rows of plane i go through LDS for
-// the E update, H lanes through cross-lane shuffles, the previous plane in
-// registers.  Everything of plane i+1 that comes from HBM (E two planes
-// ahead, H, tag, edge row and lane, z-face psi) is loaded at the top of
-// iteration i into the idle one of two register sets; the loop is unrolled
-// by two so that the sets alternate without copies and the loads stay in
-// flight across the plane's barrier.
-//
-// Arguments: the per-plane hot set is passed by value; everything the
-// x/y-face CPML, the dispersive media and the TF/SF corrections need sits in
-// a FusedCold block in device memory and is read with scalar loads on those
-// paths only, which keeps the hot loop's scalar registers from spilling.
-// ---------------------------------------------------------------------------
-template <typename T>
-struct FusedArgs {
-    const T *exi, *eyi, *ezi, *hxi, *hyi, *hzi;   // current fields
-    T *exo, *eyo, *ezo, *hxo, *hyo, *hzo;         // fields after the step
-    const uint8_t* tag;
-    const T *cx, *dex, *cex;                      // plane tables: cdhx|den_hx, den_ex, cdex|unused
-    const T* phz[2];                              // psiz_Hxz, psiz_Hyz (current)
-    T* phzo[2];                                   // psiz_Hxz, psiz_Hyz (after the step)
-    T* pez[2];                                    // psiz_Exz, psiz_Eyz (in place)
-    T dt_mu0, dt_eps0;
-    long sx;
-    int nx, ny, nz, sy, sz, off, npml, xchunk, ntz, nty;
-    unsigned cpml;
-    int ti0, ti1, tj0, tj1, tk0, tk1;       // TF/SF box
-    int disp_tag0;                          // first tag of the dispersive classes
-};
-
-template <typename T>
-struct FusedCold {
-    YeeArgs<T> a;                           // current fields / psi, tables, media
-    T* pho[6];                              // output psi of H
-    int tkb;                                // TF/SF 1D line origin
-    const double* ex1;                      // incident Ex on the 1D line
-    double dt_mu0;
-};
-
-
-// Values of one x plane p that a fused workgroup loads one plane ahead.
-template <typename T>
-struct FusedPlane {
-    T ex2, ey2, ez2;   // E of plane p+1
-    T hx, hy, hz;      // H of plane p
-    T ezj, exj;        // E of plane p at j+1 (last row; others: own cell)
-    T eyk, exk;        // E of plane p at k+1 (last lane; others: own cell)
-    T pzh0, pzh1;      // psiz_Hxz, psiz_Hyz of plane p (z-face lanes)
-    T pze0, pze1;      // psiz_Exz, psiz_Eyz of plane p (z-face lanes)
-    int tg;            // class tag of plane p
-};
-
-// Per-thread constants of the fused step.
-template <typename T>
-struct FusedLane {
-    int i0, i1, j, k, lane, row, me, plane;
-    bool hcell, own, last_row, last_lane, zh, ze;
-    long jk;                 // element offset of (j, k) in a plane, clamped into the grid
-    uint32_t sxb, side_rb, side_lb;
-    uint32_t zhb, zeb;       // byte offsets of this lane's z-face psi of H / E in plane i0-1
-    uint32_t zhs, zes;       // their plane strides in bytes
-    T dhy, dhz, dey, dez, chy, chz, cey, cez;
-    T zbh, zch, zbe, zce;    // z-face CPML coefficients of this lane (H, E)
-    // TF/SF: this lane's Hy (z face) and Hz (y face) corrections, constant
-    // over the step; applied in the planes ti0 <= i <= ti1-1
-    bool tf_y, tf_z;
-    double tfy, tfz;
-    // H at k0-1 handed from lane 63 of one z tile to lane 0 of the next:
-    // LDS at zbuf + ((parity * (xchunk+1) + plane) * FR + row) * 2
-    int zbuf, zin, zout, zstride;
-    bool zpass;              // a next z tile exists
-};
-
-// All loads are unconditional (no value depends on a branch, so the two
-// register sets of the pipeline need no copies): lanes and rows past the
-// grid load a clamped in-bounds cell, the edge-neighbour loads of the other
-// lanes and rows read their own cell, lanes outside the z faces read psi at
-// offset 0 of their (i, j) row, and the last plane of a chunk reloads itself.
-template <typename T, bool CPML, bool MEDIA>
-__device__ __forceinline__ void fused_load(const FusedArgs<T>& f, const FusedLane<T>& c, FusedPlane<T>& p,
-                                           uint32_t ob, long o, uint32_t zh, uint32_t ze) {
-    // ob / o: byte / element offset of plane p at this lane's (clamped) (j, k)
-    p.ex2 = ldb(f.exi, ob + c.sxb); p.ey2 = ldb(f.eyi, ob + c.sxb); p.ez2 = ldb(f.ezi, ob + c.sxb);
-    p.hx = ldb(f.hxi, ob); p.hy = ldb(f.hyi, ob); p.hz = ldb(f.hzi, ob);
-    if constexpr (MEDIA) p.tg = f.tag[o];
-    else p.tg = 0;
-    p.ezj = ldb(f.ezi, ob + c.side_rb); p.exj = ldb(f.exi, ob + c.side_rb);
-    p.eyk = ldb(f.eyi, ob + c.side_lb); p.exk = ldb(f.exi, ob + c.side_lb);
-    if constexpr (CPML) {
-        p.pzh0 = ldb(f.phz[0], zh); p.pzh1 = ldb(f.phz[1], zh);
-        p.pze0 = ldb(f.pez[0], ze); p.pze1 = ldb(f.pez[1], ze);
-    } else {
-        p.pzh0 = p.pzh1 = p.pze0 = p.pze1 = (T)0;
-    }
-}
-
-// One plane i of the fused step.  cur holds plane i (loaded one plane
-// earlier), nxt receives plane i+1.  LDS parity P: E rows of plane i in
-// buffer P, E rows of plane i+1 go to 1-P, H rows of plane i to P.
-template <typename T, bool CPML, bool MEDIA, bool TFSF, int P>
-__device__ __forceinline__ void fused_plane(const FusedArgs<T>& f, const FusedCold<T>* __restrict__ cold,
-                                            const FusedLane<T>& c, T* lds, int i, long o, uint32_t ob, uint32_t zh,
-                                            uint32_t ze, const FusedPlane<T>& cur, FusedPlane<T>& nxt, T& hzi,
-                                            T& hyi) {
-    const int pl = c.plane;
-    {
-        const bool more = i < c.i1;
-        fused_load<T, CPML, MEDIA>(f, c, nxt, ob + (more ? c.sxb : 0u), o + (more ? f.sx : 0),
-                                   zh + (more ? c.zhs : 0u), ze + (more ? c.zes : 0u));
-    }
-    // E of plane i and its j+1 / k+1 neighbours from LDS (edges from cur)
-    const int e0 = P * 3 * pl + c.me;
-    const T ezc = lds[e0], exc = lds[e0 + pl], eyc = lds[e0 + 2 * pl];
-    T ezj, exj, eyk, exk;
-    if (!c.last_row) { ezj = lds[e0 + kLanes]; exj = lds[e0 + pl + kLanes]; }
-    else { ezj = cur.ezj; exj = cur.exj; }
-    // lane 63 reads past its row (inside the LDS block) and keeps cur's value
-    eyk = lds[e0 + 2 * pl + 1]; exk = lds[e0 + pl + 1];
-    if (c.last_lane) { eyk = cur.eyk; exk = cur.exk; }
-    T hx = cur.hx, hy = cur.hy, hz = cur.hz;
-    const bool store = c.own && i >= c.i0;
-    const int j = c.j, k = c.k;
-    const T dmu = f.dt_mu0;
-    if (c.hcell && i >= 1) {
-        h_bulk(dmu, ldc(f.cx, i), c.dhy, c.dhz, c.chy, c.chz, ezc, cur.ez2, ezj, eyc, cur.ey2, eyk, exc, exk, exj,
-               hx, hy, hz);
-        if constexpr (CPML) {
-            if (f.cpml & 15u) {
-                const YeeArgs<T>& a = cold->a;
-                h_cpml_x(a, cold->pho, store, i, c.jk, dmu, ezc, cur.ez2, eyc, cur.ey2, hy, hz);
-                h_cpml_y(a, cold->pho, store, i, j, k, dmu, ezc, ezj, exc, exj, hx, hz);
-            }
-            if (c.zh) {
-                const T p0 = psi_step(cur.pzh0, c.zbh, c.zch, eyk - eyc);
-                const T p1 = psi_step(cur.pzh1, c.zbh, c.zch, exc - exk);
-                if (store) { stb(f.phzo[0], zh, p0); stb(f.phzo[1], zh, p1); }
-                hx = hx + dmu * p0; hy = hy + dmu * p1;
-            }
-        }
-        if constexpr (TFSF) {
-            if (i >= f.ti0 && i <= f.ti1 - 1) {
-                if (c.tf_y) hy = (T)((double)hy + c.tfy);
-                if (c.tf_z) hz = (T)((double)hz + c.tfz);
-            }
-        }
-    }
-    if (store) { stb(f.hxo, ob, hx); stb(f.hyo, ob, hy); stb(f.hzo, ob, hz); }
-    const int zp = i - c.i0 + 1;   // plane index within the chunk (x halo = 0)
-    {
-        const int h0 = 6 * pl + P * 2 * pl + c.me;
-        lds[h0] = hz;
-        lds[h0 + pl] = hx;
-        const int e1 = (1 - P) * 3 * pl + c.me;
-        lds[e1] = cur.ez2;
-        lds[e1 + pl] = cur.ex2;
-        lds[e1 + 2 * pl] = cur.ey2;
-        if (c.last_lane && c.zpass) {
-            const int zo = c.zout + zp * c.zstride;
-            lds[zo] = hy;
-            lds[zo + 1] = hx;
-        }
-    }
-    __syncthreads();
-    T hyk = __shfl_up(hy, 1), hxk = __shfl_up(hx, 1);
-    if (c.lane == 0) {
-        const int zi = c.zin + zp * c.zstride;
-        hyk = lds[zi];
-        hxk = lds[zi + 1];
-    }
-    if (store && (!MEDIA || cur.tg < f.disp_tag0)) {
-        const int h0 = 6 * pl + P * 2 * pl + c.me - kLanes;
-        const T hzj = lds[h0], hxj = lds[h0 + pl];
-        T ex = exc, ey = eyc, ez = ezc;
-        const T dep = f.dt_eps0;
-        e_bulk<T, MEDIA ? 1 : 0>(cold->a, cur.tg, dep, hz, hzj, hzi, hy, hyk, hyi, hx, hxj, hxk, ldc(f.dex, i),
-                         sizeof(T) == 4 ? ldc(f.cex, i) : (T)0, c.dey, c.dez, c.cey, c.cez, ex, ey, ez);
-        if constexpr (CPML) {
-            if (f.cpml & 15u) {
-                const YeeArgs<T>& a = cold->a;
-                e_cpml_x(a, i, c.jk, dep, hz, hzi, hy, hyi, ey, ez);
-                e_cpml_y(a, i, j, k, dep, hz, hzj, hx, hxj, ex, ez);
-            }
-            if (c.ze) {
-                const T p0 = psi_step(cur.pze0, c.zbe, c.zce, hyk - hy);
-                const T p1 = psi_step(cur.pze1, c.zbe, c.zce, hx - hxk);
-                stb(f.pez[0], ze, p0); stb(f.pez[1], ze, p1);
-                ex = ex + dep * p0; ey = ey + dep * p1;
-            }
-        }
-        stb(f.exo, ob, ex); stb(f.eyo, ob, ey); stb(f.ezo, ob, ez);
-    }
-    hzi = hz; hyi = hy;
-}
-
-// Lane constants of one z tile of the fused step.
-template <typename T, bool CPML, bool TFSF>
-__device__ __forceinline__ void fused_tile_lanes(const FusedArgs<T>& f, const FusedCold<T>* __restrict__ cold,
-                                                 FusedLane<T>& c, int tz) {
-    c.k = tz * kLanes + c.lane + 1;
-    c.hcell = c.k <= f.nz && c.j >= 1 && c.j <= f.ny;
-    c.own = c.hcell && c.row >= 1;
-    // lanes/rows past the grid work on a clamped cell: their values are
-    // never used by an owned cell, and every address stays in bounds
-    const int jc = min(c.j, f.ny + 1), kc = min(c.k, f.nz + 1);
-    c.jk = (long)jc * f.sz + kc + f.off;
-    c.side_lb = (c.last_lane && kc + 1 <= f.nz + 1) ? (uint32_t)sizeof(T) : 0u;
-    const YeeArgs<T>& a = cold->a;
-    c.dhz = a.den_hz[kc]; c.dez = a.den_ez[kc];
-    c.chz = sizeof(T) == 4 ? a.cdhz[kc] : (T)0;
-    c.cez = sizeof(T) == 4 ? a.cdez[kc] : (T)0;
-    c.zh = c.ze = false;
-    c.zbh = c.zch = c.zbe = c.zce = (T)0;
-    c.zhb = c.zeb = c.zhs = c.zes = 0u;
-    if constexpr (CPML) {
-        // z faces: H for k in [1, npml-1] and [nz+1-npml, nz-1] (coefficient
-        // index k resp. nz-k, cell m = k-1 resp. nz-1-k of zpsi_index); E for
-        // k in [2, npml] and [nz+1-npml, nz-1] (index k resp. nz+1-k, cell
-        // m = k-2 resp. nz-1-k)
-        const int npml = f.npml, nz = f.nz, k = c.k;
-        int ch = -1, sdh = 0, mh = 0, ce = -1, sde = 0, me = 0;
-        if (c.hcell) {
-            if ((f.cpml & 16u) && k <= npml - 1) { ch = k; mh = k - 1; }
-            else if ((f.cpml & 32u) && k >= nz + 1 - npml && k <= nz - 1) { ch = nz - k; sdh = 1; mh = nz - 1 - k; }
-            if ((f.cpml & 16u) && k >= 2 && k <= npml) { ce = k; me = k - 2; }
-            else if ((f.cpml & 32u) && k >= nz + 1 - npml && k <= nz - 1) { ce = nz + 1 - k; sde = 1; me = nz - 1 - k; }
-        }
-        c.zh = ch >= 0; c.ze = ce >= 0;
-        if (c.zh) { c.zbh = a.bh[ch]; c.zch = a.chd[ch]; }
-        if (c.ze) { c.zbe = a.be[ce]; c.zce = a.ced[ce]; }
-        const long row_h = zpsi_index(sdh, jc, c.i0 - 1, mh, f.sy, f.nx, npml);
-        const long row_e = zpsi_index(sde, jc, c.i0 - 1, me, f.sy, f.nx, npml);
-        c.zhb = (uint32_t)(row_h * (long)sizeof(T));
-        c.zeb = (uint32_t)(row_e * (long)sizeof(T));
-        c.zhs = (uint32_t)((long)(npml - 1) * (long)sizeof(T));
-        c.zes = c.zhs;
-    }
-    c.tf_y = c.tf_z = false;
-    c.tfy = c.tfz = 0.0;
-    if constexpr (TFSF) {
-        // the TF/SF H corrections: Hy -/+ dt/mu0 den_hz(k) Ex_inc on the
-        // z faces k = tk0-1 / tk1 (tj0 <= j <= tj1), Hz -/+ dt/mu0 den_hy(j)
-        // Ex_inc(k) on the y faces j = tj0-1 / tj1 (tk0 <= k <= tk1)
-        const FusedCold<T>& q = *cold;
-        const int j = c.j, k = c.k;
-        if (c.hcell && j >= f.tj0 && j <= f.tj1) {
-            if (k == f.tk0 - 1) { c.tf_y = true; c.tfy = q.dt_mu0 * (double)q.a.den_hz[k] * q.ex1[f.tk0 - q.tkb]; }
-            else if (k == f.tk1) { c.tf_y = true; c.tfy = -(q.dt_mu0 * (double)q.a.den_hz[k] * q.ex1[f.tk1 - q.tkb]); }
-        }
-        if (c.hcell && k >= f.tk0 && k <= f.tk1) {
-            if (j == f.tj0 - 1) { c.tf_z = true; c.tfz = -(q.dt_mu0 * (double)q.a.den_hy[j] * q.ex1[k - q.tkb]); }
-            else if (j == f.tj1) { c.tf_z = true; c.tfz = q.dt_mu0 * (double)q.a.den_hy[j] * q.ex1[k - q.tkb]; }
-        }
-    }
-    // H at k0-1: read from parity tz&1, written for the next tile to 1-(tz&1)
-    c.zin = c.zbuf + (tz & 1) * (f.xchunk + 1) * c.zstride + c.row * 2;
-    c.zout = c.zbuf + ((tz + 1) & 1) * (f.xchunk + 1) * c.zstride + c.row * 2;
-    c.zpass = tz + 1 < f.ntz;
-}
-
-template <typename T, bool CPML, bool MEDIA, bool TFSF>
-__global__ __launch_bounds__(512) void k_step_fused(const FusedArgs<T> f, const FusedCold<T>* __restrict__ cold) {
-    extern __shared__ unsigned char omx_lds[];
-    T* lds = reinterpret_cast<T*>(omx_lds);
-    FusedLane<T> c;
-    const int FR = blockDim.y;
-    // one wavefront per row: the row index is uniform, which turns the row
-    // conditions into scalar branches
-    c.lane = threadIdx.x; c.row = __builtin_amdgcn_readfirstlane(threadIdx.y);
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
-    const int ty = t % f.nty;
-    const int tx = t / f.nty;
-    c.j = ty * (FR - 1) + c.row;             // row 0: halo at j0-1
-    c.i0 = tx * f.xchunk + 1;
-    c.i1 = min(c.i0 + f.xchunk - 1, f.nx);
-    c.last_row = c.row == FR - 1;
-    c.last_lane = c.lane == kLanes - 1;
-    const int jc = min(c.j, f.ny + 1);
-    c.sxb = (uint32_t)(f.sx * (long)sizeof(T));
-    c.side_rb = (c.last_row && jc + 1 <= f.ny + 1) ? (uint32_t)(f.sz * sizeof(T)) : 0u;
-    {
-        const YeeArgs<T>& a = cold->a;
-        c.dhy = a.den_hy[jc]; c.dey = a.den_ey[jc];
-        c.chy = sizeof(T) == 4 ? a.cdhy[jc] : (T)0;
-        c.cey = sizeof(T) == 4 ? a.cdey[jc] : (T)0;
-    }
-    c.plane = FR * kLanes;
-    c.me = c.row * kLanes + c.lane;
-    c.zbuf = 10 * c.plane;
-    c.zstride = FR * 2;
-    // LDS: E rows (ez, ex, ey) of a plane at 3*plane*parity, H rows (hz, hx) at
-    // 6*plane + 2*plane*parity, the H handed between z tiles at zbuf.  Integer
-    // offsets keep every access a ds_ op.
-    for (int tz = 0; tz < f.ntz; ++tz) {
-        fused_tile_lanes<T, CPML, TFSF>(f, cold, c, tz);
-        long o = (long)(c.i0 - 1) * f.sx + c.jk;
-        uint32_t ob = (uint32_t)(o * (long)sizeof(T));
-        uint32_t zh = c.zhb, ze = c.zeb;
-        // prologue: E of plane i0-1 into LDS buffer 0, plane i0-1 into A;
-        // for the first tile lane 0 stores the boundary H at k = 0
-        FusedPlane<T> A, B;
-        lds[c.me] = ldb(f.ezi, ob);
-        lds[c.me + c.plane] = ldb(f.exi, ob);
-        lds[c.me + 2 * c.plane] = ldb(f.eyi, ob);
-        if (tz == 0 && c.lane == 0) {
-            for (int p = 0; p <= c.i1 - c.i0 + 1; ++p) {
-                const uint32_t b0 = ob + (uint32_t)p * c.sxb - (uint32_t)sizeof(T);
-                lds[c.zin + p * c.zstride] = ldb(f.hyi, b0);
-                lds[c.zin + p * c.zstride + 1] = ldb(f.hxi, b0);
-            }
-        }
-        fused_load<T, CPML, MEDIA>(f, c, A, ob, o, zh, ze);
-        __syncthreads();
-        T hzi = (T)0, hyi = (T)0;
-        for (int i = c.i0 - 1;; i += 2) {
-            fused_plane<T, CPML, MEDIA, TFSF, 0>(f, cold, c, lds, i, o, ob, zh, ze, A, B, hzi, hyi);
-            if (i + 1 > c.i1) break;
-            o += f.sx; ob += c.sxb; zh += c.zhs; ze += c.zes;
-            fused_plane<T, CPML, MEDIA, TFSF, 1>(f, cold, c, lds, i + 1, o, ob, zh, ze, B, A, hzi, hyi);
-            if (i + 2 > c.i1) break;
-            o += f.sx; ob += c.sxb; zh += c.zhs; ze += c.zes;
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
 // Batched launches: one launch covers the tiles of many independent grids.
 // tile_start[g] is the first tile of grid g (tile_start[n] = total); each
 // workgroup finds its grid by binary search and runs the single-grid tile
@@ -2235,3 +1894,5 @@ __global__ void k_tfsf_1d_h(double* __restrict__ ex1, double* __restrict__ hy1, 
 }
 
 }  // namespace omx
+
+#include "omx_fused.hpp"

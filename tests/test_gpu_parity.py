@@ -265,11 +265,11 @@ def test_tiling_variants_identical():
         else:
             np.testing.assert_array_equal(rec, ref)
         g.kill()
-    # fused H+E step under several tile shapes: identical among themselves,
-    # and equal to the separate kernels up to the order in which the point
-    # source enters E at its own cell
-    fref = None
-    for rows, xc in ((8, 8), (2, 1), (7, 3), (5, 64)):
+    # fused H+E step under several tile shapes (rows per wavefront, planes
+    # per chunk): each equal to the separate kernels up to rounding (the
+    # point source enters E at its own cell after the update, and each row
+    # count is its own kernel, contracted to multiply-adds on its own)
+    for rows, xc in ((4, 8), (2, 1), (4, 3), (1, 64), (2, 5)):
         dr, dt = case.dr_dt
         g = omx_fdtd.Grid3D(case.grid, case.npml, case.boundaries, dt, dr, precision=8)
         g.set_step_mode(0, rows, xc)
@@ -277,11 +277,10 @@ def test_tiling_variants_identical():
         g.run(1, case.steps, 1)
         assert g.last_run_fused()
         rec = g.read_records()
-        if fref is None:
-            fref = rec
-            assert rel_l2(rec, ref) <= 1e-13
-        else:
-            np.testing.assert_array_equal(rec, fref)
+        r = rel_l2(rec, ref)
+        m = np.abs(rec - ref).max() / np.abs(ref).max()
+        print(f"fused rows {rows} xchunk {xc}: relL2 vs separate {r:.3e}, max |diff|/max|ref| {m:.3e}")
+        assert r <= 1e-13 and m <= 1e-13
         g.kill()
 
 

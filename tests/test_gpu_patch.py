@@ -65,6 +65,9 @@ def test_patch_boxes():
     print("[patch] box construction", flush=True)
     case = point_case(n=60, npml=8, cells=[(30, 30, 30), (36, 31, 29)])
     g = engine(case, 4)
+    assert g.source_patch()[0] == 16             # the default radius
+    g.kill()
+    g = engine(case, 4, radius=8)
     r, nb, nc = g.source_patch()
     assert (r, nb) == (8, 1)                     # two touching boxes merge
     assert nc == (36 + 8 - 22 + 1) * (31 + 8 - 22 + 1) * (30 + 8 - 21 + 1)
@@ -75,7 +78,7 @@ def test_patch_boxes():
     g.kill()
     # clipped to the cells outside the CPML faces (9..n-8 with npml 8)
     case = point_case(n=60, npml=8, cells=[(12, 30, 50)])
-    g = engine(case, 4)
+    g = engine(case, 4, radius=8)
     assert g.source_patch() == (8, 1, (20 - 9 + 1) * 17 * (52 - 42 + 1))
     g.kill()
     # a source inside the CPML gets no box
@@ -101,15 +104,15 @@ def test_patch_brings_point_source_within_tolerance(mode):
     want = oracle_series(case)
     assert np.abs(want).max() > 0
     res = {}
-    for radius in (0, 8):
+    for radius in (0, 8, 16):
         g = engine(case, 4, radius, mode)
         g.run(1, case.steps, case.record_every)
         if mode == 0:
             assert g.last_run_fused()
         res[radius] = rel_l2(g.read_records(), want)
         g.kill()
-    print(f"[patch] FP32 vs oracle relL2: plain {res[0]:.3e}, patch r=8 {res[8]:.3e}", flush=True)
-    assert res[8] <= TOL32
+    print(f"[patch] FP32 vs oracle relL2: plain {res[0]:.3e}, r=8 {res[8]:.3e}, r=16 {res[16]:.3e}", flush=True)
+    assert res[8] <= TOL32 and res[16] <= TOL32
     assert res[0] >= 5 * res[8]
 
 
