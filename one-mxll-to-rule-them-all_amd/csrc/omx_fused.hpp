@@ -115,23 +115,25 @@ __device__ __forceinline__ Coef<T> coef_at(const YeeArgs<T>& a, const T* den_h, 
     return c;
 }
 
-// TF/SF correction of H at cell (i,j,k) (the H side of the plane-wave box;
-// same values as tfsf_h_lane): cy to Hy, cz to Hz.
-template <typename T>
-__device__ __forceinline__ void tfsf_h_cell(const YeeArgs<T>& a, const FusedOut<T>& fo, int i, int j, int k,
-                                            double& cy, double& cz) {
+// TF/SF corrections of H in z cell k (same values as tfsf_h_lane), read
+// once per wavefront: cy is the Hy correction of the cells of the box's z
+// faces (k = k0-1 or k1) for rows j0..j1, e1 the incident Ex at k (k0..k1,
+// else 0) that the Hz correction of rows j0-1 and j1 scales.  The cell's
+// plane decides whether either applies.
+struct TfLane {
+    double cy, e1;
+};
+template <typename T, bool U>
+__device__ __forceinline__ TfLane tf_lane(const YeeArgs<T>& a, const FusedOut<T>& fo, int k) {
     const TfsfArgs& t = fo.t;
-    cy = 0.0;
-    cz = 0.0;
-    if (i < t.i0 || i > t.i1 - 1) return;
-    if (j >= t.j0 && j <= t.j1) {
-        if (k == t.k0 - 1) cy = t.dt_mu0 * (double)gld(a.den_hz, k) * fo.ex1[t.k0 - t.kb];
-        else if (k == t.k1) cy = -(t.dt_mu0 * (double)gld(a.den_hz, k) * fo.ex1[t.k1 - t.kb]);
+    TfLane v{0.0, 0.0};
+    if (k == t.k0 - 1 || k == t.k1) {
+        const bool hi = k == t.k1;
+        const double c = t.dt_mu0 * (double)(U ? ldc(a.den_hz, k) : gld(a.den_hz, k)) * fo.ex1[(hi ? t.k1 : t.k0) - t.kb];
+        v.cy = hi ? -c : c;
     }
-    if (k >= t.k0 && k <= t.k1) {
-        if (j == t.j0 - 1) cz = -(t.dt_mu0 * (double)gld(a.den_hy, j) * fo.ex1[k - t.kb]);
-        else if (j == t.j1) cz = t.dt_mu0 * (double)gld(a.den_hy, j) * fo.ex1[k - t.kb];
-    }
+    if (k >= t.k0 && k <= t.k1) v.e1 = fo.ex1[k - t.kb];
+    return v;
 }
 
 // CPML psi offsets of cell (i,j,k) (bytes; x and y faces: slot form, z
@@ -191,7 +193,7 @@ __device__ __forceinline__ Psi<T> psi_read(const YeeArgs<T>& a, const PsiOff& o,
 // H^{n+1/2} of the components WX/WY/WZ of cell (i,j,k) (h in: H^{n-1/2}):
 // the bulk curl, then the CPML psi of the faces the cell lies in (x, y, z,
 // as the reference; ps: psi of H at n-1/2 read by psi_read), then the TF/SF
-// correction.  STORE writes the new psi to fo.ph at po where act is set
+// correction (tl: the cell's z cell's values, tf_lane).  STORE writes the new psi to fo.ph at po where act is set
 // (owned cells).
 // cx is the plane's coefficient (cdhx[i] in FP32, den_hx[i] in FP64) and px
 // its face; ry / lz the row's and the lane's coefficients and faces.
@@ -199,8 +201,8 @@ __device__ __forceinline__ Psi<T> psi_read(const YeeArgs<T>& a, const PsiOff& o,
 template <typename T, int CPML, bool TF, bool WX, bool WY, bool WZ, bool STORE>
 __device__ __forceinline__ void h_fused(const YeeArgs<T>& a, const FusedOut<T>& fo, int i, int j, int k, T cx,
                                         const Coef<T>& px, const Coef<T>& ry, const Coef<T>& lz, const Psi<T>& ps,
-                                        const PsiOff& po, T ezc, T ezj, T ezn, T eyc, T eyk, T eyn, T exc, T exk,
-                                        T exj, T& hx, T& hy, T& hz, bool act = true) {
+                                        const PsiOff& po, const TfLane& tl, T ezc, T ezj, T ezn, T eyc, T eyk, T eyn,
+                                        T exc, T exk, T exj, T& hx, T& hy, T& hz, bool act = true) {
     using O = Off<T, true>;
     const T dmu = a.dt_mu0;
     if constexpr (sizeof(T) == 4) {
@@ -251,21 +253,25 @@ __device__ __forceinline__ void h_fused(const YeeArgs<T>& a, const FusedOut<T>& 
         }
     }
     if constexpr (TF) {
-        double cy, cz;
-        tfsf_h_cell(a, fo, i, j, k, cy, cz);
-        if (WY && cy != 0.0) hy = (T)((double)hy + cy);
-        if (WZ && cz != 0.0) hz = (T)((double)hz + cz);
+        const TfsfArgs& t = fo.t;
+        if (i >= t.i0 && i <= t.i1 - 1) {
+            if (WY && j >= t.j0 && j <= t.j1 && tl.cy != 0.0) hy = (T)((double)hy + tl.cy);
+            if (WZ && (j == t.j0 - 1 || j == t.j1) && tl.e1 != 0.0) {
+                const double c = t.dt_mu0 * (double)ldc(a.den_hy, j) * tl.e1;
+                hz = (T)((double)hz + (j == t.j1 ? c : -c));
+            }
+        }
     }
 }
 
 // The H update of a halo cell: its psi read here, nothing stored.
 template <typename T, int CPML, bool TF, bool WX, bool WY, bool WZ>
 __device__ __forceinline__ void h_halo(const YeeArgs<T>& a, const FusedOut<T>& fo, int i, int j, int k, T cx,
-                                       const Coef<T>& px, const Coef<T>& ry, const Coef<T>& lz, T ezc, T ezj, T ezn,
-                                       T eyc, T eyk, T eyn, T exc, T exk, T exj, T& hx, T& hy, T& hz) {
+                                       const Coef<T>& px, const Coef<T>& ry, const Coef<T>& lz, const TfLane& tl, T ezc,
+                                       T ezj, T ezn, T eyc, T eyk, T eyn, T exc, T exk, T exj, T& hx, T& hy, T& hz) {
     const PsiOff po = psi_off<T, false>(a, i, j, k, px, ry, lz);
     const Psi<T> ps = psi_read<T, CPML, false, WX, WY, WZ>(a, po, px, ry, lz);
-    h_fused<T, CPML, TF, WX, WY, WZ, false>(a, fo, i, j, k, cx, px, ry, lz, ps, po, ezc, ezj, ezn, eyc, eyk, eyn,
+    h_fused<T, CPML, TF, WX, WY, WZ, false>(a, fo, i, j, k, cx, px, ry, lz, ps, po, tl, ezc, ezj, ezn, eyc, eyk, eyn,
                                             exc, exk, exj, hx, hy, hz);
 }
 
@@ -346,6 +352,11 @@ __global__ __launch_bounds__(256) void k_step_fused(const YeeArgs<T> a, const Fu
     const int kh = k0 - 1;                      // lane 0's halo cell (k0 > 1)
     Coef<T> lh = lz;
     if (kh >= 1) lh = coef_at<T, (CPML & 4) != 0, true, true>(a, a.den_hz, a.cdhz, a.den_ez, a.cdez, kh, a.nz, 16u, 32u);
+    TfLane tfl{0.0, 0.0}, tfh{0.0, 0.0};        // TF/SF values of the lane's cell and of lane 0's halo cell
+    if constexpr (TF) {
+        tfl = tf_lane<T, false>(a, fo, k);
+        if (kh >= 1) tfh = tf_lane<T, true>(a, fo, kh);
+    }
     Coef<T> ry[R + 1];                          // rows j0-1 .. j0+R-1
 #pragma unroll
     for (int r = 0; r <= R; ++r)
@@ -375,17 +386,20 @@ __global__ __launch_bounds__(256) void k_step_fused(const YeeArgs<T> a, const Fu
                 const T exj = O::ld(a.ex, p + sz);
                 const T ex63 = (full && lane == 63) ? O::ld(a.ex, p + S) : (T)0;
                 const T exk = dpp_shl1(ex63, exq);
-                h_halo<T, CPML, TF, false, true, true>(a, fo, i0 - 1, j0 + r, kc, cx, px, ry[r + 1], lz, ezq,
+                h_halo<T, CPML, TF, false, true, true>(a, fo, i0 - 1, j0 + r, kc, cx, px, ry[r + 1], lz, tfl, ezq,
                                                               (T)0, ezc[r + 1], eyq, (T)0, eyc[r + 1], exq, exk, exj,
                                                               hx, hy, hz);
             }
             hyp[r] = hy; hzp[r] = hz;
         }
     }
+    // Each plane in two phases: every load of the plane first (so the
+    // wavefront waits on memory once per plane), then the arithmetic.
     for (int i = i0; i <= i1; ++i, o += sx) {
         const T cx = ldc(cxs, i);
         const T dex = ldc(a.den_ex, i), cex = sizeof(T) == 4 ? ldc(a.cdex, i) : (T)0;
         const Coef<T> px = coef_at<T, (CPML & 1) != 0, false, true>(a, a.den_hx, a.cdhx, a.den_ex, a.cdex, i, a.nx, 1u, 2u);
+        // -- loads --
         // E^n of plane i+1 (the next plane's current values and this plane's i+1 neighbours)
         T exn[R + 1], eyn[R + 1], ezn[R + 1];
 #pragma unroll
@@ -396,15 +410,56 @@ __global__ __launch_bounds__(256) void k_step_fused(const YeeArgs<T> a, const Fu
         }
         const uint32_t ob = o + (uint32_t)(nr * (int)sz);          // row j0+nr
         const T ezB = O::ld(a.ez, ob), exB = O::ld(a.ex, ob);
-        // halo row j0-1: Hx, Hz (the boundary row for j0 = 1)
+        // halo row j0-1: Hx, Hz (the boundary row for j0 = 1), its psi and Ey(k0+64)
         T hxm = O::ld(a.hx, o - sz), hzm = O::ld(a.hz, o - sz);
+        PsiOff pom;
+        Psi<T> psm;
+        T eym63 = (T)0;
         if (yhalo) {
-            const T ey63 = (full && lane == 63) ? O::ld(a.ey, o - sz + S) : (T)0;
-            const T eyk = dpp_shl1(ey63, eyc[0]);
+            pom = psi_off<T, false>(a, i, j0 - 1, kc, px, ry[0], lz);
+            psm = psi_read<T, CPML, false, true, false, true>(a, pom, px, ry[0], lz);
+            if (full && lane == 63) eym63 = O::ld(a.ey, o - sz + S);
+        }
+        // per row: H^{n-1/2}, E(k+1) of lane 63, psi of H and E, class tag,
+        // and lane 0's inputs for H at k0-1
+        T hxo[R], hyo[R], hzo[R], ex63[R], ey63[R];
+        T hx0[R], hy0[R], ezh[R], ezjh[R], eznh[R], eyh[R], exh[R];
+        int tg[R];
+        PsiOff poh[R], poe[R], pol[R];
+        Psi<T> psh[R], pse[R], psl[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (r >= nr) break;
+            const int j = j0 + r;
+            const uint32_t q = o + (uint32_t)(r * (int)sz);
+            const Coef<T>& cy = ry[r + 1];
+            poh[r] = psi_off<T, false>(a, i, j, kc, px, cy, lz);
+            poe[r] = psi_off<T, true>(a, i, j, kc, px, cy, lz);
+            psh[r] = psi_read<T, CPML, false, true, true, true>(a, poh[r], px, cy, lz);
+            pse[r] = psi_read<T, CPML, true, true, true, true>(a, poe[r], px, cy, lz);
+            tg[r] = (MED != 0 && a.tag != nullptr) ? O::tag(a.tag, q) : 0;
+            hxo[r] = O::ld(a.hx, q); hyo[r] = O::ld(a.hy, q); hzo[r] = O::ld(a.hz, q);
+            ex63[r] = (T)0; ey63[r] = (T)0;
+            if (full && lane == 63) { ex63[r] = O::ld(a.ex, q + S); ey63[r] = O::ld(a.ey, q + S); }
+            hx0[r] = (T)0; hy0[r] = (T)0;
+            ezh[r] = ezjh[r] = eznh[r] = eyh[r] = exh[r] = (T)0;
+            if (lane == 0) {
+                hx0[r] = O::ld(a.hx, q - S); hy0[r] = O::ld(a.hy, q - S);
+                if (kh >= 1) {
+                    ezh[r] = O::ld(a.ez, q - S); ezjh[r] = O::ld(a.ez, q - S + sz); eznh[r] = O::ld(a.ez, q - S + sx);
+                    eyh[r] = O::ld(a.ey, q - S); exh[r] = O::ld(a.ex, q - S);
+                    pol[r] = psi_off<T, false>(a, i, j, kh, px, cy, lh);
+                    psl[r] = psi_read<T, CPML, false, true, true, false>(a, pol[r], px, cy, lh);
+                }
+            }
+        }
+        // -- arithmetic --
+        if (yhalo) {
+            const T eyk = dpp_shl1(eym63, eyc[0]);
             T hy = (T)0;
-            h_halo<T, CPML, TF, true, false, true>(a, fo, i, j0 - 1, kc, cx, px, ry[0], lz, ezc[0], ezc[1],
-                                                          (T)0, eyc[0], eyk, eyn[0], exc[0], (T)0, exc[1], hxm, hy,
-                                                          hzm);
+            h_fused<T, CPML, TF, true, false, true, false>(a, fo, i, j0 - 1, kc, cx, px, ry[0], lz, psm, pom, tfl, ezc[0],
+                                                          ezc[1], (T)0, eyc[0], eyk, eyn[0], exc[0], (T)0, exc[1],
+                                                          hxm, hy, hzm);
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -412,40 +467,27 @@ __global__ __launch_bounds__(256) void k_step_fused(const YeeArgs<T> a, const Fu
             const int j = j0 + r;
             const uint32_t q = o + (uint32_t)(r * (int)sz);
             const Coef<T>& cy = ry[r + 1];
-            // psi of H and E of this cell, read with its fields
-            const PsiOff poh = psi_off<T, false>(a, i, j, kc, px, cy, lz);
-            const PsiOff poe = psi_off<T, true>(a, i, j, kc, px, cy, lz);
-            const Psi<T> psh = psi_read<T, CPML, false, true, true, true>(a, poh, px, cy, lz);
-            const Psi<T> pse = psi_read<T, CPML, true, true, true, true>(a, poe, px, cy, lz);
-            const int tg = (MED != 0 && a.tag != nullptr) ? O::tag(a.tag, q) : 0;
-            T hx = O::ld(a.hx, q), hy = O::ld(a.hy, q), hz = O::ld(a.hz, q);
-            T ex63 = (T)0, ey63 = (T)0;
-            if (full && lane == 63) { ex63 = O::ld(a.ex, q + S); ey63 = O::ld(a.ey, q + S); }
-            const T exk = dpp_shl1(ex63, exc[r + 1]), eyk = dpp_shl1(ey63, eyc[r + 1]);
+            T hx = hxo[r], hy = hyo[r], hz = hzo[r];
+            const T exk = dpp_shl1(ex63[r], exc[r + 1]), eyk = dpp_shl1(ey63[r], eyc[r + 1]);
             const T ezj = r + 1 < nr ? ezc[r + 1 < R ? r + 2 : R] : ezB;
             const T exj = r + 1 < nr ? exc[r + 1 < R ? r + 2 : R] : exB;
             // lane 0: H of cell k0-1 (the boundary k = 0 for the first z tile)
-            T hxk0 = (T)0, hyk0 = (T)0;
-            if (lane == 0) {
-                hxk0 = O::ld(a.hx, q - S); hyk0 = O::ld(a.hy, q - S);
-                if (kh >= 1) {
-                    const T ezh = O::ld(a.ez, q - S), ezjh = O::ld(a.ez, q - S + sz), eznh = O::ld(a.ez, q - S + sx);
-                    const T eyh = O::ld(a.ey, q - S), exh = O::ld(a.ex, q - S);
-                    T hzd = (T)0;
-                    h_halo<T, CPML, TF, true, true, false>(a, fo, i, j, kh, cx, px, cy, lh, ezh, ezjh, eznh,
-                                                                  eyh, eyc[r + 1], (T)0, exh, exc[r + 1], (T)0, hxk0,
-                                                                  hyk0, hzd);
-                }
+            T hxk0 = hx0[r], hyk0 = hy0[r];
+            if (lane == 0 && kh >= 1) {
+                T hzd = (T)0;
+                h_fused<T, CPML, TF, true, true, false, false>(a, fo, i, j, kh, cx, px, cy, lh, psl[r], pol[r], tfh, ezh[r],
+                                                              ezjh[r], eznh[r], eyh[r], eyc[r + 1], (T)0, exh[r],
+                                                              exc[r + 1], (T)0, hxk0, hyk0, hzd);
             }
             // this cell's H^{n+1/2} (psi of H stored by lanes inside the grid)
-            h_fused<T, CPML, TF, true, true, true, true>(a, fo, i, j, kc, cx, px, cy, lz, psh, poh, ezc[r + 1], ezj,
-                                                        ezn[r + 1], eyc[r + 1], eyk, eyn[r + 1], exc[r + 1], exk, exj,
-                                                        hx, hy, hz, act);
+            h_fused<T, CPML, TF, true, true, true, true>(a, fo, i, j, kc, cx, px, cy, lz, psh[r], poh[r], tfl, ezc[r + 1],
+                                                        ezj, ezn[r + 1], eyc[r + 1], eyk, eyn[r + 1], exc[r + 1], exk,
+                                                        exj, hx, hy, hz, act);
             const T hxk = dpp_shr1(hxk0, hx), hyk = dpp_shr1(hyk0, hy);
-            const bool own = MED != 1 || tg < a.disp_tag0;
+            const bool own = MED != 1 || tg[r] < a.disp_tag0;
             T ex = exc[r + 1], ey = eyc[r + 1], ez = ezc[r + 1];
-            e_fused<T, CPML, MED>(a, tg, own && act, dex, cex, px, cy, lz, pse, poe, hz, hzm, hzp[r], hy, hyk, hyp[r],
-                                  hx, hxm, hxk, ex, ey, ez);
+            e_fused<T, CPML, MED>(a, tg[r], own && act, dex, cex, px, cy, lz, pse[r], poe[r], hz, hzm, hzp[r], hy,
+                                  hyk, hyp[r], hx, hxm, hxk, ex, ey, ez);
             if (act) {
                 O::st(fo.hx, q, hx); O::st(fo.hy, q, hy); O::st(fo.hz, q, hz);
                 if (own) { O::st(fo.ex, q, ex); O::st(fo.ey, q, ey); O::st(fo.ez, q, ez); }
