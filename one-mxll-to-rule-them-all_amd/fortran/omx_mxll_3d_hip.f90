@@ -1,9 +1,12 @@
-! omx_mxll_3d_hip.f90 -- TMxll_3D_hip, the GPU 3D solver behind the TMxll
-! interface.  Host side of the drop-in: the four TMxll bindings keep the
-! reference's signatures (src/FDTD/mxll_base_mod.f90:27-30,36-67); init also
-! reads the reference's medium_XXXX.in files (classical_medium_mod.f90:603-690)
-! and hands the cell medium map to the engine, which derives the staggered
-! per-component map and the update coefficients on the device.
+! omx_mxll_3d_hip.f90 -- TMxll_3D_hip, the GPU 3D solver: a TMxll_3D (so
+! callers that branch on `class is (TMxll_3D)` reach it), made by
+! maxwell_factory(3).  Host side of the drop-in: the four TMxll bindings keep
+! the reference's signatures (src/FDTD/mxll_base_mod.f90:27-30,36-67); init
+! also reads the reference's medium_XXXX.in files
+! (classical_medium_mod.f90:603-690) and hands the cell medium map to the
+! engine, which derives the staggered per-component map and the update
+! coefficients on the device.  sync_to_host / sync_to_device copy E and H
+! between the device and TMxll_3D's host mirrors (reference bounds).
 !
 ! The couplings the reference performs by touching the solver's arrays
 ! (sources, E at quantum sites, J scatter, detectors, field dumps) are type-
@@ -16,13 +19,12 @@ module omx_mxll_3d_hip_mod
     use iso_c_binding
     use omx_units_mod
     use omx_mxll_base_mod
+    use omx_mxll_3d_mod
     use omx_fdtd_cabi
     implicit none
 
-    type, extends(TMxll) :: TMxll_3D_hip
+    type, extends(TMxll_3D) :: TMxll_3D_hip
         type(c_ptr) :: ctx = c_null_ptr
-        integer :: nx = 0, ny = 0, nz = 0
-        integer :: boundaries(3) = 0
         integer :: precision = 0
         integer :: n_samples = 0
         integer :: n_sites = 0
@@ -40,6 +42,9 @@ module omx_mxll_3d_hip_mod
         procedure :: scatter_J => scatter_J_3D_hip
         procedure :: energy => energy_3D_hip
         procedure :: get_field => get_field_3D_hip
+        procedure :: set_field => set_field_3D_hip
+        procedure :: sync_to_host => sync_to_host_3D_hip
+        procedure :: sync_to_device => sync_to_device_3D_hip
         procedure :: synchronize => synchronize_3D_hip
     end type TMxll_3D_hip
 
@@ -61,6 +66,7 @@ contains
             error stop 1
         end if
         this%dimensions = 3
+        this%chunk_coor = mpi_coords
         this%nx = grid_Ndims(1); this%ny = grid_Ndims(2); this%nz = grid_Ndims(3)
         this%npml = npml
         this%boundaries = boundaries
@@ -164,6 +170,7 @@ contains
             rc = omx3d_destroy(this%ctx)
             this%ctx = c_null_ptr
         end if
+        call this%free_mirrors()
     end subroutine kill_3D_hip
 
     subroutine td_propagate_H_3D_hip(this)
@@ -291,6 +298,41 @@ contains
         real(dp), intent(inout) :: host(:, :, :)
         call omx_check(omx3d_copy_field(this%ctx, field, host, 0), "omx3d_copy_field")
     end subroutine get_field_3D_hip
+
+    ! Host array of one field with the reference's bounds -> the device.
+    subroutine set_field_3D_hip(this, field, host)
+        class(TMxll_3D_hip), intent(inout) :: this
+        integer, intent(in) :: field
+        real(dp), intent(inout) :: host(:, :, :)
+        call omx_check(omx3d_copy_field(this%ctx, field, host, 1), "omx3d_copy_field")
+    end subroutine set_field_3D_hip
+
+    ! E and H of the device into the mirrors this%Ex .. this%Hz (waits for
+    ! the queued steps; FP32 grids are widened to FP64).
+    subroutine sync_to_host_3D_hip(this)
+        class(TMxll_3D_hip), intent(inout) :: this
+        call this%alloc_mirrors()
+        call omx_check(omx3d_copy_field(this%ctx, Ex_FIELD, this%Ex, 0), "omx3d_copy_field")
+        call omx_check(omx3d_copy_field(this%ctx, Ey_FIELD, this%Ey, 0), "omx3d_copy_field")
+        call omx_check(omx3d_copy_field(this%ctx, Ez_FIELD, this%Ez, 0), "omx3d_copy_field")
+        call omx_check(omx3d_copy_field(this%ctx, Hx_FIELD, this%Hx, 0), "omx3d_copy_field")
+        call omx_check(omx3d_copy_field(this%ctx, Hy_FIELD, this%Hy, 0), "omx3d_copy_field")
+        call omx_check(omx3d_copy_field(this%ctx, Hz_FIELD, this%Hz, 0), "omx3d_copy_field")
+    end subroutine sync_to_host_3D_hip
+
+    ! The mirrors (as the caller left them) onto the device: a caller that
+    ! edits fields on the host, as the reference's couplings do, hands them
+    ! back before the next step.
+    subroutine sync_to_device_3D_hip(this)
+        class(TMxll_3D_hip), intent(inout) :: this
+        if (.not. allocated(this%Ex)) return
+        call omx_check(omx3d_copy_field(this%ctx, Ex_FIELD, this%Ex, 1), "omx3d_copy_field")
+        call omx_check(omx3d_copy_field(this%ctx, Ey_FIELD, this%Ey, 1), "omx3d_copy_field")
+        call omx_check(omx3d_copy_field(this%ctx, Ez_FIELD, this%Ez, 1), "omx3d_copy_field")
+        call omx_check(omx3d_copy_field(this%ctx, Hx_FIELD, this%Hx, 1), "omx3d_copy_field")
+        call omx_check(omx3d_copy_field(this%ctx, Hy_FIELD, this%Hy, 1), "omx3d_copy_field")
+        call omx_check(omx3d_copy_field(this%ctx, Hz_FIELD, this%Hz, 1), "omx3d_copy_field")
+    end subroutine sync_to_device_3D_hip
 
     subroutine synchronize_3D_hip(this)
         class(TMxll_3D_hip), intent(inout) :: this

@@ -55,6 +55,7 @@ module omx_mxll_base_mod
         real(dp) :: dt_mu0
         real(dp) :: dt
         real(dp) :: dr
+        integer  :: chunk_coor(3) = 0
     contains
         procedure(init_interface),      deferred :: init
         procedure(kill_interface),      deferred :: kill
@@ -96,3 +97,58 @@ module omx_mxll_base_mod
     end interface
 
 end module omx_mxll_base_mod
+
+
+! omx_mxll_3d_mod declares TMxll_3D, the shape of the reference's 3D solver
+! type (src/FDTD/mxll_3D_mod.f90:9-58) as callers see it through
+! `class is (TMxll_3D)`: the grid size and boundaries, and host mirrors of E
+! and H with the reference's bounds (mxll_3D_mod.f90:123-131), e.g.
+! Ex(0:nx, 1:ny+1, 1:nz+1).  An implementation that keeps the fields on a
+! device fills the mirrors in sync_to_host and writes them back in
+! sync_to_device; the mirrors are allocated (and zeroed) on first use, so a
+! run that never syncs holds no host copy of the grid.
+module omx_mxll_3d_mod
+
+    use omx_units_mod
+    use omx_mxll_base_mod
+    implicit none
+
+    type, abstract, extends(TMxll) :: TMxll_3D
+        integer :: nx = 0, ny = 0, nz = 0
+        integer :: boundaries(3) = 0
+        real(dp), allocatable :: Ex(:, :, :), Ey(:, :, :), Ez(:, :, :)
+        real(dp), allocatable :: Hx(:, :, :), Hy(:, :, :), Hz(:, :, :)
+    contains
+        procedure(sync_interface), deferred :: sync_to_host
+        procedure(sync_interface), deferred :: sync_to_device
+        procedure :: alloc_mirrors => alloc_mirrors_3D
+        procedure :: free_mirrors => free_mirrors_3D
+    end type TMxll_3D
+
+    abstract interface
+        subroutine sync_interface(this)
+            import :: TMxll_3D
+            class(TMxll_3D), intent(inout) :: this
+        end subroutine sync_interface
+    end interface
+
+contains
+
+    subroutine alloc_mirrors_3D(this)
+        class(TMxll_3D), intent(inout) :: this
+        integer :: nx, ny, nz
+        nx = this%nx; ny = this%ny; nz = this%nz
+        if (.not. allocated(this%Ex)) then
+            allocate(this%Ex(0:nx, ny + 1, nz + 1), this%Ey(nx + 1, 0:ny, nz + 1), this%Ez(nx + 1, ny + 1, 0:nz))
+            allocate(this%Hx(nx, 0:ny, 0:nz), this%Hy(0:nx, ny, 0:nz), this%Hz(0:nx, 0:ny, nz))
+            this%Ex = 0.0d0; this%Ey = 0.0d0; this%Ez = 0.0d0
+            this%Hx = 0.0d0; this%Hy = 0.0d0; this%Hz = 0.0d0
+        end if
+    end subroutine alloc_mirrors_3D
+
+    subroutine free_mirrors_3D(this)
+        class(TMxll_3D), intent(inout) :: this
+        if (allocated(this%Ex)) deallocate(this%Ex, this%Ey, this%Ez, this%Hx, this%Hy, this%Hz)
+    end subroutine free_mirrors_3D
+
+end module omx_mxll_3d_mod

@@ -129,8 +129,13 @@ program omx_ref_driver
                 dirname = "./output_detector_" // trim(number)
                 call execute_command_line("mkdir -p " // trim(dirname))
                 if (detectors(n)%detector_type == POINT_DETECTOR) then
+                    ! header with the point's position (output_mod.f90:131-161)
+                    x_min = (detectors(n)%i_min*dr - int(mpi_dims(1)*grid_Ndims(1)/2)*dr)*au_to_nm
+                    y_min = (detectors(n)%j_min*dr - int(mpi_dims(2)*grid_Ndims(2)/2)*dr)*au_to_nm
+                    z_min = (detectors(n)%k_min*dr - int(mpi_dims(3)*grid_Ndims(3)/2)*dr)*au_to_nm
                     open(newunit=funit, file=trim(dirname) // "/point.dat", status="replace", action="write")
-                    write(funit, '("# point")')
+                    write(funit, '("# Time (a.u.)              ' // detectors(n)%f_ch // ' (a.u.),      &
+                        &x = ", F10.5, " nm,   y = ", F10.5, " nm,   z = ", F10.5, " nm")') x_min, y_min, z_min
                     close(funit)
                 end if
             end do

@@ -160,6 +160,26 @@ def parse_series(d, case, nrec):
     return series
 
 
+def detector_headers(d, case):
+    """The header lines the reference wrote for each detector: point.dat's
+    first line, or the two header lines of the first and of the last print
+    step's file (text as written, NUL bytes included)."""
+    out = []
+    for n in range(1, len(case["detectors"]) + 1):
+        dd = os.path.join(d, f"output_detector_{n:07d}")
+        if os.path.exists(os.path.join(dd, "point.dat")):
+            with open(os.path.join(dd, "point.dat"), encoding="latin-1") as f:
+                out.append([f.readline()])
+            continue
+        files = sorted(glob.glob(os.path.join(dd, "*.dat")), key=lambda p: int(re.findall(r"_(\d+)\.dat$", p)[0]))
+        hdr = []
+        for p in (files[0], files[-1]):
+            with open(p, encoding="latin-1") as f:
+                hdr.append(f.readline() + f.readline())
+        out.append(hdr)
+    return out
+
+
 def run_case(name, case):
     d = tempfile.mkdtemp(prefix=f"omxgold_{name}_")
     try:
@@ -188,6 +208,7 @@ def run_case(name, case):
             moves = np.fromfile(os.path.join(d, "qmoves.bin"), dtype=np.int32)
             out["qsites_e"] = e.reshape((moves.size, 3, len(case["qsites"]))).transpose(0, 2, 1)
             out["qmoves"] = moves
+        out["headers_json"] = np.array(json.dumps(detector_headers(d, case)))
         desc = {k: v for k, v in case.items() if k != "media"}
         desc["media"] = [[h, p] for (h, p, _) in case["media"]]
         out["case_json"] = np.array(json.dumps(desc))

@@ -1,6 +1,11 @@
-"""The Fortran host layer (TMxll_3D_hip through iso_c_binding) driven by the
-reference's own input files, against the golden vectors of the reference's
-Fortran modules."""
+"""The Fortran host layer driven by the reference's own input files: the
+solver made by maxwell_factory(3) (a TMxll_3D_hip, reached through
+`class is (TMxll_3D_hip)`), the reference's step order, and its detector
+files, against the golden vectors of the reference's Fortran modules: the
+series at 1e-12 and the file headers byte for byte.  Rows come from the
+engine's detector samples, or (OMX_FDTD_DET_MIRROR=1) from the TMxll_3D host
+mirrors synced at every print step and read with the reference's stencils."""
+import json
 import os
 import subprocess
 import sys
@@ -21,8 +26,9 @@ AU_TO_FS = 2.4188843265864e-2
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("rows", ["samples", "mirror"])
 @pytest.mark.parametrize("name", ["media_mix", "au_al_periodic", "vacuum64"])
-def test_fortran_driver_matches_reference(name):
+def test_fortran_driver_matches_reference(name, rows):
     assert os.path.exists(EXE), "omx_fdtd_3d not built"
     case = make_golden.golden_cases()[name]
     desc, fx = _golden.load(name)
@@ -41,14 +47,16 @@ def test_fortran_driver_matches_reference(name):
             f.write(" mxll_dr = %r\n" % case["dx"])
             f.write(" mxll_total_time = %r\n" % ((case["steps"] + 0.5) * dt * AU_TO_FS))
             f.write(" mxll_dt_det_print = %r\n/\n" % ((case["record_every"] + 0.5) * dt * AU_TO_FS))
-        print(f"[fortran] {name} in {d}", flush=True)
-        env = dict(os.environ, OMX_FDTD_PRECISION="8")
+        print(f"[fortran] {name} ({rows}) in {d}", flush=True)
+        env = dict(os.environ, OMX_FDTD_PRECISION="8", OMX_FDTD_DET_MIRROR="1" if rows == "mirror" else "0")
         p = subprocess.run([EXE], cwd=d, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
                            env=env, timeout=300)
         print(p.stdout[-2000:])
         assert p.returncode == 0
         nrec = fx["series"].shape[0]
         series = make_golden.parse_series(d, case, nrec)
+        headers = make_golden.detector_headers(d, case)
     r = np.linalg.norm(series - fx["series"]) / np.linalg.norm(fx["series"])
-    print(f"{name}: Fortran driver (FP64 GPU) vs reference relL2 = {r:.3e}")
+    print(f"{name} ({rows}): Fortran driver (FP64 GPU) vs reference relL2 = {r:.3e}")
     assert r <= 1e-12
+    assert headers == json.loads(str(fx["headers_json"]))
