@@ -46,6 +46,10 @@ extern "C" {
 #define OMX_MEDIUM_DRUDE       2
 #define OMX_MEDIUM_DL          3
 
+/* q-group types (constants_mod.f90:74-75) */
+#define OMX_Q_MATERIAL  1
+#define OMX_Q_SINGLE    2
+
 #define OMX_DET_POINT     1
 #define OMX_DET_LINE_X    2
 #define OMX_DET_LINE_Y    3
@@ -180,12 +184,26 @@ int omx3d_record(omx3d_ctx* ctx);
 int omx3d_read_records(omx3d_ctx* ctx, double* out, int32_t* rows);
 int omx3d_sample(omx3d_ctx* ctx, double* out);
 
-/* ---- embedded quantum systems, "material" groups ------------------------ */
-/* Sites: map(n,3:5) of q_group_mod.f90:505-509 as an (n,3) Fortran array.
+/* ---- embedded quantum systems (q-groups) -------------------------------- */
+/* n quantum systems in n_groups q-groups (TQ_group, q_group_mod.f90), the
+ * systems listed in group order (group_of[q], 0-based, non-decreasing; may be
+ * null for one group).  ijk: the cell of each system as an (n,3) Fortran
+ * array -- map(n,3:5) of a "material" group (q_group_mod.f90:505-509), the
+ * kernel centre kernel_map(n,0,0,0,3:5) of a "single" one (:717-734).  E is
+ * gathered at that cell (interactions_mod.f90:654-691).  J: a material
+ * system writes density*dPt/dt into its cell, a single one
+ * density*w*dPt/dt into the 7^3 cells around it with the unnormalised
+ * Blackman-Harris weights w of q_group_mod.f90:216-239; the last writer of a
+ * cell wins and, as the reference copies J_old <- J at every group's
+ * scatter, only the last group's writes leave a dJ
+ * (interactions_mod.f90:1046-1218).  Every kernel cell must lie in the grid.
  * dt_q (au): the requested quantum step.  As the driver loop does
  * (mxim_mxll.f90:73-79), the systems move every n_skip = NINT(dt_q/dt) steps
  * when dt_q > dt (else every step) and are propagated over n_skip*dt;
  * omx3d_run sets t_skip at each new quantum step (mxim_mxll.f90:97-104). */
+int omx3d_set_q_groups(omx3d_ctx* ctx, int32_t n_groups, const int32_t* group_type, int32_t n,
+                       const int32_t* group_of, const int32_t* ijk, double density, double dt_q);
+/* One "material" group of n systems. */
 int omx3d_set_q_sites(omx3d_ctx* ctx, int32_t n, const int32_t* ijk, double density, double dt_q);
 int omx3d_gather_e(omx3d_ctx* ctx, double* e_sites);              /* (n,3) */
 int omx3d_scatter_j(omx3d_ctx* ctx, const double* dpdt_sites);    /* (n,3) */

@@ -611,8 +611,8 @@ int launch_two_level(omx3d_ctx* c) {
                        c->d_q_off, c->nq, c->d_q_e);
     hipLaunchKernelGGL(k_tls_step, dim3(blocks_for(c->nq, 256)), dim3(256), 0, c->s, c->d_tls, c->d_q_e, c->nq,
                        c->q_dt, c->d_q_dpdt);
-    hipLaunchKernelGGL(k_scatter_j, dim3(blocks_for(c->nslot, 256)), dim3(256), 0, c->s, c->d_q_winner, c->nslot,
-                       c->d_q_dpdt, c->q_density, c->q_dt, c->d_J);
+    hipLaunchKernelGGL(k_scatter_j, dim3(blocks_for(c->nslot, 256)), dim3(256), 0, c->s, c->d_q_slot, c->nslot,
+                       c->d_q_dpdt, c->q_dt, c->d_J);
     HIPCHK(hipGetLastError());
     return OMX_OK;
 }
@@ -1684,7 +1684,7 @@ int omx3d_destroy(omx3d_ctx* c) {
     }
     for (auto& s : c->sources) { dfreet(s.d_off); dfreet(s.d_ka); dfreet(s.d_kb); }
     dfreet(c->d_det_field); dfreet(c->d_det_off); dfreet(c->d_rec);
-    dfreet(c->d_q_off); dfreet(c->d_q_winner); dfreet(c->d_J); dfreet(c->d_q_e); dfreet(c->d_q_dpdt); dfreet(c->d_tls);
+    dfreet(c->d_q_off); dfreet(c->d_q_slot); dfreet(c->d_J); dfreet(c->d_q_e); dfreet(c->d_q_dpdt); dfreet(c->d_tls);
     dfreet(c->d_tf); dfreet(c->d_red);
     for (auto e : c->ev) (void)hipEventDestroy(e);
     if (c->s) (void)hipStreamDestroy(c->s);
@@ -1957,40 +1957,105 @@ int omx3d_sample(omx3d_ctx* c, double* out) {
     return rc;
 }
 
-int omx3d_set_q_sites(omx3d_ctx* c, int32_t n, const int32_t* ijk, double density, double dt_q) {
+namespace {
+
+// The 7^3 Blackman-Harris weights of a "single" q-group, unnormalised, in
+// cell units: r_max = 2*3+4 cells, r0 = r_max/2, negative values set to 0
+// (q_group_mod.f90:216-239; the norm it sums is never applied).  w[(k+3)*49
+// + (j+3)*7 + (i+3)].
+void single_kernel(double w[343]) {
+    const double a[4] = {0.353222222, -0.488, 0.145, -0.010222222};
+    const double r_max = 10.0, r0 = r_max / 2.0;
+    for (int k = -3; k <= 3; ++k)
+        for (int j = -3; j <= 3; ++j)
+            for (int i = -3; i <= 3; ++i) {
+                const double x = i, y = j, z = k;
+                const double r = std::sqrt(x * x + y * y + z * z);
+                double v = a[0] + a[1] * std::cos(2.0 * kPi * (r - r0) / r_max) +
+                           a[2] * std::cos(2.0 * kPi * 2.0 * (r - r0) / r_max) +
+                           a[3] * std::cos(2.0 * kPi * 3.0 * (r - r0) / r_max);
+                if (v < 0.0) v = 0.0;
+                w[(k + 3) * 49 + (j + 3) * 7 + (i + 3)] = v;
+            }
+}
+
+}  // namespace
+
+int omx3d_set_q_groups(omx3d_ctx* c, int32_t n_groups, const int32_t* group_type, int32_t n, const int32_t* group_of,
+                       const int32_t* ijk, double density, double dt_q) {
     CHECK_CTX(c);
     if (c->nq) return fail(OMX_ERR_STATE, "quantum sites already set");
     if (n <= 0 || !ijk) return fail(OMX_ERR_ARG, "bad site list");
+    if (n_groups <= 0 || !group_type || (n_groups > 1 && !group_of)) return fail(OMX_ERR_ARG, "bad group list");
     if (!(dt_q > 0.0)) return fail(OMX_ERR_ARG, "dt_q must be positive");
+    for (int g = 0; g < n_groups; ++g)
+        if (group_type[g] != OMX_Q_MATERIAL && group_type[g] != OMX_Q_SINGLE)
+            return fail(OMX_ERR_ARG, "q-group type must be OMX_Q_MATERIAL or OMX_Q_SINGLE");
     const int nn[3] = {c->d.nx, c->d.ny, c->d.nz};
-    std::map<std::tuple<int, int, int>, int> slot_of;
-    std::vector<std::tuple<int, int, int>> slot_cell;
-    std::vector<int> winner;
+    double wk[343];
+    single_kernel(wk);
+    // the cells each system writes, in the reference's order (systems in
+    // group order; a kernel k, then j, then i fastest)
+    struct Entry { int cell[3]; int sys, grp; double coef; };
+    std::vector<Entry> ent;
     std::vector<long> q_off(n);
+    int last_g = -1;
     for (int q = 0; q < n; ++q) {
+        const int g = n_groups > 1 ? group_of[q] : 0;
+        if (g < 0 || g >= n_groups || g < last_g) return fail(OMX_ERR_ARG, "systems must be listed in group order");
+        last_g = g;
         const int v[3] = {ijk[q], ijk[q + n], ijk[q + 2 * n]};
         for (int a = 0; a < 3; ++a)
             if (v[a] < 1 || v[a] > nn[a]) return fail(OMX_ERR_RANGE, "site index outside the grid");
-        auto key = std::make_tuple(v[0], v[1], v[2]);
-        auto it = slot_of.find(key);
-        if (it == slot_of.end()) {
-            slot_of[key] = (int)slot_cell.size();
-            slot_cell.push_back(key);
-            winner.push_back(q);
-        } else {
-            winner[it->second] = q;
-        }
         q_off[q] = c->L.at(v[0], v[1], v[2]);
+        if (group_type[g] == OMX_Q_MATERIAL) {
+            ent.push_back({{v[0], v[1], v[2]}, q, g, density});
+            continue;
+        }
+        for (int k = -3; k <= 3; ++k)
+            for (int j = -3; j <= 3; ++j)
+                for (int i = -3; i <= 3; ++i) {
+                    const int u[3] = {v[0] + i, v[1] + j, v[2] + k};
+                    for (int a = 0; a < 3; ++a)
+                        if (u[a] < 1 || u[a] > nn[a])
+                            return fail(OMX_ERR_RANGE, "a single q-group's 7^3 kernel reaches outside the grid");
+                    ent.push_back({{u[0], u[1], u[2]}, q, g, density * wk[(k + 3) * 49 + (j + 3) * 7 + (i + 3)]});
+                }
     }
-    const int ns = (int)slot_cell.size();
+    // the last group's scatter copies J_old <- J over the whole grid, so only
+    // its own writes leave a dJ (an empty last group leaves none)
+    const int G = n_groups - 1;
+    std::map<std::tuple<int, int, int>, int> slot_of;
+    std::vector<std::tuple<int, int, int>> slot_cell;
+    std::vector<QSlot> slot;
+    for (const Entry& e : ent) {
+        auto key = std::make_tuple(e.cell[0], e.cell[1], e.cell[2]);
+        auto it = slot_of.find(key);
+        int s;
+        if (it == slot_of.end()) {
+            s = (int)slot.size();
+            slot_of[key] = s;
+            slot_cell.push_back(key);
+            slot.push_back({-1, -1, 1, 0, 0.0, 0.0});
+        } else {
+            s = it->second;
+        }
+        QSlot& S = slot[s];
+        if (e.grp < G) {        // an earlier group: the value the last group's J_old will see
+            S.prev = e.sys; S.coef_prev = e.coef;
+        }
+        S.sys = e.sys; S.coef = e.coef;
+        S.mode = e.grp == G ? 0 : 1;
+    }
+    const int ns = (int)slot.size();
     int rc = dalloc(c, (void**)&c->d_q_off, sizeof(long) * n);
-    if (!rc) rc = dalloc(c, (void**)&c->d_q_winner, sizeof(int) * ns);
+    if (!rc) rc = dalloc(c, (void**)&c->d_q_slot, sizeof(QSlot) * ns);
     if (!rc) rc = dalloc(c, (void**)&c->d_J, sizeof(double) * 9 * ns);
     if (!rc) rc = dalloc(c, (void**)&c->d_q_e, sizeof(double) * 3 * n);
     if (!rc) rc = dalloc(c, (void**)&c->d_q_dpdt, sizeof(double) * 3 * n);
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(c->d_q_off, q_off.data(), sizeof(long) * n, hipMemcpyHostToDevice, c->s));
-    HIPCHK(hipMemcpyAsync(c->d_q_winner, winner.data(), sizeof(int) * ns, hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(c->d_q_slot, slot.data(), sizeof(QSlot) * ns, hipMemcpyHostToDevice, c->s));
     // cells whose E component sees J: the site cell (term i) and the cell one
     // below along the component axis (term i+1)
     for (int comp = 0; comp < 3; ++comp) {
@@ -2032,6 +2097,7 @@ int omx3d_set_q_sites(omx3d_ctx* c, int32_t n, const int32_t* ijk, double densit
     c->nq = n;
     c->nslot = ns;
     c->q_density = density;
+    c->q_groups = n_groups;
     // mxim_mxll.f90:73-79: n_skip_steps = NINT(dt_q/dt) when dt_q > dt, else
     // 1; the quantum step is then n_skip_steps * dt
     c->n_skip = dt_q > c->d.dt ? (int)std::lround(dt_q / c->d.dt) : 1;
@@ -2039,6 +2105,11 @@ int omx3d_set_q_sites(omx3d_ctx* c, int32_t n, const int32_t* ijk, double densit
     c->tq_step_old = 0;
     c->patches_built = false;
     return OMX_OK;
+}
+
+int omx3d_set_q_sites(omx3d_ctx* c, int32_t n, const int32_t* ijk, double density, double dt_q) {
+    const int32_t material = OMX_Q_MATERIAL;
+    return omx3d_set_q_groups(c, 1, &material, n, nullptr, ijk, density, dt_q);
 }
 
 int omx3d_gather_e(omx3d_ctx* c, double* e) {
@@ -2068,8 +2139,8 @@ int omx3d_scatter_j(omx3d_ctx* c, const double* dpdt) {
     for (int q = 0; q < c->nq; ++q)
         for (int a = 0; a < 3; ++a) h[3 * q + a] = dpdt[q + (long)c->nq * a];
     HIPCHK(hipMemcpyAsync(c->d_q_dpdt, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->s));
-    hipLaunchKernelGGL(k_scatter_j, dim3(blocks_for(c->nslot, 256)), dim3(256), 0, c->s, c->d_q_winner, c->nslot,
-                       c->d_q_dpdt, c->q_density, c->q_dt, c->d_J);
+    hipLaunchKernelGGL(k_scatter_j, dim3(blocks_for(c->nslot, 256)), dim3(256), 0, c->s, c->d_q_slot, c->nslot,
+                       c->d_q_dpdt, c->q_dt, c->d_J);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->s));
     return OMX_OK;

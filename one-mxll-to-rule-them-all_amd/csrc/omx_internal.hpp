@@ -98,6 +98,17 @@ struct SourceDev {
     double** d_sh;                   // device copy of h_sh (null when no point has one)
 };
 
+// One cell that quantum systems write J into.  sys / coef: the last writer
+// of the step (J = coef * dPt/dt of system sys, coef = density * kernel
+// weight); mode 0: the last q-group writes the cell, and J_old is what the
+// groups before it left there (prev / coef_prev: their last writer, -1: the
+// cell's J of the previous quantum step); mode 1: only earlier groups write
+// it, so the last group's J_old <- J copy leaves J_old = J and dJ = 0.
+struct QSlot {
+    int sys, prev, mode, pad;
+    double coef, coef_prev;
+};
+
 // A near-field patch box of an FP32 grid (kernels: k_patch).
 struct PatchBox {
     int lo[3], w[3];            // first cell (1-based) and extent per axis
@@ -193,7 +204,7 @@ struct omx3d_ctx {
     std::vector<long> h_det_off;
 
     // quantum sites ("material" group)
-    int nq = 0, nslot = 0;
+    int nq = 0, nslot = 0, q_groups = 0;
     double q_density = 0.0, q_dt = 0.0;
     // quantum sub-stepping of the driver loop (mxim_mxll.f90:73-104): the
     // systems move every n_skip steps, q_dt = n_skip * dt; tq_step_old and
@@ -201,7 +212,7 @@ struct omx3d_ctx {
     int n_skip = 1, tq_step_old = 0;
     double t_skip = 0.0;
     long* d_q_off = nullptr;        // per site: cell offset
-    int* d_q_winner = nullptr;      // per slot: last site mapping to it
+    omx::QSlot* d_q_slot = nullptr;      // per slot: its writers (k_scatter_j)
     double* d_J = nullptr;          // per slot: Jx Jy Jz Jold(3) dJ(3) -> 9*nslot
     double* d_q_e = nullptr;        // per site E (3)
     double* d_q_dpdt = nullptr;     // per site dPt/dt (3)

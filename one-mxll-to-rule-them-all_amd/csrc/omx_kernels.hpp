@@ -1441,21 +1441,28 @@ __global__ void k_gather_e(const YeeArgs<T> a, const long* __restrict__ off, int
     e[3 * q + 2] = 0.5 * ((double)a.ez[o - 1] + (double)a.ez[o]);
 }
 
-// J_old <- J; J(site) = density * dPt/dt; dJ = (J - J_old)/dt_q, restricted
-// to the cells that hold a site (send_J_to_E_3D_ranks,
-// interactions_mod.f90:1067-1069,1184-1190,1216-1218).  The last site mapped
-// to a cell wins, as in the reference's sequential loop.
-__global__ void k_scatter_j(const int* __restrict__ winner, int nslot, const double* __restrict__ dpdt,
-                            double density, double dt_q, double* __restrict__ J) {
+// J_old <- J; J(cell) = coef * dPt/dt; dJ = (J - J_old)/dt_q on the cells
+// the q-groups write, group after group (send_J_to_E_3D_ranks,
+// interactions_mod.f90:1067-1069,1179-1210,1216-1218): the last writer of a
+// cell wins, as in the reference's sequential loops, and with several
+// groups only the last group's change survives in dJ.
+__global__ void k_scatter_j(const QSlot* __restrict__ slots, int nslot, const double* __restrict__ dpdt,
+                            double dt_q, double* __restrict__ J) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nslot) return;
-    const int w = winner[s];
+    const QSlot q = slots[s];
     for (int c = 0; c < 3; ++c) {
-        const double old = J[9 * s + c];
-        const double nw = density * dpdt[3 * w + c];
-        J[9 * s + 3 + c] = old;
-        J[9 * s + c] = nw;
-        J[9 * s + 6 + c] = (nw - old) / dt_q;
+        const double nw = q.coef * dpdt[3 * q.sys + c];
+        if (q.mode == 1) {
+            J[9 * s + 3 + c] = nw;
+            J[9 * s + c] = nw;
+            J[9 * s + 6 + c] = 0.0;
+        } else {
+            const double old = q.prev >= 0 ? q.coef_prev * dpdt[3 * q.prev + c] : J[9 * s + c];
+            J[9 * s + 3 + c] = old;
+            J[9 * s + c] = nw;
+            J[9 * s + 6 + c] = (nw - old) / dt_q;
+        }
     }
 }
 

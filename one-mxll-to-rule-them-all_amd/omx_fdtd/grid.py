@@ -238,6 +238,18 @@ class Grid3D:
         L.call("omx3d_set_q_sites", self.h, idx.shape[0], L.iptr(flat), float(density), float(dt_q))
         self.n_sites = idx.shape[0]
 
+    def set_q_groups(self, group_types, group_of, ijk, density, dt_q):
+        """Quantum systems in q-groups (omx3d_set_q_groups): group_types 1
+        (material) or 2 (single) per group, group_of each system's group
+        (0-based, systems in group order), ijk (n,3) cells."""
+        idx = np.asarray(ijk, dtype=np.int32).reshape(-1, 3)
+        flat = L.i32(idx.ravel(order="F"))
+        gt = L.i32(group_types)
+        gof = L.i32(group_of)
+        L.call("omx3d_set_q_groups", self.h, gt.size, L.iptr(gt), idx.shape[0], L.iptr(gof), L.iptr(flat),
+               float(density), float(dt_q))
+        self.n_sites = idx.shape[0]
+
     def gather_E(self):
         e = np.zeros(3 * self.n_sites)
         L.call("omx3d_gather_e", self.h, L.dptr(e))

@@ -15,7 +15,8 @@
  *   staggered media map                         golden vectors of the reference build
  *   H/E updates, CPML psi, media ADE, sources   golden detector time series of the reference build
  *   detector stencils                           golden detector time series of the reference build
- *   E gather / J scatter (material groups)      golden vectors of the reference build (prescribed dP/dt)
+ *   E gather / J scatter (material and single   golden vectors of the reference build (prescribed dP/dt,
+ *   groups, several groups)                     q-groups restated in its test driver)
  *   two-level emitter RK4 (no reference code)   closed-form Rabi / free-induction decay in tests
  *   TF/SF plane wave (no reference code)        vacuum leakage and transmitted-amplitude checks in tests
  */
@@ -98,6 +99,7 @@ typedef struct orc_sim {
     long n_samples;
     /* quantum sites */
     int nq; int *q_ijk; double q_density, q_dt;
+    int q_ngroups; int *q_gtype, *q_gof; double q_w[343];   /* q-groups; single-group kernel weights */
     int n_skip, tq_step_old; double t_skip;   /* quantum sub-stepping (mxim_mxll.f90:73-104) */
     double *tls;             /* 16 per site, same packing as the engine */
     /* tf/sf */
@@ -229,7 +231,7 @@ void orc_destroy(orc_sim* s) {
     free(s->PLx); free(s->PLy); free(s->PLz); free(s->PLx_old); free(s->PLy_old); free(s->PLz_old);
     for (int q = 0; q < s->n_src; ++q) { free(s->src[q].ind); free(s->src[q].ker); free(s->src[q].jmat); }
     for (int q = 0; q < s->n_det; ++q) free(s->det[q].idx);
-    free(s->q_ijk); free(s->tls); free(s->ex1); free(s->hy1);
+    free(s->q_ijk); free(s->q_gtype); free(s->q_gof); free(s->tls); free(s->ex1); free(s->hy1);
     free(s);
 }
 
@@ -761,12 +763,38 @@ void orc_sample(const orc_sim* s, double* out) {
 }
 
 /* ------------------------------------------------------------------------ */
-/* quantum sites, "material" groups, non-MPI branches */
-void orc_set_q_sites(orc_sim* s, int n, const int* ijk, double density, double dt_q) {
+/* quantum systems in q-groups, non-MPI branches.  group_type: 1 material,
+ * 2 single; group_of: each system's group (0-based, systems in group order;
+ * NULL for one group); ijk: the system's cell, an (n,3) Fortran array. */
+void orc_set_q_groups(orc_sim* s, int n_groups, const int* group_type, int n, const int* group_of, const int* ijk,
+                      double density, double dt_q) {
     const int nx = s->nx, ny = s->ny, nz = s->nz;
     s->nq = n;
     s->q_ijk = (int*)malloc(sizeof(int) * 3 * n);
     memcpy(s->q_ijk, ijk, sizeof(int) * 3 * n);
+    s->q_ngroups = n_groups;
+    s->q_gtype = (int*)malloc(sizeof(int) * n_groups);
+    memcpy(s->q_gtype, group_type, sizeof(int) * n_groups);
+    s->q_gof = (int*)calloc((size_t)n, sizeof(int));
+    if (group_of) memcpy(s->q_gof, group_of, sizeof(int) * n);
+    /* "single" kernel, q_group_mod.f90:216-239 (3D): n_ker = 3, r_max =
+     * dble(n_ker*2+4), r0 = r_max/2, Blackman-Harris in r = |(i,j,k)|,
+     * clipped at 0; the norm it sums is never applied */
+    {
+        const double aBH[4] = {0.353222222, -0.488, 0.145, -0.010222222};
+        const double pi0 = 3.141592653589793, r_max = (double)(3 * 2 + 4), r0 = r_max / 2.0;
+        for (int i = -3; i <= 3; ++i)
+            for (int j = -3; j <= 3; ++j)
+                for (int k = -3; k <= 3; ++k) {
+                    const double x = (double)i, y = (double)j, z = (double)k;
+                    const double r = sqrt(x * x + y * y + z * z);
+                    double v = aBH[0] + aBH[1] * cos(2.0 * pi0 * (r - r0) / r_max) +
+                               aBH[2] * cos(2.0 * pi0 * 2.0 * (r - r0) / r_max) +
+                               aBH[3] * cos(2.0 * pi0 * 3.0 * (r - r0) / r_max);
+                    if (v < 0.0) v = 0.0;
+                    s->q_w[(k + 3) * 49 + (j + 3) * 7 + (i + 3)] = v;
+                }
+    }
     /* mxim_mxll.f90:73-79: n_skip_steps = NINT(dt_q/dt) when dt_q > dt, else 1,
      * and the quantum step becomes n_skip_steps*dt */
     s->n_skip = dt_q > s->dt ? (int)lround(dt_q / s->dt) : 1;
@@ -778,7 +806,13 @@ void orc_set_q_sites(orc_sim* s, int n, const int* ijk, double density, double d
     s->have_j = 1;
 }
 
-/* send_E_3D_to_J_ranks, non-MPI material branch (interactions_mod.f90:656-672) */
+void orc_set_q_sites(orc_sim* s, int n, const int* ijk, double density, double dt_q) {
+    const int material = 1;
+    orc_set_q_groups(s, 1, &material, n, NULL, ijk, density, dt_q);
+}
+
+/* send_E_3D_to_J_ranks, non-MPI branches (interactions_mod.f90:656-691): at
+ * the system's cell (a single group's kernel centre) */
 void orc_gather_e(const orc_sim* s, double* e) {
     const int n = s->nq;
     for (int q = 0; q < n; ++q) {
@@ -789,21 +823,36 @@ void orc_gather_e(const orc_sim* s, double* e) {
     }
 }
 
-/* send_J_to_E_3D_ranks, non-MPI material branch (interactions_mod.f90:1067-1069,1184-1190,1216-1218) */
+/* send_J_to_E_3D_ranks, non-MPI branches, once per group in group order
+ * (interactions_mod.f90:1067-1069,1177-1218, called from mxim_mxll.f90:117-121) */
 void orc_scatter_j(orc_sim* s, const double* dpdt) {
     const int n = s->nq;
-    memcpy(s->Jx_old.p, s->Jx.p, sizeof(double) * s->Jx.n);
-    memcpy(s->Jy_old.p, s->Jy.p, sizeof(double) * s->Jy.n);
-    memcpy(s->Jz_old.p, s->Jz.p, sizeof(double) * s->Jz.n);
-    for (int q = 0; q < n; ++q) {
-        const int i = s->q_ijk[q], j = s->q_ijk[q + n], k = s->q_ijk[q + 2 * n];
-        AT(s->Jx, i, j, k) = s->q_density * dpdt[q];
-        AT(s->Jy, i, j, k) = s->q_density * dpdt[q + n];
-        AT(s->Jz, i, j, k) = s->q_density * dpdt[q + 2 * n];
+    for (int g = 0; g < s->q_ngroups; ++g) {
+        memcpy(s->Jx_old.p, s->Jx.p, sizeof(double) * s->Jx.n);
+        memcpy(s->Jy_old.p, s->Jy.p, sizeof(double) * s->Jy.n);
+        memcpy(s->Jz_old.p, s->Jz.p, sizeof(double) * s->Jz.n);
+        for (int q = 0; q < n; ++q) {
+            if (s->q_gof[q] != g) continue;
+            const int i = s->q_ijk[q], j = s->q_ijk[q + n], k = s->q_ijk[q + 2 * n];
+            if (s->q_gtype[g] == 1) {
+                AT(s->Jx, i, j, k) = s->q_density * dpdt[q];
+                AT(s->Jy, i, j, k) = s->q_density * dpdt[q + n];
+                AT(s->Jz, i, j, k) = s->q_density * dpdt[q + 2 * n];
+                continue;
+            }
+            for (int kk = -3; kk <= 3; ++kk)
+                for (int jj = -3; jj <= 3; ++jj)
+                    for (int ii = -3; ii <= 3; ++ii) {
+                        const double w = s->q_w[(kk + 3) * 49 + (jj + 3) * 7 + (ii + 3)];
+                        AT(s->Jx, i + ii, j + jj, k + kk) = s->q_density * w * dpdt[q];
+                        AT(s->Jy, i + ii, j + jj, k + kk) = s->q_density * w * dpdt[q + n];
+                        AT(s->Jz, i + ii, j + jj, k + kk) = s->q_density * w * dpdt[q + 2 * n];
+                    }
+        }
+        for (long p = 0; p < s->Jx.n; ++p) s->dJx.p[p] = (s->Jx.p[p] - s->Jx_old.p[p]) / s->q_dt;
+        for (long p = 0; p < s->Jy.n; ++p) s->dJy.p[p] = (s->Jy.p[p] - s->Jy_old.p[p]) / s->q_dt;
+        for (long p = 0; p < s->Jz.n; ++p) s->dJz.p[p] = (s->Jz.p[p] - s->Jz_old.p[p]) / s->q_dt;
     }
-    for (long p = 0; p < s->Jx.n; ++p) s->dJx.p[p] = (s->Jx.p[p] - s->Jx_old.p[p]) / s->q_dt;
-    for (long p = 0; p < s->Jy.n; ++p) s->dJy.p[p] = (s->Jy.p[p] - s->Jy_old.p[p]) / s->q_dt;
-    for (long p = 0; p < s->Jz.n; ++p) s->dJz.p[p] = (s->Jz.p[p] - s->Jz_old.p[p]) / s->q_dt;
 }
 
 /* Two-level emitter (no reference counterpart; TQ_sys_base contract,

@@ -50,6 +50,7 @@ def load():
             "orc_add_detector": ([P, I, I, I, _IP], I),
             "orc_sample": ([P, _DP], V),
             "orc_set_q_sites": ([P, I, _IP, D, D], V),
+            "orc_set_q_groups": ([P, I, _IP, I, _IP, _IP, D, D], V),
             "orc_gather_e": ([P, _DP], V),
             "orc_scatter_j": ([P, _DP], V),
             "orc_set_two_level": ([P, _DP, _DP, _DP, _DP], V),
@@ -204,6 +205,17 @@ class OracleGrid:
         idx = np.asarray(ijk, dtype=np.int32).reshape(-1, 3)
         flat = np.ascontiguousarray(idx.ravel(order="F"))
         self.lib.orc_set_q_sites(self.h, idx.shape[0], _i(flat), float(density), float(dt_q))
+        self.n_sites = idx.shape[0]
+
+    def set_q_groups(self, group_types, group_of, ijk, density, dt_q):
+        """group_types: 1 material / 2 single per group; group_of: each
+        system's group (0-based, in group order); ijk: (n,3) cells."""
+        idx = np.asarray(ijk, dtype=np.int32).reshape(-1, 3)
+        flat = np.ascontiguousarray(idx.ravel(order="F"))
+        gt = np.ascontiguousarray(group_types, dtype=np.int32)
+        gof = np.ascontiguousarray(group_of, dtype=np.int32)
+        self.lib.orc_set_q_groups(self.h, gt.size, _i(gt), idx.shape[0], _i(gof), _i(flat), float(density),
+                                  float(dt_q))
         self.n_sites = idx.shape[0]
 
     def gather_E(self):

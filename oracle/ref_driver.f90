@@ -20,7 +20,10 @@
 !
 ! Input (current directory): case.nml (namelist &case), plus the reference's
 ! own medium_XXXX.in, sources.in and detectors.in when used, and qsites.in
-! ("i j k a_x a_y a_z" per site: dPt/dt(c, tt) = a_c*sin(0.3*tt + c - 1)).
+! ("i j k a_x a_y a_z" per site, plus its 1-based q-group when n_qg > 1:
+! dPt/dt(c, tt) = a_c*sin(0.3*tt + c - 1)); q-group g is "material"
+! (qg_type(g) = 1) or "single" (2: J over the 7^3 kernel of
+! q_group_mod.f90:216-239 around the site).
 ! Output: coeffs.bin, media.bin, qsites_e.bin (stream doubles) and the
 ! reference's output_detector_XXXXXXX/ text files.
 program omx_ref_driver
@@ -37,11 +40,11 @@ program omx_ref_driver
 
     integer  :: nx = 16, ny = 16, nz = 16, npml = 4, bc(3) = 3
     integer  :: n_steps = 10, record_every = 1, n_media = 0, n_src = 0, n_det = 0, n_q = 0
-    integer  :: timing = 0, n_skip = 1
+    integer  :: timing = 0, n_skip = 1, n_qg = 1, qg_type(8) = 1
     real(dp) :: dx_nm = 5.0d0, q_density_nm3 = 1.0d-3
     real(dp) :: diel_eps(15) = 1.0d0
     namelist /case/ nx, ny, nz, npml, bc, n_steps, record_every, n_media, n_src, n_det, n_q, &
-                    timing, dx_nm, q_density_nm3, diel_eps, n_skip
+                    timing, dx_nm, q_density_nm3, diel_eps, n_skip, n_qg, qg_type
 
     type(TMxll_3D)               :: mxll
     type(TSource), allocatable   :: source_list(:)
@@ -54,7 +57,8 @@ program omx_ref_driver
     character(len=10) :: detector_type
     character(len=20) :: number
     character(len=99) :: dirname
-    integer,  allocatable :: q_ijk(:, :)
+    integer,  allocatable :: q_ijk(:, :), q_grp(:)
+    real(dp) :: q_ker(-3:3, -3:3, -3:3)
     real(dp), allocatable :: q_amp(:, :), e_sites(:, :), dpdt(:, :)
     integer(8) :: c0_clock, c1_clock, rate
     integer  :: tt_0
@@ -144,12 +148,18 @@ program omx_ref_driver
 
     ! ---- quantum sites -----------------------------------------------------
     if (n_q > 0) then
-        allocate(q_ijk(n_q, 3), q_amp(n_q, 3), e_sites(n_q, 3), dpdt(n_q, 3))
+        allocate(q_ijk(n_q, 3), q_amp(n_q, 3), e_sites(n_q, 3), dpdt(n_q, 3), q_grp(n_q))
+        q_grp = 1
         open(newunit=funit, file="qsites.in", action="read")
         do n = 1, n_q
-            read(funit, *) q_ijk(n, :), q_amp(n, :)
+            if (n_qg > 1) then
+                read(funit, *) q_ijk(n, :), q_amp(n, :), q_grp(n)
+            else
+                read(funit, *) q_ijk(n, :), q_amp(n, :)
+            end if
         end do
         close(funit)
+        call single_kernel()
         if (timing == 0) then
             open(unit=77, file="qsites_e.bin", access="stream", form="unformatted", status="replace")
             open(unit=78, file="qmoves.bin", access="stream", form="unformatted", status="replace")
@@ -247,7 +257,32 @@ contains
         end do
     end subroutine inject_sources
 
-    ! send_E_3D_to_J_ranks, material, non-MPI (interactions_mod.f90:656-672)
+    ! the 3D "single" group kernel, q_group_mod.f90:216-239 (n_ker = 3; its
+    ! norm is summed there but never applied)
+    subroutine single_kernel()
+        integer  :: i, j, k
+        real(dp) :: r_max, r0, x, y, z, r
+        r_max = dble(3*2+4)
+        r0    = r_max/2.0d0
+        do i = -3, 3
+        do j = -3, 3
+        do k = -3, 3
+            x = dble(i)
+            y = dble(j)
+            z = dble(k)
+            r = SQRT(x**2 + y**2 + z**2)
+            q_ker(i,j,k) = aBH(1)+ &
+                  aBH(2)*DCOS(2.0*pi0*(r-r0)/r_max)+ &
+                  aBH(3)*DCOS(2.0*pi0*2.0*(r-r0)/r_max)+ &
+                  aBH(4)*DCOS(2.0*pi0*3.0*(r-r0)/r_max)
+            if (q_ker(i,j,k) < M_ZERO) q_ker(i,j,k) = M_ZERO
+        end do
+        end do
+        end do
+    end subroutine single_kernel
+
+    ! send_E_3D_to_J_ranks, non-MPI (interactions_mod.f90:656-691): at the
+    ! system's cell, a single group's kernel centre
     subroutine gather_e()
         integer :: q, ii, jj, kk
         do q = 1, n_q
@@ -258,21 +293,37 @@ contains
         end do
     end subroutine gather_e
 
-    ! send_J_to_E_3D_ranks, material, non-MPI (interactions_mod.f90:1067-1069,1184-1190,1216-1218)
+    ! send_J_to_E_3D_ranks, non-MPI, once per q-group in group order
+    ! (interactions_mod.f90:1067-1069,1179-1218; mxim_mxll.f90:117-121)
     subroutine scatter_j()
-        integer :: q, ii, jj, kk
-        mxll%Jx_old = mxll%Jx
-        mxll%Jy_old = mxll%Jy
-        mxll%Jz_old = mxll%Jz
-        do q = 1, n_q
-            ii = q_ijk(q, 1); jj = q_ijk(q, 2); kk = q_ijk(q, 3)
-            mxll%Jx(ii, jj, kk) = density * dpdt(q, 1)
-            mxll%Jy(ii, jj, kk) = density * dpdt(q, 2)
-            mxll%Jz(ii, jj, kk) = density * dpdt(q, 3)
+        integer :: q, g, ii, jj, kk, i, j, k
+        do g = 1, n_qg
+            mxll%Jx_old = mxll%Jx
+            mxll%Jy_old = mxll%Jy
+            mxll%Jz_old = mxll%Jz
+            do q = 1, n_q
+                if (q_grp(q) /= g) cycle
+                ii = q_ijk(q, 1); jj = q_ijk(q, 2); kk = q_ijk(q, 3)
+                if (qg_type(g) == 1) then
+                    mxll%Jx(ii, jj, kk) = density * dpdt(q, 1)
+                    mxll%Jy(ii, jj, kk) = density * dpdt(q, 2)
+                    mxll%Jz(ii, jj, kk) = density * dpdt(q, 3)
+                else
+                    do k = -3, 3
+                    do j = -3, 3
+                    do i = -3, 3
+                        mxll%Jx(ii + i, jj + j, kk + k) = density*q_ker(i,j,k)*dpdt(q, 1)
+                        mxll%Jy(ii + i, jj + j, kk + k) = density*q_ker(i,j,k)*dpdt(q, 2)
+                        mxll%Jz(ii + i, jj + j, kk + k) = density*q_ker(i,j,k)*dpdt(q, 3)
+                    end do
+                    end do
+                    end do
+                end if
+            end do
+            mxll%dJx = (mxll%Jx - mxll%Jx_old) / dt_q
+            mxll%dJy = (mxll%Jy - mxll%Jy_old) / dt_q
+            mxll%dJz = (mxll%Jz - mxll%Jz_old) / dt_q
         end do
-        mxll%dJx = (mxll%Jx - mxll%Jx_old) / dt_q
-        mxll%dJy = (mxll%Jy - mxll%Jy_old) / dt_q
-        mxll%dJz = (mxll%Jz - mxll%Jz_old) / dt_q
     end subroutine scatter_j
 
 end program omx_ref_driver

@@ -9,7 +9,7 @@ from omx_fdtd import cases, host
 from omx_fdtd.units import DETECTOR_CODES, FIELD_CODES, NM_TO_AU, index_to_nm, grid_dr_dt
 
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-NAMES = ("vacuum64", "media_mix", "au_al_periodic", "qsites", "qsites_skip3")
+NAMES = ("vacuum64", "media_mix", "au_al_periodic", "qsites", "qsites_skip3", "qgroups")
 
 
 def load(name):
@@ -79,9 +79,12 @@ def drive(g, desc, case):
         return g.run(1, case.steps, case.record_every), None
     q = np.array(desc["qsites"], dtype=float)
     ijk = q[:, :3].astype(np.int32)
-    amp = q[:, 3:]
+    amp = q[:, 3:6]
     n_skip = desc.get("n_skip", 1)
-    g.set_q_sites(ijk, 1.0e-3 / NM_TO_AU ** 3, n_skip * dt)
+    if desc.get("q_groups"):
+        g.set_q_groups(desc["q_groups"], q[:, 6].astype(np.int32) - 1, ijk, 1.0e-3 / NM_TO_AU ** 3, n_skip * dt)
+    else:
+        g.set_q_sites(ijk, 1.0e-3 / NM_TO_AU ** 3, n_skip * dt)
     moves = set(q_moves(n_skip, case.steps))
     rows, es = [], []
     for tt in range(1, case.steps + 1):

@@ -92,6 +92,25 @@ def golden_cases():
     # the same sites with the quantum systems moved every 3rd step
     # (n_skip_steps = 3: J interpolated over tau = time - t_skip in between)
     out["qsites_skip3"] = dict(out["qsites"], steps=61, n_skip=3)
+
+    # three q-groups (q_group_mod.f90): "single" (7^3 kernels, overlapping),
+    # "material" (one site inside the first group's kernels, one inside the
+    # third's), "single" again (overlapping both); qsites rows carry their
+    # 1-based group.  Cells written by the first groups only keep J_old = J,
+    # the last group's cells take J_old from the groups before it.
+    g = (24, 22, 20)
+    out["qgroups"] = dict(
+        grid=g, npml=4, bc=(3, 3, 3), dx=5.0, steps=60, record_every=1,
+        media=[("dielectric", "2.0", lambda I, J, K: K <= 3),
+               ("drude-lorentz", "Ag 1.0", lambda I, J, K: (I >= 17) & (I <= 19) & (J >= 13) & (J <= 15) & (K >= 13) & (K <= 15))],
+        sources=["z 1.0 1.549802 0.6 0.3 0.0 0.0 0.0 0.0 0.0 1.0e6 0.0"],
+        detectors=[("point", f, (12, 11, 10), (12, 11, 10)) for f in ("Ex", "Ey", "Ez", "Hx")] +
+                  [("point", f, (8, 14, 12), (8, 14, 12)) for f in ("Ex", "Ez")] +
+                  [("plane_xy", "Ez", (1, 1, 10), (24, 22, 10))],
+        q_groups=[2, 1, 2],
+        qsites=[(9, 9, 9, 1e-2, -2e-2, 5e-3, 1), (11, 10, 9, 3e-3, 1e-2, -1e-2, 1),
+                (14, 12, 10, -4e-3, 2e-3, 7e-3, 2), (10, 9, 10, 2e-2, 1e-3, -3e-3, 2), (16, 8, 12, 1e-2, 1e-2, 1e-2, 2),
+                (15, 12, 11, -1e-2, 5e-3, 2e-3, 3), (12, 11, 10, 6e-3, -4e-3, 1e-2, 3)])
     return out
 
 
@@ -138,6 +157,8 @@ def write_inputs(case, d):
         f.write(f" n_steps={case['steps']}, record_every={case['record_every']}, n_media={len(case['media'])},\n")
         f.write(f" n_src={len(case['sources'])}, n_det={len(case['detectors'])}, n_q={nq}, timing=0,\n")
         f.write(f" n_skip={case.get('n_skip', 1)},\n")
+        if case.get("q_groups"):
+            f.write(f" n_qg={len(case['q_groups'])}, qg_type=" + ",".join(str(t) for t in case["q_groups"]) + ",\n")
         f.write(f" dx_nm={dx!r}, diel_eps=" + ",".join(repr(v) for v in diel) + "\n/\n")
     return aux
 
