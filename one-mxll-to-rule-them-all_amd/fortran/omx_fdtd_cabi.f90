@@ -92,6 +92,28 @@ module omx_fdtd_cabi
             integer(c_int32_t), intent(in) :: ijk(*)
             real(c_double), value :: density, dt_q
         end function
+        integer(c_int) function omx3d_set_q_groups(ctx, n_groups, group_type, n, group_of, ijk, density, dt_q) &
+                bind(C, name="omx3d_set_q_groups")
+            import :: c_int, c_ptr, c_double, c_int32_t
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: n_groups, n
+            integer(c_int32_t), intent(in) :: group_type(*), group_of(*), ijk(*)
+            real(c_double), value :: density, dt_q
+        end function
+        integer(c_int) function omx3d_set_two_level(ctx, omega0, mu, pol, gamma2) bind(C, name="omx3d_set_two_level")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(in) :: omega0(*), mu(*), pol(*), gamma2(*)
+        end function
+        integer(c_int) function omx3d_two_level_step(ctx) bind(C, name="omx3d_two_level_step")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function omx3d_get_two_level(ctx, dipole, rho) bind(C, name="omx3d_get_two_level")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: dipole(*), rho(*)
+        end function
         integer(c_int) function omx3d_gather_e(ctx, e) bind(C, name="omx3d_gather_e")
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: ctx

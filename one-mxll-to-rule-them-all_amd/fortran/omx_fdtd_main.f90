@@ -4,10 +4,11 @@
 !   sources.in     one soft source per line (source_mod.f90:194-209)
 !   medium_XXXX.in read by TMxll_3D_hip%init
 !   detectors.in   "type field xmin xmax ymin ymax zmin zmax" [nm] (output_mod.f90:64)
+!   mol_group_XXXXXXX.in  quantum groups (omx_qgroup_io: two-level systems)
 ! The solver comes from maxwell_factory (factory_mod.f90:15-30) and the time
-! loop is that of mxim_mxll.f90:92-132, on the GPU.  Quantum groups (DFTB+)
-! are not part of this driver; a run that asks for them stops.
-! Output: the reference's output_detector_XXXXXXX/ files (omx_detector_io).
+! loop is that of mxim_mxll.f90:66-132, on the GPU, quantum sub-steps
+! included.  Output: the reference's output_detector_XXXXXXX/ files
+! (omx_detector_io) and output_q_group_XXXX/energy_and_dipole.dat.
 ! Detector rows come from the engine's samples; with OMX_FDTD_DET_MIRROR=1
 ! they come instead from the TMxll_3D host mirrors (sync_to_host at every
 ! print step), the way the reference's writers read its arrays.
@@ -20,6 +21,7 @@ program omx_fdtd_3d
     use omx_mxll_3d_hip_mod
     use omx_factory_mod
     use omx_detector_io_mod
+    use omx_qgroup_io_mod
     implicit none
 
     character(len=10) :: mxll_boundaries(3) = "none", mxll_2D_mode = "none"
@@ -35,8 +37,9 @@ program omx_fdtd_3d
 
     class(TMxll), allocatable :: mxll
     type(TDetectorOut), allocatable :: dets(:)
-    integer  :: grid(3), bc(3), Nt, t_det_print, i, funit, ierr, stat
-    real(dp) :: dr, dt
+    type(TQGroupIn), allocatable :: qgroups(:)
+    integer  :: grid(3), bc(3), Nt, t_det_print, t_q_print, i, funit, ierr, stat
+    real(dp) :: dr, dt, dt_q, dt_q_print, density
     character(len=8) :: env
     logical :: use_mirror
 
@@ -48,10 +51,6 @@ program omx_fdtd_3d
     read (funit, nml=OMxRTA, iostat=ierr)
     if (ierr /= 0) write (*, '("Error: invalid Namelist format")')
     close (funit)
-    if (mxll_n_q_groups > 0) then
-        write (*, '("Error: quantum groups need the DFTB+ host path; not available in omx_fdtd_3d")')
-        error stop 1
-    end if
     do i = 1, 3
         select case (mxll_boundaries(i))
         case ("close");    bc(i) = CLOSE_BOUNDARIES
@@ -71,6 +70,24 @@ program omx_fdtd_3d
     if (t_det_print <= 0) then
         write (*, *) "Error: dt_det_print must be greater than Maxwell dt."
         error stop 1
+    end if
+    ! quantum step and print stride as input_mod.f90:123,125,149-160 set them
+    ! (dt_q is taken as given, not converted from fs)
+    dt_q = mxll_dt_q
+    density = mxll_density_factor / (nm_to_au)**3
+    t_q_print = 1
+    if (mxll_n_q_groups > 0) then
+        if (mxll_dt_q_print == 0.0d0) then
+            dt_q_print = dt
+        else if (mxll_dt_q_print < dt_q * au_to_fs) then
+            write (*, '("Error: mxll_dt_q_print must be greater than or equal to the quantum system time step ' // &
+                      'dt_q = ", E12.4, "fs")') dt_q * au_to_fs
+            error stop 1
+        else
+            dt_q_print = mxll_dt_q_print * fs_to_au
+        end if
+        t_q_print = int(dt_q_print / dt)
+        if (t_q_print <= 0) t_q_print = 1
     end if
 
     call get_environment_variable("OMX_FDTD_DET_MIRROR", env, status=stat)
@@ -93,9 +110,10 @@ contains
         class(TMxll_3D_hip), intent(inout) :: m
         character(len=2)  :: dir
         real(dp) :: j_amp, freq, t0, tau, r0(3), radius, t_init, t_final, phase, time
-        real(dp), allocatable :: values(:)
-        integer, allocatable :: idx(:, :)
-        integer :: tt, n, print_step
+        real(dp), allocatable :: values(:), omega0(:), mu(:), pol(:, :), gamma2(:), dip(:, :), rho(:, :), et(:)
+        integer, allocatable :: idx(:, :), qijk(:, :), gtype(:), gof(:)
+        integer :: tt, n, g, nq, print_step, print_q_step, tt_0
+        logical :: move_q_system
 
         if (mxll_n_src > 0) then
             open (newunit=funit, file="sources.in", action="read", iostat=ierr)
@@ -123,15 +141,53 @@ contains
         end if
         allocate(values(max(m%n_samples, 1)))
 
-        print_step = 0
+        nq = 0
+        if (mxll_n_q_groups > 0) then
+            call read_q_groups(mxll_n_q_groups, grid, dr, qgroups)
+            do g = 1, mxll_n_q_groups
+                nq = nq + qgroups(g)%n_systems
+            end do
+            if (nq > 0) then
+                allocate(gtype(mxll_n_q_groups), gof(nq), qijk(nq, 3))
+                do g = 1, mxll_n_q_groups
+                    gtype(g) = qgroups(g)%group_type
+                    do i = 1, qgroups(g)%n_systems
+                        gof(qgroups(g)%first + i - 1) = g
+                        qijk(qgroups(g)%first + i - 1, :) = qgroups(g)%ijk(i, :)
+                    end do
+                end do
+                call m%set_q_groups(gtype, gof, qijk, density, dt_q)
+                call read_two_level(qgroups, omega0, mu, pol, gamma2)
+                call m%set_two_level(omega0, mu, pol, gamma2)
+                allocate(dip(nq, 3), rho(nq, 4), et(nq))
+            end if
+            call init_q_group_outputs(qgroups)
+        end if
+
+        ! quantum sub-steps (mxim_mxll.f90:73-79): the systems move every
+        ! n_skip_steps = NINT(dt_q/dt) steps when dt_q > dt, else every step
         m%n_skip_steps = 1
+        if (dt_q > dt) m%n_skip_steps = NINT(dt_q / dt)
+        tt_0 = 0
+        if (m%n_skip_steps == 1) tt_0 = 1
+        m%tq_step_old = 0
+        print_step = 0
+        print_q_step = 0
         do tt = 1, Nt
             time = dt * (DBLE(tt) - 0.5d0)
             m%time = time
-            m%t_skip = time
+            m%tq_step = 1 + int((tt - tt_0) / m%n_skip_steps)
+            move_q_system = .false.
+            if (m%tq_step > m%tq_step_old) then
+                m%t_skip = time
+                m%tq_step_old = m%tq_step
+                move_q_system = .true.
+                if (nq == 0) move_q_system = .false.
+            end if
             call m%td_propagate_H_field()
             call m%inject_sources(time)
             call m%td_propagate_E_field(tt)
+            if (move_q_system) call m%two_level_step()
             if (mxll_n_detectors > 0 .and. mod(tt, t_det_print) == 0) then
                 call write_detector_headers(dets, print_step, time, grid, dr)
                 if (use_mirror) then
@@ -142,6 +198,14 @@ contains
                     call write_detector_rows(dets, print_step, time, grid, dr, values)
                 end if
                 print_step = print_step + 1
+            end if
+            if (mxll_n_q_groups > 0 .and. mod(tt, t_q_print) == 0) then
+                if (nq > 0) then
+                    call m%get_two_level(dip, rho)
+                    et = 0.5d0 * omega0 * (rho(:, 2) - rho(:, 1))
+                    call write_q_group_outputs(qgroups, print_q_step, time, et, dip)
+                end if
+                print_q_step = print_q_step + 1
             end if
         end do
         call m%synchronize()

@@ -38,6 +38,10 @@ module omx_mxll_3d_hip_mod
         procedure :: add_detector => add_detector_3D_hip
         procedure :: sample_detectors => sample_3D_hip
         procedure :: set_q_sites => set_q_sites_3D_hip
+        procedure :: set_q_groups => set_q_groups_3D_hip
+        procedure :: set_two_level => set_two_level_3D_hip
+        procedure :: two_level_step => two_level_step_3D_hip
+        procedure :: get_two_level => get_two_level_3D_hip
         procedure :: gather_E => gather_E_3D_hip
         procedure :: scatter_J => scatter_J_3D_hip
         procedure :: energy => energy_3D_hip
@@ -269,6 +273,43 @@ contains
         idx = reshape(ijk, (/3 * this%n_sites/))
         call omx_check(omx3d_set_q_sites(this%ctx, this%n_sites, idx, density, dt_q), "omx3d_set_q_sites")
     end subroutine set_q_sites_3D_hip
+
+    ! Systems of q-groups (omx3d_set_q_groups): group_type(g) Q_MATERIAL or
+    ! Q_SINGLE, group_of(q) the 1-based group of system q (systems in group
+    ! order), ijk(q,:) its cell (a single group's kernel centre).
+    subroutine set_q_groups_3D_hip(this, group_type, group_of, ijk, density, dt_q)
+        class(TMxll_3D_hip), intent(inout) :: this
+        integer, intent(in) :: group_type(:), group_of(:), ijk(:, :)
+        real(dp), intent(in) :: density, dt_q
+        integer(c_int32_t), allocatable :: idx(:), gt(:), gof(:)
+        this%n_sites = size(ijk, 1)
+        allocate(idx(3 * this%n_sites), gt(size(group_type)), gof(this%n_sites))
+        idx = reshape(ijk, (/3 * this%n_sites/))
+        gt = group_type
+        gof = group_of - 1
+        call omx_check(omx3d_set_q_groups(this%ctx, size(group_type), gt, this%n_sites, gof, idx, density, dt_q), &
+                       "omx3d_set_q_groups")
+    end subroutine set_q_groups_3D_hip
+
+    ! Two-level emitters at the sites (au; pol(n,3)).
+    subroutine set_two_level_3D_hip(this, omega0, mu, pol, gamma2)
+        class(TMxll_3D_hip), intent(inout) :: this
+        real(dp), intent(in) :: omega0(:), mu(:), pol(:, :), gamma2(:)
+        call omx_check(omx3d_set_two_level(this%ctx, omega0, mu, pol, gamma2), "omx3d_set_two_level")
+    end subroutine set_two_level_3D_hip
+
+    ! One quantum step of every emitter: gather E, RK4 over dt_q, scatter J.
+    subroutine two_level_step_3D_hip(this)
+        class(TMxll_3D_hip), intent(inout) :: this
+        call omx_check(omx3d_two_level_step(this%ctx), "omx3d_two_level_step")
+    end subroutine two_level_step_3D_hip
+
+    ! dipole(n,3) and density matrix (rho_gg, rho_ee, Re rho_eg, Im rho_eg)(n,4).
+    subroutine get_two_level_3D_hip(this, dipole, rho)
+        class(TMxll_3D_hip), intent(inout) :: this
+        real(dp), intent(out) :: dipole(:, :), rho(:, :)
+        call omx_check(omx3d_get_two_level(this%ctx, dipole, rho), "omx3d_get_two_level")
+    end subroutine get_two_level_3D_hip
 
     ! E_field_list(n,3) of a q-group (interactions_mod.f90:656-691)
     subroutine gather_E_3D_hip(this, E_field_list)

@@ -22,6 +22,7 @@ module omx_units_mod
     real(dp), parameter :: nm_to_au = real(18.897261259077823_sp, dp)
     real(dp), parameter :: au_to_nm = 1.0d0 / nm_to_au
     real(dp), parameter :: ev_to_au = real(1.0_sp / 27.2114_sp, dp)
+    real(dp), parameter :: au_to_ev = real(27.2114_sp, dp)
     real(dp), parameter :: fs_to_au = 1.0d0 / 2.4188843265864d-2
     real(dp), parameter :: au_to_fs = 2.4188843265864d-2
     real(dp), parameter :: a_bh(4) = (/0.353222222d0, -0.488d0, 0.145d0, -0.010222222d0/)
@@ -32,6 +33,8 @@ module omx_units_mod
                           LINE_Z_DETECTOR = 4, PLANE_XY_DETECTOR = 5, PLANE_YZ_DETECTOR = 6, &
                           PLANE_ZX_DETECTOR = 7, VOLUME_DETECTOR = 8
     integer, parameter :: Ex_FIELD = 1, Ey_FIELD = 2, Ez_FIELD = 3, Hx_FIELD = 4, Hy_FIELD = 5, Hz_FIELD = 6
+    integer, parameter :: Q_MATERIAL = 1, Q_SINGLE = 2
+    integer, parameter :: PRINT_NONE = 0, PRINT_ALL = 1, PRINT_SELECTED = 2
 
 end module omx_units_mod
 
