@@ -235,6 +235,11 @@ int omx3d_energy_noalloc(omx3d_ctx* ctx, double* w_dev);
 /* ---- total-field / scattered-field plane wave (x-polarised, along +z) ---- */
 int omx3d_set_tfsf(omx3d_ctx* ctx, const int32_t lo[3], const int32_t hi[3],
                    double e0, double w0, double t0, double tau, int32_t max_steps);
+/* The plane wave's incident line (after the steps queued so far): Ex at
+ * k = kb .. kb+n-1 and Hy at k+1/2, n values each (ex1 / hy1 may be null to
+ * query kb and n).  No reference counterpart: it lets a caller compare the
+ * total field inside the box with the incident wave. */
+int omx3d_get_tfsf_line(omx3d_ctx* ctx, double* ex1, double* hy1, int32_t* kb, int32_t* n);
 
 /* ---- whole time loop on the device (mxim_mxll.f90:92-132 step order) ----- */
 /* Steps tt = tt_first .. tt_first+n_steps-1; records detectors when

@@ -2230,6 +2230,17 @@ int omx3d_energy_noalloc(omx3d_ctx* c, double* w_dev) {
     return rc;
 }
 
+int omx3d_get_tfsf_line(omx3d_ctx* c, double* ex1, double* hy1, int32_t* kb, int32_t* n) {
+    CHECK_CTX(c);
+    if (!c->tfsf) return fail(OMX_ERR_STATE, "no TF/SF plane wave");
+    if (kb) *kb = c->tf_kb;
+    if (n) *n = c->tf_n;
+    if (ex1) HIPCHK(hipMemcpyAsync(ex1, c->d_tf, sizeof(double) * c->tf_n, hipMemcpyDeviceToHost, c->s));
+    if (hy1) HIPCHK(hipMemcpyAsync(hy1, c->d_tf + c->tf_n, sizeof(double) * c->tf_n, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return OMX_OK;
+}
+
 int omx3d_set_tfsf(omx3d_ctx* c, const int32_t lo[3], const int32_t hi[3], double e0, double w0, double t0,
                    double tau, int32_t max_steps) {
     CHECK_CTX(c);

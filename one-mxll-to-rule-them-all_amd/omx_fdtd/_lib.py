@@ -80,6 +80,7 @@ _SIGS = {
     "omx3d_sample": ([_P, _DP], C.c_int),
     "omx3d_set_q_sites": ([_P, _I, _IP, _D, _D], C.c_int),
     "omx3d_set_q_groups": ([_P, _I, _IP, _I, _IP, _IP, _D, _D], C.c_int),
+    "omx3d_get_tfsf_line": ([_P, _DP, _DP, _IP, _IP], C.c_int),
     "omx3d_gather_e": ([_P, _DP], C.c_int),
     "omx3d_scatter_j": ([_P, _DP], C.c_int),
     "omx3d_set_two_level": ([_P, _DP, _DP, _DP, _DP], C.c_int),

@@ -295,6 +295,15 @@ class Grid3D:
         L.call("omx3d_energy", self.h, C.byref(w64), C.byref(w32))
         return w64.value, w32.value
 
+    def tfsf_line(self):
+        """(kb, ex1, hy1): the incident line's Ex at k = kb + m and Hy at
+        k + 1/2 (omx3d_get_tfsf_line)."""
+        kb, n = L.i32([0]), L.i32([0])
+        L.call("omx3d_get_tfsf_line", self.h, None, None, L.iptr(kb), L.iptr(n))
+        ex1, hy1 = np.zeros(int(n[0])), np.zeros(int(n[0]))
+        L.call("omx3d_get_tfsf_line", self.h, L.dptr(ex1), L.dptr(hy1), L.iptr(kb), L.iptr(n))
+        return int(kb[0]), ex1, hy1
+
     def set_tfsf(self, lo, hi, e0, w0, t0, tau, max_steps):
         lo = L.i32(lo)
         hi = L.i32(hi)

@@ -937,6 +937,13 @@ void orc_set_tfsf(orc_sim* s, const int* lo, const int* hi, double e0, double w0
     s->tfsf = 1;
 }
 
+/* The incident line: ex1 / hy1 at k = kb .. kb+n-1 (n values each). */
+void orc_get_tfsf_line(const orc_sim* s, double* ex1, double* hy1, int* kb, int* n) {
+    *kb = s->tf_kb; *n = s->tf_n;
+    if (ex1) memcpy(ex1, s->ex1, sizeof(double) * s->tf_n);
+    if (hy1) memcpy(hy1, s->hy1, sizeof(double) * s->tf_n);
+}
+
 static void tfsf_h(orc_sim* s) {
     const int i0 = s->tf_lo[0], j0 = s->tf_lo[1], k0 = s->tf_lo[2], i1 = s->tf_hi[0], j1 = s->tf_hi[1], k1 = s->tf_hi[2];
     const int kb = s->tf_kb;

@@ -58,6 +58,7 @@ def load():
             "orc_get_two_level": ([P, _DP, _DP], V),
             "orc_copy_field": ([P, I, _DP, I], V),
             "orc_set_tfsf": ([P, _IP, _IP, D, D, D, D, I], V),
+            "orc_get_tfsf_line": ([P, _DP, _DP, _IP, _IP], V),
             "orc_n_samples": ([P], C.c_long),
             "orc_run": ([P, I, I, I, _DP, _IP], V),
             "orc_num_threads": ([], I),
@@ -252,6 +253,15 @@ class OracleGrid:
         f = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz").index(name) + 1
         flat = np.ascontiguousarray(np.asarray(arr, dtype=np.float64).ravel(order="F"))
         self.lib.orc_copy_field(self.h, f, _d(flat), 1)
+
+    def tfsf_line(self):
+        """(kb, ex1, hy1): the incident line's Ex at k = kb + m and Hy at
+        k + 1/2."""
+        kb, n = np.zeros(1, dtype=np.int32), np.zeros(1, dtype=np.int32)
+        self.lib.orc_get_tfsf_line(self.h, None, None, _i(kb), _i(n))
+        ex1, hy1 = np.zeros(int(n[0])), np.zeros(int(n[0]))
+        self.lib.orc_get_tfsf_line(self.h, _d(ex1), _d(hy1), _i(kb), _i(n))
+        return int(kb[0]), ex1, hy1
 
     def set_tfsf(self, lo, hi, e0, w0, t0, tau, max_steps):
         lo = np.ascontiguousarray(lo, dtype=np.int32)
