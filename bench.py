@@ -471,6 +471,13 @@ def bench_batched(args):
     for prec in precisions(args):
         log(f"[bench] batch of {len(cs)} grids, {cells} cells, FP{8 * prec}")
         grids = [build_grid(c, prec) for c in cs]
+        # each grid's planes and tile order tuned on its own at the batch's
+        # common row count (one workgroup height for all grids; the batch
+        # keeps every grid's tiling)
+        if not args.no_autotune:
+            rows = grids[0].tiling()[0]
+            tiles = [g.autotune(steps=2, rows=rows)[:3] for g in grids]
+            log(f"[bench] per-grid tilings (rows, xchunk, order): {sorted(set(tiles))}")
         b = omx_fdtd.Batch(grids)
         if W:
             b.run(1, W, cs[0].record_every)
@@ -510,7 +517,8 @@ def bench_batched(args):
     r32 = out_prec[4]
     out = base_record("batched_64", K, W, r32["gcells"], r32["ms_step"], "fp32",
                       {"grids": len(cs), "edges": [96, 128, 160, 192], "total_cells": cells, "npml": 8,
-                       "record_every": cs[0].record_every, "graph": True})
+                       "record_every": cs[0].record_every, "graph": True,
+                       "tiling": "defaults" if args.no_autotune else "autotuned per grid"})
     out["one_by_one_gcells"] = round(seq, 4)
     out["single_grid_gcells"] = round(single, 4)
     out["single_grid_edge"] = edge
@@ -539,6 +547,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="batched_64: keep the default tiling of every grid")
     ap.add_argument("--mode", choices=("separate", "fused"), default="separate",
                     help="separate H and E kernels, or the fused H+E pass")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default=HEADLINE)

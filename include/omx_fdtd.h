@@ -283,8 +283,9 @@ int omx3d_synchronize(omx3d_ctx* ctx);
 int omx3d_device_bytes(omx3d_ctx* ctx, int64_t* bytes);
 /* Tile-shape autotuner: times rows_y x xchunk x tile order candidates with
  * `steps` H+E steps each, keeps the fastest, then clears fields, CPML and
- * dispersive state (call before a run).  No reference counterpart (engine
- * tuning). */
+ * dispersive state (call before a run).  *rows_y on entry: 0 tunes the row
+ * count too; 1, 2 or 4 keeps it (the grids of a batch share one workgroup
+ * height).  No reference counterpart (engine tuning). */
 int omx3d_autotune(omx3d_ctx* ctx, int32_t steps, int32_t* rows_y, int32_t* xchunk, int32_t* tile_order,
                    double* ms_per_step);
 

@@ -1199,8 +1199,6 @@ struct omxb_ctx {
     long* d_step = nullptr;
     bool periodic = false;
     int rows = 1;                   // workgroup height: the largest rows_y of the grids
-    int tile_order = 0;             // tile order of the batched H/E kernels: the first grid's
-    int ntz = 1;                    // z tiles per row block of the first grid (tile order 2)
     bool b32 = true;                // every grid addressed by 32-bit byte offsets
     long* d_disp = nullptr;         // cells of dispersive classes of every grid (packed as per grid)
     int* d_disp_grid = nullptr;     // grid of each workgroup of dispersive cells
@@ -1252,18 +1250,18 @@ int batch_setup(omxb_ctx* b) {
     std::vector<YeeArgs<T>> args;
     std::vector<int> ts;
     int tot = 0;
+    // each grid keeps its own tiling (rows, planes, order); its block range
+    // is padded to a multiple of 8 (batch_tile)
     for (auto* c : b->g) {
         YeeArgs<T> a = make_args<T>(c);
         args.push_back(a);
         ts.push_back(tot);
-        tot += (int)yee_blocks(c, a);
+        tot += ((int)yee_blocks(c, a) + 7) / 8 * 8;
     }
     ts.push_back(tot);
     b->total_tiles = tot;
     b->rows = 1;
     for (auto* c : b->g) b->rows = std::max(b->rows, c->rows_y);
-    b->tile_order = b->g[0]->tile_order;
-    b->ntz = args[0].ntz;
     b->b32 = true;
     for (auto* c : b->g) b->b32 = b->b32 && offsets32(c);
     // cells of dispersive classes of all grids, for the first workgroups of
@@ -1385,10 +1383,10 @@ int batch_step_launches(omxb_ctx* b) {
         for (auto* c : b->g) { int rc = wrap_e<T>(c, b->s); if (rc) return rc; }
     if (b->b32)
         hipLaunchKernelGGL((k_h_update_batched<T, true>), dim3(b->total_tiles), blk, 0, b->s,
-                           (const YeeArgs<T>*)b->d_args, b->d_tile_start, b->n, b->tile_order, b->ntz);
+                           (const YeeArgs<T>*)b->d_args, b->d_tile_start, b->n);
     else
         hipLaunchKernelGGL((k_h_update_batched<T, false>), dim3(b->total_tiles), blk, 0, b->s,
-                           (const YeeArgs<T>*)b->d_args, b->d_tile_start, b->n, b->tile_order, b->ntz);
+                           (const YeeArgs<T>*)b->d_args, b->d_tile_start, b->n);
     if (b->n_pcell)
         hipLaunchKernelGGL(k_patch_batched<T>, dim3(blocks_for(b->n_pcell, 256)), dim3(256), 0, b->s,
                            (const YeeArgs<T>*)b->d_args, b->d_pbox, b->n_pbox, b->n_pcell, 0);
@@ -1405,12 +1403,12 @@ int batch_step_launches(omxb_ctx* b) {
     const int db = b->disp_blocks;
     if (b->b32)
         hipLaunchKernelGGL((k_e_update_batched<T, true>), dim3(b->total_tiles + db), blk, 0, b->s,
-                           (const YeeArgs<T>*)b->d_args, b->d_tile_start, b->n, b->tile_order, b->ntz, b->d_disp,
-                           b->d_disp_grid, b->n_disp, db);
+                           (const YeeArgs<T>*)b->d_args, b->d_tile_start, b->n, b->d_disp, b->d_disp_grid,
+                           b->n_disp, db);
     else
         hipLaunchKernelGGL((k_e_update_batched<T, false>), dim3(b->total_tiles + db), blk, 0, b->s,
-                           (const YeeArgs<T>*)b->d_args, b->d_tile_start, b->n, b->tile_order, b->ntz, b->d_disp,
-                           b->d_disp_grid, b->n_disp, db);
+                           (const YeeArgs<T>*)b->d_args, b->d_tile_start, b->n, b->d_disp, b->d_disp_grid,
+                           b->n_disp, db);
     if (b->n_pcell)
         hipLaunchKernelGGL(k_patch_batched<T>, dim3(blocks_for(b->n_pcell, 256)), dim3(256), 0, b->s,
                            (const YeeArgs<T>*)b->d_args, b->d_pbox, b->n_pbox, b->n_pcell, 1);
@@ -2405,7 +2403,12 @@ int omx3d_autotune(omx3d_ctx* c, int32_t steps, int32_t* rows_y, int32_t* xchunk
                    double* ms_per_step) {
     CHECK_CTX(c);
     if (steps < 1) return fail(OMX_ERR_ARG, "steps must be >= 1");
-    const int rows_c[3] = {1, 2, 4};
+    // *rows_y of 1, 2 or 4 on entry: keep that row count (a batch runs all
+    // its grids with one workgroup height), tune planes and order only
+    const int fixed = rows_y ? *rows_y : 0;
+    if (fixed != 0 && fixed != 1 && fixed != 2 && fixed != 4) return fail(OMX_ERR_ARG, "rows_y must be 0, 1, 2 or 4");
+    int rows_c[3] = {1, 2, 4};
+    if (fixed) rows_c[0] = rows_c[1] = rows_c[2] = fixed;
     const int xc_c[5] = {1, 2, 4, 8, 16};
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
@@ -2413,8 +2416,9 @@ int omx3d_autotune(omx3d_ctx* c, int32_t steps, int32_t* rows_y, int32_t* xchunk
     double best = 1e30;
     int br = c->rows_y, bx = c->xchunk, bo = c->tile_order;
     for (int order = 0; order < 3; ++order)
-    for (int r : rows_c)
+    for (int ri = 0; ri < (fixed ? 1 : 3); ++ri)
         for (int x : xc_c) {
+            const int r = rows_c[ri];
             c->rows_y = r; c->xchunk = x; c->tile_order = order;
             for (int w = 0; w < 2; ++w) {   // warm-up launch pair
                 int rc = c->prec == 4 ? launch_h<float>(c, false) : launch_h<double>(c, false);

@@ -1232,17 +1232,33 @@ __device__ __forceinline__ int find_grid(const int* __restrict__ tile_start, int
     return lo;
 }
 
+// Tile of launch block b of a batch: the grids' blocks follow one another,
+// each grid's range padded to a multiple of 8 (tile_start: the ranges'
+// starts, n+1 entries), so that the blocks of one XCD keep one residue of 8
+// inside every grid and each grid deals its tiles in its own order
+// (a.tile_order over its own z tiles).  -1: a padding block.
+template <typename T>
+__device__ __forceinline__ int batch_tile(const YeeArgs<T>* __restrict__ args, const int* __restrict__ tile_start,
+                                          int n, int b, int& g) {
+    g = find_grid(tile_start, n, b);
+    const YeeArgs<T>& a = args[g];
+    const int nb = tile_start[g + 1] - tile_start[g];
+    const int t = tile_of_block(a.tile_order, b - tile_start[g], nb, a.ntz);
+    const int nt = a.ntz * a.nty * ((a.nx + a.xchunk - 1) / a.xchunk);
+    return t < nt ? t : -1;
+}
+
 template <typename T, bool B32>
 __global__ __launch_bounds__(256) void k_h_update_batched(const YeeArgs<T>* __restrict__ args,
-                                                          const int* __restrict__ tile_start, int n, int order,
-                                                          int ntz) {
-    const int t = tile_of_block(order, blockIdx.x, gridDim.x, ntz);
-    const int g = find_grid(tile_start, n, t);
+                                                          const int* __restrict__ tile_start, int n) {
+    int g;
+    const int t = batch_tile(args, tile_start, n, (int)blockIdx.x, g);
+    if (t < 0) return;
     const YeeArgs<T>& a = args[g];
     if ((int)threadIdx.y >= a.rows) return;
     const TfsfLaunch none{};
-    if (a.cpml) h_tile<T, true, B32, false>(a, none, t - tile_start[g]);
-    else h_tile<T, false, B32, false>(a, none, t - tile_start[g]);
+    if (a.cpml) h_tile<T, true, B32, false>(a, none, t);
+    else h_tile<T, false, B32, false>(a, none, t);
 }
 
 // Batched E tiles; the first db workgroups take the dispersive cells of
@@ -1250,8 +1266,8 @@ __global__ __launch_bounds__(256) void k_h_update_batched(const YeeArgs<T>* __re
 // -1 to whole workgroups; grid_of[b]: the grid of workgroup b).
 template <typename T, bool B32>
 __global__ __launch_bounds__(256) void k_e_update_batched(const YeeArgs<T>* __restrict__ args,
-                                                          const int* __restrict__ tile_start, int n, int order,
-                                                          int ntz, const long* __restrict__ disp,
+                                                          const int* __restrict__ tile_start, int n,
+                                                          const long* __restrict__ disp,
                                                           const int* __restrict__ grid_of, int n_disp, int db) {
     const int b = blockIdx.x;
     if (b < db) {
@@ -1264,15 +1280,15 @@ __global__ __launch_bounds__(256) void k_e_update_batched(const YeeArgs<T>* __re
         e_dispersive_cell<T, true, B32>(a, a.ex, a.ey, a.ez, e, p - a.disp0);
         return;
     }
-    // backwards, as in k_e_update
-    const int nt = (int)gridDim.x - db;
-    const int t = nt - 1 - tile_of_block(order, b - db, nt, ntz);
-    const int g = find_grid(tile_start, n, t);
+    // backwards, as in k_e_update: the last grid's last tiles first
+    int g;
+    const int t = batch_tile(args, tile_start, n, tile_start[n] - 1 - (b - db), g);
+    if (t < 0) return;
     const YeeArgs<T>& a = args[g];
     if ((int)threadIdx.y >= a.rows) return;
     const TfsfLaunch none{};
-    if (a.cpml) e_tile<T, true, 1, B32, false>(a, none, t - tile_start[g]);
-    else e_tile<T, false, 1, B32, false>(a, none, t - tile_start[g]);
+    if (a.cpml) e_tile<T, true, 1, B32, false>(a, none, t);
+    else e_tile<T, false, 1, B32, false>(a, none, t);
 }
 
 // ---------------------------------------------------------------------------

@@ -130,9 +130,10 @@ class Grid3D:
         L.call("omx3d_get_step_mode", self.h, C.byref(f))
         return bool(f.value)
 
-    def autotune(self, steps=4):
-        """Pick the fastest tile shape; clears the fields (call before a run)."""
-        r, x, o, ms = C.c_int32(), C.c_int32(), C.c_int32(), C.c_double()
+    def autotune(self, steps=4, rows=0):
+        """Pick the fastest tile shape (rows 1/2/4: keep that row count);
+        clears the fields (call before a run)."""
+        r, x, o, ms = C.c_int32(int(rows)), C.c_int32(), C.c_int32(), C.c_double()
         L.call("omx3d_autotune", self.h, int(steps), C.byref(r), C.byref(x), C.byref(o), C.byref(ms))
         return r.value, x.value, o.value, ms.value
 

@@ -1,5 +1,6 @@
 """Batched launches: every grid of a batch computes exactly (bitwise) what
-its own single-grid run computes, with and without the captured graph."""
+its own single-grid run computes, with and without the captured graph, each
+grid under its own tiling (rows, planes, tile order)."""
 import numpy as np
 import pytest
 
@@ -24,7 +25,7 @@ def test_batch_equals_single_grid_runs(precision, graph):
     cs = cases.config_batched(n_grids=6, edges=(24, 32, 40), npml=4, steps=60, record_every=3)
     # a second source sharing a cell with the first of grid 0 exercises source passes
     cs[0].sources.append(cs[0].sources[2])
-    # the batch takes the tile order of its first grid
+    # every grid keeps its own rows, planes and tile order inside the batch
     tilings = [(4, 16, 1 if graph else 2), (2, 5, 0), (1, 64, 1), (4, 3, 0), (4, 8, 1), (1, 1, 2)]
     print(f"[batch] precision={precision} graph={graph} grids={[c.grid for c in cs]}", flush=True)
     singles = []

@@ -101,6 +101,12 @@ def test_batched_scaled(graph, precision):
     cs = cases.config_batched(n_grids=4, edges=(32, 40, 48), npml=4, steps=200, record_every=4)
     print(f"[workload] batched grids={[c.grid for c in cs]} graph={graph} precision={precision}", flush=True)
     grids = [engine(c, precision, 1) for c in cs]
+    # each grid with its own autotuned planes and order at a common row
+    # count, as the batched bench runs them
+    for g in grids:
+        rows, xc, order, ms = g.autotune(steps=2, rows=2)
+        assert rows == 2
+        print(f"[workload] autotuned rows {rows} xchunk {xc} order {order} ({ms:.3f} ms/step)", flush=True)
     b = omx_fdtd.Batch(grids)
     b.run(1, cs[0].steps, cs[0].record_every, graph=graph)
     for c, g in zip(cs, grids):
