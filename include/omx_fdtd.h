@@ -303,6 +303,64 @@ int omxb_create(int32_t n, omx3d_ctx* const* grids, omxb_ctx** out);
 int omxb_run(omxb_ctx* batch, int32_t tt_first, int32_t n_steps, int32_t record_every, int32_t use_graph);
 int omxb_destroy(omxb_ctx* batch);
 
+/* ---- 2D and 1D grids (TMxll_2D, mxll_2D_mod.f90; TMxll_1D, mxll_1D_mod.f90) */
+/* FP64 only.  2D: n1 = nx, n2 = ny, mode OMX_2D_TMZ (Ez, Hx, Hy), OMX_2D_TEZ
+ * (Ex, Ey, Hz) or OMX_2D_FULL (both families, which do not couple in 2D),
+ * boundaries[0..1] per axis.  1D (the reference's z line: Ex at k, Hy at
+ * k+1/2): n1 = nz, n2 and mode ignored, boundaries[0].  Arrays keep the
+ * reference's bounds (Ex(0:nx,1:ny+1), Ey(1:nx+1,0:ny), Ez(1:nx+1,1:ny+1),
+ * Hx(1:nx,0:ny), Hy(0:nx,1:ny), Hz(0:nx,0:ny); 1D Ex(1:nz), Hy(1:nz-1)),
+ * which omxl_copy_field reads and writes in Fortran order.  The reference's
+ * FULL mode allocates only the TEz arrays (mxll_2D_mod.f90:110-163) and
+ * cannot run; here it runs both families. */
+#define OMX_2D_TMZ  1
+#define OMX_2D_TEZ  2
+#define OMX_2D_FULL 3
+typedef struct omxl_desc {
+    int32_t dims;              /* 1 or 2                                      */
+    int32_t mode;              /* OMX_2D_* (2D)                               */
+    int32_t n1, n2;            /* cells: nx, ny (2D) or nz (1D)               */
+    int32_t npml;              /* CPML thickness (>= 2 if any CPML)           */
+    int32_t boundaries[2];     /* OMX_BC_* per axis                           */
+    int32_t precision;         /* OMX_FP64                                    */
+    double  dt, dr;            /* atomic units                                */
+} omxl_desc;
+typedef struct omxl_ctx omxl_ctx;
+/* init_2Dgrid / init_1Dgrid (mxll_2D_mod.f90:177-420, mxll_1D_mod.f90:52-210):
+ * fields zero, CPML profiles and den arrays on the device. */
+int omxl_create(const omxl_desc* desc, omxl_ctx** out);
+int omxl_destroy(omxl_ctx* ctx);
+/* read_init_media_2D / read_init_media_1D (classical_medium_mod.f90:341-538,
+ * 203-337): aux is the cell medium map (nx*ny Fortran order, or nz), values
+ * 0..n_media.  2D keeps the reference's staggering: component maps from the
+ * x / y neighbours, the edge row and column set only in the Ex map, and
+ * every component's eps from the Ex map's dielectric. */
+int omxl_set_media(omxl_ctx* ctx, int32_t n_media, const omx_medium* table, const int8_t* aux);
+/* td_propagate_H_2Dfield / _1Dfield and td_propagate_E_* (mxll_2D_mod.f90:450-864,
+ * mxll_1D_mod.f90:252-352), with the periodic copies of their non-MPI branches. */
+int omxl_step_h(omxl_ctx* ctx);
+int omxl_step_e(omxl_ctx* ctx);
+/* A source of read_init_sources (source_mod.f90:40-100): dir 1..3 (x, y, z;
+ * 1D: 1), ind(w*w, 2) (2D, Fortran order; 1D: ind(w)) 1-based indices and
+ * ker the kernel, w = 2 n_ker + 1.  source_interactions, 1D / 2D
+ * (interactions_mod.f90:32-100): E += dt_eps0/dr/c0/2 * J, with J the mean of
+ * the point and its +x (+y) neighbour in the kernel for 2D Ex (Ey), the
+ * point's own J for 2D Ez and 1D Ex. */
+int omxl_add_source(omxl_ctx* ctx, int32_t dir, double j_amp, double w0, double t0, double tau, double t_init,
+                    double t_final, double phase, int32_t n_ker, const int32_t* ind, const double* ker);
+int omxl_inject_sources(omxl_ctx* ctx, double time);
+/* Samples of field at nd cells idx(nd, dims) (Fortran order, 1-based) with the
+ * stencils of write_2D_field / write_1D_field (write_fields_subs_mod.f90:88-628). */
+int omxl_add_detector(omxl_ctx* ctx, int32_t field, int32_t nd, const int32_t* idx);
+int omxl_sample(omxl_ctx* ctx, double* out);
+/* Steps tt = tt_first .. tt_first+n_steps-1 in the driver loop's order
+ * (mxim_mxll.f90:92-132); after every step with mod(tt, record_every) == 0
+ * the samples of every detector go to the next row of out. */
+int omxl_run(omxl_ctx* ctx, int32_t tt_first, int32_t n_steps, int32_t record_every, double* out);
+int omxl_copy_field(omxl_ctx* ctx, int32_t field, double* host, int32_t to_device);
+int omxl_synchronize(omxl_ctx* ctx);
+int omxl_device_bytes(omxl_ctx* ctx, int64_t* bytes);
+
 #ifdef __cplusplus
 }
 #endif

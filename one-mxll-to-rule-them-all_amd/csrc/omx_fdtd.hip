@@ -2468,3 +2468,6 @@ int omx3d_autotune(omx3d_ctx* c, int32_t steps, int32_t* rows_y, int32_t* xchunk
 }
 
 }  // extern "C"
+
+// The 2D and 1D solvers.
+#include "omx_lowdim.hpp"

@@ -29,6 +29,12 @@ class Desc(C.Structure):
                 ("dt", C.c_double), ("dr", C.c_double)]
 
 
+class LowDesc(C.Structure):
+    _fields_ = [("dims", C.c_int32), ("mode", C.c_int32), ("n1", C.c_int32), ("n2", C.c_int32),
+                ("npml", C.c_int32), ("boundaries", C.c_int32 * 2), ("precision", C.c_int32),
+                ("dt", C.c_double), ("dr", C.c_double)]
+
+
 class Medium(C.Structure):
     _fields_ = [("kind", C.c_int32), ("n_poles", C.c_int32), ("eps_r", C.c_double),
                 ("omega_d", C.c_double), ("gamma_d", C.c_double),
@@ -103,6 +109,19 @@ _SIGS = {
     "omxb_create": ([_I, C.POINTER(_P), C.POINTER(_P)], C.c_int),
     "omxb_run": ([_P, _I, _I, _I, _I], C.c_int),
     "omxb_destroy": ([_P], C.c_int),
+    "omxl_create": ([C.c_void_p, C.POINTER(_P)], C.c_int),
+    "omxl_destroy": ([_P], C.c_int),
+    "omxl_set_media": ([_P, _I, C.c_void_p, C.POINTER(C.c_int8)], C.c_int),
+    "omxl_step_h": ([_P], C.c_int),
+    "omxl_step_e": ([_P], C.c_int),
+    "omxl_add_source": ([_P, _I, _D, _D, _D, _D, _D, _D, _D, _I, _IP, _DP], C.c_int),
+    "omxl_inject_sources": ([_P, _D], C.c_int),
+    "omxl_add_detector": ([_P, _I, _I, _IP], C.c_int),
+    "omxl_sample": ([_P, _DP], C.c_int),
+    "omxl_run": ([_P, _I, _I, _I, _DP], C.c_int),
+    "omxl_copy_field": ([_P, _I, _DP, _I], C.c_int),
+    "omxl_synchronize": ([_P], C.c_int),
+    "omxl_device_bytes": ([_P, C.POINTER(C.c_int64)], C.c_int),
 }
 
 # every entry point include/omx_fdtd.h declares

@@ -111,8 +111,8 @@ typedef struct orc_sim {
 /* ------------------------------------------------------------------------ */
 /* init_3Dgrid, CPML part (mxll_3D_mod.f90:282-420).  The ratios
  * (npml-kk)/(npml-1.0) are default-REAL and cubed in single precision. */
-static void grade_e(const orc_sim* s, double sigmaCPML, double sig_to_au, int kk, double* sig, double* alpha, double* kappa) {
-    const int npml = s->npml;
+static void grade_e(int npml_, double sigmaCPML, double sig_to_au, int kk, double* sig, double* alpha, double* kappa) {
+    const int npml = npml_;
     float r = (float)(npml - kk) / ((float)npml - 1.0f);
     float r3 = r * r * r;
     float ra = (float)(kk - 1) / ((float)npml - 1.0f);
@@ -120,8 +120,8 @@ static void grade_e(const orc_sim* s, double sigmaCPML, double sig_to_au, int kk
     *alpha = (double)0.05f * sig_to_au * (double)ra;
     *kappa = 1.0 + (5.0 - 1.0) * (double)r3;
 }
-static void grade_h(const orc_sim* s, double sigmaCPML, double sig_to_au, int kk, double* sig, double* alpha, double* kappa) {
-    const int npml = s->npml;
+static void grade_h(int npml_, double sigmaCPML, double sig_to_au, int kk, double* sig, double* alpha, double* kappa) {
+    const int npml = npml_;
     float r = ((float)(npml - kk) - 0.5f) / ((float)npml - 1.0f);
     float r3 = r * r * r;
     float ra = ((float)kk - 0.5f) / ((float)npml - 1.0f);
@@ -185,13 +185,13 @@ orc_sim* orc_create(int nx, int ny, int nz, int npml, const int* bc, double dt, 
         const double sigmaCPML = (double)(0.8f * (float)(3 + 1)) / (dr * sqrt(MU0 / EPS0));
         double sig, al, kap;
         for (int kk = 1; kk <= npml; ++kk) {
-            grade_e(s, sigmaCPML, sig_to_au, kk, &sig, &al, &kap);
+            grade_e(s->npml, sigmaCPML, sig_to_au, kk, &sig, &al, &kap);
             s->be[kk] = exp(-(sig / kap + al) * dt / EPS0);
             if (sig == 0.0 && al == 0.0 && kk == npml) s->ce[kk] = 0.0;
             else s->ce[kk] = sig * (s->be[kk] - 1.0) / (sig + kap * al) / kap;
         }
         for (int kk = 1; kk <= npml - 1; ++kk) {
-            grade_h(s, sigmaCPML, sig_to_au, kk, &sig, &al, &kap);
+            grade_h(s->npml, sigmaCPML, sig_to_au, kk, &sig, &al, &kap);
             s->bh[kk] = exp(-(sig / kap + al) * dt / EPS0);
             s->ch[kk] = sig * (s->bh[kk] - 1.0) / (sig + kap * al) / kap;
         }
@@ -201,17 +201,17 @@ orc_sim* orc_create(int nx, int ny, int nz, int npml, const int* bc, double dt, 
         for (int a = 0; a < 3; ++a) {
             if (s->cpml_pos[2 * a]) {
                 for (int k = 1; k <= n[a]; ++k)
-                    if (k <= npml) { grade_e(s, sigmaCPML, sig_to_au, k, &sig, &al, &kap); de[a][k] = 1.0 / (kap * dr); }
+                    if (k <= npml) { grade_e(s->npml, sigmaCPML, sig_to_au, k, &sig, &al, &kap); de[a][k] = 1.0 / (kap * dr); }
                 for (int k = 1; k <= n[a]; ++k)
-                    if (k <= npml - 1) { grade_h(s, sigmaCPML, sig_to_au, k, &sig, &al, &kap); dh[a][k] = 1.0 / (kap * dr); }
+                    if (k <= npml - 1) { grade_h(s->npml, sigmaCPML, sig_to_au, k, &sig, &al, &kap); dh[a][k] = 1.0 / (kap * dr); }
             }
             if (s->cpml_pos[2 * a + 1]) {
                 int kk = npml;
                 for (int k = 1; k <= n[a] - 1; ++k)
-                    if (k >= n[a] + 1 - npml) { grade_e(s, sigmaCPML, sig_to_au, kk, &sig, &al, &kap); de[a][k] = 1.0 / (kap * dr); kk--; }
+                    if (k >= n[a] + 1 - npml) { grade_e(s->npml, sigmaCPML, sig_to_au, kk, &sig, &al, &kap); de[a][k] = 1.0 / (kap * dr); kk--; }
                 kk = npml - 1;
                 for (int k = 1; k <= n[a] - 1; ++k)
-                    if (k >= n[a] + 1 - npml) { grade_h(s, sigmaCPML, sig_to_au, kk, &sig, &al, &kap); dh[a][k] = 1.0 / (kap * dr); kk--; }
+                    if (k >= n[a] + 1 - npml) { grade_h(s->npml, sigmaCPML, sig_to_au, kk, &sig, &al, &kap); dh[a][k] = 1.0 / (kap * dr); kk--; }
             }
         }
     }
@@ -371,10 +371,10 @@ void orc_get_medium_coeffs(const orc_sim* s, int m, double* out) {
 }
 
 /* get_medium_polarization (classical_medium_mod.f90:773-833) */
-static int medium_polarization(const orc_sim* s, int idx, double* PDi, double* PLi, double* PLi_old, long pstride,
+static int medium_polarization(const medium_co* media, int idx, double* PDi, double* PLi, double* PLi_old, long pstride,
                                double* Ei, double* Ei_old, double rotH) {
     if (idx == 0) return 1;
-    const medium_co* m = &s->media[idx];
+    const medium_co* m = &media[idx];
     if (m->type == 2) {
         double tmpE = m->C1 * *Ei + m->C3 * rotH - m->C4 * *PDi;
         *PDi = m->A1 * *PDi + m->A2 * (tmpE + *Ei);
@@ -542,7 +542,7 @@ void orc_step_e(orc_sim* s, double time, double t_skip) {
                     double rotH = (AT(s->Hz, i, j, k) - AT(s->Hz, i, j - 1, k)) * s->den_ey[j] +
                                   (AT(s->Hy, i, j, k - 1) - AT(s->Hy, i, j, k)) * s->den_ez[k];
                     const long p = PIDX(i, j, k);
-                    int no_medium = medium_polarization(s, MM(i, j, k, 0), &AT(s->PDx, i, j, k), s->PLx ? s->PLx + p : 0,
+                    int no_medium = medium_polarization(s->media, MM(i, j, k, 0), &AT(s->PDx, i, j, k), s->PLx ? s->PLx + p : 0,
                                                         s->PLx_old ? s->PLx_old + p : 0, pst, &AT(s->Ex, i, j, k), &AT(s->Ex_old, i, j, k), rotH);
                     if (no_medium) {
                         double Jav = 0.0;
@@ -557,7 +557,7 @@ void orc_step_e(orc_sim* s, double time, double t_skip) {
                     double rotH = (AT(s->Hz, i - 1, j, k) - AT(s->Hz, i, j, k)) * s->den_ex[i] +
                                   (AT(s->Hx, i, j, k) - AT(s->Hx, i, j, k - 1)) * s->den_ez[k];
                     const long p = PIDX(i, j, k);
-                    int no_medium = medium_polarization(s, MM(i, j, k, 1), &AT(s->PDy, i, j, k), s->PLy ? s->PLy + p : 0,
+                    int no_medium = medium_polarization(s->media, MM(i, j, k, 1), &AT(s->PDy, i, j, k), s->PLy ? s->PLy + p : 0,
                                                         s->PLy_old ? s->PLy_old + p : 0, pst, &AT(s->Ey, i, j, k), &AT(s->Ey_old, i, j, k), rotH);
                     if (no_medium) {
                         double Jav = 0.0;
@@ -572,7 +572,7 @@ void orc_step_e(orc_sim* s, double time, double t_skip) {
                     double rotH = (AT(s->Hy, i, j, k) - AT(s->Hy, i - 1, j, k)) * s->den_ex[i] +
                                   (AT(s->Hx, i, j - 1, k) - AT(s->Hx, i, j, k)) * s->den_ey[j];
                     const long p = PIDX(i, j, k);
-                    int no_medium = medium_polarization(s, MM(i, j, k, 2), &AT(s->PDz, i, j, k), s->PLz ? s->PLz + p : 0,
+                    int no_medium = medium_polarization(s->media, MM(i, j, k, 2), &AT(s->PDz, i, j, k), s->PLz ? s->PLz + p : 0,
                                                         s->PLz_old ? s->PLz_old + p : 0, pst, &AT(s->Ez, i, j, k), &AT(s->Ez_old, i, j, k), rotH);
                     if (no_medium) {
                         double Jav = 0.0;
@@ -1017,4 +1017,641 @@ int orc_num_threads(void) {
 #else
     return 1;
 #endif
+}
+
+/* ======================================================================== */
+/* 2D and 1D grids: TMxll_2D (mxll_2D_mod.f90) and TMxll_1D (mxll_1D_mod.f90),
+ * non-MPI.  Arrays keep the reference's bounds (third index 1..1); the 1D
+ * line is stored along the first index (Ex(1:nz), Hy(1:nz-1)), with Hy(0)
+ * and Hy(nz) kept at 0 for the Hy detector stencil at the two ends (the
+ * reference reads past its array there).  mode: 1 TMz, 2 TEz, 3 both (the
+ * reference's FULL mode allocates only the TEz arrays, mxll_2D_mod.f90:110-163,
+ * and cannot run; here both families run, each as in its own mode). */
+typedef struct {
+    int dir, n_ker, npts;
+    double j_amp, w0, t0, tau, t_init, t_final, phase;
+    int *ind;        /* npts*2 (1D: npts) */
+    double *ker, *jmat;
+} lsource;
+
+typedef struct orl_sim {
+    int dims, mode, nx, ny, npml, bc[2], cpml_pos[4];
+    double dt, dr, dt_eps0, dt_mu0;
+    arr3 Ex, Ey, Ez, Hx, Hy, Hz, Ex_old, Ey_old, Ez_old, eps_x, eps_y, eps_z, PDx, PDy, PDz;
+    double *PLx, *PLy, *PLz, *PLx_old, *PLy_old, *PLz_old;
+    arr3 psix_Ezx, psix_Eyx, psiy_Exy, psiy_Ezy, psix_Hyx, psix_Hzx, psiy_Hxy, psiy_Hzy;
+    double *den_ex, *den_ey, *den_hx, *den_hy, *be, *ce, *bh, *ch;
+    int *media_map, n_media, max_poles;
+    medium_co media[16];
+    int n_src; lsource src[16];
+    int n_det; int det_field[64], det_nd[64]; int* det_idx[64]; long n_samples;
+} orl_sim;
+
+/* CPML profiles of init_2Dgrid / init_1Dgrid (mxll_2D_mod.f90:270-300,
+ * mxll_1D_mod.f90:150-176), the same formulas as the 3D grid. */
+static void orl_profiles(orl_sim* s, double* kappae, double* kappah) {
+    const int npml = s->npml;
+    const double sec_to_au = 1.0 / 2.4188843265864e-17, C_to_au = 1.0 / 1.602176634e-19;
+    const double kg_to_au = 1.0 / 9.1093837139e-31, m_to_au = 1.0 / 5.29177210544e-11;
+    const double sig_to_au = C_to_au * C_to_au * sec_to_au / (kg_to_au * (m_to_au * m_to_au * m_to_au));
+    const double sigmaCPML = (double)(0.8f * (float)(3 + 1)) / (s->dr * sqrt(MU0 / EPS0));
+    double sig, al, kap;
+    for (int kk = 1; kk <= npml; ++kk) {
+        grade_e(npml, sigmaCPML, sig_to_au, kk, &sig, &al, &kap);
+        kappae[kk] = kap;
+        s->be[kk] = exp(-(sig / kap + al) * s->dt / EPS0);
+        if (sig == 0.0 && al == 0.0 && kk == npml) s->ce[kk] = 0.0;
+        else s->ce[kk] = sig * (s->be[kk] - 1.0) / (sig + kap * al) / kap;
+    }
+    for (int kk = 1; kk <= npml - 1; ++kk) {
+        grade_h(npml, sigmaCPML, sig_to_au, kk, &sig, &al, &kap);
+        kappah[kk] = kap;
+        s->bh[kk] = exp(-(sig / kap + al) * s->dt / EPS0);
+        s->ch[kk] = sig * (s->bh[kk] - 1.0) / (sig + kap * al) / kap;
+    }
+}
+
+/* init_2Dgrid (mxll_2D_mod.f90:177-420) / init_1Dgrid (mxll_1D_mod.f90:52-210). */
+orl_sim* orl_create(int dims, int mode, int n1, int n2, int npml, const int* bc, double dt, double dr) {
+    orl_sim* s = (orl_sim*)calloc(1, sizeof(orl_sim));
+    s->dims = dims; s->mode = dims == 1 ? 0 : mode; s->npml = npml;
+    s->nx = n1; s->ny = dims == 1 ? 1 : n2;
+    s->bc[0] = bc[0]; s->bc[1] = dims == 1 ? 1 : bc[1];
+    s->dt = dt; s->dr = dr; s->dt_eps0 = dt / EPS0; s->dt_mu0 = dt / MU0;
+    const int nx = s->nx, ny = s->ny;
+    s->den_ex = (double*)calloc(nx + 2, sizeof(double)); s->den_hx = (double*)calloc(nx + 2, sizeof(double));
+    s->den_ey = (double*)calloc(ny + 2, sizeof(double)); s->den_hy = (double*)calloc(ny + 2, sizeof(double));
+    for (int k = 1; k <= nx; ++k) s->den_ex[k] = s->den_hx[k] = 1.0 / dr;
+    for (int k = 1; k <= ny; ++k) s->den_ey[k] = s->den_hy[k] = 1.0 / dr;
+    const int np = npml > 2 ? npml : 2;
+    s->be = (double*)calloc(np + 1, sizeof(double)); s->ce = (double*)calloc(np + 1, sizeof(double));
+    s->bh = (double*)calloc(np + 1, sizeof(double)); s->ch = (double*)calloc(np + 1, sizeof(double));
+    double* kappae = (double*)calloc(np + 1, sizeof(double));
+    double* kappah = (double*)calloc(np + 1, sizeof(double));
+    if (dims == 1) {
+        const int nz = nx;
+        arr_init(&s->Ex, 1, nz, 1, 1, 1, 1); arr_init(&s->Ex_old, 1, nz, 1, 1, 1, 1);
+        arr_init(&s->Hy, 0, nz, 1, 1, 1, 1);
+        arr_init(&s->eps_x, 1, nz, 1, 1, 1, 1);
+        for (long q = 0; q < s->eps_x.n; ++q) s->eps_x.p[q] = 1.0;
+        arr_init(&s->PDx, 1, nz, 1, 1, 1, 1);
+        s->media_map = (int*)calloc((size_t)nz + 1, sizeof(int));
+        if (bc[0] == 3) {
+            s->cpml_pos[0] = s->cpml_pos[1] = 1;
+            arr_init(&s->psix_Ezx, 1, 2 * npml, 1, 1, 1, 1);        /* psiz_Exz */
+            arr_init(&s->psix_Hyx, 1, 2 * (npml - 1), 1, 1, 1, 1);  /* psiz_Hyz */
+            orl_profiles(s, kappae, kappah);
+            int kk = npml;
+            for (int k = 1; k <= nz - 1; ++k) {
+                if (k <= npml) s->den_ex[k] = 1.0 / (kappae[k] * dr);
+                else if (k >= nz + 1 - npml) { s->den_ex[k] = 1.0 / (kappae[kk] * dr); kk = kk - 1; }
+                else s->den_ex[k] = 1.0 / dr;
+            }
+            kk = npml - 1;
+            for (int k = 1; k <= nz - 1; ++k) {
+                if (k <= npml - 1) s->den_hx[k] = 1.0 / (kappah[k] * dr);
+                else if (k >= nz + 1 - npml) { s->den_hx[k] = 1.0 / (kappah[kk] * dr); kk = kk - 1; }
+                else s->den_hx[k] = 1.0 / dr;
+            }
+        }
+    } else {
+        const int te = mode & 2, tm = mode & 1;
+        if (te) {
+            arr_init(&s->Ex, 0, nx, 1, ny + 1, 1, 1); arr_init(&s->Ex_old, 0, nx, 1, ny + 1, 1, 1);
+            arr_init(&s->Ey, 1, nx + 1, 0, ny, 1, 1); arr_init(&s->Ey_old, 1, nx + 1, 0, ny, 1, 1);
+            arr_init(&s->Hz, 0, nx, 0, ny, 1, 1);
+        }
+        if (tm) {
+            arr_init(&s->Ez, 1, nx + 1, 1, ny + 1, 1, 1); arr_init(&s->Ez_old, 1, nx + 1, 1, ny + 1, 1, 1);
+            arr_init(&s->Hx, 1, nx, 0, ny, 1, 1);
+            arr_init(&s->Hy, 0, nx, 1, ny, 1, 1);
+        }
+        arr_init(&s->eps_x, 1, nx, 1, ny, 1, 1); arr_init(&s->eps_y, 1, nx, 1, ny, 1, 1); arr_init(&s->eps_z, 1, nx, 1, ny, 1, 1);
+        for (long q = 0; q < s->eps_x.n; ++q) s->eps_x.p[q] = s->eps_y.p[q] = s->eps_z.p[q] = 1.0;
+        arr_init(&s->PDx, 1, nx, 1, ny, 1, 1); arr_init(&s->PDy, 1, nx, 1, ny, 1, 1); arr_init(&s->PDz, 1, nx, 1, ny, 1, 1);
+        s->media_map = (int*)calloc((size_t)nx * ny * 3, sizeof(int));
+        s->cpml_pos[0] = s->cpml_pos[1] = bc[0] == 3;
+        s->cpml_pos[2] = s->cpml_pos[3] = bc[1] == 3;
+        const int n_sec = 2;
+        if (s->cpml_pos[0]) {
+            if (te) { arr_init(&s->psix_Eyx, 1, n_sec * npml, 1, ny, 1, 1); arr_init(&s->psix_Hzx, 1, n_sec * (npml - 1), 1, ny, 1, 1); }
+            if (tm) { arr_init(&s->psix_Hyx, 1, n_sec * (npml - 1), 1, ny, 1, 1); arr_init(&s->psix_Ezx, 1, n_sec * npml, 1, ny, 1, 1); }
+        }
+        if (s->cpml_pos[2]) {
+            if (te) { arr_init(&s->psiy_Exy, 1, nx, 1, n_sec * npml, 1, 1); arr_init(&s->psiy_Hzy, 1, nx, 1, n_sec * (npml - 1), 1, 1); }
+            if (tm) { arr_init(&s->psiy_Ezy, 1, nx, 1, n_sec * npml, 1, 1); arr_init(&s->psiy_Hxy, 1, nx, 1, n_sec * (npml - 1), 1, 1); }
+        }
+        if (s->cpml_pos[0] || s->cpml_pos[2]) {
+            orl_profiles(s, kappae, kappah);
+            const int n[2] = {nx, ny};
+            double* de[2] = {s->den_ex, s->den_ey};
+            double* dh[2] = {s->den_hx, s->den_hy};
+            for (int a = 0; a < 2; ++a) {
+                if (s->cpml_pos[2 * a]) {
+                    for (int k = 1; k <= n[a]; ++k) if (k <= npml) de[a][k] = 1.0 / (kappae[k] * dr);
+                    for (int k = 1; k <= n[a]; ++k) if (k <= npml - 1) dh[a][k] = 1.0 / (kappah[k] * dr);
+                }
+                if (s->cpml_pos[2 * a + 1]) {
+                    int kk = npml;
+                    for (int k = 1; k <= n[a] - 1; ++k) if (k >= n[a] + 1 - npml) { de[a][k] = 1.0 / (kappae[kk] * dr); kk = kk - 1; }
+                    kk = npml - 1;
+                    for (int k = 1; k <= n[a] - 1; ++k) if (k >= n[a] + 1 - npml) { dh[a][k] = 1.0 / (kappah[kk] * dr); kk = kk - 1; }
+                }
+            }
+        }
+    }
+    free(kappae); free(kappah);
+    s->max_poles = 1;
+    return s;
+}
+
+void orl_destroy(orl_sim* s) {
+    if (!s) return;
+    arr3* arrs[] = {&s->Ex, &s->Ey, &s->Ez, &s->Hx, &s->Hy, &s->Hz, &s->Ex_old, &s->Ey_old, &s->Ez_old, &s->eps_x,
+                    &s->eps_y, &s->eps_z, &s->PDx, &s->PDy, &s->PDz, &s->psix_Ezx, &s->psix_Eyx, &s->psiy_Exy,
+                    &s->psiy_Ezy, &s->psix_Hyx, &s->psix_Hzx, &s->psiy_Hxy, &s->psiy_Hzy};
+    for (size_t q = 0; q < sizeof(arrs) / sizeof(arrs[0]); ++q) free(arrs[q]->p);
+    free(s->PLx); free(s->PLy); free(s->PLz); free(s->PLx_old); free(s->PLy_old); free(s->PLz_old);
+    free(s->den_ex); free(s->den_ey); free(s->den_hx); free(s->den_hy);
+    free(s->be); free(s->ce); free(s->bh); free(s->ch); free(s->media_map);
+    for (int q = 0; q < s->n_src; ++q) { free(s->src[q].ind); free(s->src[q].ker); free(s->src[q].jmat); }
+    for (int q = 0; q < s->n_det; ++q) free(s->det_idx[q]);
+    free(s);
+}
+
+void orl_get_den(const orl_sim* s, double* out) {   /* den_ex(1:n1) den_hx den_ey(1:n2) den_hy */
+    long w = 0;
+    for (int k = 1; k <= s->nx; ++k) out[w++] = s->den_ex[k];
+    for (int k = 1; k <= s->nx; ++k) out[w++] = s->den_hx[k];
+    for (int k = 1; k <= s->ny; ++k) out[w++] = s->den_ey[k];
+    for (int k = 1; k <= s->ny; ++k) out[w++] = s->den_hy[k];
+}
+
+/* read_init_media_2D after the point lists are read (classical_medium_mod.f90:484-538)
+ * and read_init_media_1D (:203-337); aux is the cell medium map. */
+int orl_set_media(orl_sim* s, int n_media, const orc_medium* table, const int8_t* aux) {
+    const int nx = s->nx, ny = s->ny;
+    s->n_media = n_media;
+    for (int n = 1; n <= n_media; ++n) {
+        medium_coeffs(&table[n - 1], s->dt, &s->media[n]);
+        if (table[n - 1].kind == 3 && table[n - 1].n_poles > s->max_poles) s->max_poles = table[n - 1].n_poles;
+    }
+    if (s->dims == 1) {
+        /* dielectric cells: eps_x = 1/eps_r; the map takes each cell's medium */
+        for (int i = 1; i <= nx; ++i) {
+            const int n = aux[i - 1];
+            s->media_map[i] = n;
+            if (n && s->media[n].type == 1) AT(s->eps_x, i, 1, 1) = 1.0 / s->media[n].eps_r;
+        }
+    } else {
+#define AUX2(i, j) (((i) > nx || (j) > ny) ? 0 : (int)aux[((i) - 1) + (long)nx * ((j) - 1)])
+#define MAP2(i, j, c) s->media_map[((i) - 1) + (long)nx * (((j) - 1) + (long)ny * (c))]
+        for (int n = 1; n <= n_media; ++n)
+            for (int j = 1; j <= ny; ++j)
+                for (int i = 1; i <= nx; ++i) {
+                    if (i == nx && AUX2(i, j) == n) MAP2(i, j, 0) = n;
+                    else if (j == ny && AUX2(i, j) == n) MAP2(i, j, 0) = n;
+                    else {
+                        if (AUX2(i, j) == n && AUX2(i + 1, j) == n) MAP2(i, j, 0) = n;
+                        else if (AUX2(i, j) != 0 && AUX2(i + 1, j) == n) MAP2(i, j, 0) = n;
+                        if (AUX2(i, j) == n && AUX2(i, j + 1) == n) MAP2(i, j, 1) = n;
+                        else if (AUX2(i, j) != 0 && AUX2(i, j + 1) == n) MAP2(i, j, 1) = n;
+                        if (AUX2(i, j) == n) MAP2(i, j, 2) = n;
+                    }
+                }
+        for (int n = 1; n <= n_media; ++n) {
+            if (s->media[n].type != 1) continue;
+            for (int j = 1; j <= ny; ++j)
+                for (int i = 1; i <= nx; ++i)
+                    if (MAP2(i, j, 0) == n) {
+                        AT(s->eps_x, i, j, 1) = 1.0 / s->media[n].eps_r;
+                        AT(s->eps_y, i, j, 1) = 1.0 / s->media[n].eps_r;
+                        AT(s->eps_z, i, j, 1) = 1.0 / s->media[n].eps_r;
+                    }
+        }
+#undef AUX2
+    }
+    const size_t np = (size_t)nx * ny * s->max_poles;
+    s->PLx = (double*)calloc(np, sizeof(double)); s->PLy = (double*)calloc(np, sizeof(double));
+    s->PLz = (double*)calloc(np, sizeof(double)); s->PLx_old = (double*)calloc(np, sizeof(double));
+    s->PLy_old = (double*)calloc(np, sizeof(double)); s->PLz_old = (double*)calloc(np, sizeof(double));
+    return 0;
+}
+
+void orl_get_media_map(const orl_sim* s, int32_t* out) {
+    if (s->dims == 1) memcpy(out, s->media_map + 1, sizeof(int) * (size_t)s->nx);
+    else memcpy(out, s->media_map, sizeof(int) * (size_t)s->nx * s->ny * 3);
+}
+
+/* td_propagate_H_2Dfield (mxll_2D_mod.f90:450-636) / td_propagate_H_1Dfield
+ * (mxll_1D_mod.f90:252-293) */
+void orl_step_h(orl_sim* s) {
+    const int nx = s->nx, ny = s->ny, npml = s->npml;
+    if (s->dims == 1) {
+        const int nz = nx;
+        /* periodic: Ex(nz+1) = Ex(1) lands past Ex(1:nz) and is never read */
+        for (int i = 1; i <= nz - 1; ++i)
+            AT(s->Hy, i, 1, 1) = AT(s->Hy, i, 1, 1) + s->dt_mu0 * (AT(s->Ex, i, 1, 1) - AT(s->Ex, i + 1, 1, 1)) * s->den_hx[i];
+        if (s->bc[0] == 3) {
+            for (int i = 1; i <= npml - 1; ++i) {
+                AT(s->psix_Hyx, i, 1, 1) = s->bh[i] * AT(s->psix_Hyx, i, 1, 1) + s->ch[i] * (AT(s->Ex, i, 1, 1) - AT(s->Ex, i + 1, 1, 1)) / s->dr;
+                AT(s->Hy, i, 1, 1) = AT(s->Hy, i, 1, 1) + s->dt_mu0 * AT(s->psix_Hyx, i, 1, 1);
+            }
+            int kk = 2 * (npml - 1);
+            for (int i = nz + 1 - npml; i <= nz - 1; ++i) {
+                AT(s->psix_Hyx, kk, 1, 1) = s->bh[kk - (npml - 1)] * AT(s->psix_Hyx, kk, 1, 1) +
+                                            s->ch[kk - (npml - 1)] * (AT(s->Ex, i, 1, 1) - AT(s->Ex, i + 1, 1, 1)) / s->dr;
+                AT(s->Hy, i, 1, 1) = AT(s->Hy, i, 1, 1) + s->dt_mu0 * AT(s->psix_Hyx, kk, 1, 1);
+                kk = kk - 1;
+            }
+        }
+        return;
+    }
+    const int n_sec = 2, n_base = npml - 1;
+    if (s->mode & 1) {
+        if (s->bc[0] == 2) for (int j = 1; j <= ny + 1; ++j) AT(s->Ez, nx + 1, j, 1) = AT(s->Ez, 1, j, 1);
+        if (s->bc[1] == 2) for (int i = 1; i <= nx + 1; ++i) AT(s->Ez, i, ny + 1, 1) = AT(s->Ez, i, 1, 1);
+        for (int i = 1; i <= nx; ++i)
+            for (int j = 1; j <= ny; ++j) {
+                const double rotE = (AT(s->Ez, i, j, 1) - AT(s->Ez, i, j + 1, 1)) * s->den_hy[j];
+                AT(s->Hx, i, j, 1) = AT(s->Hx, i, j, 1) + s->dt_mu0 * rotE;
+            }
+        for (int i = 1; i <= nx; ++i)
+            for (int j = 1; j <= ny; ++j) {
+                const double rotE = (AT(s->Ez, i + 1, j, 1) - AT(s->Ez, i, j, 1)) * s->den_hx[i];
+                AT(s->Hy, i, j, 1) = AT(s->Hy, i, j, 1) + s->dt_mu0 * rotE;
+            }
+        if (s->cpml_pos[0])
+            for (int j = 1; j <= ny; ++j)
+                for (int i = 1; i <= npml - 1; ++i) {
+                    const double rotE = (AT(s->Ez, i + 1, j, 1) - AT(s->Ez, i, j, 1)) / s->dr;
+                    AT(s->psix_Hyx, i, j, 1) = s->bh[i] * AT(s->psix_Hyx, i, j, 1) + s->ch[i] * rotE;
+                    AT(s->Hy, i, j, 1) = AT(s->Hy, i, j, 1) + s->dt_mu0 * AT(s->psix_Hyx, i, j, 1);
+                }
+        if (s->cpml_pos[1])
+            for (int j = 1; j <= ny; ++j) {
+                int ii = n_sec * (npml - 1);
+                for (int i = nx + 1 - npml; i <= nx - 1; ++i) {
+                    const double rotE = (AT(s->Ez, i + 1, j, 1) - AT(s->Ez, i, j, 1)) / s->dr;
+                    AT(s->psix_Hyx, ii, j, 1) = s->bh[ii - n_base] * AT(s->psix_Hyx, ii, j, 1) + s->ch[ii - n_base] * rotE;
+                    AT(s->Hy, i, j, 1) = AT(s->Hy, i, j, 1) + s->dt_mu0 * AT(s->psix_Hyx, ii, j, 1);
+                    ii = ii - 1;
+                }
+            }
+        if (s->cpml_pos[2])
+            for (int i = 1; i <= nx; ++i)
+                for (int j = 1; j <= npml - 1; ++j) {
+                    const double rotE = (AT(s->Ez, i, j, 1) - AT(s->Ez, i, j + 1, 1)) / s->dr;
+                    AT(s->psiy_Hxy, i, j, 1) = s->bh[j] * AT(s->psiy_Hxy, i, j, 1) + s->ch[j] * rotE;
+                    AT(s->Hx, i, j, 1) = AT(s->Hx, i, j, 1) + s->dt_mu0 * AT(s->psiy_Hxy, i, j, 1);
+                }
+        if (s->cpml_pos[3])
+            for (int i = 1; i <= nx; ++i) {
+                int jj = n_sec * (npml - 1);
+                for (int j = ny + 1 - npml; j <= ny - 1; ++j) {
+                    const double rotE = (AT(s->Ez, i, j, 1) - AT(s->Ez, i, j + 1, 1)) / s->dr;
+                    AT(s->psiy_Hxy, i, jj, 1) = s->bh[jj - n_base] * AT(s->psiy_Hxy, i, jj, 1) + s->ch[jj - n_base] * rotE;
+                    AT(s->Hx, i, j, 1) = AT(s->Hx, i, j, 1) + s->dt_mu0 * AT(s->psiy_Hxy, i, jj, 1);
+                    jj = jj - 1;
+                }
+            }
+    }
+    if (s->mode & 2) {
+        if (s->bc[0] == 2) for (int j = 0; j <= ny; ++j) AT(s->Ey, nx + 1, j, 1) = AT(s->Ey, 1, j, 1);
+        if (s->bc[1] == 2) for (int i = 0; i <= nx; ++i) AT(s->Ex, i, ny + 1, 1) = AT(s->Ex, i, 1, 1);
+        for (int i = 1; i <= nx; ++i)
+            for (int j = 1; j <= ny; ++j) {
+                const double rotE = (AT(s->Ey, i, j, 1) - AT(s->Ey, i + 1, j, 1)) * s->den_hx[i] +
+                                    (AT(s->Ex, i, j + 1, 1) - AT(s->Ex, i, j, 1)) * s->den_hy[j];
+                AT(s->Hz, i, j, 1) = AT(s->Hz, i, j, 1) + s->dt_mu0 * rotE;
+            }
+        if (s->cpml_pos[0])
+            for (int j = 1; j <= ny; ++j)
+                for (int i = 1; i <= npml - 1; ++i) {
+                    const double rotE = (AT(s->Ey, i, j, 1) - AT(s->Ey, i + 1, j, 1)) / s->dr;
+                    AT(s->psix_Hzx, i, j, 1) = s->bh[i] * AT(s->psix_Hzx, i, j, 1) + s->ch[i] * rotE;
+                    AT(s->Hz, i, j, 1) = AT(s->Hz, i, j, 1) + s->dt_mu0 * AT(s->psix_Hzx, i, j, 1);
+                }
+        if (s->cpml_pos[1])
+            for (int j = 1; j <= ny; ++j) {
+                int ii = n_sec * (npml - 1);
+                for (int i = nx + 1 - npml; i <= nx - 1; ++i) {
+                    const double rotE = (AT(s->Ey, i, j, 1) - AT(s->Ey, i + 1, j, 1)) / s->dr;
+                    AT(s->psix_Hzx, ii, j, 1) = s->bh[ii - n_base] * AT(s->psix_Hzx, ii, j, 1) + s->ch[ii - n_base] * rotE;
+                    AT(s->Hz, i, j, 1) = AT(s->Hz, i, j, 1) + s->dt_mu0 * AT(s->psix_Hzx, ii, j, 1);
+                    ii = ii - 1;
+                }
+            }
+        if (s->cpml_pos[2])
+            for (int i = 1; i <= nx; ++i)
+                for (int j = 1; j <= npml - 1; ++j) {
+                    const double rotE = (AT(s->Ex, i, j + 1, 1) - AT(s->Ex, i, j, 1)) / s->dr;
+                    AT(s->psiy_Hzy, i, j, 1) = s->bh[j] * AT(s->psiy_Hzy, i, j, 1) + s->ch[j] * rotE;
+                    AT(s->Hz, i, j, 1) = AT(s->Hz, i, j, 1) + s->dt_mu0 * AT(s->psiy_Hzy, i, j, 1);
+                }
+        if (s->cpml_pos[3])
+            for (int i = 1; i <= nx; ++i) {
+                int jj = n_sec * (npml - 1);
+                for (int j = ny + 1 - npml; j <= ny - 1; ++j) {
+                    const double rotE = (AT(s->Ex, i, j + 1, 1) - AT(s->Ex, i, j, 1)) / s->dr;
+                    AT(s->psiy_Hzy, i, jj, 1) = s->bh[jj - n_base] * AT(s->psiy_Hzy, i, jj, 1) + s->ch[jj - n_base] * rotE;
+                    AT(s->Hz, i, j, 1) = AT(s->Hz, i, j, 1) + s->dt_mu0 * AT(s->psiy_Hzy, i, jj, 1);
+                    jj = jj - 1;
+                }
+            }
+    }
+}
+
+/* one E component of a cell: get_medium_polarization, else the dielectric /
+ * vacuum update with J = 0 (no quantum groups on these grids) */
+static double orl_e_cell(const orl_sim* s, int m, arr3* PD, double* PL, double* PLo, arr3* Eo, double e, double eps,
+                         double rotH, int i, int j) {
+    const long c = (i - 1) + (long)s->nx * (j - 1);
+    const long ps = (long)s->nx * s->ny;
+    double E = e;
+    if (medium_polarization(s->media, m, &AT(*PD, i, j, 1), PL + c, PLo + c, ps, &E, &AT(*Eo, i, j, 1), rotH))
+        E = E + s->dt_eps0 * eps * rotH - s->dt_eps0 * 0.0;
+    return E;
+}
+
+/* td_propagate_E_2Dfield (mxll_2D_mod.f90:638-864) / td_propagate_E_1Dfield
+ * (mxll_1D_mod.f90:296-350) */
+void orl_step_e(orl_sim* s) {
+    const int nx = s->nx, ny = s->ny, npml = s->npml;
+    if (s->dims == 1) {
+        const int nz = nx;
+        for (int i = 2; i <= nz - 1; ++i) {
+            const double rotH = (AT(s->Hy, i - 1, 1, 1) - AT(s->Hy, i, 1, 1)) * s->den_ex[i];
+            AT(s->Ex, i, 1, 1) = orl_e_cell(s, s->media_map[i], &s->PDx, s->PLx, s->PLx_old, &s->Ex_old, AT(s->Ex, i, 1, 1),
+                                            AT(s->eps_x, i, 1, 1), rotH, i, 1);
+        }
+        if (s->bc[0] == 3) {
+            for (int i = 2; i <= npml; ++i) {
+                const double rotH = (AT(s->Hy, i - 1, 1, 1) - AT(s->Hy, i, 1, 1)) / s->dr;
+                AT(s->psix_Ezx, i, 1, 1) = s->be[i] * AT(s->psix_Ezx, i, 1, 1) + s->ce[i] * rotH;
+                AT(s->Ex, i, 1, 1) = AT(s->Ex, i, 1, 1) + s->dt_eps0 * AT(s->psix_Ezx, i, 1, 1);
+            }
+            int kk = 2 * npml;
+            for (int i = nz + 1 - npml; i <= nz - 1; ++i) {
+                const double rotH = (AT(s->Hy, i - 1, 1, 1) - AT(s->Hy, i, 1, 1)) / s->dr;
+                AT(s->psix_Ezx, kk, 1, 1) = s->be[kk - npml] * AT(s->psix_Ezx, kk, 1, 1) + s->ce[kk - npml] * rotH;
+                AT(s->Ex, i, 1, 1) = AT(s->Ex, i, 1, 1) + s->dt_eps0 * AT(s->psix_Ezx, kk, 1, 1);
+                kk = kk - 1;
+            }
+        }
+        return;
+    }
+    const int n_sec = 2, n_base = npml;
+    const long mstride = (long)nx * ny;
+#define MM(i, j, c) s->media_map[((i) - 1) + (long)nx * ((j) - 1) + mstride * (c)]
+    if (s->mode & 1) {
+        if (s->bc[0] == 2) for (int j = 1; j <= ny; ++j) AT(s->Hy, 0, j, 1) = AT(s->Hy, nx, j, 1);
+        if (s->bc[1] == 2) for (int i = 1; i <= nx; ++i) AT(s->Hx, i, 0, 1) = AT(s->Hx, i, ny, 1);
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                const double rotH = (AT(s->Hy, i, j, 1) - AT(s->Hy, i - 1, j, 1)) * s->den_ex[i] +
+                                    (AT(s->Hx, i, j - 1, 1) - AT(s->Hx, i, j, 1)) * s->den_ey[j];
+                AT(s->Ez, i, j, 1) = orl_e_cell(s, MM(i, j, 2), &s->PDz, s->PLz, s->PLz_old, &s->Ez_old, AT(s->Ez, i, j, 1),
+                                                AT(s->eps_z, i, j, 1), rotH, i, j);
+            }
+        if (s->cpml_pos[0])
+            for (int j = 1; j <= ny; ++j)
+                for (int i = 2; i <= npml; ++i) {
+                    const double rotH = (AT(s->Hy, i, j, 1) - AT(s->Hy, i - 1, j, 1)) / s->dr;
+                    AT(s->psix_Ezx, i, j, 1) = s->be[i] * AT(s->psix_Ezx, i, j, 1) + s->ce[i] * rotH;
+                    AT(s->Ez, i, j, 1) = AT(s->Ez, i, j, 1) + s->dt_eps0 * AT(s->psix_Ezx, i, j, 1);
+                }
+        if (s->cpml_pos[1])
+            for (int j = 1; j <= ny; ++j) {
+                int ii = n_sec * npml;
+                for (int i = nx + 1 - npml; i <= nx - 1; ++i) {
+                    const double rotH = (AT(s->Hy, i, j, 1) - AT(s->Hy, i - 1, j, 1)) / s->dr;
+                    AT(s->psix_Ezx, ii, j, 1) = s->be[ii - n_base] * AT(s->psix_Ezx, ii, j, 1) + s->ce[ii - n_base] * rotH;
+                    AT(s->Ez, i, j, 1) = AT(s->Ez, i, j, 1) + s->dt_eps0 * AT(s->psix_Ezx, ii, j, 1);
+                    ii = ii - 1;
+                }
+            }
+        if (s->cpml_pos[2])
+            for (int i = 1; i <= nx; ++i)
+                for (int j = 2; j <= npml; ++j) {
+                    const double rotH = (AT(s->Hx, i, j - 1, 1) - AT(s->Hx, i, j, 1)) / s->dr;
+                    AT(s->psiy_Ezy, i, j, 1) = s->be[j] * AT(s->psiy_Ezy, i, j, 1) + s->ce[j] * rotH;
+                    AT(s->Ez, i, j, 1) = AT(s->Ez, i, j, 1) + s->dt_eps0 * AT(s->psiy_Ezy, i, j, 1);
+                }
+        if (s->cpml_pos[3])
+            for (int i = 1; i <= nx; ++i) {
+                int jj = n_sec * npml;
+                for (int j = ny + 1 - npml; j <= ny - 1; ++j) {
+                    const double rotH = (AT(s->Hx, i, j - 1, 1) - AT(s->Hx, i, j, 1)) / s->dr;
+                    AT(s->psiy_Ezy, i, jj, 1) = s->be[jj - n_base] * AT(s->psiy_Ezy, i, jj, 1) + s->ce[jj - n_base] * rotH;
+                    AT(s->Ez, i, j, 1) = AT(s->Ez, i, j, 1) + s->dt_eps0 * AT(s->psiy_Ezy, i, jj, 1);
+                    jj = jj - 1;
+                }
+            }
+    }
+    if (s->mode & 2) {
+        if (s->bc[0] == 2) for (int j = 0; j <= ny; ++j) AT(s->Hz, 0, j, 1) = AT(s->Hz, nx, j, 1);
+        if (s->bc[1] == 2) for (int i = 0; i <= nx; ++i) AT(s->Hz, i, 0, 1) = AT(s->Hz, i, ny, 1);
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                const double rotH = (AT(s->Hz, i, j, 1) - AT(s->Hz, i, j - 1, 1)) * s->den_ey[j];
+                AT(s->Ex, i, j, 1) = orl_e_cell(s, MM(i, j, 0), &s->PDx, s->PLx, s->PLx_old, &s->Ex_old, AT(s->Ex, i, j, 1),
+                                                AT(s->eps_x, i, j, 1), rotH, i, j);
+            }
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                const double rotH = (AT(s->Hz, i - 1, j, 1) - AT(s->Hz, i, j, 1)) * s->den_ex[i];
+                AT(s->Ey, i, j, 1) = orl_e_cell(s, MM(i, j, 1), &s->PDy, s->PLy, s->PLy_old, &s->Ey_old, AT(s->Ey, i, j, 1),
+                                                AT(s->eps_y, i, j, 1), rotH, i, j);
+            }
+        if (s->cpml_pos[0])
+            for (int j = 1; j <= ny; ++j)
+                for (int i = 2; i <= npml; ++i) {
+                    const double rotH = (AT(s->Hz, i - 1, j, 1) - AT(s->Hz, i, j, 1)) / s->dr;
+                    AT(s->psix_Eyx, i, j, 1) = s->be[i] * AT(s->psix_Eyx, i, j, 1) + s->ce[i] * rotH;
+                    AT(s->Ey, i, j, 1) = AT(s->Ey, i, j, 1) + s->dt_eps0 * AT(s->psix_Eyx, i, j, 1);
+                }
+        if (s->cpml_pos[1])
+            for (int j = 1; j <= ny; ++j) {
+                int ii = n_sec * npml;
+                for (int i = nx + 1 - npml; i <= nx - 1; ++i) {
+                    const double rotH = (AT(s->Hz, i - 1, j, 1) - AT(s->Hz, i, j, 1)) / s->dr;
+                    AT(s->psix_Eyx, ii, j, 1) = s->be[ii - n_base] * AT(s->psix_Eyx, ii, j, 1) + s->ce[ii - n_base] * rotH;
+                    AT(s->Ey, i, j, 1) = AT(s->Ey, i, j, 1) + s->dt_eps0 * AT(s->psix_Eyx, ii, j, 1);
+                    ii = ii - 1;
+                }
+            }
+        if (s->cpml_pos[2])
+            for (int i = 1; i <= nx; ++i)
+                for (int j = 2; j <= npml; ++j) {
+                    const double rotH = (AT(s->Hz, i, j, 1) - AT(s->Hz, i, j - 1, 1)) / s->dr;
+                    AT(s->psiy_Exy, i, j, 1) = s->be[j] * AT(s->psiy_Exy, i, j, 1) + s->ce[j] * rotH;
+                    AT(s->Ex, i, j, 1) = AT(s->Ex, i, j, 1) + s->dt_eps0 * AT(s->psiy_Exy, i, j, 1);
+                }
+        if (s->cpml_pos[3])
+            for (int i = 1; i <= nx; ++i) {
+                int jj = 2 * npml;
+                for (int j = ny + 1 - npml; j <= ny - 1; ++j) {
+                    const double rotH = (AT(s->Hz, i, j, 1) - AT(s->Hz, i, j - 1, 1)) / s->dr;
+                    AT(s->psiy_Exy, i, jj, 1) = s->be[jj - n_base] * AT(s->psiy_Exy, i, jj, 1) + s->ce[jj - n_base] * rotH;
+                    AT(s->Ex, i, j, 1) = AT(s->Ex, i, j, 1) + s->dt_eps0 * AT(s->psiy_Exy, i, jj, 1);
+                    jj = jj - 1;
+                }
+            }
+    }
+#undef MM
+}
+
+/* sources: compute_kernel 1D / 2D (source_mod.f90:251-300) and
+ * determine_indx_and_ranks (:340-346) */
+void orl_source_kernel(int dims, int n_ker, double dr, double* ker) {
+    const double aBH[4] = {0.353222222, -0.488, 0.145, -0.010222222};
+    const double r_max = (n_ker * 2 + 2) * dr, r0 = r_max / 2.0;
+    const int w = 2 * n_ker + 1;
+    double norm = 0.0;
+    if (dims == 1) {
+        for (int i = -n_ker; i <= n_ker; ++i) {
+            const double x = i * dr;
+            const double v = aBH[0] + aBH[1] * cos(2.0 * pi0 * (x - r0) / r_max) + aBH[2] * cos(2.0 * pi0 * 2.0 * (x - r0) / r_max) +
+                             aBH[3] * cos(2.0 * pi0 * 3.0 * (x - r0) / r_max);
+            ker[i + n_ker] = v;
+            norm = norm + v;
+        }
+        for (int q = 0; q < w; ++q) ker[q] = ker[q] / norm;
+        return;
+    }
+    for (int j = -n_ker; j <= n_ker; ++j)
+        for (int i = -n_ker; i <= n_ker; ++i) {
+            const double x = i * dr, y = j * dr;
+            const double r = sqrt(x * x + y * y);
+            double v = aBH[0] + aBH[1] * cos(2.0 * pi0 * (r - r0) / r_max) + aBH[2] * cos(2.0 * pi0 * 2.0 * (r - r0) / r_max) +
+                       aBH[3] * cos(2.0 * pi0 * 3.0 * (r - r0) / r_max);
+            if (v < 0.0) v = 0.0;
+            ker[(i + n_ker) + w * (j + n_ker)] = v;
+            norm = norm + v;
+        }
+    for (long q = 0; q < (long)w * w; ++q) ker[q] = ker[q] / norm;
+}
+
+int orl_add_source(orl_sim* s, int dir, double j_amp, double w0, double t0, double tau, double t_init, double t_final,
+                   double phase, int n_ker, const int* ind, const double* ker) {
+    lsource* q = &s->src[s->n_src++];
+    const int w = 2 * n_ker + 1;
+    const long np = s->dims == 1 ? w : (long)w * w;
+    q->dir = dir; q->n_ker = n_ker; q->npts = (int)np;
+    q->j_amp = j_amp; q->w0 = w0; q->t0 = t0; q->tau = tau; q->t_init = t_init; q->t_final = t_final; q->phase = phase;
+    q->ind = (int*)malloc(sizeof(int) * s->dims * np); memcpy(q->ind, ind, sizeof(int) * s->dims * np);
+    q->ker = (double*)malloc(sizeof(double) * np); memcpy(q->ker, ker, sizeof(double) * np);
+    q->jmat = (double*)calloc((size_t)np, sizeof(double));
+    return 0;
+}
+
+/* update_source (source_mod.f90:106-132) + source_interactions 1D / 2D
+ * (interactions_mod.f90:32-100); J_mat one step past the kernel reads 0. */
+void orl_inject_sources(orl_sim* s, double time) {
+    const double c_src = s->dt_eps0 / s->dr / c0 / 2.0;
+    for (int q = 0; q < s->n_src; ++q) {
+        lsource* S = &s->src[q];
+        const int n = S->n_ker, w = 2 * n + 1;
+        const long np = S->npts;
+        if (time >= S->t_init && time <= S->t_final) {
+            const double x = (time - S->t0) / S->tau;
+            const double envelope = exp(-(x * x));
+            const double cos_t = cos(S->w0 * (time - S->t0) + S->phase);
+            for (long p = 0; p < np; ++p) S->jmat[p] = S->j_amp * envelope * cos_t * S->ker[p];
+        } else {
+            for (long p = 0; p < np; ++p) S->jmat[p] = 0.0;
+        }
+        if (s->dims == 1) {
+            for (int i = -n; i <= n; ++i) {
+                const int ii = S->ind[i + n];
+                AT(s->Ex, ii, 1, 1) = AT(s->Ex, ii, 1, 1) + c_src * S->jmat[i + n];
+            }
+            continue;
+        }
+        for (int i = -n; i <= n; ++i)
+            for (int j = -n; j <= n; ++j) {
+                const long p = (i + n) + w * (long)(j + n);
+                const int ii = S->ind[p], jj = S->ind[p + np];
+                if (S->dir == 1) {
+                    const double jb = i + 1 <= n ? S->jmat[p + 1] : 0.0;
+                    const double J_av = 0.5 * (S->jmat[p] + jb);
+                    AT(s->Ex, ii, jj, 1) = AT(s->Ex, ii, jj, 1) + c_src * J_av;
+                } else if (S->dir == 2) {
+                    const double jb = j + 1 <= n ? S->jmat[p + w] : 0.0;
+                    const double J_av = 0.5 * (S->jmat[p] + jb);
+                    AT(s->Ey, ii, jj, 1) = AT(s->Ey, ii, jj, 1) + c_src * J_av;
+                } else {
+                    const double J_av = S->jmat[p];
+                    AT(s->Ez, ii, jj, 1) = AT(s->Ez, ii, jj, 1) + c_src * J_av;
+                }
+            }
+    }
+}
+
+/* detectors: write_1D_field / write_2D_field stencils (write_fields_subs_mod.f90:88-628) */
+int orl_add_detector(orl_sim* s, int field, int nd, const int* idx) {
+    const int q = s->n_det++;
+    s->det_field[q] = field; s->det_nd[q] = nd;
+    s->det_idx[q] = (int*)malloc(sizeof(int) * s->dims * (nd > 0 ? nd : 1));
+    memcpy(s->det_idx[q], idx, sizeof(int) * s->dims * nd);
+    s->n_samples += nd;
+    return 0;
+}
+
+void orl_sample(const orl_sim* s, double* out) {
+    long w = 0;
+    for (int q = 0; q < s->n_det; ++q) {
+        const int nd = s->det_nd[q];
+        for (int n = 0; n < nd; ++n) {
+            const int i = s->det_idx[q][n], j = s->dims == 1 ? 1 : s->det_idx[q][n + nd];
+            double v = 0.0;
+            if (s->dims == 1) {
+                v = s->det_field[q] == 1 ? AT(s->Ex, i, 1, 1) : 0.5 * (AT(s->Hy, i - 1, 1, 1) + AT(s->Hy, i, 1, 1));
+            } else {
+                switch (s->det_field[q]) {
+                    case 1: v = 0.5 * (AT(s->Ex, i - 1, j, 1) + AT(s->Ex, i, j, 1)); break;
+                    case 2: v = 0.5 * (AT(s->Ey, i, j - 1, 1) + AT(s->Ey, i, j, 1)); break;
+                    case 3: v = AT(s->Ez, i, j, 1); break;
+                    case 4: v = 0.5 * (AT(s->Hx, i, j, 1) + AT(s->Hx, i, j - 1, 1)); break;
+                    case 5: v = 0.5 * (AT(s->Hy, i, j, 1) + AT(s->Hy, i - 1, j, 1)); break;
+                    case 6: v = 0.25 * (AT(s->Hz, i, j, 1) + AT(s->Hz, i - 1, j - 1, 1) + AT(s->Hz, i - 1, j, 1) + AT(s->Hz, i, j - 1, 1)); break;
+                }
+            }
+            out[w++] = v;
+        }
+    }
+}
+
+long orl_n_samples(const orl_sim* s) { return s->n_samples; }
+
+/* the driver loop's step order (mxim_mxll.f90:92-132) */
+void orl_run(orl_sim* s, int tt_first, int n_steps, int record_every, double* rec, int* rows) {
+    int r = 0;
+    for (int tt = tt_first; tt < tt_first + n_steps; ++tt) {
+        const double time = s->dt * ((double)tt - 0.5);
+        orl_step_h(s);
+        orl_inject_sources(s, time);
+        orl_step_e(s);
+        if (record_every > 0 && tt % record_every == 0 && s->n_samples > 0) {
+            orl_sample(s, rec + (long)r * s->n_samples);
+            ++r;
+        }
+    }
+    if (rows) *rows = r;
+}
+
+/* copy one field with the reference's bounds, Fortran order (to_sim: host -> oracle) */
+void orl_copy_field(orl_sim* s, int field, double* host, int to_sim) {
+    arr3* a[6] = {&s->Ex, &s->Ey, &s->Ez, &s->Hx, &s->Hy, &s->Hz};
+    arr3* f = a[field - 1];
+    if (s->dims == 1 && field == 5) {   /* Hy(1:nz-1) */
+        for (int k = 1; k <= s->nx - 1; ++k) {
+            if (to_sim) AT(*f, k, 1, 1) = host[k - 1];
+            else host[k - 1] = AT(*f, k, 1, 1);
+        }
+        return;
+    }
+    if (to_sim) memcpy(f->p, host, sizeof(double) * f->n);
+    else memcpy(host, f->p, sizeof(double) * f->n);
 }

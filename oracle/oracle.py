@@ -3,8 +3,9 @@
 by __graft_entry__.smoke() and by bench.py's cpu_baseline leg, always as the
 checker, never as the thing measured or shipped.
 
-OracleGrid mirrors the method names of omx_fdtd.Grid3D so that a parity test
-drives both with the same calls.
+OracleGrid mirrors the method names of omx_fdtd.Grid3D, and OracleLowGrid
+those of omx_fdtd.Grid2D / Grid1D, so that a parity test drives both with
+the same calls.
 """
 import ctypes as C
 import os
@@ -62,6 +63,21 @@ def load():
             "orc_n_samples": ([P], C.c_long),
             "orc_run": ([P, I, I, I, _DP, _IP], V),
             "orc_num_threads": ([], I),
+            "orl_create": ([I, I, I, I, I, _IP, D, D], P),
+            "orl_destroy": ([P], V),
+            "orl_get_den": ([P, _DP], V),
+            "orl_set_media": ([P, I, C.POINTER(OrcMedium), C.POINTER(C.c_int8)], I),
+            "orl_get_media_map": ([P, _IP], V),
+            "orl_step_h": ([P], V),
+            "orl_step_e": ([P], V),
+            "orl_source_kernel": ([I, I, D, _DP], V),
+            "orl_add_source": ([P, I, D, D, D, D, D, D, D, I, _IP, _DP], I),
+            "orl_inject_sources": ([P, D], V),
+            "orl_add_detector": ([P, I, I, _IP], I),
+            "orl_sample": ([P, _DP], V),
+            "orl_n_samples": ([P], C.c_long),
+            "orl_run": ([P, I, I, I, _DP, _IP], V),
+            "orl_copy_field": ([P, I, _DP, I], V),
         }
         for name, (args, res) in sig.items():
             f = getattr(lib, name)
@@ -278,6 +294,122 @@ class OracleGrid:
         rows = C.c_int32()
         self.lib.orc_run(self.h, int(tt_first), int(n_steps), int(record_every), _d(rec), C.byref(rows))
         return rec[:rows.value, :ns]
+
+
+class OracleLowGrid:
+    """The 2D (dims 2, mode 1 TMz / 2 TEz / 3 both) or 1D (dims 1) grid of
+    omx_oracle.c, with the method names of omx_fdtd.Grid2D / Grid1D."""
+
+    def __init__(self, dims, grid, npml, boundaries, dt, dr, mode=0):
+        self.lib = load()
+        self.dims, self.mode = int(dims), int(mode)
+        self.grid = tuple(int(v) for v in grid)[:self.dims]
+        self.npml = int(npml)
+        self.dt, self.dr = float(dt), float(dr)
+        bc = np.ascontiguousarray(list(boundaries)[:self.dims] + [1] * (2 - self.dims), dtype=np.int32)
+        n2 = self.grid[1] if self.dims == 2 else 1
+        self.h = self.lib.orl_create(self.dims, self.mode, self.grid[0], n2, self.npml, _i(bc), self.dt, self.dr)
+
+    def kill(self):
+        if getattr(self, "h", None):
+            self.lib.orl_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.kill()
+        except Exception:
+            pass
+
+    def den(self):
+        """den_ex, den_hx (1..n1), den_ey, den_hy (1..n2) as one array."""
+        n1 = self.grid[0]
+        n2 = self.grid[1] if self.dims == 2 else 1
+        out = np.zeros(2 * (n1 + n2))
+        self.lib.orl_get_den(self.h, _d(out))
+        return out
+
+    def set_media(self, media, aux):
+        arr = (OrcMedium * max(1, len(media)))(*[to_orc_medium(m) if not isinstance(m, OrcMedium) else m
+                                                  for m in media])
+        a = np.ascontiguousarray(np.asarray(aux, dtype=np.int8).ravel(order="F"))
+        self.lib.orl_set_media(self.h, len(media), arr, a.ctypes.data_as(C.POINTER(C.c_int8)))
+
+    def media_map(self):
+        if self.dims == 1:
+            m = np.zeros(self.grid[0], dtype=np.int32)
+            self.lib.orl_get_media_map(self.h, _i(m))
+            return m
+        m = np.zeros(self.grid[0] * self.grid[1] * 3, dtype=np.int32)
+        self.lib.orl_get_media_map(self.h, _i(m))
+        return m.reshape(self.grid + (3,), order="F")
+
+    def td_propagate_H_field(self):
+        self.lib.orl_step_h(self.h)
+
+    def td_propagate_E_field(self, t=0):
+        self.lib.orl_step_e(self.h)
+
+    def add_source(self, direction, j_amp, w0, t0, tau, t_init, t_final, phase, n_ker, ind, ker):
+        dirc = {"x": 1, "y": 2, "z": 3}.get(direction, direction)
+        ind = np.ascontiguousarray(np.asarray(ind).ravel(order="F"), dtype=np.int32)
+        ker = np.ascontiguousarray(np.asarray(ker).ravel(order="F"), dtype=np.float64)
+        self.lib.orl_add_source(self.h, int(dirc), j_amp, w0, t0, tau, t_init, t_final, phase, int(n_ker),
+                                _i(ind), _d(ker))
+
+    def inject_sources(self, time):
+        self.lib.orl_inject_sources(self.h, float(time))
+
+    def add_detector(self, field, cells):
+        f = {"Ex": 1, "Ey": 2, "Ez": 3, "Hx": 4, "Hy": 5, "Hz": 6}.get(field, field)
+        idx = np.asarray(cells, dtype=np.int32).reshape(-1, self.dims)
+        flat = np.ascontiguousarray(idx.ravel(order="F"))
+        self.lib.orl_add_detector(self.h, int(f), idx.shape[0], _i(flat))
+
+    @property
+    def n_samples(self):
+        return int(self.lib.orl_n_samples(self.h))
+
+    def sample(self):
+        out = np.zeros(max(self.n_samples, 1))
+        self.lib.orl_sample(self.h, _d(out))
+        return out[:self.n_samples]
+
+    def run(self, tt_first, n_steps, record_every=0):
+        ns = self.n_samples
+        nrow = 0
+        if record_every > 0:
+            nrow = sum(1 for tt in range(tt_first, tt_first + n_steps) if tt % record_every == 0)
+        rec = np.zeros((max(nrow, 1), max(ns, 1)))
+        rows = C.c_int32()
+        self.lib.orl_run(self.h, int(tt_first), int(n_steps), int(record_every), _d(rec), C.byref(rows))
+        return rec[:rows.value, :ns]
+
+    def _shape(self, f):
+        if self.dims == 1:
+            return (self.grid[0],) if f == 1 else (self.grid[0] - 1,)
+        nx, ny = self.grid
+        return {1: (nx + 1, ny + 1), 2: (nx + 1, ny + 1), 3: (nx + 1, ny + 1),
+                4: (nx, ny + 1), 5: (nx + 1, ny), 6: (nx + 1, ny + 1)}[f]
+
+    def get_field(self, name):
+        f = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz").index(name) + 1
+        shp = self._shape(f)
+        out = np.zeros(int(np.prod(shp)))
+        self.lib.orl_copy_field(self.h, f, _d(out), 0)
+        return out.reshape(shp, order="F")
+
+    def set_field(self, name, arr):
+        f = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz").index(name) + 1
+        flat = np.ascontiguousarray(np.asarray(arr, dtype=np.float64).ravel(order="F"))
+        self.lib.orl_copy_field(self.h, f, _d(flat), 1)
+
+
+def source_kernel_lowdim(dims, n_ker, dr):
+    w = 2 * n_ker + 1
+    ker = np.zeros(w ** dims)
+    load().orl_source_kernel(int(dims), int(n_ker), float(dr), _d(ker))
+    return ker if dims == 1 else ker.reshape((w, w), order="F")
 
 
 def rasterize(shapes, nx, ny, nz):
