@@ -39,8 +39,8 @@ def build_engine():
 
 
 FC = os.environ.get("OMX_FC", "/opt/rocm/bin/amdflang")
-F_SRC = ["omx_fdtd_cabi.f90", "omx_mxll_base.f90", "omx_mxll_3d_hip.f90", "omx_factory.f90", "omx_detector_io.f90",
-         "omx_qgroup_io.f90", "omx_fdtd_main.f90"]
+F_SRC = ["omx_fdtd_cabi.f90", "omx_mxll_base.f90", "omx_mxll_3d_hip.f90", "omx_mxll_lowdim_hip.f90", "omx_factory.f90",
+         "omx_detector_io.f90", "omx_qgroup_io.f90", "omx_fdtd_main.f90"]
 
 
 def build_fortran():

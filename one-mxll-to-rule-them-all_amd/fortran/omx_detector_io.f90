@@ -13,6 +13,13 @@
 !                 (write_fields_subs_mod.f90:631-748), then list-directed
 !                 "coordinates value" rows (:751-1509).
 !
+! 2D grids (dims 2) take point, line_x, line_y and plane_xy detectors at
+! (x, y) (detector_mod.f90:239-371) and write <field>_line_ / <field>_plane_xy_
+! files (write_fields_subs_mod.f90:197-628); 1D grids (dims 1) take point and
+! line_z detectors at z along their one axis (:207-238) and write Ex_line_ /
+! Hy_line_z_ files (:16-193).  A 2D point.dat header prints z = 0 (the
+! reference prints a k index init_detector never sets, output_mod.f90:133-137).
+!
 ! Rows come either from the engine's detector samples (write_detector_rows)
 ! or, for a caller that holds a TMxll_3D mirror, from the mirror with the
 ! reference's averaging stencils (write_3D_field_mirror).  One domain (no
@@ -36,6 +43,7 @@ module omx_detector_io_mod
         integer :: nd = 0                         ! cells inside the grid
         integer, allocatable :: indx_list(:, :)   ! (nd, 3), i fastest
         integer :: first = 0                      ! first of its values in the engine's sample vector
+        integer :: dims = 3                       ! dimensions of the grid it samples
     end type TDetectorOut
 
 
@@ -50,14 +58,15 @@ contains
         coord = idx * dr - int(n / 2) * dr
     end function coord
 
-    subroutine read_detectors(n_det, grid, dr, dets)
+    subroutine read_detectors(n_det, grid, dr, dets, dims)
         integer,  intent(in) :: n_det, grid(3)
         real(dp), intent(in) :: dr
         type(TDetectorOut), allocatable, intent(out) :: dets(:)
+        integer, intent(in), optional :: dims
         character(len=10) :: type_ch
         character(len=4)  :: field_ch
         real(dp) :: v(6)
-        integer :: funit, ierr, n, lo(3), hi(3), a, q, ii, jj, kk
+        integer :: funit, ierr, n, lo(3), hi(3), a, q, ii, jj, kk, nd_axes
 
         allocate(dets(n_det))
         open (newunit=funit, file="detectors.in", action="read", iostat=ierr)
@@ -93,16 +102,31 @@ contains
                 error stop 1
             end select
             dets(n)%f_ch = fnames(dets(n)%field)
-            do a = 1, 3
-                lo(a) = int(v(2 * a - 1) / dr) + int(grid(a) / 2)
-                hi(a) = int(v(2 * a) / dr) + int(grid(a) / 2)
-            end do
+            nd_axes = 3
+            if (present(dims)) nd_axes = dims
+            dets(n)%dims = nd_axes
+            call check_lowdim(nd_axes, dets(n))
+            if (nd_axes == 1) then
+                ! the 1D line is the z axis: its cells come from z (detector_mod.f90:207-238)
+                lo = 1; hi = 1
+                lo(1) = int(v(5) / dr) + int(grid(1) / 2)
+                hi(1) = int(v(6) / dr) + int(grid(1) / 2)
+            else
+                do a = 1, nd_axes
+                    lo(a) = int(v(2 * a - 1) / dr) + int(grid(a) / 2)
+                    hi(a) = int(v(2 * a) / dr) + int(grid(a) / 2)
+                end do
+                if (nd_axes == 2) then
+                    lo(3) = 1; hi(3) = 1
+                end if
+            end if
             ! a point, line or plane spans its first corner along the other axes
             select case (dets(n)%detector_type)
             case (POINT_DETECTOR);    hi = lo
             case (LINE_X_DETECTOR);   hi(2:3) = lo(2:3)
             case (LINE_Y_DETECTOR);   hi(1) = lo(1); hi(3) = lo(3)
-            case (LINE_Z_DETECTOR);   hi(1:2) = lo(1:2)
+            case (LINE_Z_DETECTOR)
+                if (nd_axes == 3) hi(1:2) = lo(1:2)
             case (PLANE_XY_DETECTOR); hi(3) = lo(3)
             case (PLANE_YZ_DETECTOR); hi(1) = lo(1)
             case (PLANE_ZX_DETECTOR); hi(2) = lo(2)
@@ -110,9 +134,12 @@ contains
             dets(n)%i_min = lo(1); dets(n)%i_max = hi(1)
             dets(n)%j_min = lo(2); dets(n)%j_max = hi(2)
             dets(n)%k_min = lo(3); dets(n)%k_max = hi(3)
+            if (nd_axes == 1) then
+                dets(n)%k_min = lo(1); dets(n)%k_max = hi(1)
+            end if
             do a = 1, 3
                 lo(a) = max(lo(a), 1)
-                hi(a) = min(hi(a), grid(a))
+                if (a <= nd_axes) hi(a) = min(hi(a), grid(a))
             end do
             dets(n)%nd = max(0, hi(1) - lo(1) + 1) * max(0, hi(2) - lo(2) + 1) * max(0, hi(3) - lo(3) + 1)
             allocate(dets(n)%indx_list(dets(n)%nd, 3))
@@ -128,6 +155,30 @@ contains
         end do
         close (funit)
     end subroutine read_detectors
+
+    ! Detector types and fields a 1D or 2D grid cannot take (detector_mod.f90:86-201).
+    subroutine check_lowdim(dims, det)
+        integer, intent(in) :: dims
+        type(TDetectorOut), intent(in) :: det
+        if (dims == 1) then
+            if (det%field /= Ex_FIELD .and. det%field /= Hy_FIELD) then
+                write (*, '("Error: ",A," field cannot be detected in 1D")') det%f_ch
+                write (*, '("Use Ex or Hy fields instead")')
+                error stop 1
+            end if
+            if (det%detector_type /= POINT_DETECTOR .and. det%detector_type /= LINE_Z_DETECTOR) then
+                write (*, '("Error: invalid combination of detector type and dimensions")')
+                error stop 1
+            end if
+        else if (dims == 2) then
+            select case (det%detector_type)
+            case (POINT_DETECTOR, LINE_X_DETECTOR, LINE_Y_DETECTOR, PLANE_XY_DETECTOR)
+            case default
+                write (*, '("Error: invalid combination of detector type and dimensions")')
+                error stop 1
+            end select
+        end if
+    end subroutine check_lowdim
 
     ! The detectors' directories and the point detectors' point.dat headers.
     subroutine init_detector_files(dets, grid, dr)
@@ -153,9 +204,19 @@ contains
                 write (*, *) "Error: Could not open file for writing: ", trim(dname) // "/point.dat"
                 error stop 1
             end if
-            x = coord(dets(n)%i_min, grid(1), dr) * au_to_nm
-            y = coord(dets(n)%j_min, grid(2), dr) * au_to_nm
-            z = coord(dets(n)%k_min, grid(3), dr) * au_to_nm
+            select case (dets(n)%dims)
+            case (1)
+                x = 0.0d0; y = 0.0d0
+                z = coord(dets(n)%k_min, grid(1), dr) * au_to_nm
+            case (2)
+                x = coord(dets(n)%i_min, grid(1), dr) * au_to_nm
+                y = coord(dets(n)%j_min, grid(2), dr) * au_to_nm
+                z = 0.0d0
+            case default
+                x = coord(dets(n)%i_min, grid(1), dr) * au_to_nm
+                y = coord(dets(n)%j_min, grid(2), dr) * au_to_nm
+                z = coord(dets(n)%k_min, grid(3), dr) * au_to_nm
+            end select
             write (funit, '("# Time (a.u.)              ' // dets(n)%f_ch // &
                           ' (a.u.),      x = ", F10.5, " nm,   y = ", F10.5, " nm,   z = ", F10.5, " nm")') x, y, z
             close (funit)
@@ -167,7 +228,7 @@ contains
         type(TDetectorOut), intent(in) :: det
         character(len=99) :: name
         character(len=20) :: number, pnum
-        character(len=8)  :: kind_ch
+        character(len=10) :: kind_ch
         write (number, '(I7.7)') n
         write (pnum, '(I9.9)') print_step
         select case (det%detector_type)
@@ -175,6 +236,8 @@ contains
         case (VOLUME_DETECTOR); kind_ch = "_volume_"
         case default; kind_ch = "_plane_"
         end select
+        if (det%dims == 2 .and. det%detector_type == PLANE_XY_DETECTOR) kind_ch = "_plane_xy_"
+        if (det%dims == 1 .and. det%field == Hy_FIELD) kind_ch = "_line_z_"
         name = dir0 // "_" // trim(number) // "/" // det%f_ch // trim(kind_ch) // trim(pnum) // ".dat"
     end function step_file
 
@@ -198,6 +261,12 @@ contains
         hx = 0.0d0; hy = 0.0d0; hz = 0.0d0; hv1 = 0.0d0; hv2 = 0.0d0
         hw2 = achar(0)
 
+        if (size(dets) > 0) then
+            if (dets(1)%dims /= 3) then
+                call write_lowdim_headers(dets, print_step, time, grid, dr)
+                return
+            end if
+        end if
         do n = 1, size(dets)
             select case (dets(n)%detector_type)
             case (LINE_X_DETECTOR, LINE_Y_DETECTOR, LINE_Z_DETECTOR)
@@ -246,6 +315,48 @@ contains
         end do
     end subroutine write_detector_headers
 
+    ! write_2D_headers (write_fields_subs_mod.f90:197-276) and write_1D_header
+    ! (:16-85): this print step's line and plane files with their two
+    ! header lines.
+    subroutine write_lowdim_headers(dets, print_step, time, grid, dr)
+        type(TDetectorOut), intent(in) :: dets(:)
+        integer,  intent(in) :: print_step, grid(3)
+        real(dp), intent(in) :: time, dr
+        character(len=1) :: ch, cw
+        real(dp) :: cv
+        integer :: n, funit, ierr
+        do n = 1, size(dets)
+            select case (dets(n)%detector_type)
+            case (LINE_X_DETECTOR, LINE_Y_DETECTOR)
+                if (dets(n)%detector_type == LINE_X_DETECTOR) then
+                    ch = "y"; cw = "x"
+                    cv = coord(dets(n)%j_min, grid(2), dr) * au_to_nm
+                else
+                    ch = "x"; cw = "y"
+                    cv = coord(dets(n)%i_min, grid(1), dr) * au_to_nm
+                end if
+                open (newunit=funit, file=trim(step_file(n, dets(n), print_step)), status="replace", &
+                      action="write", iostat=ierr)
+                write (funit, '("# Time = ", ES18.8, " (a.u.), ' // ch // ' = ", F12.6, " (nm)")') time, cv
+                write (funit, '("# ' // cw // ' (nm)                   ' // trim(dets(n)%f_ch) // ' (a.u.)")')
+                close (funit)
+            case (PLANE_XY_DETECTOR)
+                open (newunit=funit, file=trim(step_file(n, dets(n), print_step)), status="replace", &
+                      action="write", iostat=ierr)
+                write (funit, '("# Time = ", ES18.8, " (a.u.)")') time
+                write (funit, '("# x (nm)                   y (nm)                   ' // trim(dets(n)%f_ch) // &
+                              ' (a.u.)")')
+                close (funit)
+            case (LINE_Z_DETECTOR)
+                open (newunit=funit, file=trim(step_file(n, dets(n), print_step)), status="replace", &
+                      action="write", iostat=ierr)
+                write (funit, '("# Time = ", ES18.8, " (a.u.)")') time
+                write (funit, '("# z (nm)                   ' // dets(n)%f_ch // ' (a.u.)")')
+                close (funit)
+            end select
+        end do
+    end subroutine write_lowdim_headers
+
     ! One row per cell of the detectors that see the grid, after the
     ! headers of this print step: point.dat gets "time value", the others
     ! "coordinates value" (the coordinates the reference writes for each
@@ -258,6 +369,7 @@ contains
         x = coord(det%indx_list(q, 1), grid(1), dr) * au_to_nm
         y = coord(det%indx_list(q, 2), grid(2), dr) * au_to_nm
         z = coord(det%indx_list(q, 3), grid(3), dr) * au_to_nm
+        if (det%dims == 1) z = x
         select case (det%detector_type)
         case (POINT_DETECTOR);    write (funit, *) time, val
         case (LINE_X_DETECTOR);   write (funit, *) x, val

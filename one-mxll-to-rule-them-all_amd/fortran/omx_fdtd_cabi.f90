@@ -16,6 +16,13 @@ module omx_fdtd_cabi
         real(c_double)     :: dt, dr
     end type omx3d_desc
 
+    type, bind(C) :: omxl_desc
+        integer(c_int32_t) :: dims, mode, n1, n2, npml
+        integer(c_int32_t) :: boundaries(2)
+        integer(c_int32_t) :: precision
+        real(c_double)     :: dt, dr
+    end type omxl_desc
+
     type, bind(C) :: omx_medium
         integer(c_int32_t) :: kind = 0, n_poles = 0
         real(c_double)     :: eps_r = 1.0d0, omega_d = 0.0d0, gamma_d = 0.0d0
@@ -143,6 +150,66 @@ module omx_fdtd_cabi
             import :: c_ptr, c_size_t
             type(c_ptr), value :: s
             integer(c_size_t) :: n
+        end function
+        ! 2D / 1D grids
+        integer(c_int) function omxl_create(desc, ctx) bind(C, name="omxl_create")
+            import :: c_int, c_ptr, omxl_desc
+            type(omxl_desc), intent(in) :: desc
+            type(c_ptr), intent(out) :: ctx
+        end function
+        integer(c_int) function omxl_destroy(ctx) bind(C, name="omxl_destroy")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function omxl_set_media(ctx, n, table, aux) bind(C, name="omxl_set_media")
+            import :: c_int, c_ptr, c_int32_t, c_int8_t, omx_medium
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: n
+            type(omx_medium), intent(in) :: table(*)
+            integer(c_int8_t), intent(in) :: aux(*)
+        end function
+        integer(c_int) function omxl_step_h(ctx) bind(C, name="omxl_step_h")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function omxl_step_e(ctx) bind(C, name="omxl_step_e")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function omxl_add_source(ctx, dir, j_amp, w0, t0, tau, t_init, t_final, phase, &
+                                                n_ker, ind, ker) bind(C, name="omxl_add_source")
+            import :: c_int, c_ptr, c_double, c_int32_t
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: dir, n_ker
+            real(c_double), value :: j_amp, w0, t0, tau, t_init, t_final, phase
+            integer(c_int32_t), intent(in) :: ind(*)
+            real(c_double), intent(in) :: ker(*)
+        end function
+        integer(c_int) function omxl_inject_sources(ctx, time) bind(C, name="omxl_inject_sources")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), value :: time
+        end function
+        integer(c_int) function omxl_add_detector(ctx, field, nd, idx) bind(C, name="omxl_add_detector")
+            import :: c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: field, nd
+            integer(c_int32_t), intent(in) :: idx(*)
+        end function
+        integer(c_int) function omxl_sample(ctx, out) bind(C, name="omxl_sample")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: out(*)
+        end function
+        integer(c_int) function omxl_copy_field(ctx, field, host, to_device) bind(C, name="omxl_copy_field")
+            import :: c_int, c_ptr, c_double, c_int32_t
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: field, to_device
+            real(c_double), intent(inout) :: host(*)
+        end function
+        integer(c_int) function omxl_synchronize(ctx) bind(C, name="omxl_synchronize")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
         end function
     end interface
 
