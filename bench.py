@@ -57,10 +57,15 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+PATCH_RADIUS = None   # --patch-radius: FP64 near-field box radius of FP32 grids (None: the engine default)
+
+
 def build_grid(case, precision, mode="separate"):
     dr, dt = case.dr_dt
     g = omx_fdtd.Grid3D(case.grid, case.npml, case.boundaries, dt, dr, precision=precision)
     g.set_step_mode(1 if mode == "separate" else 0)
+    if PATCH_RADIUS is not None:
+        g.set_source_patch(PATCH_RADIUS)
     cases.apply(case, g, library=G.library_medium, make_medium=G.medium)
     return g
 
@@ -272,7 +277,9 @@ def base_record(workload, K, W, value, ms_step, dtype, config):
     return {"metric": METRIC if workload == HEADLINE else f"Gcell-updates/s, FP32 3D FDTD ({workload})",
             "value": round(value, 4), "unit": "Gcell-updates/s", "n_gpus": 1, "steps": K, "warmup": W,
             "ms_per_step": round(ms_step, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-            "dtype": dtype, "data": "synthetic", "config": dict({"workload": workload}, **config)}
+            "dtype": dtype, "data": "synthetic",
+            "config": dict({"workload": workload}, **config,
+                           **({} if PATCH_RADIUS is None else {"patch_radius": PATCH_RADIUS}))}
 
 
 def precisions(args):
@@ -551,9 +558,13 @@ def main():
     ap.add_argument("--mode", choices=("separate", "fused"), default="separate",
                     help="separate H and E kernels, or the fused H+E pass")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default=HEADLINE)
+    ap.add_argument("--patch-radius", type=int, default=None,
+                    help="FP64 near-field box radius around point sources of FP32 grids (0: off)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (HBM counter calibration)")
     args = ap.parse_args()
+    global PATCH_RADIUS
+    PATCH_RADIUS = args.patch_radius
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return

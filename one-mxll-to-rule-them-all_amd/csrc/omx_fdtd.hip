@@ -245,6 +245,9 @@ E_kernel<T> pick_e(bool cp, bool md, bool b32, bool tf) {
 // clipped to the cells outside the CPML faces and off the outer planes
 // (2..n-1), then merged while two boxes overlap or touch.
 // ---------------------------------------------------------------------------
+constexpr long kPatchAlign = 256;
+inline long patch_padded(long n) { return (n + kPatchAlign - 1) / kPatchAlign * kPatchAlign; }
+
 void free_patches(omx3d_ctx* c) {
     c->device_bytes -= c->patch_bytes;
     c->patch_bytes = 0;
@@ -329,8 +332,16 @@ int build_patches_impl(omx3d_ctx* c) {
             }
     }
     if (bx.empty()) return OMX_OK;
+    // each box starts on a multiple of kPatchAlign cells (the patch kernels'
+    // workgroup size), in the storage and in the launch's flat index
     long cells = 0;
-    for (const Bx& b : bx) cells += (long)(b.hi[0] - b.lo[0] + 1) * (b.hi[1] - b.lo[1] + 1) * (b.hi[2] - b.lo[2] + 1);
+    for (const Bx& b : bx) {
+        const long n = (long)(b.hi[0] - b.lo[0] + 1) * (b.hi[1] - b.lo[1] + 1) * (b.hi[2] - b.lo[2] + 1);
+        if (n > INT32_MAX - kPatchAlign)   // the patch kernels index a box's cells in 32 bits
+            return fail(OMX_ERR_RANGE, "near-field patch box of " + std::to_string(n) +
+                                           " cells: lower the patch radius or set it to 0");
+        cells += patch_padded(n);
+    }
     int rc = dalloc(c, (void**)&c->d_patch, sizeof(double) * 6 * cells);
     if (rc) return rc;
     c->patch_cells = cells;
@@ -363,7 +374,7 @@ int build_patches_impl(omx3d_ctx* c) {
         B.dt_eps0 = dt_eps0;
         B.g = 0;
         c->h_boxes.push_back(B);
-        first += (long)B.w[0] * B.w[1] * B.w[2];
+        first += patch_padded((long)B.w[0] * B.w[1] * B.w[2]);
     }
     // cells of Drude / Drude-Lorentz classes in the boxes: their position in
     // the dispersive list, and an FP64 copy of their ADE state
@@ -1301,7 +1312,7 @@ int batch_setup(omxb_ctx* b) {
             for (PatchBox B : b->g[q]->h_boxes) {
                 B.g = q;
                 B.first = first;
-                first += (long)B.w[0] * B.w[1] * B.w[2];
+                first += patch_padded((long)B.w[0] * B.w[1] * B.w[2]);
                 pb.push_back(B);
             }
         b->n_pbox = (int)pb.size();
@@ -2332,7 +2343,11 @@ int omx3d_get_source_patch(omx3d_ctx* c, int32_t* radius, int32_t* n_boxes, int6
     }
     if (radius) *radius = c->patch_radius;
     if (n_boxes) *n_boxes = (int32_t)c->h_boxes.size();
-    if (n_cells) *n_cells = c->patch_cells;
+    if (n_cells) {   // the boxes' cells (the storage pads each box to kPatchAlign)
+        long n = 0;
+        for (const PatchBox& B : c->h_boxes) n += (long)B.w[0] * B.w[1] * B.w[2];
+        *n_cells = n;
+    }
     return OMX_OK;
 }
 

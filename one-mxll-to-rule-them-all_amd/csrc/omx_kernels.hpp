@@ -1679,20 +1679,23 @@ __device__ __forceinline__ double patch_val(const PatchBox& B, const double* sh,
     return (double)f[(long)i * a.sx + (long)j * a.sz + k + a.off];
 }
 
-__device__ __forceinline__ void patch_cell(const PatchBox& B, long q, int& i, int& j, int& k, long& r) {
-    r = q - B.first;
-    k = B.lo[2] + (int)(r % B.w[2]);
-    j = B.lo[1] + (int)((r / B.w[2]) % B.w[1]);
-    i = B.lo[0] + (int)(r / ((long)B.w[2] * B.w[1]));
+// Cell (i,j,k) of box cell r (k fastest); 32-bit divisions by the box's
+// uniform extents.
+__device__ __forceinline__ void patch_cell(const PatchBox& B, int r, int& i, int& j, int& k) {
+    const unsigned w2 = (unsigned)B.w[2], w1 = (unsigned)B.w[1];
+    const unsigned t = (unsigned)r / w2;
+    const unsigned u = t / w1;
+    k = B.lo[2] + (int)((unsigned)r - t * w2);
+    j = B.lo[1] + (int)(t - u * w1);
+    i = B.lo[0] + (int)u;
 }
 
 // H of a box cell from the E of a.ex/a.ey/a.ez (E^n) into the shadow and
 // a.hx/a.hy/a.hz.
 template <typename T>
-__device__ __forceinline__ void patch_h_cell(const YeeArgs<T>& a, const PatchBox& B, long q) {
+__device__ __forceinline__ void patch_h_cell(const YeeArgs<T>& a, const PatchBox& B, int r) {
     int i, j, k;
-    long r;
-    patch_cell(B, q, i, j, k, r);
+    patch_cell(B, r, i, j, k);
     const double* ex = B.pe;
     const double* ey = B.pe + B.cs;
     const double* ez = B.pe + 2 * B.cs;
@@ -1762,10 +1765,9 @@ __device__ __forceinline__ double patch_medium(const YeeArgs<T>& a, const PatchB
 // and a.ex/a.ey/a.ez; a cell of a dispersive class keeps the FP32 result of
 // the dispersive update and its shadow takes that value.
 template <typename T>
-__device__ __forceinline__ void patch_e_cell(const YeeArgs<T>& a, const PatchBox& B, long q) {
+__device__ __forceinline__ void patch_e_cell(const YeeArgs<T>& a, const PatchBox& B, int r) {
     int i, j, k;
-    long r;
-    patch_cell(B, q, i, j, k, r);
+    patch_cell(B, r, i, j, k);
     const long o = (long)i * a.sx + (long)j * a.sz + k + a.off;
     const int tg = a.tag ? (int)a.tag[o] : 0;
     const int dp = tg >= a.disp_tag0 && B.dl ? B.dl[r] : -1;
@@ -1806,10 +1808,9 @@ __device__ __forceinline__ void patch_e_cell(const YeeArgs<T>& a, const PatchBox
 // Shadows from the FP32 fields (set-up, and after fields were written from
 // the host).
 template <typename T>
-__device__ __forceinline__ void patch_sync_cell(const YeeArgs<T>& a, const PatchBox& B, long q) {
+__device__ __forceinline__ void patch_sync_cell(const YeeArgs<T>& a, const PatchBox& B, int r) {
     int i, j, k;
-    long r;
-    patch_cell(B, q, i, j, k, r);
+    patch_cell(B, r, i, j, k);
     const long o = (long)i * a.sx + (long)j * a.sz + k + a.off;
     B.pe[r] = (double)a.ex[o]; B.pe[B.cs + r] = (double)a.ey[o]; B.pe[2 * B.cs + r] = (double)a.ez[o];
     B.ph[r] = (double)a.hx[o]; B.ph[B.cs + r] = (double)a.hy[o]; B.ph[2 * B.cs + r] = (double)a.hz[o];
@@ -1820,17 +1821,22 @@ __device__ __forceinline__ void patch_sync_cell(const YeeArgs<T>& a, const Patch
                 B.ds[((long)c * B.nv + v) * B.cs + r] = (double)a.dstate[c * a.dcomp + (long)v * a.pstride + dp];
 }
 
-// phase 0: H, 1: E, 2: shadows from the fields.  One grid: the argument
-// block by value; ncell: cells of all boxes.
+// phase 0: H, 1: E, 2: shadows from the fields.  Every box starts on a
+// multiple of the workgroup size (256) in the launch's flat index, so a
+// workgroup's box is found once, from uniform values; the padding threads
+// past a box's cells return.  One grid: the argument block by value; ncell:
+// the padded cells of all boxes.
 template <typename T>
 __global__ __launch_bounds__(256) void k_patch(const YeeArgs<T> a, const PatchBox* __restrict__ boxes, int nbox,
                                                long ncell, int phase) {
-    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= ncell) return;
-    const PatchBox& B = boxes[find_box(boxes, nbox, q)];
-    if (phase == 0) patch_h_cell(a, B, q);
-    else if (phase == 1) patch_e_cell(a, B, q);
-    else patch_sync_cell(a, B, q);
+    const long q0 = (long)blockIdx.x * blockDim.x;
+    if (q0 >= ncell) return;
+    const PatchBox& B = boxes[find_box(boxes, nbox, q0)];
+    const int r = (int)(q0 - B.first) + (int)threadIdx.x;
+    if (r >= B.w[0] * B.w[1] * B.w[2]) return;
+    if (phase == 0) patch_h_cell(a, B, r);
+    else if (phase == 1) patch_e_cell(a, B, r);
+    else patch_sync_cell(a, B, r);
 }
 
 // The boxes of every grid of a batch: box B belongs to grid B.g.
@@ -1838,13 +1844,15 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_patch_batched(const YeeArgs<T>* __restrict__ args,
                                                        const PatchBox* __restrict__ boxes, int nbox, long ncell,
                                                        int phase) {
-    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= ncell) return;
-    const PatchBox& B = boxes[find_box(boxes, nbox, q)];
+    const long q0 = (long)blockIdx.x * blockDim.x;
+    if (q0 >= ncell) return;
+    const PatchBox& B = boxes[find_box(boxes, nbox, q0)];
+    const int r = (int)(q0 - B.first) + (int)threadIdx.x;
+    if (r >= B.w[0] * B.w[1] * B.w[2]) return;
     const YeeArgs<T>& a = args[B.g];
-    if (phase == 0) patch_h_cell(a, B, q);
-    else if (phase == 1) patch_e_cell(a, B, q);
-    else patch_sync_cell(a, B, q);
+    if (phase == 0) patch_h_cell(a, B, r);
+    else if (phase == 1) patch_e_cell(a, B, r);
+    else patch_sync_cell(a, B, r);
 }
 
 // ---------------------------------------------------------------------------
