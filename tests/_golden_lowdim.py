@@ -11,7 +11,7 @@ from omx_fdtd import cases, lowdim
 from omx_fdtd.units import grid_dr_dt
 
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-NAMES = ("tmz2d", "tez2d", "tmz2d_periodic", "line1d", "line1d_periodic")
+NAMES = ("tmz2d", "tez2d", "tmz2d_periodic", "tez2d_ycpml", "line1d", "line1d_periodic", "line1d_close")
 # fields of one snapshot, in the order the reference driver wrote them
 SNAP_FIELDS = {1: ("Ez", "Hx", "Hy"), 2: ("Ex", "Ey", "Hz"), 0: ("Ex", "Hy")}
 

@@ -69,6 +69,17 @@ def golden_cases():
                  "7.5 0.0 1.0e6 0.0"],
         detectors=[(f, c) for c in ((12, 18), (16, 18), (2, 5), (39, 34), (40, 36)) for f in ("Ez", "Hx", "Hy")])
 
+    g = (44, 50)
+    out["tez2d_ycpml"] = dict(
+        dims=2, mode=2, grid=g, npml=6, bc=(1, 3), dx=5.0, steps=240, record_every=2, field_every=120,
+        media=[("drude", "9.0 0.07 1.5", lambda I, J: (I >= 20) & (I <= 26) & (J >= 22) & (J <= 28)),
+               ("dielectric", "3.0", lambda I, J: (I >= 8) & (I <= 14) & (J >= 10) & (J <= 40))],
+        sources=[f"y 1.0 1.8 0.4 0.2 {index_to_nm(30, 5.0, g[0])!r} {index_to_nm(16, 5.0, g[1])!r} 0.0 "
+                 "7.5 0.0 1.0e6 0.0",
+                 f"x 0.7 2.4 0.5 0.2 {index_to_nm(18, 5.0, g[0])!r} {index_to_nm(36, 5.0, g[1])!r} 0.0 "
+                 "0.0 0.0 1.0e6 0.5"],
+        detectors=[(f, c) for c in ((22, 25), (11, 20), (30, 3), (40, 48), (44, 50)) for f in ("Ex", "Ey", "Hz")])
+
     n = 400
     out["line1d"] = dict(
         dims=1, grid=(n,), npml=12, bc=(3,), dx=2.0, steps=1500, record_every=1, field_every=750,
@@ -84,6 +95,13 @@ def golden_cases():
         media=[("drude", "6.0 0.2 1.0", lambda K: (K >= 60) & (K <= 70))],
         sources=[f"x 1.0 2.0 0.3 0.1 {index_to_nm(100, 5.0, n)!r} 0.0 0.0 3.0 0.0 1.0e6 0.2"],
         detectors=[(f, (k,)) for k in (2, 50, 65, 150, 199) for f in ("Ex", "Hy")])
+
+    n = 150
+    out["line1d_close"] = dict(
+        dims=1, grid=(n,), npml=4, bc=(1,), dx=5.0, steps=500, record_every=1, field_every=250,
+        media=[("dielectric", "2.25", lambda K: (K >= 90) & (K <= 120))],
+        sources=[f"x 1.0 2.0 0.3 0.1 {index_to_nm(60, 5.0, n)!r} 0.0 0.0 0.0 0.0 1.0e6 0.0"],
+        detectors=[(f, (k,)) for k in (20, 100, 149) for f in ("Ex", "Hy")])
     return out
 
 

@@ -59,7 +59,7 @@ def point_series(d, n_det):
     return np.array(cols).T
 
 
-@pytest.mark.parametrize("name", ["tmz2d", "tez2d", "tmz2d_periodic", "line1d"])
+@pytest.mark.parametrize("name", ["tmz2d", "tez2d", "tmz2d_periodic", "tez2d_ycpml", "line1d", "line1d_close"])
 def test_driver_deck_matches_reference(name):
     assert os.path.exists(EXE), "omx_fdtd_3d not built"
     desc, fx = G.load(name)
