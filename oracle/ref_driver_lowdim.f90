@@ -23,7 +23,9 @@
 ! read_init_media_2D assigns them (:525-535).
 !
 ! Input (current directory): case.nml (namelist &case) and the reference's
-! own medium_XXXX.in, sources.in and detectors.in.
+! own medium_XXXX.in, sources.in and detectors.in.  With timing /= 0 the
+! driver writes no files and prints the wall time of its step loop (the CPU
+! baseline of the 2D / 1D bench workloads).
 ! Output: coeffs.bin (be ce bh ch, then den_ex den_ey den_hx den_hy, or 1D
 ! den_ez den_hz), media.bin (media_map), fields.bin (the mode's field arrays,
 ! whole, at every step with mod(tt, field_every) == 0), and the reference's
@@ -37,15 +39,16 @@ program omx_ref_driver_lowdim
     use source_mod
     use detector_mod
     use write_fields_subs_mod
+    use omp_lib
 
     implicit none
 
     integer  :: dims = 2, mode = 1, n1 = 32, n2 = 32, npml = 4, bc(3) = 3
-    integer  :: n_steps = 10, record_every = 1, field_every = 0, n_media = 0, n_src = 0, n_det = 0
+    integer  :: n_steps = 10, record_every = 1, field_every = 0, n_media = 0, n_src = 0, n_det = 0, timing = 0
     real(dp) :: dx_nm = 5.0d0
     real(dp) :: diel_eps(15) = 1.0d0
     namelist /case/ dims, mode, n1, n2, npml, bc, n_steps, record_every, field_every, n_media, n_src, n_det, dx_nm, &
-                    diel_eps
+                    diel_eps, timing
 
     type(TMxll_2D)               :: m2
     type(TMxll_1D)               :: m1
@@ -53,6 +56,7 @@ program omx_ref_driver_lowdim
     type(TDetector), allocatable :: detectors(:)
     integer  :: grid_Ndims(3), mpi_coords(3), mpi_dims(3)
     integer  :: tt, n, i, j, funit, ffield, print_det_step
+    integer(8) :: c0_clock, c1_clock, rate
     real(dp) :: dr, dt, time
     real(dp) :: x_min, x_max, y_min, y_max, z_min, z_max
     character(len=4)  :: field_name
@@ -98,6 +102,11 @@ program omx_ref_driver_lowdim
     end if
 
     ! ---- golden tables ----------------------------------------------------
+    if (timing /= 0) then
+        n_det = 0
+        field_every = 0
+    end if
+    if (timing == 0) then
     open(newunit=funit, file="coeffs.bin", access="stream", form="unformatted", status="replace")
     if (dims == 1) then
         if (allocated(m1%be)) write(funit) m1%be(1:npml), m1%ce(1:npml), m1%bh(1:npml-1), m1%ch(1:npml-1)
@@ -114,6 +123,7 @@ program omx_ref_driver_lowdim
         write(funit) m2%media_map
     end if
     close(funit)
+    end if
 
     ! ---- point detectors (output_mod.f90:40-168) ----------------------------
     if (n_det > 0) then
@@ -136,8 +146,9 @@ program omx_ref_driver_lowdim
         end do
     end if
 
-    open(newunit=ffield, file="fields.bin", access="stream", form="unformatted", status="replace")
+    if (timing == 0) open(newunit=ffield, file="fields.bin", access="stream", form="unformatted", status="replace")
     print_det_step = 0
+    call system_clock(c0_clock, rate)
     do tt = 1, n_steps
         time = dt * (DBLE(tt) - 0.5d0)
         if (dims == 1) then
@@ -178,8 +189,11 @@ program omx_ref_driver_lowdim
             end if
         end if
     end do
-    close(ffield)
+    call system_clock(c1_clock)
+    if (timing == 0) close(ffield)
     write(*, '("REF_LOWDIM dims=", I0, " mode=", I0, " steps=", I0)') dims, mode, n_steps
+    write(*, '("REF_TIMING steps=", I0, " seconds=", ES16.8, " threads=", I0, " cells=", I0)') &
+        n_steps, dble(c1_clock - c0_clock) / dble(rate), omp_get_max_threads(), n1 * merge(n2, 1, dims == 2)
 
 contains
 
