@@ -226,7 +226,7 @@ def roofline(case, g, prec, res, workload):
     b_h, b_e, b_disp, _ = algorithmic_bytes(case, g, prec)
     dtype = "fp32" if prec == 4 else "fp64"
     if res["fused"]:
-        kname, b_dom, ms_dom = "k_step_fused", b_h + b_e - b_disp, res["ms_e"]
+        kname, b_dom, ms_dom = "k_step_wg", b_h + b_e - b_disp, res["ms_e"]
     elif res["ms_e"] >= res["ms_h"]:
         kname, b_dom, ms_dom = "k_e_update", b_e, res["ms_e"]
     else:

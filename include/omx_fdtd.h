@@ -247,11 +247,17 @@ int omx3d_get_tfsf_line(omx3d_ctx* ctx, double* ex1, double* hy1, int32_t* kb, i
 int omx3d_run(omx3d_ctx* ctx, int32_t tt_first, int32_t n_steps, int32_t record_every);
 /* Step kernels of omx3d_run: mode 1 (default) runs the separate H and E
  * kernels; mode 0 runs H and E as one fused pass over HBM (fields
- * double-buffered) wherever it applies -- no periodic axis and no source
- * point in a Drude/DL component.  rows (1, 2 or 4 rows of 64 z cells per
- * wavefront) and xchunk (planes per wavefront) set the fused tile; 0 keeps
- * the current value.  No reference counterpart. */
+ * double-buffered) wherever it applies -- no periodic axis, no source point
+ * in a Drude/DL component, and a z extent one workgroup covers (nz <= 1024
+ * in FP32, 512 in FP64).  A fused workgroup holds every z cell of a block of
+ * rows and marches a chunk of planes; rows (1 or 2 rows of 64 z cells per
+ * wavefront) and xchunk (planes per workgroup) set its tile; 0 keeps the
+ * current value.  No reference counterpart. */
 int omx3d_set_step_mode(omx3d_ctx* ctx, int32_t mode, int32_t rows, int32_t xchunk);
+/* Wavefronts along y of a fused workgroup (1..16, at most 16 (FP32) or 8
+ * (FP64) wavefronts in all); 0 chooses them from the z extent.  No
+ * reference counterpart. */
+int omx3d_set_fused_waves(omx3d_ctx* ctx, int32_t ny_waves);
 /* 1 if the last omx3d_run used the fused step. */
 int omx3d_get_step_mode(omx3d_ctx* ctx, int32_t* fused_last_run);
 /* Per-kernel timing: HIP events around the H and E launches of every

@@ -691,8 +691,22 @@ int check_sources_dispersive(omx3d_ctx* c) {
     return OMX_OK;
 }
 
+// Wavefronts along z and y of a k_step_wg workgroup for this grid (nzw = 0:
+// the z extent does not fit one workgroup).  FP32 takes up to 16 wavefronts,
+// FP64 up to 8 (LDS of the hand-overs, 64 KiB at most).
+void wg_shape(const omx3d_ctx* c, int& nzw, int& nyw) {
+    nzw = (c->d.nz + 63) / 64;
+    const int maxw = c->prec == 4 ? 16 : 8;
+    if (nzw > maxw) { nzw = 0; nyw = 0; return; }
+    nyw = c->wg_ny > 0 ? c->wg_ny : std::max(1, maxw / nzw);
+    nyw = std::max(1, std::min(nyw, maxw / nzw));
+}
+
 bool fused_applies(omx3d_ctx* c) {
     if (c->step_mode != 0) return false;
+    int nzw, nyw;
+    wg_shape(c, nzw, nyw);
+    if (nzw == 0) return false;
     for (int a = 0; a < 3; ++a)
         if (c->d.boundaries[a] == OMX_BC_PERIODIC) return false;
     // the fused kernel addresses field and z-face psi arrays by 32-bit byte
@@ -708,83 +722,89 @@ bool fused_applies(omx3d_ctx* c) {
 }
 
 template <typename T, int R, int CP>
-void launch_fused_kernel(omx3d_ctx* c, bool md, bool tf, unsigned nb, const YeeArgs<T>& a, const FusedOut<T>& fo) {
-#define OMX_FUSED(MD, TF) \
-    hipLaunchKernelGGL((k_step_fused<T, R, CP, MD, TF>), dim3(nb), dim3(256), 0, c->s, a, fo)
-    if (md && tf) OMX_FUSED(1, true);
-    else if (md) OMX_FUSED(1, false);
-    else if (tf) OMX_FUSED(0, true);
-    else OMX_FUSED(0, false);
-#undef OMX_FUSED
+void launch_wg_kernel(omx3d_ctx* c, bool md, bool tf, const YeeArgs<T>& a, const FusedOut<T>& fo, const WgPlan& P,
+                      hipStream_t st) {
+    const size_t lds = wg_lds_bytes(P.nzw * P.nyw, R, sizeof(T), tf);
+    const dim3 blk(64 * P.nzw * P.nyw);
+    const unsigned nb = (unsigned)P.box[P.nbox - 1].b_end;
+#define OMX_WG(MD, TF) hipLaunchKernelGGL((k_step_wg<T, R, CP, MD, TF>), dim3(nb), blk, lds, st, a, fo, P)
+    if (md && tf) OMX_WG(1, true);
+    else if (md) OMX_WG(1, false);
+    else if (tf) OMX_WG(0, true);
+    else OMX_WG(0, false);
+#undef OMX_WG
 }
 
 template <typename T, int CP>
-void launch_fused_rows(omx3d_ctx* c, bool md, bool tf, unsigned nb, const YeeArgs<T>& a, const FusedOut<T>& fo) {
-    if (c->fused_rows == 4) launch_fused_kernel<T, 4, CP>(c, md, tf, nb, a, fo);
-    else if (c->fused_rows == 2) launch_fused_kernel<T, 2, CP>(c, md, tf, nb, a, fo);
-    else launch_fused_kernel<T, 1, CP>(c, md, tf, nb, a, fo);
+void launch_wg_rows(omx3d_ctx* c, bool md, bool tf, const YeeArgs<T>& a, const FusedOut<T>& fo, const WgPlan& P,
+                    hipStream_t st) {
+    if (c->wg_rows == 2) launch_wg_kernel<T, 2, CP>(c, md, tf, a, fo, P, st);
+    else launch_wg_kernel<T, 1, CP>(c, md, tf, a, fo, P, st);
 }
 
-// The fused step's regions.  Planes xa..xb and rows ya..yb hold no cell of
-// an x or y CPML face (nor, one plane / row below, a halo cell of one); z
-// tiles tza..tzb-1 likewise for the z faces.  Three launches over disjoint
-// tiles: that interior without CPML code, its z-face tiles with the z-face
-// code only, and the x and y slabs with all of it (so the few CPML cells do
-// not cost the interior its occupancy).  The slabs are thin, so their
-// wavefronts march short chunks (kSlabChunk planes) to give the launch
-// enough wavefronts to fill the chip.
-struct FusedPlan {
-    FusedOut<double>::Box box[3][4];   // per launch (CPML 0, 4, 7)
-    int nbox[3];
-};
-
-FusedPlan fused_plan(const omx3d_ctx* c) {
-    FusedPlan P;
+// Boxes of the fused step's launch, heaviest first: the x slabs holding the
+// x CPML faces (planes 1..npml, nx-npml+1..nx; chunks of 4 planes), the y
+// slabs (whole row blocks covering rows 1..npml and ny-npml+1..ny of the
+// other planes; half chunks), then the interior (full chunks).  A slab tile
+// carries the x/y psi, up to twice the bytes of an interior one; listed
+// first, the slab tiles start first and the interior ones fill in behind
+// them.  Without x or y CPML faces the interior is the whole grid.
+WgPlan wg_plan(const omx3d_ctx* c) {
+    WgPlan P;
     std::memset(&P, 0, sizeof(P));
-    const int R = c->fused_rows, X = c->fused_xchunk, np = c->d.npml;
-    const int nx = c->L.nx, ny = c->L.ny, nz = c->L.nz;
+    wg_shape(c, P.nzw, P.nyw);
+    const int R = c->wg_rows, H = P.nyw * R;
+    const int nx = c->d.nx, ny = c->d.ny, np = c->d.npml;
     const unsigned m = c->cpml_mask;
-    const int ntz = (nz + kLanes - 1) / kLanes;
-    int xa = (m & 1u) ? np + 1 : 1, xb = (m & 2u) ? nx - np : nx;
-    int ya = (m & 4u) ? np + 1 : 1, yb = (m & 8u) ? ny - np : ny;
-    if (xa > xb || ya > yb) { xa = 1; xb = 0; ya = 1; yb = 0; }   // no interior: all of it slabs
-    // z tiles k0..k0+63 (k0 = 64 tz + 1) free of z faces, k0-1 included
-    int tza = 0, tzb = ntz;
-    if (m & 16u) while (tza < ntz && 64 * tza + 1 < np + 1) ++tza;
-    if (m & 32u) while (tzb > tza && 64 * (tzb - 1) + 64 > nz - np) --tzb;
-    constexpr int kSlabChunk = 4;
-    auto add = [&](int l, int i_lo, int i_hi, int j_lo, int j_hi, int tz_lo, int tz_hi) {
-        const int xc = l == 2 ? std::min(X, kSlabChunk) : X;
-        if (i_lo > i_hi || j_lo > j_hi || tz_lo >= tz_hi) return;
-        FusedOut<double>::Box& B = P.box[l][P.nbox[l]];
-        const int prev = P.nbox[l] ? P.box[l][P.nbox[l] - 1].t_end : 0;
+    const int X = std::max(1, c->wg_xchunk);
+    auto add = [&](int i_lo, int i_hi, int j_lo, int j_hi, int xc) {
+        if (i_lo > i_hi || j_lo > j_hi) return;
+        WgBox& B = P.box[P.nbox];
+        const int prev = P.nbox ? P.box[P.nbox - 1].b_end : 0;
         B.i_lo = i_lo; B.i_hi = i_hi; B.j_lo = j_lo; B.j_hi = j_hi;
-        B.tz_lo = tz_lo; B.ntz = tz_hi - tz_lo;
-        B.nty = (j_hi - j_lo + R) / R;
-        B.ntx = (i_hi - i_lo + xc) / xc;
-        B.xc = xc;
-        B.t_end = prev + B.ntz * B.nty * B.ntx;
-        ++P.nbox[l];
+        B.xc = std::max(1, std::min(xc, i_hi - i_lo + 1));
+        B.nyb = (j_hi - j_lo + H) / H;
+        B.b_end = prev + B.nyb * ((i_hi - i_lo + B.xc) / B.xc);
+        ++P.nbox;
     };
-    add(0, xa, xb, ya, yb, tza, tzb);
-    add(1, xa, xb, ya, yb, 0, tza);
-    add(1, xa, xb, ya, yb, tzb, ntz);
-    if (xb < xa) {
-        add(2, 1, nx, 1, ny, 0, ntz);
-    } else {
-        add(2, 1, xa - 1, 1, ny, 0, ntz);
-        add(2, xb + 1, nx, 1, ny, 0, ntz);
-        add(2, xa, xb, 1, ya - 1, 0, ntz);
-        add(2, xa, xb, yb + 1, ny, 0, ntz);
+    const int xa = (m & 1u) ? np + 1 : 1, xb = (m & 2u) ? nx - np : nx;
+    const int hs = (np + H - 1) / H * H;
+    const int ya = (m & 4u) ? hs + 1 : 1, yb = (m & 8u) ? ny - hs : ny;
+    if (xa > xb || ya > yb) {
+        add(1, nx, 1, ny, 4);
+        return P;
     }
+    add(1, xa - 1, 1, ny, 4);
+    add(xb + 1, nx, 1, ny, 4);
+    add(xa, xb, 1, ya - 1, std::max(1, X / 2));
+    add(xa, xb, yb + 1, ny, std::max(1, X / 2));
+    add(xa, xb, ya, yb, X);
     return P;
 }
 
-// One fused step (k_step_fused): reads the current buffers, writes the other
-// set, then swaps them; the cells of dispersive classes and the near-field
-// patches follow in their own launches.
+// The fused step: one launch over every box, with the CPML code of the
+// faces the grid has compiled in (the interior tiles skip the x/y psi code
+// by uniform branches).  Separate launches for slabs and interior, in turn
+// or on two streams, measured slower: the slab launch alone cannot fill the
+// chip, and the interior tiles then run without slab tiles to balance them.
 template <typename T>
-int launch_fused(omx3d_ctx* c, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
+int launch_wg(omx3d_ctx* c, bool md, bool tf, const YeeArgs<T>& a, const FusedOut<T>& fo) {
+    const WgPlan P = wg_plan(c);
+    if (P.nzw == 0) return fail(OMX_ERR_STATE, "fused step: the z extent does not fit one workgroup");
+    if (c->cpml_mask & 15u) launch_wg_rows<T, 7>(c, md, tf, a, fo, P, c->s);
+    else if (c->cpml_mask & 48u) launch_wg_rows<T, 4>(c, md, tf, a, fo, P, c->s);
+    else launch_wg_rows<T, 0>(c, md, tf, a, fo, P, c->s);
+    HIPCHK(hipGetLastError());
+    return OMX_OK;
+}
+
+// One fused step (k_step_wg): reads the current buffers, writes the other
+// set, then swaps them; the cells of dispersive classes (with the TF/SF
+// line's E half step) and the near-field patches follow in their own
+// launches.
+template <typename T>
+int launch_fused(omx3d_ctx* c, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr, bool fold_e = false,
+                 double t_tf = 0.0) {
     int rc = ensure_patches<T>(c);
     if (rc) return rc;
     rc = ensure_alt(c);
@@ -796,33 +816,39 @@ int launch_fused(omx3d_ctx* c, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullpt
     fo.hx = (T*)c->f_alt[3]; fo.hy = (T*)c->f_alt[4]; fo.hz = (T*)c->f_alt[5];
     for (int q = 0; q < 6; ++q) fo.ph[q] = (T*)c->psi_h_alt[q];
     if (c->tfsf) { fo.ex1 = c->d_tf; fo.t = tfsf_args(c); }
-    fo.tile_order = c->tile_order == 0 ? 0 : 1;
     if (c->tfsf)   // the incident line's H half step (the kernel reads its E only)
         hipLaunchKernelGGL(k_tfsf_1d_h, dim3(blocks_for(c->tf_n, 256)), dim3(256), 0, c->s, c->d_tf,
                            c->d_tf + c->tf_n, c->tf_n, (c->d.dt / kMu0) * (1.0 / c->d.dr));
     const bool md = c->media_set && c->n_classes > 1, tf = c->tfsf;
-    const FusedPlan P = fused_plan(c);
+    if (tf && fold_e) fo.hy1 = c->d_tf + c->tf_n;
     if (ev0) HIPCHK(hipEventRecord(ev0, c->s));
-    for (int l = 0; l < 3; ++l) {
-        if (!P.nbox[l]) continue;
-        fo.nbox = P.nbox[l];
-        for (int q = 0; q < 4; ++q) std::memcpy(&fo.box[q], &P.box[l][q], sizeof(fo.box[q]));
-        const unsigned nb = (unsigned)((P.box[l][P.nbox[l] - 1].t_end + 3) / 4);   // four wavefronts per workgroup
-        if (l == 0) launch_fused_rows<T, 0>(c, md, tf, nb, a, fo);
-        else if (l == 1) launch_fused_rows<T, 4>(c, md, tf, nb, a, fo);
-        else launch_fused_rows<T, 7>(c, md, tf, nb, a, fo);
-    }
+    rc = launch_wg<T>(c, md, tf, a, fo);
+    if (rc) return rc;
     if (ev1) HIPCHK(hipEventRecord(ev1, c->s));
     HIPCHK(hipGetLastError());
     for (int q = 0; q < 6; ++q) std::swap(c->f[q], c->f_alt[q]);
     for (int q = 0; q < 6; ++q) std::swap(c->psi_h[q], c->psi_h_alt[q]);
-    if (md && c->n_disp > 0) {
-        // cells of dispersive classes: H^{n+1/2} now current, E^n in the
-        // other buffer, E^{n+1} into the current one
+    if ((md && c->n_disp > 0) || (tf && fold_e)) {
+        // one launch for the cells of dispersive classes (H^{n+1/2} now
+        // current, E^n in the other buffer, E^{n+1} into the current one) and
+        // the TF/SF line's E half step
         YeeArgs<T> an = make_args<T>(c);
         an.ex = (T*)c->f_alt[0]; an.ey = (T*)c->f_alt[1]; an.ez = (T*)c->f_alt[2];
-        rc = launch_e_dispersive<T>(c, an, (T*)c->f[0], (T*)c->f[1], (T*)c->f[2], c->s);
-        if (rc) return rc;
+        const int nd = md ? (int)blocks_for(c->n_disp, 256) : 0;
+        const int nl = tf && fold_e ? (int)blocks_for(c->tf_n, 256) : 0;
+        const TfsfArgs t = tf ? tfsf_args(c) : TfsfArgs{};
+        const double src = tf && fold_e ? tfsf_source(c, t_tf) : 0.0;
+        double* ex1 = tf ? c->d_tf : nullptr;
+        const double* hy1 = tf ? c->d_tf + c->tf_n : nullptr;
+        const int n = md ? c->n_disp : 0;
+        T *exo = (T*)c->f[0], *eyo = (T*)c->f[1], *ezo = (T*)c->f[2];
+        if (c->cpml_mask)
+            hipLaunchKernelGGL((k_wg_post<T, true, true>), dim3(nd + nl), dim3(256), 0, c->s, an, exo, eyo, ezo,
+                               c->d_disp, n, nd, t, ex1, hy1, src);
+        else
+            hipLaunchKernelGGL((k_wg_post<T, false, true>), dim3(nd + nl), dim3(256), 0, c->s, an, exo, eyo, ezo,
+                               c->d_disp, n, nd, t, ex1, hy1, src);
+        HIPCHK(hipGetLastError());
     }
     if (c->patch_cells) {
         // near-field patches: H^{n+1/2} from E^n (the other buffer) into the
@@ -858,6 +884,9 @@ int run_impl(omx3d_ctx* c, int tt_first, int n_steps, int record_every) {
     rc = tfsf_fold_check(c);
     if (rc) return rc;
     const bool fold = !fused && c->tfsf, fold_e = fold && c->tf_fold_e == 1;
+    // the fused step adds the TF/SF E corrections itself and advances the
+    // line's E after the kernel (same condition as fold_e)
+    const bool fold_wg = fused && c->tfsf && c->tf_fold_e == 1;
     const int tt_0 = c->n_skip == 1 ? 1 : 0;
     for (int tt = tt_first; tt < tt_first + n_steps; ++tt) {
         const double time = dt * ((double)tt - 0.5);
@@ -884,7 +913,7 @@ int run_impl(omx3d_ctx* c, int tt_first, int n_steps, int record_every) {
                 HIPCHK(hipEventRecord(ev[0], c->s));
                 HIPCHK(hipEventRecord(ev[1], c->s));
             }
-            rc = launch_fused<T>(c, ev ? ev[2] : nullptr, ev ? ev[3] : nullptr);
+            rc = launch_fused<T>(c, ev ? ev[2] : nullptr, ev ? ev[3] : nullptr, fold_wg, dt * (double)tt);
             if (rc) return rc;
             rc = launch_sources<T>(c, time);
             if (rc) return rc;
@@ -898,7 +927,7 @@ int run_impl(omx3d_ctx* c, int tt_first, int n_steps, int record_every) {
             rc = launch_e<T>(c, time, t_skip, fold_e, dt * (double)tt, ev ? ev[2] : nullptr, ev ? ev[3] : nullptr);
             if (rc) return rc;
         }
-        if (!fold_e) {
+        if (!fold_e && !fold_wg) {
             rc = launch_tfsf_e<T>(c, dt * (double)tt);
             if (rc) return rc;
         }
@@ -1593,8 +1622,7 @@ int omx3d_create(const omx3d_desc* desc, omx3d_ctx** out) {
     c->xchunk = 4;
     c->tile_order = 2;
     // fused step: FP32 4 rows x 16 planes per wavefront, FP64 2 x 8
-    c->fused_rows = d.precision == OMX_FP32 ? 4 : 2;
-    c->fused_xchunk = d.precision == OMX_FP32 ? 16 : 8;
+    c->wg_rows = d.precision == OMX_FP32 ? 2 : 1;
     c->cpml_mask = mask;
     Layout& L = c->L;
     const int A = 128 / (int)c->esz;
@@ -2290,11 +2318,18 @@ int omx3d_run(omx3d_ctx* c, int32_t tt_first, int32_t n_steps, int32_t record_ev
 int omx3d_set_step_mode(omx3d_ctx* c, int32_t mode, int32_t rows, int32_t xchunk) {
     CHECK_CTX(c);
     if (mode != 0 && mode != 1) return fail(OMX_ERR_ARG, "step mode must be 0 (fused where it applies) or 1");
-    if (rows != 0 && rows != 1 && rows != 2 && rows != 4) return fail(OMX_ERR_ARG, "fused rows must be 1, 2 or 4");
+    if (rows != 0 && rows != 1 && rows != 2) return fail(OMX_ERR_ARG, "fused rows must be 1 or 2");
     if (xchunk < 0) return fail(OMX_ERR_ARG, "xchunk must be >= 0");
     c->step_mode = mode;
-    if (rows) c->fused_rows = rows;
-    if (xchunk) c->fused_xchunk = xchunk;
+    if (rows) c->wg_rows = rows;
+    if (xchunk) c->wg_xchunk = xchunk;
+    return OMX_OK;
+}
+
+int omx3d_set_fused_waves(omx3d_ctx* c, int32_t ny_waves) {
+    CHECK_CTX(c);
+    if (ny_waves < 0 || ny_waves > 16) return fail(OMX_ERR_ARG, "wavefronts along y must be 0..16");
+    c->wg_ny = ny_waves;
     return OMX_OK;
 }
 

@@ -7,23 +7,33 @@
 // separate kernels (h_tile / e_tile: td_propagate_H_3Dfield and
 // td_propagate_E_3Dfield, mxll_3D_mod.f90:505-923), face order included.
 //
-// A wavefront owns 64 cells along z (lanes), R rows along y and a chunk of
-// planes along x, which it marches carrying the next plane's E and the
-// previous plane's H in registers.  The E update of a cell needs H^{n+1/2}
-// at j-1, k-1 and i-1: the rows of the wavefront take j-1 from the row
-// before (the first row from a halo row computed again), lanes take k-1 from
-// the lane before by a DPP shift (lane 0 computes its k0-1 cell itself),
-// planes take i-1 from the plane before (the chunk's first plane from a halo
-// plane computed again).  Halo cells run the same H routine as their owners
-// with the CPML psi of H read from the input buffer and nothing stored, so
-// their values equal the owner's bit for bit; the psi of H is therefore
-// double-buffered like the fields.  E(k+1) comes by the opposite DPP shift,
-// lane 63 loading its k0+64 neighbour.  No LDS and no barrier.
+// A workgroup of NZ x NY wavefronts covers every z cell of the grid (NZ =
+// ceil(nz/64) wavefronts of 64 lanes along z) and NY*R rows along y (each
+// wavefront R rows), and marches a chunk of xc planes.  Per plane, each
+// wavefront loads E^n of the next plane and H^{n-1/2} of its own rows,
+// computes H^{n+1/2}, and hands the values its neighbours need to them
+// through LDS, one barrier per plane:
+//   - the first row's E^n (Ex, Ez) of plane i+1 to the wavefront below (its
+//     last row's j+1 neighbour at the next plane),
+//   - lane 0's E^n (Ey, Ex) of plane i+1 to the wavefront before in z (its
+//     lane 63's k+1 neighbour at the next plane),
+//   - the last row's H^{n+1/2} (Hx, Hz) to the wavefront above (its first
+//     row's j-1 neighbour),
+//   - lane 63's H^{n+1/2} (Hx, Hy) to the wavefront after in z (its lane 0's
+//     k-1 neighbour).
+// k+-1 inside a wavefront come by DPP lane shifts.  Only the workgroup's
+// edges recompute: the row below the first row (H of row j0-1 from inputs;
+// j0-1 = 0 is the boundary row, read) and the plane before the chunk (H of
+// plane i0-1).  The z edges are the grid's boundary cells (k = 0 read, k =
+// nz+1 read), so nothing is recomputed along z.  Halo cells run the same H
+// routine as their owners with the CPML psi of H read from the input buffer
+// and nothing stored, so their values equal the owner's bit for bit; the psi
+// of H is therefore double-buffered like the fields.
 //
 // Cells of Drude / Drude-Lorentz classes are left to the dispersive pass
-// after the kernel (their E and E psi are not stored); the TF/SF H
-// corrections are applied inside (owned and halo cells alike), the E side
-// after the kernel.
+// after the kernel (their E and E psi are not stored).  The TF/SF H
+// corrections are applied inside (owned and halo cells alike), and the E
+// corrections too unless a dispersive cell lies on their faces.
 #pragma once
 
 namespace omx {
@@ -51,22 +61,15 @@ __device__ __forceinline__ double dpp_shl1(double old, double v) {
     return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
-// Outputs and layout of the fused step (the inputs are the YeeArgs block).
+// Outputs of the fused step (the inputs are the YeeArgs block) and its
+// TF/SF data.
 template <typename T>
 struct FusedOut {
     T *ex, *ey, *ez, *hx, *hy, *hz;   // E^{n+1}, H^{n+1/2}
     T* ph[6];                         // psi of H at n+1/2 (x, y, z slab arrays as YeeArgs::ph)
     const double* ex1;                // TF/SF: the incident line's E^n (null without TF/SF)
+    const double* hy1;                // TF/SF: the line's H^{n+1/2} when the kernel adds the E corrections too
     TfsfArgs t;
-    int tile_order;                   // 0: block order, 1: one contiguous slab of tiles per XCD
-    // The launch covers up to four boxes of tiles, numbered one box after
-    // the other: planes i_lo..i_hi, rows j_lo..j_hi, z tiles tz_lo ..
-    // tz_lo+ntz-1 (z tile fastest, then row block of R rows, then chunk of
-    // xc planes); t_end is one past the box's last tile.
-    struct Box {
-        int i_lo, i_hi, j_lo, j_hi, tz_lo, ntz, nty, ntx, xc, t_end;
-    } box[4];
-    int nbox;
 };
 
 // Per-row, per-lane or per-plane coefficients of the H and E updates and
@@ -115,95 +118,108 @@ __device__ __forceinline__ Coef<T> coef_at(const YeeArgs<T>& a, const T* den_h, 
     return c;
 }
 
-// TF/SF corrections of H in z cell k (same values as tfsf_h_lane), read
-// once per wavefront: cy is the Hy correction of the cells of the box's z
-// faces (k = k0-1 or k1) for rows j0..j1, e1 the incident Ex at k (k0..k1,
-// else 0) that the Hz correction of rows j0-1 and j1 scales.  The cell's
-// plane decides whether either applies.
-struct TfLane {
-    double cy, e1;
-};
-template <typename T, bool U>
-__device__ __forceinline__ TfLane tf_lane(const YeeArgs<T>& a, const FusedOut<T>& fo, int k) {
-    const TfsfArgs& t = fo.t;
-    TfLane v{0.0, 0.0};
-    if (k == t.k0 - 1 || k == t.k1) {
-        const bool hi = k == t.k1;
-        const double c = t.dt_mu0 * (double)(U ? ldc(a.den_hz, k) : gld(a.den_hz, k)) * fo.ex1[(hi ? t.k1 : t.k0) - t.kb];
-        v.cy = hi ? -c : c;
-    }
-    if (k >= t.k0 && k <= t.k1) v.e1 = fo.ex1[k - t.kb];
-    return v;
-}
-
-// CPML psi offsets of cell (i,j,k) (bytes; x and y faces: slot form, z
-// faces: the lane's z-face base) and the psi values read ahead of their use
-// (zero where the cell has no such face).  Reading a row's psi together
-// with its fields keeps them in one memory round trip instead of one per
-// face.
-struct PsiOff {
-    uint32_t x, y, z;
-};
+// The CPML psi values of a cell read ahead of their use (zero where the cell
+// has no such face): x face (Hy,Hz | Ey,Ez), y face (Hx,Hz | Ex,Ez), z face
+// (Hx,Hy | Ex,Ey).  Reading a row's psi together with its fields keeps them
+// in one memory round trip instead of one per face.
 template <typename T>
 struct Psi {
-    T x0, x1, y0, y1, z0, z1;   // x face (Hy,Hz | Ey,Ez), y face (Hx,Hz | Ex,Ez), z face (Hx,Hy | Ex,Ey)
+    T x0, x1, y0, y1, z0, z1;
 };
 
-// H (E false) or E (E true) psi offsets of cell (i,j,k).
-template <typename T, bool E>
-__device__ __forceinline__ PsiOff psi_off(const YeeArgs<T>& a, int i, int j, int k, const Coef<T>& px,
-                                          const Coef<T>& ry, const Coef<T>& lz) {
-    const uint32_t S = (uint32_t)sizeof(T);
-    const uint32_t kb = (uint32_t)(k + a.off);
-    const int sx = E ? px.se : px.sh, sy = E ? ry.se : ry.sh, sz = E ? lz.se : lz.sh;
-    const uint32_t ny2 = (uint32_t)(E ? 2 * a.npml : 2 * (a.npml - 1));
-    PsiOff o;
-    o.x = ((uint32_t)sx * (uint32_t)a.sx + (uint32_t)j * (uint32_t)a.sz + kb) * S;
-    o.y = (((uint32_t)i * ny2 + (uint32_t)sy) * (uint32_t)a.sz + kb) * S;
-    o.z = ((uint32_t)sz + ((uint32_t)j * (uint32_t)(a.nx + 2) + (uint32_t)i) * (uint32_t)(a.npml - 1)) * S;
-    return o;
+// Workgroup tiles of k_step_wg, in boxes numbered one after the other:
+// planes i_lo..i_hi and rows j_lo..j_hi in row blocks of NY*R rows (nyb of
+// them, fastest) and chunks of xc planes; b_end is one past the box's last
+// workgroup.  The host lists the boxes holding CPML faces first, so the
+// heaviest tiles start first.
+struct WgBox {
+    int i_lo, i_hi, j_lo, j_hi, nyb, xc, b_end;
+};
+constexpr int kWgBoxes = 5;
+struct WgPlan {
+    WgBox box[kWgBoxes];
+    int nbox;
+    int nzw, nyw;        // wavefronts per workgroup along z and y
+};
+
+// LDS of one workgroup: with TF/SF, each wavefront's per-lane TF/SF values
+// (NW x 2 x 64 doubles), then two buffers (plane parity) of the row
+// hand-overs (NW x 2 components x 64 lanes, H then E) and of the lane
+// hand-overs (NW x R rows x 2 components, H then E).
+__host__ __device__ inline int wg_lds_elems(int nw, int r) { return 8 * nw * 64 + 8 * nw * r; }
+__host__ __device__ inline size_t wg_lds_bytes(int nw, int r, size_t esz, bool tf) {
+    return (tf ? (size_t)nw * 128 * sizeof(double) : 0) + esz * (size_t)wg_lds_elems(nw, r);
 }
 
-// Reads the psi of H (E false: the components WX/WY/WZ need) or of E (E
-// true: all) of the faces cell (i,j,k) lies in.
+// Element of array p at the uniform byte offset uo plus the lane's byte
+// offset v: one 32-bit vector offset from the array's scalar base (adding
+// the two parts in 32 bits keeps the compiler from holding 64-bit addresses
+// of every array in vector registers across the plane loop).
+template <typename T>
+__device__ __forceinline__ T ldu(const T* p, uint32_t uo, uint32_t v) {
+    return ldb(p, uo + v);
+}
+template <typename T>
+__device__ __forceinline__ void stu(T* p, uint32_t uo, uint32_t v, T x) {
+    stb(p, uo + v, x);
+}
+
+// Uniform byte offsets of the psi of an x face slot / a y face slot (ny2
+// slots per plane) / the z faces at (i, j).
+template <typename T>
+__device__ __forceinline__ uint32_t uo_psix(const YeeArgs<T>& a, int slot, int j) {
+    return ((uint32_t)slot * (uint32_t)a.sx + (uint32_t)j * (uint32_t)a.sz) * (uint32_t)sizeof(T);
+}
+template <typename T>
+__device__ __forceinline__ uint32_t uo_psiy(const YeeArgs<T>& a, int i, int slot, int ny2) {
+    return (((uint32_t)i * (uint32_t)ny2 + (uint32_t)slot) * (uint32_t)a.sz) * (uint32_t)sizeof(T);
+}
+template <typename T>
+__device__ __forceinline__ uint32_t uo_psiz(const YeeArgs<T>& a, int i, int j) {
+    return (((uint32_t)j * (uint32_t)(a.nx + 2) + (uint32_t)i) * (uint32_t)(a.npml - 1)) * (uint32_t)sizeof(T);
+}
+
+// The psi of H (E false: the components WX/WY/WZ need) or of E (E true:
+// all) of the faces cell (i,j,k) lies in, zero elsewhere.  The x and y faces
+// are uniform over the wavefront (plane, row), the z faces per lane.  kS:
+// the lane's byte offset in a row; zS: its z-face psi byte offset.
 template <typename T, int CPML, bool E, bool WX, bool WY, bool WZ>
-__device__ __forceinline__ Psi<T> psi_read(const YeeArgs<T>& a, const PsiOff& o, const Coef<T>& px, const Coef<T>& ry,
-                                           const Coef<T>& lz) {
-    using O = Off<T, true>;
+__device__ __forceinline__ Psi<T> psi_wg(const YeeArgs<T>& a, int i, int j, uint32_t kS, uint32_t zS, const Coef<T>& px,
+                                         const Coef<T>& ry, const Coef<T>& lz, bool zface) {
     Psi<T> v;
     v.x0 = v.x1 = v.y0 = v.y1 = v.z0 = v.z1 = (T)0;
     if constexpr (CPML != 0) {
         T* const* ps = E ? a.pe : a.ph;
-        if ((CPML & 1) && (E ? px.se : px.sh) >= 0) {
-            if (E || WY) v.x0 = O::ld(ps[0], o.x);
-            if (E || WZ) v.x1 = O::ld(ps[1], o.x);
+        const int sx_ = E ? px.se : px.sh, sy_ = E ? ry.se : ry.sh, sz_ = E ? lz.se : lz.sh;
+        if ((CPML & 1) && sx_ >= 0) {
+            const uint32_t uo = uo_psix(a, sx_, j);
+            if (E || WY) v.x0 = ldu(ps[0], uo, kS);
+            if (E || WZ) v.x1 = ldu(ps[1], uo, kS);
         }
-        if ((CPML & 2) && (E ? ry.se : ry.sh) >= 0) {
-            if (E || WX) v.y0 = O::ld(ps[2], o.y);
-            if (E || WZ) v.y1 = O::ld(ps[3], o.y);
+        if ((CPML & 2) && sy_ >= 0) {
+            const uint32_t uo = uo_psiy(a, i, sy_, E ? 2 * a.npml : 2 * (a.npml - 1));
+            if (E || WX) v.y0 = ldu(ps[2], uo, kS);
+            if (E || WZ) v.y1 = ldu(ps[3], uo, kS);
         }
-        if ((CPML & 4) && (E ? lz.se : lz.sh) >= 0) {
-            if (E || WX) v.z0 = O::ld(ps[4], o.z);
-            if (E || WY) v.z1 = O::ld(ps[5], o.z);
+        if ((CPML & 4) && zface && sz_ >= 0) {
+            const uint32_t uo = uo_psiz(a, i, j);
+            if (E || WX) v.z0 = ldu(ps[4], uo, zS);
+            if (E || WY) v.z1 = ldu(ps[5], uo, zS);
         }
     }
     return v;
 }
 
-// H^{n+1/2} of the components WX/WY/WZ of cell (i,j,k) (h in: H^{n-1/2}):
-// the bulk curl, then the CPML psi of the faces the cell lies in (x, y, z,
-// as the reference; ps: psi of H at n-1/2 read by psi_read), then the TF/SF
-// correction (tl: the cell's z cell's values, tf_lane).  STORE writes the new psi to fo.ph at po where act is set
-// (owned cells).
-// cx is the plane's coefficient (cdhx[i] in FP32, den_hx[i] in FP64) and px
-// its face; ry / lz the row's and the lane's coefficients and faces.
-// CPML: bit 0 x faces, bit 1 y faces, bit 2 z faces.
+// H^{n+1/2} of the components WX/WY/WZ of cell (i,j,k) with the arithmetic
+// of h_tile: the bulk curl, the CPML psi of the x, y and z faces in that
+// order (ps: psi of H at n-1/2 from psi_wg), then the TF/SF correction where
+// the plane and row lie on a face of the box (tfv: the lane's values of
+// this step in LDS).  STORE writes the new psi to fo.ph where act is set.
 template <typename T, int CPML, bool TF, bool WX, bool WY, bool WZ, bool STORE>
-__device__ __forceinline__ void h_fused(const YeeArgs<T>& a, const FusedOut<T>& fo, int i, int j, int k, T cx,
-                                        const Coef<T>& px, const Coef<T>& ry, const Coef<T>& lz, const Psi<T>& ps,
-                                        const PsiOff& po, const TfLane& tl, T ezc, T ezj, T ezn, T eyc, T eyk, T eyn,
-                                        T exc, T exk, T exj, T& hx, T& hy, T& hz, bool act = true) {
-    using O = Off<T, true>;
+__device__ __forceinline__ void h_wg(const YeeArgs<T>& a, const FusedOut<T>& fo, const double* tfv, int i, int j, int k,
+                                     uint32_t kS, uint32_t zhS, T cx, const Coef<T>& px, const Coef<T>& ry,
+                                     const Coef<T>& lz, const Psi<T>& ps, T ezc, T ezj, T ezn, T eyc, T eyk, T eyn, T exc,
+                                     T exk, T exj, T& hx, T& hy, T& hz, bool act) {
     const T dmu = a.dt_mu0;
     if constexpr (sizeof(T) == 4) {
         if constexpr (WX) hx = hx + (ezc - ezj) * ry.ch + (eyk - eyc) * lz.ch;
@@ -215,39 +231,42 @@ __device__ __forceinline__ void h_fused(const YeeArgs<T>& a, const FusedOut<T>& 
         if constexpr (WZ) hz = hz + dmu * ((eyc - eyn) * cx + (exj - exc) * ry.dh);
     }
     if constexpr (CPML != 0) {
-        if ((CPML & 1) && px.sh >= 0 && (WY || WZ)) {
+        if ((CPML & 1) && (WY || WZ) && px.sh >= 0) {
+            const uint32_t uo = uo_psix(a, px.sh, j);
             if constexpr (WY) {
                 const T p0 = psi_step(ps.x0, px.bh, px.chh, ezn - ezc);
-                if (STORE && act) O::st(fo.ph[0], po.x, p0);
+                if (STORE && act) stu(fo.ph[0], uo, kS, p0);
                 hy = hy + dmu * p0;
             }
             if constexpr (WZ) {
                 const T p1 = psi_step(ps.x1, px.bh, px.chh, eyc - eyn);
-                if (STORE && act) O::st(fo.ph[1], po.x, p1);
+                if (STORE && act) stu(fo.ph[1], uo, kS, p1);
                 hz = hz + dmu * p1;
             }
         }
-        if ((CPML & 2) && ry.sh >= 0 && (WX || WZ)) {
+        if ((CPML & 2) && (WX || WZ) && ry.sh >= 0) {
+            const uint32_t uo = uo_psiy(a, i, ry.sh, 2 * (a.npml - 1));
             if constexpr (WX) {
                 const T p0 = psi_step(ps.y0, ry.bh, ry.chh, ezc - ezj);
-                if (STORE && act) O::st(fo.ph[2], po.y, p0);
+                if (STORE && act) stu(fo.ph[2], uo, kS, p0);
                 hx = hx + dmu * p0;
             }
             if constexpr (WZ) {
                 const T p1 = psi_step(ps.y1, ry.bh, ry.chh, exj - exc);
-                if (STORE && act) O::st(fo.ph[3], po.y, p1);
+                if (STORE && act) stu(fo.ph[3], uo, kS, p1);
                 hz = hz + dmu * p1;
             }
         }
-        if ((CPML & 4) && lz.sh >= 0 && (WX || WY)) {
+        if ((CPML & 4) && (WX || WY) && lz.sh >= 0) {
+            const uint32_t uo = uo_psiz(a, i, j);
             if constexpr (WX) {
                 const T p0 = psi_step(ps.z0, lz.bh, lz.chh, eyk - eyc);
-                if (STORE && act) O::st(fo.ph[4], po.z, p0);
+                if (STORE && act) stu(fo.ph[4], uo, zhS, p0);
                 hx = hx + dmu * p0;
             }
             if constexpr (WY) {
                 const T p1 = psi_step(ps.z1, lz.bh, lz.chh, exc - exk);
-                if (STORE && act) O::st(fo.ph[5], po.z, p1);
+                if (STORE && act) stu(fo.ph[5], uo, zhS, p1);
                 hy = hy + dmu * p1;
             }
         }
@@ -255,251 +274,341 @@ __device__ __forceinline__ void h_fused(const YeeArgs<T>& a, const FusedOut<T>& 
     if constexpr (TF) {
         const TfsfArgs& t = fo.t;
         if (i >= t.i0 && i <= t.i1 - 1) {
-            if (WY && j >= t.j0 && j <= t.j1 && tl.cy != 0.0) hy = (T)((double)hy + tl.cy);
-            if (WZ && (j == t.j0 - 1 || j == t.j1) && tl.e1 != 0.0) {
-                const double c = t.dt_mu0 * (double)ldc(a.den_hy, j) * tl.e1;
-                hz = (T)((double)hz + (j == t.j1 ? c : -c));
+            if (WY && j >= t.j0 && j <= t.j1) {
+                const double cy = tfv[0];
+                if (cy != 0.0) hy = (T)((double)hy + cy);
+            }
+            if (WZ && (j == t.j0 - 1 || j == t.j1) && k >= t.k0 && k <= t.k1) {
+                const double e1 = fo.ex1[k - t.kb];
+                if (e1 != 0.0) {
+                    const double c = t.dt_mu0 * (double)ldc(a.den_hy, j) * e1;
+                    hz = (T)((double)hz + (j == t.j1 ? c : -c));
+                }
             }
         }
     }
 }
 
-// The H update of a halo cell: its psi read here, nothing stored.
-template <typename T, int CPML, bool TF, bool WX, bool WY, bool WZ>
-__device__ __forceinline__ void h_halo(const YeeArgs<T>& a, const FusedOut<T>& fo, int i, int j, int k, T cx,
-                                       const Coef<T>& px, const Coef<T>& ry, const Coef<T>& lz, const TfLane& tl, T ezc,
-                                       T ezj, T ezn, T eyc, T eyk, T eyn, T exc, T exk, T exj, T& hx, T& hy, T& hz) {
-    const PsiOff po = psi_off<T, false>(a, i, j, k, px, ry, lz);
-    const Psi<T> ps = psi_read<T, CPML, false, WX, WY, WZ>(a, po, px, ry, lz);
-    h_fused<T, CPML, TF, WX, WY, WZ, false>(a, fo, i, j, k, cx, px, ry, lz, ps, po, tl, ezc, ezj, ezn, eyc, eyk, eyn,
-                                            exc, exk, exj, hx, hy, hz);
-}
-
-// E^{n+1} of an owned cell (e in: E^n): e_bulk, then the CPML psi of E in
-// place (x, y, z; ps: psi of E at n read by psi_read at po); a cell of a
-// dispersive class (own false) stores nothing.
-template <typename T, int CPML, int MED>
-__device__ __forceinline__ void e_fused(const YeeArgs<T>& a, int tg, bool own, T dex, T cex, const Coef<T>& px,
-                                        const Coef<T>& ry, const Coef<T>& lz, const Psi<T>& ps, const PsiOff& po,
-                                        T hzc, T hzj, T hzi, T hyc, T hyk, T hyi, T hxc, T hxj, T hxk, T& ex, T& ey,
-                                        T& ez) {
-    using O = Off<T, true>;
+// E^{n+1} of an owned cell with the arithmetic of e_tile: e_bulk, then the
+// CPML psi of E in place (x, y, z faces; ps: psi of E at n from psi_wg),
+// then the TF/SF E corrections when fo.hy1 is set; own false (a cell of a
+// dispersive class, or a lane past the grid) stores no psi.
+template <typename T, int CPML, int MED, bool TF>
+__device__ __forceinline__ void e_wg(const YeeArgs<T>& a, const FusedOut<T>& fo, const double* tfv, int i, int j, int k,
+                                     uint32_t kS,
+                                     uint32_t zeS, int tg, bool own, T dex, T cex, const Coef<T>& px, const Coef<T>& ry,
+                                     const Coef<T>& lz, const Psi<T>& ps, T hzc, T hzj, T hzi, T hyc, T hyk, T hyi,
+                                     T hxc, T hxj, T hxk, T& ex, T& ey, T& ez) {
     const T dep = a.dt_eps0;
-    e_bulk<T, MED>(a, tg, dep, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dex, cex, ry.de, lz.de, ry.ce, lz.ce, ex,
+    // a cell of a dispersive class is updated by the dispersive pass (nothing
+    // stored here): the vacuum form spares its class lookups
+    const int tgb = MED == 1 && tg >= a.disp_tag0 ? 0 : tg;
+    e_bulk<T, MED>(a, tgb, dep, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dex, cex, ry.de, lz.de, ry.ce, lz.ce, ex,
                    ey, ez);
     if constexpr (CPML != 0) {
         if ((CPML & 1) && px.se >= 0) {
+            const uint32_t uo = uo_psix(a, px.se, j);
             const T p0 = psi_step(ps.x0, px.be, px.cee, hzi - hzc);
             const T p1 = psi_step(ps.x1, px.be, px.cee, hyc - hyi);
-            if (own) { O::st(a.pe[0], po.x, p0); O::st(a.pe[1], po.x, p1); }
+            if (own) { stu(a.pe[0], uo, kS, p0); stu(a.pe[1], uo, kS, p1); }
             ey = ey + dep * p0; ez = ez + dep * p1;
         }
         if ((CPML & 2) && ry.se >= 0) {
+            const uint32_t uo = uo_psiy(a, i, ry.se, 2 * a.npml);
             const T p0 = psi_step(ps.y0, ry.be, ry.cee, hzc - hzj);
             const T p1 = psi_step(ps.y1, ry.be, ry.cee, hxj - hxc);
-            if (own) { O::st(a.pe[2], po.y, p0); O::st(a.pe[3], po.y, p1); }
+            if (own) { stu(a.pe[2], uo, kS, p0); stu(a.pe[3], uo, kS, p1); }
             ex = ex + dep * p0; ez = ez + dep * p1;
         }
         if ((CPML & 4) && lz.se >= 0) {
+            const uint32_t uo = uo_psiz(a, i, j);
             const T p0 = psi_step(ps.z0, lz.be, lz.cee, hyk - hyc);
             const T p1 = psi_step(ps.z1, lz.be, lz.cee, hxc - hxk);
-            if (own) { O::st(a.pe[4], po.z, p0); O::st(a.pe[5], po.z, p1); }
+            if (own) { stu(a.pe[4], uo, zeS, p0); stu(a.pe[5], uo, zeS, p1); }
             ex = ex + dep * p0; ey = ey + dep * p1;
+        }
+    }
+    if constexpr (TF) {
+        // TF/SF E corrections (k_tfsf_e's values), when the host folds them in
+        const TfsfArgs& t = fo.t;
+        if (fo.hy1 != nullptr && j >= t.j0 && j <= t.j1) {
+            if (i >= t.i0 && i <= t.i1 - 1) {
+                const double cx = tfv[64];
+                if (cx != 0.0) ex = (T)((double)ex + cx);
+            }
+            if ((i == t.i0 || i == t.i1) && k >= t.k0 && k <= t.k1 - 1) {
+                const double c = t.dt_eps0 * (double)ldc(a.den_ex, i) * fo.hy1[k - t.kb];
+                ez = (T)((double)ez + (i == t.i1 ? c : -c));
+            }
         }
     }
 }
 
-// The fused step over the tiles of fo's boxes.  Workgroups of four
-// wavefronts, each its own tile (z tile tz, row block ty, x chunk tx; tz
-// fastest).  a: inputs (E^n, H^{n-1/2}, psi of H at n-1/2, psi of E in
-// place); fo: outputs.  CPML: the CPML faces the boxes' cells and halo
-// cells may lie in (bit 0 x, 1 y, 2 z; the code of the others is left out,
-// so a launch over the interior carries no CPML registers).  Requires every
-// array below 4 GiB (32-bit byte offsets) and no periodic axis.
 template <typename T, int R, int CPML, int MED, bool TF>
-__global__ __launch_bounds__(256) void k_step_fused(const YeeArgs<T> a, const FusedOut<T> fo) {
-    using O = Off<T, true>;
+__global__ __launch_bounds__(1024) void k_step_wg(const YeeArgs<T> a, const FusedOut<T> fo, const WgPlan P) {
+    extern __shared__ __attribute__((aligned(16))) char wg_lds_raw[];
+    const int NZ = P.nzw, NY = P.nyw, NW = NZ * NY;
     const int lane = (int)(threadIdx.x & 63);
-    const int wpb = (int)(blockDim.x >> 6);
-    int b = (int)blockIdx.x;
-    if (fo.tile_order == 1) b = xcd_tile(b, (int)gridDim.x);
-    // the wavefront's tile: uniform, so its rows, planes and coefficients sit in scalar registers
-    const int t = b * wpb + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    if (t >= fo.box[fo.nbox - 1].t_end) return;
-    int tl = t, bq = 0;
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int zw = wv % NZ, yw = wv / NZ;
+    double* const tfv = reinterpret_cast<double*>(wg_lds_raw) + wv * 128 + lane;   // TF: this lane's values
+    T* const lds = reinterpret_cast<T*>(wg_lds_raw + (TF ? NW * 128 * sizeof(double) : 0));
+    const int b = (int)blockIdx.x;
+    int bq = 0, bl = b;
 #pragma unroll
-    for (int q = 1; q < 4; ++q)
-        if (q < fo.nbox && t >= fo.box[q - 1].t_end) { bq = q; tl = t - fo.box[q - 1].t_end; }
-    const typename FusedOut<T>::Box& B = fo.box[bq];
-    const int tz = B.tz_lo + tl % B.ntz;
-    const int r0 = tl / B.ntz;
-    const int ty = r0 % B.nty;
-    const int tx = r0 / B.nty;
-    const int k0 = tz * 64 + 1;
-    const int k = k0 + lane;
-    const int kc = min(k, a.nz + 1);            // lanes past the grid work on the k = nz+1 cells (kept in
-                                                // bounds; their results reach no active lane and are not stored)
+    for (int q = 1; q < kWgBoxes; ++q)
+        if (q < P.nbox && b >= P.box[q - 1].b_end) { bq = q; bl = b - P.box[q - 1].b_end; }
+    const WgBox& B = P.box[bq];
+    const int yb = bl % B.nyb, xb = bl / B.nyb;
+    const int j0 = B.j_lo + (yb * NY + yw) * R;
+    const int nr = max(0, min(R, B.j_hi - j0 + 1));        // rows of this wavefront (0: it only joins the barriers)
+    const int i0 = B.i_lo + xb * B.xc, i1 = min(i0 + B.xc - 1, B.i_hi);
+    const int k0 = 1 + 64 * zw, k = k0 + lane;
+    const int kc = min(k, a.nz + 1);                        // lanes past the grid read the k = nz+1 cells
     const bool act = k <= a.nz;
-    const bool full = k0 + 63 <= a.nz;          // lane 63 is a cell: its k+1 neighbour is loaded
-    const int j0 = B.j_lo + ty * R;
-    const int nr = min(R, B.j_hi - j0 + 1);
-    const int i0 = B.i_lo + tx * B.xc;
-    const int i1 = min(i0 + B.xc - 1, B.i_hi);
-    const uint32_t S = (uint32_t)sizeof(T);
-    const uint32_t sx = (uint32_t)(a.sx * S), sz = (uint32_t)(a.sz * S);
+    const bool full = k0 + 63 <= a.nz;                      // lane 63 is a cell: its k+1 neighbour is read
+    const bool zlo = zw == 0, zhi = zw == NZ - 1;
+    // the wavefront's k range meets a z face (the high face may straddle two
+    // wavefronts when nz is not a multiple of 64)
+    const bool zface = (CPML & 4) && (k0 <= a.npml || k0 + 63 >= a.nz + 1 - a.npml);
+    const bool top_lds = yw < NY - 1 && j0 + R <= B.j_hi;   // the wavefront above has rows: j+1 of the last row from it
+    const bool ybot = yw == 0;
+    const bool yhalo = ybot && j0 > 1;                      // row j0-1 recomputed (j0 = 1: the boundary row, read)
+    T* const hrow = lds;                                    // [2][NW][2][64]: Hx, Hz of the last row
+    T* const erow = lds + 4 * NW * 64;                      // [2][NW][2][64]: Ex, Ez of the first row (next plane)
+    T* const hlz = lds + 8 * NW * 64;                       // [2][NW][R][2]: lane 63's Hx, Hy
+    T* const elz = hlz + 4 * NW * R;                        // [2][NW][R][2]: lane 0's Ey, Ex (next plane)
+    constexpr uint32_t S = (uint32_t)sizeof(T);
+    const uint32_t kS = (uint32_t)(kc + a.off) * S;         // the lane's byte offset in a row
+    const uint32_t rS = (uint32_t)a.sz * S, pS = (uint32_t)a.sx * S;
     const T* const cxs = sizeof(T) == 4 ? a.cdhx : a.den_hx;
     const Coef<T> lz = coef_at<T, (CPML & 4) != 0, true, false>(a, a.den_hz, a.cdhz, a.den_ez, a.cdez, kc, a.nz, 16u, 32u);
-    const int kh = k0 - 1;                      // lane 0's halo cell (k0 > 1)
-    Coef<T> lh = lz;
-    if (kh >= 1) lh = coef_at<T, (CPML & 4) != 0, true, true>(a, a.den_hz, a.cdhz, a.den_ez, a.cdez, kh, a.nz, 16u, 32u);
-    TfLane tfl{0.0, 0.0}, tfh{0.0, 0.0};        // TF/SF values of the lane's cell and of lane 0's halo cell
+    const uint32_t zhS = (uint32_t)max(lz.sh, 0) * S, zeS = (uint32_t)max(lz.se, 0) * S;
     if constexpr (TF) {
-        tfl = tf_lane<T, false>(a, fo, k);
-        if (kh >= 1) tfh = tf_lane<T, true>(a, fo, kh);
+        // the lane's TF/SF corrections of this step: Hy on the box's z faces
+        // (k = k0-1: + dt/mu0 den_hz(k) Ex_inc(k0), k = k1: - ... Ex_inc(k1))
+        // and, when the E corrections ride here, Ex on them (k = k0: + dt/eps0
+        // den_ez(k) Hy_inc(k0-1), k = k1: - ... Hy_inc(k1)); 0 elsewhere
+        const TfsfArgs& t = fo.t;
+        double cy = 0.0, cx = 0.0;
+        if (k == t.k0 - 1 || k == t.k1) {
+            const bool hi = k == t.k1;
+            const double c = t.dt_mu0 * (double)gld(a.den_hz, k) * fo.ex1[(hi ? t.k1 : t.k0) - t.kb];
+            cy = hi ? -c : c;
+        }
+        if (fo.hy1 != nullptr && (k == t.k0 || k == t.k1)) {
+            const bool hi = k == t.k1;
+            const double c = t.dt_eps0 * (double)gld(a.den_ez, k) * (hi ? fo.hy1[t.k1 - t.kb] : fo.hy1[t.k0 - 1 - t.kb]);
+            cx = hi ? -c : c;
+        }
+        tfv[0] = cy;
+        tfv[64] = cx;
     }
-    Coef<T> ry[R + 1];                          // rows j0-1 .. j0+R-1
+    Coef<T> ry[R + 1];                                      // rows j0-1 .. j0+R-1
+    T exc[R], eyc[R], ezc[R], hyp[R], hzp[R];
+    T exh = (T)0, eyh = (T)0, ezh = (T)0;                   // E^n of row j0-1 (yhalo)
+    if (nr > 0) {
 #pragma unroll
-    for (int r = 0; r <= R; ++r)
-        ry[r] = coef_at<T, (CPML & 2) != 0, false, true>(a, a.den_hy, a.cdhy, a.den_ey, a.cdey, min(j0 - 1 + r, a.ny + 1), a.ny, 4u, 8u);
-    const bool yhalo = j0 > 1;
-    uint32_t o = (uint32_t)(((long)i0 * a.sx + (long)j0 * a.sz + kc + a.off) * S);   // (i0, j0, kc)
-    // E^n of plane i0, rows j0-1 .. j0+nr-1 (index r+1)
-    T exc[R + 1], eyc[R + 1], ezc[R + 1], hyp[R], hzp[R];
+        for (int r = 0; r <= R; ++r)
+            ry[r] = coef_at<T, (CPML & 2) != 0, false, true>(a, a.den_hy, a.cdhy, a.den_ey, a.cdey,
+                                                            min(max(j0 - 1 + r, 0), a.ny + 1), a.ny, 4u, 8u);
+        const uint32_t u0 = (uint32_t)i0 * pS + (uint32_t)j0 * rS;
 #pragma unroll
-    for (int r = -1; r < R; ++r) {
-        if (r >= nr) break;
-        const uint32_t q = o + (uint32_t)(r * (int)sz);
-        exc[r + 1] = O::ld(a.ex, q); eyc[r + 1] = O::ld(a.ey, q); ezc[r + 1] = O::ld(a.ez, q);
-    }
-    // H^{n+1/2} Hy, Hz of plane i0-1 (the x halo): the boundary plane for i0 = 1
-    {
-        const Coef<T> px = coef_at<T, (CPML & 1) != 0, false, true>(a, a.den_hx, a.cdhx, a.den_ex, a.cdex, i0 - 1, a.nx, 1u,
-                                                             2u);
+        for (int r = 0; r < R; ++r) {
+            if (r >= nr) break;
+            const uint32_t u = u0 + (uint32_t)r * rS;
+            exc[r] = ldu(a.ex, u, kS); eyc[r] = ldu(a.ey, u, kS); ezc[r] = ldu(a.ez, u, kS);
+        }
+        if (yhalo) { exh = ldu(a.ex, u0 - rS, kS); eyh = ldu(a.ey, u0 - rS, kS); ezh = ldu(a.ez, u0 - rS, kS); }
+        // H^{n+1/2} Hy, Hz of plane i0-1 (i0 = 1: the boundary plane, read)
+        const Coef<T> px = coef_at<T, (CPML & 1) != 0, false, true>(a, a.den_hx, a.cdhx, a.den_ex, a.cdex, i0 - 1, a.nx,
+                                                                   1u, 2u);
         const T cx = i0 > 1 ? ldc(cxs, i0 - 1) : (T)0;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             if (r >= nr) break;
-            const uint32_t p = o + (uint32_t)(r * (int)sz) - sx;
-            T hx = (T)0, hy = O::ld(a.hy, p), hz = O::ld(a.hz, p);
+            const uint32_t u = u0 + (uint32_t)r * rS - pS;
+            T hx = (T)0, hy = ldu(a.hy, u, kS), hz = ldu(a.hz, u, kS);
             if (i0 > 1) {
-                const T exq = O::ld(a.ex, p), eyq = O::ld(a.ey, p), ezq = O::ld(a.ez, p);
-                const T exj = O::ld(a.ex, p + sz);
-                const T ex63 = (full && lane == 63) ? O::ld(a.ex, p + S) : (T)0;
+                const T exq = ldu(a.ex, u, kS), eyq = ldu(a.ey, u, kS), ezq = ldu(a.ez, u, kS);
+                const T exj = ldu(a.ex, u + rS, kS);
+                const T ex63 = (full && lane == 63) ? ldu(a.ex, u, kS + S) : (T)0;
                 const T exk = dpp_shl1(ex63, exq);
-                h_halo<T, CPML, TF, false, true, true>(a, fo, i0 - 1, j0 + r, kc, cx, px, ry[r + 1], lz, tfl, ezq,
-                                                              (T)0, ezc[r + 1], eyq, (T)0, eyc[r + 1], exq, exk, exj,
-                                                              hx, hy, hz);
+                const Psi<T> ps = psi_wg<T, CPML, false, false, true, true>(a, i0 - 1, j0 + r, kS, zhS, px, ry[r + 1], lz,
+                                                                          zface);
+                h_wg<T, CPML, TF, false, true, true, false>(a, fo, tfv, i0 - 1, j0 + r, k, kS, zhS, cx, px, ry[r + 1], lz, ps,
+                                                           ezq, (T)0, ezc[r], eyq, (T)0, eyc[r], exq, exk, exj, hx, hy,
+                                                           hz, false);
             }
             hyp[r] = hy; hzp[r] = hz;
         }
+        const int pb = i0 & 1;
+        erow[((pb * NW + wv) * 2 + 0) * 64 + lane] = exc[0];
+        erow[((pb * NW + wv) * 2 + 1) * 64 + lane] = ezc[0];
+        if (lane == 0) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (r >= nr) break;
+                elz[((pb * NW + wv) * R + r) * 2 + 0] = eyc[r];
+                elz[((pb * NW + wv) * R + r) * 2 + 1] = exc[r];
+            }
+        }
     }
-    // Each plane in two phases: every load of the plane first (so the
-    // wavefront waits on memory once per plane), then the arithmetic.
-    for (int i = i0; i <= i1; ++i, o += sx) {
-        const T cx = ldc(cxs, i);
-        const T dex = ldc(a.den_ex, i), cex = sizeof(T) == 4 ? ldc(a.cdex, i) : (T)0;
-        const Coef<T> px = coef_at<T, (CPML & 1) != 0, false, true>(a, a.den_hx, a.cdhx, a.den_ex, a.cdex, i, a.nx, 1u, 2u);
-        // -- loads --
-        // E^n of plane i+1 (the next plane's current values and this plane's i+1 neighbours)
-        T exn[R + 1], eyn[R + 1], ezn[R + 1];
-#pragma unroll
-        for (int r = -1; r < R; ++r) {
-            if (r >= nr) break;
-            const uint32_t q = o + (uint32_t)(r * (int)sz) + sx;
-            exn[r + 1] = O::ld(a.ex, q); eyn[r + 1] = O::ld(a.ey, q); ezn[r + 1] = O::ld(a.ez, q);
-        }
-        const uint32_t ob = o + (uint32_t)(nr * (int)sz);          // row j0+nr
-        const T ezB = O::ld(a.ez, ob), exB = O::ld(a.ex, ob);
-        // halo row j0-1: Hx, Hz (the boundary row for j0 = 1), its psi and Ey(k0+64)
-        T hxm = O::ld(a.hx, o - sz), hzm = O::ld(a.hz, o - sz);
-        PsiOff pom;
-        Psi<T> psm;
-        T eym63 = (T)0;
-        if (yhalo) {
-            pom = psi_off<T, false>(a, i, j0 - 1, kc, px, ry[0], lz);
-            psm = psi_read<T, CPML, false, true, false, true>(a, pom, px, ry[0], lz);
-            if (full && lane == 63) eym63 = O::ld(a.ey, o - sz + S);
-        }
-        // per row: H^{n-1/2}, E(k+1) of lane 63, psi of H and E, class tag,
-        // and lane 0's inputs for H at k0-1
-        T hxo[R], hyo[R], hzo[R], ex63[R], ey63[R];
-        T hx0[R], hy0[R], ezh[R], ezjh[R], eznh[R], eyh[R], exh[R];
+    __syncthreads();
+    for (int i = i0; i <= i1; ++i) {
+        const int pb = i & 1, nb = pb ^ 1;
+        const uint32_t u0 = (uint32_t)i * pS + (uint32_t)max(j0, 0) * rS;
+        T exn[R], eyn[R], ezn[R], hx[R], hy[R], hz[R], ex63[R], ey63[R], hx0[R], hy0[R];
+        Psi<T> psh[R], pse[R];
         int tg[R];
-        PsiOff poh[R], poe[R], pol[R];
-        Psi<T> psh[R], pse[R], psl[R];
+        T exnh = (T)0, eynh = (T)0, eznh = (T)0, hxm = (T)0, hzm = (T)0, eym63 = (T)0, ezB = (T)0, exB = (T)0;
+        T dex = (T)0, cex = (T)0;
+        Coef<T> px;
+        if (nr > 0) {
+            const T cx = ldc(cxs, i);
+            dex = ldc(a.den_ex, i);
+            cex = sizeof(T) == 4 ? ldc(a.cdex, i) : (T)0;
+            px = coef_at<T, (CPML & 1) != 0, false, true>(a, a.den_hx, a.cdhx, a.den_ex, a.cdex, i, a.nx, 1u, 2u);
+            // -- loads: E^n of plane i+1, H^{n-1/2}, psi and tags of the rows; the edges' inputs --
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            if (r >= nr) break;
-            const int j = j0 + r;
-            const uint32_t q = o + (uint32_t)(r * (int)sz);
-            const Coef<T>& cy = ry[r + 1];
-            poh[r] = psi_off<T, false>(a, i, j, kc, px, cy, lz);
-            poe[r] = psi_off<T, true>(a, i, j, kc, px, cy, lz);
-            psh[r] = psi_read<T, CPML, false, true, true, true>(a, poh[r], px, cy, lz);
-            pse[r] = psi_read<T, CPML, true, true, true, true>(a, poe[r], px, cy, lz);
-            tg[r] = (MED != 0 && a.tag != nullptr) ? O::tag(a.tag, q) : 0;
-            hxo[r] = O::ld(a.hx, q); hyo[r] = O::ld(a.hy, q); hzo[r] = O::ld(a.hz, q);
-            ex63[r] = (T)0; ey63[r] = (T)0;
-            if (full && lane == 63) { ex63[r] = O::ld(a.ex, q + S); ey63[r] = O::ld(a.ey, q + S); }
-            hx0[r] = (T)0; hy0[r] = (T)0;
-            ezh[r] = ezjh[r] = eznh[r] = eyh[r] = exh[r] = (T)0;
+            for (int r = 0; r < R; ++r) {
+                if (r >= nr) break;
+                const uint32_t u = u0 + (uint32_t)r * rS;
+                exn[r] = ldu(a.ex, u + pS, kS); eyn[r] = ldu(a.ey, u + pS, kS); ezn[r] = ldu(a.ez, u + pS, kS);
+                hx[r] = ldu(a.hx, u, kS); hy[r] = ldu(a.hy, u, kS); hz[r] = ldu(a.hz, u, kS);
+                psh[r] = psi_wg<T, CPML, false, true, true, true>(a, i, j0 + r, kS, zhS, px, ry[r + 1], lz, zface);
+                pse[r] = psi_wg<T, CPML, true, true, true, true>(a, i, j0 + r, kS, zeS, px, ry[r + 1], lz, zface);
+                tg[r] = (MED != 0 && a.tag != nullptr) ? (int)ldb(a.tag, u / S + (uint32_t)(kc + a.off)) : 0;
+                ex63[r] = ey63[r] = hx0[r] = hy0[r] = (T)0;
+                if (zhi && full && lane == 63) { ex63[r] = ldu(a.ex, u, kS + S); ey63[r] = ldu(a.ey, u, kS + S); }
+                if (zlo && lane == 0) { hx0[r] = ldu(a.hx, u, kS - S); hy0[r] = ldu(a.hy, u, kS - S); }
+            }
+            Psi<T> psm;
+            if (yhalo) {
+                exnh = ldu(a.ex, u0 - rS + pS, kS); eynh = ldu(a.ey, u0 - rS + pS, kS); eznh = ldu(a.ez, u0 - rS + pS, kS);
+                if (full && lane == 63) eym63 = ldu(a.ey, u0 - rS, kS + S);
+                psm = psi_wg<T, CPML, false, true, false, true>(a, i, j0 - 1, kS, zhS, px, ry[0], lz, zface);
+            }
+            if (ybot) { hxm = ldu(a.hx, u0 - rS, kS); hzm = ldu(a.hz, u0 - rS, kS); }
+            if (!top_lds) {
+                ezB = ldu(a.ez, u0 + (uint32_t)nr * rS, kS);
+                exB = ldu(a.ex, u0 + (uint32_t)nr * rS, kS);
+            }
+            // -- hand E^n of plane i+1 to the wavefronts below and before; take this plane's from above and after --
+            erow[((nb * NW + wv) * 2 + 0) * 64 + lane] = exn[0];
+            erow[((nb * NW + wv) * 2 + 1) * 64 + lane] = ezn[0];
             if (lane == 0) {
-                hx0[r] = O::ld(a.hx, q - S); hy0[r] = O::ld(a.hy, q - S);
-                if (kh >= 1) {
-                    ezh[r] = O::ld(a.ez, q - S); ezjh[r] = O::ld(a.ez, q - S + sz); eznh[r] = O::ld(a.ez, q - S + sx);
-                    eyh[r] = O::ld(a.ey, q - S); exh[r] = O::ld(a.ex, q - S);
-                    pol[r] = psi_off<T, false>(a, i, j, kh, px, cy, lh);
-                    psl[r] = psi_read<T, CPML, false, true, true, false>(a, pol[r], px, cy, lh);
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    if (r >= nr) break;
+                    elz[((nb * NW + wv) * R + r) * 2 + 0] = eyn[r];
+                    elz[((nb * NW + wv) * R + r) * 2 + 1] = exn[r];
+                }
+            }
+            if (top_lds) {
+                exB = erow[((pb * NW + wv + NZ) * 2 + 0) * 64 + lane];
+                ezB = erow[((pb * NW + wv + NZ) * 2 + 1) * 64 + lane];
+            }
+            if (!zhi) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    if (r >= nr) break;
+                    ey63[r] = elz[((pb * NW + wv + 1) * R + r) * 2 + 0];
+                    ex63[r] = elz[((pb * NW + wv + 1) * R + r) * 2 + 1];
+                }
+            }
+            // -- H^{n+1/2} of the rows (and of row j0-1) --
+            if (yhalo) {
+                const T eyk = dpp_shl1(eym63, eyh);
+                T hyd = (T)0;
+                h_wg<T, CPML, TF, true, false, true, false>(a, fo, tfv, i, j0 - 1, k, kS, zhS, cx, px, ry[0], lz, psm, ezh,
+                                                           ezc[0], (T)0, eyh, eyk, eynh, exh, (T)0, exc[0], hxm, hyd, hzm,
+                                                           false);
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (r >= nr) break;
+                const T ezj = r + 1 < nr ? ezc[r + 1 < R ? r + 1 : 0] : ezB;
+                const T exj = r + 1 < nr ? exc[r + 1 < R ? r + 1 : 0] : exB;
+                const T exk = dpp_shl1(ex63[r], exc[r]), eyk = dpp_shl1(ey63[r], eyc[r]);
+                h_wg<T, CPML, TF, true, true, true, true>(a, fo, tfv, i, j0 + r, k, kS, zhS, cx, px, ry[r + 1], lz, psh[r],
+                                                         ezc[r], ezj, ezn[r], eyc[r], eyk, eyn[r], exc[r], exk, exj,
+                                                         hx[r], hy[r], hz[r], act);
+            }
+            // -- hand H^{n+1/2} to the wavefronts above and after --
+            if (nr == R) {
+                hrow[((pb * NW + wv) * 2 + 0) * 64 + lane] = hx[R - 1];
+                hrow[((pb * NW + wv) * 2 + 1) * 64 + lane] = hz[R - 1];
+            }
+            if (lane == 63) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    if (r >= nr) break;
+                    hlz[((pb * NW + wv) * R + r) * 2 + 0] = hx[r];
+                    hlz[((pb * NW + wv) * R + r) * 2 + 1] = hy[r];
                 }
             }
         }
-        // -- arithmetic --
-        if (yhalo) {
-            const T eyk = dpp_shl1(eym63, eyc[0]);
-            T hy = (T)0;
-            h_fused<T, CPML, TF, true, false, true, false>(a, fo, i, j0 - 1, kc, cx, px, ry[0], lz, psm, pom, tfl, ezc[0],
-                                                          ezc[1], (T)0, eyc[0], eyk, eyn[0], exc[0], (T)0, exc[1],
-                                                          hxm, hy, hzm);
-        }
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            if (r >= nr) break;
-            const int j = j0 + r;
-            const uint32_t q = o + (uint32_t)(r * (int)sz);
-            const Coef<T>& cy = ry[r + 1];
-            T hx = hxo[r], hy = hyo[r], hz = hzo[r];
-            const T exk = dpp_shl1(ex63[r], exc[r + 1]), eyk = dpp_shl1(ey63[r], eyc[r + 1]);
-            const T ezj = r + 1 < nr ? ezc[r + 1 < R ? r + 2 : R] : ezB;
-            const T exj = r + 1 < nr ? exc[r + 1 < R ? r + 2 : R] : exB;
-            // lane 0: H of cell k0-1 (the boundary k = 0 for the first z tile)
-            T hxk0 = hx0[r], hyk0 = hy0[r];
-            if (lane == 0 && kh >= 1) {
-                T hzd = (T)0;
-                h_fused<T, CPML, TF, true, true, false, false>(a, fo, i, j, kh, cx, px, cy, lh, psl[r], pol[r], tfh, ezh[r],
-                                                              ezjh[r], eznh[r], eyh[r], eyc[r + 1], (T)0, exh[r],
-                                                              exc[r + 1], (T)0, hxk0, hyk0, hzd);
+        __syncthreads();
+        if (nr > 0) {
+            if (!ybot) {
+                hxm = hrow[((pb * NW + wv - NZ) * 2 + 0) * 64 + lane];
+                hzm = hrow[((pb * NW + wv - NZ) * 2 + 1) * 64 + lane];
             }
-            // this cell's H^{n+1/2} (psi of H stored by lanes inside the grid)
-            h_fused<T, CPML, TF, true, true, true, true>(a, fo, i, j, kc, cx, px, cy, lz, psh[r], poh[r], tfl, ezc[r + 1],
-                                                        ezj, ezn[r + 1], eyc[r + 1], eyk, eyn[r + 1], exc[r + 1], exk,
-                                                        exj, hx, hy, hz, act);
-            const T hxk = dpp_shr1(hxk0, hx), hyk = dpp_shr1(hyk0, hy);
-            const bool own = MED != 1 || tg[r] < a.disp_tag0;
-            T ex = exc[r + 1], ey = eyc[r + 1], ez = ezc[r + 1];
-            e_fused<T, CPML, MED>(a, tg[r], own && act, dex, cex, px, cy, lz, pse[r], poe[r], hz, hzm, hzp[r], hy,
-                                  hyk, hyp[r], hx, hxm, hxk, ex, ey, ez);
-            if (act) {
-                O::st(fo.hx, q, hx); O::st(fo.hy, q, hy); O::st(fo.hz, q, hz);
-                if (own) { O::st(fo.ex, q, ex); O::st(fo.ey, q, ey); O::st(fo.ez, q, ez); }
-            }
-            hyp[r] = hy; hzp[r] = hz;
-            hxm = hx; hzm = hz;
-        }
+            if (!zlo) {
 #pragma unroll
-        for (int r = 0; r <= R; ++r) {
-            if (r > nr) break;
-            exc[r] = exn[r]; eyc[r] = eyn[r]; ezc[r] = ezn[r];
+                for (int r = 0; r < R; ++r) {
+                    if (r >= nr) break;
+                    hx0[r] = hlz[((pb * NW + wv - 1) * R + r) * 2 + 0];
+                    hy0[r] = hlz[((pb * NW + wv - 1) * R + r) * 2 + 1];
+                }
+            }
+            // -- E^{n+1} of the rows, stores --
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (r >= nr) break;
+                const uint32_t u = u0 + (uint32_t)r * rS;
+                const T hxk = dpp_shr1(hx0[r], hx[r]), hyk = dpp_shr1(hy0[r], hy[r]);
+                const T hzj = r > 0 ? hz[r > 0 ? r - 1 : 0] : hzm;
+                const T hxj = r > 0 ? hx[r > 0 ? r - 1 : 0] : hxm;
+                const bool own = MED != 1 || tg[r] < a.disp_tag0;
+                T ex = exc[r], ey = eyc[r], ez = ezc[r];
+                e_wg<T, CPML, MED, TF>(a, fo, tfv, i, j0 + r, k, kS, zeS, tg[r], own && act, dex, cex, px, ry[r + 1], lz,
+                                       pse[r], hz[r], hzj, hzp[r], hy[r], hyk, hyp[r], hx[r], hxj, hxk, ex, ey, ez);
+                if (act) {
+                    stu(fo.hx, u, kS, hx[r]); stu(fo.hy, u, kS, hy[r]); stu(fo.hz, u, kS, hz[r]);
+                    if (own) { stu(fo.ex, u, kS, ex); stu(fo.ey, u, kS, ey); stu(fo.ez, u, kS, ez); }
+                }
+                hyp[r] = hy[r]; hzp[r] = hz[r];
+                exc[r] = exn[r]; eyc[r] = eyn[r]; ezc[r] = ezn[r];
+            }
+            if (yhalo) { exh = exnh; eyh = eynh; ezh = eznh; }
         }
+    }
+}
+
+
+// After the workgroup-tile fused step: the cells of dispersive classes
+// (workgroups [0, nd), one cell per thread, as k_e_dispersive) and, when
+// the kernel added the TF/SF E corrections, the 1D line's E half step with
+// the source (the remaining workgroups, as the end of k_tfsf_e).  The two
+// parts touch disjoint data.
+template <typename T, bool CPML, bool B32>
+__global__ __launch_bounds__(256) void k_wg_post(const YeeArgs<T> a, T* __restrict__ exo, T* __restrict__ eyo,
+                                                 T* __restrict__ ezo, const long* __restrict__ list, int n, int nd,
+                                                 TfsfArgs t, double* __restrict__ ex1, const double* __restrict__ hy1,
+                                                 double src) {
+    const int b = (int)blockIdx.x;
+    if (b < nd) {
+        const int p = b * (int)blockDim.x + (int)threadIdx.x;
+        if (p < n) e_dispersive_cell<T, CPML, B32>(a, exo, eyo, ezo, list[p], p);
+        return;
+    }
+    const int m = (b - nd) * (int)blockDim.x + (int)threadIdx.x;
+    if (m >= 1 && m < t.n1 - 1) {
+        double v = ex1[m] + t.dt_eps0 * t.inv_dr * (hy1[m - 1] - hy1[m]);
+        if (m == t.ks - t.kb) v += src;
+        ex1[m] = v;
     }
 }
 

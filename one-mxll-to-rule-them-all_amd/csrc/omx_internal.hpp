@@ -152,8 +152,9 @@ struct omx3d_ctx {
     void* f_alt[6] = {};        // second buffers of the fused step
     void* psi_h_alt[6] = {};
     int step_mode = 1;          // 0: fused step where it applies, 1: separate H and E kernels
-    int fused_rows = 4;         // rows per wavefront of the fused step (2 or 4)
-    int fused_xchunk = 16;      // planes per wavefront of the fused step
+    int wg_rows = 2;            // fused step: rows per wavefront (1 or 2)
+    int wg_xchunk = 32;         // fused step: planes per workgroup tile
+    int wg_ny = 0;              // fused step: wavefronts along y per workgroup (0: chosen from the z extent)
     int src_in_dispersive = -1; // -1 unknown, 0 no, 1 a source point sits in a Drude/DL component
     bool last_fused = false;
     // FP64 master coefficients: be ce (npml+1) bh ch (npml) den_e{x,y,z} den_h{x,y,z} (n+2 each)

@@ -121,9 +121,13 @@ class Grid3D:
         L.call("omx3d_get_source_patch", self.h, C.byref(r), C.byref(nb), C.byref(nc))
         return r.value, nb.value, nc.value
 
-    def set_step_mode(self, mode=0, rows=0, xchunk=0):
-        """mode 0: fused H+E pass where it applies; 1: separate H and E kernels."""
+    def set_step_mode(self, mode=0, rows=0, xchunk=0, ywaves=None):
+        """mode 0: fused H+E pass where it applies; 1: separate H and E
+        kernels.  Fused tile: rows (1/2) per wavefront, xchunk planes per
+        workgroup, ywaves wavefronts along y per workgroup (0 automatic)."""
         L.call("omx3d_set_step_mode", self.h, int(mode), int(rows), int(xchunk))
+        if ywaves is not None:
+            L.call("omx3d_set_fused_waves", self.h, int(ywaves))
 
     def last_run_fused(self):
         f = C.c_int32()

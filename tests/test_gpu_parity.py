@@ -266,22 +266,48 @@ def test_tiling_variants_identical():
             np.testing.assert_array_equal(rec, ref)
         g.kill()
     # fused H+E step under several tile shapes (rows per wavefront, planes
-    # per chunk): each equal to the separate kernels up to rounding (the
-    # point source enters E at its own cell after the update, and each row
-    # count is its own kernel, contracted to multiply-adds on its own)
-    for rows, xc in ((4, 8), (2, 1), (4, 3), (1, 64), (2, 5)):
+    # per workgroup chunk, wavefronts along y): each equal to the separate
+    # kernels up to rounding (the point source enters E at its own cell after
+    # the update, and each row count is its own kernel, contracted to
+    # multiply-adds on its own)
+    for rows, xc, ny in ((2, 8, 0), (1, 1, 1), (2, 3, 3), (1, 64, 8), (2, 5, 1), (1, 7, 2)):
         dr, dt = case.dr_dt
         g = omx_fdtd.Grid3D(case.grid, case.npml, case.boundaries, dt, dr, precision=8)
-        g.set_step_mode(0, rows, xc)
+        g.set_step_mode(0, rows, xc, ny)
         cases.apply(case, g, library=G.library_medium, make_medium=G.medium)
         g.run(1, case.steps, 1)
         assert g.last_run_fused()
         rec = g.read_records()
         r = rel_l2(rec, ref)
         m = np.abs(rec - ref).max() / np.abs(ref).max()
-        print(f"fused rows {rows} xchunk {xc}: relL2 vs separate {r:.3e}, max |diff|/max|ref| {m:.3e}")
+        print(f"fused rows {rows} xchunk {xc} y-waves {ny}: relL2 vs separate {r:.3e}, max |diff|/max|ref| {m:.3e}")
         assert r <= 1e-13 and m <= 1e-13
         g.kill()
+
+
+@pytest.mark.parametrize("precision", [8, 4])
+def test_fused_z_face_across_two_wavefronts(precision):
+    """nz = 70 with CPML 8: the high z face (k = 63..69) spans the first and
+    the second wavefront along z of a fused workgroup.  The fused step must
+    equal the separate kernels there (same arithmetic per cell)."""
+    spec = ("drude-lorentz", "Ag", 1.0)
+    case = small_case("zface2w", n=(20, 18, 70), npml=8, media=[spec], shapes=[(1, 1, [10.0, 9.0, 40.0, 3.0])],
+                      steps=60)
+    case.detectors.append((DET_PLANE_XY, EZ, cases.plane_xy(case.grid, 65)))
+    out = []
+    for mode in (1, 0):
+        dr, dt = case.dr_dt
+        g = omx_fdtd.Grid3D(case.grid, case.npml, case.boundaries, dt, dr, precision=precision)
+        g.set_step_mode(mode, 2 if mode == 0 else 0, 4 if mode == 0 else 0)
+        cases.apply(case, g, library=G.library_medium, make_medium=G.medium)
+        g.run(1, case.steps, 1)
+        assert g.last_run_fused() == (mode == 0)
+        out.append(g.read_records())
+        g.kill()
+    assert np.abs(out[0]).max() > 0
+    r = rel_l2(out[1], out[0])
+    print(f"FP{8 * precision}: fused vs separate relL2 = {r:.3e}")
+    assert r <= (1e-13 if precision == 8 else 5e-6)
 
 
 @pytest.mark.parametrize("precision", [8, 4])
