@@ -294,6 +294,16 @@ int omx3d_device_bytes(omx3d_ctx* ctx, int64_t* bytes);
  * height).  No reference counterpart (engine tuning). */
 int omx3d_autotune(omx3d_ctx* ctx, int32_t steps, int32_t* rows_y, int32_t* xchunk, int32_t* tile_order,
                    double* ms_per_step);
+/* Step-kernel autotuner: times `steps` steps of the separate H and E
+ * kernels (at the current tiling) and, where the fused step applies, of
+ * fused workgroup tiles of several shapes, on the grid itself (TF/SF and
+ * near-field patches included, sources and detectors left out); keeps the
+ * faster step mode and fused shape (omx3d_set_step_mode), frees the fused
+ * step's second buffers when the separate kernels win, then clears fields,
+ * CPML, dispersive state and the TF/SF line (call before a run).  *mode: 0
+ * fused, 1 separate; *ms_fused < 0 when the fused step does not apply.  No
+ * reference counterpart (engine tuning). */
+int omx3d_autotune_step(omx3d_ctx* ctx, int32_t steps, int32_t* mode, double* ms_separate, double* ms_fused);
 
 /* ---- batches of independent grids (no reference counterpart) ------------ */
 /* One launch per phase (H, sources, E, detectors) covers every grid of the

@@ -666,6 +666,33 @@ int ensure_alt(omx3d_ctx* c) {
     return OMX_OK;
 }
 
+// Back to the state before the first step: fields (both buffer sets), CPML
+// psi (both H sets), dispersive state and the TF/SF line at zero, near-field
+// patches rebuilt from the fields at the next launch (the tuners time steps
+// on the grid itself and leave it as they found it, before a run).
+int reset_run_state(omx3d_ctx* c) {
+    const size_t fb = c->esz * c->L.nalloc;
+    for (auto p : c->f) HIPCHK(hipMemsetAsync(p, 0, fb, c->s));
+    for (auto p : c->f_alt) if (p) HIPCHK(hipMemsetAsync(p, 0, fb, c->s));
+    const size_t A = 128 / c->esz;
+    const size_t xs[2] = {c->nsh * c->L.sx + A, c->nse * c->L.sx + A};
+    const size_t ys[2] = {(size_t)(c->d.nx + 2) * c->nsh * c->L.sz + A, (size_t)(c->d.nx + 2) * c->nse * c->L.sz + A};
+    const size_t zs[2] = {(size_t)zpsi_elems(c->L.sy, c->d.nx, c->d.npml), (size_t)zpsi_elems(c->L.sy, c->d.nx, c->d.npml)};
+    for (int a = 0; a < 3; ++a) {
+        const size_t* sz = a == 0 ? xs : (a == 1 ? ys : zs);
+        for (int q = 0; q < 2; ++q) {
+            if (c->psi_h[2 * a + q]) HIPCHK(hipMemsetAsync(c->psi_h[2 * a + q], 0, c->esz * sz[0], c->s));
+            if (c->psi_h_alt[2 * a + q]) HIPCHK(hipMemsetAsync(c->psi_h_alt[2 * a + q], 0, c->esz * sz[0], c->s));
+            if (c->psi_e[2 * a + q]) HIPCHK(hipMemsetAsync(c->psi_e[2 * a + q], 0, c->esz * sz[1], c->s));
+        }
+    }
+    if (c->disp_state) HIPCHK(hipMemsetAsync(c->disp_state, 0, disp_state_bytes(c), c->s));
+    if (c->d_tf) HIPCHK(hipMemsetAsync(c->d_tf, 0, sizeof(double) * 2 * (size_t)c->tf_n, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    c->patch_dirty = true;
+    return OMX_OK;
+}
+
 // Whether a source point lies in a Drude/DL component (the fused step adds
 // sources after the E update, which equals the reference order only where
 // the E update is linear in E without memory, i.e. vacuum and dielectric).
@@ -2449,6 +2476,82 @@ int omxb_destroy(omxb_ctx* b) {
 // Times the tile shapes (rows_y in {1,2,4} x xchunk in {1,2,4,8,16} x tile
 // order 0/1/2) with `steps` H+E steps each and keeps the fastest.  The fields,
 // CPML and dispersive state are cleared afterwards, so call it before a run.
+int omx3d_autotune_step(omx3d_ctx* c, int32_t steps, int32_t* mode, double* ms_separate, double* ms_fused) {
+    CHECK_CTX(c);
+    if (steps < 1) return fail(OMX_ERR_ARG, "steps must be >= 1");
+    int rc = check_sources_dispersive(c);
+    if (rc) return rc;
+    rc = tfsf_fold_check(c);
+    if (rc) return rc;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    // times `steps` steps of one kind after two untimed ones (fields as they
+    // come: the state is cleared at the end)
+    auto timed = [&](bool fused, double& ms) -> int {
+        const bool fold = !fused && c->tfsf, fold_e = fold && c->tf_fold_e == 1;
+        const bool fold_wg = fused && c->tfsf && c->tf_fold_e == 1;
+        for (int t = 0; t < steps + 2; ++t) {
+            if (t == 2) HIPCHK(hipEventRecord(e0, c->s));
+            int r;
+            if (fused) {
+                r = c->prec == 4 ? launch_fused<float>(c, nullptr, nullptr, fold_wg, 0.0)
+                                 : launch_fused<double>(c, nullptr, nullptr, fold_wg, 0.0);
+            } else {
+                r = c->prec == 4 ? launch_h<float>(c, fold) : launch_h<double>(c, fold);
+                if (!r)
+                    r = c->prec == 4 ? launch_e<float>(c, 0.0, 0.0, fold_e, 0.0)
+                                     : launch_e<double>(c, 0.0, 0.0, fold_e, 0.0);
+            }
+            if (!r && !fold_e && !fold_wg)
+                r = c->prec == 4 ? launch_tfsf_e<float>(c, 0.0) : launch_tfsf_e<double>(c, 0.0);
+            if (r) return r;
+        }
+        HIPCHK(hipEventRecord(e1, c->s));
+        HIPCHK(hipEventSynchronize(e1));
+        float f = 0.f;
+        HIPCHK(hipEventElapsedTime(&f, e0, e1));
+        ms = f / steps;
+        return OMX_OK;
+    };
+    const int mode0 = c->step_mode, r0 = c->wg_rows, x0 = c->wg_xchunk, y0 = c->wg_ny;
+    double t_sep = 0.0, t_fus = -1.0;
+    rc = timed(false, t_sep);
+    int best_r = r0, best_x = x0, best_y = y0;
+    c->step_mode = 0;
+    if (!rc && fused_applies(c)) {
+        // fused tile shapes: rows per wavefront, planes per chunk, wavefronts along y
+        const int cand32[5][3] = {{2, 32, 0}, {2, 16, 0}, {1, 32, 0}, {2, 32, 2}, {2, 64, 0}};
+        const int cand64[4][3] = {{1, 32, 0}, {1, 16, 0}, {2, 32, 0}, {1, 32, 2}};
+        const int nc = c->prec == 4 ? 5 : 4;
+        for (int q = 0; q < nc && !rc; ++q) {
+            const int* cd = c->prec == 4 ? cand32[q] : cand64[q];
+            c->wg_rows = cd[0]; c->wg_xchunk = cd[1]; c->wg_ny = cd[2];
+            double t = 0.0;
+            rc = timed(true, t);
+            if (!rc && (t_fus < 0.0 || t < t_fus)) { t_fus = t; best_r = cd[0]; best_x = cd[1]; best_y = cd[2]; }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) { c->step_mode = mode0; c->wg_rows = r0; c->wg_xchunk = x0; c->wg_ny = y0; return rc; }
+    const bool use_fused = t_fus >= 0.0 && t_fus < t_sep;
+    c->step_mode = use_fused ? 0 : 1;
+    c->wg_rows = best_r; c->wg_xchunk = best_x; c->wg_ny = best_y;
+    if (!use_fused) {
+        // the separate kernels do not need the fused step's second buffers
+        HIPCHK(hipStreamSynchronize(c->s));
+        for (auto& p : c->f_alt) dfree(p);
+        for (auto& p : c->psi_h_alt) dfree(p);
+    }
+    rc = reset_run_state(c);
+    if (rc) return rc;
+    if (mode) *mode = c->step_mode;
+    if (ms_separate) *ms_separate = t_sep;
+    if (ms_fused) *ms_fused = t_fus;
+    return OMX_OK;
+}
+
 int omx3d_autotune(omx3d_ctx* c, int32_t steps, int32_t* rows_y, int32_t* xchunk, int32_t* tile_order,
                    double* ms_per_step) {
     CHECK_CTX(c);
@@ -2492,24 +2595,8 @@ int omx3d_autotune(omx3d_ctx* c, int32_t steps, int32_t* rows_y, int32_t* xchunk
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     c->rows_y = br; c->xchunk = bx; c->tile_order = bo;
-    // back to the initial state: zero fields, psi and dispersive state
-    const size_t fb = c->esz * c->L.nalloc;
-    for (auto p : c->f) HIPCHK(hipMemsetAsync(p, 0, fb, c->s));
-    for (auto p : c->f_alt) if (p) HIPCHK(hipMemsetAsync(p, 0, fb, c->s));
-    const size_t A = 128 / c->esz;
-    const size_t xs[2] = {c->nsh * c->L.sx + A, c->nse * c->L.sx + A};
-    const size_t ys[2] = {(size_t)(c->d.nx + 2) * c->nsh * c->L.sz + A, (size_t)(c->d.nx + 2) * c->nse * c->L.sz + A};
-    const size_t zs[2] = {(size_t)zpsi_elems(c->L.sy, c->d.nx, c->d.npml), (size_t)zpsi_elems(c->L.sy, c->d.nx, c->d.npml)};
-    for (int a = 0; a < 3; ++a) {
-        const size_t* sz = a == 0 ? xs : (a == 1 ? ys : zs);
-        for (int q = 0; q < 2; ++q) {
-            if (c->psi_h[2 * a + q]) HIPCHK(hipMemsetAsync(c->psi_h[2 * a + q], 0, c->esz * sz[0], c->s));
-            if (c->psi_e[2 * a + q]) HIPCHK(hipMemsetAsync(c->psi_e[2 * a + q], 0, c->esz * sz[1], c->s));
-        }
-    }
-    if (c->disp_state) HIPCHK(hipMemsetAsync(c->disp_state, 0, disp_state_bytes(c), c->s));
-    HIPCHK(hipStreamSynchronize(c->s));
-    c->patch_dirty = true;
+    int rc = reset_run_state(c);
+    if (rc) return rc;
     if (rows_y) *rows_y = br;
     if (xchunk) *xchunk = bx;
     if (tile_order) *tile_order = bo;

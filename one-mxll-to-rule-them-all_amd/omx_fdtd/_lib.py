@@ -104,6 +104,7 @@ _SIGS = {
     "omx3d_synchronize": ([_P], C.c_int),
     "omx3d_device_bytes": ([_P, C.POINTER(C.c_int64)], C.c_int),
     "omx3d_autotune": ([_P, _I, _IP, _IP, _IP, _DP], C.c_int),
+    "omx3d_autotune_step": ([_P, _I, _IP, _DP, _DP], C.c_int),
     "omx3d_set_step_mode": ([_P, _I, _I, _I], C.c_int),
     "omx3d_set_fused_waves": ([_P, _I], C.c_int),
     "omx3d_get_step_mode": ([_P, _IP], C.c_int),

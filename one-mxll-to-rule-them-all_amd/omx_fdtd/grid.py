@@ -141,6 +141,14 @@ class Grid3D:
         L.call("omx3d_autotune", self.h, int(steps), C.byref(r), C.byref(x), C.byref(o), C.byref(ms))
         return r.value, x.value, o.value, ms.value
 
+    def autotune_step(self, steps=8):
+        """Keep the faster of the separate H/E kernels and the fused step
+        (several fused tile shapes tried); clears the fields (call before a
+        run).  Returns (mode, ms_separate, ms_fused or None)."""
+        m, ts, tf = C.c_int32(), C.c_double(), C.c_double()
+        L.call("omx3d_autotune_step", self.h, int(steps), C.byref(m), C.byref(ts), C.byref(tf))
+        return m.value, ts.value, (tf.value if tf.value >= 0.0 else None)
+
     # -- coefficients ------------------------------------------------------
     def cpml_tables(self):
         p = self.npml

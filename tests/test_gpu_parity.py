@@ -286,6 +286,48 @@ def test_tiling_variants_identical():
 
 
 @pytest.mark.parametrize("precision", [8, 4])
+def test_autotune_step_leaves_the_grid_as_found(precision):
+    """omx3d_autotune_step times both step modes on the grid itself (TF/SF
+    line, DL sphere, CPML), keeps one and clears the state: the run after it
+    equals a fresh grid's run with the separate kernels to rounding."""
+    case = cases.config_dl_sphere_tfsf(n=40, npml=6, steps=60, record_every=2, radius_nm=10.0)
+    dr, dt = case.dr_dt
+    ref = omx_fdtd.Grid3D(case.grid, case.npml, case.boundaries, dt, dr, precision=precision)
+    ref.set_step_mode(1)
+    cases.apply(case, ref, library=G.library_medium, make_medium=G.medium)
+    ref.run(1, case.steps, case.record_every)
+    want = ref.read_records()
+    ref.kill()
+    g = omx_fdtd.Grid3D(case.grid, case.npml, case.boundaries, dt, dr, precision=precision)
+    cases.apply(case, g, library=G.library_medium, make_medium=G.medium)
+    mode, t_sep, t_fus = g.autotune_step(steps=3)
+    print(f"FP{8 * precision}: mode {mode}, separate {t_sep:.4f} ms, fused {t_fus} ms", flush=True)
+    assert mode in (0, 1) and t_sep > 0.0 and t_fus is not None and t_fus > 0.0
+    g.run(1, case.steps, case.record_every)
+    assert g.last_run_fused() == (mode == 0)
+    got = g.read_records()
+    g.kill()
+    assert np.abs(want).max() > 0
+    r = rel_l2(got, want)
+    print(f"after the tuner vs fresh separate run: relL2 {r:.3e}", flush=True)
+    assert r <= (1e-13 if precision == 8 else 5e-6)
+
+
+def test_autotune_step_periodic_grid_keeps_separate():
+    """A periodic axis: the fused step does not apply, the tuner keeps the
+    separate kernels and reports no fused time."""
+    case = small_case("periodic", n=(20, 18, 16), bc=(BC_PERIODIC, BC_CPML, BC_CPML), steps=10)
+    dr, dt = case.dr_dt
+    g = omx_fdtd.Grid3D(case.grid, case.npml, case.boundaries, dt, dr, precision=8)
+    cases.apply(case, g, library=G.library_medium, make_medium=G.medium)
+    mode, t_sep, t_fus = g.autotune_step(steps=2)
+    assert mode == 1 and t_fus is None and t_sep > 0.0
+    g.run(1, case.steps, 1)
+    assert not g.last_run_fused()
+    g.kill()
+
+
+@pytest.mark.parametrize("precision", [8, 4])
 def test_fused_z_face_across_two_wavefronts(precision):
     """nz = 70 with CPML 8: the high z face (k = 63..69) spans the first and
     the second wavefront along z of a fused workgroup.  The fused step must
