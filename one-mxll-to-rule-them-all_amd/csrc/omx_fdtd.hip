@@ -2521,9 +2521,11 @@ int omx3d_autotune_step(omx3d_ctx* c, int32_t steps, int32_t* mode, double* ms_s
     c->step_mode = 0;
     if (!rc && fused_applies(c)) {
         // fused tile shapes: rows per wavefront, planes per chunk, wavefronts along y
-        const int cand32[5][3] = {{2, 32, 0}, {2, 16, 0}, {1, 32, 0}, {2, 32, 2}, {2, 64, 0}};
-        const int cand64[4][3] = {{1, 32, 0}, {1, 16, 0}, {2, 32, 0}, {1, 32, 2}};
-        const int nc = c->prec == 4 ? 5 : 4;
+        // (small workgroups share a CU with others, large ones recompute fewer
+        // halo rows: the best trade differs by grid content)
+        const int cand32[6][3] = {{2, 32, 0}, {2, 16, 0}, {1, 32, 0}, {2, 32, 2}, {2, 16, 1}, {2, 8, 1}};
+        const int cand64[5][3] = {{1, 32, 0}, {1, 16, 0}, {2, 32, 0}, {1, 32, 2}, {1, 16, 1}};
+        const int nc = c->prec == 4 ? 6 : 5;
         for (int q = 0; q < nc && !rc; ++q) {
             const int* cd = c->prec == 4 ? cand32[q] : cand64[q];
             c->wg_rows = cd[0]; c->wg_xchunk = cd[1]; c->wg_ny = cd[2];
