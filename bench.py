@@ -9,7 +9,11 @@ eV, two seeded Lorentz poles) in vacuum, x-polarised TF/SF plane wave along
 
 A step is one full Yee step (H and E updates with CPML, the Drude-Lorentz
 ADE, TF/SF corrections and detector recording at its cadence) over the whole
-grid, inputs resident in HBM.  The FP32 engine is timed (the headline), then
+grid, inputs resident in HBM.  Before the timed run of a single-grid workload
+the engine's tuners choose, on the grid itself, the tile shape and then the
+step kernels (separate H and E launches, or the fused H+E pass and its
+workgroup shape); the choice is recorded in the JSON line (--mode separate /
+fused forces a step mode at the default tiles).  The FP32 engine is timed (the headline), then
 the FP64 engine on the same steps; the FP32 detector series is compared
 with the FP64 one (relative L2).  The dominant kernel's roofline fraction
 comes from HIP events recorded around its launches in every PROF_EVERY-th
@@ -58,15 +62,40 @@ def log(*a):
 
 
 PATCH_RADIUS = None   # --patch-radius: FP64 near-field box radius of FP32 grids (None: the engine default)
+FORCED_TILES = None        # --tiles rows,xchunk,order: tile shape of the separate kernels (profiler runs)
+FORCED_FUSED_SHAPE = None  # --fused-shape rows,xchunk,ywaves: the fused step's tile (profiler runs)
+
+
+TUNE_STEPS = 16        # timed steps per candidate of the tile and step-kernel tuners
 
 
 def build_grid(case, precision, mode="separate"):
+    """A grid set up for `case`.  mode "auto": the project's tuners pick the
+    tile shape (omx3d_autotune) and then the step kernels, separate H/E or
+    fused, with the fused tile shape (omx3d_autotune_step), on the grid
+    itself; "separate" / "fused": that step mode at the default tiles."""
     dr, dt = case.dr_dt
     g = omx_fdtd.Grid3D(case.grid, case.npml, case.boundaries, dt, dr, precision=precision)
-    g.set_step_mode(1 if mode == "separate" else 0)
+    g.set_step_mode(0 if mode == "fused" else 1)
+    if FORCED_TILES is not None:
+        g.set_tiling(*FORCED_TILES)
+    if FORCED_FUSED_SHAPE is not None:
+        g.set_step_mode(0 if mode == "fused" else 1, *FORCED_FUSED_SHAPE)
     if PATCH_RADIUS is not None:
         g.set_source_patch(PATCH_RADIUS)
     cases.apply(case, g, library=G.library_medium, make_medium=G.medium)
+    g.tuned = None
+    if mode == "auto":
+        t0 = time.perf_counter()
+        rows, xc, order, _ = g.autotune(steps=TUNE_STEPS)
+        m, t_sep, t_fus = g.autotune_step(steps=TUNE_STEPS)
+        g.tuned = {"tiling": [rows, xc, order], "step_kernels": "fused" if m == 0 else "separate",
+                   "fused_shape": list(g.fused_shape()) if m == 0 else None,
+                   "tuner_ms_per_step": {"separate": round(t_sep, 5),
+                                         "fused": None if t_fus is None else round(t_fus, 5)},
+                   "tuning_s": round(time.perf_counter() - t0, 2)}
+        log(f"[bench] FP{8 * precision} tuned: tiles {rows}x{xc} order {order}, "
+            f"{g.tuned['step_kernels']} {g.tuned['fused_shape'] or ''} (separate {t_sep:.4f} ms, fused {t_fus} ms)")
     return g
 
 
@@ -266,7 +295,7 @@ def run_single(case, prec, W, K, mode, energy_every=0):
     ms_h, ms_e, n = g.profile()
     g.set_profiling(False)
     res = dict(secs=secs, gcells=int(np.prod(case.grid)) * K / secs / 1e9, ms_step=1e3 * secs / K,
-               ms_h=ms_h / max(n, 1), ms_e=ms_e / max(n, 1), fused=g.last_run_fused(),
+               ms_h=ms_h / max(n, 1), ms_e=ms_e / max(n, 1), fused=g.last_run_fused(), tuned=g.tuned,
                series=g.read_records(), energies=np.array(energies), dev_bytes=g.device_bytes())
     log(f"[bench] FP{8 * prec}: {res['gcells']:.2f} Gcell/s, {res['ms_step']:.3f} ms/step, "
         f"H {res['ms_h']:.3f} ms, E {res['ms_e']:.3f} ms")
@@ -291,7 +320,10 @@ def add_fp64(out, results, r32):
         r64 = results[8]
         out["fp64"] = {"value": round(r64["gcells"], 4), "ms_per_step": round(r64["ms_step"], 5),
                        "step_roofline_frac": round(r64["step_bytes"] / (r64["ms_step"] * 1e-3) / 1e9
-                                                   / HBM_PEAK_GBS, 4)}
+                                                   / HBM_PEAK_GBS, 4),
+                       "step_kernels": "fused" if r64["fused"] else "separate"}
+        if r64.get("tuned"):
+            out["fp64"]["tuning"] = r64["tuned"]
         out["fp32_over_fp64"] = round(r32["gcells"] / r64["gcells"], 3)
 
 
@@ -350,6 +382,8 @@ def single_grid_runs(case, args, W, K, workload, energy_every=0, extra=None):
 
 def finish(out, results):
     r32 = results[4]
+    if r32.get("tuned"):
+        out["config"]["tuning"] = r32["tuned"]
     out["roofline"] = r32["roof"]
     out["step_roofline_frac"] = round(r32["step_bytes"] / (r32["ms_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
     add_fp64(out, results, r32)
@@ -528,6 +562,15 @@ def bench_batched(args):
     g.kill()
     single = edge ** 3 * Ks / t_big / 1e9
     log(f"[bench] single {edge}^3 grid FP32: {single:.2f} Gcell/s")
+    # the same large grid with the tuners' choice (which may be the fused
+    # step, a kernel the batch does not run)
+    g = build_grid(big, 4, "auto")
+    big_tuning = g.tuned
+    g.run(1, 2, big.record_every)
+    t_big_t = timed_run(g, 3, Ks, big.record_every)
+    g.kill()
+    single_t = edge ** 3 * Ks / t_big_t / 1e9
+    log(f"[bench] single {edge}^3 grid FP32, tuned: {single_t:.2f} Gcell/s")
     r32 = out_prec[4]
     out = base_record("batched_64", K, W, r32["gcells"], r32["ms_step"], "fp32",
                       {"grids": len(cs), "edges": [96, 128, 160, 192], "total_cells": cells, "npml": 8,
@@ -537,6 +580,9 @@ def bench_batched(args):
     out["single_grid_gcells"] = round(single, 4)
     out["single_grid_edge"] = edge
     out["batched_over_single_grid"] = round(r32["gcells"] / single, 4)
+    out["single_grid_tuned_gcells"] = round(single_t, 4)
+    out["single_grid_tuning"] = big_tuning
+    out["batched_over_single_grid_tuned"] = round(r32["gcells"] / single_t, 4)
     out["batched_over_one_by_one"] = round(r32["gcells"] / seq, 4)
     if 8 in out_prec:
         r64 = out_prec[8]
@@ -562,18 +608,32 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="batched_64: keep the default tiling of every grid")
-    ap.add_argument("--mode", choices=("separate", "fused"), default="separate",
-                    help="separate H and E kernels, or the fused H+E pass")
+    ap.add_argument("--mode", choices=("auto", "separate", "fused"), default="auto",
+                    help="single-grid workloads: auto (the tuners pick tiles and step kernels on the grid), "
+                         "separate H and E kernels, or the fused H+E pass, at default tiles")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default=HEADLINE)
     ap.add_argument("--no-validation", action="store_true",
                     help="skip the FP32-vs-FP64 validation window (profiler runs); the record then states no error")
     ap.add_argument("--patch-radius", type=int, default=None,
                     help="FP64 near-field box radius around point sources of FP32 grids (0: off)")
+    ap.add_argument("--tiles", default=None, metavar="R,X,O",
+                    help="with --mode separate/fused: rows, x chunk and tile order of the separate kernels")
+    ap.add_argument("--fused-shape", default=None, metavar="R,X,Y",
+                    help="with --mode fused: rows per wavefront, planes per chunk, wavefronts along y "
+                         "(profiler runs repeat a tuned configuration this way: a profiler slows the "
+                         "launches the tuners time)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (HBM counter calibration)")
     args = ap.parse_args()
-    global PATCH_RADIUS
+    global PATCH_RADIUS, FORCED_TILES, FORCED_FUSED_SHAPE
     PATCH_RADIUS = args.patch_radius
+    if args.tiles:
+        FORCED_TILES = tuple(int(v) for v in args.tiles.split(","))
+    if args.fused_shape:
+        FORCED_FUSED_SHAPE = tuple(int(v) for v in args.fused_shape.split(","))
+    if (FORCED_TILES or FORCED_FUSED_SHAPE) and args.mode == "auto":
+        print("--tiles / --fused-shape need --mode separate or fused", file=sys.stderr)
+        sys.exit(2)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return

@@ -258,6 +258,10 @@ int omx3d_set_step_mode(omx3d_ctx* ctx, int32_t mode, int32_t rows, int32_t xchu
  * (FP64) wavefronts in all); 0 chooses them from the z extent.  No
  * reference counterpart. */
 int omx3d_set_fused_waves(omx3d_ctx* ctx, int32_t ny_waves);
+/* The fused tile in use: rows per wavefront, planes per chunk, wavefronts
+ * along y (as resolved for this grid; 0 when the z extent does not fit).
+ * No reference counterpart. */
+int omx3d_get_fused_shape(omx3d_ctx* ctx, int32_t* rows, int32_t* xchunk, int32_t* ny_waves);
 /* 1 if the last omx3d_run used the fused step. */
 int omx3d_get_step_mode(omx3d_ctx* ctx, int32_t* fused_last_run);
 /* Per-kernel timing: HIP events around the H and E launches of every

@@ -2353,6 +2353,16 @@ int omx3d_set_step_mode(omx3d_ctx* c, int32_t mode, int32_t rows, int32_t xchunk
     return OMX_OK;
 }
 
+int omx3d_get_fused_shape(omx3d_ctx* c, int32_t* rows, int32_t* xchunk, int32_t* ny_waves) {
+    CHECK_CTX(c);
+    int nzw = 0, nyw = 0;
+    wg_shape(c, nzw, nyw);
+    if (rows) *rows = c->wg_rows;
+    if (xchunk) *xchunk = c->wg_xchunk;
+    if (ny_waves) *ny_waves = nyw;
+    return OMX_OK;
+}
+
 int omx3d_set_fused_waves(omx3d_ctx* c, int32_t ny_waves) {
     CHECK_CTX(c);
     if (ny_waves < 0 || ny_waves > 16) return fail(OMX_ERR_ARG, "wavefronts along y must be 0..16");

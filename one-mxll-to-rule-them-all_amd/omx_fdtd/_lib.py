@@ -107,6 +107,7 @@ _SIGS = {
     "omx3d_autotune_step": ([_P, _I, _IP, _DP, _DP], C.c_int),
     "omx3d_set_step_mode": ([_P, _I, _I, _I], C.c_int),
     "omx3d_set_fused_waves": ([_P, _I], C.c_int),
+    "omx3d_get_fused_shape": ([_P, _IP, _IP, _IP], C.c_int),
     "omx3d_get_step_mode": ([_P, _IP], C.c_int),
     "omxb_create": ([_I, C.POINTER(_P), C.POINTER(_P)], C.c_int),
     "omxb_run": ([_P, _I, _I, _I, _I], C.c_int),

@@ -129,6 +129,12 @@ class Grid3D:
         if ywaves is not None:
             L.call("omx3d_set_fused_waves", self.h, int(ywaves))
 
+    def fused_shape(self):
+        """(rows per wavefront, planes per chunk, wavefronts along y) of the fused step."""
+        r, x, y = C.c_int32(), C.c_int32(), C.c_int32()
+        L.call("omx3d_get_fused_shape", self.h, C.byref(r), C.byref(x), C.byref(y))
+        return r.value, x.value, y.value
+
     def last_run_fused(self):
         f = C.c_int32()
         L.call("omx3d_get_step_mode", self.h, C.byref(f))
