@@ -302,7 +302,8 @@ int omx3d_autotune(omx3d_ctx* ctx, int32_t steps, int32_t* rows_y, int32_t* xchu
  * kernels (at the current tiling) and, where the fused step applies, of
  * fused workgroup tiles of several shapes, on the grid itself (TF/SF and
  * near-field patches included, sources and detectors left out); keeps the
- * faster step mode and fused shape (omx3d_set_step_mode), frees the fused
+ * fused step (and its fastest shape) when it beats the separate kernels by
+ * more than 2 %, else the separate kernels (omx3d_set_step_mode); frees the fused
  * step's second buffers when the separate kernels win, then clears fields,
  * CPML, dispersive state and the TF/SF line (call before a run).  *mode: 0
  * fused, 1 separate; *ms_fused < 0 when the fused step does not apply.  No

@@ -809,6 +809,33 @@ WgPlan wg_plan(const omx3d_ctx* c) {
     return P;
 }
 
+// Whether the fused kernel must tell media apart (its MED variant: class
+// tags, the dielectric update, no store for cells of dispersive classes).
+// Not when every class besides vacuum has a Drude / Drude-Lorentz component
+// (those cells are all recomputed by the dispersive pass after the kernel,
+// which then overwrites the vacuum update the kernel stored) and no such
+// cell lies on a CPML face (where the kernel would also step its E psi).
+int wg_media_check(omx3d_ctx* c) {
+    if (c->wg_med >= 0) return OMX_OK;
+    c->wg_med = 1;
+    if (!c->media_set || c->n_classes <= 1) { c->wg_med = 0; return OMX_OK; }
+    if (c->disp_tag0 != 1 || c->n_disp <= 0) return OMX_OK;
+    std::vector<long> list(c->n_disp);
+    HIPCHK(hipMemcpy(list.data(), c->d_disp, sizeof(long) * list.size(), hipMemcpyDeviceToHost));
+    const int np = c->d.npml;
+    const int n[3] = {c->d.nx, c->d.ny, c->d.nz};
+    for (long e : list) {
+        const int ijk[3] = {(int)(e & 0xffff), (int)((e >> 16) & 0xffff), (int)((e >> 32) & 0xffff)};
+        for (int a = 0; a < 3; ++a) {
+            const bool lo = (c->cpml_mask >> (2 * a)) & 1u, hi = (c->cpml_mask >> (2 * a + 1)) & 1u;
+            if ((lo && ijk[a] >= 2 && ijk[a] <= np) || (hi && ijk[a] >= n[a] + 1 - np && ijk[a] <= n[a] - 1))
+                return OMX_OK;
+        }
+    }
+    c->wg_med = 0;
+    return OMX_OK;
+}
+
 // The fused step: one launch over every box, with the CPML code of the
 // faces the grid has compiled in (the interior tiles skip the x/y psi code
 // by uniform branches).  Separate launches for slabs and interior, in turn
@@ -818,6 +845,9 @@ template <typename T>
 int launch_wg(omx3d_ctx* c, bool md, bool tf, const YeeArgs<T>& a, const FusedOut<T>& fo) {
     const WgPlan P = wg_plan(c);
     if (P.nzw == 0) return fail(OMX_ERR_STATE, "fused step: the z extent does not fit one workgroup");
+    int rc = wg_media_check(c);
+    if (rc) return rc;
+    md = md && c->wg_med == 1;
     if (c->cpml_mask & 15u) launch_wg_rows<T, 7>(c, md, tf, a, fo, P, c->s);
     else if (c->cpml_mask & 48u) launch_wg_rows<T, 4>(c, md, tf, a, fo, P, c->s);
     else launch_wg_rows<T, 0>(c, md, tf, a, fo, P, c->s);
@@ -1231,6 +1261,7 @@ int media_finish(omx3d_ctx* c) {
     c->media_set = true;
     c->src_in_dispersive = -1;
     c->tf_fold_e = -1;
+    c->wg_med = -1;
     c->patches_built = false;
     return OMX_OK;
 }
@@ -2547,7 +2578,9 @@ int omx3d_autotune_step(omx3d_ctx* c, int32_t steps, int32_t* mode, double* ms_s
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) { c->step_mode = mode0; c->wg_rows = r0; c->wg_xchunk = x0; c->wg_ny = y0; return rc; }
-    const bool use_fused = t_fus >= 0.0 && t_fus < t_sep;
+    // the fused step replaces the default only for a clear gain (a margin over
+    // the timing noise of a few steps)
+    const bool use_fused = t_fus >= 0.0 && t_fus < 0.98 * t_sep;
     c->step_mode = use_fused ? 0 : 1;
     c->wg_rows = best_r; c->wg_xchunk = best_x; c->wg_ny = best_y;
     if (!use_fused) {

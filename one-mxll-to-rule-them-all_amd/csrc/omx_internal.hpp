@@ -155,6 +155,7 @@ struct omx3d_ctx {
     int wg_rows = 2;            // fused step: rows per wavefront (1 or 2)
     int wg_xchunk = 32;         // fused step: planes per workgroup tile
     int wg_ny = 0;              // fused step: wavefronts along y per workgroup (0: chosen from the z extent)
+    int wg_med = -1;            // fused step needs its media variant: -1 unknown, 0 no, 1 yes
     int src_in_dispersive = -1; // -1 unknown, 0 no, 1 a source point sits in a Drude/DL component
     bool last_fused = false;
     // FP64 master coefficients: be ce (npml+1) bh ch (npml) den_e{x,y,z} den_h{x,y,z} (n+2 each)
