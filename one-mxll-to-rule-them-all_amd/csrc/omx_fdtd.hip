@@ -811,28 +811,14 @@ WgPlan wg_plan(const omx3d_ctx* c) {
 
 // Whether the fused kernel must tell media apart (its MED variant: class
 // tags, the dielectric update, no store for cells of dispersive classes).
-// Not when every class besides vacuum has a Drude / Drude-Lorentz component
-// (those cells are all recomputed by the dispersive pass after the kernel,
-// which then overwrites the vacuum update the kernel stored) and no such
-// cell lies on a CPML face (where the kernel would also step its E psi).
+// Not when every class besides vacuum has a Drude / Drude-Lorentz component:
+// those cells are all recomputed by the dispersive pass after the kernel,
+// which overwrites the vacuum update the kernel stored; on CPML faces the
+// kernel has then already advanced their E psi (a recursion in H only), and
+// the pass adds those values instead of stepping them again.
 int wg_media_check(omx3d_ctx* c) {
     if (c->wg_med >= 0) return OMX_OK;
-    c->wg_med = 1;
-    if (!c->media_set || c->n_classes <= 1) { c->wg_med = 0; return OMX_OK; }
-    if (c->disp_tag0 != 1 || c->n_disp <= 0) return OMX_OK;
-    std::vector<long> list(c->n_disp);
-    HIPCHK(hipMemcpy(list.data(), c->d_disp, sizeof(long) * list.size(), hipMemcpyDeviceToHost));
-    const int np = c->d.npml;
-    const int n[3] = {c->d.nx, c->d.ny, c->d.nz};
-    for (long e : list) {
-        const int ijk[3] = {(int)(e & 0xffff), (int)((e >> 16) & 0xffff), (int)((e >> 32) & 0xffff)};
-        for (int a = 0; a < 3; ++a) {
-            const bool lo = (c->cpml_mask >> (2 * a)) & 1u, hi = (c->cpml_mask >> (2 * a + 1)) & 1u;
-            if ((lo && ijk[a] >= 2 && ijk[a] <= np) || (hi && ijk[a] >= n[a] + 1 - np && ijk[a] <= n[a] - 1))
-                return OMX_OK;
-        }
-    }
-    c->wg_med = 0;
+    c->wg_med = (!c->media_set || c->n_classes <= 1 || (c->disp_tag0 == 1 && c->n_disp > 0)) ? 0 : 1;
     return OMX_OK;
 }
 
@@ -899,12 +885,15 @@ int launch_fused(omx3d_ctx* c, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullpt
         const double* hy1 = tf ? c->d_tf + c->tf_n : nullptr;
         const int n = md ? c->n_disp : 0;
         T *exo = (T*)c->f[0], *eyo = (T*)c->f[1], *ezo = (T*)c->f[2];
-        if (c->cpml_mask)
-            hipLaunchKernelGGL((k_wg_post<T, true, true>), dim3(nd + nl), dim3(256), 0, c->s, an, exo, eyo, ezo,
-                               c->d_disp, n, nd, t, ex1, hy1, src);
+        if (c->cpml_mask && c->wg_med == 0)
+            hipLaunchKernelGGL((k_wg_post<T, true, true, true>), dim3(nd + nl), dim3(256), 0, c->s, an, exo, eyo,
+                               ezo, c->d_disp, n, nd, t, ex1, hy1, src);
+        else if (c->cpml_mask)
+            hipLaunchKernelGGL((k_wg_post<T, true, true, false>), dim3(nd + nl), dim3(256), 0, c->s, an, exo, eyo,
+                               ezo, c->d_disp, n, nd, t, ex1, hy1, src);
         else
-            hipLaunchKernelGGL((k_wg_post<T, false, true>), dim3(nd + nl), dim3(256), 0, c->s, an, exo, eyo, ezo,
-                               c->d_disp, n, nd, t, ex1, hy1, src);
+            hipLaunchKernelGGL((k_wg_post<T, false, true, false>), dim3(nd + nl), dim3(256), 0, c->s, an, exo, eyo,
+                               ezo, c->d_disp, n, nd, t, ex1, hy1, src);
         HIPCHK(hipGetLastError());
     }
     if (c->patch_cells) {

@@ -589,11 +589,13 @@ __global__ __launch_bounds__(1024) void k_step_wg(const YeeArgs<T> a, const Fuse
 
 
 // After the workgroup-tile fused step: the cells of dispersive classes
-// (workgroups [0, nd), one cell per thread, as k_e_dispersive) and, when
+// (workgroups [0, nd), one cell per thread, as k_e_dispersive; STEPPED when
+// the fused step ran without its media variant and so already advanced
+// their E psi) and, when
 // the kernel added the TF/SF E corrections, the 1D line's E half step with
 // the source (the remaining workgroups, as the end of k_tfsf_e).  The two
 // parts touch disjoint data.
-template <typename T, bool CPML, bool B32>
+template <typename T, bool CPML, bool B32, bool STEPPED>
 __global__ __launch_bounds__(256) void k_wg_post(const YeeArgs<T> a, T* __restrict__ exo, T* __restrict__ eyo,
                                                  T* __restrict__ ezo, const long* __restrict__ list, int n, int nd,
                                                  TfsfArgs t, double* __restrict__ ex1, const double* __restrict__ hy1,
@@ -601,7 +603,7 @@ __global__ __launch_bounds__(256) void k_wg_post(const YeeArgs<T> a, T* __restri
     const int b = (int)blockIdx.x;
     if (b < nd) {
         const int p = b * (int)blockDim.x + (int)threadIdx.x;
-        if (p < n) e_dispersive_cell<T, CPML, B32>(a, exo, eyo, ezo, list[p], p);
+        if (p < n) e_dispersive_cell<T, CPML, B32, STEPPED>(a, exo, eyo, ezo, list[p], p);
         return;
     }
     const int m = (b - nd) * (int)blockDim.x + (int)threadIdx.x;

@@ -882,7 +882,7 @@ __device__ __forceinline__ void e_bulk(const YeeArgs<T>& a, int tg, T dep, T hzc
 }
 
 // x faces: psix_Eyx, psix_Ezx (low side i = 2..npml)
-template <typename T>
+template <typename T, bool STEPPED = false>
 __device__ __forceinline__ void e_cpml_x(const YeeArgs<T>& a, int i, long jk, T dep, T hzc, T hzi, T hyc, T hyi,
                                          T& ey, T& ez) {
     const unsigned cp = a.cpml;
@@ -890,23 +890,23 @@ __device__ __forceinline__ void e_cpml_x(const YeeArgs<T>& a, int i, long jk, T 
     if ((cp & 1u) && i >= 2 && i <= npml) {
         const int c = i;
         const long ps = (long)(i - 1) * a.sx + jk;
-        const T p0 = psi_step(gld(a.pe[0], ps), gld(a.be, c), gld(a.ced, c), hzi - hzc);
-        const T p1 = psi_step(gld(a.pe[1], ps), gld(a.be, c), gld(a.ced, c), hyc - hyi);
-        gst(a.pe[0], ps, p0); gst(a.pe[1], ps, p1);
+        const T p0 = STEPPED ? gld(a.pe[0], ps) : psi_step(gld(a.pe[0], ps), gld(a.be, c), gld(a.ced, c), hzi - hzc);
+        const T p1 = STEPPED ? gld(a.pe[1], ps) : psi_step(gld(a.pe[1], ps), gld(a.be, c), gld(a.ced, c), hyc - hyi);
+        if (!STEPPED) { gst(a.pe[0], ps, p0); gst(a.pe[1], ps, p1); }
         ey = ey + dep * p0; ez = ez + dep * p1;
     }
     if ((cp & 2u) && i >= a.nx + 1 - npml && i <= a.nx - 1) {
         const int c = a.nx + 1 - i;
         const long ps = (long)(npml - 1 + c) * a.sx + jk;
-        const T p0 = psi_step(gld(a.pe[0], ps), gld(a.be, c), gld(a.ced, c), hzi - hzc);
-        const T p1 = psi_step(gld(a.pe[1], ps), gld(a.be, c), gld(a.ced, c), hyc - hyi);
-        gst(a.pe[0], ps, p0); gst(a.pe[1], ps, p1);
+        const T p0 = STEPPED ? gld(a.pe[0], ps) : psi_step(gld(a.pe[0], ps), gld(a.be, c), gld(a.ced, c), hzi - hzc);
+        const T p1 = STEPPED ? gld(a.pe[1], ps) : psi_step(gld(a.pe[1], ps), gld(a.be, c), gld(a.ced, c), hyc - hyi);
+        if (!STEPPED) { gst(a.pe[0], ps, p0); gst(a.pe[1], ps, p1); }
         ey = ey + dep * p0; ez = ez + dep * p1;
     }
 }
 
 // y faces: psiy_Exy, psiy_Ezy
-template <typename T>
+template <typename T, bool STEPPED = false>
 __device__ __forceinline__ void e_cpml_y(const YeeArgs<T>& a, int i, int j, int k, T dep, T hzc, T hzj, T hxc, T hxj,
                                          T& ex, T& ez) {
     const unsigned cp = a.cpml;
@@ -915,23 +915,23 @@ __device__ __forceinline__ void e_cpml_y(const YeeArgs<T>& a, int i, int j, int 
     if ((cp & 4u) && j >= 2 && j <= npml) {
         const int c = j;
         const long ps = ((long)i * nse + (j - 1)) * a.sz + k + a.off;
-        const T p0 = psi_step(gld(a.pe[2], ps), gld(a.be, c), gld(a.ced, c), hzc - hzj);
-        const T p1 = psi_step(gld(a.pe[3], ps), gld(a.be, c), gld(a.ced, c), hxj - hxc);
-        gst(a.pe[2], ps, p0); gst(a.pe[3], ps, p1);
+        const T p0 = STEPPED ? gld(a.pe[2], ps) : psi_step(gld(a.pe[2], ps), gld(a.be, c), gld(a.ced, c), hzc - hzj);
+        const T p1 = STEPPED ? gld(a.pe[3], ps) : psi_step(gld(a.pe[3], ps), gld(a.be, c), gld(a.ced, c), hxj - hxc);
+        if (!STEPPED) { gst(a.pe[2], ps, p0); gst(a.pe[3], ps, p1); }
         ex = ex + dep * p0; ez = ez + dep * p1;
     }
     if ((cp & 8u) && j >= a.ny + 1 - npml && j <= a.ny - 1) {
         const int c = a.ny + 1 - j;
         const long ps = ((long)i * nse + (npml - 1 + c)) * a.sz + k + a.off;
-        const T p0 = psi_step(gld(a.pe[2], ps), gld(a.be, c), gld(a.ced, c), hzc - hzj);
-        const T p1 = psi_step(gld(a.pe[3], ps), gld(a.be, c), gld(a.ced, c), hxj - hxc);
-        gst(a.pe[2], ps, p0); gst(a.pe[3], ps, p1);
+        const T p0 = STEPPED ? gld(a.pe[2], ps) : psi_step(gld(a.pe[2], ps), gld(a.be, c), gld(a.ced, c), hzc - hzj);
+        const T p1 = STEPPED ? gld(a.pe[3], ps) : psi_step(gld(a.pe[3], ps), gld(a.be, c), gld(a.ced, c), hxj - hxc);
+        if (!STEPPED) { gst(a.pe[2], ps, p0); gst(a.pe[3], ps, p1); }
         ex = ex + dep * p0; ez = ez + dep * p1;
     }
 }
 
 // z faces: psiz_Exz, psiz_Eyz
-template <typename T>
+template <typename T, bool STEPPED = false>
 __device__ __forceinline__ void e_cpml_z(const YeeArgs<T>& a, int i, int j, int k, T dep, T hyc, T hyk, T hxc, T hxk,
                                          T& ex, T& ey) {
     const unsigned cp = a.cpml;
@@ -939,17 +939,17 @@ __device__ __forceinline__ void e_cpml_z(const YeeArgs<T>& a, int i, int j, int 
     if ((cp & 16u) && k >= 2 && k <= npml) {
         const int c = k;
         const long ps = zpsi_index(0, j, i, k - 2, a.sy, a.nx, npml);
-        const T p0 = psi_step(gld(a.pe[4], ps), gld(a.be, c), gld(a.ced, c), hyk - hyc);
-        const T p1 = psi_step(gld(a.pe[5], ps), gld(a.be, c), gld(a.ced, c), hxc - hxk);
-        gst(a.pe[4], ps, p0); gst(a.pe[5], ps, p1);
+        const T p0 = STEPPED ? gld(a.pe[4], ps) : psi_step(gld(a.pe[4], ps), gld(a.be, c), gld(a.ced, c), hyk - hyc);
+        const T p1 = STEPPED ? gld(a.pe[5], ps) : psi_step(gld(a.pe[5], ps), gld(a.be, c), gld(a.ced, c), hxc - hxk);
+        if (!STEPPED) { gst(a.pe[4], ps, p0); gst(a.pe[5], ps, p1); }
         ex = ex + dep * p0; ey = ey + dep * p1;
     }
     if ((cp & 32u) && k >= a.nz + 1 - npml && k <= a.nz - 1) {
         const int c = a.nz + 1 - k;
         const long ps = zpsi_index(1, j, i, a.nz - 1 - k, a.sy, a.nx, npml);
-        const T p0 = psi_step(gld(a.pe[4], ps), gld(a.be, c), gld(a.ced, c), hyk - hyc);
-        const T p1 = psi_step(gld(a.pe[5], ps), gld(a.be, c), gld(a.ced, c), hxc - hxk);
-        gst(a.pe[4], ps, p0); gst(a.pe[5], ps, p1);
+        const T p0 = STEPPED ? gld(a.pe[4], ps) : psi_step(gld(a.pe[4], ps), gld(a.be, c), gld(a.ced, c), hyk - hyc);
+        const T p1 = STEPPED ? gld(a.pe[5], ps) : psi_step(gld(a.pe[5], ps), gld(a.be, c), gld(a.ced, c), hxc - hxk);
+        if (!STEPPED) { gst(a.pe[4], ps, p0); gst(a.pe[5], ps, p1); }
         ex = ex + dep * p0; ey = ey + dep * p1;
     }
 }
@@ -1110,7 +1110,7 @@ __device__ __forceinline__ void e_tile(const YeeArgs<T>& a, const TfsfLaunch& tf
 // path, the other buffer after the fused step).  list holds the cells'
 // indices (i, j, k) with the media of their three components; p, the cell's
 // position in its grid's list, indexes its compact Drude / DL state.
-template <typename T, bool CPML, bool B32>
+template <typename T, bool CPML, bool B32, bool STEPPED = false>
 __device__ __forceinline__ void e_dispersive_cell(const YeeArgs<T>& a, T* __restrict__ exo, T* __restrict__ eyo,
                                                   T* __restrict__ ezo, long e, long p) {
     using O = Off<T, B32>;
@@ -1141,9 +1141,11 @@ __device__ __forceinline__ void e_dispersive_cell(const YeeArgs<T>& a, T* __rest
     ey = medium_update(ey, ry, vy, m1, a.med, p, st1, st1 + ps, st1 + pl0, st1 + plo0, ps);
     ez = medium_update(ez, rz, vz, m2, a.med, p, st2, st2 + ps, st2 + pl0, st2 + plo0, ps);
     if constexpr (CPML) {
-        e_cpml_x(a, i, jk, dep, hzc, hzi, hyc, hyi, ey, ez);
-        e_cpml_y(a, i, j, k, dep, hzc, hzj, hxc, hxj, ex, ez);
-        e_cpml_z(a, i, j, k, dep, hyc, hyk, hxc, hxk, ex, ey);
+        // STEPPED: the fused step already advanced this cell's E psi (the
+        // recursion depends on H only), so its values are added as they are
+        e_cpml_x<T, STEPPED>(a, i, jk, dep, hzc, hzi, hyc, hyi, ey, ez);
+        e_cpml_y<T, STEPPED>(a, i, j, k, dep, hzc, hzj, hxc, hxj, ex, ez);
+        e_cpml_z<T, STEPPED>(a, i, j, k, dep, hyc, hyk, hxc, hxk, ex, ey);
     }
     O::st(exo, o, ex); O::st(eyo, o, ey); O::st(ezo, o, ez);
 }

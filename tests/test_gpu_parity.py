@@ -328,6 +328,33 @@ def test_autotune_step_periodic_grid_keeps_separate():
 
 
 @pytest.mark.parametrize("precision", [8, 4])
+def test_fused_dispersive_cells_on_cpml_faces(precision):
+    """A Drude-Lorentz sphere across the x-low CPML face and nothing but
+    vacuum besides: the fused step runs without its media variant, steps the
+    sphere's E psi itself and the dispersive pass adds it as it is.  Must
+    equal the separate kernels (which step the psi in the dispersive pass)."""
+    spec = ("drude-lorentz", "Ag", 1.0)
+    case = small_case("dispcpml", n=(30, 28, 40), npml=6, media=[spec], shapes=[(1, 1, [3.0, 14.0, 20.0, 8.0])],
+                      steps=80)
+    out = []
+    for mode in (1, 0):
+        dr, dt = case.dr_dt
+        g = omx_fdtd.Grid3D(case.grid, case.npml, case.boundaries, dt, dr, precision=precision)
+        g.set_step_mode(mode)
+        cases.apply(case, g, library=G.library_medium, make_medium=G.medium)
+        mm = g.media_map()
+        assert (mm[:case.npml] != 0).any()          # dispersive cells on the CPML face
+        g.run(1, case.steps, 1)
+        assert g.last_run_fused() == (mode == 0)
+        out.append(g.read_records())
+        g.kill()
+    assert np.abs(out[0]).max() > 0
+    r = rel_l2(out[1], out[0])
+    print(f"FP{8 * precision}: fused vs separate relL2 = {r:.3e}")
+    assert r <= (1e-13 if precision == 8 else 5e-6)
+
+
+@pytest.mark.parametrize("precision", [8, 4])
 def test_fused_z_face_across_two_wavefronts(precision):
     """nz = 70 with CPML 8: the high z face (k = 63..69) spans the first and
     the second wavefront along z of a fused workgroup.  The fused step must
