@@ -753,7 +753,8 @@ void launch_wg_kernel(omx3d_ctx* c, bool md, bool tf, const YeeArgs<T>& a, const
                       hipStream_t st) {
     const size_t lds = wg_lds_bytes(P.nzw * P.nyw, R, sizeof(T), tf);
     const dim3 blk(64 * P.nzw * P.nyw);
-    const unsigned nb = (unsigned)P.box[P.nbox - 1].b_end;
+    // with TF/SF one workgroup more: the incident line's H half step
+    const unsigned nb = (unsigned)P.box[P.nbox - 1].b_end + (tf ? 1u : 0u);
 #define OMX_WG(MD, TF) hipLaunchKernelGGL((k_step_wg<T, R, CP, MD, TF>), dim3(nb), blk, lds, st, a, fo, P)
     if (md && tf) OMX_WG(1, true);
     else if (md) OMX_WG(1, false);
@@ -858,12 +859,10 @@ int launch_fused(omx3d_ctx* c, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullpt
     fo.ex = (T*)c->f_alt[0]; fo.ey = (T*)c->f_alt[1]; fo.ez = (T*)c->f_alt[2];
     fo.hx = (T*)c->f_alt[3]; fo.hy = (T*)c->f_alt[4]; fo.hz = (T*)c->f_alt[5];
     for (int q = 0; q < 6; ++q) fo.ph[q] = (T*)c->psi_h_alt[q];
-    if (c->tfsf) { fo.ex1 = c->d_tf; fo.t = tfsf_args(c); }
-    if (c->tfsf)   // the incident line's H half step (the kernel reads its E only)
-        hipLaunchKernelGGL(k_tfsf_1d_h, dim3(blocks_for(c->tf_n, 256)), dim3(256), 0, c->s, c->d_tf,
-                           c->d_tf + c->tf_n, c->tf_n, (c->d.dt / kMu0) * (1.0 / c->d.dr));
+    // TF/SF: the kernel reads the incident line's E and advances its H in
+    // an extra workgroup
+    if (c->tfsf) { fo.ex1 = c->d_tf; fo.hy1 = c->d_tf + c->tf_n; fo.t = tfsf_args(c); }
     const bool md = c->media_set && c->n_classes > 1, tf = c->tfsf;
-    if (tf && fold_e) fo.hy1 = c->d_tf + c->tf_n;
     if (ev0) HIPCHK(hipEventRecord(ev0, c->s));
     rc = launch_wg<T>(c, md, tf, a, fo);
     if (rc) return rc;
@@ -874,12 +873,13 @@ int launch_fused(omx3d_ctx* c, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullpt
     if ((md && c->n_disp > 0) || (tf && fold_e)) {
         // one launch for the cells of dispersive classes (H^{n+1/2} now
         // current, E^n in the other buffer, E^{n+1} into the current one) and
-        // the TF/SF line's E half step
+        // the TF/SF E step (corrections to E^{n+1}, the line's E half step)
         YeeArgs<T> an = make_args<T>(c);
         an.ex = (T*)c->f_alt[0]; an.ey = (T*)c->f_alt[1]; an.ez = (T*)c->f_alt[2];
         const int nd = md ? (int)blocks_for(c->n_disp, 256) : 0;
-        const int nl = tf && fold_e ? (int)blocks_for(c->tf_n, 256) : 0;
         const TfsfArgs t = tf ? tfsf_args(c) : TfsfArgs{};
+        const long n_tf = 2L * (t.i1 - t.i0) * (t.j1 - t.j0 + 1) + 2L * (t.j1 - t.j0 + 1) * (t.k1 - t.k0) + t.n1;
+        const int nl = tf && fold_e ? (int)blocks_for(n_tf, 256) : 0;
         const double src = tf && fold_e ? tfsf_source(c, t_tf) : 0.0;
         double* ex1 = tf ? c->d_tf : nullptr;
         const double* hy1 = tf ? c->d_tf + c->tf_n : nullptr;
@@ -930,8 +930,8 @@ int run_impl(omx3d_ctx* c, int tt_first, int n_steps, int record_every) {
     rc = tfsf_fold_check(c);
     if (rc) return rc;
     const bool fold = !fused && c->tfsf, fold_e = fold && c->tf_fold_e == 1;
-    // the fused step adds the TF/SF E corrections itself and advances the
-    // line's E after the kernel (same condition as fold_e)
+    // the fused step's post kernel adds the TF/SF E corrections and advances
+    // the line's E (same condition as fold_e)
     const bool fold_wg = fused && c->tfsf && c->tf_fold_e == 1;
     const int tt_0 = c->n_skip == 1 ? 1 : 0;
     for (int tt = tt_first; tt < tt_first + n_steps; ++tt) {

@@ -32,8 +32,11 @@
 //
 // Cells of Drude / Drude-Lorentz classes are left to the dispersive pass
 // after the kernel (their E and E psi are not stored).  The TF/SF H
-// corrections are applied inside (owned and halo cells alike), and the E
-// corrections too unless a dispersive cell lies on their faces.
+// corrections are applied inside (owned and halo cells alike), by the lanes
+// and rows on the box's faces only; one extra workgroup advances the
+// incident line's H meanwhile (the tiles read only its E).  The E
+// corrections are added to the stored E after the kernel, as the separate
+// step's k_tfsf_e does.
 #pragma once
 
 namespace omx {
@@ -68,7 +71,7 @@ struct FusedOut {
     T *ex, *ey, *ez, *hx, *hy, *hz;   // E^{n+1}, H^{n+1/2}
     T* ph[6];                         // psi of H at n+1/2 (x, y, z slab arrays as YeeArgs::ph)
     const double* ex1;                // TF/SF: the incident line's E^n (null without TF/SF)
-    const double* hy1;                // TF/SF: the line's H^{n+1/2} when the kernel adds the E corrections too
+    double* hy1;                      // TF/SF: the line's H, advanced to n+1/2 by the last workgroup
     TfsfArgs t;
 };
 
@@ -142,13 +145,13 @@ struct WgPlan {
     int nzw, nyw;        // wavefronts per workgroup along z and y
 };
 
-// LDS of one workgroup: with TF/SF, each wavefront's per-lane TF/SF values
-// (NW x 2 x 64 doubles), then two buffers (plane parity) of the row
+// LDS of one workgroup: with TF/SF, each wavefront's per-lane TF/SF value
+// (NW x 64 doubles), then two buffers (plane parity) of the row
 // hand-overs (NW x 2 components x 64 lanes, H then E) and of the lane
 // hand-overs (NW x R rows x 2 components, H then E).
 __host__ __device__ inline int wg_lds_elems(int nw, int r) { return 8 * nw * 64 + 8 * nw * r; }
 __host__ __device__ inline size_t wg_lds_bytes(int nw, int r, size_t esz, bool tf) {
-    return (tf ? (size_t)nw * 128 * sizeof(double) : 0) + esz * (size_t)wg_lds_elems(nw, r);
+    return (tf ? (size_t)nw * 64 * sizeof(double) : 0) + esz * (size_t)wg_lds_elems(nw, r);
 }
 
 // Element of array p at the uniform byte offset uo plus the lane's byte
@@ -213,7 +216,7 @@ __device__ __forceinline__ Psi<T> psi_wg(const YeeArgs<T>& a, int i, int j, uint
 // H^{n+1/2} of the components WX/WY/WZ of cell (i,j,k) with the arithmetic
 // of h_tile: the bulk curl, the CPML psi of the x, y and z faces in that
 // order (ps: psi of H at n-1/2 from psi_wg), then the TF/SF correction where
-// the plane and row lie on a face of the box (tfv: the lane's values of
+// the plane, row and lane lie on a face of the box (tfv: the lane's value of
 // this step in LDS).  STORE writes the new psi to fo.ph where act is set.
 template <typename T, int CPML, bool TF, bool WX, bool WY, bool WZ, bool STORE>
 __device__ __forceinline__ void h_wg(const YeeArgs<T>& a, const FusedOut<T>& fo, const double* tfv, int i, int j, int k,
@@ -274,10 +277,7 @@ __device__ __forceinline__ void h_wg(const YeeArgs<T>& a, const FusedOut<T>& fo,
     if constexpr (TF) {
         const TfsfArgs& t = fo.t;
         if (i >= t.i0 && i <= t.i1 - 1) {
-            if (WY && j >= t.j0 && j <= t.j1) {
-                const double cy = tfv[0];
-                if (cy != 0.0) hy = (T)((double)hy + cy);
-            }
+            if (WY && j >= t.j0 && j <= t.j1 && (k == t.k0 - 1 || k == t.k1)) hy = (T)((double)hy + tfv[0]);
             if (WZ && (j == t.j0 - 1 || j == t.j1) && k >= t.k0 && k <= t.k1) {
                 const double e1 = fo.ex1[k - t.kb];
                 if (e1 != 0.0) {
@@ -290,14 +290,12 @@ __device__ __forceinline__ void h_wg(const YeeArgs<T>& a, const FusedOut<T>& fo,
 }
 
 // E^{n+1} of an owned cell with the arithmetic of e_tile: e_bulk, then the
-// CPML psi of E in place (x, y, z faces; ps: psi of E at n from psi_wg),
-// then the TF/SF E corrections when fo.hy1 is set; own false (a cell of a
-// dispersive class, or a lane past the grid) stores no psi.
-template <typename T, int CPML, int MED, bool TF>
-__device__ __forceinline__ void e_wg(const YeeArgs<T>& a, const FusedOut<T>& fo, const double* tfv, int i, int j, int k,
-                                     uint32_t kS,
-                                     uint32_t zeS, int tg, bool own, T dex, T cex, const Coef<T>& px, const Coef<T>& ry,
-                                     const Coef<T>& lz, const Psi<T>& ps, T hzc, T hzj, T hzi, T hyc, T hyk, T hyi,
+// CPML psi of E in place (x, y, z faces; ps: psi of E at n from psi_wg);
+// own false (a cell of a dispersive class, or a lane past the grid) stores
+// no psi.
+template <typename T, int CPML, int MED>
+__device__ __forceinline__ void e_wg(const YeeArgs<T>& a, int i, int j, uint32_t kS, uint32_t zeS, int tg, bool own,
+                                     T dex, T cex, const Coef<T>& px, const Coef<T>& ry, const Coef<T>& lz, const Psi<T>& ps, T hzc, T hzj, T hzi, T hyc, T hyk, T hyi,
                                      T hxc, T hxj, T hxk, T& ex, T& ey, T& ez) {
     const T dep = a.dt_eps0;
     // a cell of a dispersive class is updated by the dispersive pass (nothing
@@ -328,20 +326,6 @@ __device__ __forceinline__ void e_wg(const YeeArgs<T>& a, const FusedOut<T>& fo,
             ex = ex + dep * p0; ey = ey + dep * p1;
         }
     }
-    if constexpr (TF) {
-        // TF/SF E corrections (k_tfsf_e's values), when the host folds them in
-        const TfsfArgs& t = fo.t;
-        if (fo.hy1 != nullptr && j >= t.j0 && j <= t.j1) {
-            if (i >= t.i0 && i <= t.i1 - 1) {
-                const double cx = tfv[64];
-                if (cx != 0.0) ex = (T)((double)ex + cx);
-            }
-            if ((i == t.i0 || i == t.i1) && k >= t.k0 && k <= t.k1 - 1) {
-                const double c = t.dt_eps0 * (double)ldc(a.den_ex, i) * fo.hy1[k - t.kb];
-                ez = (T)((double)ez + (i == t.i1 ? c : -c));
-            }
-        }
-    }
 }
 
 template <typename T, int R, int CPML, int MED, bool TF>
@@ -351,9 +335,23 @@ __global__ __launch_bounds__(1024) void k_step_wg(const YeeArgs<T> a, const Fuse
     const int lane = (int)(threadIdx.x & 63);
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int zw = wv % NZ, yw = wv / NZ;
-    double* const tfv = reinterpret_cast<double*>(wg_lds_raw) + wv * 128 + lane;   // TF: this lane's values
-    T* const lds = reinterpret_cast<T*>(wg_lds_raw + (TF ? NW * 128 * sizeof(double) : 0));
+    double* const tfv = reinterpret_cast<double*>(wg_lds_raw) + wv * 64 + lane;   // TF: this lane's value
+    T* const lds = reinterpret_cast<T*>(wg_lds_raw + (TF ? NW * 64 * sizeof(double) : 0));
     const int b = (int)blockIdx.x;
+    if constexpr (TF) {
+        // the workgroup after the last box: the incident line's H half step
+        // (n-1/2 -> n+1/2), which the E corrections after the kernel read
+        int nb = P.box[0].b_end;
+#pragma unroll
+        for (int q = 1; q < kWgBoxes; ++q)
+            if (q < P.nbox) nb = P.box[q].b_end;
+        if (b == nb) {
+            const TfsfArgs& t = fo.t;
+            for (int m = (int)threadIdx.x; m < t.n1 - 1; m += (int)blockDim.x)
+                fo.hy1[m] = fo.hy1[m] + t.dt_mu0 * t.inv_dr * (fo.ex1[m] - fo.ex1[m + 1]);
+            return;
+        }
+    }
     int bq = 0, bl = b;
 #pragma unroll
     for (int q = 1; q < kWgBoxes; ++q)
@@ -385,24 +383,17 @@ __global__ __launch_bounds__(1024) void k_step_wg(const YeeArgs<T> a, const Fuse
     const Coef<T> lz = coef_at<T, (CPML & 4) != 0, true, false>(a, a.den_hz, a.cdhz, a.den_ez, a.cdez, kc, a.nz, 16u, 32u);
     const uint32_t zhS = (uint32_t)max(lz.sh, 0) * S, zeS = (uint32_t)max(lz.se, 0) * S;
     if constexpr (TF) {
-        // the lane's TF/SF corrections of this step: Hy on the box's z faces
-        // (k = k0-1: + dt/mu0 den_hz(k) Ex_inc(k0), k = k1: - ... Ex_inc(k1))
-        // and, when the E corrections ride here, Ex on them (k = k0: + dt/eps0
-        // den_ez(k) Hy_inc(k0-1), k = k1: - ... Hy_inc(k1)); 0 elsewhere
+        // the lane's TF/SF correction of this step: Hy on the box's z faces
+        // (k = k0-1: + dt/mu0 den_hz(k) Ex_inc(k0), k = k1: - ... Ex_inc(k1));
+        // read only by the lanes on those faces
         const TfsfArgs& t = fo.t;
-        double cy = 0.0, cx = 0.0;
+        double cy = 0.0;
         if (k == t.k0 - 1 || k == t.k1) {
             const bool hi = k == t.k1;
             const double c = t.dt_mu0 * (double)gld(a.den_hz, k) * fo.ex1[(hi ? t.k1 : t.k0) - t.kb];
             cy = hi ? -c : c;
         }
-        if (fo.hy1 != nullptr && (k == t.k0 || k == t.k1)) {
-            const bool hi = k == t.k1;
-            const double c = t.dt_eps0 * (double)gld(a.den_ez, k) * (hi ? fo.hy1[t.k1 - t.kb] : fo.hy1[t.k0 - 1 - t.kb]);
-            cx = hi ? -c : c;
-        }
         tfv[0] = cy;
-        tfv[64] = cx;
     }
     Coef<T> ry[R + 1];                                      // rows j0-1 .. j0+R-1
     T exc[R], eyc[R], ezc[R], hyp[R], hzp[R];
@@ -573,8 +564,7 @@ __global__ __launch_bounds__(1024) void k_step_wg(const YeeArgs<T> a, const Fuse
                 const T hxj = r > 0 ? hx[r > 0 ? r - 1 : 0] : hxm;
                 const bool own = MED != 1 || tg[r] < a.disp_tag0;
                 T ex = exc[r], ey = eyc[r], ez = ezc[r];
-                e_wg<T, CPML, MED, TF>(a, fo, tfv, i, j0 + r, k, kS, zeS, tg[r], own && act, dex, cex, px, ry[r + 1], lz,
-                                       pse[r], hz[r], hzj, hzp[r], hy[r], hyk, hyp[r], hx[r], hxj, hxk, ex, ey, ez);
+                e_wg<T, CPML, MED>(a, i, j0 + r, kS, zeS, tg[r], own && act, dex, cex, px, ry[r + 1], lz, pse[r], hz[r], hzj, hzp[r], hy[r], hyk, hyp[r], hx[r], hxj, hxk, ex, ey, ez);
                 if (act) {
                     stu(fo.hx, u, kS, hx[r]); stu(fo.hy, u, kS, hy[r]); stu(fo.hz, u, kS, hz[r]);
                     if (own) { stu(fo.ex, u, kS, ex); stu(fo.ey, u, kS, ey); stu(fo.ez, u, kS, ez); }
@@ -591,10 +581,10 @@ __global__ __launch_bounds__(1024) void k_step_wg(const YeeArgs<T> a, const Fuse
 // After the workgroup-tile fused step: the cells of dispersive classes
 // (workgroups [0, nd), one cell per thread, as k_e_dispersive; STEPPED when
 // the fused step ran without its media variant and so already advanced
-// their E psi) and, when
-// the kernel added the TF/SF E corrections, the 1D line's E half step with
-// the source (the remaining workgroups, as the end of k_tfsf_e).  The two
-// parts touch disjoint data.
+// their E psi) and, when no such cell lies on the box's E faces, the TF/SF
+// E step (the remaining workgroups: k_tfsf_e's corrections to E^{n+1} and
+// the 1D line's E half step with the source).  The two parts touch disjoint
+// data.
 template <typename T, bool CPML, bool B32, bool STEPPED>
 __global__ __launch_bounds__(256) void k_wg_post(const YeeArgs<T> a, T* __restrict__ exo, T* __restrict__ eyo,
                                                  T* __restrict__ ezo, const long* __restrict__ list, int n, int nd,
@@ -606,12 +596,7 @@ __global__ __launch_bounds__(256) void k_wg_post(const YeeArgs<T> a, T* __restri
         if (p < n) e_dispersive_cell<T, CPML, B32, STEPPED>(a, exo, eyo, ezo, list[p], p);
         return;
     }
-    const int m = (b - nd) * (int)blockDim.x + (int)threadIdx.x;
-    if (m >= 1 && m < t.n1 - 1) {
-        double v = ex1[m] + t.dt_eps0 * t.inv_dr * (hy1[m - 1] - hy1[m]);
-        if (m == t.ks - t.kb) v += src;
-        ex1[m] = v;
-    }
+    tfsf_e_item<T>(a, exo, ezo, t, ex1, hy1, src, (b - nd) * (int)blockDim.x + (int)threadIdx.x);
 }
 
 }  // namespace omx

@@ -1859,15 +1859,19 @@ __global__ __launch_bounds__(256) void k_patch_batched(const YeeArgs<T>* __restr
 
 // ---------------------------------------------------------------------------
 // TF/SF plane wave (TfsfArgs above): the E corrections and the 1D line's E
-// half step after the E launch (both step modes), and the 1D line's H half
-// step of the fused step (the separate H launch carries the H corrections and
-// the line's H half step itself).
+// half step after the E launch or the fused step (the separate H and E
+// launches otherwise carry the corrections and the line's half steps
+// themselves, and the fused step the H corrections and the line's H half
+// step).
 // ---------------------------------------------------------------------------
 
+// Item q of the TF/SF E step (ex, ez: the arrays holding E^{n+1}): Ex on the
+// box's z faces (q < 2 sa), Ez on its x faces (the next 2 sb), then the 1D
+// line's E update (n1 items).
 template <typename T>
-__global__ void k_tfsf_e(const YeeArgs<T> a, TfsfArgs t, double* __restrict__ ex1, const double* __restrict__ hy1,
-                         double src) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void tfsf_e_item(const YeeArgs<T>& a, T* __restrict__ ex, T* __restrict__ ez,
+                                            const TfsfArgs& t, double* __restrict__ ex1,
+                                            const double* __restrict__ hy1, double src, int q) {
     const int nxi = t.i1 - t.i0, nyj = t.j1 - t.j0 + 1;
     const int nyz = t.j1 - t.j0 + 1, nzk = t.k1 - t.k0;
     const int sa = nxi * nyj, sb = nyz * nzk;
@@ -1877,11 +1881,11 @@ __global__ void k_tfsf_e(const YeeArgs<T> a, TfsfArgs t, double* __restrict__ ex
         if (side == 0) {
             const int k = t.k0;
             const long o = a.sx * i + (long)a.sz * j + k + a.off;
-            a.ex[o] = (T)((double)a.ex[o] + t.dt_eps0 * (double)a.den_ez[k] * hy1[t.k0 - 1 - t.kb]);
+            ex[o] = (T)((double)ex[o] + t.dt_eps0 * (double)a.den_ez[k] * hy1[t.k0 - 1 - t.kb]);
         } else {
             const int k = t.k1;
             const long o = a.sx * i + (long)a.sz * j + k + a.off;
-            a.ex[o] = (T)((double)a.ex[o] - t.dt_eps0 * (double)a.den_ez[k] * hy1[t.k1 - t.kb]);
+            ex[o] = (T)((double)ex[o] - t.dt_eps0 * (double)a.den_ez[k] * hy1[t.k1 - t.kb]);
         }
         return;
     }
@@ -1893,11 +1897,11 @@ __global__ void k_tfsf_e(const YeeArgs<T> a, TfsfArgs t, double* __restrict__ ex
         if (side == 0) {
             const int i = t.i0;
             const long o = a.sx * i + (long)a.sz * j + k + a.off;
-            a.ez[o] = (T)((double)a.ez[o] - t.dt_eps0 * (double)a.den_ex[i] * inc);
+            ez[o] = (T)((double)ez[o] - t.dt_eps0 * (double)a.den_ex[i] * inc);
         } else {
             const int i = t.i1;
             const long o = a.sx * i + (long)a.sz * j + k + a.off;
-            a.ez[o] = (T)((double)a.ez[o] + t.dt_eps0 * (double)a.den_ex[i] * inc);
+            ez[o] = (T)((double)ez[o] + t.dt_eps0 * (double)a.den_ex[i] * inc);
         }
         return;
     }
@@ -1907,6 +1911,12 @@ __global__ void k_tfsf_e(const YeeArgs<T> a, TfsfArgs t, double* __restrict__ ex
         if (m == t.ks - t.kb) v += src;
         ex1[m] = v;
     }
+}
+
+template <typename T>
+__global__ void k_tfsf_e(const YeeArgs<T> a, TfsfArgs t, double* __restrict__ ex1, const double* __restrict__ hy1,
+                         double src) {
+    tfsf_e_item<T>(a, a.ex, a.ez, t, ex1, hy1, src, (int)(blockIdx.x * blockDim.x + threadIdx.x));
 }
 
 // Sets *flag when a listed dispersive cell (entries as for
@@ -1919,11 +1929,6 @@ __global__ void k_disp_on_tfsf(const long* __restrict__ list, int n, TfsfArgs t,
     const bool ex_face = (k == t.k0 || k == t.k1) && i >= t.i0 && i <= t.i1 - 1 && j >= t.j0 && j <= t.j1;
     const bool ez_face = (i == t.i0 || i == t.i1) && j >= t.j0 && j <= t.j1 && k >= t.k0 && k <= t.k1 - 1;
     if (ex_face || ez_face) flag[0] = 1;
-}
-
-__global__ void k_tfsf_1d_h(double* __restrict__ ex1, double* __restrict__ hy1, int n1, double c) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m < n1 - 1) hy1[m] = hy1[m] + c * (ex1[m] - ex1[m + 1]);
 }
 
 }  // namespace omx

@@ -537,12 +537,14 @@ def test_field_copy_roundtrip_and_energy():
     assert abs(w32 - want) <= 1e-6 * want
 
 
+@pytest.mark.parametrize("mode", [1, 0])
 @pytest.mark.parametrize("precision", [8, 4])
-def test_tfsf_dispersive_cells_on_the_faces(precision):
+def test_tfsf_dispersive_cells_on_the_faces(precision, mode):
     """A Drude-Lorentz sphere across the k1 and i1 faces of the TF/SF box:
-    the E corrections of its cells cannot ride in the E tiles (the
-    dispersive workgroups update those cells), so the E side of the TF/SF
-    runs after the E launch while the H side stays inside the H launch."""
+    the E corrections of its cells cannot ride in the E tiles or in the
+    fused step's post kernel (the dispersive workgroups update those cells),
+    so the E side of the TF/SF runs after the E launch (mode 1) or after the
+    fused step (mode 0) while the H side stays inside."""
     n = (40, 40, 60)
     lo, hi = (10, 10, 12), (30, 30, 48)
     case = cases.Case(name="tfsf-disp", grid=n, dx_nm=2.0, npml=6, boundaries=(BC_CPML,) * 3, steps=300,
@@ -551,7 +553,12 @@ def test_tfsf_dispersive_cells_on_the_faces(precision):
                       tfsf=dict(lo=lo, hi=hi, e0=1.0, w0=2.066 * EV_TO_AU, t0=0.6 * FS_TO_AU, tau=0.2 * FS_TO_AU))
     case.detectors = cases.point_detectors([(20, 20, 40), (29, 20, 52), (34, 20, 44), (20, 20, 30)],
                                            ("Ex", "Ez", "Hy"))
-    g, o, got, want = run_both(case, precision=precision)
+    g, o = both(case, precision)
+    g.set_step_mode(mode)
+    g.run(1, case.steps, case.record_every)
+    assert g.last_run_fused() == (mode == 0)
+    got = g.read_records()
+    want = o.run(1, case.steps, case.record_every)
     assert np.abs(want).max() > 0
     r = rel_l2(got, want)
     print(f"FP{8 * precision} relL2 vs oracle = {r:.3e}")
