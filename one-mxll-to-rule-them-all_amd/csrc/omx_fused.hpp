@@ -64,6 +64,17 @@ __device__ __forceinline__ double dpp_shl1(double old, double v) {
     return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
+// Workgroup barrier ordering LDS only.  The waves of a fused workgroup
+// exchange data through LDS alone (fields, psi of H and the TF/SF line are
+// read from one buffer and written to another; the psi of E is read and
+// written by its owning lane only), so the barrier need not wait for their
+// global loads and stores, as a full __syncthreads would.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Outputs of the fused step (the inputs are the YeeArgs block) and its
 // TF/SF data.
 template <typename T>
@@ -445,7 +456,7 @@ __global__ __launch_bounds__(1024) void k_step_wg(const YeeArgs<T> a, const Fuse
             }
         }
     }
-    __syncthreads();
+    lds_barrier();
     for (int i = i0; i <= i1; ++i) {
         const int pb = i & 1, nb = pb ^ 1;
         const uint32_t u0 = (uint32_t)i * pS + (uint32_t)max(j0, 0) * rS;
@@ -540,7 +551,7 @@ __global__ __launch_bounds__(1024) void k_step_wg(const YeeArgs<T> a, const Fuse
                 }
             }
         }
-        __syncthreads();
+        lds_barrier();
         if (nr > 0) {
             if (!ybot) {
                 hxm = hrow[((pb * NW + wv - NZ) * 2 + 0) * 64 + lane];
@@ -584,8 +595,10 @@ __global__ __launch_bounds__(1024) void k_step_wg(const YeeArgs<T> a, const Fuse
 // their E psi) and, when no such cell lies on the box's E faces, the TF/SF
 // E step (the remaining workgroups: k_tfsf_e's corrections to E^{n+1} and
 // the 1D line's E half step with the source).  The two parts touch disjoint
-// data.
-template <typename T, bool CPML, bool B32, bool STEPPED>
+// data.  LF: each cell loads its three components' state and its E psi
+// before any update (for lists short enough that the pass is bound by
+// memory latency, not bandwidth: the extra registers cost waves).
+template <typename T, bool CPML, bool B32, bool STEPPED, bool LF>
 __global__ __launch_bounds__(256) void k_wg_post(const YeeArgs<T> a, T* __restrict__ exo, T* __restrict__ eyo,
                                                  T* __restrict__ ezo, const long* __restrict__ list, int n, int nd,
                                                  TfsfArgs t, double* __restrict__ ex1, const double* __restrict__ hy1,
@@ -593,7 +606,7 @@ __global__ __launch_bounds__(256) void k_wg_post(const YeeArgs<T> a, T* __restri
     const int b = (int)blockIdx.x;
     if (b < nd) {
         const int p = b * (int)blockDim.x + (int)threadIdx.x;
-        if (p < n) e_dispersive_cell<T, CPML, B32, STEPPED>(a, exo, eyo, ezo, list[p], p);
+        if (p < n) e_dispersive_cell<T, CPML, B32, STEPPED, LF>(a, exo, eyo, ezo, list[p], p);
         return;
     }
     tfsf_e_item<T>(a, exo, ezo, t, ex1, hy1, src, (b - nd) * (int)blockDim.x + (int)threadIdx.x);

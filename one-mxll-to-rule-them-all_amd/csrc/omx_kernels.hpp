@@ -775,6 +775,91 @@ __device__ __forceinline__ void h_tile(const YeeArgs<T>& a, const TfsfLaunch& tf
 // ---------------------------------------------------------------------------
 // p: the cell's position in the dispersive list, its index into the compact
 // state eold, pd and into every pole plane of pl / plold (planes ps apart).
+// The per-cell state of one component, loaded before the update.
+template <typename T>
+struct MedState {
+    T eo, pv;                                        // E_old, PD
+    T L[OMX_MAX_POLES], Lo[OMX_MAX_POLES];           // PL_k, PL_old_k
+};
+
+// A component's state loaded before its medium is known: every component's
+// state block holds E_old, PD and maxp pole planes of PL and PL_old, so the
+// loads depend on nothing but the cell (values a medium does not use are
+// ignored); 0 for a vacuum component.
+template <typename T>
+__device__ __forceinline__ void medium_load_all(int m, long p, const T* __restrict__ st, long ps, int maxp,
+                                                MedState<T>& s) {
+    s.eo = (T)0; s.pv = (T)0;
+#pragma unroll
+    for (int q = 0; q < OMX_MAX_POLES; ++q) { s.L[q] = (T)0; s.Lo[q] = (T)0; }
+    if (m == 0) return;
+    s.eo = gld(st, p);
+    s.pv = gld(st + ps, p);
+#pragma unroll
+    for (int q = 0; q < OMX_MAX_POLES; ++q)
+        if (q < maxp) { s.L[q] = gld(st + (2 + q) * ps, p); s.Lo[q] = gld(st + (2 + maxp + q) * ps, p); }
+}
+
+// medium_update from a state loaded beforehand (medium_load_all): the same
+// expressions in the same order (the fused step's post kernel, checked
+// against the separate kernels' dispersive cells to 1e-13 in FP64), for a
+// pass that loads every component's state before the first update.
+template <typename T>
+__device__ __forceinline__ T medium_apply(T e, T rot, T vac, int m, const MedDev<T>* __restrict__ med, long p,
+                                          T* __restrict__ eold, T* __restrict__ pd, T* __restrict__ pl,
+                                          T* __restrict__ plold, long ps, const MedState<T>& s) {
+    if (m == 0) return e + vac;   // vac = dt_eps0 * rotH
+    const OMX_GLOBAL MedDev<T>& M = *((const OMX_GLOBAL MedDev<T>*)med + m);
+    if (M.kind == OMX_MEDIUM_DRUDE) {
+        const T pv = s.pv;
+        if constexpr (sizeof(T) == 8) {
+            const T t = M.C1 * e + M.C3 * rot - M.C4 * pv;
+            gst(pd, p, M.A1 * pv + M.A2 * (t + e));
+            return t;
+        } else {
+            const T t = e + (M.C1 * e + M.C3 * rot - M.C4 * pv);
+            gst(pd, p, pv + (M.A1 * pv + M.A2 * (t + e)));
+            return t;
+        }
+    }
+    if (M.kind == OMX_MEDIUM_DL) {
+        const T eo = s.eo;
+        const T pv = s.pv;
+        const int np = M.np;
+        T t;
+        if constexpr (sizeof(T) == 8) {
+            t = M.C1 * e + M.C2 * eo + M.C3 * rot - M.C4 * pv;
+#pragma unroll
+            for (int q = 0; q < OMX_MAX_POLES; ++q)
+                if (q < np) t = t - M.B1[q] * s.L[q] - M.B2[q] * s.Lo[q];
+            gst(pd, p, M.A1 * pv + M.A2 * (t + e));
+        } else {
+            T dlt = M.C1 * e + M.C2 * eo + M.C3 * rot - M.C4 * pv;
+#pragma unroll
+            for (int q = 0; q < OMX_MAX_POLES; ++q)
+                if (q < np) dlt = dlt - M.B1[q] * s.L[q] - M.B2[q] * s.Lo[q];
+            t = e + dlt;
+            gst(pd, p, pv + (M.A1 * pv + M.A2 * (t + e)));
+        }
+        const T de = t - eo;
+#pragma unroll
+        for (int q = 0; q < OMX_MAX_POLES; ++q) {
+            if (q < np) {
+                const long pq = p + q * ps;
+                const T L = s.L[q], Lo = s.Lo[q];
+                T nl;
+                if constexpr (sizeof(T) == 8) nl = M.al[q] * L + M.bt[q] * Lo + M.ga[q] * de;
+                else nl = L + (((L - Lo) + M.al[q] * L) + M.bt[q] * Lo + M.ga[q] * de);
+                gst(plold, pq, L);
+                gst(pl, pq, nl);
+            }
+        }
+        gst(eold, p, e);
+        return t;
+    }
+    return e + M.cdt * rot;  // dielectric: dt_eps0 * (1/eps_r) * rotH
+}
+
 template <typename T>
 __device__ __forceinline__ T medium_update(T e, T rot, T vac, int m, const MedDev<T>* __restrict__ med, long p,
                                            T* __restrict__ eold, T* __restrict__ pd, T* __restrict__ pl,
@@ -839,6 +924,20 @@ __device__ __forceinline__ T medium_update(T e, T rot, T vac, int m, const MedDe
         return t;
     }
     return e + M.cdt * rot;  // dielectric: dt_eps0 * (1/eps_r) * rotH
+}
+
+// The three components of a dispersive cell, each with its own medium (st0,
+// st1, st2: the components' disjoint state blocks, E_old, PD, then the PL and
+// PL_old planes, ps apart), each loading what its medium needs.
+template <typename T>
+__device__ __forceinline__ void medium_update3(T& ex, T& ey, T& ez, T rx, T ry, T rz, T vx, T vy, T vz, int m0, int m1,
+                                               int m2, const MedDev<T>* __restrict__ med, long p,
+                                               T* __restrict__ st0, T* __restrict__ st1, T* __restrict__ st2, long ps,
+                                               int maxp) {
+    const long pl0 = 2 * ps, plo0 = (2 + (long)maxp) * ps;
+    ex = medium_update(ex, rx, vx, m0, med, p, st0, st0 + ps, st0 + pl0, st0 + plo0, ps);
+    ey = medium_update(ey, ry, vy, m1, med, p, st1, st1 + ps, st1 + pl0, st1 + plo0, ps);
+    ez = medium_update(ez, rz, vz, m2, med, p, st2, st2 + ps, st2 + pl0, st2 + plo0, ps);
 }
 
 // Curl of H at an E cell (rot, for the medium updates) and the vacuum
@@ -1103,6 +1202,62 @@ __device__ __forceinline__ void e_tile(const YeeArgs<T>& a, const TfsfLaunch& tf
     else face_loop(std::false_type{});
 }
 
+// The E psi of a cell on CPML faces, loaded ahead of its update: the same
+// faces, slots and coefficient indices as e_cpml_x / e_cpml_y / e_cpml_z (x,
+// y, z: psi element, -1 off the faces; cx, cy, cz: coefficient index).
+template <typename T>
+struct PsiE {
+    long x = -1, y = -1, z = -1;
+    int cx = 0, cy = 0, cz = 0;
+    T x0 = (T)0, x1 = (T)0, y0 = (T)0, y1 = (T)0, z0 = (T)0, z1 = (T)0;
+};
+
+template <typename T>
+__device__ __forceinline__ void e_psi_load(const YeeArgs<T>& a, int i, int j, int k, long jk, PsiE<T>& q) {
+    const unsigned cp = a.cpml;
+    const int npml = a.npml, nse = 2 * npml;
+    if ((cp & 1u) && i >= 2 && i <= npml) { q.cx = i; q.x = (long)(i - 1) * a.sx + jk; }
+    else if ((cp & 2u) && i >= a.nx + 1 - npml && i <= a.nx - 1) {
+        q.cx = a.nx + 1 - i; q.x = (long)(npml - 1 + q.cx) * a.sx + jk;
+    }
+    if ((cp & 4u) && j >= 2 && j <= npml) { q.cy = j; q.y = ((long)i * nse + (j - 1)) * a.sz + k + a.off; }
+    else if ((cp & 8u) && j >= a.ny + 1 - npml && j <= a.ny - 1) {
+        q.cy = a.ny + 1 - j; q.y = ((long)i * nse + (npml - 1 + q.cy)) * a.sz + k + a.off;
+    }
+    if ((cp & 16u) && k >= 2 && k <= npml) { q.cz = k; q.z = zpsi_index(0, j, i, k - 2, a.sy, a.nx, npml); }
+    else if ((cp & 32u) && k >= a.nz + 1 - npml && k <= a.nz - 1) {
+        q.cz = a.nz + 1 - k; q.z = zpsi_index(1, j, i, a.nz - 1 - k, a.sy, a.nx, npml);
+    }
+    if (q.x >= 0) { q.x0 = gld(a.pe[0], q.x); q.x1 = gld(a.pe[1], q.x); }
+    if (q.y >= 0) { q.y0 = gld(a.pe[2], q.y); q.y1 = gld(a.pe[3], q.y); }
+    if (q.z >= 0) { q.z0 = gld(a.pe[4], q.z); q.z1 = gld(a.pe[5], q.z); }
+}
+
+// e_cpml_x, e_cpml_y, e_cpml_z from the loaded psi (STEPPED: already
+// advanced, added as they are).
+template <typename T, bool STEPPED>
+__device__ __forceinline__ void e_psi_apply(const YeeArgs<T>& a, const PsiE<T>& q, T dep, T hzc, T hzj, T hzi, T hyc,
+                                            T hyk, T hyi, T hxc, T hxj, T hxk, T& ex, T& ey, T& ez) {
+    if (q.x >= 0) {
+        const T p0 = STEPPED ? q.x0 : psi_step(q.x0, gld(a.be, q.cx), gld(a.ced, q.cx), hzi - hzc);
+        const T p1 = STEPPED ? q.x1 : psi_step(q.x1, gld(a.be, q.cx), gld(a.ced, q.cx), hyc - hyi);
+        if (!STEPPED) { gst(a.pe[0], q.x, p0); gst(a.pe[1], q.x, p1); }
+        ey = ey + dep * p0; ez = ez + dep * p1;
+    }
+    if (q.y >= 0) {
+        const T p0 = STEPPED ? q.y0 : psi_step(q.y0, gld(a.be, q.cy), gld(a.ced, q.cy), hzc - hzj);
+        const T p1 = STEPPED ? q.y1 : psi_step(q.y1, gld(a.be, q.cy), gld(a.ced, q.cy), hxj - hxc);
+        if (!STEPPED) { gst(a.pe[2], q.y, p0); gst(a.pe[3], q.y, p1); }
+        ex = ex + dep * p0; ez = ez + dep * p1;
+    }
+    if (q.z >= 0) {
+        const T p0 = STEPPED ? q.z0 : psi_step(q.z0, gld(a.be, q.cz), gld(a.ced, q.cz), hyk - hyc);
+        const T p1 = STEPPED ? q.z1 : psi_step(q.z1, gld(a.be, q.cz), gld(a.ced, q.cz), hxc - hxk);
+        if (!STEPPED) { gst(a.pe[4], q.z, p0); gst(a.pe[5], q.z, p1); }
+        ex = ex + dep * p0; ey = ey + dep * p1;
+    }
+}
+
 // E of the cells of dispersive classes (Drude / Drude-Lorentz components),
 // after the tile pass has updated every other cell: curl of H, the full
 // medium update and the CPML psi recursions of the cell, from E in
@@ -1110,7 +1265,7 @@ __device__ __forceinline__ void e_tile(const YeeArgs<T>& a, const TfsfLaunch& tf
 // path, the other buffer after the fused step).  list holds the cells'
 // indices (i, j, k) with the media of their three components; p, the cell's
 // position in its grid's list, indexes its compact Drude / DL state.
-template <typename T, bool CPML, bool B32, bool STEPPED = false>
+template <typename T, bool CPML, bool B32, bool STEPPED = false, bool LOADS_FIRST = false>
 __device__ __forceinline__ void e_dispersive_cell(const YeeArgs<T>& a, T* __restrict__ exo, T* __restrict__ eyo,
                                                   T* __restrict__ ezo, long e, long p) {
     using O = Off<T, B32>;
@@ -1122,6 +1277,18 @@ __device__ __forceinline__ void e_dispersive_cell(const YeeArgs<T>& a, T* __rest
     const long jk = (long)j * a.sz + k + a.off;
     const I o = O::of((long)i * a.sx + jk);
     const I sx = O::of(a.sx), sz = O::of(a.sz), s1 = O::of(1);
+    // LOADS_FIRST: the three components' state and the E psi go out with the
+    // fields, before any store (a latency-bound pass: a few memory round
+    // trips per cell instead of one per component and face)
+    MedState<T> s0, s1s, s2;
+    PsiE<T> qe;
+    if constexpr (LOADS_FIRST) {
+        T* const st0 = a.dstate;
+        medium_load_all(m0, p, st0, a.pstride, a.maxp, s0);
+        medium_load_all(m1, p, st0 + a.dcomp, a.pstride, a.maxp, s1s);
+        medium_load_all(m2, p, st0 + 2 * a.dcomp, a.pstride, a.maxp, s2);
+        if constexpr (CPML) e_psi_load(a, i, j, k, jk, qe);
+    }
     const T hzc = O::ld(a.hz, o), hyc = O::ld(a.hy, o), hxc = O::ld(a.hx, o);
     const T hzj = O::ld(a.hz, o - sz), hxj = O::ld(a.hx, o - sz);
     const T hyk = O::ld(a.hy, o - s1), hxk = O::ld(a.hx, o - s1);
@@ -1133,19 +1300,26 @@ __device__ __forceinline__ void e_dispersive_cell(const YeeArgs<T>& a, T* __rest
     const T dep = a.dt_eps0;
     T rx, ry, rz, vx, vy, vz;
     e_curl(dep, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, dex, cex, dey, dez, cey, cez, rx, ry, rz, vx, vy, vz);
-    const long ps = a.pstride, pl0 = 2 * ps, plo0 = (2 + (long)a.maxp) * ps;
-    T* const st0 = a.dstate;
-    T* const st1 = st0 + a.dcomp;
-    T* const st2 = st1 + a.dcomp;
-    ex = medium_update(ex, rx, vx, m0, a.med, p, st0, st0 + ps, st0 + pl0, st0 + plo0, ps);
-    ey = medium_update(ey, ry, vy, m1, a.med, p, st1, st1 + ps, st1 + pl0, st1 + plo0, ps);
-    ez = medium_update(ez, rz, vz, m2, a.med, p, st2, st2 + ps, st2 + pl0, st2 + plo0, ps);
-    if constexpr (CPML) {
-        // STEPPED: the fused step already advanced this cell's E psi (the
-        // recursion depends on H only), so its values are added as they are
-        e_cpml_x<T, STEPPED>(a, i, jk, dep, hzc, hzi, hyc, hyi, ey, ez);
-        e_cpml_y<T, STEPPED>(a, i, j, k, dep, hzc, hzj, hxc, hxj, ex, ez);
-        e_cpml_z<T, STEPPED>(a, i, j, k, dep, hyc, hyk, hxc, hxk, ex, ey);
+    if constexpr (LOADS_FIRST) {
+        const long ps = a.pstride, pl0 = 2 * ps, plo0 = (2 + (long)a.maxp) * ps;
+        T* const st0 = a.dstate;
+        T* const st1 = st0 + a.dcomp;
+        T* const st2 = st1 + a.dcomp;
+        ex = medium_apply(ex, rx, vx, m0, a.med, p, st0, st0 + ps, st0 + pl0, st0 + plo0, ps, s0);
+        ey = medium_apply(ey, ry, vy, m1, a.med, p, st1, st1 + ps, st1 + pl0, st1 + plo0, ps, s1s);
+        ez = medium_apply(ez, rz, vz, m2, a.med, p, st2, st2 + ps, st2 + pl0, st2 + plo0, ps, s2);
+        if constexpr (CPML)
+            e_psi_apply<T, STEPPED>(a, qe, dep, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, ex, ey, ez);
+    } else {
+        medium_update3(ex, ey, ez, rx, ry, rz, vx, vy, vz, m0, m1, m2, a.med, p, a.dstate, a.dstate + a.dcomp,
+                       a.dstate + 2 * a.dcomp, a.pstride, a.maxp);
+        if constexpr (CPML) {
+            // STEPPED: the fused step already advanced this cell's E psi (the
+            // recursion depends on H only), so its values are added as they are
+            e_cpml_x<T, STEPPED>(a, i, jk, dep, hzc, hzi, hyc, hyi, ey, ez);
+            e_cpml_y<T, STEPPED>(a, i, j, k, dep, hzc, hzj, hxc, hxj, ex, ez);
+            e_cpml_z<T, STEPPED>(a, i, j, k, dep, hyc, hyk, hxc, hxk, ex, ey);
+        }
     }
     O::st(exo, o, ex); O::st(eyo, o, ey); O::st(ezo, o, ez);
 }
