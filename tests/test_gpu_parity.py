@@ -427,6 +427,65 @@ def test_fused_equals_separate_kernels_tfsf(precision):
     assert r <= (1e-13 if precision == 8 else 5e-6)
 
 
+def test_fused_tile_shapes_tfsf():
+    """The TF/SF plane wave in the fused step across tile shapes (rows, planes
+    per chunk, wavefronts along y): the H corrections of the face lanes, the
+    extra workgroup that advances the incident line's H after the last box,
+    and the E corrections of the post kernel, against the separate kernels
+    (FP64)."""
+    case = cases.config_dl_sphere_tfsf(n=40, npml=6, steps=80, record_every=4, radius_nm=10.0)
+    dr, dt = case.dr_dt
+    ref = None
+    for shape in (None, (2, 8, 0), (1, 1, 1), (2, 3, 3), (1, 7, 2), (2, 16, 4)):
+        g = omx_fdtd.Grid3D(case.grid, case.npml, case.boundaries, dt, dr, precision=8)
+        if shape is None:
+            g.set_step_mode(1)
+        else:
+            g.set_step_mode(0, *shape)
+        cases.apply(case, g, library=G.library_medium, make_medium=G.medium)
+        g.run(1, case.steps, case.record_every)
+        assert g.last_run_fused() == (shape is not None)
+        got = g.read_records()
+        g.kill()
+        if ref is None:
+            ref = got
+            assert np.abs(ref).max() > 0
+            continue
+        r = rel_l2(got, ref)
+        print(f"fused shape {shape} vs separate kernels relL2 = {r:.3e}", flush=True)
+        assert r <= 1e-13
+
+
+def test_fused_long_dispersive_list():
+    """A dispersive list longer than the post kernel's loads-first limit
+    (2^19 cells): a Drude-Lorentz block of 85 x 85 x 81 cells takes the
+    per-component loads of the separate kernels' dispersive cells; the
+    fused step against the separate kernels (FP64)."""
+    from omx_fdtd.units import index_to_nm
+    n = (100, 100, 100)
+    assert 85 * 85 * 81 > 1 << 19
+    dx = 5.0
+    pos = [index_to_nm(c, dx, n[a]) for a, c in enumerate((50, 50, 93))]
+    src = [("z", 1.0, 1.549802, 0.6, 0.3, pos[0], pos[1], pos[2], 0.0, 0.0, 1.0e6, 0.0)]
+    dets = cases.point_detectors([(50, 50, 60), (30, 70, 20), (50, 50, 92)], ("Ex", "Ez", "Hy"))
+    case = small_case("long-list", n=n, npml=6, steps=40, media=[("drude-lorentz", "Ag", 1.0)],
+                      shapes=[(3, 1, [8.0, 8.0, 10.0, 92.0, 92.0, 90.0])], sources=src, dets=dets, dx=dx)
+    dr, dt = case.dr_dt
+    out = []
+    for mode in (1, 0):
+        g = omx_fdtd.Grid3D(case.grid, case.npml, case.boundaries, dt, dr, precision=8)
+        g.set_step_mode(mode)
+        cases.apply(case, g, library=G.library_medium, make_medium=G.medium)
+        g.run(1, case.steps, case.record_every)
+        assert g.last_run_fused() == (mode == 0)
+        out.append(g.read_records())
+        g.kill()
+    assert np.abs(out[0]).max() > 0
+    r = rel_l2(out[1], out[0])
+    print(f"fused vs separate kernels relL2 = {r:.3e}", flush=True)
+    assert r <= 1e-13
+
+
 def test_stepwise_api_with_quantum_sites():
     """Drive the reference step order call by call (mxim_mxll.f90:92-132) with
     a prescribed dPt/dt per site, checking E at the sites and the detectors."""
