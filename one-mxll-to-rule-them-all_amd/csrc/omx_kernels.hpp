@@ -1280,13 +1280,13 @@ __device__ __forceinline__ void e_dispersive_cell(const YeeArgs<T>& a, T* __rest
     // LOADS_FIRST: the three components' state and the E psi go out with the
     // fields, before any store (a latency-bound pass: a few memory round
     // trips per cell instead of one per component and face)
-    MedState<T> s0, s1s, s2;
+    MedState<T> ms0, ms1, ms2;
     PsiE<T> qe;
     if constexpr (LOADS_FIRST) {
         T* const st0 = a.dstate;
-        medium_load_all(m0, p, st0, a.pstride, a.maxp, s0);
-        medium_load_all(m1, p, st0 + a.dcomp, a.pstride, a.maxp, s1s);
-        medium_load_all(m2, p, st0 + 2 * a.dcomp, a.pstride, a.maxp, s2);
+        medium_load_all(m0, p, st0, a.pstride, a.maxp, ms0);
+        medium_load_all(m1, p, st0 + a.dcomp, a.pstride, a.maxp, ms1);
+        medium_load_all(m2, p, st0 + 2 * a.dcomp, a.pstride, a.maxp, ms2);
         if constexpr (CPML) e_psi_load(a, i, j, k, jk, qe);
     }
     const T hzc = O::ld(a.hz, o), hyc = O::ld(a.hy, o), hxc = O::ld(a.hx, o);
@@ -1305,9 +1305,9 @@ __device__ __forceinline__ void e_dispersive_cell(const YeeArgs<T>& a, T* __rest
         T* const st0 = a.dstate;
         T* const st1 = st0 + a.dcomp;
         T* const st2 = st1 + a.dcomp;
-        ex = medium_apply(ex, rx, vx, m0, a.med, p, st0, st0 + ps, st0 + pl0, st0 + plo0, ps, s0);
-        ey = medium_apply(ey, ry, vy, m1, a.med, p, st1, st1 + ps, st1 + pl0, st1 + plo0, ps, s1s);
-        ez = medium_apply(ez, rz, vz, m2, a.med, p, st2, st2 + ps, st2 + pl0, st2 + plo0, ps, s2);
+        ex = medium_apply(ex, rx, vx, m0, a.med, p, st0, st0 + ps, st0 + pl0, st0 + plo0, ps, ms0);
+        ey = medium_apply(ey, ry, vy, m1, a.med, p, st1, st1 + ps, st1 + pl0, st1 + plo0, ps, ms1);
+        ez = medium_apply(ez, rz, vz, m2, a.med, p, st2, st2 + ps, st2 + pl0, st2 + plo0, ps, ms2);
         if constexpr (CPML)
             e_psi_apply<T, STEPPED>(a, qe, dep, hzc, hzj, hzi, hyc, hyk, hyi, hxc, hxj, hxk, ex, ey, ez);
     } else {
