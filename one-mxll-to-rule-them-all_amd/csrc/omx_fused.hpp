@@ -339,8 +339,11 @@ __device__ __forceinline__ void e_wg(const YeeArgs<T>& a, int i, int j, uint32_t
     }
 }
 
+// FP64 workgroups hold at most 8 wavefronts (the host's shape rule), so the
+// FP64 kernels are bounded at 512 threads: a lane may then take up to 256
+// VGPRs, and the CPML variants (137-218) do not spill as they did at 128.
 template <typename T, int R, int CPML, int MED, bool TF>
-__global__ __launch_bounds__(1024) void k_step_wg(const YeeArgs<T> a, const FusedOut<T> fo, const WgPlan P) {
+__global__ __launch_bounds__(sizeof(T) == 8 ? 512 : 1024) void k_step_wg(const YeeArgs<T> a, const FusedOut<T> fo, const WgPlan P) {
     extern __shared__ __attribute__((aligned(16))) char wg_lds_raw[];
     const int NZ = P.nzw, NY = P.nyw, NW = NZ * NY;
     const int lane = (int)(threadIdx.x & 63);
